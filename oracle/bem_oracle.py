@@ -1,0 +1,695 @@
+"""FP64 CPU oracle: numpy/scipy restatement of MNPBEM's retarded BEM path.
+
+TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, ``__graft_entry__.smoke()``
+and the CPU-baseline leg of ``bench.py`` as the checker; the product package
+never imports it.
+
+Every function cites the reference file:line (paths relative to the
+reference repository root) whose algorithm it restates.  The restatement is
+written from the algorithm, in plain numpy; loops and vector forms are chosen
+for clarity at the oracle's sizes (N <= 2884 faces), not to mirror the
+reference's code.
+
+Pinning status (how each part is checked, see tests/test_oracle_*.py):
+
+* materials (EpsDrude/EpsConst/EpsTable): closed-form Drude value, table
+  round trip at the tabulated energies.
+* quadrature (rule 18, LGL, polar): polynomial exactness, weight sums.
+* curved geometry (pos/nvec/area): area -> pi d^2, |nvec| = 1, outward.
+* refinement coefficients + Green functions: far-field entries against the
+  closed form; the refined order-0 F against the Fuchs-Liu identity
+  (sum of -F over a closed surface = 2 pi); brute-force quadrature.
+* BEMRet init/solve: the 8N operator identity of GdA&H Eqs. (10,11,14,17)
+  (``bem_operator_apply``) to round-off.
+* whole path: the reference's committed spectra
+  validation/03_bemret/sphere/data/bemret_python.csv (EpsTable gold,
+  trisphere(144, 20), 41 wavelengths) for ext and sca.
+"""
+
+import math
+import os
+
+import numpy as np
+import scipy.linalg as sla
+from scipy.interpolate import CubicSpline
+
+__all__ = [
+    "EV2NM", "EpsConst", "EpsDrude", "EpsTable",
+    "lglnodes", "rule18", "polar_rule_tri", "tri6_shape", "tri6_deriv",
+    "read_tri_bin", "trisphere_mesh", "CurvedSurface", "Composite",
+    "refinematrix", "refine_coefficients", "closed_surface_correction",
+    "GreenSetup", "green_matrices", "BEMRetOracle", "planewave_potential",
+    "farfield", "extinction", "scattering", "read_pinfty",
+    "bem_operator_apply", "DATA_DIR", "solve_spectrum",
+]
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "mnpbem_amd", "data")
+
+# eV <-> nm conversion, mnpbem/utils/constants.py:9
+EV2NM = 1.0 / 8.0655477e-4
+
+
+# --------------------------------------------------------------------------
+# Materials  (mnpbem/materials/eps_const.py:48-72, eps_drude.py:73-107,
+#             eps_table.py:51-152)
+# --------------------------------------------------------------------------
+class EpsConst:
+    """Constant dielectric function; k = 2 pi / lambda * sqrt(eps)."""
+
+    def __init__(self, eps):
+        self.eps = eps
+
+    def __call__(self, enei):
+        eps = complex(self.eps)
+        return eps, 2.0 * np.pi / enei * np.sqrt(eps)
+
+
+class EpsDrude:
+    """Drude metal: eps = eps0 - wp^2 / (w (w + i gammad)), w = EV2NM / lambda."""
+
+    def __init__(self, eps0, wp, gammad):
+        self.eps0, self.wp, self.gammad = eps0, wp, gammad
+
+    def __call__(self, enei):
+        w = EV2NM / enei
+        eps = self.eps0 - self.wp ** 2 / (w * (w + 1j * self.gammad))
+        return eps, 2.0 * np.pi / enei * np.sqrt(eps)
+
+
+class EpsTable:
+    """Tabulated (energy eV, n, k) file; not-a-knot cubic splines of n and k
+    over wavelength (eps_table.py:74-109), eps = (n + i k)^2."""
+
+    def __init__(self, path):
+        rows = []
+        with open(path) as fh:
+            for line in fh:
+                s = line.strip()
+                if not s or s[0] in "%#":
+                    continue
+                try:
+                    vals = [float(t) for t in s.split()]
+                except ValueError:
+                    continue
+                if len(vals) >= 3:
+                    rows.append(vals[:3])
+        tab = np.array(rows)
+        lam = EV2NM / tab[:, 0]
+        order = np.argsort(lam)
+        self.lam = lam[order]
+        self._n = CubicSpline(self.lam, tab[order, 1])
+        self._k = CubicSpline(self.lam, tab[order, 2])
+
+    def __call__(self, enei):
+        if enei < self.lam[0] or enei > self.lam[-1]:
+            raise ValueError("wavelength out of tabulated range")
+        nc = complex(self._n(enei)) + 1j * complex(self._k(enei))
+        eps = nc * nc
+        return eps, 2.0 * np.pi / enei * np.sqrt(eps)
+
+
+# --------------------------------------------------------------------------
+# Quadrature  (mnpbem/utils/quadrature.py:15-88, 182-224;
+#              mnpbem/utils/quadface.py:106-182)
+# --------------------------------------------------------------------------
+def lglnodes(n):
+    """Legendre-Gauss-Lobatto nodes/weights on [-1, 1] (n+1 points), Newton
+    iteration from Chebyshev-Gauss-Lobatto start (quadrature.py:15-88)."""
+    x = np.cos(np.pi * np.arange(n + 1) / n)
+    P = np.zeros((n + 1, n + 1))
+    xold = np.full(n + 1, 2.0)
+    while np.max(np.abs(x - xold)) > np.finfo(float).eps:
+        xold = x.copy()
+        P[:, 0] = 1.0
+        P[:, 1] = x
+        for k in range(2, n + 1):
+            P[:, k] = ((2 * k - 1) * x * P[:, k - 1] - (k - 1) * P[:, k - 2]) / k
+        x = xold - (x * P[:, n] - P[:, n - 1]) / ((n + 1) * P[:, n])
+    w = 2.0 / (n * (n + 1) * P[:, n] ** 2)
+    return x, w
+
+
+def rule18():
+    """28-point, degree-11 triangle rule on the unit triangle (published table,
+    quadrature.py:182-224); weights normalised to sum 1."""
+    a = 1.0 / 3.0
+    b, c = 0.9480217181434233, 0.02598914092828833
+    d, e = 0.8114249947041546, 0.09428750264792270
+    f, g = 0.01072644996557060, 0.4946367750172147
+    p, q = 0.5853132347709715, 0.2073433826145142
+    r, s = 0.1221843885990187, 0.4389078057004907
+    t, u, v = 0.6779376548825902, 0.04484167758913055, 0.27722066752827925
+    ww, xx, yy = 0.8588702812826364, 0.0, 0.1411297187173636
+    W = [0.08797730116222190, 0.008744311553736190, 0.03808157199393533,
+         0.01885544805613125, 0.07215969754474100, 0.06932913870553720,
+         0.04105631542928860, 0.007362383783300573]
+    x = [a, b, c, c, d, e, e, f, g, g, p, q, q, r, s, s, t, t, u, u, v, v, ww, ww, xx, xx, yy, yy]
+    y = [a, c, b, c, e, d, e, g, f, g, q, p, q, s, r, s, u, v, t, v, t, u, xx, yy, ww, yy, ww, xx]
+    wt = ([W[0]] + [W[1]] * 3 + [W[2]] * 3 + [W[3]] * 3 + [W[4]] * 3 + [W[5]] * 3
+          + [W[6]] * 6 + [W[7]] * 6)
+    wt = np.array(wt)
+    return np.array(x), np.array(y), wt / wt.sum()
+
+
+def polar_rule_tri(npol=(7, 5)):
+    """Polar integration rule for a triangle around its centroid: three
+    120-degree-rotated 60-degree sectors, LGL in radius and angle
+    (quadface.py:106-182, MATLAB @quadface/private/init.m:25-53)."""
+    x1, w1 = lglnodes(npol[0])
+    rho = 0.5 * (x1 + 1.0 + 1e-6)
+    x2, w2 = lglnodes(npol[1])
+    phi = np.deg2rad(270.0 + 60.0 * x2)
+    R, PH = np.meshgrid(rho, phi, indexing="ij")
+    R, PH = R.ravel(), PH.ravel()
+    rad = 1.0 / np.abs(2.0 * np.sin(PH))
+    ang = np.concatenate([PH, PH + np.deg2rad(120.0), PH + 2 * np.deg2rad(120.0)])
+    rr = np.tile(R * rad, 3)
+    xp, yp = np.cos(ang) * rr, np.sin(ang) * rr
+    xt = (1.0 - np.sqrt(3.0) * xp - yp) / 3.0
+    yt = (1.0 + np.sqrt(3.0) * xp - yp) / 3.0
+    w = np.tile(np.outer(w1, w2).ravel() * R * rad ** 2, 3)
+    return xt, yt, w / w.sum()
+
+
+def tri6_shape(x, y):
+    """Quadratic 6-node triangle shape functions, node order
+    (v0, v1, v2, m01, m12, m20) with L1=x, L2=y, L3=1-x-y (particle.py:1607-1621)."""
+    L1, L2, L3 = x, y, 1.0 - x - y
+    return np.stack([L1 * (2 * L1 - 1), L2 * (2 * L2 - 1), L3 * (2 * L3 - 1),
+                     4 * L1 * L2, 4 * L2 * L3, 4 * L3 * L1], axis=-1)
+
+
+def tri6_deriv(x, y):
+    """d/dx and d/dy of ``tri6_shape`` (particle.py:1624-1649)."""
+    L1, L2, L3 = x, y, 1.0 - x - y
+    z = np.zeros_like(x)
+    dx = np.stack([4 * L1 - 1, z, 1 - 4 * L3, 4 * L2, -4 * L2, 4 * (L3 - L1)], axis=-1)
+    dy = np.stack([z, 4 * L2 - 1, 1 - 4 * L3, 4 * L1, 4 * (L3 - L2), -4 * L1], axis=-1)
+    return dx, dy
+
+
+# --------------------------------------------------------------------------
+# Mesh  (mnpbem/geometry/mesh_generators.py:13-130, 133-307)
+# --------------------------------------------------------------------------
+def read_tri_bin(path):
+    """sphere{n}_tri.bin: int32 nv, nf; f64 x[nv], y[nv], z[nv];
+    int32 faces[nf*3] 1-based (mesh_generators.py:76-85)."""
+    with open(path, "rb") as fh:
+        nv, nf = np.fromfile(fh, dtype=np.int32, count=2)
+        xyz = np.fromfile(fh, dtype=np.float64, count=3 * nv).reshape(3, nv).T
+        faces = np.fromfile(fh, dtype=np.int32, count=3 * nf).reshape(nf, 3) - 1
+    return xyz, faces.astype(np.int64)
+
+
+_SAVED = np.array([32, 60, 144, 169, 225, 256, 289, 324, 361, 400, 441, 484,
+                   529, 576, 625, 676, 729, 784, 841, 900, 961, 1024, 1225, 1444])
+
+
+def trisphere_mesh(nverts, diameter):
+    """Curved sphere mesh: saved vertex set scaled to the diameter, one
+    midpoint per edge, every node projected onto the sphere
+    (mesh_generators.py:13-130, _add_midpoints_flat :133-196).
+
+    Returns (nodes, tri6) where tri6[f] = node indices (v0, v1, v2, m01, m12, m20).
+    """
+    n = int(_SAVED[np.argmin(np.abs(_SAVED - nverts))])
+    verts, faces = read_tri_bin(os.path.join(DATA_DIR, "sphere%d_tri.bin" % n))
+    verts = 0.5 * diameter * verts
+    nv = len(verts)
+    e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+    e = np.sort(e, axis=1)
+    uniq, inv = np.unique(e, axis=0, return_inverse=True)
+    inv = inv.reshape(3, -1).T                                   # (nf, 3): edges 01, 12, 20
+    nodes = np.vstack([verts, 0.5 * (verts[uniq[:, 0]] + verts[uniq[:, 1]])])
+    nodes = 0.5 * diameter * nodes / np.linalg.norm(nodes, axis=1, keepdims=True)
+    tri6 = np.hstack([faces, nv + inv])
+    return nodes, tri6
+
+
+# --------------------------------------------------------------------------
+# Curved surface geometry  (mnpbem/geometry/particle.py:551-613, 1364-1429,
+#                           1520-1575, 2848-2880)
+# --------------------------------------------------------------------------
+class CurvedSurface:
+    """All-triangle surface of quadratic (6-node) elements."""
+
+    def __init__(self, nodes, tri6):
+        self.nodes = np.asarray(nodes, float)
+        self.tri6 = np.asarray(tri6, np.int64)
+        self.n = len(self.tri6)
+        X = self.nodes[self.tri6]                                 # (n, 6, 3)
+        self._X = X
+        # area = sum of rule-18 weights on the curved element (particle.py:551-560)
+        _, w, _ = self.quad()
+        self.area = w.sum(axis=1)
+        tri = np.array([-1, -1, -1, 4, 4, 4]) / 9.0
+        trix = np.array([1, 0, -1, 4, -4, 0]) / 3.0
+        triy = np.array([0, 1, -1, 4, 0, -4]) / 3.0
+        self.pos = np.einsum("c,fcd->fd", tri, X)
+        v1 = np.einsum("c,fcd->fd", triy, X)
+        v2 = np.einsum("c,fcd->fd", trix, X)
+        nv = np.cross(v1, v2)
+        nv /= np.maximum(np.linalg.norm(nv, axis=1, keepdims=True), 1e-14)
+        v1 /= np.maximum(np.linalg.norm(v1, axis=1, keepdims=True), 1e-14)
+        self.nvec, self.tvec1 = nv, v1
+        self.tvec2 = np.cross(nv, v1)
+        corners = self.nodes[self.tri6[:, :3]]
+        self.bradius = np.linalg.norm(self.pos[:, None, :] - corners, axis=2).max(axis=1)
+
+    def _points(self, xq, yq, wq, ind):
+        X = self._X if ind is None else self._X[ind]
+        S = tri6_shape(xq, yq)
+        Dx, Dy = tri6_deriv(xq, yq)
+        pts = np.einsum("qc,fcd->fqd", S, X)
+        jac = np.linalg.norm(np.cross(np.einsum("qc,fcd->fqd", Dx, X),
+                                      np.einsum("qc,fcd->fqd", Dy, X)), axis=2)
+        return pts, 0.5 * wq[None, :] * jac
+
+    def quad(self, ind=None):
+        """Rule-18 points (nf, 28, 3) and weights (nf, 28) on curved faces
+        (particle.py:1520-1575)."""
+        xq, yq, wq = rule18()
+        pts, w = self._points(xq, yq, wq, ind)
+        return pts, w, None
+
+    def quadpol(self, ind=None):
+        """Polar points (nf, 144, 3) and weights around each centroid
+        (particle.py:1364-1429)."""
+        xq, yq, wq = polar_rule_tri()
+        return self._points(xq, yq, wq, ind)
+
+
+class Composite:
+    """Minimal ComParticle: surfaces, eps list, inout (1-based) per surface
+    (mnpbem/geometry/comparticle.py:55-127, eps1/eps2 :596-654)."""
+
+    def __init__(self, eps, surfaces, inout):
+        self.eps = list(eps)
+        self.parts = list(surfaces)
+        self.inout = np.atleast_2d(np.asarray(inout, dtype=int))
+        self.offsets = np.cumsum([0] + [s.n for s in self.parts])
+        self.n = int(self.offsets[-1])
+        cat = lambda name: np.concatenate([getattr(s, name) for s in self.parts])
+        self.pos, self.nvec, self.area = cat("pos"), cat("nvec"), cat("area")
+        self.tvec1, self.tvec2, self.bradius = cat("tvec1"), cat("tvec2"), cat("bradius")
+        self.inout_faces = np.concatenate(
+            [np.tile(self.inout[i], (s.n, 1)) for i, s in enumerate(self.parts)])
+
+    def face_range(self, i):
+        return np.arange(self.offsets[i], self.offsets[i + 1])
+
+    def eps_faces(self, enei, side):
+        vals = [self.eps[m - 1](enei)[0] for m in self.inout_faces[:, side]]
+        return np.array(vals, dtype=complex)
+
+
+# --------------------------------------------------------------------------
+# Refinement  (mnpbem/greenfun/refine_utils.py:12-158;
+#              greenret_refined.py:78-516; compgreen_ret.py:192-400;
+#              compgreen_stat.py:219-465)
+# --------------------------------------------------------------------------
+def refinematrix(pos1, pos2, rad1, rad2, rel_cutoff=3.0, abs_cutoff=0.0):
+    """Dense int matrix: 2 where centroids coincide, 1 where the relative
+    distance (d - r_j) / r_i < RelCutoff (or d - r_j < AbsCutoff), else 0.
+    Distances by the algebraic form |a|^2 + |b|^2 - 2 a.b with squares below
+    1e-10 set to zero (refine_utils.py:12-26, 29-158)."""
+    n1 = np.sum(pos1 * pos1, axis=1)
+    n2 = np.sum(pos2 * pos2, axis=1)
+    dsq = n1[:, None] + n2[None, :] - 2.0 * (pos1 @ pos2.T)
+    dsq[dsq < 1e-10] = 0.0
+    d = np.sqrt(dsq)
+    d2 = d - rad2[None, :]
+    idr = d2 / rad1[:, None]
+    ir = np.zeros(d.shape, dtype=np.int8)
+    ir[((d2 < abs_cutoff) | (idr < rel_cutoff)) & (d != 0)] = 1
+    ir[d == 0] = 2
+    return ir
+
+
+def refine_coefficients(surf, ir, order=5):
+    """Order-``order`` (ik)^n expansion coefficients of the refined entries of
+    one surface with itself, deriv='cart' (greenret_refined.py:122-516,
+    MATLAB Greenfun/@greenret/private/init.m:42-177).
+
+    Returns row, col (n_ref,), g (n_ref, order+1), f (n_ref, 3, order+1), all
+    real FP64, sorted by (row, col).
+    """
+    row, col = np.nonzero(ir)
+    nref = len(row)
+    g = np.zeros((nref, order + 1))
+    f = np.zeros((nref, 3, order + 1))
+    fact = np.array([math.factorial(n) for n in range(order + 1)], float)
+    tiny = np.finfo(float).eps
+
+    # self terms: polar integration around the centroid
+    dsel = np.nonzero(ir[row, col] == 2)[0]
+    if len(dsel):
+        fi, fj = row[dsel], col[dsel]
+        pts, w = surf.quadpol(fj)
+        vec = surf.pos[fi][:, None, :] - pts
+        r = np.maximum(np.linalg.norm(vec, axis=2), tiny)
+        ndr = np.einsum("fd,fqd->fq", surf.nvec[fi], vec)
+        in1 = np.einsum("fd,fqd->fq", surf.tvec1[fi], vec)
+        in2 = np.einsum("fd,fqd->fq", surf.tvec2[fi], vec)
+        rr = np.maximum(r, 1e-4 * r.max(axis=1, keepdims=True))
+        for n in range(order + 1):
+            g[dsel, n] = np.sum(w * r ** (n - 1), axis=1) / fact[n]
+            f1 = np.sum(w * (n - 1) * ndr * r ** (n - 3), axis=1) / fact[n]
+            f2 = np.sum(w * (n - 1) * in1 * r ** n / rr ** 3, axis=1) / fact[n]
+            f3 = np.sum(w * (n - 1) * in2 * r ** n / rr ** 3, axis=1) / fact[n]
+            f[dsel, :, n] = (surf.nvec[fi] * f1[:, None] + surf.tvec1[fi] * f2[:, None]
+                             + surf.tvec2[fi] * f3[:, None])
+
+    # near pairs: 28-point rule on the source face, expanded about r0
+    nsel = np.nonzero(ir[row, col] == 1)[0]
+    if len(nsel):
+        fi, fj = row[nsel], col[nsel]
+        pts_all, w_all, _ = surf.quad()
+        pts, w = pts_all[fj], w_all[fj]
+        rv = surf.pos[fi][:, None, :] - pts                      # (p, 28, 3)
+        r = np.maximum(np.linalg.norm(rv, axis=2), tiny)
+        r0 = np.linalg.norm(surf.pos[fj] - surf.pos[fi], axis=1)
+        dl = r - r0[:, None]
+        for n in range(order + 1):
+            g[nsel, n] = np.sum(w * dl ** n / r, axis=1) / fact[n]
+            if n == 0:
+                fs = -1.0 / r ** 3
+            else:
+                fs = -dl ** n / (fact[n] * r ** 3) + dl ** (n - 1) / (fact[n - 1] * r ** 2)
+            f[nsel, :, n] = np.einsum("pq,pqd->pd", w * fs, rv)
+    return row, col, g, f
+
+
+def closed_surface_correction(surf, ir):
+    """Fuchs-Liu correction for a closed surface, outward normals (dir = +1):
+    corr_j = -2 pi - sum_i a_i F_ij / a_j with F the refined quasistatic normal
+    derivative (compgreen_ret.py:192-356, compgreen_stat.py:219-465,
+    MATLAB Greenfun/@compgreenret/private/initclosed.m).  Added to the order-0
+    Cartesian coefficient of the self term as corr_j * nvec_j
+    (compgreen_ret.py:376-400, MATLAB @greenret/diag.m)."""
+    pos, nv, a = surf.pos, surf.nvec, surf.area
+    tiny = np.finfo(float).eps
+    rv = pos[:, None, :] - pos[None, :, :]
+    d = np.maximum(np.linalg.norm(rv, axis=2), tiny)
+    F = -np.einsum("id,ijd->ij", nv, rv) / d ** 3 * a[None, :]
+    np.fill_diagonal(F, 0.0)
+    # self terms
+    pts, w = surf.quadpol()
+    vec = pos[:, None, :] - pts
+    r = np.maximum(np.linalg.norm(vec, axis=2), tiny)
+    F[np.arange(surf.n), np.arange(surf.n)] = -np.sum(
+        w * np.einsum("fd,fqd->fq", nv, vec) / r ** 3, axis=1)
+    # near pairs
+    fi, fj = np.nonzero(ir == 1)
+    if len(fi):
+        qp, qw, _ = surf.quad()
+        v = pos[fi][:, None, :] - qp[fj]
+        rr = np.maximum(np.linalg.norm(v, axis=2), tiny)
+        F[fi, fj] = -np.sum(qw[fj] * np.einsum("pd,pqd->pq", nv[fi], v) / rr ** 3, axis=1)
+    fsum = (a[:, None] * F / a[None, :]).sum(axis=0)
+    return -2.0 * np.pi - fsum
+
+
+class GreenSetup:
+    """Wavelength-independent data of CompGreenRet for a Composite whose
+    surfaces are each closed (compgreen_ret.py:70-190, 402-433, 1572-1602).
+
+    Per surface block (i, j): refinement entries (row/col local to the
+    block), g, f.  Diagonal blocks carry the closed-surface correction.
+    """
+
+    def __init__(self, comp, order=5, rel_cutoff=3.0, abs_cutoff=0.0):
+        self.comp = comp
+        self.order = order
+        self.blocks = {}
+        P = comp.parts
+        for i, si in enumerate(P):
+            for j, sj in enumerate(P):
+                ir = refinematrix(si.pos, sj.pos, si.bradius, sj.bradius, rel_cutoff, abs_cutoff)
+                if i == j:
+                    row, col, g, f = refine_coefficients(si, ir, order)
+                    corr = closed_surface_correction(si, ir)
+                    dsel = row == col
+                    f[dsel, :, 0] += corr[row[dsel]][:, None] * si.nvec[row[dsel]]
+                    self.blocks[(i, j)] = (row, col, g, f)
+                else:
+                    self.blocks[(i, j)] = _cross_coefficients(si, sj, ir, order)
+
+
+def _cross_coefficients(s1, s2, ir, order):
+    """Near-pair coefficients between two different surfaces
+    (greenret_refined.py:287-516 with p1 is not p2: no self terms)."""
+    row, col = np.nonzero(ir == 1)
+    g = np.zeros((len(row), order + 1))
+    f = np.zeros((len(row), 3, order + 1))
+    if len(row):
+        fact = [math.factorial(n) for n in range(order + 1)]
+        pts_all, w_all, _ = s2.quad()
+        pts, w = pts_all[col], w_all[col]
+        rv = s1.pos[row][:, None, :] - pts
+        r = np.maximum(np.linalg.norm(rv, axis=2), np.finfo(float).eps)
+        r0 = np.linalg.norm(s2.pos[col] - s1.pos[row], axis=1)
+        dl = r - r0[:, None]
+        for n in range(order + 1):
+            g[:, n] = np.sum(w * dl ** n / r, axis=1) / fact[n]
+            fs = -1.0 / r ** 3 if n == 0 else (
+                -dl ** n / (fact[n] * r ** 3) + dl ** (n - 1) / (fact[n - 1] * r ** 2))
+            f[:, :, n] = np.einsum("pq,pqd->pd", w * fs, rv)
+    return row, col, g, f
+
+
+def _block_green(s1, s2, same, blk, k, order):
+    """G and F (cart) of one surface pair at wavenumber k
+    (greenret_refined.py:654-800, MATLAB @greenret/private/eval1.m)."""
+    rv = s1.pos[:, None, :] - s2.pos[None, :, :]
+    d = np.maximum(np.sqrt(np.sum(rv * rv, axis=2)), np.finfo(float).eps)
+    G = (s2.area[None, :] / d).astype(complex)
+    faux = (1j * k - 1.0 / d) / d ** 2
+    F = np.einsum("id,ijd->ij", s1.nvec, rv) * faux * s2.area[None, :]
+    row, col, g, f = blk
+    if len(row):
+        ikn = (1j * k) ** np.arange(order + 1)
+        G[row, col] = g @ ikn
+        F[row, col] = np.einsum("pd,pdn,n->p", s1.nvec[row], f, ikn)
+    ph = np.exp(1j * k * d)
+    return G * ph, F * ph
+
+
+def green_matrices(setup, enei):
+    """G1, H1 (inside medium) and G2, H2 (outside medium) of the BEM system
+    (compgreen_ret.py:739-858 with connectivity :435-505; bem_ret.py:400-432):
+    block (i, j) of region r is present when surfaces i and j have the same
+    material on side r, evaluated with that material's k; H = F +/- 2 pi on the
+    self diagonal."""
+    comp = setup.comp
+    ks = [e(enei)[1] for e in comp.eps]
+    N = comp.n
+    out = []
+    for side, sgn in ((0, 1.0), (1, -1.0)):
+        G = np.zeros((N, N), complex)
+        H = np.zeros((N, N), complex)
+        for i, si in enumerate(comp.parts):
+            for j, sj in enumerate(comp.parts):
+                mi, mj = comp.inout[i, side], comp.inout[j, side]
+                if mi != mj:
+                    continue
+                Gb, Fb = _block_green(si, sj, i == j, setup.blocks[(i, j)], ks[mi - 1], setup.order)
+                if i == j:
+                    Fb = Fb + sgn * 2.0 * np.pi * np.eye(si.n)
+                ri, rj = comp.face_range(i), comp.face_range(j)
+                G[np.ix_(ri, rj)] = Gb
+                H[np.ix_(ri, rj)] = Fb
+        out += [G, H]
+    return out  # G1, H1, G2, H2
+
+
+# --------------------------------------------------------------------------
+# BEM solver  (mnpbem/bem/bem_ret.py:293-525, 1360-1831;
+#              MATLAB BEM/@bemret/private/initmat.m, BEM/@bemret/mldivide.m)
+# --------------------------------------------------------------------------
+class BEMRetOracle:
+    """Dense retarded BEM with the reference CPU algorithm: explicit G^-1 and
+    Delta^-1 from LU solves against the identity, two GEMMs, Sigma build,
+    LU of Sigma.  Scalar eps per side (single material per region)."""
+
+    def __init__(self, setup):
+        self.setup = setup
+        self.comp = setup.comp
+        self.enei = None
+
+    def init(self, enei):
+        if self.enei is not None and np.isclose(self.enei, enei):
+            return self
+        comp = self.comp
+        self.enei = enei
+        self.k = 2.0 * np.pi / enei
+        e1, e2 = comp.eps_faces(enei, 0), comp.eps_faces(enei, 1)
+        if not (np.allclose(e1, e1[0]) and np.allclose(e2, e2[0])):
+            raise NotImplementedError("oracle handles one material per side")
+        self.eps1, self.eps2 = e1[0], e2[0]
+        G1, H1, G2, H2 = green_matrices(self.setup, enei)
+        self.G1lu, self.G2lu = sla.lu_factor(G1), sla.lu_factor(G2)
+        I = np.eye(comp.n)
+        G1i, G2i = sla.lu_solve(self.G1lu, I), sla.lu_solve(self.G2lu, I)
+        self.Sigma1 = H1 @ G1i
+        Sigma2 = H2 @ G2i
+        self.Dlu = sla.lu_factor(self.Sigma1 - Sigma2)
+        Di = sla.lu_solve(self.Dlu, I)
+        L = self.eps1 - self.eps2
+        nn = comp.nvec @ comp.nvec.T
+        Sigma = self.Sigma1 * self.eps1 - Sigma2 * self.eps2 + self.k ** 2 * L * (Di * nn) * L
+        self.Slu = sla.lu_factor(Sigma)
+        self.mats = (G1, H1, G2, H2)
+        return self
+
+    def solve(self, exc):
+        """Eqs. (19)-(20) of GdA&H for excitation dict with a1, a1p, a2, a2p
+        (N, 3, npol) and optional phi*, returns sig1, sig2 (N, npol), h1, h2
+        (N, 3, npol) (bem_ret.py:1569-1831, _excitation :1360-1429)."""
+        enei = exc["enei"]
+        self.init(enei)
+        comp, k, nv = self.comp, self.k, self.comp.nvec
+        N = comp.n
+        npol = exc["a2"].shape[2]
+        z3 = np.zeros((N, 3, npol), complex)
+        z1 = np.zeros((N, npol), complex)
+        a1, a1p = exc.get("a1", z3), exc.get("a1p", z3)
+        a2, a2p = exc.get("a2", z3), exc.get("a2p", z3)
+        phi1, phi1p = exc.get("phi1", z1), exc.get("phi1p", z1)
+        phi2, phi2p = exc.get("phi2", z1), exc.get("phi2p", z1)
+        e1, e2 = self.eps1, self.eps2
+        phi = phi2 - phi1
+        a = a2 - a1
+        alpha = (a2p - a1p) - 1j * k * (nv[:, :, None] * (phi2 * e2 - phi1 * e1)[:, None, :])
+        De = (e2 * phi2p - e1 * phi1p) - 1j * k * (
+            np.einsum("id,idp->ip", nv, a2) * e2 - np.einsum("id,idp->ip", nv, a1) * e1)
+        lsv = lambda lu, b: sla.lu_solve(lu, b.reshape(N, -1)).reshape(b.shape)
+        S1 = self.Sigma1
+        alpha = alpha - (S1 @ a.reshape(N, -1)).reshape(N, 3, npol) \
+            + 1j * k * nv[:, :, None] * (e1 * phi)[:, None, :]
+        De = De - S1 @ (e1 * phi) + 1j * k * np.einsum("id,idp->ip", nv, e1 * a)
+        Ld = e1 - e2
+        inner = np.einsum("id,idp->ip", nv, Ld * lsv(self.Dlu, alpha))
+        sig2 = lsv(self.Slu, De + 1j * k * inner)
+        h2 = lsv(self.Dlu, 1j * k * nv[:, :, None] * (Ld * sig2)[:, None, :] + alpha)
+        return {
+            "enei": enei,
+            "sig1": lsv(self.G1lu, sig2 + phi), "h1": lsv(self.G1lu, h2 + a),
+            "sig2": lsv(self.G2lu, sig2), "h2": lsv(self.G2lu, h2),
+        }
+
+
+def bem_operator_apply(mats, eps1, eps2, k, nvec, sig):
+    """The 8N BEM operator of GdA&H Eqs. (10), (11), (14), (17) applied to
+    (sig1, h1, sig2, h2) -> (phi, a, alpha, De) (bem_ret_iter.py:1362-1501,
+    scalar-eps branch).  Used to check the direct solve."""
+    G1, H1, G2, H2 = mats
+    N = len(nvec)
+    s1, h1, s2, h2 = sig["sig1"], sig["h1"], sig["sig2"], sig["h2"]
+    mv = lambda M, x: (M @ x.reshape(N, -1)).reshape(x.shape)
+    Gs1, Gh1, Gs2, Gh2 = mv(G1, s1), mv(G1, h1), mv(G2, s2), mv(G2, h2)
+    Hs1, Hh1, Hs2, Hh2 = mv(H1, s1), mv(H1, h1), mv(H2, s2), mv(H2, h2)
+    phi = Gs1 - Gs2
+    a = Gh1 - Gh2
+    alpha = Hh1 - Hh2 - 1j * k * nvec[:, :, None] * (eps1 * Gs1 - eps2 * Gs2)[:, None, :]
+    De = eps1 * Hs1 - eps2 * Hs2 - 1j * k * np.einsum("id,idp->ip", nvec, eps1 * Gh1 - eps2 * Gh2)
+    return phi, a, alpha, De
+
+
+# --------------------------------------------------------------------------
+# Plane wave and far field  (mnpbem/simulation/planewave_ret.py:257-504;
+#                            mnpbem/spectrum/spectrum_ret.py:14-459)
+# --------------------------------------------------------------------------
+def planewave_potential(comp, pol, dirs, enei, medium=1):
+    """a = exp(i k pos.dir)/(i k0) pol and a' = i k (nvec.dir) a on faces whose
+    side touches ``medium`` (planewave_ret.py:257-361).  Returns excitation
+    dict with a1, a1p (inside side), a2, a2p (outside side), (N, 3, npol)."""
+    pol, dirs = np.atleast_2d(pol).astype(float), np.atleast_2d(dirs).astype(float)
+    k0 = 2.0 * np.pi / enei
+    k = k0 * np.sqrt(comp.eps[medium - 1](enei)[0])
+    N, npol = comp.n, len(pol)
+    exc = {"enei": enei}
+    for side, name in ((0, "a1"), (1, "a2")):
+        a = np.zeros((N, 3, npol), complex)
+        ap = np.zeros((N, 3, npol), complex)
+        sel = np.nonzero(comp.inout_faces[:, side] == medium)[0]
+        for ip in range(npol):
+            ph = np.exp(1j * k * comp.pos[sel] @ dirs[ip]) / (1j * k0)
+            a[sel, :, ip] = ph[:, None] * pol[ip]
+            ap[sel, :, ip] = (1j * k * (comp.nvec[sel] @ dirs[ip]) * ph)[:, None] * pol[ip]
+        exc[name], exc[name + "p"] = a, ap
+    return exc
+
+
+def farfield(comp, sig, dirs, enei, medium=1):
+    """Far-field amplitudes e, h (ndir, 3, npol) from sig1/h1 on faces whose
+    inside is ``medium`` and sig2/h2 on faces whose outside is ``medium``
+    (spectrum_ret.py:231-392)."""
+    k = comp.eps[medium - 1](enei)[1]
+    k0 = 2.0 * np.pi / enei
+    dirs = np.atleast_2d(dirs)
+    phase = np.exp(-1j * k * dirs @ comp.pos.T) * comp.area[None, :]
+    npol = sig["sig2"].shape[1]
+    e = np.zeros((len(dirs), 3, npol), complex)
+    h = np.zeros((len(dirs), 3, npol), complex)
+    for side, s_key, h_key in ((0, "sig1", "h1"), (1, "sig2", "h2")):
+        sel = np.nonzero(comp.inout_faces[:, side] == medium)[0]
+        if not len(sel):
+            continue
+        ph = phase[:, sel]
+        for ip in range(npol):
+            ph_h = ph @ sig[h_key][sel, :, ip]
+            ph_s = ph @ sig[s_key][sel, ip]
+            e[:, :, ip] += 1j * k0 * ph_h - 1j * k * dirs * ph_s[:, None]
+            h[:, :, ip] += 1j * k * np.cross(dirs, ph_h)
+    return e, h
+
+
+def extinction(comp, sig, pol, dirs, enei, medium=1):
+    """Optical theorem: ext_i = 4 pi / k Im(conj(pol_i) . e_i(dir_i))
+    (planewave_ret.py:437-504)."""
+    pol = np.atleast_2d(pol)
+    k = comp.eps[medium - 1](enei)[1]
+    e, _ = farfield(comp, sig, dirs, enei, medium)
+    val = np.array([np.sum(np.conj(pol[i]) * e[i, :, i]) for i in range(len(pol))])
+    return np.real(4.0 * np.pi / k * np.imag(val))
+
+
+def read_pinfty(path=None):
+    """pinfty256.bin: int32 n; f64 nx, ny, nz [n]; then pos (3n) + area (n)
+    or area (n) only (spectrum_ret.py:14-45)."""
+    path = path or os.path.join(DATA_DIR, "pinfty256.bin")
+    with open(path, "rb") as fh:
+        n = int(np.fromfile(fh, dtype=np.int32, count=1)[0])
+        nvec = np.fromfile(fh, dtype=np.float64, count=3 * n).reshape(3, n).T
+        rest = np.fromfile(fh, dtype=np.float64)
+    area = rest[3 * n:4 * n] if rest.size == 4 * n else rest[:n]
+    return nvec, area
+
+
+def scattering(comp, sig, enei, medium=1, pinfty=None):
+    """sca = sum_dir area * 0.5 Re(n . (e x conj h)) / (0.5 nb)
+    (spectrum_ret.py:395-459, planewave_ret.py:391-435)."""
+    nv, ar = pinfty if pinfty is not None else read_pinfty()
+    e, h = farfield(comp, sig, nv, enei, medium)
+    dsca = 0.5 * np.real(np.einsum("dc,dcp->dp", nv, np.cross(e, np.conj(h), axis=1)))
+    nb = np.real(np.sqrt(comp.eps[medium - 1](enei)[0]))
+    return (ar @ dsca) / (0.5 * nb)
+
+
+def solve_spectrum(comp, enei_list, pol, dirs, with_sca=False, setup=None):
+    """User loop of the reference validation scripts
+    (validation/03_bemret/sphere/run_python.py:27-45): per wavelength
+    potential -> solve -> extinction (and scattering)."""
+    setup = setup or GreenSetup(comp)
+    bem = BEMRetOracle(setup)
+    ext, sca = [], []
+    for lam in enei_list:
+        exc = planewave_potential(comp, pol, dirs, lam)
+        sig = bem.solve(exc)
+        ext.append(extinction(comp, sig, pol, dirs, lam))
+        if with_sca:
+            sca.append(scattering(comp, sig, lam))
+    ext = np.array(ext)
+    return (ext, np.array(sca)) if with_sca else ext
