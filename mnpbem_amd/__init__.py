@@ -1,0 +1,28 @@
+"""MI355X-native retarded BEM (MNPBEM's BEMRet path) on HIP kernels.
+
+Public API mirrors mnpbem: materials (EpsConst, EpsDrude, EpsTable),
+geometry (trisphere, Particle, ComParticle), CompGreenRet, BEMRet,
+PlaneWaveRet and CompStruct.  Device work runs in libmnpbem_amd.so
+(built by ``python -m mnpbem_amd.build``); there is no CPU fallback.
+"""
+
+from .materials import EV2NM, EpsConst, EpsDrude, EpsTable
+from .geometry import ComParticle, Particle, trisphere
+
+__all__ = ["EV2NM", "EpsConst", "EpsDrude", "EpsTable", "ComParticle", "Particle", "trisphere",
+           "CompGreenRet", "BEMRet", "CompStruct", "PlaneWaveRet"]
+
+
+def __getattr__(name):
+    # device-side classes are imported lazily so that host-only use (mesh,
+    # materials) does not need torch or the HIP library
+    if name == "CompGreenRet":
+        from .greenfun import CompGreenRet
+        return CompGreenRet
+    if name in ("BEMRet", "CompStruct"):
+        from . import bem
+        return getattr(bem, name)
+    if name == "PlaneWaveRet":
+        from .planewave import PlaneWaveRet
+        return PlaneWaveRet
+    raise AttributeError(name)

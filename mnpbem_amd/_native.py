@@ -1,0 +1,86 @@
+"""ctypes binding of ``libmnpbem_amd.so`` (the C ABI declared in
+include/mnpbem_amd.h).
+
+The library is required: there is no CPU fallback for the BEM path.  Loading
+fails with ImportError when the shared object has not been built
+(``python -m mnpbem_amd.build``), and every entry point raises RuntimeError
+with the library's message when it returns a non-zero status.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmnpbem_amd.so")
+
+DTYPE_C64 = 1
+DTYPE_C128 = 2
+
+_i, _l, _d, _p = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
+
+# name -> argument types (all return int status)
+SIGNATURES = {
+    "mnpbem_abi_version": [],
+    "mnpbem_device_sync": [],
+    "mnpbem_cgemm_sub_batched": [_i, _i, _i, _i, _p, _l, _l, _p, _l, _l, _p, _l, _l, _i, _p],
+    "mnpbem_getrf_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _i, _p],
+    "mnpbem_getrs_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _i, _p],
+    "mnpbem_refine_coeffs": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _p],
+    "mnpbem_closed_correction": [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _i, _p, _p, _p],
+    "mnpbem_green_assemble": [_i, _i, _i, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _d, _p, _p, _l, _l,
+                              _p, _p, _i, _p],
+    "mnpbem_gemv_t_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _l, _p, _l, _l, _d, _d, _d, _i, _p],
+    "mnpbem_lu_solve_t_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _i, _p],
+    "mnpbem_identity_batched": [_i, _i, _p, _l, _l, _i, _p],
+    "mnpbem_sigma_batched": [_i, _i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _i, _p],
+    "mnpbem_bem_solve_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l, _l,
+                                 _p, _p, _i, _p],
+    "mnpbem_bem_residual_batched": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _l, _l, _p, _l, _l, _p, _p,
+                                    _p, _i, _p],
+    "mnpbem_accumulate": [_i, _l, _p, _p, _p],
+    "mnpbem_convert": [_i, _l, _p, _p, _p],
+    "mnpbem_planewave_batched": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _p, _l, _i, _p],
+    "mnpbem_extinction_batched": [_i, _i, _p, _l, _l, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p],
+}
+
+_lib = None
+
+
+def load(path=None):
+    """Load the shared library and declare its entry points."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise ImportError("mnpbem_amd: HIP library %s is not built; run `python -m mnpbem_amd.build`" % path)
+    lib = ctypes.CDLL(path)
+    for name, argtypes in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = ctypes.c_int
+    lib.mnpbem_last_error.argtypes = []
+    lib.mnpbem_last_error.restype = ctypes.c_char_p
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call an entry point and raise RuntimeError on a non-zero status."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.mnpbem_last_error().decode(errors="replace")
+        raise RuntimeError("%s failed (status %d): %s" % (name, rc, msg))
+    return rc
+
+
+def ptr(t):
+    """Raw device (or host) address of a torch tensor, or None."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(device=None):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

@@ -1,0 +1,261 @@
+"""Dense retarded BEM solver on the GPU, batched over wavelengths.
+
+``BEMRet`` keeps the interface of mnpbem/bem/bem_ret.py (``init(enei)``,
+``solve(exc) -> (sig, self)``, ``bem \\ exc`` as ``bem / exc``,
+``bem(enei)``) and adds ``solve_sweep`` which factors and solves a whole
+wavelength sweep in batched launches.
+
+Per wavelength (bem_ret.py:293-525; MATLAB BEM/@bemret/private/initmat.m):
+G1, H1 (inside k) and G2, H2 (outside k) are assembled; with the matrices
+stored transposed, LU(G^T) and the LU solve of H^T give Sigma^T = (H G^-1)^T;
+Delta^T = Sigma1^T - Sigma2^T is factored and inverted; Sigma^T =
+eps1 Sigma1^T - eps2 Sigma2^T + k^2 (eps1 - eps2)^2 (Delta^-1)^T o (n n^T) is
+factored.  The solve is GdA&H Eqs. (19)-(20) (bem_ret.py:1569-1831).
+
+Precision.  ``precision='double'`` runs everything in complex128.
+``precision='mixed'`` assembles G/H once in complex128 and keeps a complex64
+copy; the factorisations and the direct solve run in complex64, and the
+solution is refined in FP64 against the complex128 8N operator of GdA&H
+Eqs. (10, 11, 14, 17) until ||R - A X|| <= refine_tol ||R|| for every
+wavelength (at most ``refine_maxit`` corrections).
+"""
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .greenfun import CompGreenRet
+
+
+class CompStruct:
+    """Container of named fields with attribute and item access
+    (mnpbem/greenfun/compgreen_stat.py:1154)."""
+
+    def __init__(self, p, enei, **fields):
+        self.p = p
+        self.enei = enei
+        self._fields = dict(fields)
+
+    def __getattr__(self, name):
+        f = self.__dict__.get("_fields", {})
+        if name in f:
+            return f[name]
+        raise AttributeError(name)
+
+    def __getitem__(self, name):
+        if name == "enei":
+            return self.enei
+        return self._fields[name]
+
+    def get(self, name, default=None):
+        return self._fields.get(name, default)
+
+    def set(self, **fields):
+        self._fields.update(fields)
+        return self
+
+    def keys(self):
+        return list(self._fields.keys())
+
+
+class _Factorization:
+    """Device buffers and factors of one batch of wavelengths."""
+
+
+def _par_tensor(eps_in, eps_out, k0, device):
+    par = np.stack([np.real(eps_in), np.imag(eps_in), np.real(eps_out), np.imag(eps_out), k0], axis=1)
+    return torch.as_tensor(np.ascontiguousarray(par), dtype=torch.float64).to(device)
+
+
+class BEMRet:
+    """Retarded BEM solver (dense, direct) for a ComParticle."""
+
+    name = "bemsolver"
+    needs = {"sim": "ret"}
+
+    def __init__(self, p, enei=None, precision="double", green=None, refine_tol=1e-12, refine_maxit=8,
+                 **options):
+        if precision not in ("double", "mixed"):
+            raise ValueError("BEMRet: precision must be 'double' or 'mixed'")
+        self.p = p
+        self.precision = precision
+        self.refine_tol = refine_tol
+        self.refine_maxit = refine_maxit
+        self.g = green if green is not None else CompGreenRet(p, p, **options)
+        self.device = self.g.device
+        self.enei = None
+        self._fac = None
+        self.last_refine_iters = None
+        self.last_residual = None
+        io = p.inout
+        if p.np > 1 and (len(np.unique(io[:, 0])) > 1 or len(np.unique(io[:, 1])) > 1):
+            raise NotImplementedError("BEMRet: one material per side (scalar eps1, eps2) is supported")
+        if np.any(self.g.con[0][1]) or np.any(self.g.con[1][0]):
+            raise NotImplementedError("BEMRet: nested particles (connected inside/outside regions)")
+        if enei is not None:
+            self.init(enei)
+
+    # ------------------------------------------------------------------
+    def _eps_sides(self, enei_list):
+        p = self.p
+        m_in, m_out = p.inout[0, 0], p.inout[0, 1]
+        e_in = np.array([complex(p.eps[m_in - 1](lam)[0]) for lam in enei_list])
+        e_out = np.array([complex(p.eps[m_out - 1](lam)[0]) for lam in enei_list])
+        return e_in, e_out
+
+    def factor(self, enei_list):
+        """Assemble and factor the BEM matrices for a batch of wavelengths."""
+        enei_list = np.atleast_1d(np.asarray(enei_list, dtype=float))
+        g, dev = self.g, self.device
+        B, n = len(enei_list), g.n
+        mixed = self.precision == "mixed"
+        cdt = torch.complex64 if mixed else torch.complex128
+        dt = nat.DTYPE_C64 if mixed else nat.DTYPE_C128
+        st = nat.stream_handle(dev)
+        F = _Factorization()
+        F.enei = enei_list
+        F.B, F.n, F.dtype, F.cdt = B, n, dt, cdt
+        kmat = g.wavenumbers(enei_list)
+        e_in, e_out = self._eps_sides(enei_list)
+        k0 = 2.0 * np.pi / enei_list
+        F.par = _par_tensor(e_in, e_out, k0, dev)
+        F.e_in, F.e_out, F.k0 = e_in, e_out, k0
+        nn = n * n
+        # GG = [G1^T ; G2^T], HH = [H1^T ; H2^T] stacked so both sides factor in one batch
+        GG = torch.empty((2, B, n, n), dtype=cdt, device=dev)
+        HH = torch.empty((2, B, n, n), dtype=cdt, device=dev)
+        if mixed:
+            F.G64 = torch.empty((2, B, n, n), dtype=torch.complex128, device=dev)
+            F.H64 = torch.empty((2, B, n, n), dtype=torch.complex128, device=dev)
+            for s, hs in ((0, 1.0), (1, -1.0)):
+                g.assemble(s, kmat, F.G64[s], F.H64[s], hs, nat.DTYPE_C128, GG[s], HH[s])
+        else:
+            for s, hs in ((0, 1.0), (1, -1.0)):
+                g.assemble(s, kmat, GG[s], HH[s], hs, nat.DTYPE_C128)
+        piv = torch.empty((4, B, n), dtype=torch.int32, device=dev)  # G1, G2, Delta, Sigma
+        info = torch.empty((4, B), dtype=torch.int32, device=dev)
+        # LU(G^T), then Sigma^T = (G^T)^-1 H^T for both sides in one batch of 2B
+        nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(GG), n, nn, nat.ptr(piv), n, nat.ptr(info), 2 * B, st)
+        nat.call("mnpbem_getrs_batched", dt, n, n, nat.ptr(GG), n, nn, nat.ptr(piv), n, nat.ptr(HH), n, nn,
+                 2 * B, st)
+        S1, S2 = HH[0], HH[1]
+        Dt = torch.empty((B, n, n), dtype=cdt, device=dev)
+        nat.call("mnpbem_sigma_batched", dt, 0, n, nat.ptr(S1), nat.ptr(S2), nat.ptr(Dt), None, n, nn,
+                 nat.ptr(g.nvec), nat.ptr(F.par), B, st)
+        nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(info[2]), B, st)
+        Et = torch.empty((B, n, n), dtype=cdt, device=dev)
+        nat.call("mnpbem_identity_batched", dt, n, nat.ptr(Et), n, nn, B, st)
+        nat.call("mnpbem_getrs_batched", dt, n, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(Et), n, nn, B,
+                 st)
+        del Dt
+        # Sigma^T overwrites Sigma2^T
+        nat.call("mnpbem_sigma_batched", dt, 1, n, nat.ptr(S1), nat.ptr(S2), None, nat.ptr(Et), n, nn,
+                 nat.ptr(g.nvec), nat.ptr(F.par), B, st)
+        nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(piv[3]), n, nat.ptr(info[3]), B, st)
+        F.GG, F.HH, F.Et, F.piv, F.info = GG, HH, Et, piv, info
+        return F
+
+    def _solve_direct(self, F, R, X, W, P):
+        n = F.n
+        nn = n * n
+        nat.call("mnpbem_bem_solve_batched", F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.piv[0]),
+                 nat.ptr(F.GG[1]), nat.ptr(F.piv[1]), nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]),
+                 nat.ptr(F.piv[3]), n, nn, n, nat.ptr(R), nat.ptr(W), nat.ptr(X), 8 * P, n * 8 * P,
+                 nat.ptr(self.g.nvec), nat.ptr(F.par), F.B, nat.stream_handle(self.device))
+
+    def solve_batch(self, F, R):
+        """Solve A X = R for the batch (R: (B, n, 8P) complex128 on device).
+        Returns X (B, n, 8P) complex128."""
+        B, n = F.B, F.n
+        P = R.shape[2] // 8
+        dev = self.device
+        st = nat.stream_handle(dev)
+        if F.dtype == nat.DTYPE_C128:
+            X = torch.empty_like(R)
+            W = torch.empty_like(R)
+            self._solve_direct(F, R, X, W, P)
+            self.last_refine_iters = 0
+            return X
+        # mixed: complex64 direct solve + FP64 refinement
+        R32 = torch.empty(R.shape, dtype=torch.complex64, device=dev)
+        X32 = torch.empty_like(R32)
+        W32 = torch.empty_like(R32)
+        X = torch.zeros_like(R)
+        Rres = torch.empty_like(R)
+        Y = torch.empty((B, n, 16 * P), dtype=torch.complex128, device=dev)
+        nrm = torch.empty((B, 2), dtype=torch.float64, device=dev)
+        cnt = 2 * R.numel()
+        src = R
+        it = 0
+        rel = None
+        while True:
+            nat.call("mnpbem_convert", nat.DTYPE_C64, cnt, nat.ptr(src), nat.ptr(R32), st)
+            self._solve_direct(F, R32, X32, W32, P)
+            nat.call("mnpbem_accumulate", nat.DTYPE_C64, cnt, nat.ptr(X32), nat.ptr(X), st)
+            it += 1
+            nat.call("mnpbem_bem_residual_batched", n, P, nat.ptr(F.G64[0]), nat.ptr(F.H64[0]),
+                     nat.ptr(F.G64[1]), nat.ptr(F.H64[1]), n, n * n, nat.ptr(R), nat.ptr(X), nat.ptr(Rres),
+                     8 * P, n * 8 * P, nat.ptr(Y), 16 * P, n * 16 * P, nat.ptr(self.g.nvec), nat.ptr(F.par),
+                     nat.ptr(nrm), B, st)
+            h = nrm.cpu().numpy()
+            rel = np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300))
+            if np.all(rel <= self.refine_tol) or it > self.refine_maxit:
+                break
+            src = Rres
+        self.last_refine_iters = it
+        self.last_residual = rel
+        return X
+
+    # ------------------------------------------------------------------
+    def init(self, enei):
+        if self.enei is not None and np.isclose(self.enei, enei):
+            return self
+        self._fac = self.factor([enei])
+        self.enei = enei
+        return self
+
+    def __call__(self, enei):
+        return self.init(enei)
+
+    def solve(self, exc):
+        """Surface charges and currents sig1, sig2, h1, h2 for a plane-wave
+        excitation from PlaneWaveRet.potential (bem_ret.py:1569-1831)."""
+        enei = exc.enei
+        self.init(enei)
+        R = exc.get("_R")
+        if R is None:
+            raise NotImplementedError("BEMRet.solve: excitations other than PlaneWaveRet.potential")
+        X = self.solve_batch(self._fac, R)
+        return _unpack(self.p, enei, X[0], R.shape[2] // 8), self
+
+    def __truediv__(self, exc):
+        return self.solve(exc)[0]
+
+    def solve_sweep(self, enei_list, planewave, chunk=16, with_x=False):
+        """Extinction spectrum (n_lambda, npol) of a plane-wave sweep, factored
+        and solved ``chunk`` wavelengths per batched launch."""
+        enei_list = np.atleast_1d(np.asarray(enei_list, dtype=float))
+        out = []
+        xs = []
+        for c0 in range(0, len(enei_list), chunk):
+            lam = enei_list[c0:c0 + chunk]
+            F = self.factor(lam)
+            R = planewave.rhs(self.p, lam, self.g)
+            X = self.solve_batch(F, R)
+            out.append(planewave.extinction_batch(self.p, lam, X, self.g))
+            if with_x:
+                xs.append(X)
+            del F
+        ext = torch.cat(out).cpu().numpy()
+        return (ext, torch.cat(xs)) if with_x else ext
+
+
+def _unpack(p, enei, X, P):
+    n = X.shape[0]
+    sig1 = X[:, 0:P]
+    h1 = X[:, P:4 * P].reshape(n, 3, P)
+    sig2 = X[:, 4 * P:5 * P]
+    h2 = X[:, 5 * P:8 * P].reshape(n, 3, P)
+    if P == 1:
+        sig1, sig2, h1, h2 = sig1[:, 0], sig2[:, 0], h1[:, :, 0], h2[:, :, 0]
+    return CompStruct(p, enei, sig1=sig1, sig2=sig2, h1=h1, h2=h2, _X=X[None])

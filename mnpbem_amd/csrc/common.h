@@ -1,0 +1,138 @@
+// Shared device/host helpers for the MNPBEM retarded-BEM HIP library (gfx950).
+//
+// Complex numbers are stored interleaved (re, im), the layout of numpy /
+// torch complex128 and complex64, so device buffers are torch tensors handed
+// over as raw pointers.  Matrix buffers are row-major with a leading
+// dimension counted in complex elements.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <string>
+
+#define MNP_DTYPE_C64 1   // complex64  (float pairs)
+#define MNP_DTYPE_C128 2  // complex128 (double pairs)
+
+// ---------------------------------------------------------------------------
+// error reporting across the C ABI: status code + thread-local message
+// ---------------------------------------------------------------------------
+void mnp_set_error(const std::string& msg);
+
+#define MNP_HIP_CHECK(expr)                                                       \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      mnp_set_error(std::string(#expr) + ": " + hipGetErrorString(_e) + " (" +    \
+                    __FILE__ + ":" + std::to_string(__LINE__) + ")");             \
+      return 2;                                                                   \
+    }                                                                             \
+  } while (0)
+
+#define MNP_REQUIRE(cond, msg)          \
+  do {                                  \
+    if (!(cond)) {                      \
+      mnp_set_error(std::string(msg));  \
+      return 1;                         \
+    }                                   \
+  } while (0)
+
+#define MNP_LAUNCH_CHECK() MNP_HIP_CHECK(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// complex arithmetic (T = float or double)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct cplx {
+  T re, im;
+};
+
+template <typename T>
+struct vec2_of;
+template <>
+struct vec2_of<double> {
+  typedef double2 type;
+};
+template <>
+struct vec2_of<float> {
+  typedef float2 type;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmake(T r, T i) {
+  cplx<T> z;
+  z.re = r;
+  z.im = i;
+  return z;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
+  return cmake(a.re + b.re, a.im + b.im);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
+  return cmake(a.re - b.re, a.im - b.im);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return cmake(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) {
+  return cmake(a.re * s, a.im * s);
+}
+// a - b*c
+template <typename T>
+__device__ __forceinline__ cplx<T> cfms(cplx<T> a, cplx<T> b, cplx<T> c) {
+  return cmake(a.re - (b.re * c.re - b.im * c.im), a.im - (b.re * c.im + b.im * c.re));
+}
+// a + b*c
+template <typename T>
+__device__ __forceinline__ cplx<T> cfma(cplx<T> a, cplx<T> b, cplx<T> c) {
+  return cmake(a.re + (b.re * c.re - b.im * c.im), a.im + (b.re * c.im + b.im * c.re));
+}
+// 1 / a, scaled to avoid overflow (Smith's algorithm)
+template <typename T>
+__device__ __forceinline__ cplx<T> crecip(cplx<T> a) {
+  T ar = a.re, ai = a.im;
+  if (fabs(ar) >= fabs(ai)) {
+    T r = ai / ar, d = ar + ai * r;
+    return cmake(T(1) / d, -r / d);
+  } else {
+    T r = ar / ai, d = ai + ar * r;
+    return cmake(r / d, T(-1) / d);
+  }
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cdiv(cplx<T> a, cplx<T> b) {
+  return cmul(a, crecip(b));
+}
+template <typename T>
+__device__ __forceinline__ T cabs1(cplx<T> a) {
+  return fabs(a.re) + fabs(a.im);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cld(const T* p) {
+  typedef typename vec2_of<T>::type V;
+  V v = *reinterpret_cast<const V*>(p);
+  return cmake(v.x, v.y);
+}
+template <typename T>
+__device__ __forceinline__ void cst(T* p, cplx<T> z) {
+  typedef typename vec2_of<T>::type V;
+  V v;
+  v.x = z.re;
+  v.y = z.im;
+  *reinterpret_cast<V*>(p) = v;
+}
+// exp(i z) for complex z = zr + i zi  ->  e^{-zi} (cos zr + i sin zr)
+template <typename T>
+__device__ __forceinline__ cplx<T> cexpi(T zr, T zi) {
+  T s, c;
+  sincos(zr, &s, &c);
+  T e = exp(-zi);
+  return cmake(e * c, e * s);
+}
+
+inline int mnp_ceil_div(long a, long b) { return (int)((a + b - 1) / b); }

@@ -1,0 +1,555 @@
+// Batched dense complex linear algebra on CDNA4 (gfx950): MFMA complex GEMM,
+// recursive LU with partial pivoting, recursive triangular solves, and the
+// LU-solve with many right-hand sides.
+//
+// These replace scipy.linalg.lu_factor / lu_solve (LAPACK zgetrf / zgetrs) as
+// called by BEMRet.init (mnpbem/bem/bem_ret.py:437-523, via
+// mnpbem/utils/gpu.py:145-279).  Matrices are row-major, complex interleaved,
+// ``batch`` matrices of one size spaced ``stride`` complex elements apart.
+// Pivot choice follows LAPACK: largest |re|+|im| in the column, first index
+// on ties, no scaling of the column when the pivot is exactly zero (the
+// 1-based column is then reported in ``info``).
+#include "common.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// MFMA traits: 16x16x4 single-block matrix instructions.
+// Operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four
+// accumulator values of lane l are D[4*(l/16) + r][l%16], r = 0..3.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// C[b] -= A[b] * B[b]   (complex, row-major; M x K times K x N)
+// Workgroup: 256 threads = 4 waves, 64x64 output tile, each wave 32x32 as
+// 2x2 blocks of 16x16.  K is staged through LDS in slices of 16 with the real
+// and imaginary parts split into separate planes; a complex product is four
+// real MFMAs (re*re - im*im, re*im + im*re) accumulated in FP32/FP64.
+// ---------------------------------------------------------------------------
+constexpr int GT = 64;   // output tile edge
+constexpr int GK = 16;   // K slice
+constexpr int GP = 1;    // LDS row padding (elements)
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_cgemm_sub(int M, int N, int K, const T* __restrict__ A,
+                                                   long lda, long sA, const T* __restrict__ B, long ldb,
+                                                   long sB, T* __restrict__ C, long ldc, long sC) {
+  typedef typename Mfma<T>::v4 v4;
+  const long bz = blockIdx.z;
+  A += 2 * bz * sA;
+  B += 2 * bz * sB;
+  C += 2 * bz * sC;
+  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  __shared__ T Ar[GK][GT + GP], Ai[GK][GT + GP], Br[GK][GT + GP], Bi[GK][GT + GP];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  // global load assignment: A tile 64 x 16 -> row tid/4, k (tid%4)*4 .. +3
+  const int la_r = tid >> 2, la_k = (tid & 3) * 4;
+  // B tile 16 x 64 -> k tid/16, col (tid%16)*4 .. +3
+  const int lb_k = tid >> 4, lb_n = (tid & 15) * 4;
+
+  cplx<T> ra[4], rb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int gr = m0 + la_r, gk = k0 + la_k + q;
+      ra[q] = (gr < M && gk < K) ? cld(A + 2 * ((long)gr * lda + gk)) : cmake<T>(0, 0);
+      int gk2 = k0 + lb_k, gc = n0 + lb_n + q;
+      rb[q] = (gk2 < K && gc < N) ? cld(B + 2 * ((long)gk2 * ldb + gc)) : cmake<T>(0, 0);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      Ar[la_k + q][la_r] = ra[q].re;
+      Ai[la_k + q][la_r] = ra[q].im;
+      Br[lb_k][lb_n + q] = rb[q].re;
+      Bi[lb_k][lb_n + q] = rb[q].im;
+    }
+  };
+
+  v4 accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = v4{0, 0, 0, 0};
+      accI[i][j] = v4{0, 0, 0, 0};
+    }
+
+  gload(0);
+  sstore();
+  __syncthreads();
+  for (int k0 = 0; k0 < K; k0 += GK) {
+    const bool more = (k0 + GK) < K;
+    if (more) gload(k0 + GK);
+#pragma unroll
+    for (int kk = 0; kk < GK; kk += 4) {
+      const int kl = kk + (lane >> 4);
+      T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = Ar[kl][wm + 16 * i + (lane & 15)];
+        ai[i] = Ai[kl][wm + 16 * i + (lane & 15)];
+        br[i] = Br[kl][wn + 16 * i + (lane & 15)];
+        bi[i] = Bi[kl][wn + 16 * i + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = Mfma<T>::mma(ar[i], br[j], accR[i][j]);
+          accR[i][j] = Mfma<T>::mma(-ai[i], bi[j], accR[i][j]);
+          accI[i][j] = Mfma<T>::mma(ar[i], bi[j], accI[i][j]);
+          accI[i][j] = Mfma<T>::mma(ai[i], br[j], accI[i][j]);
+        }
+    }
+    __syncthreads();
+    if (more) {
+      sstore();
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int gc = n0 + wn + 16 * j + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        if (gr < M && gc < N) {
+          T* p = C + 2 * ((long)gr * ldc + gc);
+          cplx<T> c = cld(p);
+          c.re -= accR[i][j][r];
+          c.im -= accI[i][j][r];
+          cst(p, c);
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LU of the panel A[c0:N, c0:c0+w] (w <= W) with partial pivoting, one
+// workgroup per matrix.  Thread t holds rows t, t+NT, ... (S rows) of the
+// panel in registers; per column: workgroup argmax of |re|+|im|, row
+// exchange through LDS, scaling and rank-1 update of the remaining columns.
+// ipiv receives global (0-based) row indices.  Swaps are applied only inside
+// the panel columns; the callers apply them to the other columns.
+// ---------------------------------------------------------------------------
+template <typename T, int W, int S, int NT>
+__global__ __launch_bounds__(NT) void k_lu_leaf(int N, int c0, int w, T* A, long lda, long sA,
+                                                int* ipiv, long sP, int* info) {
+  const long b = blockIdx.x;
+  A += 2 * b * sA;
+  ipiv += b * sP;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int NW = NT / 64;
+  const int m = N - c0;
+  cplx<T> a[S][W];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int r = tid + s * NT;
+#pragma unroll
+    for (int c = 0; c < W; ++c)
+      a[s][c] = (r < m && c < w) ? cld(A + 2 * ((long)(c0 + r) * lda + c0 + c)) : cmake<T>(0, 0);
+  }
+  __shared__ T sval[NW];
+  __shared__ int sidx[NW];
+  __shared__ int spiv;
+  __shared__ cplx<T> rowc[W], rowp[W];
+
+#pragma unroll
+  for (int c = 0; c < W; ++c) {
+    if (c < w) {
+    T best = T(-1);
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int r = tid + s * NT;
+      if (r >= c && r < m) {
+        const T v = cabs1(a[s][c]);
+        if (v > best) {
+          best = v;
+          bi = r;
+        }
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const T ov = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > best || (ov == best && oi < bi)) {
+        best = ov;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      sval[wv] = best;
+      sidx[wv] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      T bb = sval[0];
+      int ii = sidx[0];
+      for (int q = 1; q < NW; ++q)
+        if (sval[q] > bb || (sval[q] == bb && sidx[q] < ii)) {
+          bb = sval[q];
+          ii = sidx[q];
+        }
+      if (ii < c || ii >= m) ii = c;  // only NaN entries left: keep the diagonal
+      spiv = ii;
+      ipiv[c0 + c] = c0 + ii;
+    }
+    __syncthreads();
+    const int p = spiv;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int r = tid + s * NT;
+      if (r == p)
+#pragma unroll
+        for (int cc = 0; cc < W; ++cc) rowp[cc] = a[s][cc];
+      if (r == c)
+#pragma unroll
+        for (int cc = 0; cc < W; ++cc) rowc[cc] = a[s][cc];
+    }
+    __syncthreads();
+    if (p != c) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int r = tid + s * NT;
+        if (r == c)
+#pragma unroll
+          for (int cc = 0; cc < W; ++cc) a[s][cc] = rowp[cc];
+        if (r == p)
+#pragma unroll
+          for (int cc = 0; cc < W; ++cc) a[s][cc] = rowc[cc];
+      }
+    }
+    const cplx<T> pv = rowp[c];
+    const bool zero = (pv.re == T(0) && pv.im == T(0));
+    if (zero && tid == 0 && info[b] == 0) info[b] = c0 + c + 1;
+    const cplx<T> inv = zero ? cmake<T>(1, 0) : crecip(pv);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int r = tid + s * NT;
+      if (r > c && r < m) {
+        const cplx<T> l = cmul(a[s][c], inv);
+        a[s][c] = l;
+#pragma unroll
+        for (int cc = c + 1; cc < W; ++cc) a[s][cc] = cfms(a[s][cc], l, rowp[cc]);
+      }
+    }
+    __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int r = tid + s * NT;
+    if (r < m)
+#pragma unroll
+      for (int c = 0; c < W; ++c)
+        if (c < w) cst(A + 2 * ((long)(c0 + r) * lda + c0 + c), a[s][c]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Apply row interchanges ipiv[k0..k1) in order (forward) or reverse order to
+// columns [clo, chi).  Pivots are staged through LDS; interchanges with
+// ipiv[i] == i touch no memory.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp(int k0, int k1, int clo, int chi, T* A, long lda,
+                                               long sA, const int* ipiv, long sP, int reverse) {
+  const long b = blockIdx.y;
+  A += 2 * b * sA;
+  ipiv += b * sP;
+  const int col = clo + blockIdx.x * 256 + threadIdx.x;
+  __shared__ int sp[1024];
+  const int n = k1 - k0;
+  for (int base = 0; base < n; base += 1024) {
+    const int cnt = min(1024, n - base);
+    __syncthreads();
+    for (int q = threadIdx.x; q < cnt; q += 256) {
+      const int i = reverse ? (k1 - 1 - base - q) : (k0 + base + q);
+      sp[q] = ipiv[i];
+    }
+    __syncthreads();
+    for (int q = 0; q < cnt; ++q) {
+      const int i = reverse ? (k1 - 1 - base - q) : (k0 + base + q);
+      const int p = sp[q];
+      if (p != i && col < chi) {
+        T* x = A + 2 * ((long)i * lda + col);
+        T* y = A + 2 * ((long)p * lda + col);
+        const cplx<T> u = cld(x), v = cld(y);
+        cst(x, v);
+        cst(y, u);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small triangular solve in LDS: X = T^-1 B for an n x n (n <= 64) block T,
+// lower with unit diagonal (LOWER) or upper with its diagonal, B n x nrhs.
+// Workgroup: 256 threads = 64 right-hand-side columns x 4 row groups.
+// ---------------------------------------------------------------------------
+constexpr int TL = 64;
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsm_leaf(int n, int nrhs, const T* Tm, long ldt, long sT, T* B,
+                                                   long ldb, long sB) {
+  const long b = blockIdx.y;
+  Tm += 2 * b * sT;
+  B += 2 * b * sB;
+  __shared__ cplx<T> Ts[TL][TL + 1];
+  __shared__ cplx<T> Xs[TL][TL + 1];
+  const int tid = threadIdx.x, cl = tid & 63, rg = tid >> 6;
+  const int c0 = blockIdx.x * TL;
+  for (int e = tid; e < n * n; e += 256) {
+    const int i = e / n, j = e % n;
+    Ts[i][j] = cld(Tm + 2 * ((long)i * ldt + j));
+  }
+  const int gc = c0 + cl;
+  for (int i = rg; i < n; i += 4) Xs[i][cl] = (gc < nrhs) ? cld(B + 2 * ((long)i * ldb + gc)) : cmake<T>(0, 0);
+  __syncthreads();
+  if (LOWER) {
+    for (int i = 0; i < n; ++i) {
+      const cplx<T> xi = Xs[i][cl];
+      for (int l = i + 1 + rg; l < n; l += 4) Xs[l][cl] = cfms(Xs[l][cl], Ts[l][i], xi);
+      __syncthreads();
+    }
+  } else {
+    for (int i = n - 1; i >= 0; --i) {
+      if (rg == 0) Xs[i][cl] = cdiv(Xs[i][cl], Ts[i][i]);
+      __syncthreads();
+      const cplx<T> xi = Xs[i][cl];
+      for (int l = rg; l < i; l += 4) Xs[l][cl] = cfms(Xs[l][cl], Ts[l][i], xi);
+      __syncthreads();
+    }
+  }
+  if (gc < nrhs)
+    for (int i = rg; i < n; i += 4) cst(B + 2 * ((long)i * ldb + gc), Xs[i][cl]);
+}
+
+__global__ void k_fill_int(int* p, long n, int v) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host-side recursion
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Bat {
+  T* A;
+  long lda, sA;
+  int batch;
+  hipStream_t st;
+  T* at(long r, long c) const { return A + 2 * (r * lda + c); }
+};
+
+template <typename T>
+int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, long ldb, long sB, T* C, long ldc,
+             long sC, int batch, hipStream_t st) {
+  if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return 0;
+  dim3 grid(mnp_ceil_div(N, GT), mnp_ceil_div(M, GT), batch);
+  hipLaunchKernelGGL(k_cgemm_sub<T>, grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// X <- T^-1 X, T n x n at Tm (lower unit or upper), X n x nrhs
+template <typename T>
+int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, long ldx, long sX, int batch,
+             hipStream_t st) {
+  if (n <= 0 || nrhs <= 0) return 0;
+  if (n <= TL) {
+    dim3 grid(mnp_ceil_div(nrhs, TL), batch);
+    if (lower)
+      hipLaunchKernelGGL((k_trsm_leaf<T, true>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx, sX);
+    else
+      hipLaunchKernelGGL((k_trsm_leaf<T, false>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx, sX);
+    MNP_LAUNCH_CHECK();
+    return 0;
+  }
+  int n1 = ((n / 2 + TL - 1) / TL) * TL;
+  if (n1 >= n) n1 = n - TL;
+  const int n2 = n - n1;
+  int rc;
+  if (lower) {
+    if ((rc = trsm_rec(true, n1, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st))) return rc;
+    if ((rc = gemm_sub(n2, nrhs, n1, Tm + 2 * ((long)n1 * ldt), ldt, sT, X, ldx, sX, X + 2 * ((long)n1 * ldx), ldx,
+                       sX, batch, st)))
+      return rc;
+    return trsm_rec(true, n2, nrhs, Tm + 2 * ((long)n1 * ldt + n1), ldt, sT, X + 2 * ((long)n1 * ldx), ldx, sX,
+                    batch, st);
+  } else {
+    if ((rc = trsm_rec(false, n2, nrhs, Tm + 2 * ((long)n1 * ldt + n1), ldt, sT, X + 2 * ((long)n1 * ldx), ldx, sX,
+                       batch, st)))
+      return rc;
+    if ((rc = gemm_sub(n1, nrhs, n2, Tm + 2 * (long)n1, ldt, sT, X + 2 * ((long)n1 * ldx), ldx, sX, X, ldx, sX,
+                       batch, st)))
+      return rc;
+    return trsm_rec(false, n1, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st);
+  }
+}
+
+template <typename T>
+int laswp(int k0, int k1, int clo, int chi, T* A, long lda, long sA, const int* ipiv, long sP, int batch, bool rev,
+          hipStream_t st) {
+  if (k1 <= k0 || chi <= clo) return 0;
+  dim3 grid(mnp_ceil_div(chi - clo, 256), batch);
+  hipLaunchKernelGGL(k_laswp<T>, grid, dim3(256), 0, st, k0, k1, clo, chi, A, lda, sA, ipiv, sP, rev ? 1 : 0);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// leaf configurations: W panel columns, S rows per thread, 512 threads.  The
+// panel (S*W complex per thread) stays in registers (at most 96 values of
+// either precision per thread), so a narrower panel is used for taller
+// matrices.
+template <typename T>
+struct LeafCfg;
+template <>
+struct LeafCfg<double> {
+  static int width(int N) { return N <= 3072 ? 8 : (N <= 6144 ? 4 : 2); }
+};
+template <>
+struct LeafCfg<float> {
+  static int width(int N) { return N <= 3072 ? 8 : (N <= 6144 ? 4 : 2); }
+};
+
+template <typename T, int W, int SMAX>
+int leaf_launch_w(int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
+                  hipStream_t st) {
+  const int m = N - c0;
+#define MNP_LEAF(S_)                                                                                       \
+  if (m <= (S_)*512) {                                                                                     \
+    hipLaunchKernelGGL((k_lu_leaf<T, W, S_, 512>), dim3(batch), dim3(512), 0, st, N, c0, w, A, lda, sA, ipiv, \
+                       sP, info);                                                                          \
+    MNP_LAUNCH_CHECK();                                                                                    \
+    return 0;                                                                                              \
+  }
+  MNP_LEAF(1)
+  MNP_LEAF(2)
+  MNP_LEAF(3)
+  MNP_LEAF(6)
+  if constexpr (SMAX >= 12) {
+    MNP_LEAF(12)
+  }
+  if constexpr (SMAX >= 24) {
+    MNP_LEAF(24)
+  }
+#undef MNP_LEAF
+  mnp_set_error("getrf: matrix too tall for the register-resident panel");
+  return 1;
+}
+
+template <typename T>
+int leaf_launch(int W, int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
+                hipStream_t st) {
+  if constexpr (sizeof(T) == 8) {
+    if (W == 8) return leaf_launch_w<T, 8, 6>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+    if (W == 4) return leaf_launch_w<T, 4, 12>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+    return leaf_launch_w<T, 2, 24>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+  } else {
+    if (W == 8) return leaf_launch_w<T, 8, 6>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+    if (W == 4) return leaf_launch_w<T, 4, 12>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+    return leaf_launch_w<T, 2, 24>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+  }
+}
+
+// recursive LU of columns [c0, c0+n) (rows c0..N-1)
+template <typename T>
+int getrf_rec(int W, int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
+              hipStream_t st) {
+  if (n <= W) return leaf_launch<T>(W, N, c0, n, A, lda, sA, ipiv, sP, info, batch, st);
+  int n1 = ((n / 2 + W - 1) / W) * W;
+  if (n1 >= n) n1 = n - W;
+  const int n2 = n - n1;
+  int rc;
+  if ((rc = getrf_rec(W, N, c0, n1, A, lda, sA, ipiv, sP, info, batch, st))) return rc;
+  if ((rc = laswp<T>(c0, c0 + n1, c0 + n1, c0 + n, A, lda, sA, ipiv, sP, batch, false, st))) return rc;
+  T* A11 = A + 2 * ((long)c0 * lda + c0);
+  T* A12 = A + 2 * ((long)c0 * lda + c0 + n1);
+  T* A21 = A + 2 * ((long)(c0 + n1) * lda + c0);
+  T* A22 = A + 2 * ((long)(c0 + n1) * lda + c0 + n1);
+  if ((rc = trsm_rec<T>(true, n1, n2, A11, lda, sA, A12, lda, sA, batch, st))) return rc;
+  if ((rc = gemm_sub<T>(N - c0 - n1, n2, n1, A21, lda, sA, A12, lda, sA, A22, lda, sA, batch, st))) return rc;
+  if ((rc = getrf_rec(W, N, c0 + n1, n2, A, lda, sA, ipiv, sP, info, batch, st))) return rc;
+  return laswp<T>(c0 + n1, c0 + n, c0, c0 + n1, A, lda, sA, ipiv, sP, batch, false, st);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int mnpbem_cgemm_sub_batched(int dtype, int M, int N, int K, const void* A, long lda, long sA,
+                                        const void* B, long ldb, long sB, void* C, long ldc, long sC, int batch,
+                                        void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "cgemm: negative size");
+  if (dtype == MNP_DTYPE_C128)
+    return gemm_sub<double>(M, N, K, (const double*)A, lda, sA, (const double*)B, ldb, sB, (double*)C, ldc, sC,
+                            batch, st);
+  if (dtype == MNP_DTYPE_C64)
+    return gemm_sub<float>(M, N, K, (const float*)A, lda, sA, (const float*)B, ldb, sB, (float*)C, ldc, sC, batch,
+                           st);
+  mnp_set_error("cgemm: unknown dtype");
+  return 1;
+}
+
+extern "C" int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA, int* ipiv, long sP, int* info,
+                                    int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n >= 0 && batch >= 0 && lda >= n, "getrf: bad size");
+  if (n == 0 || batch == 0) return 0;
+  hipLaunchKernelGGL(k_fill_int, dim3(mnp_ceil_div(batch, 256)), dim3(256), 0, st, info, (long)batch, 0);
+  MNP_LAUNCH_CHECK();
+  if (dtype == MNP_DTYPE_C128)
+    return getrf_rec<double>(LeafCfg<double>::width(n), n, 0, n, (double*)A, lda, sA, ipiv, sP, info, batch, st);
+  if (dtype == MNP_DTYPE_C64)
+    return getrf_rec<float>(LeafCfg<float>::width(n), n, 0, n, (float*)A, lda, sA, ipiv, sP, info, batch, st);
+  mnp_set_error("getrf: unknown dtype");
+  return 1;
+}
+
+// B <- A^-1 B with A = P^T L U from mnpbem_getrf_batched (no transpose)
+extern "C" int mnpbem_getrs_batched(int dtype, int n, int nrhs, const void* A, long lda, long sA, const int* ipiv,
+                                    long sP, void* B, long ldb, long sB, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n >= 0 && nrhs >= 0 && batch >= 0 && lda >= n && ldb >= nrhs, "getrs: bad size");
+  if (n == 0 || nrhs == 0 || batch == 0) return 0;
+  int rc;
+  if (dtype == MNP_DTYPE_C128) {
+    if ((rc = laswp<double>(0, n, 0, nrhs, (double*)B, ldb, sB, ipiv, sP, batch, false, st))) return rc;
+    if ((rc = trsm_rec<double>(true, n, nrhs, (const double*)A, lda, sA, (double*)B, ldb, sB, batch, st))) return rc;
+    return trsm_rec<double>(false, n, nrhs, (const double*)A, lda, sA, (double*)B, ldb, sB, batch, st);
+  }
+  if (dtype == MNP_DTYPE_C64) {
+    if ((rc = laswp<float>(0, n, 0, nrhs, (float*)B, ldb, sB, ipiv, sP, batch, false, st))) return rc;
+    if ((rc = trsm_rec<float>(true, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st))) return rc;
+    return trsm_rec<float>(false, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st);
+  }
+  mnp_set_error("getrs: unknown dtype");
+  return 1;
+}

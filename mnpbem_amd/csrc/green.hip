@@ -1,0 +1,425 @@
+// Retarded Green functions on curved boundary elements (gfx950).
+//
+// Set-up (once per geometry):
+//   mnpbem_refine_coeffs    - order-n (ik)^n expansion coefficients of the
+//                             refined (self and near) entries, deriv='cart'
+//                             (mnpbem/greenfun/greenret_refined.py:122-516,
+//                             MATLAB Greenfun/@greenret/private/init.m:42-177)
+//   mnpbem_closed_correction - Fuchs-Liu closed-surface correction added to
+//                             the order-0 coefficient of the self terms
+//                             (mnpbem/greenfun/compgreen_ret.py:192-400,
+//                             compgreen_stat.py:219-465)
+// Per wavelength batch:
+//   mnpbem_green_assemble   - G = area_j/d e^{ikd}, F = n_i.r (ik-1/d)/d^2
+//                             area_j e^{ikd} for all face pairs of one side
+//                             (greenret_refined.py:654-800), then the refined
+//                             entries sum_n c_n (ik)^n e^{ikd}, and +/-2 pi on
+//                             the self diagonal for H1/H2 (:800-810).
+// Matrices are written transposed: entry (i, j) lands at out[j*ld + i], so a
+// row-major LU of the buffer factors the transpose (the LU solve then yields
+// H G^-1 directly; see mnpbem_getrs_batched).
+#include "common.h"
+
+namespace {
+
+constexpr int QSTRIDE = 19;  // quadrature table row: shape[6], d/dx[6], d/dy[6], weight
+
+struct QPoint {
+  double p[3];
+  double w;
+};
+
+// point and weight of quadrature node q on the curved 6-node face X[6][3]:
+// p = sum N_c X_c,  w = 0.5 * w_q * |dX/dx x dX/dy|  (particle.py:1364-1429,
+// 1520-1575)
+__device__ __forceinline__ QPoint face_point(const double* __restrict__ X, const double* __restrict__ tq) {
+  double p[3] = {0, 0, 0}, tx[3] = {0, 0, 0}, ty[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double x = X[3 * c + d];
+      p[d] += tq[c] * x;
+      tx[d] += tq[6 + c] * x;
+      ty[d] += tq[12 + c] * x;
+    }
+  }
+  const double nx = tx[1] * ty[2] - tx[2] * ty[1];
+  const double ny = tx[2] * ty[0] - tx[0] * ty[2];
+  const double nz = tx[0] * ty[1] - tx[1] * ty[0];
+  QPoint q;
+  q.p[0] = p[0];
+  q.p[1] = p[1];
+  q.p[2] = p[2];
+  q.w = 0.5 * tq[18] * sqrt(nx * nx + ny * ny + nz * nz);
+  return q;
+}
+
+template <int ORD>
+__global__ __launch_bounds__(256) void k_refine_coeffs(
+    int npairs, const int* __restrict__ prow, const int* __restrict__ pcol, const int* __restrict__ pkind,
+    const double* __restrict__ pos1, const double* __restrict__ nvec1, const double* __restrict__ tvec1,
+    const double* __restrict__ tvec2, const double* __restrict__ pos2, const double* __restrict__ nodes2,
+    const double* __restrict__ qnear, int nq_near, const double* __restrict__ qself, int nq_self,
+    double* __restrict__ g, double* __restrict__ f, double* __restrict__ fstat) {
+  extern __shared__ double qs[];
+  double* qn = qs;
+  double* qp = qs + nq_near * QSTRIDE;
+  for (int e = threadIdx.x; e < nq_near * QSTRIDE; e += blockDim.x) qn[e] = qnear[e];
+  for (int e = threadIdx.x; e < nq_self * QSTRIDE; e += blockDim.x) qp[e] = qself[e];
+  __syncthreads();
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int i = prow[p], j = pcol[p], kind = pkind[p];
+  double X[18];
+#pragma unroll
+  for (int e = 0; e < 18; ++e) X[e] = nodes2[18 * (long)j + e];
+  const double pi_[3] = {pos1[3 * i], pos1[3 * i + 1], pos1[3 * i + 2]};
+  const double ni[3] = {nvec1[3 * i], nvec1[3 * i + 1], nvec1[3 * i + 2]};
+  const double tiny = 2.220446049250313e-16;
+  double fact[ORD + 1];
+  fact[0] = 1.0;
+#pragma unroll
+  for (int n = 1; n <= ORD; ++n) fact[n] = fact[n - 1] * n;
+  double gs[ORD + 1], fx[ORD + 1], fy[ORD + 1], fz[ORD + 1];
+#pragma unroll
+  for (int n = 0; n <= ORD; ++n) gs[n] = fx[n] = fy[n] = fz[n] = 0.0;
+  double fq = 0.0;
+
+  if (kind == 2) {
+    // self term: polar rule around the centroid; tangential parts use
+    // rr = max(r, 1e-4 max r)  (greenret_refined.py:208-258)
+    const double t1[3] = {tvec1[3 * i], tvec1[3 * i + 1], tvec1[3 * i + 2]};
+    const double t2[3] = {tvec2[3 * i], tvec2[3 * i + 1], tvec2[3 * i + 2]};
+    double rmax = 0.0;
+    for (int q = 0; q < nq_self; ++q) {
+      const QPoint Q = face_point(X, qp + q * QSTRIDE);
+      const double vx = pi_[0] - Q.p[0], vy = pi_[1] - Q.p[1], vz = pi_[2] - Q.p[2];
+      rmax = fmax(rmax, fmax(sqrt(vx * vx + vy * vy + vz * vz), tiny));
+    }
+    double f1[ORD + 1], f2[ORD + 1], f3[ORD + 1];
+#pragma unroll
+    for (int n = 0; n <= ORD; ++n) f1[n] = f2[n] = f3[n] = 0.0;
+    for (int q = 0; q < nq_self; ++q) {
+      const QPoint Q = face_point(X, qp + q * QSTRIDE);
+      const double vx = pi_[0] - Q.p[0], vy = pi_[1] - Q.p[1], vz = pi_[2] - Q.p[2];
+      const double r = fmax(sqrt(vx * vx + vy * vy + vz * vz), tiny);
+      const double rr = fmax(r, 1e-4 * rmax);
+      const double ndr = ni[0] * vx + ni[1] * vy + ni[2] * vz;
+      const double in1 = t1[0] * vx + t1[1] * vy + t1[2] * vz;
+      const double in2 = t2[0] * vx + t2[1] * vy + t2[2] * vz;
+      const double w = Q.w;
+      const double ir = 1.0 / r, irr3 = 1.0 / (rr * rr * rr);
+      double rnm1 = ir;               // r^(n-1)
+      double rnm3 = ir * ir * ir;     // r^(n-3)
+      double rn = 1.0;                // r^n
+#pragma unroll
+      for (int n = 0; n <= ORD; ++n) {
+        gs[n] += w * rnm1;
+        f1[n] += w * (n - 1) * ndr * rnm3;
+        f2[n] += w * (n - 1) * in1 * rn * irr3;
+        f3[n] += w * (n - 1) * in2 * rn * irr3;
+        rnm1 *= r;
+        rnm3 *= r;
+        rn *= r;
+      }
+      fq -= w * ndr * ir * ir * ir;
+    }
+#pragma unroll
+    for (int n = 0; n <= ORD; ++n) {
+      gs[n] /= fact[n];
+      const double a = f1[n] / fact[n], b = f2[n] / fact[n], c = f3[n] / fact[n];
+      fx[n] = ni[0] * a + t1[0] * b + t2[0] * c;
+      fy[n] = ni[1] * a + t1[1] * b + t2[1] * c;
+      fz[n] = ni[2] * a + t1[2] * b + t2[2] * c;
+    }
+  } else {
+    // near pair: 28-point rule on the source face, expanded about the
+    // centroid distance r0 (greenret_refined.py:357-449)
+    const double dx0 = pos2[3 * j] - pi_[0], dy0 = pos2[3 * j + 1] - pi_[1], dz0 = pos2[3 * j + 2] - pi_[2];
+    const double r0 = sqrt(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
+    for (int q = 0; q < nq_near; ++q) {
+      const QPoint Q = face_point(X, qn + q * QSTRIDE);
+      const double vx = pi_[0] - Q.p[0], vy = pi_[1] - Q.p[1], vz = pi_[2] - Q.p[2];
+      const double r = fmax(sqrt(vx * vx + vy * vy + vz * vz), tiny);
+      const double dl = r - r0;
+      const double w = Q.w;
+      const double ir = 1.0 / r, ir2 = ir * ir, ir3 = ir2 * ir;
+      double dn = 1.0, dnm1 = 0.0;  // dl^n, dl^(n-1)
+#pragma unroll
+      for (int n = 0; n <= ORD; ++n) {
+        gs[n] += w * dn * ir / fact[n];
+        const double fnm1 = fact[n > 0 ? n - 1 : 0];
+        const double fsc = (n == 0) ? -ir3 : (-dn * ir3 / fact[n] + dnm1 * ir2 / fnm1);
+        fx[n] += w * vx * fsc;
+        fy[n] += w * vy * fsc;
+        fz[n] += w * vz * fsc;
+        dnm1 = dn;
+        dn *= dl;
+      }
+      fq -= w * (ni[0] * vx + ni[1] * vy + ni[2] * vz) * ir3;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n <= ORD; ++n) {
+    g[(long)p * (ORD + 1) + n] = gs[n];
+    f[((long)p * 3 + 0) * (ORD + 1) + n] = fx[n];
+    f[((long)p * 3 + 1) * (ORD + 1) + n] = fy[n];
+    f[((long)p * 3 + 2) * (ORD + 1) + n] = fz[n];
+  }
+  fstat[p] = fq;
+}
+
+// one wavefront per column j: f_j = sum_i a_i F_ij / a_j over the rows of
+// j's closed surface, F the quasistatic normal derivative with the refined
+// entries (fstat) in place of -n_i.r_ij/d^3 a_j; corr_j = -2 pi dir - f_j is
+// added to the order-0 Cartesian coefficient of the self pair as corr_j n_j.
+__global__ __launch_bounds__(256) void k_closed_correction(
+    int n, const double* __restrict__ pos, const double* __restrict__ nvec, const double* __restrict__ area,
+    const int* __restrict__ lo, const int* __restrict__ hi, const int* __restrict__ colptr,
+    const int* __restrict__ crow, const int* __restrict__ cpair, const double* __restrict__ fstat, double dir,
+    int ord1, double* __restrict__ f, double* __restrict__ corr) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  const double pj[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+  const double aj = area[j];
+  const int c0 = colptr[j], c1 = colptr[j + 1];
+  double s = 0.0;
+  for (int i = lo[j] + lane; i < hi[j]; i += 64) {
+    // binary search i among the refined rows of column j (sorted)
+    int a = c0, b = c1;
+    while (a < b) {
+      const int m = (a + b) >> 1;
+      if (crow[m] < i)
+        a = m + 1;
+      else
+        b = m;
+    }
+    double Fij;
+    if (a < c1 && crow[a] == i) {
+      Fij = fstat[cpair[a]];
+    } else {
+      const double x = pos[3 * i] - pj[0], y = pos[3 * i + 1] - pj[1], z = pos[3 * i + 2] - pj[2];
+      const double d = fmax(sqrt(x * x + y * y + z * z), 2.220446049250313e-16);
+      Fij = -(nvec[3 * i] * x + nvec[3 * i + 1] * y + nvec[3 * i + 2] * z) / (d * d * d) * aj;
+    }
+    s += area[i] * Fij / aj;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) {
+    const double cj = -2.0 * M_PI * dir - s;
+    corr[j] = cj;
+    for (int q = c0; q < c1; ++q)
+      if (crow[q] == j) {
+        const long pr = cpair[q];
+        f[(pr * 3 + 0) * ord1] += cj * nvec[3 * j];
+        f[(pr * 3 + 1) * ord1] += cj * nvec[3 * j + 1];
+        f[(pr * 3 + 2) * ord1] += cj * nvec[3 * j + 2];
+      }
+  }
+}
+
+template <typename TO>
+__device__ __forceinline__ void store_c(TO* p, double re, double im) {
+  typedef typename vec2_of<TO>::type V;
+  V v;
+  v.x = (TO)re;
+  v.y = (TO)im;
+  *reinterpret_cast<V*>(p) = v;
+}
+
+constexpr int AS_I = 256;  // rows (field faces) per workgroup
+constexpr int AS_J = 8;    // columns (source faces) per workgroup
+
+// dense far-field pass over all (i, j) of one side, batch = wavelength.
+// Unconnected pairs (different materials on this side) are written as 0.
+template <typename TO, bool DUAL>
+__global__ __launch_bounds__(AS_I) void k_green_assemble(
+    int n, const double* __restrict__ pos, const double* __restrict__ nvec, const double* __restrict__ area,
+    const int* __restrict__ mat, const double* __restrict__ kmat, int nmat, TO* __restrict__ G, TO* __restrict__ H,
+    long ld, long stride, float* __restrict__ G2, float* __restrict__ H2) {
+  const long b = blockIdx.z;
+  const int i = blockIdx.x * AS_I + threadIdx.x;
+  const int j0 = blockIdx.y * AS_J;
+  if (i >= n) return;
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  const double nxi = nvec[3 * i], nyi = nvec[3 * i + 1], nzi = nvec[3 * i + 2];
+  const int mi = mat[i];
+  const double kr = kmat[2 * (b * nmat + mi)], ki = kmat[2 * (b * nmat + mi) + 1];
+  G += 2 * b * stride;
+  H += 2 * b * stride;
+  if (DUAL) {
+    G2 += 2 * b * stride;
+    H2 += 2 * b * stride;
+  }
+  const int jn = min(AS_J, n - j0);
+  for (int jj = 0; jj < jn; ++jj) {
+    const int j = j0 + jj;
+    double gr = 0, gi = 0, fr = 0, fi = 0;
+    if (mat[j] == mi) {
+      const double x = xi - pos[3 * j], y = yi - pos[3 * j + 1], z = zi - pos[3 * j + 2];
+      const double d = fmax(sqrt(x * x + y * y + z * z), 2.220446049250313e-16);
+      const double id = 1.0 / d, aj = area[j];
+      // phase e^{ikd} = e^{-ki d} (cos kr d + i sin kr d)
+      double s, c;
+      sincos(kr * d, &s, &c);
+      const double e = exp(-ki * d);
+      const double phr = e * c, phi = e * s;
+      const double ga = aj * id;
+      gr = ga * phr;
+      gi = ga * phi;
+      // (ik - 1/d)/d^2 * n.r * a_j
+      const double ndr = nxi * x + nyi * y + nzi * z;
+      const double sc = ndr * id * id * aj;
+      const double ar = (-ki - id) * sc, ai = kr * sc;
+      fr = ar * phr - ai * phi;
+      fi = ar * phi + ai * phr;
+    }
+    const long o = 2 * ((long)j * ld + i);
+    store_c(G + o, gr, gi);
+    store_c(H + o, fr, fi);
+    if (DUAL) {
+      store_c(G2 + o, gr, gi);
+      store_c(H2 + o, fr, fi);
+    }
+  }
+}
+
+// refined entries: value = (sum_n c_n (ik)^n) e^{ikd}; H gets +/-2 pi on the
+// self diagonal after the phase (greenret_refined.py:698-810).
+template <typename TO, bool DUAL>
+__global__ __launch_bounds__(256) void k_green_overlay(
+    int npairs, int ord1, const int* __restrict__ prow, const int* __restrict__ pcol,
+    const double* __restrict__ g, const double* __restrict__ f, const double* __restrict__ pos,
+    const double* __restrict__ nvec, const int* __restrict__ mat, const double* __restrict__ kmat, int nmat,
+    double hsign, TO* __restrict__ G, TO* __restrict__ H, long ld, long stride, float* __restrict__ G2,
+    float* __restrict__ H2) {
+  const long b = blockIdx.y;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int i = prow[p], j = pcol[p];
+  const int mi = mat[i];
+  if (mat[j] != mi) return;
+  const double kr = kmat[2 * (b * nmat + mi)], ki = kmat[2 * (b * nmat + mi) + 1];
+  const double x = pos[3 * i] - pos[3 * j], y = pos[3 * i + 1] - pos[3 * j + 1], z = pos[3 * i + 2] - pos[3 * j + 2];
+  const double d = fmax(sqrt(x * x + y * y + z * z), 2.220446049250313e-16);
+  const double nx = nvec[3 * i], ny = nvec[3 * i + 1], nz = nvec[3 * i + 2];
+  // (ik)^n with ik = -ki + i kr
+  double pr = 1.0, pim = 0.0;
+  double sgr = 0, sgi = 0, sfr = 0, sfi = 0;
+  for (int q = 0; q < ord1; ++q) {
+    const double gq = g[(long)p * ord1 + q];
+    const double fq = nx * f[((long)p * 3 + 0) * ord1 + q] + ny * f[((long)p * 3 + 1) * ord1 + q] +
+                      nz * f[((long)p * 3 + 2) * ord1 + q];
+    sgr += gq * pr;
+    sgi += gq * pim;
+    sfr += fq * pr;
+    sfi += fq * pim;
+    const double tr = pr * (-ki) - pim * kr;
+    pim = pr * kr + pim * (-ki);
+    pr = tr;
+  }
+  double s, c;
+  sincos(kr * d, &s, &c);
+  const double e = exp(-ki * d);
+  const double phr = e * c, phi = e * s;
+  const double gr = sgr * phr - sgi * phi, gi = sgr * phi + sgi * phr;
+  double hr = sfr * phr - sfi * phi, hi = sfr * phi + sfi * phr;
+  if (i == j) hr += hsign * 2.0 * M_PI;
+  const long o = 2 * (b * stride + (long)j * ld + i);
+  store_c(G + o, gr, gi);
+  store_c(H + o, hr, hi);
+  if (DUAL) {
+    store_c(G2 + o, gr, gi);
+    store_c(H2 + o, hr, hi);
+  }
+}
+
+}  // namespace
+
+extern "C" int mnpbem_refine_coeffs(int order, int npairs, const int* prow, const int* pcol, const int* pkind,
+                                    const double* pos1, const double* nvec1, const double* tvec1,
+                                    const double* tvec2, const double* pos2, const double* nodes2,
+                                    const double* qnear, int nq_near, const double* qself, int nq_self,
+                                    double* g, double* f, double* fstat, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(order >= 0 && order <= 7, "refine_coeffs: order must be in 0..7");
+  MNP_REQUIRE(nq_near > 0 && nq_self > 0 && (nq_near + nq_self) * QSTRIDE * 8 <= 60000,
+              "refine_coeffs: quadrature tables too large");
+  if (npairs == 0) return 0;
+  const size_t shm = (size_t)(nq_near + nq_self) * QSTRIDE * sizeof(double);
+  dim3 grid(mnp_ceil_div(npairs, 256));
+#define MNP_RC(O)                                                                                             \
+  case O:                                                                                                     \
+    hipLaunchKernelGGL(k_refine_coeffs<O>, grid, dim3(256), shm, st, npairs, prow, pcol, pkind, pos1, nvec1, \
+                       tvec1, tvec2, pos2, nodes2, qnear, nq_near, qself, nq_self, g, f, fstat);            \
+    break;
+  switch (order) {
+    MNP_RC(0) MNP_RC(1) MNP_RC(2) MNP_RC(3) MNP_RC(4) MNP_RC(5) MNP_RC(6) MNP_RC(7)
+  }
+#undef MNP_RC
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int mnpbem_closed_correction(int n, const double* pos, const double* nvec, const double* area,
+                                        const int* lo, const int* hi, const int* colptr, const int* crow,
+                                        const int* cpair, const double* fstat, double dir, int order, double* f,
+                                        double* corr, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_closed_correction, dim3(mnp_ceil_div(n, 4)), dim3(256), 0, st, n, pos, nvec, area, lo, hi,
+                     colptr, crow, cpair, fstat, dir, order + 1, f, corr);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// G, H of one side for ``batch`` wavelengths.  kmat: (batch, nmat) complex
+// wavenumbers; mat: per-face material index (0-based) on this side;
+// hsign = +1 (H1, inside) or -1 (H2, outside).  dtype selects the outputs:
+// C128, C64, or C128 plus a complex64 copy in G32/H32 (dual = 1).
+extern "C" int mnpbem_green_assemble(int dtype, int dual, int n, const double* pos, const double* nvec,
+                                     const double* area, const int* mat, const double* kmat, int nmat,
+                                     int order, int npairs, const int* prow, const int* pcol, const double* g,
+                                     const double* f, double hsign, void* G, void* H, long ld, long stride,
+                                     void* G32, void* H32, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n >= 0 && ld >= n && batch >= 0 && nmat >= 1, "green_assemble: bad size");
+  MNP_REQUIRE(!(dual && dtype != MNP_DTYPE_C128), "green_assemble: dual output needs complex128 primary");
+  if (n == 0 || batch == 0) return 0;
+  dim3 g1(mnp_ceil_div(n, AS_I), mnp_ceil_div(n, AS_J), batch);
+  dim3 g2(mnp_ceil_div(npairs, 256), batch);
+  const int ord1 = order + 1;
+  if (dtype == MNP_DTYPE_C128 && dual) {
+    hipLaunchKernelGGL((k_green_assemble<double, true>), g1, dim3(AS_I), 0, st, n, pos, nvec, area, mat, kmat, nmat,
+                       (double*)G, (double*)H, ld, stride, (float*)G32, (float*)H32);
+    MNP_LAUNCH_CHECK();
+    if (npairs > 0)
+      hipLaunchKernelGGL((k_green_overlay<double, true>), g2, dim3(256), 0, st, npairs, ord1, prow, pcol, g, f, pos,
+                         nvec, mat, kmat, nmat, hsign, (double*)G, (double*)H, ld, stride, (float*)G32,
+                         (float*)H32);
+  } else if (dtype == MNP_DTYPE_C128) {
+    hipLaunchKernelGGL((k_green_assemble<double, false>), g1, dim3(AS_I), 0, st, n, pos, nvec, area, mat, kmat,
+                       nmat, (double*)G, (double*)H, ld, stride, (float*)nullptr, (float*)nullptr);
+    MNP_LAUNCH_CHECK();
+    if (npairs > 0)
+      hipLaunchKernelGGL((k_green_overlay<double, false>), g2, dim3(256), 0, st, npairs, ord1, prow, pcol, g, f,
+                         pos, nvec, mat, kmat, nmat, hsign, (double*)G, (double*)H, ld, stride, (float*)nullptr,
+                         (float*)nullptr);
+  } else if (dtype == MNP_DTYPE_C64) {
+    hipLaunchKernelGGL((k_green_assemble<float, false>), g1, dim3(AS_I), 0, st, n, pos, nvec, area, mat, kmat,
+                       nmat, (float*)G, (float*)H, ld, stride, (float*)nullptr, (float*)nullptr);
+    MNP_LAUNCH_CHECK();
+    if (npairs > 0)
+      hipLaunchKernelGGL((k_green_overlay<float, false>), g2, dim3(256), 0, st, npairs, ord1, prow, pcol, g, f,
+                         pos, nvec, mat, kmat, nmat, hsign, (float*)G, (float*)H, ld, stride, (float*)nullptr,
+                         (float*)nullptr);
+  } else {
+    mnp_set_error("green_assemble: unknown dtype");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}

@@ -1,0 +1,779 @@
+// Per-wavelength BEM assembly of Sigma/Delta, the small-right-hand-side solve
+// of GdA&H Eqs. (19)-(20), the 8N operator residual for iterative refinement,
+// plane-wave excitation and the optical-theorem extinction (gfx950, batched
+// over wavelengths).
+//
+// Storage convention.  Every N x N matrix M of the algorithm is kept as the
+// row-major buffer of its transpose, buf[j*ld + i] = M[i][j]:
+//   G1t, G2t  -> LU factors of G1^T, G2^T (P G^T = L U)
+//   S1t       -> Sigma1^T = (G1^T)^-1 H1^T,  S2t likewise
+//   Dt        -> LU of Delta^T = Sigma1^T - Sigma2^T
+//   Et        -> (Delta^-1)^T
+//   St        -> LU of Sigma^T = eps1 Sigma1^T - eps2 Sigma2^T + k^2 (eps1-eps2)^2 Et o (n n^T)
+// (mnpbem/bem/bem_ret.py:437-523, MATLAB BEM/@bemret/private/initmat.m).
+// Products M x are then "transposed" GEMVs over the buffer, and solves
+// M x = v with the stored LU of M^T are transposed triangular solves.
+//
+// Vector layout per wavelength: (N, 8P) row-major, P = polarisations:
+//   unknowns  X = [sig1 (P) | h1 (3P, index c*P+p) | sig2 (P) | h2 (3P)]
+//   right side R = [phi (P) | a (3P) | De (P) | alpha (3P)]
+// the packing order of mnpbem/bem/bem_ret_iter.py:1196-1232.
+#include "common.h"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int MAXC = 16;  // right-hand-side columns handled per pass
+
+// Y[b] = beta*Y[b] + alpha * M[b]^T X[b]  (M n x n buffer, X, Y n x C)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restrict__ M, long ldm, long sM,
+                                                const T* __restrict__ X, long ldx, long sX, T* __restrict__ Y,
+                                                long ldy, long sY, double alr, double ali, double beta) {
+  const long b = blockIdx.y;
+  M += 2 * b * sM;
+  X += 2 * b * sX;
+  Y += 2 * b * sY;
+  const int tid = threadIdx.x, il = tid & 63, jg = tid >> 6;
+  const int i = blockIdx.x * 64 + il;
+  __shared__ cplx<T> xs[64][MAXC];
+  __shared__ cplx<T> red[3][64][MAXC];
+  cplx<T> acc[MAXC];
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int nj = min(64, n - j0);
+    __syncthreads();
+    for (int e = tid; e < nj * C; e += 256) {
+      const int jj = e / C, c = e % C;
+      xs[jj][c] = cld(X + 2 * ((long)(j0 + jj) * ldx + c));
+    }
+    __syncthreads();
+    if (i < n) {
+      for (int jj = jg; jj < nj; jj += 4) {
+        const cplx<T> m = cld(M + 2 * ((long)(j0 + jj) * ldm + i));
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < C) acc[c] = cfma(acc[c], m, xs[jj][c]);
+      }
+    }
+  }
+  if (jg > 0) {
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c)
+      if (c < C) red[jg - 1][il][c] = acc[c];
+  }
+  __syncthreads();
+  if (jg == 0 && i < n) {
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c)
+      if (c < C) {
+        cplx<T> s = cadd(cadd(acc[c], red[0][il][c]), cadd(red[1][il][c], red[2][il][c]));
+        T* py = Y + 2 * ((long)i * ldy + c);
+        const double sr = (double)s.re, si = (double)s.im;
+        double yr = alr * sr - ali * si, yi = alr * si + ali * sr;
+        if (beta != 0.0) {
+          const cplx<T> y0 = cld(py);
+          yr += beta * (double)y0.re;
+          yi += beta * (double)y0.im;
+        }
+        cst(py, cmake<T>((T)yr, (T)yi));
+      }
+  }
+}
+
+// Solve M x = v in place for the n x C block V, given the LU of M^T in the
+// buffer (P M^T = L U):  x = P^T L^-T U^-T v.  One workgroup per matrix,
+// 64-row blocks: a transposed GEMV over the finished rows, then the 64 x 64
+// triangle in LDS by one wavefront.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_solve_t(int n, int C, const T* __restrict__ LU, long ld, long sLU,
+                                                    const int* __restrict__ ipiv, long sP, T* V, long ldv,
+                                                    long sV) {
+  const long b = blockIdx.x;
+  LU += 2 * b * sLU;
+  ipiv += b * sP;
+  V += 2 * b * sV;
+  const int tid = threadIdx.x, il = tid & 63, lg = tid >> 6;
+  __shared__ cplx<T> blk[64][65];
+  __shared__ cplx<T> red[3][64][MAXC];
+  __shared__ int sp[256];
+  const int nblk = (n + 63) / 64;
+  // forward: U^T z = v  (U^T lower, non-unit)
+  for (int ib = 0; ib < nblk; ++ib) {
+    const int i0 = ib * 64, nb = min(64, n - i0);
+    cplx<T> acc[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
+    if (il < nb) {
+      for (int l = lg; l < i0; l += 4) {
+        const cplx<T> u = cld(LU + 2 * ((long)l * ld + i0 + il));
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < C) acc[c] = cfma(acc[c], u, cld(V + 2 * ((long)l * ldv + c)));
+      }
+    }
+    if (lg > 0)
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c)
+        if (c < C) red[lg - 1][il][c] = acc[c];
+    for (int e = tid; e < nb * nb; e += 256) {
+      const int r = e / nb, q = e % nb;
+      blk[r][q] = cld(LU + 2 * ((long)(i0 + r) * ld + i0 + q));
+    }
+    __syncthreads();
+    if (lg == 0) {
+      cplx<T> val[MAXC];
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c)
+        if (c < C && il < nb) {
+          val[c] = csub(cld(V + 2 * ((long)(i0 + il) * ldv + c)),
+                        cadd(cadd(acc[c], red[0][il][c]), cadd(red[1][il][c], red[2][il][c])));
+        } else {
+          val[c] = cmake<T>(0, 0);
+        }
+      for (int i = 0; i < nb; ++i) {
+        if (il == i) {
+          const cplx<T> inv = crecip(blk[i][i]);
+#pragma unroll
+          for (int c = 0; c < MAXC; ++c) val[c] = cmul(val[c], inv);
+        }
+        const cplx<T> uir = blk[i][il];  // U[i][r] = (U^T)[r][i]
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < C) {
+            cplx<T> zi;
+            zi.re = __shfl(val[c].re, i);
+            zi.im = __shfl(val[c].im, i);
+            if (il > i && il < nb) val[c] = cfms(val[c], uir, zi);
+          }
+      }
+      if (il < nb)
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < C) cst(V + 2 * ((long)(i0 + il) * ldv + c), val[c]);
+    }
+    __syncthreads();
+  }
+  // backward: L^T w = z  (L^T upper, unit)
+  for (int ib = nblk - 1; ib >= 0; --ib) {
+    const int i0 = ib * 64, nb = min(64, n - i0), i1 = i0 + nb;
+    cplx<T> acc[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
+    if (il < nb) {
+      for (int l = i1 + lg; l < n; l += 4) {
+        const cplx<T> u = cld(LU + 2 * ((long)l * ld + i0 + il));
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < C) acc[c] = cfma(acc[c], u, cld(V + 2 * ((long)l * ldv + c)));
+      }
+    }
+    if (lg > 0)
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c)
+        if (c < C) red[lg - 1][il][c] = acc[c];
+    for (int e = tid; e < nb * nb; e += 256) {
+      const int r = e / nb, q = e % nb;
+      blk[r][q] = cld(LU + 2 * ((long)(i0 + r) * ld + i0 + q));
+    }
+    __syncthreads();
+    if (lg == 0) {
+      cplx<T> val[MAXC];
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c)
+        if (c < C && il < nb) {
+          val[c] = csub(cld(V + 2 * ((long)(i0 + il) * ldv + c)),
+                        cadd(cadd(acc[c], red[0][il][c]), cadd(red[1][il][c], red[2][il][c])));
+        } else {
+          val[c] = cmake<T>(0, 0);
+        }
+      for (int i = nb - 1; i >= 0; --i) {
+        const cplx<T> lir = blk[i][il];  // L[i][r] = (L^T)[r][i], r < i
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < C) {
+            cplx<T> wi;
+            wi.re = __shfl(val[c].re, i);
+            wi.im = __shfl(val[c].im, i);
+            if (il < i) val[c] = cfms(val[c], lir, wi);
+          }
+      }
+      if (il < nb)
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < C) cst(V + 2 * ((long)(i0 + il) * ldv + c), val[c]);
+    }
+    __syncthreads();
+  }
+  // x = P^T w: interchanges in reverse order
+  for (int base = n; base > 0; base -= 256) {
+    const int lo = max(0, base - 256);
+    __syncthreads();
+    if (tid < base - lo) sp[tid] = ipiv[lo + tid];
+    __syncthreads();
+    if (tid < C) {
+      for (int i = base - 1; i >= lo; --i) {
+        const int p = sp[i - lo];
+        if (p != i) {
+          T* x = V + 2 * ((long)i * ldv + tid);
+          T* y = V + 2 * ((long)p * ldv + tid);
+          const cplx<T> u = cld(x), v = cld(y);
+          cst(x, v);
+          cst(y, u);
+        }
+      }
+    }
+  }
+}
+
+// A[b] = identity (n x n buffer, leading dimension ld)
+template <typename T>
+__global__ void k_identity(int n, T* A, long ld, long sA) {
+  const long b = blockIdx.y;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int r = e / n, c = e % n;
+  cst(A + 2 * (b * sA + (long)r * ld + c), cmake<T>(r == c ? T(1) : T(0), T(0)));
+}
+
+// D = S1 - S2;  (mode 0)
+// S2 <- e1 S1 - e2 S2 + k^2 (e1-e2)^2 E o (n n^T)  (mode 1)
+// per-wavelength scalars in par[b] = {e1r, e1i, e2r, e2i, k}
+template <typename T>
+__global__ void k_sigma(int mode, int n, const T* S1, T* S2, T* D, const T* E, long ld, long stride,
+                        const double* __restrict__ nvec, const double* __restrict__ par) {
+  const long b = blockIdx.y;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int r = e / n, c = e % n;
+  const long o = 2 * (b * stride + (long)r * ld + c);
+  const cplx<T> s1 = cld(S1 + o), s2 = cld(S2 + o);
+  if (mode == 0) {
+    cst(D + o, csub(s1, s2));
+    return;
+  }
+  const double* q = par + 5 * b;
+  const cplx<double> e1 = cmake(q[0], q[1]), e2 = cmake(q[2], q[3]);
+  const double k = q[4];
+  const cplx<double> L = csub(e1, e2);
+  const cplx<double> cf = cscale(cmul(L, L), k * k);
+  const double nn = nvec[3 * r] * nvec[3 * c] + nvec[3 * r + 1] * nvec[3 * c + 1] + nvec[3 * r + 2] * nvec[3 * c + 2];
+  const cplx<T> ev = cld(E + o);
+  const cplx<double> ed = cmake((double)ev.re, (double)ev.im);
+  const cplx<double> s1d = cmake((double)s1.re, (double)s1.im), s2d = cmake((double)s2.re, (double)s2.im);
+  cplx<double> v = csub(cmul(e1, s1d), cmul(e2, s2d));
+  v = cadd(v, cscale(cmul(cf, ed), nn));
+  cst(S2 + o, cmake<T>((T)v.re, (T)v.im));
+}
+
+// --- small elementwise steps of the direct solve (bem_ret.py:1725-1810) ---
+// per-wavelength parameters par[b] = {e1r, e1i, e2r, e2i, k}
+struct Par {
+  cplx<double> e1, e2;
+  double k;
+};
+__device__ __forceinline__ Par load_par(const double* par, long b) {
+  const double* q = par + 5 * b;
+  Par p;
+  p.e1 = cmake(q[0], q[1]);
+  p.e2 = cmake(q[2], q[3]);
+  p.k = q[4];
+  return p;
+}
+template <typename T>
+__device__ __forceinline__ cplx<double> ldd(const T* p) {
+  const cplx<T> v = cld(p);
+  return cmake((double)v.re, (double)v.im);
+}
+template <typename T>
+__device__ __forceinline__ void std_(T* p, cplx<double> v) {
+  cst(p, cmake<T>((T)v.re, (T)v.im));
+}
+__device__ __forceinline__ cplx<double> ik_times(double k, cplx<double> z) { return cmake(-k * z.im, k * z.re); }
+
+// step 1: W holds [Sigma1 phi | Sigma1 a] in columns [0, 4P).  Writes
+//   alpha' = alpha - Sigma1 a + ik n (e1 phi)        -> W[:, 5P:8P]
+//   De'    = De - e1 Sigma1 phi + ik n.(e1 a)        -> W[:, 4P:5P]
+template <typename T>
+__global__ void k_step_mod(int n, int P, const T* R, T* W, long ld, long stride, const double* nvec,
+                           const double* par) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Par q = load_par(par, b);
+  const double nx[3] = {nvec[3 * i], nvec[3 * i + 1], nvec[3 * i + 2]};
+  const T* r = R + 2 * (b * stride + (long)i * ld);
+  T* w = W + 2 * (b * stride + (long)i * ld);
+  for (int p = 0; p < P; ++p) {
+    const cplx<double> e1phi = cmul(q.e1, ldd(r + 2 * p));
+    cplx<double> nda = cmake(0.0, 0.0);
+    for (int c = 0; c < 3; ++c) {
+      const cplx<double> a = ldd(r + 2 * (P + c * P + p));
+      nda = cadd(nda, cscale(a, nx[c]));
+      cplx<double> al = ldd(r + 2 * (5 * P + c * P + p));
+      al = csub(al, ldd(w + 2 * (P + c * P + p)));
+      al = cadd(al, ik_times(q.k, cscale(e1phi, nx[c])));
+      std_(w + 2 * (5 * P + c * P + p), al);
+    }
+    cplx<double> de = ldd(r + 2 * (4 * P + p));
+    de = csub(de, cmul(q.e1, ldd(w + 2 * p)));
+    de = cadd(de, ik_times(q.k, cmul(q.e1, nda)));
+    std_(w + 2 * (4 * P + p), de);
+  }
+}
+
+// step 2: U = Delta^-1 alpha' in W[:, P:4P].  rhs = De' + ik n.((e1-e2) U) -> W[:, 0:P]
+template <typename T>
+__global__ void k_step_rhs(int n, int P, T* W, long ld, long stride, const double* nvec, const double* par) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Par q = load_par(par, b);
+  const cplx<double> Ld = csub(q.e1, q.e2);
+  T* w = W + 2 * (b * stride + (long)i * ld);
+  for (int p = 0; p < P; ++p) {
+    cplx<double> s = cmake(0.0, 0.0);
+    for (int c = 0; c < 3; ++c) s = cadd(s, cscale(ldd(w + 2 * (P + c * P + p)), nvec[3 * i + c]));
+    const cplx<double> v = cadd(ldd(w + 2 * (4 * P + p)), ik_times(q.k, cmul(Ld, s)));
+    std_(w + 2 * p, v);
+  }
+}
+
+// step 3: sig2 = Sigma^-1 rhs in W[:, 0:P].  T = ik n ((e1-e2) sig2) + alpha' -> W[:, P:4P]
+template <typename T>
+__global__ void k_step_h2pre(int n, int P, T* W, long ld, long stride, const double* nvec, const double* par) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Par q = load_par(par, b);
+  const cplx<double> Ld = csub(q.e1, q.e2);
+  T* w = W + 2 * (b * stride + (long)i * ld);
+  for (int p = 0; p < P; ++p) {
+    const cplx<double> ls = cmul(Ld, ldd(w + 2 * p));
+    for (int c = 0; c < 3; ++c) {
+      const cplx<double> v = cadd(ik_times(q.k, cscale(ls, nvec[3 * i + c])), ldd(w + 2 * (5 * P + c * P + p)));
+      std_(w + 2 * (P + c * P + p), v);
+    }
+  }
+}
+
+// step 4: h2 = Delta^-1 T in W[:, 4P:7P] (written there by the GEMV).
+//   X[:, 0:4P] = [sig2 + phi | h2 + a],  X[:, 4P:8P] = [sig2 | h2]
+template <typename T>
+__global__ void k_step_pack(int n, int P, const T* R, const T* W, T* X, long ld, long stride) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T* r = R + 2 * (b * stride + (long)i * ld);
+  const T* w = W + 2 * (b * stride + (long)i * ld);
+  T* x = X + 2 * (b * stride + (long)i * ld);
+  for (int p = 0; p < P; ++p) {
+    const cplx<T> s2 = cld(w + 2 * p);
+    cst(x + 2 * p, cadd(s2, cld(r + 2 * p)));
+    cst(x + 2 * (4 * P + p), s2);
+    for (int c = 0; c < 3; ++c) {
+      const cplx<T> h2 = cld(w + 2 * (4 * P + c * P + p));
+      cst(x + 2 * (P + c * P + p), cadd(h2, cld(r + 2 * (P + c * P + p))));
+      cst(x + 2 * (5 * P + c * P + p), h2);
+    }
+  }
+}
+
+// residual of the 8N operator (GdA&H Eqs. 10, 11, 14, 17; bem_ret_iter.py:1362-1501):
+// Y = [G1 X1 | H1 X1 | G2 X2 | H2 X2], each (N, 4P), in one (N, 16P) buffer.
+//   phi = G1 s1 - G2 s2;  a = G1 h1 - G2 h2
+//   alpha = H1 h1 - H2 h2 - ik n (e1 G1 s1 - e2 G2 s2)
+//   De = e1 H1 s1 - e2 H2 s2 - ik n.(e1 G1 h1 - e2 G2 h2)
+// Rout = R - A X; nrm[b] += |Rout|^2 and |R|^2 (atomic per wavelength)
+__global__ void k_residual(int n, int P, const double* R, const double* Y, double* Rout, long ldr, long sr,
+                           long ldy, long sy, const double* nvec, const double* par, double* nrm) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc_r = 0.0, acc_b = 0.0;
+  if (i < n) {
+    const Par q = load_par(par, b);
+    const double* r = R + 2 * (b * sr + (long)i * ldr);
+    const double* y = Y + 2 * (b * sy + (long)i * ldy);
+    double* o = Rout + 2 * (b * sr + (long)i * ldr);
+    const int P4 = 4 * P;
+    auto G1 = [&](int col) { return ldd(y + 2 * col); };
+    auto H1 = [&](int col) { return ldd(y + 2 * (P4 + col)); };
+    auto G2 = [&](int col) { return ldd(y + 2 * (2 * P4 + col)); };
+    auto H2 = [&](int col) { return ldd(y + 2 * (3 * P4 + col)); };
+    auto put = [&](int col, cplx<double> v) {
+      const cplx<double> rv = ldd(r + 2 * col);
+      const cplx<double> d = csub(rv, v);
+      acc_r += d.re * d.re + d.im * d.im;
+      acc_b += rv.re * rv.re + rv.im * rv.im;
+      std_(o + 2 * col, d);
+    };
+    for (int p = 0; p < P; ++p) {
+      put(p, csub(G1(p), G2(p)));
+      const cplx<double> t = csub(cmul(q.e1, G1(p)), cmul(q.e2, G2(p)));
+      cplx<double> ndg = cmake(0.0, 0.0);
+      for (int c = 0; c < 3; ++c) {
+        const int hc = P + c * P + p;
+        put(hc, csub(G1(hc), G2(hc)));
+        const cplx<double> al = csub(csub(H1(hc), H2(hc)), ik_times(q.k, cscale(t, nvec[3 * i + c])));
+        put(4 * P + hc, al);
+        ndg = cadd(ndg, cscale(csub(cmul(q.e1, G1(hc)), cmul(q.e2, G2(hc))), nvec[3 * i + c]));
+      }
+      const cplx<double> de = csub(csub(cmul(q.e1, H1(p)), cmul(q.e2, H2(p))), ik_times(q.k, ndg));
+      put(4 * P + p, de);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    acc_r += __shfl_xor(acc_r, off);
+    acc_b += __shfl_xor(acc_b, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(nrm + 2 * b, acc_r);
+    atomicAdd(nrm + 2 * b + 1, acc_b);
+  }
+}
+
+// X64 += (double) D ;  R32 = (float) R64   (refinement bookkeeping)
+template <typename TS>
+__global__ void k_axpy_cast(long count, const TS* src, double* dst) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < count) dst[e] += (double)src[e];
+}
+template <typename TD>
+__global__ void k_copy_cast(long count, const double* src, TD* dst) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < count) dst[e] = (TD)src[e];
+}
+__global__ void k_fill_double(long count, double* p, double v) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < count) p[e] = v;
+}
+
+// plane wave a = e^{i k pos.dir}/(i k0) pol, a' = i k (n.dir) a on faces whose
+// side touches the embedding medium (planewave_ret.py:257-361); then the
+// right side R = [phi=0 | a2-a1 | De | alpha=a2'-a1'] with
+// De = -i k0 (eps2 n.a2 - eps1 n.a1) (bem_ret.py:1360-1429).
+// pw[b] = {k0, kr, ki, e1r, e1i, e2r, e2i}; pol, dir (P, 3) real.
+__global__ void k_planewave(int n, int P, const double* pos, const double* nvec, const int* in_med,
+                            const int* out_med, const double* pol, const double* dir, const double* pw,
+                            double* R, long ld, long stride, double* pot, long spot) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* q = pw + 7 * b;
+  const double k0 = q[0], kr = q[1], ki = q[2];
+  const cplx<double> e1 = cmake(q[3], q[4]), e2 = cmake(q[5], q[6]);
+  double* r = R + 2 * (b * stride + (long)i * ld);
+  const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+  const double nx[3] = {nvec[3 * i], nvec[3 * i + 1], nvec[3 * i + 2]};
+  for (int p = 0; p < P; ++p) {
+    const double* dp = dir + 3 * p;
+    const double* pp = pol + 3 * p;
+    const double kd = px * dp[0] + py * dp[1] + pz * dp[2];
+    const double nd = nx[0] * dp[0] + nx[1] * dp[1] + nx[2] * dp[2];
+    // phase / (i k0) = e^{i k kd} * (-i / k0)
+    const cplx<double> ph = cexpi(kr * kd, ki * kd);
+    const cplx<double> amp = cmake(ph.im / k0, -ph.re / k0);
+    const cplx<double> ikn = cmake(-ki * nd, kr * nd);  // i k (n.dir)
+    const cplx<double> z = cmake(0.0, 0.0);
+    const cplx<double> a1 = in_med[i] ? amp : z, a2 = out_med[i] ? amp : z;
+    const cplx<double> ap1 = in_med[i] ? cmul(ikn, amp) : z, ap2 = out_med[i] ? cmul(ikn, amp) : z;
+    std_(r + 2 * p, z);
+    double npol = nx[0] * pp[0] + nx[1] * pp[1] + nx[2] * pp[2];
+    for (int c = 0; c < 3; ++c) {
+      std_(r + 2 * (P + c * P + p), cscale(csub(a2, a1), pp[c]));
+      std_(r + 2 * (5 * P + c * P + p), cscale(csub(ap2, ap1), pp[c]));
+      if (pot) {
+        double* o = pot + 2 * (b * spot + (long)i * 12 * P);
+        std_(o + 2 * (0 * 3 * P + c * P + p), cscale(a1, pp[c]));
+        std_(o + 2 * (1 * 3 * P + c * P + p), cscale(ap1, pp[c]));
+        std_(o + 2 * (2 * 3 * P + c * P + p), cscale(a2, pp[c]));
+        std_(o + 2 * (3 * 3 * P + c * P + p), cscale(ap2, pp[c]));
+      }
+    }
+    const cplx<double> t = csub(cmul(e2, cscale(a2, npol)), cmul(e1, cscale(a1, npol)));
+    std_(r + 2 * (4 * P + p), cmake(k0 * t.im, -k0 * t.re));  // -i k0 t
+  }
+}
+
+// ext[b][p] = 4 pi / k Im(conj(pol_p) . e_p),
+// e_p = sum_j e^{-i k dir_p.pos_j} area_j (i k0 h_j - i k dir_p sig_j) over
+// faces whose inside (sig1, h1) or outside (sig2, h2) is the medium
+// (spectrum_ret.py:231-392, planewave_ret.py:437-504).  One workgroup per (b, p).
+__global__ __launch_bounds__(256) void k_extinction(int n, int P, const double* X, long ld, long stride,
+                                                    const double* pos, const double* area, const int* in_med,
+                                                    const int* out_med, const double* pol, const double* dir,
+                                                    const double* pw, double* ext) {
+  const long b = blockIdx.y;
+  const int p = blockIdx.x;
+  const double* q = pw + 7 * b;
+  const double k0 = q[0], kr = q[1], ki = q[2];
+  const double* dp = dir + 3 * p;
+  const double* pp = pol + 3 * p;
+  cplx<double> acc = cmake(0.0, 0.0);
+  for (int j = threadIdx.x; j < n; j += 256) {
+    const double kd = pos[3 * j] * dp[0] + pos[3 * j + 1] * dp[1] + pos[3 * j + 2] * dp[2];
+    // e^{-i k kd} area
+    const cplx<double> ph = cscale(cexpi(-kr * kd, -ki * kd), area[j]);
+    const double* x = X + 2 * (b * stride + (long)j * ld);
+    for (int side = 0; side < 2; ++side) {
+      if (!(side == 0 ? in_med[j] : out_med[j])) continue;
+      const int off = side * 4 * P;
+      const cplx<double> sg = ldd(x + 2 * (off + p));
+      cplx<double> ph_ = cmake(0.0, 0.0);  // conj(pol).h
+      for (int c = 0; c < 3; ++c) ph_ = cadd(ph_, cscale(ldd(x + 2 * (off + P + c * P + p)), pp[c]));
+      const double pd = pp[0] * dp[0] + pp[1] * dp[1] + pp[2] * dp[2];
+      // i k0 (pol.h) - i k (pol.dir) sig
+      const cplx<double> t1 = cmake(-k0 * ph_.im, k0 * ph_.re);
+      const cplx<double> ksg = cmul(cmake(kr, ki), sg);
+      const cplx<double> t2 = cmake(-ksg.im * pd, ksg.re * pd);
+      acc = cadd(acc, cmul(ph, csub(t1, t2)));
+    }
+  }
+  __shared__ double sr[4], si[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    acc.re += __shfl_xor(acc.re, off);
+    acc.im += __shfl_xor(acc.im, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sr[threadIdx.x >> 6] = acc.re;
+    si[threadIdx.x >> 6] = acc.im;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double im = si[0] + si[1] + si[2] + si[3];
+    // Re(4 pi / k) * Im(conj(pol).e); pol is real
+    const double den = kr * kr + ki * ki;
+    ext[b * P + p] = 4.0 * M_PI * kr / den * im;
+  }
+}
+
+template <typename T>
+int gemv_t(int n, int C, const T* M, long ldm, long sM, const T* X, long ldx, long sX, T* Y, long ldy, long sY,
+           double alr, double ali, double beta, int batch, hipStream_t st) {
+  for (int c0 = 0; c0 < C; c0 += MAXC) {
+    const int cc = std::min(MAXC, C - c0);
+    hipLaunchKernelGGL(k_gemv_t<T>, dim3(mnp_ceil_div(n, 64), batch), dim3(256), 0, st, n, cc, M, ldm, sM,
+                       X + 2 * c0, ldx, sX, Y + 2 * c0, ldy, sY, alr, ali, beta);
+    MNP_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+template <typename T>
+int lu_solve_t(int n, int C, const T* LU, long ld, long sLU, const int* ipiv, long sP, T* V, long ldv, long sV,
+               int batch, hipStream_t st) {
+  for (int c0 = 0; c0 < C; c0 += MAXC) {
+    const int cc = std::min(MAXC, C - c0);
+    hipLaunchKernelGGL(k_lu_solve_t<T>, dim3(batch), dim3(256), 0, st, n, cc, LU, ld, sLU, ipiv, sP, V + 2 * c0,
+                       ldv, sV);
+    MNP_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, long ldm, long sM, const void* X,
+                                     long ldx, long sX, void* Y, long ldy, long sY, double alr, double ali,
+                                     double beta, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (n <= 0 || C <= 0 || batch <= 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    return gemv_t<double>(n, C, (const double*)M, ldm, sM, (const double*)X, ldx, sX, (double*)Y, ldy, sY, alr,
+                          ali, beta, batch, st);
+  if (dtype == MNP_DTYPE_C64)
+    return gemv_t<float>(n, C, (const float*)M, ldm, sM, (const float*)X, ldx, sX, (float*)Y, ldy, sY, alr, ali,
+                         beta, batch, st);
+  mnp_set_error("gemv_t: unknown dtype");
+  return 1;
+}
+
+extern "C" int mnpbem_lu_solve_t_batched(int dtype, int n, int C, const void* LU, long ld, long sLU,
+                                         const int* ipiv, long sP, void* V, long ldv, long sV, int batch,
+                                         void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (n <= 0 || C <= 0 || batch <= 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    return lu_solve_t<double>(n, C, (const double*)LU, ld, sLU, ipiv, sP, (double*)V, ldv, sV, batch, st);
+  if (dtype == MNP_DTYPE_C64)
+    return lu_solve_t<float>(n, C, (const float*)LU, ld, sLU, ipiv, sP, (float*)V, ldv, sV, batch, st);
+  mnp_set_error("lu_solve_t: unknown dtype");
+  return 1;
+}
+
+extern "C" int mnpbem_identity_batched(int dtype, int n, void* A, long ld, long sA, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (n <= 0 || batch <= 0) return 0;
+  dim3 grid(mnp_ceil_div((long)n * n, 256), batch);
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_identity<double>, grid, dim3(256), 0, st, n, (double*)A, ld, sA);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_identity<float>, grid, dim3(256), 0, st, n, (float*)A, ld, sA);
+  else {
+    mnp_set_error("identity: unknown dtype");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// mode 0: D = S1 - S2 ; mode 1: S2 <- Sigma (needs E)
+extern "C" int mnpbem_sigma_batched(int dtype, int mode, int n, const void* S1, void* S2, void* D, const void* E,
+                                    long ld, long stride, const double* nvec, const double* par, int batch,
+                                    void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (n <= 0 || batch <= 0) return 0;
+  dim3 grid(mnp_ceil_div((long)n * n, 256), batch);
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_sigma<double>, grid, dim3(256), 0, st, mode, n, (const double*)S1, (double*)S2,
+                       (double*)D, (const double*)E, ld, stride, nvec, par);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_sigma<float>, grid, dim3(256), 0, st, mode, n, (const float*)S1, (float*)S2, (float*)D,
+                       (const float*)E, ld, stride, nvec, par);
+  else {
+    mnp_set_error("sigma: unknown dtype");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Direct solve X = A^-1 R for ``batch`` wavelengths (bem_ret.py:1694-1810).
+// R, W, X: (batch, n, 8P) buffers of the factorisation's dtype, stride in
+// complex elements; W is work space.
+extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t, const int* piv1, const void* G2t,
+                                        const int* piv2, const void* S1t, const void* Et, const void* St,
+                                        const int* pivS, long ldm, long smat, long spiv, const void* R, void* W,
+                                        void* X, long ldv, long sv, const double* nvec, const double* par, int batch,
+                                        void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldv >= 8 * P, "bem_solve: bad size (P <= 4)");
+  if (batch <= 0) return 0;
+  dim3 eg(mnp_ceil_div(n, 256), batch);
+#define MNP_SOLVE(T)                                                                                               \
+  {                                                                                                                \
+    const T* r = (const T*)R;                                                                                      \
+    T* w = (T*)W;                                                                                                  \
+    T* x = (T*)X;                                                                                                  \
+    int rc;                                                                                                        \
+    /* W[:,0:4P] = Sigma1 [phi | a] */                                                                             \
+    if ((rc = gemv_t<T>(n, 4 * P, (const T*)S1t, ldm, smat, r, ldv, sv, w, ldv, sv, 1.0, 0.0, 0.0, batch, st)))   \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_step_mod<T>, eg, dim3(256), 0, st, n, P, r, w, ldv, sv, nvec, par);                       \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    /* W[:,P:4P] = Delta^-1 alpha' */                                                                              \
+    if ((rc = gemv_t<T>(n, 3 * P, (const T*)Et, ldm, smat, w + 2 * 5 * P, ldv, sv, w + 2 * P, ldv, sv, 1.0, 0.0, \
+                        0.0, batch, st)))                                                                          \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_step_rhs<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                          \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    if ((rc = lu_solve_t<T>(n, P, (const T*)St, ldm, smat, pivS, spiv, w, ldv, sv, batch, st))) return rc;        \
+    hipLaunchKernelGGL(k_step_h2pre<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                        \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    /* W[:,4P:7P] = Delta^-1 T */                                                                                  \
+    if ((rc = gemv_t<T>(n, 3 * P, (const T*)Et, ldm, smat, w + 2 * P, ldv, sv, w + 2 * 4 * P, ldv, sv, 1.0, 0.0, \
+                        0.0, batch, st)))                                                                          \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_step_pack<T>, eg, dim3(256), 0, st, n, P, r, w, x, ldv, sv);                              \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G1t, ldm, smat, piv1, spiv, x, ldv, sv, batch, st))) return rc;  \
+    if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G2t, ldm, smat, piv2, spiv, x + 2 * 4 * P, ldv, sv, batch, st)))  \
+      return rc;                                                                                                   \
+    return 0;                                                                                                      \
+  }
+  if (dtype == MNP_DTYPE_C128) MNP_SOLVE(double)
+  if (dtype == MNP_DTYPE_C64) MNP_SOLVE(float)
+#undef MNP_SOLVE
+  mnp_set_error("bem_solve: unknown dtype");
+  return 1;
+}
+
+// Rout = R - A X (complex128), Y: (batch, n, 16P) work buffer; nrm: (batch, 2)
+// receives ||R - A X||^2 and ||R||^2.  G1t.. are complex128 transposed
+// Green matrices (not factored).
+extern "C" int mnpbem_bem_residual_batched(int n, int P, const void* G1t, const void* H1t, const void* G2t,
+                                           const void* H2t, long ldm, long smat, const void* R, const void* X,
+                                           void* Rout, long ldv, long sv, void* Y, long ldy, long sy,
+                                           const double* nvec, const double* par, double* nrm, int batch,
+                                           void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldy >= 16 * P, "bem_residual: bad size");
+  if (batch <= 0) return 0;
+  const double* x = (const double*)X;
+  double* y = (double*)Y;
+  const int P4 = 4 * P;
+  int rc;
+  if ((rc = gemv_t<double>(n, P4, (const double*)G1t, ldm, smat, x, ldv, sv, y, ldy, sy, 1, 0, 0, batch, st)))
+    return rc;
+  if ((rc = gemv_t<double>(n, P4, (const double*)H1t, ldm, smat, x, ldv, sv, y + 2 * P4, ldy, sy, 1, 0, 0, batch,
+                           st)))
+    return rc;
+  if ((rc = gemv_t<double>(n, P4, (const double*)G2t, ldm, smat, x + 2 * P4, ldv, sv, y + 2 * 2 * P4, ldy, sy, 1,
+                           0, 0, batch, st)))
+    return rc;
+  if ((rc = gemv_t<double>(n, P4, (const double*)H2t, ldm, smat, x + 2 * P4, ldv, sv, y + 2 * 3 * P4, ldy, sy, 1,
+                           0, 0, batch, st)))
+    return rc;
+  hipLaunchKernelGGL(k_fill_double, dim3(mnp_ceil_div(2L * batch, 256)), dim3(256), 0, st, 2L * batch, nrm, 0.0);
+  MNP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_residual, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, P, (const double*)R,
+                     (const double*)Y, (double*)Rout, ldv, sv, ldy, sy, nvec, par, nrm);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// dst(double) += src (float or double), count real values
+extern "C" int mnpbem_accumulate(int src_dtype, long count, const void* src, double* dst, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (count <= 0) return 0;
+  dim3 grid(mnp_ceil_div(count, 256));
+  if (src_dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_axpy_cast<float>, grid, dim3(256), 0, st, count, (const float*)src, dst);
+  else
+    hipLaunchKernelGGL(k_axpy_cast<double>, grid, dim3(256), 0, st, count, (const double*)src, dst);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// dst = (dst type) src(double), count real values
+extern "C" int mnpbem_convert(int dst_dtype, long count, const double* src, void* dst, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (count <= 0) return 0;
+  dim3 grid(mnp_ceil_div(count, 256));
+  if (dst_dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_copy_cast<float>, grid, dim3(256), 0, st, count, src, (float*)dst);
+  else
+    hipLaunchKernelGGL(k_copy_cast<double>, grid, dim3(256), 0, st, count, src, (double*)dst);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int mnpbem_planewave_batched(int n, int P, const double* pos, const double* nvec, const int* in_med,
+                                        const int* out_med, const double* pol, const double* dir, const double* pw,
+                                        double* R, long ldv, long sv, double* pot, long spot, int batch,
+                                        void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && ldv >= 8 * P, "planewave: bad size");
+  if (batch <= 0) return 0;
+  hipLaunchKernelGGL(k_planewave, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, P, pos, nvec, in_med,
+                     out_med, pol, dir, pw, R, ldv, sv, pot, spot);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int mnpbem_extinction_batched(int n, int P, const double* X, long ldv, long sv, const double* pos,
+                                         const double* area, const int* in_med, const int* out_med,
+                                         const double* pol, const double* dir, const double* pw, double* ext,
+                                         int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0, "extinction: bad size");
+  if (batch <= 0) return 0;
+  hipLaunchKernelGGL(k_extinction, dim3(P, batch), dim3(256), 0, st, n, P, X, ldv, sv, pos, area, in_med, out_med,
+                     pol, dir, pw, ext);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}

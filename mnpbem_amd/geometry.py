@@ -1,0 +1,188 @@
+"""Particle surfaces of quadratic (6-node) triangles, the sphere mesh
+generator and the composite particle (host-side set-up).
+
+Mirrors mnpbem/geometry: ``trisphere`` (mesh_generators.py:13-130),
+``Particle`` with curved interpolation (particle.py:310-613) and
+``ComParticle`` (comparticle.py:55-127).  Only what the retarded BEM path
+needs is kept: centroids, normals, tangents, areas, element radii and the
+quadrature nodes that the GPU refinement kernel integrates over.
+"""
+
+import os
+
+import numpy as np
+
+from .quadrature import rule18, polar_rule, tri6_tables
+
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+# vertex counts of the saved sphere point sets (mesh_generators.py:58-60)
+SPHERE_SIZES = np.array([32, 60, 144, 169, 225, 256, 289, 324, 361, 400, 441, 484,
+                         529, 576, 625, 676, 729, 784, 841, 900, 961, 1024, 1225, 1444])
+
+
+def _read_sphere(n):
+    """int32 nv, nf; float64 x[nv], y[nv], z[nv]; int32 faces[nf, 3] (1-based)."""
+    path = os.path.join(_DATA, "sphere%d_tri.bin" % n)
+    if not os.path.exists(path):
+        raise FileNotFoundError("sphere mesh with %d vertices is not shipped (%s)" % (n, path))
+    raw = open(path, "rb").read()
+    nv, nf = np.frombuffer(raw, dtype=np.int32, count=2)
+    off = 8
+    xyz = np.frombuffer(raw, dtype=np.float64, count=3 * nv, offset=off).reshape(3, nv).T.copy()
+    off += 24 * nv
+    faces = np.frombuffer(raw, dtype=np.int32, count=3 * nf, offset=off).reshape(nf, 3).astype(np.int64) - 1
+    return xyz, faces
+
+
+class Particle:
+    """Surface of curved triangles given by 6 nodes per face: three corners
+    and the three edge midpoints (v0, v1, v2, m01, m12, m20).
+
+    Attributes (numpy, float64): ``pos`` centroids (n, 3), ``nvec`` unit
+    normals, ``tvec1``/``tvec2`` tangents, ``area``, ``nodes`` (n, 6, 3).
+    """
+
+    def __init__(self, verts2, faces6, interp="curv"):
+        if interp != "curv":
+            raise NotImplementedError("only curved (6-node) boundary elements are supported")
+        self.verts2 = np.asarray(verts2, dtype=float)
+        self.faces6 = np.asarray(faces6, dtype=np.int64)
+        if self.verts2.ndim != 2 or self.verts2.shape[1] != 3:
+            raise ValueError("Particle: 'verts' must have shape (N, 3); got %s." % (self.verts2.shape,))
+        if self.faces6.ndim != 2 or self.faces6.shape[1] != 6:
+            raise ValueError("Particle: faces must list 6 nodes per triangle")
+        self.interp = interp
+        self._geometry()
+
+    @property
+    def n(self):
+        return len(self.faces6)
+
+    nfaces = n
+
+    @property
+    def nvec(self):
+        return self._nvec
+
+    @property
+    def tvec1(self):
+        return self._t1
+
+    @property
+    def tvec2(self):
+        return self._t2
+
+    def _geometry(self):
+        X = self.verts2[self.faces6]                                  # (n, 6, 3)
+        self.nodes = np.ascontiguousarray(X)
+        # area = integral of the surface Jacobian with the 28-point rule
+        S, Dx, Dy, w = tri6_tables(*rule18())
+        tx = np.einsum("qc,ncd->nqd", Dx, X)
+        ty = np.einsum("qc,ncd->nqd", Dy, X)
+        jac = np.linalg.norm(np.cross(tx, ty), axis=2)
+        self.area = (0.5 * w[None, :] * jac).sum(axis=1)
+        # centroid and tangents from the shape functions at (1/3, 1/3)
+        S0, Dx0, Dy0, _ = tri6_tables(np.array([1 / 3]), np.array([1 / 3]), np.array([1.0]))
+        self.pos = X.transpose(0, 2, 1) @ S0[0]
+        a = X.transpose(0, 2, 1) @ Dy0[0]
+        b = X.transpose(0, 2, 1) @ Dx0[0]
+        nv = np.cross(a, b)
+        nv /= np.maximum(np.linalg.norm(nv, axis=1, keepdims=True), 1e-14)
+        a /= np.maximum(np.linalg.norm(a, axis=1, keepdims=True), 1e-14)
+        self._nvec, self._t1 = nv, a
+        self._t2 = np.cross(nv, a)
+        corners = X[:, :3, :]
+        self._brad = np.sqrt(((corners - self.pos[:, None, :]) ** 2).sum(axis=2)).max(axis=1)
+
+    def bradius(self):
+        """Radius of the smallest centroid-centred sphere through the corners
+        (particle.py:2848-2880)."""
+        return self._brad
+
+
+def trisphere(n, diameter=1.0, **kwargs):
+    """Sphere of curved triangles from the saved point set closest to ``n``
+    vertices: scale to the diameter, add one node per edge, project all
+    nodes onto the sphere (mesh_generators.py:13-130)."""
+    if n is None:
+        raise ValueError("trisphere: 'n' must be a positive integer, got None.")
+    n = int(n)
+    if n <= 0:
+        raise ValueError("trisphere: 'n' must be > 0, got %d." % n)
+    if not np.isfinite(diameter) or diameter <= 0:
+        raise ValueError("trisphere: 'diameter' must be a positive finite float, got %r." % diameter)
+    m = int(SPHERE_SIZES[np.argmin(np.abs(SPHERE_SIZES - n))])
+    verts, faces = _read_sphere(m)
+    verts = verts * (0.5 * diameter)
+    edges = np.sort(np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]]), axis=1)
+    uniq, inv = np.unique(edges, axis=0, return_inverse=True)
+    mid = 0.5 * (verts[uniq[:, 0]] + verts[uniq[:, 1]])
+    nodes = np.vstack([verts, mid])
+    nodes = nodes * (0.5 * diameter / np.linalg.norm(nodes, axis=1))[:, None]
+    faces6 = np.hstack([faces, len(verts) + inv.reshape(3, -1).T])
+    return Particle(nodes, faces6, **kwargs)
+
+
+class ComParticle:
+    """Composite of particles with materials inside/outside.
+
+    ``ComParticle(eps, [p1, p2, ...], [[in, out], ...], *closed)`` with
+    1-based material indices (comparticle.py:55-127).  Every particle is
+    treated as its own closed surface with outward normals.
+    """
+
+    def __init__(self, eps, particles, inout, *closed, **kwargs):
+        if eps is None:
+            raise ValueError("ComParticle: 'eps' must be a list of dielectric functions, got None.")
+        if particles is None:
+            raise ValueError("ComParticle: 'particles' must be a list of Particle, got None.")
+        if inout is None:
+            raise ValueError("ComParticle: 'inout' must be an inside/outside index table, got None.")
+        if not hasattr(eps, "__len__") or len(eps) == 0:
+            raise ValueError("ComParticle: 'eps' must contain at least one dielectric function.")
+        if isinstance(particles, Particle):
+            particles = [particles]
+        self.eps = list(eps)
+        self.p = list(particles)
+        io = np.atleast_2d(np.asarray(inout))
+        if io.ndim != 2 or io.shape[1] != 2 or io.shape[0] != len(self.p):
+            raise ValueError("ComParticle: 'inout' must have shape (nparticles, 2); got %s." % (io.shape,))
+        if int(io.max()) > len(self.eps) or int(io.min()) < 1:
+            raise ValueError("ComParticle: 'inout' indices are 1-based and must reference eps.")
+        if kwargs.get("interp", "curv") != "curv":
+            raise NotImplementedError("only interp='curv' is supported")
+        self.inout = io.astype(int)
+        self.closed = list(closed) if closed else [None] * len(self.p)
+        offs = np.cumsum([0] + [q.n for q in self.p])
+        self.offsets = offs
+        cat = lambda f: np.ascontiguousarray(np.concatenate([f(q) for q in self.p]))
+        self.pos = cat(lambda q: q.pos)
+        self.nvec = cat(lambda q: q.nvec)
+        self.tvec1 = cat(lambda q: q.tvec1)
+        self.tvec2 = cat(lambda q: q.tvec2)
+        self.area = cat(lambda q: q.area)
+        self.nodes = cat(lambda q: q.nodes)
+        self.brad = cat(lambda q: q.bradius())
+        self.inout_faces = np.concatenate([np.tile(self.inout[i], (q.n, 1)) for i, q in enumerate(self.p)])
+        self.surface = np.concatenate([np.full(q.n, i) for i, q in enumerate(self.p)])
+
+    @property
+    def n(self):
+        return int(self.offsets[-1])
+
+    nfaces = n
+
+    @property
+    def np(self):
+        return len(self.p)
+
+    def index(self, i):
+        """0-based face indices of particle i (1-based, as in MATLAB)."""
+        return np.arange(self.offsets[i - 1], self.offsets[i])
+
+    def eps1(self, enei):
+        return np.array([self.eps[m - 1](enei)[0] for m in self.inout_faces[:, 0]])
+
+    def eps2(self, enei):
+        return np.array([self.eps[m - 1](enei)[0] for m in self.inout_faces[:, 1]])

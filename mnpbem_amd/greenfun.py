@@ -1,0 +1,170 @@
+"""Composite retarded Green function with refined near-field quadrature.
+
+``CompGreenRet(p, p)`` mirrors mnpbem/greenfun/compgreen_ret.py: at
+construction the refinement classification is done on the host
+(refine_utils.py:29-158), the refinement coefficients and the Fuchs-Liu
+closed-surface correction are integrated on the GPU (HIP kernels in
+csrc/green.hip), and all of it stays resident in device memory.
+``eval(i, j, key, enei)`` assembles a full matrix on the GPU.
+"""
+
+import math
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .quadrature import kernel_table, polar_rule, rule18
+
+
+def refinement_pairs(pos, brad, rel_cutoff=3.0, abs_cutoff=0.0, chunk=4096):
+    """Face pairs needing refined integration: kind 2 where the centroid
+    distance is zero, kind 1 where (d - r_j)/r_i < RelCutoff or
+    d - r_j < AbsCutoff.  Distances use |a|^2 + |b|^2 - 2 a.b with squares
+    below 1e-10 set to zero, the form of refine_utils.py:12-26 (so the
+    classification at the cutoff boundary matches it to the last bit).
+    Returns row, col, kind (int64) sorted by (row, col)."""
+    n = len(pos)
+    sq = np.sum(pos * pos, axis=1)
+    rows, cols, kinds = [], [], []
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        dsq = sq[:, None] + sq[None, c0:c1] - 2.0 * (pos @ pos[c0:c1].T)
+        dsq[dsq < 1e-10] = 0.0
+        d = np.sqrt(dsq)
+        d2 = d - brad[None, c0:c1]
+        near = ((d2 < abs_cutoff) | (d2 / brad[:, None] < rel_cutoff)) & (d != 0)
+        r, c = np.nonzero(near | (d == 0))
+        k = np.where(d[r, c] == 0, 2, 1)
+        rows.append(r)
+        cols.append(c + c0)
+        kinds.append(k)
+    row, col, kind = (np.concatenate(a) for a in (rows, cols, kinds))
+    o = np.lexsort((col, row))
+    return row[o], col[o], kind[o]
+
+
+def _dev(a, dtype, device):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(device)
+
+
+class CompGreenRet:
+    """Retarded Green function G, F, H1 = F + 2 pi, H2 = F - 2 pi between the
+    faces of a composite particle and itself (deriv='cart', refinement order
+    ``order``, RelCutoff/AbsCutoff as in bemoptions)."""
+
+    name = "greenfunction"
+    needs = {"sim": "ret"}
+
+    def __init__(self, p1, p2, order=5, RelCutoff=3, AbsCutoff=0, device="cuda", **options):
+        if p1 is not p2:
+            raise NotImplementedError("CompGreenRet: only the BEM case p1 is p2 is supported")
+        if options.get("deriv", "cart") != "cart":
+            raise NotImplementedError("CompGreenRet: only deriv='cart' is supported")
+        self.p1 = self.p2 = p = p1
+        self.order = int(order)
+        self.device = torch.device(device)
+        dev = self.device
+        n = p.n
+        self.n = n
+        row, col, kind = refinement_pairs(p.pos, p.brad, RelCutoff, AbsCutoff)
+        self.npairs = len(row)
+        self.row, self.col, self.kind = row, col, kind
+        f64, i32 = torch.float64, torch.int32
+        self.pos = _dev(p.pos, f64, dev)
+        self.nvec = _dev(p.nvec, f64, dev)
+        self.area = _dev(p.area, f64, dev)
+        self._tv1 = _dev(p.tvec1, f64, dev)
+        self._tv2 = _dev(p.tvec2, f64, dev)
+        self._nodes = _dev(p.nodes, f64, dev)
+        self.prow = _dev(row, i32, dev)
+        self.pcol = _dev(col, i32, dev)
+        self._pkind = _dev(kind, i32, dev)
+        # per side: 0-based material index of each face
+        self.mat = [_dev(p.inout_faces[:, s] - 1, i32, dev) for s in (0, 1)]
+        self.nmat = len(p.eps)
+        qn = _dev(kernel_table(*rule18()), f64, dev)
+        qs = _dev(kernel_table(*polar_rule((7, 5))), f64, dev)
+        o1 = self.order + 1
+        self.g = torch.empty((self.npairs, o1), dtype=f64, device=dev)
+        self.f = torch.empty((self.npairs, 3, o1), dtype=f64, device=dev)
+        self.fstat = torch.empty((self.npairs,), dtype=f64, device=dev)
+        st = nat.stream_handle(dev)
+        nat.call("mnpbem_refine_coeffs", self.order, self.npairs, nat.ptr(self.prow), nat.ptr(self.pcol),
+                 nat.ptr(self._pkind), nat.ptr(self.pos), nat.ptr(self.nvec), nat.ptr(self._tv1),
+                 nat.ptr(self._tv2), nat.ptr(self.pos), nat.ptr(self._nodes), nat.ptr(qn), qn.shape[0],
+                 nat.ptr(qs), qs.shape[0], nat.ptr(self.g), nat.ptr(self.f), nat.ptr(self.fstat), st)
+        # closed-surface correction (each particle closed, outward normals)
+        oc = np.lexsort((row, col))
+        crow, cpair = row[oc], oc
+        colptr = np.concatenate([[0], np.cumsum(np.bincount(col, minlength=n))])
+        surf = p.surface
+        lo = p.offsets[surf]
+        hi = p.offsets[surf + 1]
+        self.corr = torch.empty((n,), dtype=f64, device=dev)
+        keep = [_dev(a, i32, dev) for a in (lo, hi, colptr, crow, cpair)]
+        nat.call("mnpbem_closed_correction", n, nat.ptr(self.pos), nat.ptr(self.nvec), nat.ptr(self.area),
+                 *[nat.ptr(t) for t in keep], nat.ptr(self.fstat), 1.0, self.order, nat.ptr(self.f),
+                 nat.ptr(self.corr), st)
+        torch.cuda.current_stream(dev).synchronize()
+        self.con = self._connect()
+        _GREEN_CACHE[id(p)] = self
+
+    def _connect(self):
+        """Connectivity of regions (compgreen_ret.py:435-505): con[i][j][a, b]
+        = material index when side i of particle a and side j of particle b
+        hold the same material, else 0."""
+        io = self.p1.inout
+        return [[np.where(io[:, i][:, None] == io[:, j][None, :], io[:, i][:, None], 0)
+                 for j in range(2)] for i in range(2)]
+
+    def wavenumbers(self, enei_list):
+        """(B, nmat) complex wavenumbers as a (B, nmat, 2) float64 device tensor."""
+        k = np.array([[complex(e(lam)[1]) for e in self.p1.eps] for lam in enei_list])
+        return _dev(np.stack([k.real, k.imag], axis=-1), torch.float64, self.device)
+
+    def assemble(self, side, kmat, G, H, hsign, dtype, G32=None, H32=None):
+        """Fill transposed G, H buffers (B, n, n) of one side for a batch of
+        wavelengths (kmat from ``wavenumbers``)."""
+        B = kmat.shape[0]
+        n = self.n
+        dual = 1 if G32 is not None else 0
+        nat.call("mnpbem_green_assemble", dtype, dual, n, nat.ptr(self.pos), nat.ptr(self.nvec),
+                 nat.ptr(self.area), nat.ptr(self.mat[side]), nat.ptr(kmat), self.nmat, self.order,
+                 self.npairs, nat.ptr(self.prow), nat.ptr(self.pcol), nat.ptr(self.g), nat.ptr(self.f),
+                 float(hsign), nat.ptr(G), nat.ptr(H), n, n * n, nat.ptr(G32), nat.ptr(H32), B,
+                 nat.stream_handle(self.device))
+
+    def eval(self, i, j, key, enei, ind=None):
+        """Green function of regions (i, j) (0 = inside, 1 = outside) at
+        wavelength ``enei``: an (n, n) complex128 device tensor, or the int 0
+        when no particle connects the two regions (compgreen_ret.py:507-553)."""
+        if ind is not None:
+            raise NotImplementedError("CompGreenRet.eval: element selection is not supported")
+        if key not in ("G", "F", "H1", "H2"):
+            raise ValueError("Unknown key: %s" % key)
+        con = self.con[i][j]
+        if not np.any(con):
+            return 0
+        if i != j:
+            raise NotImplementedError("CompGreenRet.eval: cross-region blocks (nested particles)")
+        n = self.n
+        kmat = self.wavenumbers([enei])
+        G = torch.empty((1, n, n), dtype=torch.complex128, device=self.device)
+        H = torch.empty_like(G)
+        hs = {"G": 0.0, "F": 0.0, "H1": 1.0, "H2": -1.0}[key]
+        self.assemble(i, kmat, G, H, hs, nat.DTYPE_C128)
+        M = G if key == "G" else H
+        return M[0].transpose(0, 1)
+
+
+_GREEN_CACHE = {}
+
+
+def green_for(p):
+    """The most recent CompGreenRet built for particle ``p`` (built on demand),
+    so that PlaneWaveRet shares the device geometry of the BEM solver."""
+    g = _GREEN_CACHE.get(id(p))
+    if g is None or g.p1 is not p:
+        g = CompGreenRet(p, p)
+    return g
