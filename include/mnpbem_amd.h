@@ -1,0 +1,161 @@
+/* C ABI of libmnpbem_amd.so: MNPBEM's retarded BEM hot path on MI355X (gfx950).
+ *
+ * The reference (rberwickt/MNPBEM) has no native ABI for this path: its
+ * surface is the Python API (ComParticle, CompGreenRet.eval, BEMRet.init /
+ * solve, PlaneWaveRet.potential / extinction).  These entry points are the
+ * device-side operations that API is built from; each comment names the
+ * reference code it replaces (paths relative to the reference root).
+ *
+ * Conventions
+ *   - all pointers are device pointers unless stated; sizes are counts of
+ *     elements; complex arrays are interleaved (re, im) like numpy/torch
+ *     complex64 / complex128;
+ *   - dtype: 1 = complex64, 2 = complex128;
+ *   - matrices are row-major with leading dimension ld (complex elements);
+ *     "batched" calls process `batch` matrices spaced `stride` elements
+ *     apart; pivot arrays are int32, 0-based, spaced `spiv` apart;
+ *   - every call is asynchronous on `stream` (a hipStream_t, NULL = default)
+ *     and returns 0 on success, 1 for invalid arguments, 2 for a HIP error;
+ *     mnpbem_last_error() returns the message of the last failure on the
+ *     calling thread;
+ *   - nothing here allocates device memory except mnpbem_prof_begin (HIP
+ *     events); calls on one stream are ordered, calls that share output
+ *     buffers on different streams race.
+ */
+#ifndef MNPBEM_AMD_H
+#define MNPBEM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MNPBEM_C64 1
+#define MNPBEM_C128 2
+
+int mnpbem_abi_version(void);
+const char* mnpbem_last_error(void);
+int mnpbem_device_sync(void);
+
+/* ---- dense complex linear algebra (replaces scipy.linalg.lu_factor /
+ *      lu_solve, i.e. LAPACK zgetrf / zgetrs, as dispatched by
+ *      mnpbem/utils/gpu.py:145-279 from mnpbem/bem/bem_ret.py:437-523) ---- */
+
+/* C[b] -= A[b] B[b]  (M x K times K x N), MFMA complex GEMM. */
+int mnpbem_cgemm_sub_batched(int dtype, int M, int N, int K, const void* A, long lda, long sA, const void* B,
+                             long ldb, long sB, void* C, long ldc, long sC, int batch, void* stream);
+
+/* In-place LU with partial pivoting, P A = L U (largest |re|+|im| pivot,
+ * LAPACK convention); info[b] = 1-based column of the first exactly-zero
+ * pivot or 0.  Replaces lu_factor_dispatch (mnpbem/utils/gpu.py:145-191). */
+int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA, int* ipiv, long spiv, int* info, int batch,
+                         void* stream);
+
+/* B <- A^-1 B for A factored by mnpbem_getrf_batched, nrhs columns.
+ * Replaces lu_solve_dispatch / lu_solve_native (mnpbem/utils/gpu.py:194-279). */
+int mnpbem_getrs_batched(int dtype, int n, int nrhs, const void* A, long lda, long sA, const int* ipiv, long spiv,
+                         void* B, long ldb, long sB, int batch, void* stream);
+
+/* Y[b] = beta Y[b] + alpha M[b]^T X[b] for n x C blocks X, Y (any C). */
+int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, long ldm, long sM, const void* X, long ldx,
+                          long sX, void* Y, long ldy, long sY, double alpha_re, double alpha_im, double beta,
+                          int batch, void* stream);
+
+/* Solve M x = v in place for the n x C block V given the LU of M^T
+ * (the transposed-storage convention of BEMRet's factors). */
+int mnpbem_lu_solve_t_batched(int dtype, int n, int C, const void* LU, long ld, long sLU, const int* ipiv,
+                              long spiv, void* V, long ldv, long sV, int batch, void* stream);
+
+int mnpbem_identity_batched(int dtype, int n, void* A, long ld, long sA, int batch, void* stream);
+
+/* ---- Green functions (replaces mnpbem/greenfun/greenret_refined.py:78-867,
+ *      compgreen_ret.py:192-505 and compgreen_stat.py:219-465) ---- */
+
+/* Order-`order` (<= 7) expansion coefficients of refined face pairs
+ * (deriv='cart'): for pair p = (row i, col j, kind 2 = self, 1 = near)
+ *   g[p][n], f[p][c][n] (c = x,y,z) and the quasistatic normal derivative
+ *   fstat[p] = -sum_q w_q n_i.(r_i - x_q)/|r_i - x_q|^3.
+ * Field faces: pos1, nvec1, tvec1, tvec2 (n1 x 3 doubles); source faces:
+ * pos2 (n2 x 3) and nodes2 (n2 x 6 x 3, quadratic triangles); qnear / qself:
+ * host-built tables of nq rows [shape(6), d/dx(6), d/dy(6), weight] for the
+ * 28-point rule and the polar rule.  (greenret_refined.py:122-516) */
+int mnpbem_refine_coeffs(int order, int npairs, const int* prow, const int* pcol, const int* pkind,
+                         const double* pos1, const double* nvec1, const double* tvec1, const double* tvec2,
+                         const double* pos2, const double* nodes2, const double* qnear, int nq_near,
+                         const double* qself, int nq_self, double* g, double* f, double* fstat, void* stream);
+
+/* Fuchs-Liu closed-surface correction: corr_j = -2 pi dir - sum_i a_i F_ij / a_j
+ * over rows lo[j] <= i < hi[j] (j's closed surface), with the refined pairs of
+ * column j listed (sorted by row) in crow[colptr[j]..colptr[j+1]) / cpair;
+ * adds corr_j n_j to f[self pair of j][:][0].  (compgreen_ret.py:192-400) */
+int mnpbem_closed_correction(int n, const double* pos, const double* nvec, const double* area, const int* lo,
+                             const int* hi, const int* colptr, const int* crow, const int* cpair,
+                             const double* fstat, double dir, int order, double* f, double* corr, void* stream);
+
+/* G and H = F + hsign 2 pi I of one side (inside: mat = inside material,
+ * hsign = +1; outside: -1) for `batch` wavelengths with wavenumbers
+ * kmat[b][m] (complex, doubles), written transposed: entry (i, j) at
+ * out[b*stride + j*ld + i].  dual = 1 (dtype complex128) also writes a
+ * complex64 copy to G32/H32.  Replaces CompGreenRet.eval(i, i, 'G'/'H1'/'H2')
+ * (compgreen_ret.py:507-858, greenret_refined.py:654-810). */
+int mnpbem_green_assemble(int dtype, int dual, int n, const double* pos, const double* nvec, const double* area,
+                          const int* mat, const double* kmat, int nmat, int order, int npairs, const int* prow,
+                          const int* pcol, const double* g, const double* f, double hsign, void* G, void* H,
+                          long ld, long stride, void* G32, void* H32, int batch, void* stream);
+
+/* ---- BEM solver (replaces mnpbem/bem/bem_ret.py:293-525 and 1360-1831) ----
+ * Vectors per wavelength are n x 8P row-major (P polarisations):
+ *   unknowns X = [sig1 | h1 (3P) | sig2 | h2 (3P)],
+ *   right side R = [phi | a (3P) | De | alpha (3P)]  (bem_ret_iter.py:1196-1232).
+ * par[b] = {Re eps1, Im eps1, Re eps2, Im eps2, k0} per wavelength. */
+
+/* mode 0: D = S1 - S2 (Delta^T); mode 1: S2 <- eps1 S1 - eps2 S2
+ * + k0^2 (eps1 - eps2)^2 E o (n n^T)  (Sigma^T, bem_ret.py:476-495). */
+int mnpbem_sigma_batched(int dtype, int mode, int n, const void* S1, void* S2, void* D, const void* E, long ld,
+                         long stride, const double* nvec, const double* par, int batch, void* stream);
+
+/* X = A^-1 R with the factors of BEMRet.init: LU(G1^T), LU(G2^T), Sigma1^T,
+ * (Delta^-1)^T, LU(Sigma^T) (Eqs. 19-20 of Garcia de Abajo & Howie,
+ * bem_ret.py:1694-1810).  W is work space of R's shape; P <= 4. */
+int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t, const int* piv1, const void* G2t,
+                             const int* piv2, const void* S1t, const void* Et, const void* St, const int* pivS,
+                             long ldm, long smat, long spiv, const void* R, void* W, void* X, long ldv, long sv,
+                             const double* nvec, const double* par, int batch, void* stream);
+
+/* Rout = R - A X for the complex128 8N operator (bem_ret_iter.py:1362-1501)
+ * with unfactored transposed G1, H1, G2, H2; Y is n x 16P work space;
+ * nrm[b] = {||R - A X||^2, ||R||^2}. */
+int mnpbem_bem_residual_batched(int n, int P, const void* G1t, const void* H1t, const void* G2t, const void* H2t,
+                                long ldm, long smat, const void* R, const void* X, void* Rout, long ldv, long sv,
+                                void* Y, long ldy, long sy, const double* nvec, const double* par, double* nrm,
+                                int batch, void* stream);
+
+/* dst (double) += src (complex64 or complex128 as reals); dst = (dtype) src. */
+int mnpbem_accumulate(int src_dtype, long count, const void* src, double* dst, void* stream);
+int mnpbem_convert(int dst_dtype, long count, const double* src, void* dst, void* stream);
+
+/* ---- plane-wave excitation and extinction (replaces
+ *      mnpbem/simulation/planewave_ret.py:257-504, bem_ret.py:1360-1429 and
+ *      mnpbem/spectrum/spectrum_ret.py:231-392) ----
+ * pw[b] = {k0, Re k, Im k, Re eps1, Im eps1, Re eps2, Im eps2}; pol, dir:
+ * P x 3 doubles; in_med / out_med: per face, 1 when the inside / outside of
+ * the face is the embedding medium.  pot (optional, NULL to skip) receives
+ * [a1 | a1' | a2 | a2'] as n x 12P per wavelength. */
+int mnpbem_planewave_batched(int n, int P, const double* pos, const double* nvec, const int* in_med,
+                             const int* out_med, const double* pol, const double* dir, const double* pw, double* R,
+                             long ldv, long sv, double* pot, long spot, int batch, void* stream);
+
+/* ext[b][p] = 4 pi / k Im(pol_p . e_p(dir_p)) from the solution X (complex128). */
+int mnpbem_extinction_batched(int n, int P, const double* X, long ldv, long sv, const double* pos,
+                              const double* area, const int* in_med, const int* out_med, const double* pol,
+                              const double* dir, const double* pw, double* ext, int batch, void* stream);
+
+/* ---- measurement: HIP-event timing of every complex-GEMM launch ----
+ * (host pointers for the results of mnpbem_prof_end) */
+int mnpbem_prof_begin(int capacity);
+int mnpbem_prof_end(double* total_ms, double* total_flops, long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MNPBEM_AMD_H */

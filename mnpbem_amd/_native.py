@@ -41,6 +41,8 @@ SIGNATURES = {
     "mnpbem_convert": [_i, _l, _p, _p, _p],
     "mnpbem_planewave_batched": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _p, _l, _i, _p],
     "mnpbem_extinction_batched": [_i, _i, _p, _l, _l, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p],
+    "mnpbem_prof_begin": [_i],
+    "mnpbem_prof_end": [_p, _p, _p],
 }
 
 _lib = None

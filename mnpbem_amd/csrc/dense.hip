@@ -11,12 +11,16 @@
 // 1-based column is then reported in ``info``).
 #include "common.h"
 
+#include <vector>
+
 namespace {
 
 // ---------------------------------------------------------------------------
 // MFMA traits: 16x16x4 single-block matrix instructions.
-// Operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four
-// accumulator values of lane l are D[4*(l/16) + r][l%16], r = 0..3.
+// Operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16].  The four
+// accumulator values of lane l (r = 0..3) sit in column l%16 and row
+//   f32: 4*(l/16) + r        f64: l/16 + 4*r
+// (the f64 row interleave was measured on gfx950 with a lane-index probe).
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Mfma;
@@ -26,6 +30,7 @@ struct Mfma<double> {
   static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -33,6 +38,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 // ---------------------------------------------------------------------------
@@ -134,7 +140,7 @@ __global__ __launch_bounds__(256) void k_cgemm_sub(int M, int N, int K, const T*
       const int gc = n0 + wn + 16 * j + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gr = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int gr = m0 + wm + 16 * i + Mfma<T>::row(lane, r);
         if (gr < M && gc < N) {
           T* p = C + 2 * ((long)gr * ldc + gc);
           cplx<T> c = cld(p);
@@ -366,13 +372,31 @@ struct Bat {
   T* at(long r, long c) const { return A + 2 * (r * lda + c); }
 };
 
+// Optional timing of every complex-GEMM launch with a pair of HIP events on
+// the launching stream (enabled by mnpbem_prof_begin), for the roofline of
+// the dominant kernel: algorithmic flops 8*M*N*K*batch per launch.
+struct GemmProf {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<double> flops;
+  size_t used = 0;
+};
+GemmProf g_prof;
+
 template <typename T>
 int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, long ldb, long sB, T* C, long ldc,
              long sC, int batch, hipStream_t st) {
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return 0;
   dim3 grid(mnp_ceil_div(N, GT), mnp_ceil_div(M, GT), batch);
+  const bool timed = g_prof.on && g_prof.used + 2 <= g_prof.ev.size();
+  if (timed) MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used], st));
   hipLaunchKernelGGL(k_cgemm_sub<T>, grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
   MNP_LAUNCH_CHECK();
+  if (timed) {
+    MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used + 1], st));
+    g_prof.used += 2;
+    g_prof.flops.push_back(8.0 * M * (double)N * K * batch);
+  }
   return 0;
 }
 
@@ -552,4 +576,38 @@ extern "C" int mnpbem_getrs_batched(int dtype, int n, int nrhs, const void* A, l
   }
   mnp_set_error("getrs: unknown dtype");
   return 1;
+}
+
+// Start timing complex-GEMM launches (up to ``capacity`` launches).
+extern "C" int mnpbem_prof_begin(int capacity) {
+  MNP_REQUIRE(capacity > 0, "prof_begin: capacity must be positive");
+  if ((int)g_prof.ev.size() < 2 * capacity) {
+    for (size_t i = g_prof.ev.size(); i < (size_t)2 * capacity; ++i) {
+      hipEvent_t e;
+      MNP_HIP_CHECK(hipEventCreate(&e));
+      g_prof.ev.push_back(e);
+    }
+  }
+  g_prof.used = 0;
+  g_prof.flops.clear();
+  g_prof.on = true;
+  return 0;
+}
+
+// Stop timing; after the stream work has completed, report the summed
+// kernel time (ms), summed algorithmic flops and the number of launches.
+extern "C" int mnpbem_prof_end(double* total_ms, double* total_flops, long* launches) {
+  g_prof.on = false;
+  double ms = 0.0, fl = 0.0;
+  for (size_t i = 0; i + 1 < g_prof.used; i += 2) {
+    MNP_HIP_CHECK(hipEventSynchronize(g_prof.ev[i + 1]));
+    float t = 0.f;
+    MNP_HIP_CHECK(hipEventElapsedTime(&t, g_prof.ev[i], g_prof.ev[i + 1]));
+    ms += t;
+  }
+  for (double f : g_prof.flops) fl += f;
+  *total_ms = ms;
+  *total_flops = fl;
+  *launches = (long)(g_prof.used / 2);
+  return 0;
 }

@@ -1,0 +1,52 @@
+"""CPU-side checks of the C ABI: the built library loads and exports every
+function declared in include/mnpbem_amd.h, and the ctypes table covers them.
+No device compute is called here."""
+
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mnpbem_amd.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(mnpbem_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "mnpbem_getrf_batched" in names and "mnpbem_bem_solve_batched" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    from mnpbem_amd import _native
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("libmnpbem_amd.so not built")
+    lib = _native.load()
+    for name in declared():
+        assert hasattr(lib, name), name
+
+
+def test_ctypes_table_matches_header():
+    from mnpbem_amd import _native
+    names = set(declared()) - {"mnpbem_last_error"}
+    assert names == set(_native.SIGNATURES)
+
+
+def test_invalid_arguments_are_rejected_without_device_work():
+    """Argument validation happens on the host before any launch."""
+    from mnpbem_amd import _native
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("libmnpbem_amd.so not built")
+    lib = _native.load()
+    rc = lib.mnpbem_getrf_batched(_native.DTYPE_C128, 10, None, 5, 100, None, 10, None, 1, None)
+    assert rc == 1
+    assert b"getrf" in lib.mnpbem_last_error()
+    rc = lib.mnpbem_refine_coeffs(9, 0, *([None] * 10), 28, None, 144, None, None, None, None)
+    assert rc == 1
+    with pytest.raises(RuntimeError, match="order"):
+        _native.call("mnpbem_refine_coeffs", 9, 0, *([None] * 10), 28, None, 144, None, None, None, None)

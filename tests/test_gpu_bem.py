@@ -40,8 +40,13 @@ def test_refinement_coefficients_match_oracle(setup284):
     assert np.array_equal(g.row, row) and np.array_equal(g.col, col)
     gg = g.g.cpu().numpy()
     ff = g.f.cpu().numpy()
-    assert np.abs(gg - gr).max() < 1e-12 * np.abs(gr).max()
-    assert np.abs(ff - fr).max() < 1e-11 * np.abs(fr).max()
+    # The polar rule puts nodes at 5e-7 of the element size from the centroid,
+    # where r = |centroid - node| loses ~9 digits to cancellation; summation
+    # order and FMA then move the self-term sums at the 1e-11 level.
+    assert np.abs(gg - gr).max() < 1e-10 * np.abs(gr).max()
+    assert np.abs(ff - fr).max() < 1e-10 * np.abs(fr).max()
+    near = g.kind == 1
+    assert np.abs(gg[near] - gr[near]).max() < 1e-13 * np.abs(gr[near]).max()
 
 
 @pytest.mark.parametrize("lam", [400.0, 520.0, 800.0])
@@ -54,7 +59,7 @@ def test_green_matrices_match_oracle(setup284, lam):
     for (i, key), M in zip([(0, "G"), (0, "H1"), (1, "G"), (1, "H2")], (G1, H1, G2, H2)):
         out = g.eval(i, i, key, lam).cpu().numpy()
         err = np.abs(out - M).max() / np.abs(M).max()
-        assert err < 1e-12, (key, i, err)
+        assert err < 1e-10, (key, i, err)  # self terms: see the coefficient test
     assert g.eval(0, 1, "G", lam) == 0
 
 

@@ -45,7 +45,7 @@ def test_getrf_getrs(cuda, dtype, tol, n):
     nat = _lib()
     rng = np.random.default_rng(n)
     B = 2
-    A = _rand(rng, (B, n, n), dtype) + (2.0 * np.sqrt(n)) * np.eye(n, dtype=dtype)[None]
+    A = (_rand(rng, (B, n, n), dtype) + (2.0 * np.sqrt(n)) * np.eye(n)[None]).astype(dtype)
     X = _rand(rng, (B, n, n + 3), dtype)
     ref = np.linalg.solve(A.astype(np.complex128), X.astype(np.complex128))
     tA, tX = torch.from_numpy(A.copy()).to(cuda), torch.from_numpy(X.copy()).to(cuda)
