@@ -61,9 +61,11 @@ int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, long ldm, long
                           int batch, void* stream);
 
 /* Solve M x = v in place for the n x C block V given the LU of M^T
- * (the transposed-storage convention of BEMRet's factors). */
+ * (the transposed-storage convention of BEMRet's factors); Z (n x C,
+ * leading dimension ldz >= C) is work space. */
 int mnpbem_lu_solve_t_batched(int dtype, int n, int C, const void* LU, long ld, long sLU, const int* ipiv,
-                              long spiv, void* V, long ldv, long sV, int batch, void* stream);
+                              long spiv, void* V, long ldv, long sV, void* Z, long ldz, long sZ, int batch,
+                              void* stream);
 
 int mnpbem_identity_batched(int dtype, int n, void* A, long ld, long sA, int batch, void* stream);
 

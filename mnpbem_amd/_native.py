@@ -30,7 +30,7 @@ SIGNATURES = {
     "mnpbem_green_assemble": [_i, _i, _i, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _d, _p, _p, _l, _l,
                               _p, _p, _i, _p],
     "mnpbem_gemv_t_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _l, _p, _l, _l, _d, _d, _d, _i, _p],
-    "mnpbem_lu_solve_t_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _i, _p],
+    "mnpbem_lu_solve_t_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _p, _l, _l, _i, _p],
     "mnpbem_identity_batched": [_i, _i, _p, _l, _l, _i, _p],
     "mnpbem_sigma_batched": [_i, _i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _i, _p],
     "mnpbem_bem_solve_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l, _l,

@@ -313,46 +313,62 @@ __global__ __launch_bounds__(256) void k_laswp(int k0, int k1, int clo, int chi,
 }
 
 // ---------------------------------------------------------------------------
-// Small triangular solve in LDS: X = T^-1 B for an n x n (n <= 64) block T,
-// lower with unit diagonal (LOWER) or upper with its diagonal, B n x nrhs.
-// Workgroup: 256 threads = 64 right-hand-side columns x 4 row groups.
+// Small triangular solve X = T^-1 B for an n x n (n <= TLN) block T: lower
+// with unit diagonal (LOWER) or upper with its diagonal.  T sits in LDS
+// (zero-padded to TLN, so no bounds tests in the loops); every thread owns
+// one right-hand-side column in registers and runs the substitution without
+// further synchronisation.
 // ---------------------------------------------------------------------------
-constexpr int TL = 64;
+template <typename T>
+struct TLeaf;
+template <>
+struct TLeaf<double> {
+  static constexpr int n = 32;
+};
+template <>
+struct TLeaf<float> {
+  static constexpr int n = 64;
+};
 
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsm_leaf(int n, int nrhs, const T* Tm, long ldt, long sT, T* B,
                                                    long ldb, long sB) {
+  constexpr int TLN = TLeaf<T>::n;
   const long b = blockIdx.y;
   Tm += 2 * b * sT;
   B += 2 * b * sB;
-  __shared__ cplx<T> Ts[TL][TL + 1];
-  __shared__ cplx<T> Xs[TL][TL + 1];
-  const int tid = threadIdx.x, cl = tid & 63, rg = tid >> 6;
-  const int c0 = blockIdx.x * TL;
-  for (int e = tid; e < n * n; e += 256) {
-    const int i = e / n, j = e % n;
-    Ts[i][j] = cld(Tm + 2 * ((long)i * ldt + j));
+  // The substitution always runs in upper-triangular form; a lower system is
+  // mapped onto it by reversing the row and column order (index n-1-i).
+  __shared__ cplx<T> Ts[TLN][TLN];
+  __shared__ cplx<T> rd[TLN];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < TLN * TLN; e += 256) {
+    const int i = e / TLN, j = e % TLN;
+    const int si = LOWER ? n - 1 - i : i, sj = LOWER ? n - 1 - j : j;
+    Ts[i][j] = (i < n && j < n) ? cld(Tm + 2 * ((long)si * ldt + sj)) : cmake<T>(0, 0);
   }
-  const int gc = c0 + cl;
-  for (int i = rg; i < n; i += 4) Xs[i][cl] = (gc < nrhs) ? cld(B + 2 * ((long)i * ldb + gc)) : cmake<T>(0, 0);
   __syncthreads();
-  if (LOWER) {
-    for (int i = 0; i < n; ++i) {
-      const cplx<T> xi = Xs[i][cl];
-      for (int l = i + 1 + rg; l < n; l += 4) Xs[l][cl] = cfms(Xs[l][cl], Ts[l][i], xi);
-      __syncthreads();
-    }
-  } else {
-    for (int i = n - 1; i >= 0; --i) {
-      if (rg == 0) Xs[i][cl] = cdiv(Xs[i][cl], Ts[i][i]);
-      __syncthreads();
-      const cplx<T> xi = Xs[i][cl];
-      for (int l = rg; l < i; l += 4) Xs[l][cl] = cfms(Xs[l][cl], Ts[l][i], xi);
-      __syncthreads();
-    }
+  if (tid < TLN) rd[tid] = (tid >= n) ? cmake<T>(0, 0) : (LOWER ? cmake<T>(1, 0) : crecip(Ts[tid][tid]));
+  __syncthreads();
+  const int gc = blockIdx.x * 256 + tid;
+  if (gc >= nrhs) return;
+  cplx<T> x[TLN];
+#pragma unroll
+  for (int i = 0; i < TLN; ++i) {
+    const int si = LOWER ? n - 1 - i : i;
+    x[i] = (i < n) ? cld(B + 2 * ((long)si * ldb + gc)) : cmake<T>(0, 0);
   }
-  if (gc < nrhs)
-    for (int i = rg; i < n; i += 4) cst(B + 2 * ((long)i * ldb + gc), Xs[i][cl]);
+#pragma unroll
+  for (int i = TLN - 1; i >= 0; --i) {
+#pragma unroll
+    for (int l = i + 1; l < TLN; ++l) x[i] = cfms(x[i], Ts[i][l], x[l]);
+    x[i] = cmul(x[i], rd[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < TLN; ++i) {
+    const int si = LOWER ? n - 1 - i : i;
+    if (i < n) cst(B + 2 * ((long)si * ldb + gc), x[i]);
+  }
 }
 
 __global__ void k_fill_int(int* p, long n, int v) {
@@ -405,8 +421,9 @@ template <typename T>
 int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, long ldx, long sX, int batch,
              hipStream_t st) {
   if (n <= 0 || nrhs <= 0) return 0;
+  constexpr int TL = TLeaf<T>::n;
   if (n <= TL) {
-    dim3 grid(mnp_ceil_div(nrhs, TL), batch);
+    dim3 grid(mnp_ceil_div(nrhs, 256), batch);
     if (lower)
       hipLaunchKernelGGL((k_trsm_leaf<T, true>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx, sX);
     else

@@ -26,7 +26,9 @@ namespace {
 
 constexpr int MAXC = 16;  // right-hand-side columns handled per pass
 
-// Y[b] = beta*Y[b] + alpha * M[b]^T X[b]  (M n x n buffer, X, Y n x C)
+// Y[b] = beta*Y[b] + alpha * M[b]^T X[b]  (M n x n buffer, X, Y n x C).
+// One thread per output row i (a column of the buffer), so every load of M
+// is a coalesced row segment; X rows are staged through LDS.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restrict__ M, long ldm, long sM,
                                                 const T* __restrict__ X, long ldx, long sX, T* __restrict__ Y,
@@ -35,10 +37,9 @@ __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restric
   M += 2 * b * sM;
   X += 2 * b * sX;
   Y += 2 * b * sY;
-  const int tid = threadIdx.x, il = tid & 63, jg = tid >> 6;
-  const int i = blockIdx.x * 64 + il;
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * 256 + tid;
   __shared__ cplx<T> xs[64][MAXC];
-  __shared__ cplx<T> red[3][64][MAXC];
   cplx<T> acc[MAXC];
 #pragma unroll
   for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
@@ -51,7 +52,8 @@ __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restric
     }
     __syncthreads();
     if (i < n) {
-      for (int jj = jg; jj < nj; jj += 4) {
+#pragma unroll 4
+      for (int jj = 0; jj < nj; ++jj) {
         const cplx<T> m = cld(M + 2 * ((long)(j0 + jj) * ldm + i));
 #pragma unroll
         for (int c = 0; c < MAXC; ++c)
@@ -59,19 +61,12 @@ __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restric
       }
     }
   }
-  if (jg > 0) {
-#pragma unroll
-    for (int c = 0; c < MAXC; ++c)
-      if (c < C) red[jg - 1][il][c] = acc[c];
-  }
-  __syncthreads();
-  if (jg == 0 && i < n) {
+  if (i < n) {
 #pragma unroll
     for (int c = 0; c < MAXC; ++c)
       if (c < C) {
-        cplx<T> s = cadd(cadd(acc[c], red[0][il][c]), cadd(red[1][il][c], red[2][il][c]));
         T* py = Y + 2 * ((long)i * ldy + c);
-        const double sr = (double)s.re, si = (double)s.im;
+        const double sr = (double)acc[c].re, si = (double)acc[c].im;
         double yr = alr * sr - ali * si, yi = alr * si + ali * sr;
         if (beta != 0.0) {
           const cplx<T> y0 = cld(py);
@@ -83,131 +78,113 @@ __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restric
   }
 }
 
-// Solve M x = v in place for the n x C block V, given the LU of M^T in the
-// buffer (P M^T = L U):  x = P^T L^-T U^-T v.  One workgroup per matrix,
-// 64-row blocks: a transposed GEMV over the finished rows, then the 64 x 64
-// triangle in LDS by one wavefront.
-template <typename T>
-__global__ __launch_bounds__(256) void k_lu_solve_t(int n, int C, const T* __restrict__ LU, long ld, long sLU,
-                                                    const int* __restrict__ ipiv, long sP, T* V, long ldv,
-                                                    long sV) {
-  const long b = blockIdx.x;
+// One 64-row block step of the solve M x = v with the LU of M^T in the
+// buffer (P M^T = L U, so x = P^T L^-T U^-T v).
+//   FWD (U^T z = v, U^T lower non-unit): the block rhs is in V (already
+//     reduced by earlier blocks); z_I -> Z; V rows below the block -= U_IK^T z_I.
+//   BWD (L^T w = z, unit upper): the block rhs is in Z; w_I -> V; Z rows above
+//     the block -= L_IK^T w_I.
+// Every workgroup solves the 64 x 64 triangle (one wavefront, shuffles) and
+// updates its own 256-row tile; workgroup 0 stores the block solution.
+template <typename T, bool FWD>
+__global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const T* __restrict__ LU, long ld,
+                                                     long sLU, T* V, long ldv, long sV, T* Z, long ldz, long sZ) {
+  const long b = blockIdx.y;
   LU += 2 * b * sLU;
-  ipiv += b * sP;
   V += 2 * b * sV;
-  const int tid = threadIdx.x, il = tid & 63, lg = tid >> 6;
+  Z += 2 * b * sZ;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int nb = min(64, n - i0), i1 = i0 + nb;
   __shared__ cplx<T> blk[64][65];
-  __shared__ cplx<T> red[3][64][MAXC];
-  __shared__ int sp[256];
-  const int nblk = (n + 63) / 64;
-  // forward: U^T z = v  (U^T lower, non-unit)
-  for (int ib = 0; ib < nblk; ++ib) {
-    const int i0 = ib * 64, nb = min(64, n - i0);
-    cplx<T> acc[MAXC];
+  __shared__ cplx<T> sol[64][MAXC];
+  for (int e = tid; e < nb * nb; e += 256) {
+    const int r = e / nb, q = e % nb;
+    blk[r][q] = cld(LU + 2 * ((long)(i0 + r) * ld + i0 + q));
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const T* src = FWD ? V : Z;
+    const long lds = FWD ? ldv : ldz;
+    cplx<T> val[MAXC];
 #pragma unroll
-    for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
-    if (il < nb) {
-      for (int l = lg; l < i0; l += 4) {
-        const cplx<T> u = cld(LU + 2 * ((long)l * ld + i0 + il));
-#pragma unroll
-        for (int c = 0; c < MAXC; ++c)
-          if (c < C) acc[c] = cfma(acc[c], u, cld(V + 2 * ((long)l * ldv + c)));
-      }
-    }
-    if (lg > 0)
-#pragma unroll
-      for (int c = 0; c < MAXC; ++c)
-        if (c < C) red[lg - 1][il][c] = acc[c];
-    for (int e = tid; e < nb * nb; e += 256) {
-      const int r = e / nb, q = e % nb;
-      blk[r][q] = cld(LU + 2 * ((long)(i0 + r) * ld + i0 + q));
-    }
-    __syncthreads();
-    if (lg == 0) {
-      cplx<T> val[MAXC];
-#pragma unroll
-      for (int c = 0; c < MAXC; ++c)
-        if (c < C && il < nb) {
-          val[c] = csub(cld(V + 2 * ((long)(i0 + il) * ldv + c)),
-                        cadd(cadd(acc[c], red[0][il][c]), cadd(red[1][il][c], red[2][il][c])));
-        } else {
-          val[c] = cmake<T>(0, 0);
-        }
+    for (int c = 0; c < MAXC; ++c)
+      val[c] = (c < C && lane < nb) ? cld(src + 2 * ((long)(i0 + lane) * lds + c)) : cmake<T>(0, 0);
+    if (FWD) {
       for (int i = 0; i < nb; ++i) {
-        if (il == i) {
+        if (lane == i) {
           const cplx<T> inv = crecip(blk[i][i]);
 #pragma unroll
           for (int c = 0; c < MAXC; ++c) val[c] = cmul(val[c], inv);
         }
-        const cplx<T> uir = blk[i][il];  // U[i][r] = (U^T)[r][i]
+        const cplx<T> u = blk[i][lane];  // U[i][r] = (U^T)[r][i]
 #pragma unroll
         for (int c = 0; c < MAXC; ++c)
           if (c < C) {
             cplx<T> zi;
             zi.re = __shfl(val[c].re, i);
             zi.im = __shfl(val[c].im, i);
-            if (il > i && il < nb) val[c] = cfms(val[c], uir, zi);
+            if (lane > i && lane < nb) val[c] = cfms(val[c], u, zi);
           }
       }
-      if (il < nb)
-#pragma unroll
-        for (int c = 0; c < MAXC; ++c)
-          if (c < C) cst(V + 2 * ((long)(i0 + il) * ldv + c), val[c]);
-    }
-    __syncthreads();
-  }
-  // backward: L^T w = z  (L^T upper, unit)
-  for (int ib = nblk - 1; ib >= 0; --ib) {
-    const int i0 = ib * 64, nb = min(64, n - i0), i1 = i0 + nb;
-    cplx<T> acc[MAXC];
-#pragma unroll
-    for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
-    if (il < nb) {
-      for (int l = i1 + lg; l < n; l += 4) {
-        const cplx<T> u = cld(LU + 2 * ((long)l * ld + i0 + il));
-#pragma unroll
-        for (int c = 0; c < MAXC; ++c)
-          if (c < C) acc[c] = cfma(acc[c], u, cld(V + 2 * ((long)l * ldv + c)));
-      }
-    }
-    if (lg > 0)
-#pragma unroll
-      for (int c = 0; c < MAXC; ++c)
-        if (c < C) red[lg - 1][il][c] = acc[c];
-    for (int e = tid; e < nb * nb; e += 256) {
-      const int r = e / nb, q = e % nb;
-      blk[r][q] = cld(LU + 2 * ((long)(i0 + r) * ld + i0 + q));
-    }
-    __syncthreads();
-    if (lg == 0) {
-      cplx<T> val[MAXC];
-#pragma unroll
-      for (int c = 0; c < MAXC; ++c)
-        if (c < C && il < nb) {
-          val[c] = csub(cld(V + 2 * ((long)(i0 + il) * ldv + c)),
-                        cadd(cadd(acc[c], red[0][il][c]), cadd(red[1][il][c], red[2][il][c])));
-        } else {
-          val[c] = cmake<T>(0, 0);
-        }
+    } else {
       for (int i = nb - 1; i >= 0; --i) {
-        const cplx<T> lir = blk[i][il];  // L[i][r] = (L^T)[r][i], r < i
+        const cplx<T> l = blk[i][lane];  // L[i][r] = (L^T)[r][i], r < i
 #pragma unroll
         for (int c = 0; c < MAXC; ++c)
           if (c < C) {
             cplx<T> wi;
             wi.re = __shfl(val[c].re, i);
             wi.im = __shfl(val[c].im, i);
-            if (il < i) val[c] = cfms(val[c], lir, wi);
+            if (lane < i) val[c] = cfms(val[c], l, wi);
           }
       }
-      if (il < nb)
-#pragma unroll
-        for (int c = 0; c < MAXC; ++c)
-          if (c < C) cst(V + 2 * ((long)(i0 + il) * ldv + c), val[c]);
     }
-    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) sol[lane][c] = val[c];
+    if (blockIdx.x == 0 && lane < nb) {
+      T* dst = FWD ? Z : V;
+      const long ldd_ = FWD ? ldz : ldv;
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c)
+        if (c < C) cst(dst + 2 * ((long)(i0 + lane) * ldd_ + c), val[c]);
+    }
   }
-  // x = P^T w: interchanges in reverse order
+  __syncthreads();
+  // tile update: FWD rows k >= i1, BWD rows k < i0
+  const int k = (FWD ? i1 : 0) + blockIdx.x * 256 + tid;
+  const int kend = FWD ? n : i0;
+  if (k < kend) {
+    T* upd = FWD ? V : Z;
+    const long ldu = FWD ? ldv : ldz;
+    cplx<T> acc[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
+#pragma unroll 4
+    for (int l = 0; l < nb; ++l) {
+      const cplx<T> m = cld(LU + 2 * ((long)(i0 + l) * ld + k));
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c)
+        if (c < C) acc[c] = cfma(acc[c], m, sol[l][c]);
+    }
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c)
+      if (c < C) {
+        T* pu = upd + 2 * ((long)k * ldu + c);
+        cst(pu, csub(cld(pu), acc[c]));
+      }
+  }
+}
+
+// x = P^T w: row interchanges ipiv[n-1], ..., ipiv[0] applied to V (n x C);
+// one workgroup per matrix, one thread per column.
+template <typename T>
+__global__ __launch_bounds__(256) void k_perm_t(int n, int C, const int* __restrict__ ipiv, long sP, T* V, long ldv,
+                                                long sV) {
+  const long b = blockIdx.x;
+  ipiv += b * sP;
+  V += 2 * b * sV;
+  const int tid = threadIdx.x;
+  __shared__ int sp[256];
   for (int base = n; base > 0; base -= 256) {
     const int lo = max(0, base - 256);
     __syncthreads();
@@ -556,20 +533,37 @@ int gemv_t(int n, int C, const T* M, long ldm, long sM, const T* X, long ldx, lo
            double alr, double ali, double beta, int batch, hipStream_t st) {
   for (int c0 = 0; c0 < C; c0 += MAXC) {
     const int cc = std::min(MAXC, C - c0);
-    hipLaunchKernelGGL(k_gemv_t<T>, dim3(mnp_ceil_div(n, 64), batch), dim3(256), 0, st, n, cc, M, ldm, sM,
+    hipLaunchKernelGGL(k_gemv_t<T>, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, cc, M, ldm, sM,
                        X + 2 * c0, ldx, sX, Y + 2 * c0, ldy, sY, alr, ali, beta);
     MNP_LAUNCH_CHECK();
   }
   return 0;
 }
 
+// V <- M^-1 V (n x C) with the LU of M^T; Z (n x C) is work space.
 template <typename T>
 int lu_solve_t(int n, int C, const T* LU, long ld, long sLU, const int* ipiv, long sP, T* V, long ldv, long sV,
-               int batch, hipStream_t st) {
+               T* Z, long ldz, long sZ, int batch, hipStream_t st) {
+  const int nblk = (n + 63) / 64;
   for (int c0 = 0; c0 < C; c0 += MAXC) {
     const int cc = std::min(MAXC, C - c0);
-    hipLaunchKernelGGL(k_lu_solve_t<T>, dim3(batch), dim3(256), 0, st, n, cc, LU, ld, sLU, ipiv, sP, V + 2 * c0,
-                       ldv, sV);
+    T* v = V + 2 * c0;
+    T* z = Z + 2 * c0;
+    for (int ib = 0; ib < nblk; ++ib) {
+      const int i0 = ib * 64, i1 = std::min(n, i0 + 64);
+      dim3 grid(std::max(1, mnp_ceil_div(n - i1, 256)), batch);
+      hipLaunchKernelGGL((k_trsv_t_step<T, true>), grid, dim3(256), 0, st, n, cc, i0, LU, ld, sLU, v, ldv, sV, z,
+                         ldz, sZ);
+      MNP_LAUNCH_CHECK();
+    }
+    for (int ib = nblk - 1; ib >= 0; --ib) {
+      const int i0 = ib * 64;
+      dim3 grid(std::max(1, mnp_ceil_div(i0, 256)), batch);
+      hipLaunchKernelGGL((k_trsv_t_step<T, false>), grid, dim3(256), 0, st, n, cc, i0, LU, ld, sLU, v, ldv, sV, z,
+                         ldz, sZ);
+      MNP_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_perm_t<T>, dim3(batch), dim3(256), 0, st, n, cc, ipiv, sP, v, ldv, sV);
     MNP_LAUNCH_CHECK();
   }
   return 0;
@@ -593,14 +587,17 @@ extern "C" int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, lon
 }
 
 extern "C" int mnpbem_lu_solve_t_batched(int dtype, int n, int C, const void* LU, long ld, long sLU,
-                                         const int* ipiv, long sP, void* V, long ldv, long sV, int batch,
-                                         void* stream) {
+                                         const int* ipiv, long sP, void* V, long ldv, long sV, void* Z, long ldz,
+                                         long sZ, int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   if (n <= 0 || C <= 0 || batch <= 0) return 0;
+  MNP_REQUIRE(ldz >= C, "lu_solve_t: work space narrower than the right-hand side");
   if (dtype == MNP_DTYPE_C128)
-    return lu_solve_t<double>(n, C, (const double*)LU, ld, sLU, ipiv, sP, (double*)V, ldv, sV, batch, st);
+    return lu_solve_t<double>(n, C, (const double*)LU, ld, sLU, ipiv, sP, (double*)V, ldv, sV, (double*)Z, ldz,
+                              sZ, batch, st);
   if (dtype == MNP_DTYPE_C64)
-    return lu_solve_t<float>(n, C, (const float*)LU, ld, sLU, ipiv, sP, (float*)V, ldv, sV, batch, st);
+    return lu_solve_t<float>(n, C, (const float*)LU, ld, sLU, ipiv, sP, (float*)V, ldv, sV, (float*)Z, ldz, sZ,
+                             batch, st);
   mnp_set_error("lu_solve_t: unknown dtype");
   return 1;
 }
@@ -671,7 +668,9 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
       return rc;                                                                                                   \
     hipLaunchKernelGGL(k_step_rhs<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                          \
     MNP_LAUNCH_CHECK();                                                                                            \
-    if ((rc = lu_solve_t<T>(n, P, (const T*)St, ldm, smat, pivS, spiv, w, ldv, sv, batch, st))) return rc;        \
+    if ((rc = lu_solve_t<T>(n, P, (const T*)St, ldm, smat, pivS, spiv, w, ldv, sv, w + 2 * P, ldv, sv, batch,    \
+                            st)))                                                                                  \
+      return rc;                                                                                                   \
     hipLaunchKernelGGL(k_step_h2pre<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                        \
     MNP_LAUNCH_CHECK();                                                                                            \
     /* W[:,4P:7P] = Delta^-1 T */                                                                                  \
@@ -680,8 +679,10 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
       return rc;                                                                                                   \
     hipLaunchKernelGGL(k_step_pack<T>, eg, dim3(256), 0, st, n, P, r, w, x, ldv, sv);                              \
     MNP_LAUNCH_CHECK();                                                                                            \
-    if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G1t, ldm, smat, piv1, spiv, x, ldv, sv, batch, st))) return rc;  \
-    if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G2t, ldm, smat, piv2, spiv, x + 2 * 4 * P, ldv, sv, batch, st)))  \
+    if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G1t, ldm, smat, piv1, spiv, x, ldv, sv, w, ldv, sv, batch, st)))  \
+      return rc;                                                                                                   \
+    if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G2t, ldm, smat, piv2, spiv, x + 2 * 4 * P, ldv, sv,                \
+                            w + 2 * 4 * P, ldv, sv, batch, st)))                                                   \
       return rc;                                                                                                   \
     return 0;                                                                                                      \
   }
