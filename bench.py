@@ -30,6 +30,10 @@ if ROOT not in sys.path:
 
 PEAK_FP32_MFMA = 157.3e12   # MI355X dense FP32 matrix, FLOP/s
 PEAK_FP64_MFMA = 78.6e12    # MI355X dense FP64 matrix, FLOP/s
+PEAK_BF16_MFMA = 2.5e15     # MI355X dense bf16 matrix, FLOP/s
+# the split-product GEMM executes three bf16 products per real product, so its
+# ceiling in algorithmic (8 M N K per complex GEMM) flops is a third of bf16 peak
+PEAK_BF16X3_ALG = PEAK_BF16_MFMA / 3.0
 DRUDE = (10.0, 9.065, 0.0708)
 
 
@@ -78,6 +82,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the complex128 comparison sweep")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "mixed-fp32"],
+                    help="mixed: complex64 GEMMs on bf16 matrix cores (3 split products); "
+                         "mixed-fp32: on FP32 matrix cores")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the sweep shards by wavelength across GPUs "
@@ -97,7 +104,7 @@ def main():
     N = cp.n
     pw = m.PlaneWaveRet(pol, dr)
     t0 = time.time()
-    bem = m.BEMRet(cp, precision="mixed")
+    bem = m.BEMRet(cp, precision=args.precision)
     torch.cuda.synchronize()
     log("setup (mesh + GPU refinement coefficients): %.2f s, N = %d" % (time.time() - t0, N))
 
@@ -157,6 +164,13 @@ def main():
         log("cpu oracle: %.3f wavelengths/s on %d threads; mixed vs oracle max rel ext err %.2e"
             % (cpu_rate, cores, acc_vs_oracle))
 
+    if args.precision == "mixed":
+        kname = ("k_cgemm_sub_bf16x3 (complex64 GEMM on bf16 MFMA, 3 split products per real product; "
+                 "achieved = 8MNK algorithmic flops / time, peak = bf16 dense peak / 3)")
+        peak = PEAK_BF16X3_ALG
+    else:
+        kname = "k_cgemm_sub<float> (complex64 GEMM, FP32 MFMA)"
+        peak = PEAK_FP32_MFMA
     line = {
         "metric": "wavelengths solved/s, 1444-face Au sphere, mixed vs FP64, <=1e-8 rel ext err",
         "value": round(value, 4),
@@ -168,7 +182,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64+f64-refine",
+        "dtype": "c64(bf16x3 GEMM)+f64-refine" if args.precision == "mixed" else "c64+f64-refine",
         "data": "synthetic",
         "config": {"workload": "Drude-gold sphere d=40nm, trisphere(1444) = %d curved faces, 64 wavelengths "
                                "400-800 nm, 2 plane waves (x,y pol, +z), mixed precision" % N,
@@ -178,9 +192,8 @@ def main():
         "max_rel_ext_err_vs_fp64_gpu": acc_vs_fp64,
         "max_rel_ext_err_vs_cpu_oracle": acc_vs_oracle,
         "path_tflops_W": round(path_flops / elapsed / 1e12, 3),
-        "roofline": {"bound": "mfma", "kernel": "k_cgemm_sub<float> (complex64 GEMM, FP32 MFMA)",
-                     "achieved": round(gemm_achieved / 1e12, 3), "peak": PEAK_FP32_MFMA / 1e12,
-                     "unit": "TFLOP/s", "frac": round(gemm_achieved / PEAK_FP32_MFMA, 4),
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": round(gemm_achieved / 1e12, 3),
+                     "peak": round(peak / 1e12, 1), "unit": "TFLOP/s", "frac": round(gemm_achieved / peak, 4),
                      "traffic": None, "launches": nl.value,
                      "avg_launch_ms": round(ms.value / max(nl.value, 1), 4)},
         "cpu_baseline": cpu,

@@ -10,7 +10,9 @@
  *   - all pointers are device pointers unless stated; sizes are counts of
  *     elements; complex arrays are interleaved (re, im) like numpy/torch
  *     complex64 / complex128;
- *   - dtype: 1 = complex64, 2 = complex128;
+ *   - dtype: 1 = complex64, 2 = complex128, 3 = complex64 data whose GEMMs run
+ *     on bf16 matrix cores with 3-term split products (~1e-5 relative per
+ *     product; used under FP64 iterative refinement);
  *   - matrices are row-major with leading dimension ld (complex elements);
  *     "batched" calls process `batch` matrices spaced `stride` elements
  *     apart; pivot arrays are int32, 0-based, spaced `spiv` apart;
@@ -31,6 +33,7 @@ extern "C" {
 
 #define MNPBEM_C64 1
 #define MNPBEM_C128 2
+#define MNPBEM_C64_BF16X3 3
 
 int mnpbem_abi_version(void);
 const char* mnpbem_last_error(void);
@@ -64,8 +67,15 @@ int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, long ldm, long
  * (the transposed-storage convention of BEMRet's factors); Z (n x C,
  * leading dimension ldz >= C) is work space. */
 int mnpbem_lu_solve_t_batched(int dtype, int n, int C, const void* LU, long ld, long sLU, const int* ipiv,
-                              long spiv, void* V, long ldv, long sV, void* Z, long ldz, long sZ, int batch,
-                              void* stream);
+                              long spiv, void* V, long ldv, long sV, void* Z, long ldz, long sZ, const void* Dinv,
+                              long sD, int batch, void* stream);
+
+/* Inverses of the 64 x 64 diagonal blocks of getrf factors ([U_II^-1 | L_II^-1]
+ * per block, sD >= ceil(n/64)*2*64*64 elements) so that the block steps of
+ * mnpbem_lu_solve_t_batched are parallel matrix-vector products (Dinv = NULL
+ * there falls back to per-block substitution). */
+int mnpbem_lu_blockinv_batched(int dtype, int n, const void* LU, long ld, long sLU, void* Dinv, long sD, int batch,
+                               void* stream);
 
 int mnpbem_identity_batched(int dtype, int n, void* A, long ld, long sA, int batch, void* stream);
 
@@ -117,11 +127,13 @@ int mnpbem_sigma_batched(int dtype, int mode, int n, const void* S1, void* S2, v
 
 /* X = A^-1 R with the factors of BEMRet.init: LU(G1^T), LU(G2^T), Sigma1^T,
  * (Delta^-1)^T, LU(Sigma^T) (Eqs. 19-20 of Garcia de Abajo & Howie,
- * bem_ret.py:1694-1810).  W is work space of R's shape; P <= 4. */
+ * bem_ret.py:1694-1810); D1, D2, DS: their inverse diagonal blocks from
+ * mnpbem_lu_blockinv_batched (or NULL).  W is work space of R's shape; P <= 4. */
 int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t, const int* piv1, const void* G2t,
                              const int* piv2, const void* S1t, const void* Et, const void* St, const int* pivS,
-                             long ldm, long smat, long spiv, const void* R, void* W, void* X, long ldv, long sv,
-                             const double* nvec, const double* par, int batch, void* stream);
+                             long ldm, long smat, long spiv, const void* D1, const void* D2, const void* DS, long sD,
+                             const void* R, void* W, void* X, long ldv, long sv, const double* nvec,
+                             const double* par, int batch, void* stream);
 
 /* Rout = R - A X for the complex128 8N operator (bem_ret_iter.py:1362-1501)
  * with unfactored transposed G1, H1, G2, H2; Y is n x 16P work space;

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libmnpbem_amd.so")
 
 DTYPE_C64 = 1
 DTYPE_C128 = 2
+DTYPE_C64X = 3  # complex64 data, GEMMs on bf16 matrix cores (3-term split products)
 
 _i, _l, _d, _p = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
 
@@ -30,11 +31,12 @@ SIGNATURES = {
     "mnpbem_green_assemble": [_i, _i, _i, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _d, _p, _p, _l, _l,
                               _p, _p, _i, _p],
     "mnpbem_gemv_t_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _l, _p, _l, _l, _d, _d, _d, _i, _p],
-    "mnpbem_lu_solve_t_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _p, _l, _l, _i, _p],
+    "mnpbem_lu_solve_t_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _p, _l, _l, _p, _l, _i, _p],
+    "mnpbem_lu_blockinv_batched": [_i, _i, _p, _l, _l, _p, _l, _i, _p],
     "mnpbem_identity_batched": [_i, _i, _p, _l, _l, _i, _p],
     "mnpbem_sigma_batched": [_i, _i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _i, _p],
-    "mnpbem_bem_solve_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l, _l,
-                                 _p, _p, _i, _p],
+    "mnpbem_bem_solve_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l, _p,
+                                 _p, _p, _l, _l, _p, _p, _i, _p],
     "mnpbem_bem_residual_batched": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _l, _l, _p, _l, _l, _p, _p,
                                     _p, _i, _p],
     "mnpbem_accumulate": [_i, _l, _p, _p, _p],

@@ -14,10 +14,13 @@ factored.  The solve is GdA&H Eqs. (19)-(20) (bem_ret.py:1569-1831).
 
 Precision.  ``precision='double'`` runs everything in complex128.
 ``precision='mixed'`` assembles G/H once in complex128 and keeps a complex64
-copy; the factorisations and the direct solve run in complex64, and the
-solution is refined in FP64 against the complex128 8N operator of GdA&H
-Eqs. (10, 11, 14, 17) until ||R - A X|| <= refine_tol ||R|| for every
-wavelength (at most ``refine_maxit`` corrections).
+copy; the factorisations and the direct solve run in complex64 with the
+GEMMs on bf16 matrix cores (each float split into two bf16 terms, three
+products per real product), and the solution is refined in FP64 against the
+complex128 8N operator of GdA&H Eqs. (10, 11, 14, 17) until
+||R - A X|| <= refine_tol ||R|| for every wavelength (at most
+``refine_maxit`` corrections).  ``precision='mixed-fp32'`` is the same with
+the complex64 GEMMs on FP32 matrix cores.
 """
 
 import numpy as np
@@ -75,8 +78,8 @@ class BEMRet:
 
     def __init__(self, p, enei=None, precision="double", green=None, refine_tol=1e-12, refine_maxit=8,
                  **options):
-        if precision not in ("double", "mixed"):
-            raise ValueError("BEMRet: precision must be 'double' or 'mixed'")
+        if precision not in ("double", "mixed", "mixed-fp32"):
+            raise ValueError("BEMRet: precision must be 'double', 'mixed' or 'mixed-fp32'")
         self.p = p
         self.precision = precision
         self.refine_tol = refine_tol
@@ -108,9 +111,9 @@ class BEMRet:
         enei_list = np.atleast_1d(np.asarray(enei_list, dtype=float))
         g, dev = self.g, self.device
         B, n = len(enei_list), g.n
-        mixed = self.precision == "mixed"
+        mixed = self.precision != "double"
         cdt = torch.complex64 if mixed else torch.complex128
-        dt = nat.DTYPE_C64 if mixed else nat.DTYPE_C128
+        dt = {"double": nat.DTYPE_C128, "mixed": nat.DTYPE_C64X, "mixed-fp32": nat.DTYPE_C64}[self.precision]
         st = nat.stream_handle(dev)
         F = _Factorization()
         F.enei = enei_list
@@ -152,6 +155,13 @@ class BEMRet:
         nat.call("mnpbem_sigma_batched", dt, 1, n, nat.ptr(S1), nat.ptr(S2), None, nat.ptr(Et), n, nn,
                  nat.ptr(g.nvec), nat.ptr(F.par), B, st)
         nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(piv[3]), n, nat.ptr(info[3]), B, st)
+        # inverse diagonal blocks of the factors used by the small-RHS solves
+        nblk = (n + 63) // 64
+        F.sD = nblk * 2 * 64 * 64
+        F.DG = torch.empty((2, B, F.sD), dtype=cdt, device=dev)
+        F.DS = torch.empty((B, F.sD), dtype=cdt, device=dev)
+        nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(GG), n, nn, nat.ptr(F.DG), F.sD, 2 * B, st)
+        nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(F.DS), F.sD, B, st)
         F.GG, F.HH, F.Et, F.piv, F.info = GG, HH, Et, piv, info
         return F
 
@@ -160,7 +170,8 @@ class BEMRet:
         nn = n * n
         nat.call("mnpbem_bem_solve_batched", F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.piv[0]),
                  nat.ptr(F.GG[1]), nat.ptr(F.piv[1]), nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]),
-                 nat.ptr(F.piv[3]), n, nn, n, nat.ptr(R), nat.ptr(W), nat.ptr(X), 8 * P, n * 8 * P,
+                 nat.ptr(F.piv[3]), n, nn, n, nat.ptr(F.DG[0]), nat.ptr(F.DG[1]), nat.ptr(F.DS), F.sD,
+                 nat.ptr(R), nat.ptr(W), nat.ptr(X), 8 * P, n * 8 * P,
                  nat.ptr(self.g.nvec), nat.ptr(F.par), F.B, nat.stream_handle(self.device))
 
     def solve_batch(self, F, R):

@@ -14,6 +14,9 @@
 
 #define MNP_DTYPE_C64 1   // complex64  (float pairs)
 #define MNP_DTYPE_C128 2  // complex128 (double pairs)
+#define MNP_DTYPE_C64X 3  // complex64 data, GEMMs on bf16 matrix cores with 3-term splitting
+
+inline bool mnp_is_c64(int dtype) { return dtype == MNP_DTYPE_C64 || dtype == MNP_DTYPE_C64X; }
 
 // ---------------------------------------------------------------------------
 // error reporting across the C ABI: status code + thread-local message

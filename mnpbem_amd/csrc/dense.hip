@@ -153,6 +153,197 @@ __global__ __launch_bounds__(256) void k_cgemm_sub(int M, int N, int K, const T*
 }
 
 // ---------------------------------------------------------------------------
+// C[b] -= A[b] * B[b] for complex64 data on bf16 matrix cores.  Every float
+// x is split into x_hi = bf16(x) and x_lo = bf16(x - x_hi); a real product is
+// x_hi y_hi + x_hi y_lo + x_lo y_hi (the lo*lo term, ~2^-16 relative, is
+// dropped), accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  A complex
+// product takes 12 such MFMAs.  Accuracy is ~1e-5 relative per product; the
+// mixed-precision BEM solve recovers FP64 accuracy by iterative refinement.
+// Workgroup: 512 threads = 8 waves, 128 x 128 output tile, each wave 64 x 32
+// (4 x 2 blocks of 16 x 16); K slices of 32 staged in LDS as four bf16
+// planes (re_hi, re_lo, im_hi, im_lo), B stored transposed so that every
+// MFMA operand is one 16-byte LDS read per lane.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int XT = 128;  // output tile edge
+constexpr int XK = 32;   // K slice
+
+__device__ __forceinline__ bf16x8 bneg(bf16x8 v) {
+  u32x4 u = __builtin_bit_cast(u32x4, v);
+  u ^= 0x80008000u;
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// LDS layout: rows of XK = 32 bf16 (four 16-byte chunks, 64 B), chunk c of
+// row r stored at chunk c ^ ((r >> 1) & 3): the 16-byte stores and the
+// fragment reads of eight consecutive rows then fall in distinct banks.
+__device__ __forceinline__ int xsw(int r, int c) { return r * XK + ((c ^ ((r >> 1) & 3)) << 3); }
+
+// split 8 floats (stride 2 in v: real or imaginary parts of 8 complex) into
+// bf16 hi and lo vectors
+__device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const float x = v[2 * t];
+    const __bf16 h = (__bf16)x;
+    hi[t] = h;
+    lo[t] = (__bf16)(x - (float)h);
+  }
+}
+
+__global__ __launch_bounds__(512) void k_cgemm_sub_bf16x3(int M, int N, int K, const float* __restrict__ A, long lda,
+                                                          long sA, const float* __restrict__ B, long ldb, long sB,
+                                                          float* __restrict__ C, long ldc, long sC) {
+  const long bz = blockIdx.z;
+  A += 2 * bz * sA;
+  B += 2 * bz * sB;
+  C += 2 * bz * sC;
+  const int m0 = blockIdx.y * XT, n0 = blockIdx.x * XT;
+  // two stages x four planes (re_hi, re_lo, im_hi, im_lo) x 128 rows x 32
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][4][XT * XK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][4][XT * XK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 2) * 64, wn = (wave & 3) * 32;
+  // staging: each thread owns 8 consecutive k of one A row and of one B column
+  const int la_r = tid >> 2, la_c = tid & 3;
+  const int lb_n = tid & 127, lb_c = tid >> 7;
+  float ra[16], rb[16];
+  auto gload = [&](int k0) {
+    const int gr = m0 + la_r;
+    const int gk = k0 + la_c * 8;
+    if (gr < M && gk + 8 <= K) {
+      const float4* src = reinterpret_cast<const float4*>(A + 2 * ((long)gr * lda + gk));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 v = src[q];
+        ra[4 * q] = v.x;
+        ra[4 * q + 1] = v.y;
+        ra[4 * q + 2] = v.z;
+        ra[4 * q + 3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const bool ok = gr < M && gk + q < K;
+        const float2 v = ok ? *reinterpret_cast<const float2*>(A + 2 * ((long)gr * lda + gk + q)) : make_float2(0.f, 0.f);
+        ra[2 * q] = v.x;
+        ra[2 * q + 1] = v.y;
+      }
+    }
+    const int gc = n0 + lb_n;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int gk2 = k0 + lb_c * 8 + q;
+      const float2 v = (gc < N && gk2 < K) ? *reinterpret_cast<const float2*>(B + 2 * ((long)gk2 * ldb + gc))
+                                           : make_float2(0.f, 0.f);
+      rb[2 * q] = v.x;
+      rb[2 * q + 1] = v.y;
+    }
+  };
+  auto sstore = [&](int stg) {
+    bf16x8 h, l;
+    const int oa = xsw(la_r, la_c), ob = xsw(lb_n, lb_c);
+    split8(ra, h, l);
+    *reinterpret_cast<bf16x8*>(&As[stg][0][oa]) = h;
+    *reinterpret_cast<bf16x8*>(&As[stg][1][oa]) = l;
+    split8(ra + 1, h, l);
+    *reinterpret_cast<bf16x8*>(&As[stg][2][oa]) = h;
+    *reinterpret_cast<bf16x8*>(&As[stg][3][oa]) = l;
+    split8(rb, h, l);
+    *reinterpret_cast<bf16x8*>(&Bs[stg][0][ob]) = h;
+    *reinterpret_cast<bf16x8*>(&Bs[stg][1][ob]) = l;
+    split8(rb + 1, h, l);
+    *reinterpret_cast<bf16x8*>(&Bs[stg][2][ob]) = h;
+    *reinterpret_cast<bf16x8*>(&Bs[stg][3][ob]) = l;
+  };
+  f32x4 accR[4][2], accI[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = f32x4{0, 0, 0, 0};
+      accI[i][j] = f32x4{0, 0, 0, 0};
+    }
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  const int fr = lane & 15, fc = lane >> 4;
+  int stg = 0;
+  for (int k0 = 0; k0 < K; k0 += XK) {
+    const bool more = (k0 + XK) < K;
+    if (more) gload(k0 + XK);
+    bf16x8 a[4][4];  // [row block][plane]
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int o = xsw(wm + 16 * i + fr, fc);
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) a[i][pl] = *reinterpret_cast<const bf16x8*>(&As[stg][pl][o]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int o = xsw(wn + 16 * j + fr, fc);
+      const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&Bs[stg][0][o]);
+      const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&Bs[stg][1][o]);
+      const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&Bs[stg][2][o]);
+      const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&Bs[stg][3][o]);
+      const bf16x8 nbih = bneg(bih), nbil = bneg(bil);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][0], brh, accR[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][0], bih, accI[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][0], brl, accR[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][0], bil, accI[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][1], brh, accR[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][1], bih, accI[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][2], nbih, accR[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][2], brh, accI[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][2], nbil, accR[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][2], brl, accI[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][3], nbih, accR[i][j]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][3], brh, accI[i][j]);
+    }
+    // the other stage is free: every wave passed the barrier after reading it
+    if (more) sstore(stg ^ 1);
+    __syncthreads();
+    stg ^= 1;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int gc = n0 + wn + 16 * j + fr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        if (gr < M && gc < N) {
+          float* p = C + 2 * ((long)gr * ldc + gc);
+          float2 c = *reinterpret_cast<float2*>(p);
+          c.x -= accR[i][j][r];
+          c.y -= accI[i][j][r];
+          *reinterpret_cast<float2*>(p) = c;
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // LU of the panel A[c0:N, c0:c0+w] (w <= W) with partial pivoting, one
 // workgroup per matrix.  Thread t holds rows t, t+NT, ... (S rows) of the
 // panel in registers; per column: workgroup argmax of |re|+|im|, row
@@ -177,9 +368,8 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int N, int c0, int w, T* A, long
     for (int c = 0; c < W; ++c)
       a[s][c] = (r < m && c < w) ? cld(A + 2 * ((long)(c0 + r) * lda + c0 + c)) : cmake<T>(0, 0);
   }
-  __shared__ T sval[NW];
-  __shared__ int sidx[NW];
-  __shared__ int spiv;
+  __shared__ T sval[2 * NW];
+  __shared__ int sidx[2 * NW];
   __shared__ cplx<T> rowc[W], rowp[W];
 
 #pragma unroll
@@ -208,24 +398,28 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int N, int c0, int w, T* A, long
       }
     }
     if (lane == 0) {
-      sval[wv] = best;
-      sidx[wv] = bi;
+      sval[(c & 1) * NW + wv] = best;
+      sidx[(c & 1) * NW + wv] = bi;
     }
     __syncthreads();
-    if (tid == 0) {
-      T bb = sval[0];
-      int ii = sidx[0];
-      for (int q = 1; q < NW; ++q)
-        if (sval[q] > bb || (sval[q] == bb && sidx[q] < ii)) {
-          bb = sval[q];
-          ii = sidx[q];
+    // every thread reduces the per-wave candidates (double-buffered by column
+    // parity, so the next column's writes cannot race these reads)
+    int p;
+    {
+      T bb = sval[(c & 1) * NW];
+      int ii = sidx[(c & 1) * NW];
+      for (int q = 1; q < NW; ++q) {
+        const T v = sval[(c & 1) * NW + q];
+        const int iq = sidx[(c & 1) * NW + q];
+        if (v > bb || (v == bb && iq < ii)) {
+          bb = v;
+          ii = iq;
         }
+      }
       if (ii < c || ii >= m) ii = c;  // only NaN entries left: keep the diagonal
-      spiv = ii;
-      ipiv[c0 + c] = c0 + ii;
+      p = ii;
     }
-    __syncthreads();
-    const int p = spiv;
+    if (tid == 0) ipiv[c0 + c] = c0 + p;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int r = tid + s * NT;
@@ -330,10 +524,9 @@ struct TLeaf<float> {
   static constexpr int n = 64;
 };
 
-template <typename T, bool LOWER>
+template <typename T, bool LOWER, int TLN>
 __global__ __launch_bounds__(256) void k_trsm_leaf(int n, int nrhs, const T* Tm, long ldt, long sT, T* B,
                                                    long ldb, long sB) {
-  constexpr int TLN = TLeaf<T>::n;
   const long b = blockIdx.y;
   Tm += 2 * b * sT;
   B += 2 * b * sB;
@@ -399,14 +592,25 @@ struct GemmProf {
 };
 GemmProf g_prof;
 
+// gmode 1 (complex64 only): bf16 split-product GEMM
 template <typename T>
 int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, long ldb, long sB, T* C, long ldc,
-             long sC, int batch, hipStream_t st) {
+             long sC, int batch, hipStream_t st, int gmode) {
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return 0;
-  dim3 grid(mnp_ceil_div(N, GT), mnp_ceil_div(M, GT), batch);
   const bool timed = g_prof.on && g_prof.used + 2 <= g_prof.ev.size();
   if (timed) MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used], st));
-  hipLaunchKernelGGL(k_cgemm_sub<T>, grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+  if constexpr (sizeof(T) == 4) {
+    if (gmode == 1) {
+      dim3 grid(mnp_ceil_div(N, XT), mnp_ceil_div(M, XT), batch);
+      hipLaunchKernelGGL(k_cgemm_sub_bf16x3, grid, dim3(512), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+    } else {
+      dim3 grid(mnp_ceil_div(N, GT), mnp_ceil_div(M, GT), batch);
+      hipLaunchKernelGGL(k_cgemm_sub<T>, grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+    }
+  } else {
+    dim3 grid(mnp_ceil_div(N, GT), mnp_ceil_div(M, GT), batch);
+    hipLaunchKernelGGL(k_cgemm_sub<T>, grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+  }
   MNP_LAUNCH_CHECK();
   if (timed) {
     MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used + 1], st));
@@ -419,16 +623,28 @@ int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, lon
 // X <- T^-1 X, T n x n at Tm (lower unit or upper), X n x nrhs
 template <typename T>
 int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, long ldx, long sX, int batch,
-             hipStream_t st) {
+             hipStream_t st, int gmode) {
   if (n <= 0 || nrhs <= 0) return 0;
   constexpr int TL = TLeaf<T>::n;
   if (n <= TL) {
     dim3 grid(mnp_ceil_div(nrhs, 256), batch);
-    if (lower)
-      hipLaunchKernelGGL((k_trsm_leaf<T, true>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx, sX);
-    else
-      hipLaunchKernelGGL((k_trsm_leaf<T, false>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx, sX);
-    MNP_LAUNCH_CHECK();
+    // smallest unrolled size that holds the triangle
+#define MNP_TRSM_LEAF(SZ)                                                                                    \
+  if (n <= (SZ)) {                                                                                           \
+    if (lower)                                                                                               \
+      hipLaunchKernelGGL((k_trsm_leaf<T, true, SZ>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx, sX); \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_trsm_leaf<T, false, SZ>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx, sX); \
+    MNP_LAUNCH_CHECK();                                                                                      \
+    return 0;                                                                                                \
+  }
+    MNP_TRSM_LEAF(8)
+    MNP_TRSM_LEAF(16)
+    MNP_TRSM_LEAF(32)
+    if constexpr (TL >= 64) {
+      MNP_TRSM_LEAF(64)
+    }
+#undef MNP_TRSM_LEAF
     return 0;
   }
   int n1 = ((n / 2 + TL - 1) / TL) * TL;
@@ -436,20 +652,20 @@ int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, 
   const int n2 = n - n1;
   int rc;
   if (lower) {
-    if ((rc = trsm_rec(true, n1, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st))) return rc;
+    if ((rc = trsm_rec(true, n1, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st, gmode))) return rc;
     if ((rc = gemm_sub(n2, nrhs, n1, Tm + 2 * ((long)n1 * ldt), ldt, sT, X, ldx, sX, X + 2 * ((long)n1 * ldx), ldx,
-                       sX, batch, st)))
+                       sX, batch, st, gmode)))
       return rc;
     return trsm_rec(true, n2, nrhs, Tm + 2 * ((long)n1 * ldt + n1), ldt, sT, X + 2 * ((long)n1 * ldx), ldx, sX,
-                    batch, st);
+                    batch, st, gmode);
   } else {
     if ((rc = trsm_rec(false, n2, nrhs, Tm + 2 * ((long)n1 * ldt + n1), ldt, sT, X + 2 * ((long)n1 * ldx), ldx, sX,
-                       batch, st)))
+                       batch, st, gmode)))
       return rc;
     if ((rc = gemm_sub(n1, nrhs, n2, Tm + 2 * (long)n1, ldt, sT, X + 2 * ((long)n1 * ldx), ldx, sX, X, ldx, sX,
-                       batch, st)))
+                       batch, st, gmode)))
       return rc;
-    return trsm_rec(false, n1, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st);
+    return trsm_rec(false, n1, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st, gmode);
   }
 }
 
@@ -521,21 +737,21 @@ int leaf_launch(int W, int N, int c0, int w, T* A, long lda, long sA, int* ipiv,
 // recursive LU of columns [c0, c0+n) (rows c0..N-1)
 template <typename T>
 int getrf_rec(int W, int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
-              hipStream_t st) {
+              hipStream_t st, int gmode) {
   if (n <= W) return leaf_launch<T>(W, N, c0, n, A, lda, sA, ipiv, sP, info, batch, st);
   int n1 = ((n / 2 + W - 1) / W) * W;
   if (n1 >= n) n1 = n - W;
   const int n2 = n - n1;
   int rc;
-  if ((rc = getrf_rec(W, N, c0, n1, A, lda, sA, ipiv, sP, info, batch, st))) return rc;
+  if ((rc = getrf_rec(W, N, c0, n1, A, lda, sA, ipiv, sP, info, batch, st, gmode))) return rc;
   if ((rc = laswp<T>(c0, c0 + n1, c0 + n1, c0 + n, A, lda, sA, ipiv, sP, batch, false, st))) return rc;
   T* A11 = A + 2 * ((long)c0 * lda + c0);
   T* A12 = A + 2 * ((long)c0 * lda + c0 + n1);
   T* A21 = A + 2 * ((long)(c0 + n1) * lda + c0);
   T* A22 = A + 2 * ((long)(c0 + n1) * lda + c0 + n1);
-  if ((rc = trsm_rec<T>(true, n1, n2, A11, lda, sA, A12, lda, sA, batch, st))) return rc;
-  if ((rc = gemm_sub<T>(N - c0 - n1, n2, n1, A21, lda, sA, A12, lda, sA, A22, lda, sA, batch, st))) return rc;
-  if ((rc = getrf_rec(W, N, c0 + n1, n2, A, lda, sA, ipiv, sP, info, batch, st))) return rc;
+  if ((rc = trsm_rec<T>(true, n1, n2, A11, lda, sA, A12, lda, sA, batch, st, gmode))) return rc;
+  if ((rc = gemm_sub<T>(N - c0 - n1, n2, n1, A21, lda, sA, A12, lda, sA, A22, lda, sA, batch, st, gmode))) return rc;
+  if ((rc = getrf_rec(W, N, c0 + n1, n2, A, lda, sA, ipiv, sP, info, batch, st, gmode))) return rc;
   return laswp<T>(c0 + n1, c0 + n, c0, c0 + n1, A, lda, sA, ipiv, sP, batch, false, st);
 }
 
@@ -551,10 +767,10 @@ extern "C" int mnpbem_cgemm_sub_batched(int dtype, int M, int N, int K, const vo
   MNP_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "cgemm: negative size");
   if (dtype == MNP_DTYPE_C128)
     return gemm_sub<double>(M, N, K, (const double*)A, lda, sA, (const double*)B, ldb, sB, (double*)C, ldc, sC,
-                            batch, st);
-  if (dtype == MNP_DTYPE_C64)
+                            batch, st, 0);
+  if (mnp_is_c64(dtype))
     return gemm_sub<float>(M, N, K, (const float*)A, lda, sA, (const float*)B, ldb, sB, (float*)C, ldc, sC, batch,
-                           st);
+                           st, dtype == MNP_DTYPE_C64X ? 1 : 0);
   mnp_set_error("cgemm: unknown dtype");
   return 1;
 }
@@ -567,9 +783,10 @@ extern "C" int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA
   hipLaunchKernelGGL(k_fill_int, dim3(mnp_ceil_div(batch, 256)), dim3(256), 0, st, info, (long)batch, 0);
   MNP_LAUNCH_CHECK();
   if (dtype == MNP_DTYPE_C128)
-    return getrf_rec<double>(LeafCfg<double>::width(n), n, 0, n, (double*)A, lda, sA, ipiv, sP, info, batch, st);
-  if (dtype == MNP_DTYPE_C64)
-    return getrf_rec<float>(LeafCfg<float>::width(n), n, 0, n, (float*)A, lda, sA, ipiv, sP, info, batch, st);
+    return getrf_rec<double>(LeafCfg<double>::width(n), n, 0, n, (double*)A, lda, sA, ipiv, sP, info, batch, st, 0);
+  if (mnp_is_c64(dtype))
+    return getrf_rec<float>(LeafCfg<float>::width(n), n, 0, n, (float*)A, lda, sA, ipiv, sP, info, batch, st,
+                            dtype == MNP_DTYPE_C64X ? 1 : 0);
   mnp_set_error("getrf: unknown dtype");
   return 1;
 }
@@ -583,13 +800,16 @@ extern "C" int mnpbem_getrs_batched(int dtype, int n, int nrhs, const void* A, l
   int rc;
   if (dtype == MNP_DTYPE_C128) {
     if ((rc = laswp<double>(0, n, 0, nrhs, (double*)B, ldb, sB, ipiv, sP, batch, false, st))) return rc;
-    if ((rc = trsm_rec<double>(true, n, nrhs, (const double*)A, lda, sA, (double*)B, ldb, sB, batch, st))) return rc;
-    return trsm_rec<double>(false, n, nrhs, (const double*)A, lda, sA, (double*)B, ldb, sB, batch, st);
+    if ((rc = trsm_rec<double>(true, n, nrhs, (const double*)A, lda, sA, (double*)B, ldb, sB, batch, st, 0)))
+      return rc;
+    return trsm_rec<double>(false, n, nrhs, (const double*)A, lda, sA, (double*)B, ldb, sB, batch, st, 0);
   }
-  if (dtype == MNP_DTYPE_C64) {
+  if (mnp_is_c64(dtype)) {
+    const int gm = dtype == MNP_DTYPE_C64X ? 1 : 0;
     if ((rc = laswp<float>(0, n, 0, nrhs, (float*)B, ldb, sB, ipiv, sP, batch, false, st))) return rc;
-    if ((rc = trsm_rec<float>(true, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st))) return rc;
-    return trsm_rec<float>(false, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st);
+    if ((rc = trsm_rec<float>(true, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st, gm)))
+      return rc;
+    return trsm_rec<float>(false, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st, gm);
   }
   mnp_set_error("getrs: unknown dtype");
   return 1;

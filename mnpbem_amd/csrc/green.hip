@@ -408,7 +408,7 @@ extern "C" int mnpbem_green_assemble(int dtype, int dual, int n, const double* p
       hipLaunchKernelGGL((k_green_overlay<double, false>), g2, dim3(256), 0, st, npairs, ord1, prow, pcol, g, f,
                          pos, nvec, mat, kmat, nmat, hsign, (double*)G, (double*)H, ld, stride, (float*)nullptr,
                          (float*)nullptr);
-  } else if (dtype == MNP_DTYPE_C64) {
+  } else if (mnp_is_c64(dtype)) {
     hipLaunchKernelGGL((k_green_assemble<float, false>), g1, dim3(AS_I), 0, st, n, pos, nvec, area, mat, kmat,
                        nmat, (float*)G, (float*)H, ld, stride, (float*)nullptr, (float*)nullptr);
     MNP_LAUNCH_CHECK();

@@ -52,8 +52,19 @@ __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restric
     }
     __syncthreads();
     if (i < n) {
-#pragma unroll 4
-      for (int jj = 0; jj < nj; ++jj) {
+      // 16 independent row loads in flight per thread
+      int jj = 0;
+      for (; jj + 16 <= nj; jj += 16) {
+        cplx<T> m[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) m[u] = cld(M + 2 * ((long)(j0 + jj + u) * ldm + i));
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+#pragma unroll
+          for (int c = 0; c < MAXC; ++c)
+            if (c < C) acc[c] = cfma(acc[c], m[u], xs[jj + u][c]);
+      }
+      for (; jj < nj; ++jj) {
         const cplx<T> m = cld(M + 2 * ((long)(j0 + jj) * ldm + i));
 #pragma unroll
         for (int c = 0; c < MAXC; ++c)
@@ -88,7 +99,8 @@ __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restric
 // updates its own 256-row tile; workgroup 0 stores the block solution.
 template <typename T, bool FWD>
 __global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const T* __restrict__ LU, long ld,
-                                                     long sLU, T* V, long ldv, long sV, T* Z, long ldz, long sZ) {
+                                                     long sLU, T* V, long ldv, long sV, T* Z, long ldz, long sZ,
+                                                     const T* __restrict__ Dinv, long sD) {
   const long b = blockIdx.y;
   LU += 2 * b * sLU;
   V += 2 * b * sV;
@@ -97,6 +109,38 @@ __global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const
   const int nb = min(64, n - i0), i1 = i0 + nb;
   __shared__ cplx<T> blk[64][65];
   __shared__ cplx<T> sol[64][MAXC];
+  if (Dinv) {
+    // block solution = (inverse diagonal block)^T * rhs, all rows in parallel
+    const T* src = FWD ? V : Z;
+    const long lds = FWD ? ldv : ldz;
+    const T* D = Dinv + 2 * (b * sD + ((long)(i0 / 64) * 2 + (FWD ? 0 : 1)) * 64 * 64);
+    for (int e = tid; e < 64 * C; e += 256) {
+      const int r = e / C, c = e % C;
+      blk[r][c] = r < nb ? cld(src + 2 * ((long)(i0 + r) * lds + c)) : cmake<T>(0, 0);
+    }
+    __syncthreads();
+    // thread (row r, column group cg): the 64 entries of column r of the
+    // inverse block are loaded once and applied to the group's columns
+    const int r = tid & 63, cg = tid >> 6;
+    cplx<T> dcol[64];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) dcol[q] = (q < nb && r < nb) ? cld(D + 2 * ((long)q * 64 + r)) : cmake<T>(0, 0);
+    for (int c = cg; c < C; c += 4) {
+      cplx<T> acc = cmake<T>(0, 0);
+#pragma unroll
+      for (int q = 0; q < 64; ++q) acc = cfma(acc, dcol[q], blk[q][c]);
+      sol[r][c] = acc;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0) {
+      T* dst = FWD ? Z : V;
+      const long ldd_ = FWD ? ldz : ldv;
+      for (int e = tid; e < nb * C; e += 256) {
+        const int rr = e / C, c = e % C;
+        cst(dst + 2 * ((long)(i0 + rr) * ldd_ + c), sol[rr][c]);
+      }
+    }
+  } else {
   for (int e = tid; e < nb * nb; e += 256) {
     const int r = e / nb, q = e % nb;
     blk[r][q] = cld(LU + 2 * ((long)(i0 + r) * ld + i0 + q));
@@ -150,6 +194,7 @@ __global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const
     }
   }
   __syncthreads();
+  }
   // tile update: FWD rows k >= i1, BWD rows k < i0
   const int k = (FWD ? i1 : 0) + blockIdx.x * 256 + tid;
   const int kend = FWD ? n : i0;
@@ -159,8 +204,18 @@ __global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const
     cplx<T> acc[MAXC];
 #pragma unroll
     for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
-#pragma unroll 4
-    for (int l = 0; l < nb; ++l) {
+    int l = 0;
+    for (; l + 16 <= nb; l += 16) {  // 16 row loads in flight
+      cplx<T> m[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) m[u] = cld(LU + 2 * ((long)(i0 + l + u) * ld + k));
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < C) acc[c] = cfma(acc[c], m[u], sol[l + u][c]);
+    }
+    for (; l < nb; ++l) {
       const cplx<T> m = cld(LU + 2 * ((long)(i0 + l) * ld + k));
 #pragma unroll
       for (int c = 0; c < MAXC; ++c)
@@ -173,6 +228,46 @@ __global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const
         cst(pu, csub(cld(pu), acc[c]));
       }
   }
+}
+
+// Inverses of the 64 x 64 diagonal blocks of the LU factors (U_II^-1 upper,
+// L_II^-1 unit lower), stored per block as [U^-1 | L^-1] (2 x 64 x 64,
+// zero outside the triangle).  One workgroup of 64 threads per (block,
+// matrix); thread j computes column j by substitution, touching only its own
+// column of the LDS result, so no barriers are needed inside the loop.
+template <typename T>
+__global__ __launch_bounds__(64) void k_lu_blockinv(int n, const T* __restrict__ LU, long ld, long sLU, T* Dinv,
+                                                    long sD) {
+  const long b = blockIdx.y;
+  const int ib = blockIdx.x, i0 = ib * 64, nb = min(64, n - i0);
+  LU += 2 * b * sLU;
+  T* D = Dinv + 2 * (b * sD + (long)ib * 2 * 64 * 64);
+  __shared__ cplx<T> Tb[64][65];
+  __shared__ cplx<T> X[64][65];
+  const int j = threadIdx.x;
+  for (int e = j; e < 64 * 64; e += 64) {
+    const int r = e / 64, q = e % 64;
+    Tb[r][q] = (r < nb && q < nb) ? cld(LU + 2 * ((long)(i0 + r) * ld + i0 + q)) : cmake<T>(0, 0);
+  }
+  __syncthreads();
+  // U X = I (upper, back substitution)
+  for (int i = 63; i >= 0; --i) {
+    cplx<T> s = cmake<T>(i == j && i < nb ? T(1) : T(0), T(0));
+    if (i < nb) {
+      for (int l = i + 1; l < nb; ++l) s = cfms(s, Tb[i][l], X[l][j]);
+      s = cdiv(s, Tb[i][i]);
+    }
+    X[i][j] = s;
+  }
+  for (int i = 0; i < 64; ++i) cst(D + 2 * ((long)i * 64 + j), X[i][j]);
+  // L X = I (unit lower, forward substitution)
+  for (int i = 0; i < 64; ++i) {
+    cplx<T> s = cmake<T>(i == j && i < nb ? T(1) : T(0), T(0));
+    if (i < nb)
+      for (int l = 0; l < i; ++l) s = cfms(s, Tb[i][l], X[l][j]);
+    X[i][j] = s;
+  }
+  for (int i = 0; i < 64; ++i) cst(D + 2 * (64 * 64 + (long)i * 64 + j), X[i][j]);
 }
 
 // x = P^T w: row interchanges ipiv[n-1], ..., ipiv[0] applied to V (n x C);
@@ -543,7 +638,7 @@ int gemv_t(int n, int C, const T* M, long ldm, long sM, const T* X, long ldx, lo
 // V <- M^-1 V (n x C) with the LU of M^T; Z (n x C) is work space.
 template <typename T>
 int lu_solve_t(int n, int C, const T* LU, long ld, long sLU, const int* ipiv, long sP, T* V, long ldv, long sV,
-               T* Z, long ldz, long sZ, int batch, hipStream_t st) {
+               T* Z, long ldz, long sZ, int batch, hipStream_t st, const T* Dinv = nullptr, long sD = 0) {
   const int nblk = (n + 63) / 64;
   for (int c0 = 0; c0 < C; c0 += MAXC) {
     const int cc = std::min(MAXC, C - c0);
@@ -553,14 +648,14 @@ int lu_solve_t(int n, int C, const T* LU, long ld, long sLU, const int* ipiv, lo
       const int i0 = ib * 64, i1 = std::min(n, i0 + 64);
       dim3 grid(std::max(1, mnp_ceil_div(n - i1, 256)), batch);
       hipLaunchKernelGGL((k_trsv_t_step<T, true>), grid, dim3(256), 0, st, n, cc, i0, LU, ld, sLU, v, ldv, sV, z,
-                         ldz, sZ);
+                         ldz, sZ, Dinv, sD);
       MNP_LAUNCH_CHECK();
     }
     for (int ib = nblk - 1; ib >= 0; --ib) {
       const int i0 = ib * 64;
       dim3 grid(std::max(1, mnp_ceil_div(i0, 256)), batch);
       hipLaunchKernelGGL((k_trsv_t_step<T, false>), grid, dim3(256), 0, st, n, cc, i0, LU, ld, sLU, v, ldv, sV, z,
-                         ldz, sZ);
+                         ldz, sZ, Dinv, sD);
       MNP_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_perm_t<T>, dim3(batch), dim3(256), 0, st, n, cc, ipiv, sP, v, ldv, sV);
@@ -579,7 +674,7 @@ extern "C" int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, lon
   if (dtype == MNP_DTYPE_C128)
     return gemv_t<double>(n, C, (const double*)M, ldm, sM, (const double*)X, ldx, sX, (double*)Y, ldy, sY, alr,
                           ali, beta, batch, st);
-  if (dtype == MNP_DTYPE_C64)
+  if (mnp_is_c64(dtype))
     return gemv_t<float>(n, C, (const float*)M, ldm, sM, (const float*)X, ldx, sX, (float*)Y, ldy, sY, alr, ali,
                          beta, batch, st);
   mnp_set_error("gemv_t: unknown dtype");
@@ -588,16 +683,16 @@ extern "C" int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, lon
 
 extern "C" int mnpbem_lu_solve_t_batched(int dtype, int n, int C, const void* LU, long ld, long sLU,
                                          const int* ipiv, long sP, void* V, long ldv, long sV, void* Z, long ldz,
-                                         long sZ, int batch, void* stream) {
+                                         long sZ, const void* Dinv, long sD, int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   if (n <= 0 || C <= 0 || batch <= 0) return 0;
   MNP_REQUIRE(ldz >= C, "lu_solve_t: work space narrower than the right-hand side");
   if (dtype == MNP_DTYPE_C128)
     return lu_solve_t<double>(n, C, (const double*)LU, ld, sLU, ipiv, sP, (double*)V, ldv, sV, (double*)Z, ldz,
-                              sZ, batch, st);
-  if (dtype == MNP_DTYPE_C64)
+                              sZ, batch, st, (const double*)Dinv, sD);
+  if (mnp_is_c64(dtype))
     return lu_solve_t<float>(n, C, (const float*)LU, ld, sLU, ipiv, sP, (float*)V, ldv, sV, (float*)Z, ldz, sZ,
-                             batch, st);
+                             batch, st, (const float*)Dinv, sD);
   mnp_set_error("lu_solve_t: unknown dtype");
   return 1;
 }
@@ -608,7 +703,7 @@ extern "C" int mnpbem_identity_batched(int dtype, int n, void* A, long ld, long 
   dim3 grid(mnp_ceil_div((long)n * n, 256), batch);
   if (dtype == MNP_DTYPE_C128)
     hipLaunchKernelGGL(k_identity<double>, grid, dim3(256), 0, st, n, (double*)A, ld, sA);
-  else if (dtype == MNP_DTYPE_C64)
+  else if (mnp_is_c64(dtype))
     hipLaunchKernelGGL(k_identity<float>, grid, dim3(256), 0, st, n, (float*)A, ld, sA);
   else {
     mnp_set_error("identity: unknown dtype");
@@ -628,7 +723,7 @@ extern "C" int mnpbem_sigma_batched(int dtype, int mode, int n, const void* S1, 
   if (dtype == MNP_DTYPE_C128)
     hipLaunchKernelGGL(k_sigma<double>, grid, dim3(256), 0, st, mode, n, (const double*)S1, (double*)S2,
                        (double*)D, (const double*)E, ld, stride, nvec, par);
-  else if (dtype == MNP_DTYPE_C64)
+  else if (mnp_is_c64(dtype))
     hipLaunchKernelGGL(k_sigma<float>, grid, dim3(256), 0, st, mode, n, (const float*)S1, (float*)S2, (float*)D,
                        (const float*)E, ld, stride, nvec, par);
   else {
@@ -644,8 +739,9 @@ extern "C" int mnpbem_sigma_batched(int dtype, int mode, int n, const void* S1, 
 // complex elements; W is work space.
 extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t, const int* piv1, const void* G2t,
                                         const int* piv2, const void* S1t, const void* Et, const void* St,
-                                        const int* pivS, long ldm, long smat, long spiv, const void* R, void* W,
-                                        void* X, long ldv, long sv, const double* nvec, const double* par, int batch,
+                                        const int* pivS, long ldm, long smat, long spiv, const void* D1,
+                                        const void* D2, const void* DS, long sD, const void* R, void* W, void* X,
+                                        long ldv, long sv, const double* nvec, const double* par, int batch,
                                         void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldv >= 8 * P, "bem_solve: bad size (P <= 4)");
@@ -669,7 +765,7 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
     hipLaunchKernelGGL(k_step_rhs<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                          \
     MNP_LAUNCH_CHECK();                                                                                            \
     if ((rc = lu_solve_t<T>(n, P, (const T*)St, ldm, smat, pivS, spiv, w, ldv, sv, w + 2 * P, ldv, sv, batch,    \
-                            st)))                                                                                  \
+                            st, (const T*)DS, sD)))                                                                \
       return rc;                                                                                                   \
     hipLaunchKernelGGL(k_step_h2pre<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                        \
     MNP_LAUNCH_CHECK();                                                                                            \
@@ -679,15 +775,16 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
       return rc;                                                                                                   \
     hipLaunchKernelGGL(k_step_pack<T>, eg, dim3(256), 0, st, n, P, r, w, x, ldv, sv);                              \
     MNP_LAUNCH_CHECK();                                                                                            \
-    if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G1t, ldm, smat, piv1, spiv, x, ldv, sv, w, ldv, sv, batch, st)))  \
+    if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G1t, ldm, smat, piv1, spiv, x, ldv, sv, w, ldv, sv, batch, st,    \
+                            (const T*)D1, sD)))                                                                    \
       return rc;                                                                                                   \
     if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G2t, ldm, smat, piv2, spiv, x + 2 * 4 * P, ldv, sv,                \
-                            w + 2 * 4 * P, ldv, sv, batch, st)))                                                   \
+                            w + 2 * 4 * P, ldv, sv, batch, st, (const T*)D2, sD)))                                 \
       return rc;                                                                                                   \
     return 0;                                                                                                      \
   }
   if (dtype == MNP_DTYPE_C128) MNP_SOLVE(double)
-  if (dtype == MNP_DTYPE_C64) MNP_SOLVE(float)
+  if (mnp_is_c64(dtype)) MNP_SOLVE(float)
 #undef MNP_SOLVE
   mnp_set_error("bem_solve: unknown dtype");
   return 1;
@@ -732,7 +829,7 @@ extern "C" int mnpbem_accumulate(int src_dtype, long count, const void* src, dou
   hipStream_t st = (hipStream_t)stream;
   if (count <= 0) return 0;
   dim3 grid(mnp_ceil_div(count, 256));
-  if (src_dtype == MNP_DTYPE_C64)
+  if (mnp_is_c64(src_dtype))
     hipLaunchKernelGGL(k_axpy_cast<float>, grid, dim3(256), 0, st, count, (const float*)src, dst);
   else
     hipLaunchKernelGGL(k_axpy_cast<double>, grid, dim3(256), 0, st, count, (const double*)src, dst);
@@ -745,7 +842,7 @@ extern "C" int mnpbem_convert(int dst_dtype, long count, const double* src, void
   hipStream_t st = (hipStream_t)stream;
   if (count <= 0) return 0;
   dim3 grid(mnp_ceil_div(count, 256));
-  if (dst_dtype == MNP_DTYPE_C64)
+  if (mnp_is_c64(dst_dtype))
     hipLaunchKernelGGL(k_copy_cast<float>, grid, dim3(256), 0, st, count, src, (float*)dst);
   else
     hipLaunchKernelGGL(k_copy_cast<double>, grid, dim3(256), 0, st, count, src, (double*)dst);
@@ -775,6 +872,30 @@ extern "C" int mnpbem_extinction_batched(int n, int P, const double* X, long ldv
   if (batch <= 0) return 0;
   hipLaunchKernelGGL(k_extinction, dim3(P, batch), dim3(256), 0, st, n, P, X, ldv, sv, pos, area, in_med, out_med,
                      pol, dir, pw, ext);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Inverses of the 64 x 64 diagonal blocks of LU factors from
+// mnpbem_getrf_batched, for mnpbem_lu_solve_t_batched / mnpbem_bem_solve_batched:
+// Dinv[b] holds ceil(n/64) blocks of [U_II^-1 | L_II^-1] (2 x 64 x 64 each);
+// sD >= ceil(n/64) * 2 * 64 * 64 complex elements.
+extern "C" int mnpbem_lu_blockinv_batched(int dtype, int n, const void* LU, long ld, long sLU, void* Dinv, long sD,
+                                          int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (n <= 0 || batch <= 0) return 0;
+  const int nblk = (n + 63) / 64;
+  MNP_REQUIRE(sD >= (long)nblk * 2 * 64 * 64, "lu_blockinv: Dinv stride too small");
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_lu_blockinv<double>, dim3(nblk, batch), dim3(64), 0, st, n, (const double*)LU, ld, sLU,
+                       (double*)Dinv, sD);
+  else if (mnp_is_c64(dtype))
+    hipLaunchKernelGGL(k_lu_blockinv<float>, dim3(nblk, batch), dim3(64), 0, st, n, (const float*)LU, ld, sLU,
+                       (float*)Dinv, sD);
+  else {
+    mnp_set_error("lu_blockinv: unknown dtype");
+    return 1;
+  }
   MNP_LAUNCH_CHECK();
   return 0;
 }

@@ -63,7 +63,7 @@ def test_green_matrices_match_oracle(setup284, lam):
     assert g.eval(0, 1, "G", lam) == 0
 
 
-@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("mixed", 1e-9)])
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("mixed", 1e-9), ("mixed-fp32", 1e-9)])
 def test_bemret_extinction_matches_oracle(setup284, precision, tol):
     import mnpbem_amd as m
     cp, comp, ref = setup284

@@ -19,9 +19,11 @@ def _rand(rng, shape, dtype):
     return a.astype(dtype)
 
 
-@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-13), (np.complex64, 2e-5)])
-@pytest.mark.parametrize("M,N,K", [(64, 64, 16), (100, 37, 53), (300, 200, 129), (1, 1, 1), (17, 130, 5)])
-def test_cgemm_sub(cuda, dtype, tol, M, N, K):
+@pytest.mark.parametrize("dtype,tol,split", [(np.complex128, 1e-13, False), (np.complex64, 2e-5, False),
+                                             (np.complex64, 1e-4, True)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 16), (100, 37, 53), (300, 200, 129), (1, 1, 1), (17, 130, 5),
+                                   (257, 129, 1000)])
+def test_cgemm_sub(cuda, dtype, tol, split, M, N, K):
     nat = _lib()
     rng = np.random.default_rng(M * 1000 + N * 10 + K)
     B = 3
@@ -30,8 +32,8 @@ def test_cgemm_sub(cuda, dtype, tol, M, N, K):
     C = _rand(rng, (B, M, N), dtype)
     ref = C.astype(np.complex128) - A.astype(np.complex128) @ Bm.astype(np.complex128)
     tA, tB, tC = (torch.from_numpy(x).to(cuda) for x in (A, Bm, C))
-    dt = nat.DTYPE_C128 if dtype == np.complex128 else nat.DTYPE_C64
-    print("cgemm", dtype.__name__, M, N, K, flush=True)
+    dt = nat.DTYPE_C128 if dtype == np.complex128 else (nat.DTYPE_C64X if split else nat.DTYPE_C64)
+    print("cgemm", dtype.__name__, split, M, N, K, flush=True)
     nat.call("mnpbem_cgemm_sub_batched", dt, M, N, K, nat.ptr(tA), K, M * K, nat.ptr(tB), N, K * N, nat.ptr(tC), N,
              M * N, B, nat.stream_handle(cuda))
     out = tC.cpu().numpy()
@@ -39,9 +41,10 @@ def test_cgemm_sub(cuda, dtype, tol, M, N, K):
     assert err < tol, err
 
 
-@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 1e-4)])
+@pytest.mark.parametrize("dtype,tol,split", [(np.complex128, 1e-12, False), (np.complex64, 1e-4, False),
+                                             (np.complex64, 1e-3, True)])
 @pytest.mark.parametrize("n", [1, 7, 64, 100, 300, 1031])
-def test_getrf_getrs(cuda, dtype, tol, n):
+def test_getrf_getrs(cuda, dtype, tol, split, n):
     nat = _lib()
     rng = np.random.default_rng(n)
     B = 2
@@ -51,9 +54,9 @@ def test_getrf_getrs(cuda, dtype, tol, n):
     tA, tX = torch.from_numpy(A.copy()).to(cuda), torch.from_numpy(X.copy()).to(cuda)
     piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
     info = torch.empty((B,), dtype=torch.int32, device=cuda)
-    dt = nat.DTYPE_C128 if dtype == np.complex128 else nat.DTYPE_C64
+    dt = nat.DTYPE_C128 if dtype == np.complex128 else (nat.DTYPE_C64X if split else nat.DTYPE_C64)
     st = nat.stream_handle(cuda)
-    print("getrf", dtype.__name__, n, flush=True)
+    print("getrf", dtype.__name__, split, n, flush=True)
     nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
     nat.call("mnpbem_getrs_batched", dt, n, n + 3, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(tX), n + 3,
              n * (n + 3), B, st)
@@ -93,3 +96,35 @@ def test_singular_matrix_reports_info(cuda):
     nat.call("mnpbem_getrf_batched", nat.DTYPE_C128, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), 1,
              nat.stream_handle(cuda))
     assert int(info.cpu()[0]) == 18
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 1e-4)])
+@pytest.mark.parametrize("n,C", [(5, 3), (64, 8), (130, 2), (700, 16), (701, 19)])
+@pytest.mark.parametrize("blockinv", [False, True])
+def test_lu_solve_transposed(cuda, dtype, tol, n, C, blockinv):
+    """Solve M x = v from the LU of the buffer M^T (BEMRet's storage), with and
+    without the precomputed inverse diagonal blocks."""
+    nat = _lib()
+    rng = np.random.default_rng(n + C)
+    B = 2
+    M = (_rand(rng, (B, n, n), dtype) + (2.0 * np.sqrt(n)) * np.eye(n)[None]).astype(dtype)
+    V = _rand(rng, (B, n, C), dtype)
+    ref = np.linalg.solve(M.astype(np.complex128), V.astype(np.complex128))
+    tA = torch.from_numpy(np.ascontiguousarray(np.transpose(M, (0, 2, 1)))).to(cuda)
+    tV = torch.from_numpy(V.copy()).to(cuda)
+    tZ = torch.empty_like(tV)
+    piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((B,), dtype=torch.int32, device=cuda)
+    dt = nat.DTYPE_C128 if dtype == np.complex128 else nat.DTYPE_C64
+    st = nat.stream_handle(cuda)
+    print("lu_solve_t", dtype.__name__, n, C, blockinv, flush=True)
+    nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
+    D, sD = None, 0
+    if blockinv:
+        sD = ((n + 63) // 64) * 2 * 64 * 64
+        D = torch.empty((B, sD), dtype=tA.dtype, device=cuda)
+        nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(tA), n, n * n, nat.ptr(D), sD, B, st)
+    nat.call("mnpbem_lu_solve_t_batched", dt, n, C, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(tV), C, n * C,
+             nat.ptr(tZ), C, n * C, nat.ptr(D), sD, B, st)
+    err = np.abs(tV.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < tol, err
