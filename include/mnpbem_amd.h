@@ -163,6 +163,68 @@ int mnpbem_extinction_batched(int n, int P, const double* X, long ldv, long sv, 
                               const double* area, const int* in_med, const int* out_med, const double* pol,
                               const double* dir, const double* pw, double* ext, int batch, void* stream);
 
+/* Combination step of the 8N operator from precomputed products
+ * Y = [G1 X1 | H1 X1 | G2 X2 | H2 X2] (n x 16P per wavelength, complex128):
+ * Out = R - A X, or Out = A X when R is NULL; nrm as for
+ * mnpbem_bem_residual_batched or NULL (the tail of BEMRetIter._afun,
+ * bem_ret_iter.py:1440-1501, after its four Green matvecs). */
+int mnpbem_bem_combine_batched(int n, int P, const void* Y, long ldy, long sy, const void* R, void* Out, long ldv,
+                               long sv, const double* nvec, const double* par, double* nrm, int batch, void* stream);
+
+/* ---- GMRES vector kernels (replace the Arnoldi / Givens vector algebra
+ *      of scipy.sparse.linalg.gmres called at mnpbem/bem/bem_iter.py:79-89) ----
+ * A multi-vector holds batch x P systems: L = n * 8P complex128 values per
+ * wavelength, system p owning the elements e with e % P == p.  Vector j of a
+ * basis starts sVk elements after vector j-1; wavelength b starts sVb (sWb,
+ * sOb) elements after b-1.  P <= 16. */
+/* out[b][j][p] = sum conj(V_j) W over system (b, p), j < k; work holds
+ * >= batch * k * 64 * P complex128 (fixed-order partial sums). */
+int mnpbem_mv_dot_batched(long L, int P, int k, const void* V, long sVk, long sVb, const void* W, long sWb,
+                          void* out, void* work, int batch, void* stream);
+/* W += sign * sum_{j<k} coef[b][j][p] V_j  (coef: batch x k x P complex128) */
+int mnpbem_mv_update_batched(long L, int P, int k, const void* V, long sVk, long sVb, const void* coef, double sign,
+                             void* W, long sWb, int batch, void* stream);
+/* mode 0: out = W * s[b][p] (complex); mode 1: out = W / sqrt(Re s[b][p]),
+ * 0 where Re s <= 0.  out may alias W. */
+int mnpbem_mv_scale_batched(long L, int P, int mode, const void* s, const void* W, long sWb, void* out, long sOb,
+                            int batch, void* stream);
+/* dst = (double) src, count reals (src complex64 or complex128) */
+int mnpbem_widen(int src_dtype, long count, const void* src, double* dst, void* stream);
+
+/* ---- H-matrices (replace HMatrix.aca / _aca_block / mtimes_vec / full,
+ *      mnpbem/greenfun/hmatrix.py:159-541, with entries taken from the
+ *      assembled Green matrix as in aca_compgreen_ret.py:164-193) ----
+ * perm: cluster position -> face (ClusterTree.ind[:, 0], int32).  Block
+ * table blk: int64 x 8 per block {r0, m, c0, n, rank, off_a, off_b, off_t}
+ * (cluster rows [r0, r0+m), columns [c0, c0+n); rank < 0: dense block,
+ * column-major values at val + off_a; rank >= 0: factors U (m x rank, at
+ * U + off_a) and V (n x rank, at V + off_b), column-major, block = U V^T;
+ * off_t = sum of the ranks of the preceding low-rank blocks).  Leaf table
+ * leafs: int64 x 4 per leaf cluster {r0, m, first, last}; lst[first:last]
+ * (int32) lists the blocks whose rows contain the leaf.  src is a complex128
+ * matrix with element strides: M[i][j] = src[i * si + j * sj].  Block values
+ * are complex64 (dtype 1) or complex128 (dtype 2). */
+/* partially pivoted ACA (FP64) of blocks bids[0:nb]; workspace factors of
+ * block l at Uw + woff[2l], Vw + woff[2l+1] with min(m, n, kmax) columns;
+ * rank_out[l] = rank (-1 if a side exceeds 65536).  kmax <= 512. */
+int mnpbem_hmat_aca(int nb, const long* blk, const int* bids, const int* perm, const void* src, long si, long sj,
+                    void* Uw, void* Vw, const long* woff, int kmax, double htol, int* rank_out, void* stream);
+/* dense blocks bids[0:nb] copied from src into val */
+int mnpbem_hmat_dense(int dtype, int nb, const long* blk, const int* bids, const int* perm, const void* src, long si,
+                      long sj, void* val, void* stream);
+/* packed U, V of the low-rank blocks bids[0:nb] (ranks in blk) from the ACA workspaces */
+int mnpbem_hmat_compact(int dtype, int nb, const long* blk, const int* bids, const void* Uw, const void* Vw,
+                        const long* woff, void* U, void* V, void* stream);
+/* Y (+)= M X, X and Y complex128 (n x ldx / ldy, face order), ncol <= 16;
+ * T: complex128 work space of (sum of ranks) * ncol; maxchunks =
+ * ceil(largest leaf / 256). */
+int mnpbem_hmat_matvec(int dtype, int nblk, int nleaf, int maxchunks, const long* blk, const long* leafs,
+                       const int* lst, const int* perm, const void* val, const void* U, const void* V, int ncol,
+                       const void* X, long ldx, void* Y, long ldy, int accumulate, void* T, void* stream);
+/* Out = M as an n x n complex128 row-major matrix (face order, leading dimension ldo) */
+int mnpbem_hmat_full(int dtype, int nblk, const long* blk, const int* perm, const void* val, const void* U,
+                     const void* V, void* Out, long ldo, void* stream);
+
 /* ---- measurement: HIP-event timing of every complex-GEMM launch ----
  * (host pointers for the results of mnpbem_prof_end) */
 int mnpbem_prof_begin(int capacity);

@@ -2,7 +2,8 @@
 
 Public API mirrors mnpbem: materials (EpsConst, EpsDrude, EpsTable),
 geometry (trisphere, Particle, ComParticle), CompGreenRet, BEMRet,
-PlaneWaveRet and CompStruct.  Device work runs in libmnpbem_amd.so
+BEMRetIter (GMRES, optionally over ACA H-matrices: ClusterTree, HMatrix,
+ACACompGreenRet), PlaneWaveRet and CompStruct.  Device work runs in libmnpbem_amd.so
 (built by ``python -m mnpbem_amd.build``); there is no CPU fallback.
 """
 
@@ -10,7 +11,8 @@ from .materials import EV2NM, EpsConst, EpsDrude, EpsTable
 from .geometry import ComParticle, Particle, trisphere
 
 __all__ = ["EV2NM", "EpsConst", "EpsDrude", "EpsTable", "ComParticle", "Particle", "trisphere",
-           "CompGreenRet", "BEMRet", "CompStruct", "PlaneWaveRet"]
+           "CompGreenRet", "BEMRet", "CompStruct", "PlaneWaveRet", "BEMIter", "BEMRetIter", "ClusterTree",
+           "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss"]
 
 
 def __getattr__(name):
@@ -22,6 +24,12 @@ def __getattr__(name):
     if name in ("BEMRet", "CompStruct"):
         from . import bem
         return getattr(bem, name)
+    if name in ("BEMIter", "BEMRetIter"):
+        from . import bem_iter
+        return getattr(bem_iter, name)
+    if name in ("ClusterTree", "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss"):
+        from . import hmatrix
+        return getattr(hmatrix, name)
     if name == "PlaneWaveRet":
         from .planewave import PlaneWaveRet
         return PlaneWaveRet

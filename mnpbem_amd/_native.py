@@ -43,6 +43,16 @@ SIGNATURES = {
     "mnpbem_convert": [_i, _l, _p, _p, _p],
     "mnpbem_planewave_batched": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _p, _l, _i, _p],
     "mnpbem_extinction_batched": [_i, _i, _p, _l, _l, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p],
+    "mnpbem_bem_combine_batched": [_i, _i, _p, _l, _l, _p, _p, _l, _l, _p, _p, _p, _i, _p],
+    "mnpbem_widen": [_i, _l, _p, _p, _p],
+    "mnpbem_mv_dot_batched": [_l, _i, _i, _p, _l, _l, _p, _l, _p, _p, _i, _p],
+    "mnpbem_mv_update_batched": [_l, _i, _i, _p, _l, _l, _p, _d, _p, _l, _i, _p],
+    "mnpbem_mv_scale_batched": [_l, _i, _i, _p, _p, _l, _p, _l, _i, _p],
+    "mnpbem_hmat_aca": [_i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _i, _d, _p, _p],
+    "mnpbem_hmat_dense": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p],
+    "mnpbem_hmat_compact": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p],
+    "mnpbem_hmat_matvec": [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p, _l, _p, _l, _i, _p, _p],
+    "mnpbem_hmat_full": [_i, _i, _p, _p, _p, _p, _p, _p, _l, _p],
     "mnpbem_prof_begin": [_i],
     "mnpbem_prof_end": [_p, _p, _p],
 }
@@ -83,6 +93,12 @@ def call(name, *args):
 def ptr(t):
     """Raw device (or host) address of a torch tensor, or None."""
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def ptr_off(t, elems):
+    """Address of element ``elems`` (counted in t's element type) of a
+    contiguous torch tensor."""
+    return ctypes.c_void_p(t.data_ptr() + int(elems) * t.element_size())
 
 
 def stream_handle(device=None):

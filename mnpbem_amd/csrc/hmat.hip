@@ -1,0 +1,516 @@
+// Hierarchical (H-) matrices of the Green functions: adaptive cross
+// approximation of the admissible blocks, dense leaf blocks, block matvec.
+//
+// Reference: mnpbem/greenfun/hmatrix.py (HMatrix.aca :159-193, _aca_block
+// :195-336, mtimes_vec :450-541, full :338-448), mnpbem/greenfun/
+// aca_compgreen_ret.py:96-208 (ACA entries are entries of the assembled
+// Green matrix), mnpbem/greenfun/clustertree.py (cluster ordering).
+//
+// Layout.  The cluster tree orders the faces: cluster position c holds face
+// perm[c].  A block covers cluster rows [r0, r0 + m) and columns [c0, c0 + n).
+// Block table `blk` (int64, 8 per block):
+//   {r0, m, c0, n, rank, off_a, off_b, off_t}
+// rank < 0 marks a dense block whose values sit column-major at val + off_a
+// (val[off_a + j m + i] = M[perm[r0+i], perm[c0+j]]); rank >= 0 marks a
+// low-rank block M ~ U V^T with U (m x rank) at U + off_a and V (n x rank) at
+// V + off_b, both column-major; off_t is the running sum of ranks (the block's
+// row offset in the matvec workspace).  Leaf table `leafs` (int64, 4 per leaf
+// cluster): {r0, m, first, last} -> lst[first:last] lists the blocks whose
+// row range contains the leaf (dense blocks of the leaf row and every
+// low-rank block over it).  Values are complex64 (dtype 1) or complex128
+// (dtype 2); vectors and all arithmetic are complex128.
+#include "common.h"
+
+namespace {
+
+constexpr int ACA_KMAX = 512;      // largest rank bound an ACA launch accepts
+constexpr int ACA_MAXMN = 65536;   // largest block side (bitmask capacity)
+
+struct Blk {
+  long r0, m, c0, n, rank, oa, ob, ot;
+};
+__device__ __forceinline__ Blk load_blk(const long* blk, long b) {
+  const long* q = blk + 8 * b;
+  Blk x;
+  x.r0 = q[0];
+  x.m = q[1];
+  x.c0 = q[2];
+  x.n = q[3];
+  x.rank = q[4];
+  x.oa = q[5];
+  x.ob = q[6];
+  x.ot = q[7];
+  return x;
+}
+
+__device__ __forceinline__ cplx<double> ld2(const double* p) {
+  const double2 v = *reinterpret_cast<const double2*>(p);
+  return cmake(v.x, v.y);
+}
+template <typename T>
+__device__ __forceinline__ cplx<double> ldw(const T* p) {
+  const cplx<T> v = cld(p);
+  return cmake((double)v.re, (double)v.im);
+}
+
+// (|v|, index, v) candidate for the pivot searches; ties keep the lower index
+struct Cand {
+  double a;
+  int idx;
+  double re, im;
+};
+__device__ __forceinline__ bool better(double a, int i, double b, int j) { return a > b || (a == b && i < j); }
+
+__device__ Cand block_argmax(Cand c, Cand* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double oa = __shfl_xor(c.a, off);
+    const int oi = __shfl_xor(c.idx, off);
+    const double ore = __shfl_xor(c.re, off);
+    const double oim = __shfl_xor(c.im, off);
+    if (better(oa, oi, c.a, c.idx)) {
+      c.a = oa;
+      c.idx = oi;
+      c.re = ore;
+      c.im = oim;
+    }
+  }
+  if (lane == 0) sh[w] = c;
+  __syncthreads();
+  Cand r = sh[0];
+  for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+    if (better(sh[q].a, sh[q].idx, r.a, r.idx)) r = sh[q];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ cplx<double> wave_sum(cplx<double> v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    v.re += __shfl_xor(v.re, off);
+    v.im += __shfl_xor(v.im, off);
+  }
+  return v;
+}
+
+// Partially pivoted ACA of one admissible block per workgroup, FP64
+// (hmatrix.py:195-336): pivot row p (first 0), residual row, pivot column =
+// argmax |row| with used columns read as 0, stop when |pivot| < 1e-15 or
+// ||u_k|| ||v_k|| < htol ||sum_r u_r v_r^T||_F (norm updated with the cross
+// terms 2 Re sum_s (u_s^H u_k)(v_s^H v_k)), next row = argmax |u_k| with
+// used rows read as 0.  Entries come from an assembled complex128 matrix
+// with element strides (si, sj): M[i][j] = src[i si + j sj] (the solver's
+// transposed buffers have si = 1, sj = n).  U, V go to a workspace with
+// per-block offsets woff[2 b], woff[2 b + 1] and column stride m (n).
+__global__ __launch_bounds__(256) void k_aca(const long* blk, const int* bids, const int* perm,
+                                             const double* src, long si, long sj, double* Uw, double* Vw,
+                                             const long* woff, int kmax, double htol, int* rank_out) {
+  const int tid = threadIdx.x;
+  const int lb = blockIdx.x;
+  const Blk B = load_blk(blk, bids[lb]);
+  if (B.m > ACA_MAXMN || B.n > ACA_MAXMN || B.m < 1 || B.n < 1) {  // outside the bitmask capacity
+    if (threadIdx.x == 0) rank_out[lb] = -1;
+    return;
+  }
+  const int m = (int)B.m, n = (int)B.n;
+  double* U = Uw + 2 * woff[2 * lb];
+  double* V = Vw + 2 * woff[2 * lb + 1];
+  const int* prow = perm + B.r0;
+  const int* pcol = perm + B.c0;
+  int maxr = m < n ? m : n;
+  if (kmax < maxr) maxr = kmax;
+
+  __shared__ unsigned usedr[ACA_MAXMN / 32], usedc[ACA_MAXMN / 32];
+  __shared__ double uk[2 * ACA_KMAX], vq[2 * ACA_KMAX];
+  __shared__ double du[2 * ACA_KMAX], dv[2 * ACA_KMAX];
+  __shared__ Cand sh[4];
+  __shared__ int s_stop;
+  for (int w = tid; w < (m + 31) / 32; w += blockDim.x) usedr[w] = 0u;
+  for (int w = tid; w < (n + 31) / 32; w += blockDim.x) usedc[w] = 0u;
+  double frob = 0.0;  // meaningful in thread 0
+  int p = 0, rank = 0;
+  __syncthreads();
+
+  for (int k = 0; k < maxr; ++k) {
+    for (int r = tid; r < k; r += blockDim.x) {
+      const cplx<double> z = ld2(U + 2 * ((long)r * m + p));
+      uk[2 * r] = z.re;
+      uk[2 * r + 1] = z.im;
+    }
+    __syncthreads();
+    // residual row p -> V[:, k]
+    const long rowoff = (long)prow[p] * si;
+    Cand c;
+    c.a = -1.0;
+    c.idx = n;
+    c.re = c.im = 0.0;
+    for (int j = tid; j < n; j += blockDim.x) {
+      cplx<double> v = ld2(src + 2 * ((long)pcol[j] * sj + rowoff));
+      for (int r = 0; r < k; ++r) v = cfms(v, cmake(uk[2 * r], uk[2 * r + 1]), ld2(V + 2 * ((long)r * n + j)));
+      double* o = V + 2 * ((long)k * n + j);
+      o[0] = v.re;
+      o[1] = v.im;
+      const double a = ((usedc[j >> 5] >> (j & 31)) & 1u) ? 0.0 : hypot(v.re, v.im);
+      if (better(a, j, c.a, c.idx)) {
+        c.a = a;
+        c.idx = j;
+        c.re = v.re;
+        c.im = v.im;
+      }
+    }
+    const Cand pc = block_argmax(c, sh);
+    if (hypot(pc.re, pc.im) < 1e-15) break;  // uniform across the workgroup
+    const int q = pc.idx;
+    for (int r = tid; r < k; r += blockDim.x) {
+      const cplx<double> z = ld2(V + 2 * ((long)r * n + q));
+      vq[2 * r] = z.re;
+      vq[2 * r + 1] = z.im;
+    }
+    if (tid == 0) {
+      usedr[p >> 5] |= 1u << (p & 31);
+      usedc[q >> 5] |= 1u << (q & 31);
+    }
+    __syncthreads();
+    // residual column q / pivot -> U[:, k]
+    const cplx<double> inv = crecip(cmake(pc.re, pc.im));
+    const long coloff = (long)pcol[q] * sj;
+    Cand cr;
+    cr.a = -1.0;
+    cr.idx = m;
+    cr.re = cr.im = 0.0;
+    for (int i = tid; i < m; i += blockDim.x) {
+      cplx<double> v = ld2(src + 2 * (coloff + (long)prow[i] * si));
+      for (int r = 0; r < k; ++r) v = cfms(v, ld2(U + 2 * ((long)r * m + i)), cmake(vq[2 * r], vq[2 * r + 1]));
+      const cplx<double> u = cmul(v, inv);
+      double* o = U + 2 * ((long)k * m + i);
+      o[0] = u.re;
+      o[1] = u.im;
+      const double a = ((usedr[i >> 5] >> (i & 31)) & 1u) ? 0.0 : hypot(u.re, u.im);
+      if (better(a, i, cr.a, cr.idx)) {
+        cr.a = a;
+        cr.idx = i;
+      }
+    }
+    const Cand pr = block_argmax(cr, sh);  // its barrier orders the U[:, k] writes before the dots
+    rank = k + 1;
+    // dots u_s^H u_k and v_s^H v_k for s <= k, one wave per s
+    const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+    for (int s = wv; s <= k; s += nw) {
+      cplx<double> a = cmake(0.0, 0.0), b = cmake(0.0, 0.0);
+      for (int i = lane; i < m; i += 64) {
+        const cplx<double> x = ld2(U + 2 * ((long)s * m + i)), y = ld2(U + 2 * ((long)k * m + i));
+        a.re += x.re * y.re + x.im * y.im;
+        a.im += x.re * y.im - x.im * y.re;
+      }
+      for (int j = lane; j < n; j += 64) {
+        const cplx<double> x = ld2(V + 2 * ((long)s * n + j)), y = ld2(V + 2 * ((long)k * n + j));
+        b.re += x.re * y.re + x.im * y.im;
+        b.im += x.re * y.im - x.im * y.re;
+      }
+      a = wave_sum(a);
+      b = wave_sum(b);
+      if (lane == 0) {
+        du[2 * s] = a.re;
+        du[2 * s + 1] = a.im;
+        dv[2 * s] = b.re;
+        dv[2 * s + 1] = b.im;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const double nt = du[2 * k] * dv[2 * k];
+      double cross = 0.0;
+      for (int s = 0; s < k; ++s) cross += du[2 * s] * dv[2 * s] - du[2 * s + 1] * dv[2 * s + 1];
+      frob += nt + 2.0 * cross;
+      s_stop = (frob > 0.0 && sqrt(nt) < htol * sqrt(fabs(frob))) ? 1 : 0;
+    }
+    __syncthreads();
+    if (s_stop) break;
+    p = pr.idx;
+  }
+  if (tid == 0) rank_out[lb] = rank;
+}
+
+// dense blocks: val[off_a + j m + i] = M[perm[r0+i]][perm[c0+j]]
+template <typename TS>
+__global__ __launch_bounds__(256) void k_hmat_dense(const long* blk, const int* bids, const int* perm,
+                                                    const double* src, long si, long sj, TS* val) {
+  const Blk B = load_blk(blk, bids[blockIdx.x]);
+  const long mn = B.m * B.n;
+  for (long e = threadIdx.x; e < mn; e += blockDim.x) {
+    const long j = e / B.m, i = e - j * B.m;
+    const cplx<double> v = ld2(src + 2 * ((long)perm[B.c0 + j] * sj + (long)perm[B.r0 + i] * si));
+    cst(val + 2 * (B.oa + e), cmake<TS>((TS)v.re, (TS)v.im));
+  }
+}
+
+// low-rank blocks: copy the first `rank` columns of the ACA workspace into
+// the packed factors (rank recorded in blk)
+template <typename TS>
+__global__ __launch_bounds__(256) void k_hmat_compact(const long* blk, const int* bids, const double* Uw,
+                                                      const double* Vw, const long* woff, TS* U, TS* V) {
+  const int lb = blockIdx.x;
+  const Blk B = load_blk(blk, bids[lb]);
+  const double* u = Uw + 2 * woff[2 * lb];
+  const double* v = Vw + 2 * woff[2 * lb + 1];
+  const long nu = B.m * B.rank, nv = B.n * B.rank;
+  for (long e = threadIdx.x; e < nu; e += blockDim.x) cst(U + 2 * (B.oa + e), cmake<TS>((TS)u[2 * e], (TS)u[2 * e + 1]));
+  for (long e = threadIdx.x; e < nv; e += blockDim.x) cst(V + 2 * (B.ob + e), cmake<TS>((TS)v[2 * e], (TS)v[2 * e + 1]));
+}
+
+// matvec phase 1: t_b = V_b^T x[cols of b] for every low-rank block (one wave per rank column)
+template <typename TS, int NC>
+__global__ __launch_bounds__(256) void k_hmat_vt(const long* blk, const int* perm, const TS* Vv, const double* X,
+                                                 long ldx, int ncol, double* T) {
+  const Blk B = load_blk(blk, blockIdx.x);
+  if (B.rank <= 0) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (long r = wv; r < B.rank; r += 4) {
+    cplx<double> acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = cmake(0.0, 0.0);
+    const TS* v = Vv + 2 * (B.ob + r * B.n);
+    for (long j = lane; j < B.n; j += 64) {
+      const cplx<double> a = ldw(v + 2 * j);
+      const double* x = X + 2 * ((long)perm[B.c0 + j] * ldx);
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c < ncol) acc[c] = cfma(acc[c], a, ld2(x + 2 * c));
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c < ncol) {
+        const cplx<double> s = wave_sum(acc[c]);
+        if (lane == 0) {
+          double* o = T + 2 * ((B.ot + r) * ncol + c);
+          o[0] = s.re;
+          o[1] = s.im;
+        }
+      }
+    }
+  }
+}
+
+// matvec phase 2: one workgroup per (leaf row cluster, 256-row chunk); the
+// waves share the leaf's rows in groups of 64 and split its block list; the
+// group's partial sums meet in LDS and row i of the leaf is written once, to
+// Y[perm[r0 + i]] (deterministic, no atomics).
+template <typename TS, int NC>
+__global__ __launch_bounds__(256) void k_hmat_rows(const long* leafs, const int* lst, const long* blk,
+                                                   const int* perm, const TS* val, const TS* Uv, const double* T,
+                                                   const double* X, long ldx, int ncol, double* Y, long ldy,
+                                                   int accumulate) {
+  const long* lf = leafs + 4 * blockIdx.x;
+  const long lr0 = lf[0], lm = lf[1], first = lf[2], last = lf[3];
+  const long i0 = (long)blockIdx.y * 256;
+  if (i0 >= lm) return;  // uniform
+  const int mc = (int)(lm - i0 < 256 ? lm - i0 : 256);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int RG = (mc + 63) / 64;  // row groups of 64
+  const int WPG = 4 / RG;         // waves per row group
+  const int g = wv / WPG, sub = wv % WPG;
+  const bool active = g < RG;
+  const long li = i0 + (long)g * 64 + lane;  // row within the leaf
+  const bool rowok = active && (g * 64 + lane) < mc;
+  cplx<double> acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = cmake(0.0, 0.0);
+  if (active) {
+    for (long e = first + sub; e < last; e += WPG) {
+      const Blk B = load_blk(blk, lst[e]);
+      if (B.rank < 0) {
+        const TS* a = val + 2 * (B.oa + li);
+        for (long j = 0; j < B.n; ++j) {
+          const double* x = X + 2 * ((long)perm[B.c0 + j] * ldx);
+          if (rowok) {
+            const cplx<double> v = ldw(a + 2 * j * B.m);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              if (c < ncol) acc[c] = cfma(acc[c], v, ld2(x + 2 * c));
+          }
+        }
+      } else {
+        const long lrow = lr0 + li - B.r0;  // row inside the low-rank block
+        const TS* u = Uv + 2 * (B.oa + lrow);
+        for (long r = 0; r < B.rank; ++r) {
+          const double* t = T + 2 * ((B.ot + r) * ncol);
+          if (rowok) {
+            const cplx<double> v = ldw(u + 2 * r * B.m);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              if (c < ncol) acc[c] = cfma(acc[c], v, ld2(t + 2 * c));
+          }
+        }
+      }
+    }
+  }
+  __shared__ double red[4][64][NC][2];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    red[wv][lane][c][0] = acc[c].re;
+    red[wv][lane][c][1] = acc[c].im;
+  }
+  __syncthreads();
+  if (rowok && sub == 0) {
+    double* y = Y + 2 * ((long)perm[lr0 + li] * ldy);
+    for (int c = 0; c < ncol; ++c) {
+      double sr = 0.0, si = 0.0;
+      for (int q = 0; q < WPG; ++q) {
+        sr += red[wv + q][lane][c][0];
+        si += red[wv + q][lane][c][1];
+      }
+      if (accumulate) {
+        sr += y[2 * c];
+        si += y[2 * c + 1];
+      }
+      y[2 * c] = sr;
+      y[2 * c + 1] = si;
+    }
+  }
+}
+
+// densify: Out[perm[r0+i]][perm[c0+j]] (row-major, ldo) for every block
+template <typename TS>
+__global__ __launch_bounds__(256) void k_hmat_full(const long* blk, const int* perm, const TS* val, const TS* Uv,
+                                                   const TS* Vv, double* Out, long ldo) {
+  const Blk B = load_blk(blk, blockIdx.x);
+  const long mn = B.m * B.n;
+  for (long e = threadIdx.x; e < mn; e += blockDim.x) {
+    const long j = e / B.m, i = e - j * B.m;
+    cplx<double> s;
+    if (B.rank < 0) {
+      s = ldw(val + 2 * (B.oa + e));
+    } else {
+      s = cmake(0.0, 0.0);
+      for (long r = 0; r < B.rank; ++r)
+        s = cfma(s, ldw(Uv + 2 * (B.oa + r * B.m + i)), ldw(Vv + 2 * (B.ob + r * B.n + j)));
+    }
+    double* o = Out + 2 * ((long)perm[B.r0 + i] * ldo + perm[B.c0 + j]);
+    o[0] = s.re;
+    o[1] = s.im;
+  }
+}
+
+template <typename TS, int NC>
+int matvec_launch(int nblk, int nleaf, int maxchunks, const long* blk, const long* leafs, const int* lst,
+                  const int* perm, const void* val, const void* U, const void* V, int ncol, const double* X,
+                  long ldx, double* Y, long ldy, int accumulate, double* T, hipStream_t st) {
+  if (nblk > 0) {
+    hipLaunchKernelGGL((k_hmat_vt<TS, NC>), dim3(nblk), dim3(256), 0, st, blk, perm, (const TS*)V, X, ldx, ncol, T);
+    MNP_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL((k_hmat_rows<TS, NC>), dim3(nleaf, maxchunks), dim3(256), 0, st, leafs, lst, blk, perm,
+                     (const TS*)val, (const TS*)U, (const double*)T, X, ldx, ncol, Y, ldy, accumulate);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace
+
+// ACA of the low-rank blocks bids[0:nb] of the block table, entries from the
+// complex128 matrix M[i][j] = src[i si + j sj] (element strides).  Uw / Vw:
+// complex128 workspaces, block l's factors at woff[2 l] / woff[2 l + 1] with
+// room for min(m, n, kmax) columns.  rank_out[l] receives the rank.
+extern "C" int mnpbem_hmat_aca(int nb, const long* blk, const int* bids, const int* perm, const void* src, long si,
+                               long sj, void* Uw, void* Vw, const long* woff, int kmax, double htol, int* rank_out,
+                               void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(kmax >= 1 && kmax <= ACA_KMAX, "hmat_aca: kmax must be in [1, 512]");
+  MNP_REQUIRE(htol >= 0.0, "hmat_aca: htol < 0");
+  if (nb <= 0) return 0;
+  hipLaunchKernelGGL(k_aca, dim3(nb), dim3(256), 0, st, blk, bids, perm, (const double*)src, si, sj, (double*)Uw,
+                     (double*)Vw, woff, kmax, htol, rank_out);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// dense blocks bids[0:nb]: copy from the complex128 matrix M[i][j] = src[i si + j sj] into val (dtype)
+extern "C" int mnpbem_hmat_dense(int dtype, int nb, const long* blk, const int* bids, const int* perm,
+                                 const void* src, long si, long sj, void* val, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (nb <= 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_hmat_dense<double>, dim3(nb), dim3(256), 0, st, blk, bids, perm, (const double*)src, si,
+                       sj, (double*)val);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_hmat_dense<float>, dim3(nb), dim3(256), 0, st, blk, bids, perm, (const double*)src, si,
+                       sj, (float*)val);
+  else {
+    mnp_set_error("hmat_dense: dtype must be 1 or 2");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// low-rank blocks bids[0:nb]: packed U, V (dtype) from the ACA workspaces
+extern "C" int mnpbem_hmat_compact(int dtype, int nb, const long* blk, const int* bids, const void* Uw,
+                                   const void* Vw, const long* woff, void* U, void* V, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (nb <= 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_hmat_compact<double>, dim3(nb), dim3(256), 0, st, blk, bids, (const double*)Uw,
+                       (const double*)Vw, woff, (double*)U, (double*)V);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_hmat_compact<float>, dim3(nb), dim3(256), 0, st, blk, bids, (const double*)Uw,
+                       (const double*)Vw, woff, (float*)U, (float*)V);
+  else {
+    mnp_set_error("hmat_compact: dtype must be 1 or 2");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Y (+)= M X for the H-matrix (hmatrix.py:450-541).  X, Y: (n, ldx / ldy)
+// complex128 row-major in face order, ncol <= 16 columns used.  T: complex128
+// workspace of (sum of ranks) * ncol elements.  maxchunks = ceil(max leaf
+// size / 256).
+extern "C" int mnpbem_hmat_matvec(int dtype, int nblk, int nleaf, int maxchunks, const long* blk, const long* leafs,
+                                  const int* lst, const int* perm, const void* val, const void* U, const void* V,
+                                  int ncol, const void* X, long ldx, void* Y, long ldy, int accumulate, void* T,
+                                  void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(ncol >= 1 && ncol <= 16 && ldx >= ncol && ldy >= ncol, "hmat_matvec: 1 <= ncol <= 16");
+  MNP_REQUIRE(nleaf >= 1 && maxchunks >= 1 && maxchunks <= 65535, "hmat_matvec: bad leaf table");
+  const double* x = (const double*)X;
+  double* y = (double*)Y;
+  double* t = (double*)T;
+#define MNP_MV(TS, NC)                                                                                          \
+  return matvec_launch<TS, NC>(nblk, nleaf, maxchunks, blk, leafs, lst, perm, val, U, V, ncol, x, ldx, y, ldy, \
+                               accumulate, t, st)
+  if (dtype == MNP_DTYPE_C128) {
+    if (ncol <= 4) MNP_MV(double, 4);
+    if (ncol <= 8) MNP_MV(double, 8);
+    MNP_MV(double, 16);
+  }
+  if (dtype == MNP_DTYPE_C64) {
+    if (ncol <= 4) MNP_MV(float, 4);
+    if (ncol <= 8) MNP_MV(float, 8);
+    MNP_MV(float, 16);
+  }
+#undef MNP_MV
+  mnp_set_error("hmat_matvec: dtype must be 1 or 2");
+  return 1;
+}
+
+// Out (n x n complex128, row-major, face order, leading dimension ldo) = the
+// H-matrix (hmatrix.py:338-448)
+extern "C" int mnpbem_hmat_full(int dtype, int nblk, const long* blk, const int* perm, const void* val,
+                                const void* U, const void* V, void* Out, long ldo, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (nblk <= 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_hmat_full<double>, dim3(nblk), dim3(256), 0, st, blk, perm, (const double*)val,
+                       (const double*)U, (const double*)V, (double*)Out, ldo);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_hmat_full<float>, dim3(nblk), dim3(256), 0, st, blk, perm, (const float*)val,
+                       (const float*)U, (const float*)V, (double*)Out, ldo);
+  else {
+    mnp_set_error("hmat_full: dtype must be 1 or 2");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}

@@ -458,7 +458,8 @@ __global__ void k_step_pack(int n, int P, const T* R, const T* W, T* X, long ld,
 //   phi = G1 s1 - G2 s2;  a = G1 h1 - G2 h2
 //   alpha = H1 h1 - H2 h2 - ik n (e1 G1 s1 - e2 G2 s2)
 //   De = e1 H1 s1 - e2 H2 s2 - ik n.(e1 G1 h1 - e2 G2 h2)
-// Rout = R - A X; nrm[b] += |Rout|^2 and |R|^2 (atomic per wavelength)
+// Rout = R - A X; nrm[b] += |Rout|^2 and |R|^2 (atomic per wavelength) when nrm
+// is given; with R == nullptr, Rout = A X.
 __global__ void k_residual(int n, int P, const double* R, const double* Y, double* Rout, long ldr, long sr,
                            long ldy, long sy, const double* nvec, const double* par, double* nrm) {
   const long b = blockIdx.y;
@@ -466,7 +467,7 @@ __global__ void k_residual(int n, int P, const double* R, const double* Y, doubl
   double acc_r = 0.0, acc_b = 0.0;
   if (i < n) {
     const Par q = load_par(par, b);
-    const double* r = R + 2 * (b * sr + (long)i * ldr);
+    const double* r = R ? R + 2 * (b * sr + (long)i * ldr) : nullptr;
     const double* y = Y + 2 * (b * sy + (long)i * ldy);
     double* o = Rout + 2 * (b * sr + (long)i * ldr);
     const int P4 = 4 * P;
@@ -475,6 +476,10 @@ __global__ void k_residual(int n, int P, const double* R, const double* Y, doubl
     auto G2 = [&](int col) { return ldd(y + 2 * (2 * P4 + col)); };
     auto H2 = [&](int col) { return ldd(y + 2 * (3 * P4 + col)); };
     auto put = [&](int col, cplx<double> v) {
+      if (R == nullptr) {  // plain product A X
+        std_(o + 2 * col, v);
+        return;
+      }
       const cplx<double> rv = ldd(r + 2 * col);
       const cplx<double> d = csub(rv, v);
       acc_r += d.re * d.re + d.im * d.im;
@@ -501,7 +506,7 @@ __global__ void k_residual(int n, int P, const double* R, const double* Y, doubl
     acc_r += __shfl_xor(acc_r, off);
     acc_b += __shfl_xor(acc_b, off);
   }
-  if ((threadIdx.x & 63) == 0) {
+  if (nrm != nullptr && (threadIdx.x & 63) == 0) {
     atomicAdd(nrm + 2 * b, acc_r);
     atomicAdd(nrm + 2 * b + 1, acc_b);
   }
@@ -790,9 +795,13 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
   return 1;
 }
 
+extern "C" int mnpbem_bem_combine_batched(int n, int P, const void* Y, long ldy, long sy, const void* R, void* Out,
+                                          long ldv, long sv, const double* nvec, const double* par, double* nrm,
+                                          int batch, void* stream);
+
 // Rout = R - A X (complex128), Y: (batch, n, 16P) work buffer; nrm: (batch, 2)
-// receives ||R - A X||^2 and ||R||^2.  G1t.. are complex128 transposed
-// Green matrices (not factored).
+// receives ||R - A X||^2 and ||R||^2 (nrm may be NULL; with R NULL, Rout =
+// A X).  G1t.. are complex128 transposed Green matrices (not factored).
 extern "C" int mnpbem_bem_residual_batched(int n, int P, const void* G1t, const void* H1t, const void* G2t,
                                            const void* H2t, long ldm, long smat, const void* R, const void* X,
                                            void* Rout, long ldv, long sv, void* Y, long ldy, long sy,
@@ -816,10 +825,27 @@ extern "C" int mnpbem_bem_residual_batched(int n, int P, const void* G1t, const 
   if ((rc = gemv_t<double>(n, P4, (const double*)H2t, ldm, smat, x + 2 * P4, ldv, sv, y + 2 * 3 * P4, ldy, sy, 1,
                            0, 0, batch, st)))
     return rc;
-  hipLaunchKernelGGL(k_fill_double, dim3(mnp_ceil_div(2L * batch, 256)), dim3(256), 0, st, 2L * batch, nrm, 0.0);
-  MNP_LAUNCH_CHECK();
+  return mnpbem_bem_combine_batched(n, P, Y, ldy, sy, R, Rout, ldv, sv, nvec, par, nrm, batch, stream);
+}
+
+// Combination step of the 8N operator from precomputed products
+// Y = [G1 X1 | H1 X1 | G2 X2 | H2 X2] (batch, n, 16P) complex128:
+// Out = R - A X when R is given (nrm, if given, receives ||R - A X||^2 and
+// ||R||^2 per wavelength), Out = A X when R is NULL
+// (mnpbem/bem/bem_ret_iter.py:1362-1501 after its four Green matvecs).
+extern "C" int mnpbem_bem_combine_batched(int n, int P, const void* Y, long ldy, long sy, const void* R, void* Out,
+                                          long ldv, long sv, const double* nvec, const double* par, double* nrm,
+                                          int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldy >= 16 * P && ldv >= 8 * P, "bem_combine: bad size");
+  if (batch <= 0) return 0;
+  if (nrm != nullptr) {
+    MNP_REQUIRE(R != nullptr, "bem_combine: norms need R");
+    hipLaunchKernelGGL(k_fill_double, dim3(mnp_ceil_div(2L * batch, 256)), dim3(256), 0, st, 2L * batch, nrm, 0.0);
+    MNP_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_residual, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, P, (const double*)R,
-                     (const double*)Y, (double*)Rout, ldv, sv, ldy, sy, nvec, par, nrm);
+                     (const double*)Y, (double*)Out, ldv, sv, ldy, sy, nvec, par, nrm);
   MNP_LAUNCH_CHECK();
   return 0;
 }

@@ -1,0 +1,204 @@
+"""GPU parity of BEMRetIter (GMRES on the GPU) and of the H-matrix kernels
+(ACA, matvec, densify) through the C ABI.
+
+Oracles: the FP64 CPU direct solve (oracle.bem_oracle), the restated scipy
+GMRES path (oracle.iter_oracle) and the CPU ACA / H-matrix restatement
+(oracle.hmatrix_oracle), fed the same matrices.  Tolerances are written at
+each assertion; the iterative solves are run at tol 1e-10 so that their
+agreement with the direct solve is limited by the GMRES tolerance times the
+condition of the system, not by the kernels.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import oracle as o  # noqa: E402
+from oracle.hmatrix_oracle import OracleClusterTree, OracleHMatrix  # noqa: E402
+
+DRUDE = (10.0, 9.065, 0.0708)
+POL2 = ([[1, 0, 0], [0, 1, 0]], [[0, 0, 1], [0, 0, 1]])
+
+
+@pytest.fixture(scope="module")
+def sphere284(cuda):
+    import mnpbem_amd as m
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [m.trisphere(144, 20.0)], [[2, 1]], 1, interp="curv")
+    nodes, tri6 = o.trisphere_mesh(144, 20.0)
+    comp = o.Composite([o.EpsConst(1.0), o.EpsDrude(*DRUDE)], [o.CurvedSurface(nodes, tri6)], [[2, 1]])
+    return cp, comp, o.GreenSetup(comp)
+
+
+def _rel(a, b):
+    a = a.cpu().numpy() if hasattr(a, "cpu") else np.asarray(a)
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_iter_dense_matches_direct_oracle(sphere284, precision):
+    import mnpbem_amd as m
+    cp, comp, ref = sphere284
+    print("BEMRetIter dense", precision, flush=True)
+    bem = m.BEMRetIter(cp, precision=precision, tol=1e-10, maxit=50)
+    exc = m.PlaneWaveRet(*POL2)
+    sig, _ = bem.solve(exc.potential(cp, 550.0))
+    d = o.BEMRetOracle(ref).solve(o.planewave_potential(comp, *POL2, 550.0))
+    for f in ("sig1", "sig2", "h1", "h2"):
+        assert _rel(sig.get(f), d[f]) < 1e-8, f
+    flag, relres, it = bem.info()
+    assert flag[-1] == 0 and relres[-1] < 1e-10
+    # the direct preconditioner makes GMRES converge in a few iterations
+    assert it[-1][1] <= (2 if precision == "double" else 6)
+
+
+def test_iter_multipol_matches_sequential(sphere284):
+    """tests/test_iter_multipol.py of the reference: a two-polarisation solve
+    equals the two single-polarisation solves."""
+    import mnpbem_amd as m
+    cp, _, _ = sphere284
+    print("BEMRetIter multipol", flush=True)
+    bm = m.BEMRetIter(cp, tol=1e-12)
+    sm, _ = bm.solve(m.PlaneWaveRet(*POL2).potential(cp, 600.0))
+    for q in range(2):
+        b1 = m.BEMRetIter(cp, tol=1e-12)
+        s1, _ = b1.solve(m.PlaneWaveRet([POL2[0][q]], [POL2[1][q]]).potential(cp, 600.0))
+        for f in ("sig1", "sig2"):
+            assert _rel(sm.get(f)[:, q], s1.get(f).cpu().numpy()) < 1e-10
+        for f in ("h1", "h2"):
+            assert _rel(sm.get(f)[:, :, q], s1.get(f).cpu().numpy()) < 1e-10
+
+
+def test_iter_unpreconditioned_and_precond_only(cuda):
+    import mnpbem_amd as m
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [m.trisphere(32, 10.0)], [[2, 1]], 1)
+    nodes, tri6 = o.trisphere_mesh(32, 10.0)
+    comp = o.Composite([o.EpsConst(1.0), o.EpsDrude(*DRUDE)], [o.CurvedSurface(nodes, tri6)], [[2, 1]])
+    ref = o.GreenSetup(comp)
+    print("BEMRetIter precond None / maxit 0, N=60", flush=True)
+    exc = m.PlaneWaveRet([[1, 0, 0]], [[0, 0, 1]]).potential(cp, 520.0)
+    d = o.BEMRetOracle(ref).solve(o.planewave_potential(comp, [[1, 0, 0]], [[0, 0, 1]], 520.0))
+    # full GMRES (restart = 8N) converges within 8N iterations
+    b0 = m.BEMRetIter(cp, precond=None, tol=1e-9, maxit=2, restart=8 * cp.n)
+    s0, _ = b0.solve(exc)
+    flag, relres, it = b0.info()
+    assert flag[-1] == 0 and relres[-1] < 1e-9
+    assert _rel(s0.sig2, d["sig2"][:, 0]) < 1e-5
+    bp = m.BEMRetIter(cp, maxit=0)
+    sp, _ = bp.solve(exc)
+    assert _rel(sp.sig2, d["sig2"][:, 0]) < 1e-10
+
+
+def test_iter_sweep_matches_direct_sweep(sphere284):
+    import mnpbem_amd as m
+    cp, _, _ = sphere284
+    print("BEMRetIter sweep", flush=True)
+    lams = np.linspace(420.0, 780.0, 5)
+    exc = m.PlaneWaveRet(*POL2)
+    e_it = m.BEMRetIter(cp, precision="mixed", tol=1e-11).solve_sweep(lams, exc, chunk=3)
+    e_d = m.BEMRet(cp).solve_sweep(lams, exc, chunk=5)
+    assert np.max(np.abs(e_it / e_d - 1)) < 1e-9
+
+
+# ------------------------------------------------------------------ H-matrices
+def _points(n, seed=42):
+    rng = np.random.RandomState(seed)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    th = np.arccos(rng.uniform(-1, 1, n))
+    return 10.0 * np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+
+
+def _kernel(pos):
+    def fun(r, c):
+        d = np.linalg.norm(pos[r] - pos[c], axis=1)
+        return 1.0 / (4.0 * np.pi * np.maximum(d, 1e-10))
+    return fun
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-5), ("complex64", 1e-5)])
+def test_hmatrix_kernel_1_over_r(cuda, dtype, tol):
+    from mnpbem_amd.hmatrix import ClusterTree, HMatrix
+    print("hmatrix 1/r", dtype, flush=True)
+    pos = _points(800)
+    fun = _kernel(pos)
+    n = len(pos)
+    rr, cc = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    A = fun(rr.ravel(), cc.ravel()).reshape(n, n)
+    t = ClusterTree(pos, cleaf=16)
+    h = HMatrix.from_func(t, fun, htol=1e-6, kmax=64, dtype=dtype)
+    ho = OracleHMatrix(OracleClusterTree(pos, cleaf=16), htol=1e-6, kmax=64).aca(fun)
+    assert np.linalg.norm(h.full(np) - A) / np.linalg.norm(A) < tol
+    # same block partition; ranks within one of the CPU ACA's (pivot ties may differ)
+    assert len(h.ranks) == len(ho.lhs)
+    assert np.max(np.abs(h.ranks - ho.ranks())) <= 1
+    assert abs(h.compression() - ho.compression()) < 0.02
+    X = np.random.RandomState(2).randn(n, 3) + 1j * np.random.RandomState(3).randn(n, 3)
+    Y = h.mtimes_vec(X)
+    assert np.linalg.norm(Y - A @ X) / np.linalg.norm(A @ X) < tol
+    Yf = h.full(np) @ X
+    assert np.linalg.norm(Y - Yf) / np.linalg.norm(Yf) < (1e-13 if dtype == "complex128" else 1e-6)
+
+
+def test_hmatrix_two_particles_known_rank(cuda):
+    """Two well-separated particles and a separable kernel: every admissible
+    block has exact rank 2, ACA must stop at rank <= 3 with round-off error."""
+    from mnpbem_amd.hmatrix import ClusterTree, HMatrix
+    print("hmatrix separable", flush=True)
+    p1, p2 = _points(300), _points(200, seed=9)
+    p2[:, 0] += 100.0
+    pos = np.concatenate([p1, p2])
+    ip = np.repeat([0, 1], [300, 200])
+    f1 = np.exp(1j * pos[:, 0] / 7.0)
+    f2 = pos[:, 1] + 2.0
+    g1, g2 = pos[:, 2] ** 2 + 1.0, np.cos(pos[:, 0])
+    fun = lambda r, c: f1[r] * g1[c] + f2[r] * g2[c]
+    t = ClusterTree(pos, cleaf=20, ipart_arr=ip)
+    h = HMatrix.from_func(t, fun, htol=1e-12, kmax=30)
+    n = len(pos)
+    rr, cc = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    A = fun(rr.ravel(), cc.ravel()).reshape(n, n)
+    assert h.ranks.max() <= 3
+    assert np.linalg.norm(h.full(np) - A) / np.linalg.norm(A) < 1e-12
+
+
+@pytest.mark.parametrize("key", ["G", "H1"])
+def test_aca_green_matches_cpu_aca(sphere284, key):
+    """ACA of the assembled Green matrix on the GPU vs the CPU ACA of the
+    same matrix: same partition, ranks within one, both within htol."""
+    from mnpbem_amd.hmatrix import ACACompGreenRet
+    from oracle.hmatrix_oracle import kaware_fadmiss
+    cp, _, _ = sphere284
+    print("aca green", key, flush=True)
+    aca = ACACompGreenRet(cp, htol=1e-6, kmax=50, cleaf=16)
+    lam = 600.0
+    h = aca.eval(0, 0, key, lam)
+    M = aca.g.eval(0, 0, key, lam).cpu().numpy()
+    ho = OracleHMatrix(OracleClusterTree(cp.pos, cleaf=16), htol=1e-6, kmax=50,
+                       fadmiss=kaware_fadmiss(2 * np.pi / lam)).aca(lambda r, c: M[r, c])
+    assert len(h.ranks) == len(ho.lhs) and np.max(np.abs(h.ranks - ho.ranks())) <= 1
+    err = np.linalg.norm(h.full(np) - M) / np.linalg.norm(M)
+    erro = np.linalg.norm(ho.full() - M) / np.linalg.norm(M)
+    assert err < 1e-5 and erro < 1e-5, (err, erro)
+
+
+@pytest.mark.parametrize("hdt", ["complex128", "complex64"])
+def test_iter_hmatrix_matches_direct(cuda, hdt):
+    """BEMRetIter(hmatrix=True): GMRES over the ACA operator; its solution
+    differs from the dense direct one by the compression error (htol)."""
+    import mnpbem_amd as m
+    print("BEMRetIter hmatrix", hdt, flush=True)
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [m.trisphere(400, 40.0)], [[2, 1]], 1)
+    lams = np.array([500.0, 650.0])
+    exc = m.PlaneWaveRet(*POL2)
+    bi = m.BEMRetIter(cp, hmatrix=True, htol=1e-8, kmax=60, cleaf=32, tol=1e-10, maxit=30, hmat_dtype=hdt)
+    e_h = bi.solve_sweep(lams, exc, chunk=2)
+    e_d = m.BEMRet(cp).solve_sweep(lams, exc)
+    lim = 1e-6 if hdt == "complex128" else 2e-5
+    assert np.max(np.abs(e_h / e_d - 1)) < lim
+    assert all(f == 0 for f in bi.info()[0])
+    # at N = 796 with htol 1e-8 and the wavelength-aware admissibility the
+    # factors cost about as much as the dense blocks they replace
+    c = bi.hinfo()
+    assert 0 < c < 1.5

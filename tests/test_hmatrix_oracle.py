@@ -1,0 +1,194 @@
+"""CPU pins of the cluster-tree / ACA / H-matrix oracle and of the product's
+host-side cluster tree (no GPU).
+
+The invariants are those of the reference's own tests
+(mnpbem/tests/test_hmatrix.py: leaf sizes, bijective index maps, bounding
+spheres, ACA accuracy on 1/(4 pi r), two-particle trees), plus checks that
+pin the ACA and truncation arithmetic: exact recovery of a matrix of known
+rank, truncation rank = the SVD criterion, and the k-aware admissibility
+against its definition.  The product's ClusterTree (mnpbem_amd/hmatrix.py,
+host code) must reproduce the oracle's tree and block partition exactly.
+"""
+
+import numpy as np
+import pytest
+
+from oracle.hmatrix_oracle import OracleClusterTree, OracleHMatrix, aca_block, kaware_fadmiss, truncate_block
+import oracle as o
+
+
+def sphere_points(n, radius=10.0, seed=42):
+    rng = np.random.RandomState(seed)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    ct = rng.uniform(-1, 1, n)
+    th = np.arccos(ct)
+    return np.stack([radius * np.sin(th) * np.cos(phi), radius * np.sin(th) * np.sin(phi), radius * np.cos(th)], 1)
+
+
+def two_spheres(n1, n2, offset=50.0):
+    p1, p2 = sphere_points(n1), sphere_points(n2, seed=7)
+    p2[:, 0] += offset
+    return np.concatenate([p1, p2]), np.concatenate([np.zeros(n1, int), np.ones(n2, int)])
+
+
+def green_kernel(pos):
+    def fun(r, c):
+        d = np.linalg.norm(pos[r] - pos[c], axis=1)
+        return 1.0 / (4.0 * np.pi * np.maximum(d, 1e-10))
+    return fun
+
+
+def dense(fun, n):
+    rr, cc = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    return fun(rr.ravel(), cc.ravel()).reshape(n, n)
+
+
+def test_tree_invariants():
+    pos = sphere_points(500)
+    t = OracleClusterTree(pos, cleaf=16)
+    leaves = np.nonzero(t.son[:, 0] == -1)[0]
+    sizes = t.cind[leaves, 1] - t.cind[leaves, 0] + 1
+    assert sizes.max() <= 16 and sizes.sum() == 500
+    assert np.array_equal(np.sort(t.ind[:, 0]), np.arange(500))
+    assert np.array_equal(t.ind[t.ind[:, 0], 1], np.arange(500))
+    v = np.arange(500) * 1.5
+    assert np.array_equal(v[t.ind[:, 0]][t.ind[:, 1]], v)
+    for i in range(len(t.son)):
+        faces = t.ind[t.cind[i, 0]:t.cind[i, 1] + 1, 0]
+        d = np.linalg.norm(pos[faces] - t.mid[i], axis=1)
+        assert np.all(d <= t.rad[i] * (1 + 1e-12) + 1e-12)
+        if t.son[i, 0] != -1:  # sons split the parent's range
+            a, b = t.son[i]
+            assert t.cind[a, 0] == t.cind[i, 0] and t.cind[b, 1] == t.cind[i, 1]
+            assert t.cind[a, 1] + 1 == t.cind[b, 0]
+
+
+@pytest.mark.parametrize("two", [False, True])
+def test_block_partition_covers_matrix_once(two):
+    if two:
+        pos, ip = two_spheres(120, 80)
+    else:
+        pos, ip = sphere_points(300), None
+    t = OracleClusterTree(pos, cleaf=16, ipart_arr=ip)
+    cover = np.zeros((len(pos), len(pos)), int)
+    for (i1, i2), v in t.admissibility().items():
+        if v == 2:
+            assert t.son[i1, 0] == -1 and t.son[i2, 0] == -1
+        cover[t.cind[i1, 0]:t.cind[i1, 1] + 1, t.cind[i2, 0]:t.cind[i2, 1] + 1] += 1
+    assert np.all(cover == 1)
+    if two:  # the root splits between the particles
+        a, b = t.son[0]
+        assert len(set(ip[t.ind[t.cind[a, 0]:t.cind[a, 1] + 1, 0]])) == 1
+        assert t.ipart[0] == -1
+
+
+def test_aca_accuracy_and_compression():
+    pos = sphere_points(400)
+    t = OracleClusterTree(pos, cleaf=16)
+    fun = green_kernel(pos)
+    h = OracleHMatrix(t, htol=1e-6).aca(fun)
+    A = dense(fun, 400)
+    assert np.linalg.norm(h.full() - A) / np.linalg.norm(A) < 1e-5
+    x = np.random.RandomState(1).randn(400)
+    assert np.linalg.norm(h.mtimes_vec(x) - A @ x) / np.linalg.norm(A @ x) < 1e-5
+    pos2, ip = two_spheres(200, 200, offset=200.0)
+    t2 = OracleClusterTree(pos2, cleaf=16, ipart_arr=ip)
+    h2 = OracleHMatrix(t2, htol=1e-4).aca(green_kernel(pos2))
+    assert 0 < h2.compression() < 1.0
+
+
+def test_aca_recovers_known_rank():
+    rng = np.random.RandomState(3)
+    m, n, r = 60, 45, 4
+    U = rng.randn(m, r) + 1j * rng.randn(m, r)
+    V = rng.randn(n, r) + 1j * rng.randn(n, r)
+    M = U @ V.T
+    fun = lambda rr, cc: M[rr, cc]
+    u, v = aca_block(fun, np.arange(m), np.arange(n), 1e-12, 50)
+    assert u.shape[1] in (r, r + 1)
+    assert np.linalg.norm(u @ v.T - M) / np.linalg.norm(M) < 1e-12
+
+
+def test_truncate_matches_svd_criterion():
+    rng = np.random.RandomState(5)
+    m, n, r = 40, 30, 12
+    s = 10.0 ** -np.arange(r)
+    U = np.linalg.qr(rng.randn(m, r))[0] * s
+    V = np.linalg.qr(rng.randn(n, r))[0]
+    M = U @ V.T
+    for htol in (1e-2, 1e-4, 1e-8):
+        u, v = truncate_block(U, V, htol)
+        sv = np.linalg.svd(M, compute_uv=False)[:r]
+        k = max(1, np.count_nonzero(np.cumsum(sv) < (1 - htol) * sv.sum()))
+        assert u.shape[1] == k
+        assert np.linalg.norm(u @ v.T - M, 2) <= sv[k] * (1 + 1e-9)
+
+
+def test_kaware_admissibility_definition():
+    k = 2 * np.pi / 500.0
+    f = kaware_fadmiss(k, 2.5)
+    assert f(10.0, 12.0, 30.0) is True        # 2.5 * 10 < 30 and k r < 1
+    assert f(10.0, 12.0, 20.0) is False       # geometric rule fails
+    assert f(100.0, 120.0, 260.0) is False    # k r > 1 and dist < wavelength
+    assert f(100.0, 120.0, 600.0) is True
+    g = kaware_fadmiss(0.0)
+    assert g(100.0, 120.0, 260.0) is True
+
+
+def _product_tree(pos, cleaf, ip):
+    hm = pytest.importorskip("mnpbem_amd.hmatrix")
+    return hm.ClusterTree(pos, cleaf=cleaf, ipart_arr=ip)
+
+
+@pytest.mark.parametrize("case", ["points", "two_points", "trisphere", "two_trispheres"])
+def test_product_tree_equals_oracle(case):
+    if case == "points":
+        pos, ip, cleaf = sphere_points(700), None, 32
+    elif case == "two_points":
+        pos, ip = two_spheres(300, 200, offset=30.0)
+        cleaf = 16
+    else:
+        nodes, tri6 = o.trisphere_mesh(400, 40.0)
+        p1 = o.CurvedSurface(nodes, tri6).pos
+        if case == "trisphere":
+            pos, ip, cleaf = p1, None, 64
+        else:
+            p2 = p1 + np.array([42.0, 0, 0])
+            pos, ip, cleaf = np.concatenate([p1, p2]), np.repeat([0, 1], len(p1)), 64
+    a = OracleClusterTree(pos, cleaf=cleaf, ipart_arr=ip)
+    b = _product_tree(pos, cleaf, ip)
+    for name in ("son", "cind", "ind", "ipart"):
+        assert np.array_equal(getattr(a, name), getattr(b, name)), name
+    assert np.array_equal(a.mid, b.mid) and np.array_equal(a.rad, b.rad)
+    hm = pytest.importorskip("mnpbem_amd.hmatrix")
+    for fad in (None, kaware_fadmiss(2 * np.pi / 600.0)):
+        adm_o = a.admissibility(fad)
+        adm_p = b.admissibility(b, fad if fad is not None else None)
+        assert list(adm_o.items()) == list(adm_p.items())
+    # the product's k-aware rule is the oracle's
+    fp, fo = hm.make_kaware_fadmiss(2 * np.pi / 600.0), kaware_fadmiss(2 * np.pi / 600.0)
+    rng = np.random.RandomState(0)
+    for r1, r2, d in rng.uniform(1, 400, size=(200, 3)):
+        assert fp(r1, r2, d) == fo(r1, r2, d)
+
+
+def test_product_block_tables_partition():
+    """Host-side block and leaf tables of the product HMatrix (no device
+    work): every leaf row lists exactly the blocks covering it."""
+    hm = pytest.importorskip("mnpbem_amd.hmatrix")
+    pos, ip = two_spheres(300, 200, offset=30.0)
+    t = hm.ClusterTree(pos, cleaf=24, ipart_arr=ip)
+    h = hm.HMatrix(t, htol=1e-6, kmax=30, device="cpu")
+    h._tables()
+    blk, leafs = h._blk_host, h._leafs_host
+    lst = h._lst.numpy()
+    n = len(pos)
+    cover = np.zeros((n, n), int)
+    for b in blk:
+        cover[b[0]:b[0] + b[1], b[2]:b[2] + b[3]] += 1
+    assert np.all(cover == 1)
+    for lf in leafs:
+        r0, m, f, l = lf
+        ids = set(lst[f:l].tolist())
+        want = {i for i, b in enumerate(blk) if b[0] <= r0 and r0 + m <= b[0] + b[1]}
+        assert ids == want
