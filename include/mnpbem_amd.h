@@ -225,6 +225,16 @@ int mnpbem_hmat_matvec(int dtype, int nblk, int nleaf, int maxchunks, const long
 int mnpbem_hmat_full(int dtype, int nblk, const long* blk, const int* perm, const void* val, const void* U,
                      const void* V, void* Out, long ldo, void* stream);
 
+/* Scattering cross section (planewave_ret.py:391-435 over the far field of
+ * spectrum_ret.py:231-459): far fields e, h in the ndir directions dirs
+ * (ndir x 3, unit vectors of the far-field sphere, weights w), dsca[b][d][p] =
+ * 0.5 Re(dir . (e x conj h)) / (0.5 nb), sca[b][p] = sum_d w_d dsca[b][d][p];
+ * E, H: NULL or (batch, ndir, 3, P) complex128 far fields.  P <= 8. */
+int mnpbem_scattering_batched(int n, int P, const double* X, long ldv, long sv, const double* pos,
+                              const double* area, const int* in_med, const int* out_med, int ndir,
+                              const double* dirs, const double* w, const double* pw, double* dsca, double* sca,
+                              double* E, double* H, int batch, void* stream);
+
 /* ---- measurement: HIP-event timing of every complex-GEMM launch ----
  * (host pointers for the results of mnpbem_prof_end) */
 int mnpbem_prof_begin(int capacity);

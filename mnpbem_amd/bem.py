@@ -242,23 +242,29 @@ class BEMRet:
     def __truediv__(self, exc):
         return self.solve(exc)[0]
 
-    def solve_sweep(self, enei_list, planewave, chunk=16, with_x=False):
+    def solve_sweep(self, enei_list, planewave, chunk=16, with_x=False, with_sca=False):
         """Extinction spectrum (n_lambda, npol) of a plane-wave sweep, factored
-        and solved ``chunk`` wavelengths per batched launch."""
+        and solved ``chunk`` wavelengths per batched launch; with ``with_sca``
+        also the scattering spectrum (absorption = ext - sca)."""
         enei_list = np.atleast_1d(np.asarray(enei_list, dtype=float))
-        out = []
-        xs = []
+        out, scas, xs = [], [], []
         for c0 in range(0, len(enei_list), chunk):
             lam = enei_list[c0:c0 + chunk]
             F = self.factor(lam)
             R = planewave.rhs(self.p, lam, self.g)
             X = self.solve_batch(F, R)
             out.append(planewave.extinction_batch(self.p, lam, X, self.g))
+            if with_sca:
+                scas.append(planewave.scattering_batch(self.p, lam, X, self.g)[0])
             if with_x:
                 xs.append(X)
             del F
-        ext = torch.cat(out).cpu().numpy()
-        return (ext, torch.cat(xs)) if with_x else ext
+        res = [torch.cat(out).cpu().numpy()]
+        if with_sca:
+            res.append(torch.cat(scas).cpu().numpy())
+        if with_x:
+            res.append(torch.cat(xs))
+        return res[0] if len(res) == 1 else tuple(res)
 
 
 def _unpack(p, enei, X, P):

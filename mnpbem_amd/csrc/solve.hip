@@ -628,6 +628,115 @@ __global__ __launch_bounds__(256) void k_extinction(int n, int P, const double* 
   }
 }
 
+// Far field in the directions dirs[d] (unit vectors, the pinfty sphere of
+// spectrum_ret.py:14-45) and its radiated power (spectrum_ret.py:231-459,
+// planewave_ret.py:391-435): with S_h = sum_j ph_j h_j, S_s = sum_j ph_j sig_j,
+// ph_j = e^{-i k dir.pos_j} area_j over the medium-side faces,
+//   e = i k0 S_h - i k dir S_s,   h = i k dir x S_h,
+//   dsca[b][d][p] = 0.5 Re(dir . (e x conj h)) / (0.5 nb),  nb = Re k / k0.
+// One workgroup per (direction, wavelength); P <= 8.  e, h (optional):
+// (batch, ndir, 3, P) complex128.
+__global__ __launch_bounds__(256) void k_farfield(int n, int P, const double* X, long ld, long stride,
+                                                  const double* pos, const double* area, const int* in_med,
+                                                  const int* out_med, const double* dirs, const double* pw,
+                                                  double* dsca, double* E, double* Hf) {
+  const int d = blockIdx.x, ndir = gridDim.x;
+  const long b = blockIdx.y;
+  const double* q = pw + 7 * b;
+  const double k0 = q[0], kr = q[1], ki = q[2];
+  const double dx = dirs[3 * d], dy = dirs[3 * d + 1], dz = dirs[3 * d + 2];
+  cplx<double> acc[8][4];  // [p][h_x, h_y, h_z, sig]
+#pragma unroll
+  for (int p = 0; p < 8; ++p)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[p][c] = cmake(0.0, 0.0);
+  for (int j = threadIdx.x; j < n; j += 256) {
+    const double kd = pos[3 * j] * dx + pos[3 * j + 1] * dy + pos[3 * j + 2] * dz;
+    const cplx<double> ph = cscale(cexpi(-kr * kd, -ki * kd), area[j]);
+    const double* x = X + 2 * (b * stride + (long)j * ld);
+    for (int side = 0; side < 2; ++side) {
+      if (!(side == 0 ? in_med[j] : out_med[j])) continue;
+      const int off = side * 4 * P;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        if (p < P) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[p][c] = cfma(acc[p][c], ph, ldd(x + 2 * (off + P + c * P + p)));
+          acc[p][3] = cfma(acc[p][3], ph, ldd(x + 2 * (off + p)));
+        }
+      }
+    }
+  }
+  __shared__ double red[4][8][4][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int p = 0; p < 8; ++p)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      cplx<double> v = acc[p][c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        v.re += __shfl_xor(v.re, off);
+        v.im += __shfl_xor(v.im, off);
+      }
+      if (lane == 0) {
+        red[wv][p][c][0] = v.re;
+        red[wv][p][c][1] = v.im;
+      }
+    }
+  __syncthreads();
+  if (threadIdx.x < P) {
+    const int p = threadIdx.x;
+    cplx<double> S[4];
+    for (int c = 0; c < 4; ++c)
+      S[c] = cmake(red[0][p][c][0] + red[1][p][c][0] + red[2][p][c][0] + red[3][p][c][0],
+                   red[0][p][c][1] + red[1][p][c][1] + red[2][p][c][1] + red[3][p][c][1]);
+    const double dv[3] = {dx, dy, dz};
+    const cplx<double> kc = cmake(kr, ki);
+    cplx<double> e[3], h[3];
+    for (int c = 0; c < 3; ++c) {
+      // i k0 S_h - i k dir S_s
+      const cplx<double> t = csub(cscale(S[c], k0), cscale(cmul(kc, S[3]), dv[c]));
+      e[c] = cmake(-t.im, t.re);
+    }
+    const cplx<double> cr[3] = {csub(cscale(S[2], dv[1]), cscale(S[1], dv[2])),
+                                csub(cscale(S[0], dv[2]), cscale(S[2], dv[0])),
+                                csub(cscale(S[1], dv[0]), cscale(S[0], dv[1]))};
+    for (int c = 0; c < 3; ++c) {
+      const cplx<double> t = cmul(kc, cr[c]);
+      h[c] = cmake(-t.im, t.re);
+    }
+    // dir . (e x conj h)
+    auto cj = [](cplx<double> z) { return cmake(z.re, -z.im); };
+    const cplx<double> px = csub(cmul(e[1], cj(h[2])), cmul(e[2], cj(h[1])));
+    const cplx<double> py = csub(cmul(e[2], cj(h[0])), cmul(e[0], cj(h[2])));
+    const cplx<double> pz = csub(cmul(e[0], cj(h[1])), cmul(e[1], cj(h[0])));
+    const double nb = kr / k0;
+    dsca[(b * ndir + d) * P + p] = 0.5 * (dx * px.re + dy * py.re + dz * pz.re) / (0.5 * nb);
+    if (E != nullptr) {
+      for (int c = 0; c < 3; ++c) {
+        double* o = E + 2 * (((b * ndir + d) * 3 + c) * P + p);
+        o[0] = e[c].re;
+        o[1] = e[c].im;
+        double* oh = Hf + 2 * (((b * ndir + d) * 3 + c) * P + p);
+        oh[0] = h[c].re;
+        oh[1] = h[c].im;
+      }
+    }
+  }
+}
+
+// sca[b][p] = sum_d w_d dsca[b][d][p] in a fixed order
+__global__ void k_sca_reduce(int ndir, int P, int total, const double* w, const double* dsca, double* sca) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // t = b * P + p
+  if (t >= total) return;
+  const int p = t % P;
+  const long b = t / P;
+  double s = 0.0;
+  for (int d = 0; d < ndir; ++d) s += w[d] * dsca[(b * ndir + d) * P + p];
+  sca[t] = s;
+}
+
 template <typename T>
 int gemv_t(int n, int C, const T* M, long ldm, long sM, const T* X, long ldx, long sX, T* Y, long ldy, long sY,
            double alr, double ali, double beta, int batch, hipStream_t st) {
@@ -898,6 +1007,28 @@ extern "C" int mnpbem_extinction_batched(int n, int P, const double* X, long ldv
   if (batch <= 0) return 0;
   hipLaunchKernelGGL(k_extinction, dim3(P, batch), dim3(256), 0, st, n, P, X, ldv, sv, pos, area, in_med, out_med,
                      pol, dir, pw, ext);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Scattering cross section of the plane wave (planewave_ret.py:391-435 over
+// spectrum_ret.py:395-459): dirs / w are the far-field sphere's directions and
+// weights (ndir); dsca: (batch, ndir, P) doubles (differential, normalised by
+// the incoming 0.5 nb); sca: (batch, P); E, H: NULL or (batch, ndir, 3, P)
+// complex128 far fields.  P <= 8.
+extern "C" int mnpbem_scattering_batched(int n, int P, const double* X, long ldv, long sv, const double* pos,
+                                         const double* area, const int* in_med, const int* out_med, int ndir,
+                                         const double* dirs, const double* w, const double* pw, double* dsca,
+                                         double* sca, double* E, double* H, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && P <= 8 && ndir > 0 && ndir <= 2147483647, "scattering: bad size");
+  MNP_REQUIRE((E == nullptr) == (H == nullptr), "scattering: give both E and H or neither");
+  if (batch <= 0) return 0;
+  hipLaunchKernelGGL(k_farfield, dim3(ndir, batch), dim3(256), 0, st, n, P, X, ldv, sv, pos, area, in_med, out_med,
+                     dirs, pw, dsca, E, H);
+  MNP_LAUNCH_CHECK();
+  const int total = batch * P;
+  hipLaunchKernelGGL(k_sca_reduce, dim3(mnp_ceil_div(total, 256)), dim3(256), 0, st, ndir, P, total, w, dsca, sca);
   MNP_LAUNCH_CHECK();
   return 0;
 }

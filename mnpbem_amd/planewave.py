@@ -96,6 +96,59 @@ class PlaneWaveRet:
                  nat.stream_handle(g.device))
         return ext
 
+    def _pinfty(self, device):
+        key = str(device)
+        cache = getattr(self, "_pinf_cache", None)
+        if cache is None or cache[0] != key:
+            dirs, w = read_pinfty()
+            self._pinf_cache = (key, torch.as_tensor(np.ascontiguousarray(dirs)).to(device),
+                                torch.as_tensor(np.ascontiguousarray(w)).to(device))
+        return self._pinf_cache[1], self._pinf_cache[2]
+
+    def scattering_batch(self, p, enei_list, X, g, fields=False):
+        """Scattering cross sections (B, npol) and differential cross sections
+        (B, ndir, npol) over the 256-direction far-field sphere from solutions
+        X (B, n, 8P); with ``fields`` also the far fields e, h (B, ndir, 3, P)."""
+        enei_list = np.atleast_1d(enei_list)
+        B, n, P = len(enei_list), p.n, self.npol
+        if P > 8:
+            raise ValueError("scattering: at most 8 polarisations per call")
+        inm, outm, pol, dr = self._side_flags(p, g)
+        pw = self._pw(p, enei_list, g.device)
+        dirs, w = self._pinfty(g.device)
+        nd = dirs.shape[0]
+        dsca = torch.empty((B, nd, P), dtype=torch.float64, device=g.device)
+        sca = torch.empty((B, P), dtype=torch.float64, device=g.device)
+        E = H = None
+        if fields:
+            E = torch.empty((B, nd, 3, P), dtype=torch.complex128, device=g.device)
+            H = torch.empty_like(E)
+        nat.call("mnpbem_scattering_batched", n, P, nat.ptr(X), 8 * P, n * 8 * P, nat.ptr(g.pos), nat.ptr(g.area),
+                 nat.ptr(inm), nat.ptr(outm), nd, nat.ptr(dirs), nat.ptr(w), nat.ptr(pw), nat.ptr(dsca),
+                 nat.ptr(sca), nat.ptr(E), nat.ptr(H), B, nat.stream_handle(g.device))
+        return (sca, dsca, E, H) if fields else (sca, dsca)
+
+    def scattering(self, sig, green=None):
+        """Scattering cross section (planewave_ret.py:391-435): (sca, dsca)
+        with sca a float for one polarisation, else (npol,); dsca a CompStruct
+        holding the differential cross section over the far-field sphere."""
+        g = green if green is not None else _green_for(sig.p)
+        X = sig.get("_X")
+        if X is None:
+            raise ValueError("scattering: CompStruct does not come from BEMRet.solve")
+        sca, dsca = self.scattering_batch(sig.p, [sig.enei], X, g)
+        s = sca[0].cpu().numpy()
+        d = dsca[0].cpu().numpy()
+        if self.npol == 1:
+            return float(s[0]), CompStruct(sig.p, sig.enei, dsca=d[:, 0])
+        return s, CompStruct(sig.p, sig.enei, dsca=d)
+
+    def absorption(self, sig, green=None):
+        """Absorption cross section ext - sca (planewave_ret.py:363-389)."""
+        ext = self.extinction(sig, green)
+        sca, _ = self.scattering(sig, green)
+        return ext - sca
+
     def extinction(self, sig, green=None):
         """Optical-theorem extinction cross section (planewave_ret.py:437-504):
         float for one polarisation, else an array (npol,)."""
@@ -110,3 +163,19 @@ class PlaneWaveRet:
 def _green_for(p):
     from .greenfun import green_for
     return green_for(p)
+
+
+def read_pinfty(path=None):
+    """Directions and weights of the far-field sphere (the pinfty256.bin file
+    of the reference: int32 count n, then the n unit vectors as three float64
+    arrays x[n], y[n], z[n], then either positions (3n) and weights (n) or the
+    n weights alone)."""
+    import os
+    path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "pinfty256.bin")
+    raw = np.fromfile(path, dtype=np.uint8)
+    n = int(raw[:4].view(np.int32)[0])
+    vals = raw[4:].view(np.float64)
+    dirs = np.stack([vals[0:n], vals[n:2 * n], vals[2 * n:3 * n]], axis=1)
+    rest = vals[3 * n:]
+    w = rest[3 * n:4 * n] if rest.size == 4 * n else rest[:n]
+    return np.ascontiguousarray(dirs), np.ascontiguousarray(w)

@@ -106,8 +106,37 @@ def test_golden_spectrum_on_gpu(cuda):
     for precision in ("double", "mixed"):
         print("golden", precision, flush=True)
         bem = m.BEMRet(cp, precision=precision)
-        ext = bem.solve_sweep(py[:, 0], m.PlaneWaveRet([1, 0, 0], [0, 0, 1]), chunk=41)[:, 0]
-        assert np.max(np.abs(ext / py[:, 1] - 1)) < 2e-10
+        ext, sca = bem.solve_sweep(py[:, 0], m.PlaneWaveRet([1, 0, 0], [0, 0, 1]), chunk=41, with_sca=True)
+        assert np.max(np.abs(ext[:, 0] / py[:, 1] - 1)) < 2e-10
+        # scattering and absorption columns of the same files (sca printed by both runs)
+        ml = np.loadtxt(os.path.join(GOLD, "sphere144_d20_goldtable_bemret_matlab.csv"), delimiter=",", skiprows=1)
+        assert np.max(np.abs(sca[:, 0] / ml[:, 2] - 1)) < 2e-10
+        assert np.max(np.abs((ext[:, 0] - sca[:, 0]) / ml[:, 3] - 1)) < 1e-9
+
+
+def test_scattering_matches_oracle(setup284):
+    """Far fields over the 508-direction sphere and the scattering cross
+    section vs the oracle, two polarisations, several wavelengths."""
+    import mnpbem_amd as m
+    cp, comp, ref = setup284
+    pol, dr = [[1, 0, 0], [0, 1, 0]], [[0, 0, 1], [0, 0, 1]]
+    exc = m.PlaneWaveRet(pol, dr)
+    bem = m.BEMRet(cp)
+    for lam in (450.0, 530.0, 700.0):
+        print("scattering", lam, flush=True)
+        sig, _ = bem.solve(exc.potential(cp, lam))
+        sca, dsca = exc.scattering(sig)
+        sig_o = o.BEMRetOracle(ref).solve(o.planewave_potential(comp, pol, dr, lam))
+        sca_o = o.scattering(comp, sig_o, lam)
+        assert np.max(np.abs(sca / sca_o - 1)) < 1e-10
+        assert dsca.dsca.shape == (508, 2) and np.all(dsca.dsca > 0)
+        ab = exc.absorption(sig)
+        assert np.all(ab > 0)
+        dirs, _ = o.read_pinfty()
+        e_o, h_o = o.farfield(comp, sig_o, dirs, lam)
+        _, _, E, H = exc.scattering_batch(cp, [lam], sig.get("_X"), bem.g, fields=True)
+        assert np.abs(E[0].cpu().numpy() - e_o).max() < 1e-10 * np.abs(e_o).max()
+        assert np.abs(H[0].cpu().numpy() - h_o).max() < 1e-10 * np.abs(h_o).max()
 
 
 def test_sweep_chunking_is_consistent(setup284):
