@@ -82,6 +82,13 @@ def main():
     ap.add_argument("--chunk", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the complex128 comparison sweep")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device with the library's plain copy kernel "
+                         "(k_copy_cast<double>), the reference point that scales HBM byte counters")
+    ap.add_argument("--traffic", type=str, default=os.path.join(ROOT, "profiles", "k_cgemm_sub_bf16x3_traffic_n2884.md"),
+                    help="rocprofv3 summary holding the dominant kernel's measured HBM bytes per launch "
+                         "(lines 'fetch_bytes_per_launch: F' and 'write_bytes_per_launch: W', from FETCH_SIZE / "
+                         "WRITE_SIZE counter passes scaled by a plain copy) to report as roofline.traffic")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "mixed-fp32"],
                     help="mixed: complex64 GEMMs on bf16 matrix cores (3 split products); "
                          "mixed-fp32: on FP32 matrix cores")
@@ -96,6 +103,14 @@ def main():
     from mnpbem_amd import _native as nat
 
     dev = torch.device("cuda:0")
+    if args.copy_probe > 0:
+        cnt = args.copy_probe * (1 << 20) // 8
+        src = torch.ones(cnt, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        nat.call("mnpbem_convert", nat.DTYPE_C128, cnt, nat.ptr(src), nat.ptr(dst), nat.stream_handle(dev))
+        torch.cuda.synchronize()
+        log("copy probe: %d bytes read, %d bytes written by k_copy_cast<double>" % (cnt * 8, cnt * 8))
+        del src, dst
     lams = np.linspace(400.0, 800.0, 64)
     pol = [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]]
     dr = [[0.0, 0.0, 1.0], [0.0, 0.0, 1.0]]
@@ -125,6 +140,8 @@ def main():
     import ctypes
     ms, fl, nl = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
     nat.call("mnpbem_prof_end", ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(nl))
+    by = ctypes.c_double()
+    nat.call("mnpbem_prof_bytes", ctypes.byref(by))
     elapsed = t1 - t0
     value = len(lams) * args.steps / elapsed
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -164,6 +181,15 @@ def main():
         log("cpu oracle: %.3f wavelengths/s on %d threads; mixed vs oracle max rel ext err %.2e"
             % (cpu_rate, cores, acc_vs_oracle))
 
+    traffic = None
+    if args.traffic and args.precision == "mixed" and os.path.exists(args.traffic):
+        tr = {}
+        with open(args.traffic) as fh:
+            for ln in fh:
+                if ":" in ln and ln.split(":")[0].strip() in ("fetch_bytes_per_launch", "write_bytes_per_launch"):
+                    tr[ln.split(":")[0].strip()] = int(ln.split(":")[1])
+        if len(tr) == 2:
+            traffic = tr["fetch_bytes_per_launch"] + tr["write_bytes_per_launch"]
     if args.precision == "mixed":
         kname = ("k_cgemm_sub_bf16x3 (complex64 GEMM on bf16 MFMA, 3 split products per real product; "
                  "achieved = 8MNK algorithmic flops / time, peak = bf16 dense peak / 3)")
@@ -194,7 +220,9 @@ def main():
         "path_tflops_W": round(path_flops / elapsed / 1e12, 3),
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": round(gemm_achieved / 1e12, 3),
                      "peak": round(peak / 1e12, 1), "unit": "TFLOP/s", "frac": round(gemm_achieved / peak, 4),
-                     "traffic": None, "launches": nl.value,
+                     "traffic": traffic,
+                     "compulsory_bytes_per_launch": round(by.value / max(nl.value, 1)),
+                     "launches": nl.value,
                      "avg_launch_ms": round(ms.value / max(nl.value, 1), 4)},
         "cpu_baseline": cpu,
     }

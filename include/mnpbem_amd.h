@@ -239,6 +239,8 @@ int mnpbem_scattering_batched(int n, int P, const double* X, long ldv, long sv, 
  * (host pointers for the results of mnpbem_prof_end) */
 int mnpbem_prof_begin(int capacity);
 int mnpbem_prof_end(double* total_ms, double* total_flops, long* launches);
+/* compulsory bytes of the timed launches: A, B read once, C read and written once */
+int mnpbem_prof_bytes(double* total_bytes);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,7 @@ SIGNATURES = {
     "mnpbem_scattering_batched": [_i, _i, _p, _l, _l, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p],
     "mnpbem_prof_begin": [_i],
     "mnpbem_prof_end": [_p, _p, _p],
+    "mnpbem_prof_bytes": [_p],
 }
 
 _lib = None

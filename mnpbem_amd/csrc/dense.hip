@@ -588,6 +588,7 @@ struct GemmProf {
   bool on = false;
   std::vector<hipEvent_t> ev;
   std::vector<double> flops;
+  double bytes = 0.0;  // compulsory bytes: A and B read once, C read and written once
   size_t used = 0;
 };
 GemmProf g_prof;
@@ -616,6 +617,7 @@ int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, lon
     MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used + 1], st));
     g_prof.used += 2;
     g_prof.flops.push_back(8.0 * M * (double)N * K * batch);
+    g_prof.bytes += (double)batch * 2.0 * sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N);
   }
   return 0;
 }
@@ -827,6 +829,7 @@ extern "C" int mnpbem_prof_begin(int capacity) {
   }
   g_prof.used = 0;
   g_prof.flops.clear();
+  g_prof.bytes = 0.0;
   g_prof.on = true;
   return 0;
 }
@@ -846,5 +849,12 @@ extern "C" int mnpbem_prof_end(double* total_ms, double* total_flops, long* laun
   *total_ms = ms;
   *total_flops = fl;
   *launches = (long)(g_prof.used / 2);
+  return 0;
+}
+
+// Compulsory HBM bytes of the launches timed since mnpbem_prof_begin (each
+// operand read once, C read and written once).
+extern "C" int mnpbem_prof_bytes(double* total_bytes) {
+  *total_bytes = g_prof.bytes;
   return 0;
 }
