@@ -8,9 +8,10 @@ ACACompGreenRet), PlaneWaveRet and CompStruct.  Device work runs in libmnpbem_am
 """
 
 from .materials import EV2NM, EpsConst, EpsDrude, EpsTable
-from .geometry import ComParticle, Particle, trisphere
+from .geometry import ComParticle, Particle, triellipsoid, trisphere, trisphere_fibonacci
 
 __all__ = ["EV2NM", "EpsConst", "EpsDrude", "EpsTable", "ComParticle", "Particle", "trisphere",
+           "trisphere_fibonacci", "triellipsoid",
            "CompGreenRet", "BEMRet", "CompStruct", "PlaneWaveRet", "BEMIter", "BEMRetIter", "ClusterTree",
            "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss"]
 

@@ -1,9 +1,12 @@
 """Particle surfaces of quadratic (6-node) triangles, the sphere mesh
 generator and the composite particle (host-side set-up).
 
-Mirrors mnpbem/geometry: ``trisphere`` (mesh_generators.py:13-130),
-``Particle`` with curved interpolation (particle.py:310-613) and
-``ComParticle`` (comparticle.py:55-127).  Only what the retarded BEM path
+Mirrors mnpbem/geometry: ``trisphere`` (mesh_generators.py:13-130), the
+golden-spiral meshes ``trisphere_fibonacci`` and ``triellipsoid``
+(mesh_generators.py:310-473), ``Particle`` with curved or flat interpolation
+(particle.py:310-613; a flat triangle is carried as the quadratic element
+whose edge nodes are the straight-edge midpoints) and ``ComParticle``
+(comparticle.py:55-127).  Only what the retarded BEM path
 needs is kept: centroids, normals, tangents, areas, element radii and the
 quadrature nodes that the GPU refinement kernel integrates over.
 """
@@ -44,14 +47,21 @@ class Particle:
     """
 
     def __init__(self, verts2, faces6, interp="curv"):
-        if interp != "curv":
-            raise NotImplementedError("only curved (6-node) boundary elements are supported")
-        self.verts2 = np.asarray(verts2, dtype=float)
-        self.faces6 = np.asarray(faces6, dtype=np.int64)
-        if self.verts2.ndim != 2 or self.verts2.shape[1] != 3:
-            raise ValueError("Particle: 'verts' must have shape (N, 3); got %s." % (self.verts2.shape,))
-        if self.faces6.ndim != 2 or self.faces6.shape[1] != 6:
-            raise ValueError("Particle: faces must list 6 nodes per triangle")
+        if interp not in ("curv", "flat"):
+            raise ValueError("Particle: interp must be 'curv' or 'flat'")
+        verts2 = np.asarray(verts2, dtype=float)
+        faces6 = np.asarray(faces6, dtype=np.int64)
+        if verts2.ndim != 2 or verts2.shape[1] != 3:
+            raise ValueError("Particle: 'verts' must have shape (N, 3); got %s." % (verts2.shape,))
+        if faces6.ndim == 2 and faces6.shape[1] == 3:
+            # plain triangles: edge midpoints on the straight edges, so the
+            # quadratic element is the flat triangle itself (its shape
+            # functions reproduce the linear map exactly)
+            verts2, faces6 = add_midpoints(verts2, faces6)
+        if faces6.ndim != 2 or faces6.shape[1] != 6:
+            raise ValueError("Particle: faces must list 3 corners or 6 nodes per triangle")
+        self.verts2 = verts2
+        self.faces6 = faces6
         self.interp = interp
         self._geometry()
 
@@ -90,6 +100,10 @@ class Particle:
         nv = np.cross(a, b)
         nv /= np.maximum(np.linalg.norm(nv, axis=1, keepdims=True), 1e-14)
         a /= np.maximum(np.linalg.norm(a, axis=1, keepdims=True), 1e-14)
+        if self.interp == "flat":
+            # particle.py _norm_flat: tvec1 along the edge from corner 1 to corner 0
+            a = X[:, 0, :] - X[:, 1, :]
+            a /= np.maximum(np.linalg.norm(a, axis=1, keepdims=True), 1e-14)
         self._nvec, self._t1 = nv, a
         self._t2 = np.cross(nv, a)
         corners = X[:, :3, :]
@@ -99,6 +113,83 @@ class Particle:
         """Radius of the smallest centroid-centred sphere through the corners
         (particle.py:2848-2880)."""
         return self._brad
+
+
+def add_midpoints(verts, faces):
+    """Append one node per edge at the edge midpoint: returns (nodes,
+    faces6) with faces6 = (v0, v1, v2, m01, m12, m20)
+    (mesh_generators.py:133-196)."""
+    verts = np.asarray(verts, dtype=float)
+    faces = np.asarray(faces, dtype=np.int64)
+    edges = np.sort(np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]]), axis=1)
+    uniq, inv = np.unique(edges, axis=0, return_inverse=True)
+    mid = 0.5 * (verts[uniq[:, 0]] + verts[uniq[:, 1]])
+    nodes = np.vstack([verts, mid])
+    faces6 = np.hstack([faces, len(verts) + inv.reshape(3, -1).T])
+    return nodes, faces6
+
+
+def fibonacci_sphere(n):
+    """n points of the golden-angle spiral on the unit sphere
+    (mesh_generators.py:341-370; the same arithmetic, so the points and the
+    triangulations built on them are identical)."""
+    k = np.arange(0, n, dtype=float) + 0.5
+    polar = np.arccos(1 - 2 * k / n)
+    azim = np.pi * (1 + 5 ** 0.5) * k
+    pts = np.column_stack([np.sin(polar) * np.cos(azim), np.sin(polar) * np.sin(azim), np.cos(polar)])
+    return pts / np.linalg.norm(pts, axis=1, keepdims=True)
+
+
+def sphere_triangulate(verts):
+    """Triangulate points on the unit sphere (mesh_generators.py:373-443,
+    MATLAB sphtriangulate.m): rotate the first point to the south pole,
+    project the others stereographically onto the plane, take their
+    Delaunay triangulation (oriented outward) and close the surface with a
+    fan from the first point to the planar convex hull."""
+    from scipy.spatial import ConvexHull, Delaunay
+    verts = np.asarray(verts, dtype=float)
+    n = len(verts)
+    c = verts[0]
+    e3 = -c
+    e2 = np.array([0, -e3[2], e3[1]]) if c[2] != 0 else np.array([-e3[1], e3[0], 0])
+    e2 = e2 / np.linalg.norm(e2)
+    e1 = np.cross(e3, e2)
+    rot = np.array([e1, e2, e3])
+    vr = verts[1:] @ rot.T
+    t = -1.0 / (vr[:, 2] + 1)
+    xy = np.column_stack([vr[:, 0] * t, vr[:, 1] * t])
+    tri = Delaunay(xy).simplices
+    p3 = np.column_stack([xy, np.zeros(n - 1)])
+    w = np.cross(p3[tri[:, 0]] - p3[tri[:, 1]], p3[tri[:, 2]] - p3[tri[:, 1]])
+    flip = w[:, 2] > 0
+    tri[flip, :] = tri[flip, :][:, [0, 2, 1]]
+    hv = ConvexHull(xy).vertices
+    nh = len(hv)
+    fan = np.array([[hv[(i + 1) % nh] + 1, hv[i] + 1, 0] for i in range(nh)], dtype=np.int64)
+    return np.vstack([tri + 1, fan]).astype(np.int64)
+
+
+def trisphere_fibonacci(n, diameter=1.0, **kwargs):
+    """Sphere of curved triangles on n golden-spiral points: Delaunay on the
+    sphere, midpoints added and all nodes projected onto the sphere
+    (mesh_generators.py:310-338)."""
+    verts = fibonacci_sphere(int(n))
+    faces = sphere_triangulate(verts)
+    nodes, faces6 = add_midpoints(verts * (diameter / 2.0), faces)
+    nodes = (diameter / 2.0) * (nodes / np.linalg.norm(nodes, axis=1, keepdims=True))
+    kwargs.setdefault("interp", "curv")
+    return Particle(nodes, faces6, **kwargs)
+
+
+def triellipsoid(n, axes):
+    """Ellipsoid of flat triangles with semi-axes ``axes`` on n golden-spiral
+    points (mesh_generators.py:446-473)."""
+    axes = np.asarray(axes, dtype=float).reshape(3)
+    if np.any(axes <= 0):
+        raise ValueError("triellipsoid: semi-axes must be positive")
+    verts = fibonacci_sphere(int(n))
+    faces = sphere_triangulate(verts)
+    return Particle(verts * axes, faces, interp="flat")
 
 
 def trisphere(n, diameter=1.0, **kwargs):
@@ -114,13 +205,9 @@ def trisphere(n, diameter=1.0, **kwargs):
         raise ValueError("trisphere: 'diameter' must be a positive finite float, got %r." % diameter)
     m = int(SPHERE_SIZES[np.argmin(np.abs(SPHERE_SIZES - n))])
     verts, faces = _read_sphere(m)
-    verts = verts * (0.5 * diameter)
-    edges = np.sort(np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]]), axis=1)
-    uniq, inv = np.unique(edges, axis=0, return_inverse=True)
-    mid = 0.5 * (verts[uniq[:, 0]] + verts[uniq[:, 1]])
-    nodes = np.vstack([verts, mid])
-    nodes = nodes * (0.5 * diameter / np.linalg.norm(nodes, axis=1))[:, None]
-    faces6 = np.hstack([faces, len(verts) + inv.reshape(3, -1).T])
+    # same arithmetic order as mesh_generators.py:104-118, so nodes are bit-identical
+    nodes, faces6 = add_midpoints(0.5 * verts * diameter, faces)
+    nodes = 0.5 * diameter * (nodes / np.linalg.norm(nodes, axis=1, keepdims=True))
     return Particle(nodes, faces6, **kwargs)
 
 

@@ -36,7 +36,8 @@ from scipy.interpolate import CubicSpline
 __all__ = [
     "EV2NM", "EpsConst", "EpsDrude", "EpsTable",
     "lglnodes", "rule18", "polar_rule_tri", "tri6_shape", "tri6_deriv",
-    "read_tri_bin", "trisphere_mesh", "CurvedSurface", "Composite",
+    "read_tri_bin", "trisphere_mesh", "golden_spiral", "spherical_delaunay", "fibonacci_sphere_mesh",
+    "triellipsoid_mesh", "CurvedSurface", "Composite",
     "refinematrix", "refine_coefficients", "closed_surface_correction",
     "GreenSetup", "green_matrices", "BEMRetOracle", "planewave_potential",
     "farfield", "extinction", "scattering", "read_pinfty",
@@ -215,16 +216,72 @@ def trisphere_mesh(nverts, diameter):
     """
     n = int(_SAVED[np.argmin(np.abs(_SAVED - nverts))])
     verts, faces = read_tri_bin(os.path.join(DATA_DIR, "sphere%d_tri.bin" % n))
-    verts = 0.5 * diameter * verts
+    verts = 0.5 * verts * diameter  # arithmetic order of mesh_generators.py:104
     nv = len(verts)
     e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
     e = np.sort(e, axis=1)
     uniq, inv = np.unique(e, axis=0, return_inverse=True)
     inv = inv.reshape(3, -1).T                                   # (nf, 3): edges 01, 12, 20
     nodes = np.vstack([verts, 0.5 * (verts[uniq[:, 0]] + verts[uniq[:, 1]])])
-    nodes = 0.5 * diameter * nodes / np.linalg.norm(nodes, axis=1, keepdims=True)
+    nodes = 0.5 * diameter * (nodes / np.linalg.norm(nodes, axis=1, keepdims=True))
     tri6 = np.hstack([faces, nv + inv])
     return nodes, tri6
+
+
+def _edge_nodes(verts, faces):
+    """Straight-edge midpoints appended as nodes 3..5 of every face."""
+    nv = len(verts)
+    e = np.sort(np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]]), axis=1)
+    uniq, inv = np.unique(e, axis=0, return_inverse=True)
+    nodes = np.vstack([verts, 0.5 * (verts[uniq[:, 0]] + verts[uniq[:, 1]])])
+    return nodes, np.hstack([faces, nv + inv.reshape(3, -1).T])
+
+
+def golden_spiral(n):
+    """mesh_generators.py:341-370: arccos(1 - 2 (k + 1/2) / n) polar angles,
+    pi (1 + sqrt 5) (k + 1/2) azimuths, renormalised."""
+    kh = np.arange(0, n, dtype=float) + 0.5
+    th = np.arccos(1 - 2 * kh / n)
+    ph = np.pi * (1 + 5 ** 0.5) * kh
+    p = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=1)
+    return p / np.linalg.norm(p, axis=1, keepdims=True)
+
+
+def spherical_delaunay(v):
+    """mesh_generators.py:373-443: stereographic projection from the first
+    point (rotated to [0, 0, -1]), planar Delaunay oriented outward, fan from
+    the first point over the planar hull."""
+    from scipy.spatial import ConvexHull, Delaunay
+    r3 = -v[0]
+    r2 = np.array([0, -r3[2], r3[1]]) if v[0][2] != 0 else np.array([-r3[1], r3[0], 0])
+    r2 = r2 / np.linalg.norm(r2)
+    R = np.array([np.cross(r3, r2), r2, r3])
+    q = v[1:] @ R.T
+    s = -1.0 / (q[:, 2] + 1)
+    uv = np.stack([q[:, 0] * s, q[:, 1] * s], axis=1)
+    f = Delaunay(uv).simplices
+    pz = np.concatenate([uv, np.zeros((len(uv), 1))], axis=1)
+    orient = np.cross(pz[f[:, 0]] - pz[f[:, 1]], pz[f[:, 2]] - pz[f[:, 1]])[:, 2]
+    f[orient > 0] = f[orient > 0][:, [0, 2, 1]]
+    h = ConvexHull(uv).vertices
+    fan = np.array([[h[(i + 1) % len(h)] + 1, h[i] + 1, 0] for i in range(len(h))])
+    return np.concatenate([f + 1, fan]).astype(np.int64)
+
+
+def fibonacci_sphere_mesh(n, diameter):
+    """_trisphere_fibonacci (mesh_generators.py:310-338): curved sphere on
+    the golden spiral, nodes projected onto the sphere.  Returns (nodes, tri6)."""
+    v = golden_spiral(n)
+    nodes, tri6 = _edge_nodes(v * (diameter / 2.0), spherical_delaunay(v))
+    return (diameter / 2.0) * (nodes / np.linalg.norm(nodes, axis=1, keepdims=True)), tri6
+
+
+def triellipsoid_mesh(n, axes):
+    """triellipsoid (mesh_generators.py:446-473): golden-spiral sphere scaled
+    by the semi-axes, flat triangles (returned as quadratic elements with
+    straight-edge midpoints, which integrate exactly as the flat triangle)."""
+    v = golden_spiral(n)
+    return _edge_nodes(v * np.asarray(axes, dtype=float), spherical_delaunay(v))
 
 
 # --------------------------------------------------------------------------

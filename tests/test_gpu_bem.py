@@ -148,3 +148,26 @@ def test_sweep_chunking_is_consistent(setup284):
     a = bem.solve_sweep(lams, exc, chunk=1)
     b = bem.solve_sweep(lams, exc, chunk=7)
     assert np.max(np.abs(a / b - 1)) < 1e-13
+
+
+@pytest.mark.parametrize("kind", ["ellipsoid", "fibonacci"])
+def test_bemret_golden_spiral_meshes_match_oracle(cuda, kind):
+    """Flat-triangle ellipsoid (triellipsoid) and curved Fibonacci sphere:
+    extinction of both precisions vs the oracle on the same mesh."""
+    import mnpbem_amd as m
+    if kind == "ellipsoid":
+        p = m.triellipsoid(144, [8.0, 12.0, 16.0])
+        nodes, tri6 = o.triellipsoid_mesh(144, [8.0, 12.0, 16.0])
+    else:
+        p = m.trisphere_fibonacci(200, 20.0)
+        nodes, tri6 = o.fibonacci_sphere_mesh(200, 20.0)
+    print("golden-spiral mesh", kind, p.n, flush=True)
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [p], [[2, 1]], 1)
+    comp = o.Composite([o.EpsConst(1.0), o.EpsDrude(*DRUDE)], [o.CurvedSurface(nodes, tri6)], [[2, 1]])
+    pol, dr = [[1, 0, 0], [0, 0, 1]], [[0, 0, 1], [1, 0, 0]]
+    lams = np.array([450.0, 540.0, 690.0])
+    ext_ref = o.solve_spectrum(comp, lams, pol, dr)
+    for precision, tol in (("double", 1e-10), ("mixed", 1e-9)):
+        ext = m.BEMRet(cp, precision=precision).solve_sweep(lams, m.PlaneWaveRet(pol, dr))
+        assert np.max(np.abs(ext / ext_ref - 1)) < tol, (precision, np.max(np.abs(ext / ext_ref - 1)))
+
