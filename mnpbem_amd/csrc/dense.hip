@@ -681,28 +681,30 @@ int laswp(int k0, int k1, int clo, int chi, T* A, long lda, long sA, const int* 
   return 0;
 }
 
-// leaf configurations: W panel columns, S rows per thread, 512 threads.  The
-// panel (S*W complex per thread) stays in registers (at most 96 values of
-// either precision per thread), so a narrower panel is used for taller
-// matrices.
+// leaf configurations: W panel columns, S rows per thread, NT threads.  The
+// panel (S*W complex per thread) stays in registers without spilling, so the
+// panel narrows as the remaining rows m = N - c0 grow.  complex64: W = 8 up
+// to 3072 rows, 4 up to 6144, 2 up to 12288 (512 threads), then one column
+// with 1024 threads up to 32768 rows.  complex128: 8 up to 3072, 4 up to
+// 6144, then one column with 512 threads up to 20480 rows.
 template <typename T>
-struct LeafCfg;
-template <>
-struct LeafCfg<double> {
-  static int width(int N) { return N <= 3072 ? 8 : (N <= 6144 ? 4 : 2); }
-};
-template <>
-struct LeafCfg<float> {
-  static int width(int N) { return N <= 3072 ? 8 : (N <= 6144 ? 4 : 2); }
-};
+int leaf_width(int m) {
+  if (m <= 3072) return 8;
+  if (m <= 6144) return 4;
+  if (sizeof(T) == 4) {
+    if (m <= 12288) return 2;
+    return m <= 32768 ? 1 : 0;
+  }
+  return m <= 20480 ? 1 : 0;
+}
 
-template <typename T, int W, int SMAX>
+template <typename T, int W, int SMAX, int NT>
 int leaf_launch_w(int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
                   hipStream_t st) {
   const int m = N - c0;
 #define MNP_LEAF(S_)                                                                                       \
-  if (m <= (S_)*512) {                                                                                     \
-    hipLaunchKernelGGL((k_lu_leaf<T, W, S_, 512>), dim3(batch), dim3(512), 0, st, N, c0, w, A, lda, sA, ipiv, \
+  if (m <= (S_)*NT) {                                                                                      \
+    hipLaunchKernelGGL((k_lu_leaf<T, W, S_, NT>), dim3(batch), dim3(NT), 0, st, N, c0, w, A, lda, sA, ipiv,  \
                        sP, info);                                                                          \
     MNP_LAUNCH_CHECK();                                                                                    \
     return 0;                                                                                              \
@@ -717,6 +719,12 @@ int leaf_launch_w(int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long
   if constexpr (SMAX >= 24) {
     MNP_LEAF(24)
   }
+  if constexpr (SMAX >= 32) {
+    MNP_LEAF(32)
+  }
+  if constexpr (SMAX >= 40) {
+    MNP_LEAF(40)
+  }
 #undef MNP_LEAF
   mnp_set_error("getrf: matrix too tall for the register-resident panel");
   return 1;
@@ -725,27 +733,33 @@ int leaf_launch_w(int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long
 template <typename T>
 int leaf_launch(int W, int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
                 hipStream_t st) {
-  if constexpr (sizeof(T) == 8) {
-    if (W == 8) return leaf_launch_w<T, 8, 6>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
-    if (W == 4) return leaf_launch_w<T, 4, 12>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
-    return leaf_launch_w<T, 2, 24>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+  if (W == 8) return leaf_launch_w<T, 8, 6, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+  if (W == 4) return leaf_launch_w<T, 4, 12, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+  if constexpr (sizeof(T) == 4) {
+    if (W == 2) return leaf_launch_w<T, 2, 24, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+    if (W == 1) return leaf_launch_w<T, 1, 32, 1024>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
   } else {
-    if (W == 8) return leaf_launch_w<T, 8, 6>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
-    if (W == 4) return leaf_launch_w<T, 4, 12>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
-    return leaf_launch_w<T, 2, 24>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+    if (W == 1) return leaf_launch_w<T, 1, 40, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
   }
+  mnp_set_error("getrf: matrix too tall for the register-resident panel");
+  return 1;
 }
 
 // recursive LU of columns [c0, c0+n) (rows c0..N-1)
 template <typename T>
-int getrf_rec(int W, int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
+int getrf_rec(int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
               hipStream_t st, int gmode) {
+  const int W = leaf_width<T>(N - c0);
+  if (W == 0) {
+    mnp_set_error("getrf: matrix too tall for the register-resident panel");
+    return 1;
+  }
   if (n <= W) return leaf_launch<T>(W, N, c0, n, A, lda, sA, ipiv, sP, info, batch, st);
   int n1 = ((n / 2 + W - 1) / W) * W;
   if (n1 >= n) n1 = n - W;
   const int n2 = n - n1;
   int rc;
-  if ((rc = getrf_rec(W, N, c0, n1, A, lda, sA, ipiv, sP, info, batch, st, gmode))) return rc;
+  if ((rc = getrf_rec(N, c0, n1, A, lda, sA, ipiv, sP, info, batch, st, gmode))) return rc;
   if ((rc = laswp<T>(c0, c0 + n1, c0 + n1, c0 + n, A, lda, sA, ipiv, sP, batch, false, st))) return rc;
   T* A11 = A + 2 * ((long)c0 * lda + c0);
   T* A12 = A + 2 * ((long)c0 * lda + c0 + n1);
@@ -753,7 +767,7 @@ int getrf_rec(int W, int N, int c0, int n, T* A, long lda, long sA, int* ipiv, l
   T* A22 = A + 2 * ((long)(c0 + n1) * lda + c0 + n1);
   if ((rc = trsm_rec<T>(true, n1, n2, A11, lda, sA, A12, lda, sA, batch, st, gmode))) return rc;
   if ((rc = gemm_sub<T>(N - c0 - n1, n2, n1, A21, lda, sA, A12, lda, sA, A22, lda, sA, batch, st, gmode))) return rc;
-  if ((rc = getrf_rec(W, N, c0 + n1, n2, A, lda, sA, ipiv, sP, info, batch, st, gmode))) return rc;
+  if ((rc = getrf_rec(N, c0 + n1, n2, A, lda, sA, ipiv, sP, info, batch, st, gmode))) return rc;
   return laswp<T>(c0 + n1, c0 + n, c0, c0 + n1, A, lda, sA, ipiv, sP, batch, false, st);
 }
 
@@ -781,14 +795,18 @@ extern "C" int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA
                                     int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n >= 0 && batch >= 0 && lda >= n, "getrf: bad size");
+  MNP_REQUIRE(dtype == MNP_DTYPE_C128 || mnp_is_c64(dtype), "getrf: unknown dtype");
+  MNP_REQUIRE(dtype != MNP_DTYPE_C128 || leaf_width<double>(n) > 0,
+              "getrf: complex128 matrices are limited to 20480 rows");
+  MNP_REQUIRE(dtype == MNP_DTYPE_C128 || leaf_width<float>(n) > 0,
+              "getrf: complex64 matrices are limited to 32768 rows");
   if (n == 0 || batch == 0) return 0;
   hipLaunchKernelGGL(k_fill_int, dim3(mnp_ceil_div(batch, 256)), dim3(256), 0, st, info, (long)batch, 0);
   MNP_LAUNCH_CHECK();
   if (dtype == MNP_DTYPE_C128)
-    return getrf_rec<double>(LeafCfg<double>::width(n), n, 0, n, (double*)A, lda, sA, ipiv, sP, info, batch, st, 0);
+    return getrf_rec<double>(n, 0, n, (double*)A, lda, sA, ipiv, sP, info, batch, st, 0);
   if (mnp_is_c64(dtype))
-    return getrf_rec<float>(LeafCfg<float>::width(n), n, 0, n, (float*)A, lda, sA, ipiv, sP, info, batch, st,
-                            dtype == MNP_DTYPE_C64X ? 1 : 0);
+    return getrf_rec<float>(n, 0, n, (float*)A, lda, sA, ipiv, sP, info, batch, st, dtype == MNP_DTYPE_C64X ? 1 : 0);
   mnp_set_error("getrf: unknown dtype");
   return 1;
 }

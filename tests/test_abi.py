@@ -48,5 +48,15 @@ def test_invalid_arguments_are_rejected_without_device_work():
     assert b"getrf" in lib.mnpbem_last_error()
     rc = lib.mnpbem_refine_coeffs(9, 0, *([None] * 10), 28, None, 144, None, None, None, None)
     assert rc == 1
+    # panels beyond the register-resident limits are refused up front
+    with pytest.raises(RuntimeError, match="limited to 20480"):
+        _native.call("mnpbem_getrf_batched", _native.DTYPE_C128, 30000, None, 30000, 0, None, 30000, None, 1, None)
+    with pytest.raises(RuntimeError, match="limited to 32768"):
+        _native.call("mnpbem_getrf_batched", _native.DTYPE_C64, 40000, None, 40000, 0, None, 40000, None, 1, None)
+    # ACA rank bound and H-matrix column limit
+    with pytest.raises(RuntimeError, match="kmax"):
+        _native.call("mnpbem_hmat_aca", 1, None, None, None, None, 1, 1, None, None, None, 600, 1e-6, None, None)
+    with pytest.raises(RuntimeError, match="ncol"):
+        _native.call("mnpbem_hmat_matvec", 2, 1, 1, 1, *([None] * 7), 17, None, 17, None, 17, 0, None, None)
     with pytest.raises(RuntimeError, match="order"):
         _native.call("mnpbem_refine_coeffs", 9, 0, *([None] * 10), 28, None, 144, None, None, None, None)

@@ -128,3 +128,36 @@ def test_lu_solve_transposed(cuda, dtype, tol, n, C, blockinv):
              nat.ptr(tZ), C, n * C, nat.ptr(D), sD, B, st)
     err = np.abs(tV.cpu().numpy() - ref).max() / np.abs(ref).max()
     assert err < tol, err
+
+
+@pytest.mark.parametrize("dtype,n,tol", [(np.complex64, 14000, 2e-4), (np.complex128, 13000, 1e-11),
+                                         (np.complex64, 6500, 1e-4)])
+def test_getrf_tall_panels(cuda, dtype, n, tol):
+    """Matrices taller than the 8-column register panel allows (one- and
+    two-column panels for the first columns): solve A x = b and check the
+    residual in FP64 on the device (diagonally weighted random A)."""
+    nat = _lib()
+    print("getrf tall", dtype.__name__, n, flush=True)
+    g = torch.Generator(device=cuda).manual_seed(n)
+    cdt = torch.complex64 if dtype == np.complex64 else torch.complex128
+    A = (torch.randn((n, n), dtype=cdt, device=cuda, generator=g) / np.sqrt(n)
+         + 3.0 * torch.eye(n, dtype=cdt, device=cuda))
+    x0 = torch.randn((n, 2), dtype=cdt, device=cuda, generator=g)
+    b = (A.to(torch.complex128) @ x0.to(torch.complex128)).to(cdt)
+    LU = A.clone()
+    piv = torch.empty((n,), dtype=torch.int32, device=cuda)
+    info = torch.empty((1,), dtype=torch.int32, device=cuda)
+    dt = nat.DTYPE_C128 if dtype == np.complex128 else nat.DTYPE_C64
+    st = nat.stream_handle(cuda)
+    nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(LU), n, n * n, nat.ptr(piv), n, nat.ptr(info), 1, st)
+    x = b.clone()
+    nat.call("mnpbem_getrs_batched", dt, n, 2, nat.ptr(LU), n, n * n, nat.ptr(piv), n, nat.ptr(x), 2, 2 * n, 1, st)
+    assert int(info.cpu()[0]) == 0
+    p = piv.cpu().numpy()
+    assert np.all(p >= np.arange(n)) and np.all(p < n)
+    r = b.to(torch.complex128) - A.to(torch.complex128) @ x.to(torch.complex128)
+    rel = (torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b.to(torch.complex128))).item()
+    assert rel < tol, rel
+    del A, LU, x, b
+
+
