@@ -82,6 +82,16 @@ int mnpbem_identity_batched(int dtype, int n, void* A, long ld, long sA, int bat
 /* ---- Green functions (replaces mnpbem/greenfun/greenret_refined.py:78-867,
  *      compgreen_ret.py:192-505 and compgreen_stat.py:219-465) ---- */
 
+/* Refinement classification (replaces refinematrix, mnpbem/greenfun/
+ * refine_utils.py:12-158): pair (i, j) gets kind 2 when the centroid distance
+ * is zero, kind 1 when (d - r_j) / r_i < rel_cutoff or d - r_j < abs_cutoff,
+ * d^2 = |a|^2 + |b|^2 - 2 a.b with squares < 1e-10 set to 0.  pos: n x 3,
+ * brad: element radii.  pass 0 writes cnt[i] (pairs per row); pass 1 writes
+ * the pairs of row i, columns ascending, from off[i] (int64 exclusive prefix
+ * sums of cnt) into prow / pcol / pkind (int32). */
+int mnpbem_refine_classify(int n, const double* pos, const double* brad, double rel_cutoff, double abs_cutoff,
+                           int pass, int* cnt, const long* off, int* prow, int* pcol, int* pkind, void* stream);
+
 /* Order-`order` (<= 7) expansion coefficients of refined face pairs
  * (deriv='cart'): for pair p = (row i, col j, kind 2 = self, 1 = near)
  *   g[p][n], f[p][c][n] (c = x,y,z) and the quasistatic normal derivative

@@ -339,6 +339,57 @@ __global__ __launch_bounds__(256) void k_green_overlay(
 
 }  // namespace
 
+// Refinement classification (refine_utils.py:29-158): pair (i, j) is refined
+// with kind 2 where the centroid distance is zero and kind 1 where
+// (d - r_j) / r_i < rel or d - r_j < abs.  The distance follows
+// refine_utils.py:12-26: d^2 = (|a|^2 + |b|^2) - 2 a.b with squares below
+// 1e-10 set to zero; |a|^2 is summed x, y, z in order and a.b accumulated with
+// fused multiply-adds, as numpy's row sums and its BLAS product do, so pairs
+// on the cutoff boundary fall on the same side.  One wave per row i scans all
+// columns in chunks of 64; pass 0 counts (cnt[i]), pass 1 writes the row's
+// pairs in column order at off[i] (row-major, columns ascending).
+__device__ __forceinline__ double sq3(const double* p) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(p[0], p[0]), __dmul_rn(p[1], p[1])), __dmul_rn(p[2], p[2]));
+}
+
+__global__ __launch_bounds__(256) void k_refine_classify(int n, const double* __restrict__ pos,
+                                                         const double* __restrict__ brad, double rel, double abs_,
+                                                         int pass, int* cnt, const long* off, int* prow, int* pcol,
+                                                         int* pkind) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;  // whole wave exits together
+  const double* a = pos + 3 * (long)i;
+  const double sa = sq3(a), ri = brad[i];
+  long base = pass ? off[i] : 0;
+  int count = 0;
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int j = j0 + lane;
+    int kind = 0;
+    if (j < n) {
+      const double* b = pos + 3 * (long)j;
+      const double dot = __fma_rn(a[2], b[2], __fma_rn(a[1], b[1], __dmul_rn(a[0], b[0])));
+      double dsq = __dsub_rn(__dadd_rn(sa, sq3(b)), __dmul_rn(2.0, dot));
+      if (dsq < 1e-10) dsq = 0.0;
+      const double d = sqrt(dsq);
+      const double d2 = __dsub_rn(d, brad[j]);
+      if (d == 0.0)
+        kind = 2;
+      else if (d2 < abs_ || d2 / ri < rel)
+        kind = 1;
+    }
+    const unsigned long long m = __ballot(kind != 0);
+    if (pass && kind) {
+      const long at = base + count + __popcll(m & ((1ull << lane) - 1ull));
+      prow[at] = i;
+      pcol[at] = j;
+      pkind[at] = kind;
+    }
+    count += __popcll(m);
+  }
+  if (!pass && lane == 0) cnt[i] = count;
+}
+
 extern "C" int mnpbem_refine_coeffs(int order, int npairs, const int* prow, const int* pcol, const int* pkind,
                                     const double* pos1, const double* nvec1, const double* tvec1,
                                     const double* tvec2, const double* pos2, const double* nodes2,
@@ -420,6 +471,23 @@ extern "C" int mnpbem_green_assemble(int dtype, int dual, int n, const double* p
     mnp_set_error("green_assemble: unknown dtype");
     return 1;
   }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Refined pairs (see k_refine_classify).  pass 0: cnt[i] = pairs of row i
+// (prow/pcol/pkind unused); pass 1: pairs written at off[i] (exclusive prefix
+// sums of cnt, int64) into prow, pcol, pkind (int32).
+extern "C" int mnpbem_refine_classify(int n, const double* pos, const double* brad, double rel_cutoff,
+                                      double abs_cutoff, int pass, int* cnt, const long* off, int* prow, int* pcol,
+                                      int* pkind, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && (pass == 0 || pass == 1), "refine_classify: bad argument");
+  MNP_REQUIRE(pass == 1 || cnt != nullptr, "refine_classify: pass 0 needs cnt");
+  MNP_REQUIRE(pass == 0 || (off != nullptr && prow != nullptr && pcol != nullptr && pkind != nullptr),
+              "refine_classify: pass 1 needs off, prow, pcol, pkind");
+  hipLaunchKernelGGL(k_refine_classify, dim3(mnp_ceil_div(n, 4)), dim3(256), 0, st, n, pos, brad, rel_cutoff,
+                     abs_cutoff, pass, cnt, off, prow, pcol, pkind);
   MNP_LAUNCH_CHECK();
   return 0;
 }

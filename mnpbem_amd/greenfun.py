@@ -1,10 +1,10 @@
 """Composite retarded Green function with refined near-field quadrature.
 
 ``CompGreenRet(p, p)`` mirrors mnpbem/greenfun/compgreen_ret.py: at
-construction the refinement classification is done on the host
-(refine_utils.py:29-158), the refinement coefficients and the Fuchs-Liu
-closed-surface correction are integrated on the GPU (HIP kernels in
-csrc/green.hip), and all of it stays resident in device memory.
+construction the refinement classification (refine_utils.py:29-158), the
+refinement coefficients and the Fuchs-Liu closed-surface correction run on
+the GPU (HIP kernels in csrc/green.hip), and all of it stays resident in
+device memory.
 ``eval(i, j, key, enei)`` assembles a full matrix on the GPU.
 """
 
@@ -17,31 +17,28 @@ from . import _native as nat
 from .quadrature import kernel_table, polar_rule, rule18
 
 
-def refinement_pairs(pos, brad, rel_cutoff=3.0, abs_cutoff=0.0, chunk=4096):
-    """Face pairs needing refined integration: kind 2 where the centroid
-    distance is zero, kind 1 where (d - r_j)/r_i < RelCutoff or
-    d - r_j < AbsCutoff.  Distances use |a|^2 + |b|^2 - 2 a.b with squares
-    below 1e-10 set to zero, the form of refine_utils.py:12-26 (so the
-    classification at the cutoff boundary matches it to the last bit).
-    Returns row, col, kind (int64) sorted by (row, col)."""
+def refinement_pairs(pos, brad, rel_cutoff=3.0, abs_cutoff=0.0, device="cuda"):
+    """Face pairs needing refined integration (refine_utils.py:29-158),
+    classified on the GPU: kind 2 where the centroid distance is zero, kind 1
+    where (d - r_j)/r_i < RelCutoff or d - r_j < AbsCutoff.  Returns device
+    int32 tensors row, col, kind sorted by (row, col)."""
+    dev = torch.device(device)
     n = len(pos)
-    sq = np.sum(pos * pos, axis=1)
-    rows, cols, kinds = [], [], []
-    for c0 in range(0, n, chunk):
-        c1 = min(n, c0 + chunk)
-        dsq = sq[:, None] + sq[None, c0:c1] - 2.0 * (pos @ pos[c0:c1].T)
-        dsq[dsq < 1e-10] = 0.0
-        d = np.sqrt(dsq)
-        d2 = d - brad[None, c0:c1]
-        near = ((d2 < abs_cutoff) | (d2 / brad[:, None] < rel_cutoff)) & (d != 0)
-        r, c = np.nonzero(near | (d == 0))
-        k = np.where(d[r, c] == 0, 2, 1)
-        rows.append(r)
-        cols.append(c + c0)
-        kinds.append(k)
-    row, col, kind = (np.concatenate(a) for a in (rows, cols, kinds))
-    o = np.lexsort((col, row))
-    return row[o], col[o], kind[o]
+    tp = _dev(pos, torch.float64, dev)
+    tb = _dev(brad, torch.float64, dev)
+    st = nat.stream_handle(dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=dev)
+    nat.call("mnpbem_refine_classify", n, nat.ptr(tp), nat.ptr(tb), float(rel_cutoff), float(abs_cutoff), 0,
+             nat.ptr(cnt), None, None, None, None, st)
+    c = cnt.cpu().numpy().astype(np.int64)
+    off = torch.as_tensor(np.concatenate([[0], np.cumsum(c)[:-1]])).to(dev)
+    total = int(c.sum())
+    row = torch.empty((total,), dtype=torch.int32, device=dev)
+    col = torch.empty_like(row)
+    kind = torch.empty_like(row)
+    nat.call("mnpbem_refine_classify", n, nat.ptr(tp), nat.ptr(tb), float(rel_cutoff), float(abs_cutoff), 1,
+             None, nat.ptr(off), nat.ptr(row), nat.ptr(col), nat.ptr(kind), st)
+    return row, col, kind
 
 
 def _dev(a, dtype, device):
@@ -67,7 +64,8 @@ class CompGreenRet:
         dev = self.device
         n = p.n
         self.n = n
-        row, col, kind = refinement_pairs(p.pos, p.brad, RelCutoff, AbsCutoff)
+        self.prow, self.pcol, self._pkind = refinement_pairs(p.pos, p.brad, RelCutoff, AbsCutoff, dev)
+        row, col, kind = (t.cpu().numpy().astype(np.int64) for t in (self.prow, self.pcol, self._pkind))
         self.npairs = len(row)
         self.row, self.col, self.kind = row, col, kind
         f64, i32 = torch.float64, torch.int32
@@ -77,9 +75,6 @@ class CompGreenRet:
         self._tv1 = _dev(p.tvec1, f64, dev)
         self._tv2 = _dev(p.tvec2, f64, dev)
         self._nodes = _dev(p.nodes, f64, dev)
-        self.prow = _dev(row, i32, dev)
-        self.pcol = _dev(col, i32, dev)
-        self._pkind = _dev(kind, i32, dev)
         # per side: 0-based material index of each face
         self.mat = [_dev(p.inout_faces[:, s] - 1, i32, dev) for s in (0, 1)]
         self.nmat = len(p.eps)

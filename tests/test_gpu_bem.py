@@ -171,3 +171,30 @@ def test_bemret_golden_spiral_meshes_match_oracle(cuda, kind):
         ext = m.BEMRet(cp, precision=precision).solve_sweep(lams, m.PlaneWaveRet(pol, dr))
         assert np.max(np.abs(ext / ext_ref - 1)) < tol, (precision, np.max(np.abs(ext / ext_ref - 1)))
 
+
+@pytest.mark.parametrize("case", ["sphere2884", "ellipsoid", "dimer"])
+def test_refinement_classification_matches_oracle(cuda, case):
+    """GPU refinement classification vs the oracle's dense refinematrix:
+    identical pair lists and kinds (row-major, columns ascending)."""
+    from mnpbem_amd.greenfun import refinement_pairs
+    import mnpbem_amd as m
+    print("refine classify", case, flush=True)
+    if case == "sphere2884":
+        p = m.trisphere(1444, 40.0)
+        pos, rad = p.pos, p.bradius()
+    elif case == "ellipsoid":
+        p = m.triellipsoid(900, [10.0, 25.0, 40.0])
+        pos, rad = p.pos, p.bradius()
+    else:
+        p = m.trisphere_fibonacci(800, 60.0)
+        pos = np.vstack([p.pos, p.pos + np.array([61.0, 0.0, 0.0])])
+        rad = np.concatenate([p.bradius(), p.bradius()])
+    row, col, kind = (t.cpu().numpy() for t in refinement_pairs(pos, rad, 3.0, 0.0, cuda))
+    ir = o.refinematrix(pos, pos, rad, rad, 3.0, 0.0)
+    r, c = np.nonzero(ir)
+    assert np.array_equal(row, r) and np.array_equal(col, c)
+    assert np.array_equal(kind, ir[r, c])
+    if case == "dimer":  # the 1 nm gap puts refined pairs between the spheres
+        n1 = len(p.pos)
+        assert np.any((row < n1) & (col >= n1))
+
