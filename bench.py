@@ -92,7 +92,16 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "mixed-fp32"],
                     help="mixed: complex64 GEMMs on bf16 matrix cores (3 split products); "
                          "mixed-fp32: on FP32 matrix cores")
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "uneven", "dimer"],
+                    help="sphere: the headline 64-wavelength sweep of the 2884-face gold sphere (default); "
+                         "uneven: 32 seeded spheres/ellipsoids of 284-2884 faces, 64 wavelengths each; "
+                         "dimer: two 12500-face spheres, 2 nm gap, H-matrix GMRES at 500/600/700 nm")
+    ap.add_argument("--seed", type=int, default=0, help="seed of the uneven-particle set")
     args = ap.parse_args()
+    if args.gpus == 1 and args.workload == "uneven":
+        return bench_uneven(args)
+    if args.gpus == 1 and args.workload == "dimer":
+        return bench_dimer(args)
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the sweep shards by wavelength across GPUs "
               "(see DESIGN.md) but no multi-GPU figure is produced", file=sys.stderr)
@@ -226,6 +235,118 @@ def main():
                      "avg_launch_ms": round(ms.value / max(nl.value, 1), 4)},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(line), flush=True)
+
+
+def bench_uneven(args):
+    """32 particles of uneven size, each swept over 64 wavelengths in one
+    batched factor/solve chunk: seeded kinds (sphere / flat ellipsoid), sizes
+    (144, 256, 400, 900, 1444 vertices -> 284 ... 2884 faces), sphere
+    diameters U(20, 80) nm, ellipsoid semi-axes U(10, 40) nm, Drude gold with
+    gamma U(0.05, 0.1) eV; mixed precision with FP64 refinement."""
+    import torch
+    import mnpbem_amd as m
+    rng = np.random.default_rng(args.seed)
+    sizes = [144, 256, 400, 900, 1444]
+    lams = np.linspace(400.0, 800.0, 64)
+    pw = m.PlaneWaveRet([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]], [[0.0, 0.0, 1.0], [0.0, 0.0, 1.0]])
+    specs = []
+    for i in range(32):
+        kind = "sphere" if rng.random() < 0.5 else "ellipsoid"
+        nv = int(rng.choice(sizes))
+        geo = float(rng.uniform(20.0, 80.0)) if kind == "sphere" else rng.uniform(10.0, 40.0, 3)
+        gam = float(rng.uniform(0.05, 0.1))
+        specs.append((kind, nv, geo, gam))
+
+    def run_one(spec):
+        kind, nv, geo, gam = spec
+        part = m.trisphere(nv, geo) if kind == "sphere" else m.triellipsoid(nv, geo)
+        cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(10.0, 9.065, gam)], [part], [[2, 1]], 1)
+        bem = m.BEMRet(cp, precision=args.precision)
+        ext = bem.solve_sweep(lams, pw, chunk=64)
+        return cp.n, ext, bem
+
+    run_one(specs[0])  # warm-up: library, caches
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    faces = []
+    exts = []
+    for k, spec in enumerate(specs):
+        n, ext, _ = run_one(spec)
+        faces.append(n)
+        exts.append(ext)
+        log("particle %d: %s %d faces done" % (k, spec[0], n))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # accuracy spot check: the largest particle again in complex128
+    big = int(np.argmax(faces))
+    kind, nv, geo, gam = specs[big]
+    part = m.trisphere(nv, geo) if kind == "sphere" else m.triellipsoid(nv, geo)
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(10.0, 9.065, gam)], [part], [[2, 1]], 1)
+    e64 = m.BEMRet(cp, precision="double").solve_sweep(lams, pw, chunk=64)
+    err = float(np.max(np.abs(exts[big] / e64 - 1)))
+    units = len(specs) * len(lams)
+    line = {"metric": "particle-wavelengths solved/s, 32 uneven Drude-gold particles (config 3)",
+            "value": round(units / dt, 3), "unit": "particle-wavelengths/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": round(1000.0 * dt, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "c64(bf16x3 GEMM)+f64-refine" if args.precision == "mixed" else "c64+f64-refine",
+            "data": "synthetic",
+            "config": {"workload": "32 seeded spheres/ellipsoids, 64 wavelengths each, 2 plane waves, seed %d"
+                                   % args.seed, "faces": faces, "total_faces": int(sum(faces)),
+                       "setup_included": True},
+            "max_rel_ext_err_vs_fp64_largest": err}
+    print(json.dumps(line), flush=True)
+
+
+def bench_dimer(args):
+    """Gold sphere dimer, two 12500-face golden-spiral spheres of 100 nm with a
+    2 nm gap (25000 faces): BEMRetIter on ACA H-matrices (leaf 64, htol 1e-6,
+    rank <= 30, blocks in complex64), GMRES restart 50, tol 1e-6, FP64
+    Arnoldi, preconditioned by the mixed-precision direct solve; 500, 600,
+    700 nm; x polarisation along the dimer axis, propagation along z."""
+    import torch
+    import mnpbem_amd as m
+    t0 = time.perf_counter()
+    p1 = m.trisphere_fibonacci(6252, 100.0)
+    p2 = p1.shift([102.0, 0.0, 0.0])
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [p1, p2], [[2, 1], [2, 1]], 1, 2)
+    lams = np.array([500.0, 600.0, 700.0])
+    pw = m.PlaneWaveRet([[1.0, 0.0, 0.0]], [[0.0, 0.0, 1.0]])
+    bem = m.BEMRetIter(cp, hmatrix=True, htol=1e-6, kmax=30, cleaf=64, tol=1e-6, restart=50, maxit=20,
+                       precision="mixed", hmat_dtype="complex64")
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    log("dimer set-up %.1f s, N = %d" % (t_setup, cp.n))
+    exts = []
+    t1 = time.perf_counter()
+    for lam in lams:
+        exts.append(bem.solve_sweep([lam], pw, chunk=1)[0, 0])
+        flag, rel, it = bem.info()
+        st = [h.stat() for h in bem.last_hmats[0]]
+        log("lambda %.0f: ext %.6e, GMRES flag %d relres %.2e it %s, compression %s, rank max %s" %
+            (lam, exts[-1], flag[-1], rel[-1], list(it[-1]), [round(x["compression"], 3) for x in st],
+             [x["rank_max"] for x in st]))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t1
+    hstat = [h.stat() for h in bem.last_hmats[0]]
+    del bem
+    torch.cuda.empty_cache()
+    # reference: the dense mixed-precision direct solve of the same system
+    t2 = time.perf_counter()
+    ed = m.BEMRet(cp, precision="mixed").solve_sweep(lams, pw, chunk=1)[:, 0]
+    torch.cuda.synchronize()
+    dt_direct = time.perf_counter() - t2
+    err = float(np.max(np.abs(np.array(exts) / ed - 1)))
+    line = {"metric": "wavelengths solved/s, 25000-face gold dimer, H-matrix GMRES (config 5)",
+            "value": round(len(lams) / dt, 4), "unit": "wavelengths/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": round(1000.0 * dt, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64 GMRES, c64 H-matrix blocks, c64 preconditioner", "data": "synthetic",
+            "config": {"workload": "2 x trisphere_fibonacci(6252, 100 nm), 2 nm gap, 3 wavelengths",
+                       "faces": cp.n, "cleaf": 64, "htol": 1e-6, "kmax": 30, "restart": 50, "tol": 1e-6},
+            "setup_s": round(t_setup, 2), "direct_mixed_value": round(len(lams) / dt_direct, 4),
+            "max_rel_ext_err_vs_direct": err,
+            "hmatrix": [{"compression": round(x["compression"], 4), "rank_max": x["rank_max"],
+                         "bytes": x["bytes"]} for x in hstat]}
     print(json.dumps(line), flush=True)
 
 

@@ -109,6 +109,10 @@ class Particle:
         corners = X[:, :3, :]
         self._brad = np.sqrt(((corners - self.pos[:, None, :]) ** 2).sum(axis=2)).max(axis=1)
 
+    def shift(self, vec):
+        """Copy of the particle translated by ``vec`` (particle.py shift)."""
+        return Particle(self.verts2 + np.asarray(vec, dtype=float).reshape(1, 3), self.faces6, interp=self.interp)
+
     def bradius(self):
         """Radius of the smallest centroid-centred sphere through the corners
         (particle.py:2848-2880)."""
