@@ -173,6 +173,20 @@ int mnpbem_extinction_batched(int n, int P, const double* X, long ldv, long sv, 
                               const double* area, const int* in_med, const int* out_med, const double* pol,
                               const double* dir, const double* pw, double* ext, int batch, void* stream);
 
+/* Surface fields (replaces CompGreenRet.field, compgreen_ret.py:912-969):
+ * e_i = i k0 sum_j G_ij h_j - sum_j Hp_ij sig_j, h_i = sum_j Hp_ij x h_j on
+ * side `side` (0 inside with H1p = Gp + 2 pi n, hsign = +1; 1 outside with
+ * H2p = Gp - 2 pi n, hsign = -1), Gp = grad G (deriv 'cart', refined pairs
+ * from the coefficients g, f of mnpbem_refine_coeffs).  rowptr (int64, n + 1):
+ * offsets of each row's pairs in the (row, col)-sorted list; k0: vacuum
+ * wavenumber per wavelength; X: batch x n x 8P densities; E, H: batch x n x 3P
+ * (index c P + p) complex128.  P <= 4. */
+int mnpbem_surface_field(int n, int P, int side, double hsign, const double* pos, const double* nvec,
+                         const double* area, const int* mat, const double* kmat, int nmat, const double* k0,
+                         int order, const long* rowptr, const int* pcol, const double* g, const double* f,
+                         const void* X, long ldx, long sx, void* E, void* H, long lde, long se, int batch,
+                         void* stream);
+
 /* Combination step of the 8N operator from precomputed products
  * Y = [G1 X1 | H1 X1 | G2 X2 | H2 X2] (n x 16P per wavelength, complex128):
  * Out = R - A X, or Out = A X when R is NULL; nrm as for

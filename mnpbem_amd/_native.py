@@ -55,6 +55,8 @@ SIGNATURES = {
     "mnpbem_hmat_full": [_i, _i, _p, _p, _p, _p, _p, _p, _l, _p],
     "mnpbem_scattering_batched": [_i, _i, _p, _l, _l, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p],
     "mnpbem_refine_classify": [_i, _p, _p, _d, _d, _i, _p, _p, _p, _p, _p, _p],
+    "mnpbem_surface_field": [_i, _i, _i, _d, _p, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _p, _p, _l, _l, _p, _p,
+                             _l, _l, _i, _p],
     "mnpbem_prof_begin": [_i],
     "mnpbem_prof_end": [_p, _p, _p],
     "mnpbem_prof_bytes": [_p],

@@ -242,6 +242,15 @@ class BEMRet:
     def __truediv__(self, exc):
         return self.solve(exc)[0]
 
+    def potential(self, sig, inout=2):
+        """Potentials at the faces from a solution (CompGreenRet.potential)."""
+        return self.g.potential(sig, inout)
+
+    def field(self, sig, inout=2):
+        """Fields at the faces from a solution, by the Green-function route
+        of CompGreenRet.field (compgreen_ret.py:912-969)."""
+        return self.g.field(sig, inout)
+
     def solve_sweep(self, enei_list, planewave, chunk=16, with_x=False, with_sca=False):
         """Extinction spectrum (n_lambda, npol) of a plane-wave sweep, factored
         and solved ``chunk`` wavelengths per batched launch; with ``with_sca``

@@ -373,6 +373,15 @@ class BEMRetIter(BEMIter):
     def __truediv__(self, exc):
         return self.solve(exc)[0]
 
+    def potential(self, sig, inout=2):
+        """Potentials at the faces from a solution (bem_ret_iter.py potential)."""
+        return self.g.potential(sig, inout)
+
+    def field(self, sig, inout=2):
+        """Fields at the faces by the Green-function route of
+        CompGreenRet.field (compgreen_ret.py:912-969)."""
+        return self.g.field(sig, inout)
+
     def solve_sweep(self, enei_list, planewave, chunk=8):
         """Extinction (n_lambda, npol) of a plane-wave sweep."""
         enei_list = np.atleast_1d(np.asarray(enei_list, dtype=float))

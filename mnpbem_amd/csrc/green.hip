@@ -491,3 +491,146 @@ extern "C" int mnpbem_refine_classify(int n, const double* pos, const double* br
   MNP_LAUNCH_CHECK();
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Fields at the faces from the surface charges and currents of one side
+// (compgreen_ret.py:912-969): with Gp = grad G (deriv='cart'),
+//   e_i = i k0 sum_j G_ij h_j - sum_j Hp_ij sig_j,   h_i = sum_j Hp_ij x h_j,
+// Hp = Gp + hsign 2 pi n_i on the diagonal (H1p inside, H2p outside).
+// G_ij = a_j e^{ikd}/d, Gp_ij = (ik - 1/d)/d^2 (r_i - r_j) a_j e^{ikd}, and
+// for refined pairs sum_n (ik)^n {g_n, f_n} e^{ikd}; only faces whose side
+// material matches the field face's contribute, with that material's k.
+// One workgroup per (field face, wavelength); the refined pairs of row i are
+// found by binary search in its (column-sorted) slice of the pair list.
+// X: (batch, n, 8P) densities [sig1 | h1 | sig2 | h2]; E, H: (batch, n, 3P).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_surface_field(
+    int n, int P, int side, double hsign, const double* __restrict__ pos, const double* __restrict__ nvec,
+    const double* __restrict__ area, const int* __restrict__ mat, const double* __restrict__ kmat, int nmat,
+    const double* __restrict__ k0v, const long* __restrict__ rowptr, const int* __restrict__ pcol,
+    const double* __restrict__ g, const double* __restrict__ f, int ord1, const double* __restrict__ X, long ldx,
+    long sx, double* __restrict__ E, double* __restrict__ Hf, long lde, long se) {
+  const int i = blockIdx.x;
+  const long b = blockIdx.y;
+  const int mi = mat[i];
+  const double kr = kmat[2 * (b * nmat + mi)], ki = kmat[2 * (b * nmat + mi) + 1];
+  const double k0 = k0v[b];
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  const long p0 = rowptr[i], p1 = rowptr[i + 1];
+  const int off = side * 4 * P;
+  const double* xb = X + 2 * b * sx;
+  cplx<double> ea[4][3], ha[4][3];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ea[p][c] = ha[p][c] = cmake(0.0, 0.0);
+  for (int j = threadIdx.x; j < n; j += 256) {
+    if (mat[j] != mi) continue;
+    const double x = xi - pos[3 * j], y = yi - pos[3 * j + 1], z = zi - pos[3 * j + 2];
+    const double d = fmax(sqrt(x * x + y * y + z * z), 2.220446049250313e-16);
+    double s, c;
+    sincos(kr * d, &s, &c);
+    const double ex = exp(-ki * d);
+    const cplx<double> ph = cmake(ex * c, ex * s);
+    // refined pair?
+    long lo = p0, hi = p1;
+    while (lo < hi) {
+      const long mid = (lo + hi) >> 1;
+      if (pcol[mid] < j) lo = mid + 1; else hi = mid;
+    }
+    cplx<double> G, Gp[3];
+    if (lo < p1 && pcol[lo] == j) {
+      double pr = 1.0, pim = 0.0;  // (ik)^q, ik = -ki + i kr
+      cplx<double> sg = cmake(0.0, 0.0), sf[3] = {cmake(0.0, 0.0), cmake(0.0, 0.0), cmake(0.0, 0.0)};
+      for (int q = 0; q < ord1; ++q) {
+        const double gq = g[lo * ord1 + q];
+        sg = cadd(sg, cmake(gq * pr, gq * pim));
+        for (int cc = 0; cc < 3; ++cc) {
+          const double fq = f[(lo * 3 + cc) * ord1 + q];
+          sf[cc] = cadd(sf[cc], cmake(fq * pr, fq * pim));
+        }
+        const double tr = pr * (-ki) - pim * kr;
+        pim = pr * kr + pim * (-ki);
+        pr = tr;
+      }
+      G = cmul(sg, ph);
+      for (int cc = 0; cc < 3; ++cc) Gp[cc] = cmul(sf[cc], ph);
+    } else {
+      const double id = 1.0 / d, aj = area[j];
+      G = cscale(ph, aj * id);
+      // (ik - 1/d)/d^2 a_j e^{ikd}
+      const cplx<double> fa = cmul(cmake((-ki - id) * id * id * aj, kr * id * id * aj), ph);
+      Gp[0] = cscale(fa, x);
+      Gp[1] = cscale(fa, y);
+      Gp[2] = cscale(fa, z);
+    }
+    if (j == i) {
+      for (int cc = 0; cc < 3; ++cc) Gp[cc].re += hsign * 2.0 * M_PI * nvec[3 * i + cc];
+    }
+    const double* xj = xb + 2 * ((long)j * ldx + off);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      if (p < P) {
+        const cplx<double> sg = cld(xj + 2 * p);
+        const cplx<double> h0 = cld(xj + 2 * (P + p)), h1 = cld(xj + 2 * (2 * P + p)), h2 = cld(xj + 2 * (3 * P + p));
+        const cplx<double> hv[3] = {h0, h1, h2};
+        for (int cc = 0; cc < 3; ++cc) {
+          // i k0 G h_c - Gp_c sig
+          const cplx<double> gh = cmul(G, hv[cc]);
+          ea[p][cc] = cadd(ea[p][cc], csub(cmake(-k0 * gh.im, k0 * gh.re), cmul(Gp[cc], sg)));
+        }
+        ha[p][0] = cadd(ha[p][0], csub(cmul(Gp[1], h2), cmul(Gp[2], h1)));
+        ha[p][1] = cadd(ha[p][1], csub(cmul(Gp[2], h0), cmul(Gp[0], h2)));
+        ha[p][2] = cadd(ha[p][2], csub(cmul(Gp[0], h1), cmul(Gp[1], h0)));
+      }
+    }
+  }
+  __shared__ double red[4][4][6][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      cplx<double> v = q < 3 ? ea[p][q] : ha[p][q - 3];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        v.re += __shfl_xor(v.re, o);
+        v.im += __shfl_xor(v.im, o);
+      }
+      if (lane == 0) {
+        red[wv][p][q][0] = v.re;
+        red[wv][p][q][1] = v.im;
+      }
+    }
+  __syncthreads();
+  if (threadIdx.x < 6 * P) {
+    const int p = threadIdx.x % P, q = threadIdx.x / P;
+    const double vr = red[0][p][q][0] + red[1][p][q][0] + red[2][p][q][0] + red[3][p][q][0];
+    const double vi = red[0][p][q][1] + red[1][p][q][1] + red[2][p][q][1] + red[3][p][q][1];
+    double* out = (q < 3 ? E : Hf) + 2 * (b * se + (long)i * lde + (q % 3) * P + p);
+    out[0] = vr;
+    out[1] = vi;
+  }
+}
+
+// Surface fields e, h on side `side` (0 inside, 1 outside) of every face from
+// the densities of that side (compgreen_ret.py:912-969; hsign = +1 for H1p,
+// -1 for H2p).  rowptr: int64 (n + 1) offsets of each row's refined pairs in
+// the (row, col)-sorted pair list (pcol, g, f as for mnpbem_green_assemble);
+// k0: vacuum wavenumber per wavelength.  P <= 4.
+extern "C" int mnpbem_surface_field(int n, int P, int side, double hsign, const double* pos, const double* nvec,
+                                    const double* area, const int* mat, const double* kmat, int nmat,
+                                    const double* k0, int order, const long* rowptr, const int* pcol,
+                                    const double* g, const double* f, const void* X, long ldx, long sx, void* E,
+                                    void* H, long lde, long se, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P >= 1 && P <= 4 && (side == 0 || side == 1) && ldx >= 8 * P && lde >= 3 * P,
+              "surface_field: bad size");
+  if (batch <= 0) return 0;
+  hipLaunchKernelGGL(k_surface_field, dim3(n, batch), dim3(256), 0, st, n, P, side, hsign, pos, nvec, area, mat,
+                     kmat, nmat, k0, rowptr, pcol, g, f, order + 1, (const double*)X, ldx, sx, (double*)E,
+                     (double*)H, lde, se);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+

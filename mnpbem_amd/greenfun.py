@@ -153,6 +153,76 @@ class CompGreenRet:
         return M[0].transpose(0, 1)
 
 
+    # ------------------------------------------------------------------
+    def _rowptr(self):
+        if getattr(self, "_rowptr_t", None) is None:
+            rp = np.concatenate([[0], np.cumsum(np.bincount(self.row, minlength=self.n))]).astype(np.int64)
+            self._rowptr_t = _dev(rp, torch.int64, self.device)
+        return self._rowptr_t
+
+    @staticmethod
+    def _sig_x(sig):
+        X = sig.get("_X")
+        if X is None:
+            raise ValueError("CompGreenRet: CompStruct does not come from BEMRet.solve / BEMRetIter.solve")
+        return X
+
+    def potential(self, sig, inout=1):
+        """Potentials and surface derivatives at the faces, inside (inout=1)
+        or outside (inout=2) (compgreen_ret.py:971-1038): phi = G sig,
+        phip = H sig, a = G h, ap = H h with the Green functions of that side."""
+        if inout not in (1, 2):
+            raise ValueError("inout must be 1 or 2")
+        X = self._sig_x(sig)
+        B, n, C = X.shape
+        P = C // 8
+        s = inout - 1
+        kmat = self.wavenumbers([sig.enei])
+        G = torch.empty((1, n, n), dtype=torch.complex128, device=self.device)
+        H = torch.empty_like(G)
+        self.assemble(s, kmat, G, H, 1.0 if s == 0 else -1.0, nat.DTYPE_C128)
+        st = nat.stream_handle(self.device)
+        Y = torch.empty((2, n, 4 * P), dtype=torch.complex128, device=self.device)
+        for q, M in enumerate((G, H)):
+            nat.call("mnpbem_gemv_t_batched", nat.DTYPE_C128, n, 4 * P, nat.ptr(M), n, n * n,
+                     nat.ptr_off(X, s * 4 * P), C, n * C, nat.ptr(Y[q]), 4 * P, n * 4 * P, 1.0, 0.0, 0.0, 1, st)
+        phi, a = Y[0, :, :P], Y[0, :, P:].reshape(n, 3, P)
+        phip, ap = Y[1, :, :P], Y[1, :, P:].reshape(n, 3, P)
+        if P == 1:
+            phi, phip, a, ap = phi[:, 0], phip[:, 0], a[:, :, 0], ap[:, :, 0]
+        from .bem import CompStruct
+        t = str(inout)
+        return CompStruct(self.p1, sig.enei, **{"phi" + t: phi, "phi" + t + "p": phip, "a" + t: a,
+                                                "a" + t + "p": ap})
+
+    def field(self, sig, inout=1):
+        """Electric and magnetic fields at the faces, inside (inout=1) or
+        outside (inout=2) (compgreen_ret.py:912-969), computed on the GPU
+        without forming the Cartesian-derivative matrices."""
+        if inout not in (1, 2):
+            raise ValueError("inout must be 1 or 2")
+        X = self._sig_x(sig)
+        B, n, C = X.shape
+        P = C // 8
+        if P > 4:
+            raise ValueError("CompGreenRet.field: at most 4 polarisations per call")
+        s = inout - 1
+        kmat = self.wavenumbers([sig.enei])
+        k0 = torch.tensor([2.0 * np.pi / sig.enei], dtype=torch.float64, device=self.device)
+        E = torch.empty((1, n, 3 * P), dtype=torch.complex128, device=self.device)
+        Hf = torch.empty_like(E)
+        nat.call("mnpbem_surface_field", n, P, s, 1.0 if s == 0 else -1.0, nat.ptr(self.pos), nat.ptr(self.nvec),
+                 nat.ptr(self.area), nat.ptr(self.mat[s]), nat.ptr(kmat), self.nmat, nat.ptr(k0), self.order,
+                 nat.ptr(self._rowptr()), nat.ptr(self.pcol), nat.ptr(self.g), nat.ptr(self.f), nat.ptr(X), C,
+                 n * C, nat.ptr(E), nat.ptr(Hf), 3 * P, n * 3 * P, 1, nat.stream_handle(self.device))
+        e = E[0].reshape(n, 3, P)
+        h = Hf[0].reshape(n, 3, P)
+        if P == 1:
+            e, h = e[:, :, 0], h[:, :, 0]
+        from .bem import CompStruct
+        return CompStruct(self.p1, sig.enei, e=e, h=h)
+
+
 _GREEN_CACHE = {}
 
 

@@ -36,6 +36,7 @@ from scipy.interpolate import CubicSpline
 __all__ = [
     "EV2NM", "EpsConst", "EpsDrude", "EpsTable",
     "lglnodes", "rule18", "polar_rule_tri", "tri6_shape", "tri6_deriv",
+    "green_hp", "surface_potential", "surface_field",
     "read_tri_bin", "trisphere_mesh", "golden_spiral", "spherical_delaunay", "fibonacci_sphere_mesh",
     "triellipsoid_mesh", "CurvedSurface", "Composite",
     "refinematrix", "refine_coefficients", "closed_surface_correction",
@@ -559,6 +560,70 @@ def green_matrices(setup, enei):
                 H[np.ix_(ri, rj)] = Fb
         out += [G, H]
     return out  # G1, H1, G2, H2
+
+
+def _block_gp(s1, s2, blk, k, order):
+    """Cartesian derivative Gp[i, :, j] = (ik - 1/d)/d^2 (r_i - r_j) a_j e^{ikd},
+    refined pairs sum_n (ik)^n f_n e^{ikd} (greenret_refined.py:812-844)."""
+    rv = s1.pos[:, None, :] - s2.pos[None, :, :]
+    d = np.maximum(np.sqrt(np.sum(rv * rv, axis=2)), np.finfo(float).eps)
+    faux = (1j * k - 1.0 / d) / d ** 2 * s2.area[None, :]
+    Gp = faux[:, :, None] * rv
+    row, col, g, f = blk
+    if len(row):
+        ikn = (1j * k) ** np.arange(order + 1)
+        Gp[row, col, :] = np.einsum("pdn,n->pd", f, ikn)
+    return np.transpose(Gp * np.exp(1j * k * d)[:, :, None], (0, 2, 1))
+
+
+def green_hp(setup, enei, side):
+    """H1p (side 0, + 2 pi n on the self diagonal) or H2p (side 1, - 2 pi n)
+    of the composite, blocks present where the two surfaces share the side's
+    material (greenret_refined.py:846-864, compgreen_ret.py:912-969)."""
+    comp = setup.comp
+    ks = [e(enei)[1] for e in comp.eps]
+    N = comp.n
+    sgn = 1.0 if side == 0 else -1.0
+    Hp = np.zeros((N, 3, N), complex)
+    for i, si in enumerate(comp.parts):
+        for j, sj in enumerate(comp.parts):
+            mi, mj = comp.inout[i, side], comp.inout[j, side]
+            if mi != mj:
+                continue
+            blk = _block_gp(si, sj, setup.blocks[(i, j)], ks[mi - 1], setup.order)
+            if i == j:
+                idx = np.arange(si.n)
+                blk[idx, :, idx] += sgn * 2.0 * np.pi * si.nvec
+            ri, rj = comp.face_range(i), comp.face_range(j)
+            Hp[np.ix_(ri, np.arange(3), rj)] = blk
+    return Hp
+
+
+def surface_potential(setup, sig, enei, inout):
+    """compgreen_ret.py:971-1038 for non-nested particles: phi = G sig,
+    phip = H sig, a = G h, ap = H h with the side's G and H."""
+    G1, H1, G2, H2 = green_matrices(setup, enei)
+    G, H = (G1, H1) if inout == 1 else (G2, H2)
+    sg, h = (sig["sig1"], sig["h1"]) if inout == 1 else (sig["sig2"], sig["h2"])
+    N = len(sg)
+    mv = lambda M, x: (M @ x.reshape(N, -1)).reshape(x.shape)
+    return {"phi": mv(G, sg), "phip": mv(H, sg), "a": mv(G, h), "ap": mv(H, h)}
+
+
+def surface_field(setup, sig, enei, inout):
+    """compgreen_ret.py:912-969: e = i k0 G h - Hp sig, h = Hp x h (per
+    component: (Hp x h)_x = Hp_y h_z - Hp_z h_y, ...)."""
+    G1, _, G2, _ = green_matrices(setup, enei)
+    side = inout - 1
+    G = G1 if inout == 1 else G2
+    Hp = green_hp(setup, enei, side)
+    sg, h = (sig["sig1"], sig["h1"]) if inout == 1 else (sig["sig2"], sig["h2"])
+    k0 = 2.0 * np.pi / enei
+    e = 1j * k0 * np.einsum("ij,jcp->icp", G, h) - np.einsum("icj,jp->icp", Hp, sg)
+    Hx, Hy, Hz = Hp[:, 0, :], Hp[:, 1, :], Hp[:, 2, :]
+    hx, hy, hz = h[:, 0, :], h[:, 1, :], h[:, 2, :]
+    hf = np.stack([Hy @ hz - Hz @ hy, Hz @ hx - Hx @ hz, Hx @ hy - Hy @ hx], axis=1)
+    return {"e": e, "h": hf}
 
 
 # --------------------------------------------------------------------------

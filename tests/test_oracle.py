@@ -175,3 +175,28 @@ def test_direct_solve_satisfies_bem_operator(sphere284):
         assert np.abs(a - a_ref).max() < 1e-11 * np.abs(a_ref).max()
         assert np.abs(alpha - al_ref).max() < 1e-10 * scale
         assert np.abs(De - De_ref).max() < 1e-10 * np.abs(De_ref).max()
+
+
+def test_cartesian_green_derivative(sphere284):
+    """Gp of the oracle: (a) the closed-form gradient of G against central
+    differences of G in the field point at a far pair, (b) n . Gp equals F
+    for every pair, refined ones included (F is pinned by brute force)."""
+    comp = o.Composite([o.EpsConst(1.0), o.EpsDrude(10.0, 9.065, 0.0708)], [sphere284], [[2, 1]])
+    setup = o.GreenSetup(comp)
+    lam = 520.0
+    Hp = o.green_hp(setup, lam, 0)
+    G1, H1, _, _ = o.green_matrices(setup, lam)
+    k = comp.eps[1](lam)[1]
+    n = comp.n
+    F = H1 - 2 * np.pi * np.eye(n)
+    nGp = np.einsum("ic,icj->ij", sphere284.nvec, Hp) - 2 * np.pi * np.eye(n)
+    assert np.abs(nGp - F).max() < 1e-13 * np.abs(F).max()
+    i, j = 3, 200
+    ir = o.refinematrix(sphere284.pos, sphere284.pos, sphere284.bradius, sphere284.bradius)
+    assert ir[i, j] == 0
+    gfun = lambda r: sphere284.area[j] * np.exp(1j * k * np.linalg.norm(r - sphere284.pos[j])) / np.linalg.norm(
+        r - sphere284.pos[j])
+    hstep = 1e-5
+    num = [(gfun(sphere284.pos[i] + hstep * e) - gfun(sphere284.pos[i] - hstep * e)) / (2 * hstep)
+           for e in np.eye(3)]
+    assert np.allclose(num, Hp[i, :, j], rtol=1e-7, atol=0)
