@@ -187,6 +187,20 @@ int mnpbem_surface_field(int n, int P, int side, double hsign, const double* pos
                          const void* X, long ldx, long sx, void* E, void* H, long lde, long se, int batch,
                          void* stream);
 
+/* ---- permittivity derivatives (the gradient path; no counterpart in the
+ *      reference, which has no derivatives) ----
+ * d/dk of the Green matrices of one side (dG = i a e^{ikd}, dF = -k (n.r) a
+ * e^{ikd} / d, refined pairs (S'(k) + i d S(k)) e^{ikd}); arguments as
+ * mnpbem_green_assemble, complex128 transposed outputs. */
+int mnpbem_green_dk(int n, const double* pos, const double* nvec, const double* area, const int* mat,
+                    const double* kmat, int nmat, int order, int npairs, const int* prow, const int* pcol,
+                    const double* g, const double* f, void* dG, void* dH, long ld, long stride, int batch,
+                    void* stream);
+/* R = -(dA/deps1) x from Y = [G1 X1 | H1 X1 | G1' X1 | H1' X1] (n x 16P,
+ * ' = d/dk1, X1 = [sig1 | h1]) and kap = dk1/deps1 per wavelength (complex). */
+int mnpbem_deps_rhs(int n, int P, const void* Y, long ldy, long sy, void* R, long ldv, long sv, const double* nvec,
+                    const double* par, const double* kap, int batch, void* stream);
+
 /* Combination step of the 8N operator from precomputed products
  * Y = [G1 X1 | H1 X1 | G2 X2 | H2 X2] (n x 16P per wavelength, complex128):
  * Out = R - A X, or Out = A X when R is NULL; nrm as for

@@ -13,7 +13,8 @@ from .geometry import ComParticle, Particle, triellipsoid, trisphere, trisphere_
 __all__ = ["EV2NM", "EpsConst", "EpsDrude", "EpsTable", "ComParticle", "Particle", "trisphere",
            "trisphere_fibonacci", "triellipsoid",
            "CompGreenRet", "BEMRet", "CompStruct", "PlaneWaveRet", "BEMIter", "BEMRetIter", "ClusterTree",
-           "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss"]
+           "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss", "DrudeExtinction",
+           "extinction_with_drude_gradient"]
 
 
 def __getattr__(name):
@@ -31,6 +32,9 @@ def __getattr__(name):
     if name in ("ClusterTree", "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss"):
         from . import hmatrix
         return getattr(hmatrix, name)
+    if name in ("DrudeExtinction", "extinction_with_drude_gradient"):
+        from . import adjoint
+        return getattr(adjoint, name)
     if name == "PlaneWaveRet":
         from .planewave import PlaneWaveRet
         return PlaneWaveRet

@@ -634,3 +634,130 @@ extern "C" int mnpbem_surface_field(int n, int P, int side, double hsign, const 
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// d/dk of the Green matrices of one side (for derivatives with respect to
+// the material's permittivity, k = k0 sqrt(eps)): from G = a_j e^{ikd}/d and
+// F = (n.r) a_j (ik - 1/d)/d^2 e^{ikd},
+//   dG/dk = i a_j e^{ikd},   dF/dk = -k (n.r) a_j e^{ikd} / d,
+// and for refined pairs (G, F) = S(k) e^{ikd} with S = sum_q c_q (ik)^q,
+//   d/dk = (S'(k) + i d S(k)) e^{ikd},  S' = sum_q q i c_q (ik)^(q-1).
+// The +-2 pi diagonal does not depend on k.  Layout as the assembly
+// (transposed buffers, complex128).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(AS_I) void k_green_dk(int n, const double* __restrict__ pos,
+                                                    const double* __restrict__ nvec, const double* __restrict__ area,
+                                                    const int* __restrict__ mat, const double* __restrict__ kmat,
+                                                    int nmat, double* __restrict__ dG, double* __restrict__ dH,
+                                                    long ld, long stride) {
+  const long b = blockIdx.z;
+  const int i = blockIdx.x * AS_I + threadIdx.x;
+  const int j0 = blockIdx.y * AS_J;
+  if (i >= n) return;
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  const double nxi = nvec[3 * i], nyi = nvec[3 * i + 1], nzi = nvec[3 * i + 2];
+  const int mi = mat[i];
+  const double kr = kmat[2 * (b * nmat + mi)], ki = kmat[2 * (b * nmat + mi) + 1];
+  dG += 2 * b * stride;
+  dH += 2 * b * stride;
+  const int jn = min(AS_J, n - j0);
+  for (int jj = 0; jj < jn; ++jj) {
+    const int j = j0 + jj;
+    double gr = 0, gi = 0, fr = 0, fi = 0;
+    if (mat[j] == mi && j != i) {
+      const double x = xi - pos[3 * j], y = yi - pos[3 * j + 1], z = zi - pos[3 * j + 2];
+      const double d = fmax(sqrt(x * x + y * y + z * z), 2.220446049250313e-16);
+      const double aj = area[j];
+      double s, c;
+      sincos(kr * d, &s, &c);
+      const double e = exp(-ki * d);
+      const double phr = e * c, phi = e * s;
+      // i a e^{ikd}
+      gr = -aj * phi;
+      gi = aj * phr;
+      // -k (n.r) a e^{ikd} / d
+      const double sc = -(nxi * x + nyi * y + nzi * z) * aj / d;
+      const double tr = sc * kr, ti = sc * ki;
+      fr = tr * phr - ti * phi;
+      fi = tr * phi + ti * phr;
+    }
+    const long o = 2 * ((long)j * ld + i);
+    dG[o] = gr;
+    dG[o + 1] = gi;
+    dH[o] = fr;
+    dH[o + 1] = fi;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_green_dk_overlay(int npairs, int ord1, const int* __restrict__ prow,
+                                                           const int* __restrict__ pcol, const double* __restrict__ g,
+                                                           const double* __restrict__ f,
+                                                           const double* __restrict__ pos,
+                                                           const double* __restrict__ nvec,
+                                                           const int* __restrict__ mat,
+                                                           const double* __restrict__ kmat, int nmat, double* dG,
+                                                           double* dH, long ld, long stride) {
+  const long b = blockIdx.y;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int i = prow[p], j = pcol[p];
+  const int mi = mat[i];
+  if (mat[j] != mi) return;
+  const double kr = kmat[2 * (b * nmat + mi)], ki = kmat[2 * (b * nmat + mi) + 1];
+  const double x = pos[3 * i] - pos[3 * j], y = pos[3 * i + 1] - pos[3 * j + 1], z = pos[3 * i + 2] - pos[3 * j + 2];
+  const double d = fmax(sqrt(x * x + y * y + z * z), 2.220446049250313e-16);
+  const double nx = nvec[3 * i], ny = nvec[3 * i + 1], nz = nvec[3 * i + 2];
+  // S = sum c_q (ik)^q and S' = sum q i c_q (ik)^(q-1), ik = -ki + i kr
+  cplx<double> ikp = cmake(1.0, 0.0), ikm = cmake(0.0, 0.0);  // (ik)^q, (ik)^(q-1)
+  cplx<double> sg = cmake(0.0, 0.0), sf = cmake(0.0, 0.0), dsg = cmake(0.0, 0.0), dsf = cmake(0.0, 0.0);
+  const cplx<double> ik = cmake(-ki, kr);
+  for (int q = 0; q < ord1; ++q) {
+    const double gq = g[(long)p * ord1 + q];
+    const double fq = nx * f[((long)p * 3 + 0) * ord1 + q] + ny * f[((long)p * 3 + 1) * ord1 + q] +
+                      nz * f[((long)p * 3 + 2) * ord1 + q];
+    sg = cadd(sg, cscale(ikp, gq));
+    sf = cadd(sf, cscale(ikp, fq));
+    if (q > 0) {
+      // q i (ik)^(q-1)
+      const cplx<double> t = cmake(-q * ikm.im, q * ikm.re);
+      dsg = cadd(dsg, cscale(t, gq));
+      dsf = cadd(dsf, cscale(t, fq));
+    }
+    ikm = ikp;
+    ikp = cmul(ikp, ik);
+  }
+  double s, c;
+  sincos(kr * d, &s, &c);
+  const double e = exp(-ki * d);
+  const cplx<double> ph = cmake(e * c, e * s);
+  const cplx<double> id = cmake(0.0, d);  // i d
+  const cplx<double> vg = cmul(cadd(dsg, cmul(id, sg)), ph);
+  const cplx<double> vf = cmul(cadd(dsf, cmul(id, sf)), ph);
+  const long o = 2 * (b * stride + (long)j * ld + i);
+  dG[o] = vg.re;
+  dG[o + 1] = vg.im;
+  dH[o] = vf.re;
+  dH[o + 1] = vf.im;
+}
+
+// dG/dk, dH/dk of side `mat` (complex128, transposed buffers) for a batch of
+// wavelengths; arguments as mnpbem_green_assemble.
+extern "C" int mnpbem_green_dk(int n, const double* pos, const double* nvec, const double* area, const int* mat,
+                               const double* kmat, int nmat, int order, int npairs, const int* prow, const int* pcol,
+                               const double* g, const double* f, void* dG, void* dH, long ld, long stride,
+                               int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n >= 0 && ld >= n && batch >= 0 && nmat >= 1, "green_dk: bad size");
+  if (n == 0 || batch == 0) return 0;
+  dim3 g1(mnp_ceil_div(n, AS_I), mnp_ceil_div(n, AS_J), batch);
+  hipLaunchKernelGGL(k_green_dk, g1, dim3(AS_I), 0, st, n, pos, nvec, area, mat, kmat, nmat, (double*)dG,
+                     (double*)dH, ld, stride);
+  MNP_LAUNCH_CHECK();
+  if (npairs > 0) {
+    dim3 g2(mnp_ceil_div(npairs, 256), batch);
+    hipLaunchKernelGGL(k_green_dk_overlay, g2, dim3(256), 0, st, npairs, order + 1, prow, pcol, g, f, pos, nvec,
+                       mat, kmat, nmat, (double*)dG, (double*)dH, ld, stride);
+    MNP_LAUNCH_CHECK();
+  }
+  return 0;
+}
+

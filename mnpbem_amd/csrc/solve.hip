@@ -959,6 +959,60 @@ extern "C" int mnpbem_bem_combine_batched(int n, int P, const void* Y, long ldy,
   return 0;
 }
 
+// Right side of the permittivity sensitivity solve A x' = -(dA/deps1) x for
+// the inside permittivity eps1 (k1 = k0 sqrt(eps1), dk1/deps1 = kap):
+// with Y = [G1 X1 | H1 X1 | G1' X1 | H1' X1] (n x 16P, ' = d/dk1, X1 =
+// [sig1 | h1]) and the operator of GdA&H Eqs. (10, 11, 14, 17),
+//   dA x = explicit:  alpha -= ik0 n (G1 s1),  De += H1 s1 - ik0 n.(G1 h1)
+//        + kap * (phi: G1' s1, a: G1' h1, alpha: H1' h1 - ik0 n eps1 G1' s1,
+//                 De: eps1 H1' s1 - ik0 n.(eps1 G1' h1)),
+// R = -dA x in the right-side layout [phi | a | De | alpha].
+__global__ void k_deps_rhs(int n, int P, const double* Y, long ldy, long sy, double* R, long ldv, long sv,
+                           const double* nvec, const double* par, const double* kap) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Par q = load_par(par, b);
+  const cplx<double> kp = cmake(kap[2 * b], kap[2 * b + 1]);
+  const double* y = Y + 2 * (b * sy + (long)i * ldy);
+  double* r = R + 2 * (b * sv + (long)i * ldv);
+  const int P4 = 4 * P;
+  auto G = [&](int col) { return ldd(y + 2 * col); };
+  auto H = [&](int col) { return ldd(y + 2 * (P4 + col)); };
+  auto Gd = [&](int col) { return ldd(y + 2 * (2 * P4 + col)); };
+  auto Hd = [&](int col) { return ldd(y + 2 * (3 * P4 + col)); };
+  auto put = [&](int col, cplx<double> v) { std_(r + 2 * col, cmake(-v.re, -v.im)); };
+  for (int p = 0; p < P; ++p) {
+    put(p, cmul(kp, Gd(p)));
+    cplx<double> ndg = cmake(0.0, 0.0), nddg = cmake(0.0, 0.0);
+    for (int c = 0; c < 3; ++c) {
+      const int hc = P + c * P + p;
+      const double nc = nvec[3 * i + c];
+      put(hc, cmul(kp, Gd(hc)));
+      // alpha_c
+      const cplx<double> ex = ik_times(q.k, cscale(G(p), -nc));
+      const cplx<double> im = csub(Hd(hc), ik_times(q.k, cscale(cmul(q.e1, Gd(p)), nc)));
+      put(4 * P + hc, cadd(ex, cmul(kp, im)));
+      ndg = cadd(ndg, cscale(G(hc), nc));
+      nddg = cadd(nddg, cscale(Gd(hc), nc));
+    }
+    const cplx<double> ex = csub(H(p), ik_times(q.k, ndg));
+    const cplx<double> im = csub(cmul(q.e1, Hd(p)), ik_times(q.k, cmul(q.e1, nddg)));
+    put(4 * P + p, cadd(ex, cmul(kp, im)));
+  }
+}
+
+extern "C" int mnpbem_deps_rhs(int n, int P, const void* Y, long ldy, long sy, void* R, long ldv, long sv,
+                               const double* nvec, const double* par, const double* kap, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && ldy >= 16 * P && ldv >= 8 * P, "deps_rhs: bad size");
+  if (batch <= 0) return 0;
+  hipLaunchKernelGGL(k_deps_rhs, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, P, (const double*)Y, ldy,
+                     sy, (double*)R, ldv, sv, nvec, par, kap);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
 // dst(double) += src (float or double), count real values
 extern "C" int mnpbem_accumulate(int src_dtype, long count, const void* src, double* dst, void* stream) {
   hipStream_t st = (hipStream_t)stream;

@@ -52,6 +52,16 @@ class EpsDrude:
         k = _k(enei, eps)
         return (complex(eps), complex(k)) if enei.ndim == 0 else (eps, k)
 
+    def deriv(self, enei):
+        """d eps / d(eps0, wp, gammad) at the wavelengths ``enei``, shape (..., 3)."""
+        enei = np.asarray(enei, dtype=float)
+        w = EV2NM / enei
+        den = w * (w + 1j * self.gammad)
+        d0 = np.ones(enei.shape, dtype=complex)
+        dwp = -2.0 * self.wp / den
+        dg = 1j * self.wp ** 2 / (w * (w + 1j * self.gammad) ** 2)
+        return np.stack([d0, dwp, dg], axis=-1)
+
 
 class EpsTable:
     """Tabulated refractive index (energy eV, n, k) with not-a-knot cubic
