@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "mixed-fp32"],
                     help="mixed: complex64 GEMMs on bf16 matrix cores (3 split products); "
                          "mixed-fp32: on FP32 matrix cores")
+    ap.add_argument("--refine-tol", type=float, default=1e-12,
+                    help="FP64 refinement stops when ||R - A X|| <= tol ||R|| for every wavelength")
     ap.add_argument("--workload", default="sphere", choices=["sphere", "uneven", "dimer"],
                     help="sphere: the headline 64-wavelength sweep of the 2884-face gold sphere (default); "
                          "uneven: 32 seeded spheres/ellipsoids of 284-2884 faces, 64 wavelengths each; "
@@ -128,7 +130,7 @@ def main():
     N = cp.n
     pw = m.PlaneWaveRet(pol, dr)
     t0 = time.time()
-    bem = m.BEMRet(cp, precision=args.precision)
+    bem = m.BEMRet(cp, precision=args.precision, refine_tol=args.refine_tol)
     torch.cuda.synchronize()
     log("setup (mesh + GPU refinement coefficients): %.2f s, N = %d" % (time.time() - t0, N))
 
@@ -143,7 +145,8 @@ def main():
     t0 = time.perf_counter()
     for k in range(args.steps):
         ext_mixed = sweep(bem)
-        log("step %d done (refinement iterations %s)" % (k, bem.last_refine_iters))
+        log("step %d done (refinement solves %s, final residual %.2e)"
+            % (k, bem.last_refine_iters, float(np.max(bem.last_residual))))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     import ctypes
