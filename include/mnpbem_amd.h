@@ -262,6 +262,37 @@ int mnpbem_hmat_matvec(int dtype, int nblk, int nleaf, int maxchunks, const long
 /* Out = M as an n x n complex128 row-major matrix (face order, leading dimension ldo) */
 int mnpbem_hmat_full(int dtype, int nblk, const long* blk, const int* perm, const void* val, const void* U,
                      const void* V, void* Out, long ldo, void* stream);
+/* Recompression of the low-rank blocks bids[0:nb] (replaces HMatrix.truncate /
+ * _truncate_block, hmatrix.py:672-737), step 1: QR of both factors (Q into the
+ * complex128 workspace Q, factor side s of block l at qoff[2 l + s]), SVD of
+ * R1 R2^T, rank r = #{cumsum(s) < (1 - htol) sum(s)} (at least 1).  Ranks
+ * 1 <= k <= kst <= 128.  RT: complex128 workspace of 2 nb kst^2 elements that
+ * returns the k x r transforms; XZ: the same size, needed only for kst > 64
+ * (else null).  rank_out[l] = r, or -1 for a block outside these limits. */
+int mnpbem_hmat_truncate(int dtype, int nb, const long* blk, const int* bids, const void* U, const void* V,
+                         const long* qoff, int kst, double htol, void* Q, void* RT, void* XZ, int* rank_out,
+                         void* stream);
+/* Step 2: U' = Q1 T1, V' = Q2 T2 into the packed factors of blk_new (the
+ * block table with the new ranks and offsets); zsplit workgroups per factor. */
+int mnpbem_hmat_trunc_form(int dtype, int nb, const long* blk_old, const long* blk_new, const int* bids,
+                           const long* qoff, const void* Q, const void* T, int kst, int zsplit, void* Un, void* Vn,
+                           void* stream);
+/* Out = alpha X + beta Y over count complex values of the storage type; Y may
+ * be null (replaces the scalar multiple, negation and dense part of the sum,
+ * hmatrix.py:553-670). */
+int mnpbem_hmat_axpby(int dtype, long count, double ar, double ai, const void* X, double br, double bi,
+                      const void* Y, void* Out, void* stream);
+/* Low-rank factors of a sum on one partition: blk_n's blocks bids[0:nb] get
+ * U = [Ua, Ub], V = [Va, Vb] (replaces the stacking in _plus_hmat,
+ * hmatrix.py:631-666; truncate follows). */
+int mnpbem_hmat_concat(int dtype, int nb, const long* blk_a, const long* blk_b, const long* blk_n, const int* bids,
+                       const void* Ua, const void* Va, const void* Ub, const void* Vb, void* Un, void* Vn,
+                       void* stream);
+/* Diagonal dense blocks bids[0:nb]: mode 0 writes their diagonal to out
+ * (complex128, face order; replaces HMatrix.diag, hmatrix.py:758-778), mode 1
+ * sets it to one (HMatrix.eye_hmat, hmatrix.py:780-798). */
+int mnpbem_hmat_diag(int dtype, int nb, const long* blk, const int* bids, const int* perm, void* val, void* out,
+                     int mode, void* stream);
 
 /* Scattering cross section (planewave_ret.py:391-435 over the far field of
  * spectrum_ret.py:231-459): far fields e, h in the ndir directions dirs

@@ -53,6 +53,11 @@ SIGNATURES = {
     "mnpbem_hmat_compact": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p],
     "mnpbem_hmat_matvec": [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p, _l, _p, _l, _i, _p, _p],
     "mnpbem_hmat_full": [_i, _i, _p, _p, _p, _p, _p, _p, _l, _p],
+    "mnpbem_hmat_truncate": [_i, _i, _p, _p, _p, _p, _p, _i, _d, _p, _p, _p, _p, _p],
+    "mnpbem_hmat_trunc_form": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p],
+    "mnpbem_hmat_axpby": [_i, _l, _d, _d, _p, _d, _d, _p, _p, _p],
+    "mnpbem_hmat_concat": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
+    "mnpbem_hmat_diag": [_i, _i, _p, _p, _p, _p, _p, _i, _p],
     "mnpbem_scattering_batched": [_i, _i, _p, _l, _l, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p],
     "mnpbem_refine_classify": [_i, _p, _p, _d, _d, _i, _p, _p, _p, _p, _p, _p],
     "mnpbem_surface_field": [_i, _i, _i, _d, _p, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _p, _p, _l, _l, _p, _p,

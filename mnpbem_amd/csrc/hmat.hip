@@ -392,6 +392,326 @@ __global__ __launch_bounds__(256) void k_hmat_full(const long* blk, const int* p
   }
 }
 
+// ---------------------------------------------------------------------------
+// Recompression of the low-rank blocks (hmatrix.py:672-737): QR of both
+// factors U = Q1 R1, V = Q2 R2; SVD of X = R1 R2^T = W S Z^H; keep the
+// leading r singular values, r = #{cumsum(s) < (1 - htol) sum(s)} (at least
+// 1); new factors U' = Q1 (W_r S_r) = Q1 (X Z_r), V' = Q2 conj(Z_r).
+//   k_trunc_qr   one workgroup per (block, factor): classical Gram-Schmidt
+//                with one reorthogonalisation pass and deflation of
+//                dependent columns, Q into a complex128 workspace, R (k x k)
+//                beside it
+//   k_trunc_svd  one wave per block: one-sided (Hestenes) Jacobi on X, lane
+//                i owning rows i and i + 64 of X and of Z, so the rotations
+//                need no barriers; the column norms give S.  X and Z sit in
+//                LDS for ranks <= 64, in a global workspace up to 128
+//   k_trunc_form U', V' = Q (X Z_r), Q conj(Z_r) in the storage type
+// Singular vectors are fixed up to one phase per pair, so the new factors may
+// differ from a LAPACK SVD's by column phases; their product does not.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum1(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+constexpr double QR_DEFLATE = 1e-13;  // residual / original norm below which a column is dependent
+constexpr int TR_KMAX = 128;   // largest rank the recompression accepts
+constexpr int TR_KLDS = 64;    // ranks up to this keep the SVD work in LDS
+
+// Q, R of factor `side` (0: U, m rows; 1: V, n rows) of block bids[lb].
+// Q at Q + qoff[2 lb + side] (column-major, rows x k); R column-major k x k at
+// R + (2 lb + side) kst^2.
+template <typename TS>
+__global__ __launch_bounds__(256) void k_trunc_qr(const long* blk, const int* bids, const TS* Uv, const TS* Vv,
+                                                  const long* qoff, double* Q, double* R, int kst) {
+  const int lb = blockIdx.x, side = blockIdx.y, tid = threadIdx.x;
+  const Blk B = load_blk(blk, bids[lb]);
+  const int k = (int)B.rank;
+  if (k < 1 || k > kst) return;  // k_trunc_svd reports the block
+  const long rows = side ? B.n : B.m;
+  const TS* src = side ? Vv + 2 * B.ob : Uv + 2 * B.oa;
+  double* q = Q + 2 * qoff[2 * lb + side];
+  double* r = R + 2 * (long)(2 * lb + side) * kst * kst;
+  __shared__ cplx<double> d[TR_KMAX];
+  __shared__ double part[4];
+  for (long e = tid; e < rows * k; e += blockDim.x) {
+    const cplx<double> v = ldw(src + 2 * e);
+    q[2 * e] = v.re;
+    q[2 * e + 1] = v.im;
+  }
+  for (int e = tid; e < k * k; e += blockDim.x) r[2 * e] = r[2 * e + 1] = 0.0;
+  __syncthreads();
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int j = 0; j < k; ++j) {
+    double* qj = q + 2 * (long)j * rows;
+    double p0 = 0.0;  // ||f_j||^2 before the projections
+    for (long i = tid; i < rows; i += blockDim.x) {
+      const cplx<double> v = ld2(qj + 2 * i);
+      p0 += v.re * v.re + v.im * v.im;
+    }
+    p0 = wave_sum1(p0);
+    if (lane == 0) part[wv] = p0;
+    __syncthreads();
+    const double n0 = sqrt(part[0] + part[1] + part[2] + part[3]);
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int c = wv; c < j; c += 4) {  // d_c = q_c^H q_j
+        const double* qc = q + 2 * (long)c * rows;
+        cplx<double> s = cmake(0.0, 0.0);
+        for (long i = lane; i < rows; i += 64) {
+          const cplx<double> x = ld2(qc + 2 * i), y = ld2(qj + 2 * i);
+          s.re += x.re * y.re + x.im * y.im;
+          s.im += x.re * y.im - x.im * y.re;
+        }
+        s = wave_sum(s);
+        if (lane == 0) {
+          d[c] = s;
+          r[2 * (j * k + c)] += s.re;
+          r[2 * (j * k + c) + 1] += s.im;
+        }
+      }
+      __syncthreads();
+      for (long i = tid; i < rows; i += blockDim.x) {
+        cplx<double> acc = ld2(qj + 2 * i);
+        for (int c = 0; c < j; ++c) acc = csub(acc, cmul(ld2(q + 2 * ((long)c * rows + i)), d[c]));
+        qj[2 * i] = acc.re;
+        qj[2 * i + 1] = acc.im;
+      }
+      __syncthreads();
+    }
+    double ps = 0.0;
+    for (long i = tid; i < rows; i += blockDim.x) {
+      const cplx<double> v = ld2(qj + 2 * i);
+      ps += v.re * v.re + v.im * v.im;
+    }
+    ps = wave_sum1(ps);
+    if (lane == 0) part[wv] = ps;
+    __syncthreads();
+    double nrm = sqrt(part[0] + part[1] + part[2] + part[3]);
+    // a column left with rounding noise only (in the span of the previous
+    // ones, e.g. the repeated factors of a sum) is deflated: zero Q column,
+    // zero diagonal.  Normalising the noise would feed non-orthogonal
+    // directions into every later projection.
+    const bool dependent = !(nrm > QR_DEFLATE * n0);
+    if (dependent) nrm = 0.0;
+    if (tid == 0) {
+      r[2 * (j * k + j)] = nrm;
+      r[2 * (j * k + j) + 1] = 0.0;
+    }
+    const double inv = dependent ? 0.0 : 1.0 / nrm;
+    for (long i = tid; i < rows; i += blockDim.x) {
+      qj[2 * i] *= inv;
+      qj[2 * i + 1] *= inv;
+    }
+    __syncthreads();
+  }
+}
+
+// SVD of X = R1 R2^T and the truncation rule.  RT: on entry R1, R2 (column-
+// major k x k at RT + (2 lb + side) kst^2); on exit T1 = X Z_r and
+// T2 = conj(Z_r) (k x r, column-major, same places).  rank_out receives r.
+// Factors whose Frobenius norm is below machine epsilon are kept as they are
+// (T = R, r = k); a block whose singular values sum below epsilon becomes a
+// zero rank-1 block (hmatrix.py:703-721).  X and Z (k x k each) live in
+// dynamic LDS when XZ is null (kst <= 64), else in XZ + 2 lb kst^2 (complex).
+__global__ __launch_bounds__(64) void k_trunc_svd(const long* blk, const int* bids, double* RT, int kst,
+                                                  double htol, double* XZ, int* rank_out) {
+  extern __shared__ double dyn[];
+  const int lb = blockIdx.x, i = threadIdx.x;
+  const Blk B = load_blk(blk, bids[lb]);
+  const int k = (int)B.rank;
+  if (k < 1 || k > kst) {
+    if (i == 0) rank_out[lb] = -1;
+    return;
+  }
+  cplx<double>* X = XZ ? reinterpret_cast<cplx<double>*>(XZ) + 2 * (long)lb * kst * kst
+                       : reinterpret_cast<cplx<double>*>(dyn);
+  cplx<double>* Z = X + k * k;
+  __shared__ double sig[TR_KMAX];
+  __shared__ int ord[TR_KMAX];
+  double* r1 = RT + 2 * (long)(2 * lb) * kst * kst;
+  double* r2 = RT + 2 * (long)(2 * lb + 1) * kst * kst;
+  const double eps = 2.220446049250313e-16;
+  // ||U||_F = ||R1||_F, ||V||_F = ||R2||_F
+  double n1 = 0.0, n2 = 0.0;
+  for (int ii = i; ii < k; ii += 64)
+    for (int l = ii; l < k; ++l) {
+      const cplx<double> a = ld2(r1 + 2 * (l * k + ii)), b = ld2(r2 + 2 * (l * k + ii));
+      n1 += a.re * a.re + a.im * a.im;
+      n2 += b.re * b.re + b.im * b.im;
+    }
+  n1 = sqrt(wave_sum1(n1));
+  n2 = sqrt(wave_sum1(n2));
+  if (n1 < eps || n2 < eps) {  // T = R already in place
+    if (i == 0) rank_out[lb] = k;
+    return;
+  }
+  // X[i][j] = sum_{l >= max(i, j)} R1[i][l] R2[j][l]; Z = I
+  for (int ii = i; ii < k; ii += 64)
+    for (int j = 0; j < k; ++j) {
+      cplx<double> s = cmake(0.0, 0.0);
+      for (int l = ii > j ? ii : j; l < k; ++l) s = cfma(s, ld2(r1 + 2 * (l * k + ii)), ld2(r2 + 2 * (l * k + j)));
+      X[j * k + ii] = s;
+      Z[j * k + ii] = cmake(ii == j ? 1.0 : 0.0, 0.0);
+    }
+  // one-sided Jacobi: rotate column pairs until all are orthogonal.  Lane i
+  // owns rows i, i + 64 of X and Z; only the sums cross lanes.
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    int rotated = 0;
+    for (int p = 0; p < k - 1; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;  // g = x_p^H x_q
+        for (int ii = i; ii < k; ii += 64) {
+          const cplx<double> xp = X[p * k + ii], xq = X[q * k + ii];
+          a += xp.re * xp.re + xp.im * xp.im;
+          b += xq.re * xq.re + xq.im * xq.im;
+          gr += xp.re * xq.re + xp.im * xq.im;
+          gi += xp.re * xq.im - xp.im * xq.re;
+        }
+        a = wave_sum1(a);
+        b = wave_sum1(b);
+        gr = wave_sum1(gr);
+        gi = wave_sum1(gi);
+        const double g = hypot(gr, gi);
+        if (!(g > eps * sqrt(a * b)) || g == 0.0) continue;
+        rotated = 1;
+        // columns [x_p, e^{-i phi} x_q] have real Gram [[a, g], [g, b]]
+        const double zeta = (b - a) / (2.0 * g);
+        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+        const cplx<double> w = cmake(gr / g, -gi / g);
+        for (int ii = i; ii < k; ii += 64) {
+          const cplx<double> xp = X[p * k + ii], yq = cmul(X[q * k + ii], w);
+          X[p * k + ii] = csub(cscale(xp, c), cscale(yq, s));
+          X[q * k + ii] = cadd(cscale(xp, s), cscale(yq, c));
+          const cplx<double> zp = Z[p * k + ii], zq = cmul(Z[q * k + ii], w);
+          Z[p * k + ii] = csub(cscale(zp, c), cscale(zq, s));
+          Z[q * k + ii] = cadd(cscale(zp, s), cscale(zq, c));
+        }
+      }
+    if (!rotated) break;
+  }
+  // singular values = column norms of X Z; order by descending value, ties by index
+  for (int p = 0; p < k; ++p) {
+    double a = 0.0;
+    for (int ii = i; ii < k; ii += 64) {
+      const cplx<double> x = X[p * k + ii];
+      a += x.re * x.re + x.im * x.im;
+    }
+    a = sqrt(wave_sum1(a));
+    if (i == 0) sig[p] = a;
+  }
+  __syncthreads();
+  for (int ii = i; ii < k; ii += 64) {
+    const double me = sig[ii];
+    int pos = 0;
+    for (int p = 0; p < k; ++p) pos += (sig[p] > me || (sig[p] == me && p < ii)) ? 1 : 0;
+    ord[pos] = ii;
+  }
+  __syncthreads();
+  double tot = 0.0;
+  for (int p = 0; p < k; ++p) tot += sig[ord[p]];
+  if (tot < eps) {  // zero rank-1 block
+    for (int ii = i; ii < k; ii += 64) {
+      r1[2 * ii] = r1[2 * ii + 1] = 0.0;
+      r2[2 * ii] = r2[2 * ii + 1] = 0.0;
+    }
+    if (i == 0) rank_out[lb] = 1;
+    return;
+  }
+  int rr = 0;
+  double cum = 0.0;
+  const double thr = (1.0 - htol) * tot;
+  for (int p = 0; p < k; ++p) {
+    cum += sig[ord[p]];
+    rr += cum < thr ? 1 : 0;
+  }
+  if (rr == 0) rr = 1;
+  for (int ii = i; ii < k; ii += 64)
+    for (int c = 0; c < rr; ++c) {
+      const int zc = ord[c];
+      const cplx<double> x = X[zc * k + ii], z = Z[zc * k + ii];
+      r1[2 * (c * k + ii)] = x.re;
+      r1[2 * (c * k + ii) + 1] = x.im;
+      r2[2 * (c * k + ii)] = z.re;
+      r2[2 * (c * k + ii) + 1] = -z.im;
+    }
+  if (i == 0) rank_out[lb] = rr;
+}
+
+// new factors: U' = Q1 T1 (m x r) at the new table's off_a, V' = Q2 T2
+// (n x r) at off_b, in the storage type
+template <typename TS>
+__global__ __launch_bounds__(256) void k_trunc_form(const long* blk_old, const long* blk_new, const int* bids,
+                                                    const long* qoff, const double* Q, const double* T, int kst,
+                                                    TS* Un, TS* Vn) {
+  const int lb = blockIdx.x, side = blockIdx.y;
+  const Blk Bo = load_blk(blk_old, bids[lb]);
+  const Blk Bn = load_blk(blk_new, bids[lb]);
+  const int k = (int)Bo.rank, r = (int)Bn.rank;
+  if (k < 1 || k > kst || r < 0 || r > k) return;
+  const long rows = side ? Bo.n : Bo.m;
+  const double* q = Q + 2 * qoff[2 * lb + side];
+  const double* t = T + 2 * (long)(2 * lb + side) * kst * kst;
+  TS* out = side ? Vn + 2 * Bn.ob : Un + 2 * Bn.oa;
+  for (long e = (long)blockIdx.z * blockDim.x + threadIdx.x; e < rows * r; e += (long)gridDim.z * blockDim.x) {
+    const long i = e % rows, c = e / rows;
+    cplx<double> s = cmake(0.0, 0.0);
+    for (int a = 0; a < k; ++a) s = cfma(s, ld2(q + 2 * ((long)a * rows + i)), ld2(t + 2 * (c * k + a)));
+    cst(out + 2 * e, cmake<TS>((TS)s.re, (TS)s.im));
+  }
+}
+
+// Out = alpha X + beta Y (Y may be null: Out = alpha X), elementwise over
+// `count` complex values of the storage type; Out may alias X or Y
+template <typename TS>
+__global__ __launch_bounds__(256) void k_hmat_axpby(long count, double ar, double ai, const TS* X, double br,
+                                                    double bi, const TS* Y, TS* Out) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (long)gridDim.x * blockDim.x) {
+    const cplx<double> x = ldw(X + 2 * e);
+    cplx<double> o = cmul(cmake(ar, ai), x);
+    if (Y) o = cfma(o, cmake(br, bi), ldw(Y + 2 * e));
+    cst(Out + 2 * e, cmake<TS>((TS)o.re, (TS)o.im));
+  }
+}
+
+// low-rank blocks of a sum: U = [Ua, Ub], V = [Va, Vb] (columns of a first);
+// the three tables share the block geometry, ranks add up
+template <typename TS>
+__global__ __launch_bounds__(256) void k_hmat_concat(const long* blk_a, const long* blk_b, const long* blk_n,
+                                                     const int* bids, const TS* Ua, const TS* Va, const TS* Ub,
+                                                     const TS* Vb, TS* Un, TS* Vn) {
+  const long b = bids[blockIdx.x];
+  const Blk A = load_blk(blk_a, b), Bb = load_blk(blk_b, b), N = load_blk(blk_n, b);
+  const long ka = A.rank > 0 ? A.rank : 0, kb = Bb.rank > 0 ? Bb.rank : 0;
+  for (long e = threadIdx.x; e < N.m * (ka + kb); e += blockDim.x) {
+    const TS* s = e < N.m * ka ? Ua + 2 * (A.oa + e) : Ub + 2 * (Bb.oa + e - N.m * ka);
+    cst(Un + 2 * (N.oa + e), cld(s));
+  }
+  for (long e = threadIdx.x; e < N.n * (ka + kb); e += blockDim.x) {
+    const TS* s = e < N.n * ka ? Va + 2 * (A.ob + e) : Vb + 2 * (Bb.ob + e - N.n * ka);
+    cst(Vn + 2 * (N.ob + e), cld(s));
+  }
+}
+
+// diagonal of the diagonal dense blocks (r0 == c0, m == n) into out (complex128,
+// face order), or (mode 1) set those blocks to the identity
+template <typename TS>
+__global__ __launch_bounds__(256) void k_hmat_diag(const long* blk, const int* bids, const int* perm, TS* val,
+                                                   double* out, int mode) {
+  const Blk B = load_blk(blk, bids[blockIdx.x]);
+  for (long i = threadIdx.x; i < B.m; i += blockDim.x) {
+    TS* v = val + 2 * (B.oa + i * B.m + i);
+    if (mode == 0) {
+      const cplx<double> d = ldw(v);
+      out[2 * (long)perm[B.r0 + i]] = d.re;
+      out[2 * (long)perm[B.r0 + i] + 1] = d.im;
+    } else {
+      cst(v, cmake<TS>((TS)1, (TS)0));
+    }
+  }
+}
+
 template <typename TS, int NC>
 int matvec_launch(int nblk, int nleaf, int maxchunks, const long* blk, const long* leafs, const int* lst,
                   const int* perm, const void* val, const void* U, const void* V, int ncol, const double* X,
@@ -509,6 +829,138 @@ extern "C" int mnpbem_hmat_full(int dtype, int nblk, const long* blk, const int*
                        (const float*)U, (const float*)V, (double*)Out, ldo);
   else {
     mnp_set_error("hmat_full: dtype must be 1 or 2");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Recompression, step 1 (hmatrix.py:672-737): for the low-rank blocks
+// bids[0:nb] (each of rank 1 <= k <= kst <= 128), QR of both factors into Q
+// (complex128, factor `side` of block l at qoff[2 l + side], column-major),
+// SVD of R1 R2^T and the truncation rule at htol.  RT: complex128 workspace of
+// 2 nb kst^2 elements; on return it holds the k x r transforms that
+// mnpbem_hmat_trunc_form applies.  XZ: complex128 workspace of 2 nb kst^2
+// elements, required for kst > 64 (null otherwise).  rank_out[l] receives the
+// new rank (-1: block outside the supported ranks).
+extern "C" int mnpbem_hmat_truncate(int dtype, int nb, const long* blk, const int* bids, const void* U, const void* V,
+                                    const long* qoff, int kst, double htol, void* Q, void* RT, void* XZ,
+                                    int* rank_out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(kst >= 1 && kst <= TR_KMAX, "hmat_truncate: ranks above 128 are not supported");
+  MNP_REQUIRE(kst <= TR_KLDS || XZ != nullptr, "hmat_truncate: ranks above 64 need the XZ workspace");
+  MNP_REQUIRE(htol >= 0.0 && htol < 1.0, "hmat_truncate: htol must lie in [0, 1)");
+  if (nb <= 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_trunc_qr<double>, dim3(nb, 2), dim3(256), 0, st, blk, bids, (const double*)U,
+                       (const double*)V, qoff, (double*)Q, (double*)RT, kst);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_trunc_qr<float>, dim3(nb, 2), dim3(256), 0, st, blk, bids, (const float*)U,
+                       (const float*)V, qoff, (double*)Q, (double*)RT, kst);
+  else {
+    mnp_set_error("hmat_truncate: dtype must be 1 or 2");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  const bool lds = kst <= TR_KLDS;
+  const size_t bytes = lds ? (size_t)2 * kst * kst * 2 * sizeof(double) : 0;
+  if (bytes > 65536)
+    MNP_HIP_CHECK(
+        hipFuncSetAttribute((const void*)k_trunc_svd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  hipLaunchKernelGGL(k_trunc_svd, dim3(nb), dim3(64), bytes, st, blk, bids, (double*)RT, kst, htol,
+                     lds ? nullptr : (double*)XZ, rank_out);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Recompression, step 2: U' = Q1 T1, V' = Q2 T2 into the packed factors of
+// the new block table (same geometry, ranks from mnpbem_hmat_truncate; T is
+// its RT workspace).  zsplit: workgroups per (block, factor).
+extern "C" int mnpbem_hmat_trunc_form(int dtype, int nb, const long* blk_old, const long* blk_new, const int* bids,
+                                      const long* qoff, const void* Q, const void* T, int kst, int zsplit, void* Un,
+                                      void* Vn, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(kst >= 1 && kst <= TR_KMAX, "hmat_trunc_form: ranks above 128 are not supported");
+  MNP_REQUIRE(zsplit >= 1 && zsplit <= 65535, "hmat_trunc_form: bad zsplit");
+  if (nb <= 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_trunc_form<double>, dim3(nb, 2, zsplit), dim3(256), 0, st, blk_old, blk_new, bids, qoff,
+                       (const double*)Q, (const double*)T, kst, (double*)Un, (double*)Vn);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_trunc_form<float>, dim3(nb, 2, zsplit), dim3(256), 0, st, blk_old, blk_new, bids, qoff,
+                       (const double*)Q, (const double*)T, kst, (float*)Un, (float*)Vn);
+  else {
+    mnp_set_error("hmat_trunc_form: dtype must be 1 or 2");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Out = alpha X + beta Y over `count` complex values of the storage type
+// (scalar multiple, negation and the dense part of a sum, hmatrix.py:553-670);
+// Y may be null.
+extern "C" int mnpbem_hmat_axpby(int dtype, long count, double ar, double ai, const void* X, double br, double bi,
+                                 const void* Y, void* Out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(count >= 0, "hmat_axpby: negative count");
+  if (count == 0) return 0;
+  const long want = (count + 255) / 256;
+  const int grid = (int)(want < 8192 ? want : 8192);
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_hmat_axpby<double>, dim3(grid), dim3(256), 0, st, count, ar, ai, (const double*)X, br, bi,
+                       (const double*)Y, (double*)Out);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_hmat_axpby<float>, dim3(grid), dim3(256), 0, st, count, ar, ai, (const float*)X, br, bi,
+                       (const float*)Y, (float*)Out);
+  else {
+    mnp_set_error("hmat_axpby: dtype must be 1 or 2");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Low-rank blocks of a sum of two H-matrices on one tree (hmatrix.py:592-670):
+// factors of blk_n's blocks bids[0:nb] become [Ua, Ub], [Va, Vb].
+extern "C" int mnpbem_hmat_concat(int dtype, int nb, const long* blk_a, const long* blk_b, const long* blk_n,
+                                  const int* bids, const void* Ua, const void* Va, const void* Ub, const void* Vb,
+                                  void* Un, void* Vn, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (nb <= 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_hmat_concat<double>, dim3(nb), dim3(256), 0, st, blk_a, blk_b, blk_n, bids,
+                       (const double*)Ua, (const double*)Va, (const double*)Ub, (const double*)Vb, (double*)Un,
+                       (double*)Vn);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_hmat_concat<float>, dim3(nb), dim3(256), 0, st, blk_a, blk_b, blk_n, bids,
+                       (const float*)Ua, (const float*)Va, (const float*)Ub, (const float*)Vb, (float*)Un,
+                       (float*)Vn);
+  else {
+    mnp_set_error("hmat_concat: dtype must be 1 or 2");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Diagonal dense blocks bids[0:nb] (r0 == c0, m == n): mode 0 writes their
+// diagonal to out (complex128, face order; hmatrix.py:758-778), mode 1 sets
+// their diagonal to one (identity, hmatrix.py:780-798, after a zero fill).
+extern "C" int mnpbem_hmat_diag(int dtype, int nb, const long* blk, const int* bids, const int* perm, void* val,
+                                void* out, int mode, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(mode == 0 || mode == 1, "hmat_diag: mode must be 0 or 1");
+  MNP_REQUIRE(mode == 1 || out != nullptr, "hmat_diag: output required");
+  if (nb <= 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL(k_hmat_diag<double>, dim3(nb), dim3(256), 0, st, blk, bids, perm, (double*)val, (double*)out,
+                       mode);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL(k_hmat_diag<float>, dim3(nb), dim3(256), 0, st, blk, bids, perm, (float*)val, (double*)out,
+                       mode);
+  else {
+    mnp_set_error("hmat_diag: dtype must be 1 or 2");
     return 1;
   }
   MNP_LAUNCH_CHECK();

@@ -190,6 +190,7 @@ class HMatrix:
         self.row1 = self.col1 = self.row2 = self.col2 = np.zeros(0, dtype=np.int64)
         self.ranks = np.zeros(0, dtype=np.int64)
         self._built = False
+        self._lu = None
         if tree is not None:
             self._init(tree, fadmiss)
 
@@ -311,6 +312,7 @@ class HMatrix:
         self._rsum = int(ranks.sum())
         self._T = torch.empty((max(self._rsum, 1) * 16,), dtype=torch.complex128, device=dev)
         self._built = True
+        self._lu = None
         return self
 
     @staticmethod
@@ -355,7 +357,7 @@ class HMatrix:
 
     def __matmul__(self, other):
         if isinstance(other, HMatrix):
-            raise NotImplementedError("HMatrix @ HMatrix is not provided")
+            return self._mtimes_hmat(other)
         return self.mtimes_vec(other)
 
     def full(self, xp=None):
@@ -392,6 +394,352 @@ class HMatrix:
     def bytes_per_matvec(self):
         """Block bytes one matvec reads (dense values plus both factors)."""
         return self.stat()["bytes"]
+
+    # ------------------------------------------------------- block inspection
+    def _host_blocks(self, which):
+        self._check()
+        blk = self._blk_host
+        nd = self._nd
+        if which == "val":
+            h = self._val.cpu().numpy().astype(np.complex128)
+            return [h[b[5]:b[5] + b[1] * b[3]].reshape(b[3], b[1]).T for b in blk[:nd]]
+        buf = (self._U if which == "lhs" else self._V).cpu().numpy().astype(np.complex128)
+        col, off = (1, 5) if which == "lhs" else (3, 6)
+        return [buf[b[off]:b[off] + b[col] * max(b[4], 0)].reshape(max(b[4], 0), b[col]).T for b in blk[nd:]]
+
+    @property
+    def val(self):
+        """Dense blocks as host arrays (row1/col1 order, rows and columns in
+        cluster order), as HMatrix.val of hmatrix.py."""
+        return self._host_blocks("val")
+
+    @property
+    def lhs(self):
+        """Left factors U (m x rank) of the low-rank blocks, host arrays."""
+        return self._host_blocks("lhs")
+
+    @property
+    def rhs(self):
+        """Right factors V (n x rank), block = U V^T, host arrays."""
+        return self._host_blocks("rhs")
+
+    # ------------------------------------------------------------------ algebra
+    def _set_lowrank(self, blk, U, V):
+        """Install a block table whose low-rank ranks (column 4) changed:
+        recompute the factor offsets and the matvec workspace."""
+        nd = self._nd
+        m, nn, r = blk[nd:, 1], blk[nd:, 3], np.maximum(blk[nd:, 4], 0)
+        blk[nd:, 5] = np.concatenate([[0], np.cumsum(m * r)[:-1]]) if len(r) else []
+        blk[nd:, 6] = np.concatenate([[0], np.cumsum(nn * r)[:-1]]) if len(r) else []
+        blk[nd:, 7] = np.concatenate([[0], np.cumsum(r)[:-1]]) if len(r) else []
+        self._blk_host = blk
+        self._blk = torch.as_tensor(blk).to(self.device)
+        self._U, self._V = U, V
+        self.ranks = blk[nd:, 4].copy()
+        self._rsum = int(r.sum())
+        self._T = torch.empty((max(self._rsum, 1) * 16,), dtype=torch.complex128, device=self.device)
+
+    def _factor_bufs(self, blk):
+        nd = self._nd
+        r = np.maximum(blk[nd:, 4], 0)
+        tdt = _DT[self.dtype][0]
+        nu = int((blk[nd:, 1] * r).sum())
+        nv = int((blk[nd:, 3] * r).sum())
+        return (torch.empty((max(nu, 1),), dtype=tdt, device=self.device),
+                torch.empty((max(nv, 1),), dtype=tdt, device=self.device))
+
+    def _copy(self):
+        """Deep copy: same tree and tables, own device blocks."""
+        self._check()
+        c = HMatrix.__new__(HMatrix)
+        c.__dict__.update(self.__dict__)
+        c._val, c._U, c._V = self._val.clone(), self._U.clone(), self._V.clone()
+        c._blk_host = self._blk_host.copy()
+        c._blk = self._blk.clone()
+        c.ranks = self.ranks.copy()
+        c._T = torch.empty_like(self._T)
+        c._lu = None
+        return c
+
+    def truncate(self, htol=None):
+        """Recompress every low-rank block in place (hmatrix.py:672-737): QR of
+        both factors, SVD of R1 R2^T, keep the leading singular values that
+        hold a (1 - htol) share of their sum (at least one).  Runs on the GPU
+        in FP64 for ranks up to 128."""
+        self._check()
+        htol = self.htol if htol is None else float(htol)
+        blk = self._blk_host
+        nd = self._nd
+        lr = np.nonzero(blk[nd:, 4] > 0)[0] + nd
+        if len(lr):
+            k = blk[lr, 4]
+            kst = int(k.max())
+            if kst > 128:
+                raise ValueError("HMatrix.truncate: ranks above 128 are not supported (largest rank %d)" % kst)
+            dev = self.device
+            st = nat.stream_handle(dev)
+            code = _DT[self.dtype][1]
+            nb = len(lr)
+            sizes = np.stack([blk[lr, 1] * k, blk[lr, 3] * k], axis=1).ravel()
+            qoff = torch.as_tensor(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)).to(dev)
+            Q = torch.empty((int(sizes.sum()),), dtype=torch.complex128, device=dev)
+            RT = torch.empty((2 * nb * kst * kst,), dtype=torch.complex128, device=dev)
+            XZ = torch.empty((2 * nb * kst * kst,), dtype=torch.complex128, device=dev) if kst > 64 else None
+            bids = torch.as_tensor(lr.astype(np.int32)).to(dev)
+            rk = torch.empty((nb,), dtype=torch.int32, device=dev)
+            nat.call("mnpbem_hmat_truncate", code, nb, nat.ptr(self._blk), nat.ptr(bids), nat.ptr(self._U),
+                     nat.ptr(self._V), nat.ptr(qoff), kst, htol, nat.ptr(Q), nat.ptr(RT),
+                     nat.ptr(XZ) if XZ is not None else None, nat.ptr(rk), st)
+            r = rk.cpu().numpy().astype(np.int64)
+            if np.any(r < 0):
+                raise RuntimeError("HMatrix.truncate: block outside the kernel's capacity")
+            new = blk.copy()
+            new[lr, 4] = r
+            old_dev = self._blk
+            Un, Vn = self._factor_bufs(new)
+            self._set_lowrank(new, Un, Vn)
+            el = int(max((blk[lr, 1] * r).max(), (blk[lr, 3] * r).max()))
+            zsplit = int(min(64, max(1, -(-el // 4096))))
+            nat.call("mnpbem_hmat_trunc_form", code, nb, nat.ptr(old_dev), nat.ptr(self._blk), nat.ptr(bids),
+                     nat.ptr(qoff), nat.ptr(Q), nat.ptr(RT), kst, zsplit, nat.ptr(Un), nat.ptr(Vn), st)
+        self.htol = htol
+        self._lu = None
+        return self
+
+    def _axpby(self, a, x, b=0.0, y=None, out=None):
+        out = x if out is None else out
+        ar, ai = complex(a).real, complex(a).imag
+        br, bi = complex(b).real, complex(b).imag
+        nat.call("mnpbem_hmat_axpby", _DT[self.dtype][1], x.numel(), ar, ai, nat.ptr(x), br, bi,
+                 nat.ptr(y) if y is not None else None, nat.ptr(out), nat.stream_handle(self.device))
+        return out
+
+    def __rmul__(self, scalar):
+        """scalar * H (hmatrix.py:553-559): dense blocks and left factors scaled."""
+        if not isinstance(scalar, (int, float, complex, np.number)):
+            return NotImplemented
+        c = self._copy()
+        if self._nval:
+            c._axpby(scalar, c._val)
+        if self._rsum:
+            c._axpby(scalar, c._U)
+        return c
+
+    def __mul__(self, other):
+        if isinstance(other, (int, float, complex, np.number)):
+            return self.__rmul__(other)
+        if isinstance(other, HMatrix):
+            return self._mtimes_hmat(other)
+        return self.mtimes_vec(other)
+
+    def __neg__(self):
+        return self.__rmul__(-1.0)
+
+    def __add__(self, other):
+        if not isinstance(other, HMatrix):
+            raise TypeError("HMatrix: only an HMatrix can be added")
+        return self._plus_hmat(other)
+
+    def __sub__(self, other):
+        return self.__add__(-other)
+
+    def _plus_hmat(self, other):
+        """Sum of two H-matrices on one block partition (hmatrix.py:592-670):
+        dense blocks added, low-rank factors stacked [U1 U2], [V1 V2], then
+        recompressed with truncate()."""
+        self._check()
+        other._check()
+        a, b = self._blk_host, other._blk_host
+        if a.shape != b.shape or not np.array_equal(a[:, :4], b[:, :4]):
+            raise ValueError("HMatrix: the operands have different block partitions")
+        if self.dtype != other.dtype:
+            raise ValueError("HMatrix: the operands have different storage types")
+        c = self._copy()
+        if self._nval:
+            c._axpby(1.0, self._val, 1.0, other._val, out=c._val)
+        nd = self._nd
+        if len(a) > nd:
+            new = a.copy()
+            new[nd:, 4] = np.maximum(a[nd:, 4], 0) + np.maximum(b[nd:, 4], 0)
+            Un, Vn = c._factor_bufs(new)
+            c._set_lowrank(new, Un, Vn)
+            lr = np.nonzero(new[nd:, 4] > 0)[0] + nd
+            if len(lr):
+                bids = torch.as_tensor(lr.astype(np.int32)).to(self.device)
+                nat.call("mnpbem_hmat_concat", _DT[self.dtype][1], len(lr), nat.ptr(self._blk), nat.ptr(other._blk),
+                         nat.ptr(c._blk), nat.ptr(bids), nat.ptr(self._U), nat.ptr(self._V), nat.ptr(other._U),
+                         nat.ptr(other._V), nat.ptr(Un), nat.ptr(Vn), nat.stream_handle(self.device))
+        return c.truncate()
+
+    def _diag_bids(self):
+        blk = self._blk_host[:self._nd]
+        ids = np.nonzero((blk[:, 0] == blk[:, 2]) & (blk[:, 1] == blk[:, 3]))[0]
+        return torch.as_tensor(ids.astype(np.int32)).to(self.device), len(ids)
+
+    def diag(self, xp=None):
+        """Diagonal in face order (hmatrix.py:758-778), from the diagonal dense
+        blocks: a complex128 device tensor, or numpy when xp is numpy."""
+        self._check()
+        n = self.tree.n
+        out = torch.zeros((n,), dtype=torch.complex128, device=self.device)
+        bids, nb = self._diag_bids()
+        nat.call("mnpbem_hmat_diag", _DT[self.dtype][1], nb, nat.ptr(self._blk), nat.ptr(bids), nat.ptr(self._perm),
+                 nat.ptr(self._val), nat.ptr(out), 0, nat.stream_handle(self.device))
+        return out.cpu().numpy() if xp is np else out
+
+    def pad(self):
+        """Zero blocks for an H-matrix whose blocks were never filled
+        (hmatrix.py:800-832): dense blocks zero, low-rank blocks zero rank-1."""
+        if self._built:
+            return self
+        self._tables()
+        blk = self._blk_host
+        nd = self._nd
+        tdt = _DT[self.dtype][0]
+        self._val = torch.zeros((max(self._nval, 1),), dtype=tdt, device=self.device)
+        blk[nd:, 4] = 1
+        Un, Vn = self._factor_bufs(blk)
+        Un.zero_()
+        Vn.zero_()
+        self._set_lowrank(blk, Un, Vn)
+        self._built = True
+        return self
+
+    def eye_hmat(self):
+        """Identity on this block partition (hmatrix.py:780-798): diagonal
+        dense blocks are identities, everything else zero."""
+        c = self._empty_like()
+        c.pad()
+        bids, nb = c._diag_bids()
+        nat.call("mnpbem_hmat_diag", _DT[c.dtype][1], nb, nat.ptr(c._blk), nat.ptr(bids), nat.ptr(c._perm),
+                 nat.ptr(c._val), None, 1, nat.stream_handle(c.device))
+        return c
+
+    def fillval(self, fun):
+        """Refill the dense blocks from fun (hmatrix.py:834-852): an (n, n)
+        matrix (device or host) or a callable fun(row, col) in face indices."""
+        self._check()
+        n = self.tree.n
+        dev = self.device
+        if callable(fun) and not isinstance(fun, (torch.Tensor, np.ndarray)):
+            blk = self._blk_host[:self._nd]
+            perm = self.tree.ind[:, 0]
+            host = np.empty(max(self._nval, 1), dtype=np.complex128)
+            for b in blk:
+                rows = perm[b[0]:b[0] + b[1]]
+                cols = perm[b[2]:b[2] + b[3]]
+                cc, rr = np.meshgrid(cols, rows, indexing="ij")  # column-major block values
+                host[b[5]:b[5] + b[1] * b[3]] = np.asarray(fun(rr.ravel(), cc.ravel()))
+            self._val = torch.as_tensor(host).to(dev, _DT[self.dtype][0])
+            self._lu = None
+            return self
+        src = torch.as_tensor(fun).to(dev, torch.complex128) if isinstance(fun, np.ndarray) else fun.to(torch.complex128)
+        if tuple(src.shape) != (n, n):
+            raise ValueError("HMatrix.fillval: expected an (n, n) matrix")
+        si, sj = src.stride()
+        dblk = torch.as_tensor(self._blk_host).to(dev)
+        bids = torch.arange(self._nd, dtype=torch.int32, device=dev)
+        nat.call("mnpbem_hmat_dense", _DT[self.dtype][1], self._nd, nat.ptr(dblk), nat.ptr(bids), nat.ptr(self._perm),
+                 nat.ptr(src), si, sj, nat.ptr(self._val), nat.stream_handle(dev))
+        self._lu = None
+        return self
+
+    # ------------------------------------------------------------ dense solves
+    def _lu_dense(self, A):
+        n = A.shape[0]
+        dev = self.device
+        piv = torch.empty((n,), dtype=torch.int32, device=dev)
+        info = torch.empty((1,), dtype=torch.int32, device=dev)
+        nat.call("mnpbem_getrf_batched", nat.DTYPE_C128, n, nat.ptr(A), n, n * n, nat.ptr(piv), n, nat.ptr(info), 1,
+                 nat.stream_handle(dev))
+        if int(info.cpu()[0]) != 0:
+            raise np.linalg.LinAlgError("HMatrix: singular matrix (zero pivot in column %d)" % int(info.cpu()[0]))
+        return A, piv
+
+    def _getrs(self, lu, piv, B):
+        n = lu.shape[0]
+        nrhs = B.shape[1]
+        nat.call("mnpbem_getrs_batched", nat.DTYPE_C128, n, nrhs, nat.ptr(lu), n, n * n, nat.ptr(piv), n,
+                 nat.ptr(B), nrhs, n * nrhs, 1, nat.stream_handle(self.device))
+        return B
+
+    def lu(self, method="auto"):
+        """LU factorisation for solve() (hmatrix.py:854-918).  The reference's
+        'dense' and 'hierarchical' (2 x 2 Schur complement of the densified
+        matrix) methods give the same solution up to rounding; both run here as
+        the dense LU with partial pivoting on the GPU."""
+        if method not in ("auto", "dense", "hierarchical"):
+            raise ValueError("HMatrix.lu: method must be 'auto', 'dense' or 'hierarchical'")
+        self._lu = self._lu_dense(self.full())
+        self._lu_method = "dense"
+        return self
+
+    def solve(self, b):
+        """x = A^-1 b for b (n,) or (n, k), numpy or torch (hmatrix.py:920-945);
+        factorises on first use."""
+        if getattr(self, "_lu", None) is None:
+            self.lu()
+        as_np = isinstance(b, np.ndarray)
+        x = torch.as_tensor(b) if as_np else b
+        one = x.dim() == 1
+        X = x.reshape(x.shape[0], -1).to(self.device, torch.complex128).contiguous().clone()
+        self._getrs(*self._lu, X)
+        X = X[:, 0] if one else X
+        return X.cpu().numpy() if as_np else X.to(x.device)
+
+    def _empty_like(self):
+        """Unfilled H-matrix with this tree, partition and settings."""
+        c = HMatrix(None, htol=self.htol, kmax=self.kmax, dtype=self.dtype, device=self.device)
+        c.tree = self.tree
+        c.row1, c.col1, c.row2, c.col2 = self.row1, self.col1, self.row2, self.col2
+        return c
+
+    def _from_dense(self, M):
+        """H-matrix on this partition compressed from the (n, n) device matrix M."""
+        return self._empty_like().aca(M)
+
+    def _dense_inv(self):
+        lu, piv = self._lu_dense(self.full())
+        n = self.tree.n
+        E = torch.empty((n, n), dtype=torch.complex128, device=self.device)
+        nat.call("mnpbem_identity_batched", nat.DTYPE_C128, n, nat.ptr(E), n, n * n, 1, nat.stream_handle(self.device))
+        return self._getrs(lu, piv, E)
+
+    def _gemm(self, A, B):
+        """A @ B for (n, n) complex128 device matrices, on the matrix-core GEMM."""
+        n = A.shape[0]
+        C = torch.zeros((n, n), dtype=torch.complex128, device=self.device)
+        negA = torch.empty_like(A)
+        self._axpby_c128(-1.0, A, negA)
+        nat.call("mnpbem_cgemm_sub_batched", nat.DTYPE_C128, n, n, n, nat.ptr(negA), n, n * n, nat.ptr(B), n, n * n,
+                 nat.ptr(C), n, n * n, 1, nat.stream_handle(self.device))
+        return C
+
+    def _axpby_c128(self, a, x, out):
+        nat.call("mnpbem_hmat_axpby", nat.DTYPE_C128, x.numel(), float(np.real(a)), float(np.imag(a)), nat.ptr(x),
+                 0.0, 0.0, None, nat.ptr(out), nat.stream_handle(self.device))
+
+    def _mtimes_hmat(self, other):
+        """H @ H (hmatrix.py:947-965): product of the densified operands,
+        compressed again by ACA on this partition."""
+        return self._from_dense(self._gemm(self.full(), other.full()))
+
+    def inv(self):
+        """Inverse (hmatrix.py:982-996): dense LU inverse, compressed by ACA."""
+        return self._from_dense(self._dense_inv())
+
+    def lsolve(self, lu_hmat, key="N"):
+        """X = A^-1 B with B = self, A = lu_hmat (hmatrix.py:998-1025), key 'N'."""
+        if key != "N":
+            raise NotImplementedError("HMatrix.lsolve: only key 'N' (full solve) is provided")
+        lu, piv = lu_hmat._lu_dense(lu_hmat.full())
+        return self._from_dense(self._getrs(lu, piv, self.full()))
+
+    def rsolve(self, lu_hmat, key="N"):
+        """X = B A^-1 with B = self, A = lu_hmat (hmatrix.py:1027-1056), key 'N'."""
+        if key != "N":
+            raise NotImplementedError("HMatrix.rsolve: only key 'N' (full solve) is provided")
+        return self._from_dense(self._gemm(self.full(), lu_hmat._dense_inv()))
 
 
 class ACACompGreenRet:

@@ -60,3 +60,15 @@ def test_invalid_arguments_are_rejected_without_device_work():
         _native.call("mnpbem_hmat_matvec", 2, 1, 1, 1, *([None] * 7), 17, None, 17, None, 17, 0, None, None)
     with pytest.raises(RuntimeError, match="order"):
         _native.call("mnpbem_refine_coeffs", 9, 0, *([None] * 10), 28, None, 144, None, None, None, None)
+    # recompression rank limits and tolerance
+    with pytest.raises(RuntimeError, match="above 128"):
+        _native.call("mnpbem_hmat_truncate", 2, 1, None, None, None, None, None, 129, 1e-6, None, None, None, None,
+                     None)
+    with pytest.raises(RuntimeError, match="XZ workspace"):
+        _native.call("mnpbem_hmat_truncate", 2, 1, None, None, None, None, None, 80, 1e-6, None, None, None, None,
+                     None)
+    with pytest.raises(RuntimeError, match="htol"):
+        _native.call("mnpbem_hmat_truncate", 2, 1, None, None, None, None, None, 8, 1.5, None, None, None, None,
+                     None)
+    with pytest.raises(RuntimeError, match="mode"):
+        _native.call("mnpbem_hmat_diag", 2, 1, None, None, None, None, None, 3, None)

@@ -192,3 +192,19 @@ def test_product_block_tables_partition():
         ids = set(lst[f:l].tolist())
         want = {i for i, b in enumerate(blk) if b[0] <= r0 and r0 + m <= b[0] + b[1]}
         assert ids == want
+
+
+def test_truncate_of_stacked_factors():
+    """The sum path stacks factors ([U U], [V V] for B + B); its
+    recompression must see the spectrum of 2 B: same rank as truncating B,
+    product 2 B_r (hmatrix.py:592-670 then :688-737)."""
+    rng = np.random.RandomState(8)
+    m, n, r = 70, 50, 15
+    s = 10.0 ** -np.linspace(0, 9, r)
+    U = np.linalg.qr(rng.randn(m, r) + 1j * rng.randn(m, r))[0] * s
+    V = rng.randn(n, r) + 1j * rng.randn(n, r)
+    for htol in (1e-3, 1e-7):
+        u1, v1 = truncate_block(U, V, htol)
+        u2, v2 = truncate_block(np.hstack([U, U]), np.hstack([V, V]), htol)
+        assert u2.shape[1] == u1.shape[1]
+        assert np.linalg.norm(u2 @ v2.T - 2 * u1 @ v1.T) < 1e-12 * np.linalg.norm(U @ V.T)
