@@ -130,6 +130,9 @@ def main():
     N = cp.n
     pw = m.PlaneWaveRet(pol, dr)
     t0 = time.time()
+    gemm_tiles = ("autotuned table %s" % os.path.relpath(nat.TUNING_PATH, ROOT)
+                  if os.path.exists(nat.TUNING_PATH) and os.environ.get("MNPBEM_GEMM_TUNING", "") != "default"
+                  else "built-in default table")
     bem = m.BEMRet(cp, precision=args.precision, refine_tol=args.refine_tol)
     torch.cuda.synchronize()
     log("setup (mesh + GPU refinement coefficients): %.2f s, N = %d" % (time.time() - t0, N))
@@ -224,7 +227,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "Drude-gold sphere d=40nm, trisphere(1444) = %d curved faces, 64 wavelengths "
                                "400-800 nm, 2 plane waves (x,y pol, +z), mixed precision" % N,
-                   "faces": N, "wavelengths": len(lams), "rhs": 2, "chunk": args.chunk},
+                   "faces": N, "wavelengths": len(lams), "rhs": 2, "chunk": args.chunk,
+                   "gemm_tiles": gemm_tiles},
         "fp64_value": None if fp64 is None else round(fp64, 4),
         "speedup_vs_fp64": None if fp64 is None else round(value / fp64, 3),
         "max_rel_ext_err_vs_fp64_gpu": acc_vs_fp64,

@@ -47,6 +47,20 @@ int mnpbem_device_sync(void);
 int mnpbem_cgemm_sub_batched(int dtype, int M, int N, int K, const void* A, long lda, long sA, const void* B,
                              long ldb, long sB, void* C, long ldc, long sC, int batch, void* stream);
 
+/* Tile selection of the complex64 bf16x3 GEMM (no counterpart in the
+ * reference, whose LU runs in LAPACK / cuSOLVER; this is the tile autotuner
+ * of the mixed-precision path).  A 7 x 7 x 7 table over shape classes of
+ * (M, N, K) (class c: x <= 64 * 2^c, class 6 unbounded; cell (cm*7+cn)*7+ck)
+ * picks variant 0 (128 x 128 tile) or 1 (64 x 64 tile) per launch; both give
+ * identical results.  _variant forces a variant (complex64 data). */
+int mnpbem_cgemm_sub_variant(int variant, int M, int N, int K, const void* A, long lda, long sA, const void* B,
+                             long ldb, long sB, void* C, long ldc, long sC, int batch, void* stream);
+int mnpbem_gemm_tuning_set(const unsigned char* table, int ncells);
+int mnpbem_gemm_tuning_get(unsigned char* table, int ncells);
+/* shapes[4c..4c+3] = M, N, K, batch of one launch per class since the last
+ * clear (M = 0: none); clear != 0 resets the record. */
+int mnpbem_gemm_tuning_seen(int* shapes, int ncells, int clear);
+
 /* In-place LU with partial pivoting, P A = L U (largest |re|+|im| pivot,
  * LAPACK convention); info[b] = 1-based column of the first exactly-zero
  * pivot or 0.  Replaces lu_factor_dispatch (mnpbem/utils/gpu.py:145-191). */

@@ -12,6 +12,9 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmnpbem_amd.so")
+# GEMM tile table written by mnpbem_amd.autotune (applied at load when present;
+# MNPBEM_GEMM_TUNING=default keeps the library's built-in table)
+TUNING_PATH = os.path.join(_HERE, "data", "gemm_tuning_gfx950.json")
 
 DTYPE_C64 = 1
 DTYPE_C128 = 2
@@ -24,6 +27,10 @@ SIGNATURES = {
     "mnpbem_abi_version": [],
     "mnpbem_device_sync": [],
     "mnpbem_cgemm_sub_batched": [_i, _i, _i, _i, _p, _l, _l, _p, _l, _l, _p, _l, _l, _i, _p],
+    "mnpbem_cgemm_sub_variant": [_i, _i, _i, _i, _p, _l, _l, _p, _l, _l, _p, _l, _l, _i, _p],
+    "mnpbem_gemm_tuning_set": [_p, _i],
+    "mnpbem_gemm_tuning_get": [_p, _i],
+    "mnpbem_gemm_tuning_seen": [_p, _i, _i],
     "mnpbem_getrf_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _i, _p],
     "mnpbem_getrs_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _i, _p],
     "mnpbem_refine_coeffs": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _p],
@@ -88,8 +95,21 @@ def load(path=None):
     lib.mnpbem_last_error.argtypes = []
     lib.mnpbem_last_error.restype = ctypes.c_char_p
     if path == LIB_PATH:
+        _apply_tuning(lib)
         _lib = lib
     return lib
+
+
+def _apply_tuning(lib):
+    if os.environ.get("MNPBEM_GEMM_TUNING", "") == "default" or not os.path.exists(TUNING_PATH):
+        return
+    import json
+    with open(TUNING_PATH) as fh:
+        table = json.load(fh)["table"]
+    buf = (ctypes.c_ubyte * len(table))(*table)
+    if lib.mnpbem_gemm_tuning_set(buf, len(table)) != 0:
+        raise RuntimeError("mnpbem_amd: bad GEMM tuning table %s: %s"
+                           % (TUNING_PATH, lib.mnpbem_last_error().decode(errors="replace")))
 
 
 def call(name, *args):

@@ -159,16 +159,21 @@ __global__ __launch_bounds__(256) void k_cgemm_sub(int M, int N, int K, const T*
 // dropped), accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  A complex
 // product takes 12 such MFMAs.  Accuracy is ~1e-5 relative per product; the
 // mixed-precision BEM solve recovers FP64 accuracy by iterative refinement.
-// Workgroup: 512 threads = 8 waves, 128 x 128 output tile, each wave 64 x 32
-// (4 x 2 blocks of 16 x 16); K slices of 32 staged in LDS as four bf16
-// planes (re_hi, re_lo, im_hi, im_lo), B stored transposed so that every
-// MFMA operand is one 16-byte LDS read per lane.
+// Tile shapes (template BM x BN, waves of WM x 32): 128 x 128 with 8 waves of
+// 64 x 32 (4 x 2 blocks of 16 x 16, 128 KB LDS, one workgroup per CU) for
+// the bulk of the factorisation; 64 x 64 with 4 waves of 32 x 32 (64 KB LDS,
+// two workgroups per CU) for the thin products deep in the recursion, where
+// the per-workgroup latency dominates.  A shape-class table chooses between
+// them (GemmTuning below).  Both accumulate every output element over K in
+// the same order, so the choice does not change the result.  K slices of 32
+// are staged in LDS as four bf16 planes (re_hi, re_lo, im_hi, im_lo), B
+// stored transposed so that every MFMA operand is one 16-byte LDS read per
+// lane.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int XT = 128;  // output tile edge
 constexpr int XK = 32;   // K slice
 
 __device__ __forceinline__ bf16x8 bneg(bf16x8 v) {
@@ -198,22 +203,24 @@ __device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
   }
 }
 
-__global__ __launch_bounds__(512) void k_cgemm_sub_bf16x3(int M, int N, int K, const float* __restrict__ A, long lda,
+template <int BM, int BN, int WM>
+__global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3(int M, int N, int K, const float* __restrict__ A, long lda,
                                                           long sA, const float* __restrict__ B, long ldb, long sB,
                                                           float* __restrict__ C, long ldc, long sC) {
   const long bz = blockIdx.z;
   A += 2 * bz * sA;
   B += 2 * bz * sB;
   C += 2 * bz * sC;
-  const int m0 = blockIdx.y * XT, n0 = blockIdx.x * XT;
+  constexpr int WN = 32, FI = WM / 16, FJ = WN / 16, NWN = BN / WN;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   // two stages x four planes (re_hi, re_lo, im_hi, im_lo) x 128 rows x 32
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][4][XT * XK];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][4][XT * XK];
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][4][BM * XK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][4][BN * XK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 2) * 64, wn = (wave & 3) * 32;
+  const int wm = (wave / NWN) * WM, wn = (wave % NWN) * WN;
   // staging: each thread owns 8 consecutive k of one A row and of one B column
   const int la_r = tid >> 2, la_c = tid & 3;
-  const int lb_n = tid & 127, lb_c = tid >> 7;
+  const int lb_n = tid % BN, lb_c = tid / BN;
   float ra[16], rb[16];
   auto gload = [&](int k0) {
     const int gr = m0 + la_r;
@@ -263,11 +270,11 @@ __global__ __launch_bounds__(512) void k_cgemm_sub_bf16x3(int M, int N, int K, c
     *reinterpret_cast<bf16x8*>(&Bs[stg][2][ob]) = h;
     *reinterpret_cast<bf16x8*>(&Bs[stg][3][ob]) = l;
   };
-  f32x4 accR[4][2], accI[4][2];
+  f32x4 accR[FI][FJ], accI[FI][FJ];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < FJ; ++j) {
       accR[i][j] = f32x4{0, 0, 0, 0};
       accI[i][j] = f32x4{0, 0, 0, 0};
     }
@@ -279,15 +286,15 @@ __global__ __launch_bounds__(512) void k_cgemm_sub_bf16x3(int M, int N, int K, c
   for (int k0 = 0; k0 < K; k0 += XK) {
     const bool more = (k0 + XK) < K;
     if (more) gload(k0 + XK);
-    bf16x8 a[4][4];  // [row block][plane]
+    bf16x8 a[FI][4];  // [row block][plane]
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < FI; ++i) {
       const int o = xsw(wm + 16 * i + fr, fc);
 #pragma unroll
       for (int pl = 0; pl < 4; ++pl) a[i][pl] = *reinterpret_cast<const bf16x8*>(&As[stg][pl][o]);
     }
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < FJ; ++j) {
       const int o = xsw(wn + 16 * j + fr, fc);
       const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&Bs[stg][0][o]);
       const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&Bs[stg][1][o]);
@@ -295,29 +302,29 @@ __global__ __launch_bounds__(512) void k_cgemm_sub_bf16x3(int M, int N, int K, c
       const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&Bs[stg][3][o]);
       const bf16x8 nbih = bneg(bih), nbil = bneg(bil);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][0], brh, accR[i][j]);
+      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][0], brh, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][0], bih, accI[i][j]);
+      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][0], bih, accI[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][0], brl, accR[i][j]);
+      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][0], brl, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][0], bil, accI[i][j]);
+      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][0], bil, accI[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][1], brh, accR[i][j]);
+      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][1], brh, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][1], bih, accI[i][j]);
+      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][1], bih, accI[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][2], nbih, accR[i][j]);
+      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][2], nbih, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][2], brh, accI[i][j]);
+      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][2], brh, accI[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][2], nbil, accR[i][j]);
+      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][2], nbil, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][2], brl, accI[i][j]);
+      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][2], brl, accI[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accR[i][j] = mfma_bf16(a[i][3], nbih, accR[i][j]);
+      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][3], nbih, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) accI[i][j] = mfma_bf16(a[i][3], brh, accI[i][j]);
+      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][3], brh, accI[i][j]);
     }
     // the other stage is free: every wave passed the barrier after reading it
     if (more) sstore(stg ^ 1);
@@ -325,9 +332,9 @@ __global__ __launch_bounds__(512) void k_cgemm_sub_bf16x3(int M, int N, int K, c
     stg ^= 1;
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < FJ; ++j) {
       const int gc = n0 + wn + 16 * j + fr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -593,6 +600,48 @@ struct GemmProf {
 };
 GemmProf g_prof;
 
+// Tile choice of the bf16x3 GEMM per shape class: class of a dimension x is
+// 0 for x <= 64, 1 for <= 128, ... 5 for <= 2048, 6 above; cell (cm, cn, ck)
+// holds the variant (0: 128 x 128, 1: 64 x 64).  The library starts from the
+// measured default (64 x 64 for the products with min(M, N) <= 64 and
+// K <= 64) and takes a table set by the autotuner.  It also records, per
+// cell, one shape that was launched, so the autotuner can time the shapes a
+// workload actually uses.
+constexpr int GC = 7;
+struct GemmTuning {
+  unsigned char var[GC * GC * GC];
+  int seen[GC * GC * GC][4];  // M, N, K, batch of one launch (M = 0: none)
+  GemmTuning() {
+    for (int cm = 0; cm < GC; ++cm)
+      for (int cn = 0; cn < GC; ++cn)
+        for (int ck = 0; ck < GC; ++ck) var[(cm * GC + cn) * GC + ck] = ((cm == 0 || cn == 0) && ck == 0) ? 1 : 0;
+    for (auto& q : seen) q[0] = q[1] = q[2] = q[3] = 0;
+  }
+};
+GemmTuning g_tune;
+
+inline int gclass(int x) {
+  int c = 0;
+  for (int t = 64; x > t && c < GC - 1; t <<= 1) ++c;
+  return c;
+}
+inline int gcell(int M, int N, int K) { return (gclass(M) * GC + gclass(N)) * GC + gclass(K); }
+
+int launch_bf16x3(int variant, int M, int N, int K, const float* A, long lda, long sA, const float* B, long ldb,
+                  long sB, float* C, long ldc, long sC, int batch, hipStream_t st) {
+  if (variant == 1) {
+    dim3 grid(mnp_ceil_div(N, 64), mnp_ceil_div(M, 64), batch);
+    hipLaunchKernelGGL((k_cgemm_sub_bf16x3<64, 64, 32>), grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB,
+                       C, ldc, sC);
+  } else {
+    dim3 grid(mnp_ceil_div(N, 128), mnp_ceil_div(M, 128), batch);
+    hipLaunchKernelGGL((k_cgemm_sub_bf16x3<128, 128, 64>), grid, dim3(512), 0, st, M, N, K, A, lda, sA, B, ldb, sB,
+                       C, ldc, sC);
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
 // gmode 1 (complex64 only): bf16 split-product GEMM
 template <typename T>
 int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, long ldb, long sB, T* C, long ldc,
@@ -602,8 +651,15 @@ int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, lon
   if (timed) MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used], st));
   if constexpr (sizeof(T) == 4) {
     if (gmode == 1) {
-      dim3 grid(mnp_ceil_div(N, XT), mnp_ceil_div(M, XT), batch);
-      hipLaunchKernelGGL(k_cgemm_sub_bf16x3, grid, dim3(512), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+      const int cell = gcell(M, N, K);
+      if (g_tune.seen[cell][0] == 0) {
+        g_tune.seen[cell][0] = M;
+        g_tune.seen[cell][1] = N;
+        g_tune.seen[cell][2] = K;
+        g_tune.seen[cell][3] = batch;
+      }
+      int rc = launch_bf16x3(g_tune.var[cell], M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, st);
+      if (rc) return rc;
     } else {
       dim3 grid(mnp_ceil_div(N, GT), mnp_ceil_div(M, GT), batch);
       hipLaunchKernelGGL(k_cgemm_sub<T>, grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
@@ -789,6 +845,45 @@ extern "C" int mnpbem_cgemm_sub_batched(int dtype, int M, int N, int K, const vo
                            st, dtype == MNP_DTYPE_C64X ? 1 : 0);
   mnp_set_error("cgemm: unknown dtype");
   return 1;
+}
+
+// C[b] -= A[b] B[b] for complex64 data on the bf16x3 kernel with a given tile
+// variant (0: 128 x 128, 1: 64 x 64), bypassing the shape-class table: the
+// autotuner times the variants with it.
+extern "C" int mnpbem_cgemm_sub_variant(int variant, int M, int N, int K, const void* A, long lda, long sA,
+                                        const void* B, long ldb, long sB, void* C, long ldc, long sC, int batch,
+                                        void* stream) {
+  MNP_REQUIRE(variant == 0 || variant == 1, "cgemm_variant: variant must be 0 or 1");
+  MNP_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "cgemm_variant: negative size");
+  if (M == 0 || N == 0 || K == 0 || batch == 0) return 0;
+  return launch_bf16x3(variant, M, N, K, (const float*)A, lda, sA, (const float*)B, ldb, sB, (float*)C, ldc, sC,
+                       batch, (hipStream_t)stream);
+}
+
+// Shape-class table of the bf16x3 GEMM (7 x 7 x 7 cells, index (cm * 7 + cn)
+// * 7 + ck, class c of a dimension x: x <= 64 * 2^c, last class unbounded).
+// set: table[343] of variants (0 or 1); get: copies it out.
+extern "C" int mnpbem_gemm_tuning_set(const unsigned char* table, int ncells) {
+  MNP_REQUIRE(ncells == GC * GC * GC && table != nullptr, "gemm_tuning: expected 343 cells");
+  for (int c = 0; c < ncells; ++c) MNP_REQUIRE(table[c] <= 1, "gemm_tuning: variants are 0 or 1");
+  for (int c = 0; c < ncells; ++c) g_tune.var[c] = table[c];
+  return 0;
+}
+extern "C" int mnpbem_gemm_tuning_get(unsigned char* table, int ncells) {
+  MNP_REQUIRE(ncells == GC * GC * GC && table != nullptr, "gemm_tuning: expected 343 cells");
+  for (int c = 0; c < ncells; ++c) table[c] = g_tune.var[c];
+  return 0;
+}
+// One launched shape per cell: shapes[4 c .. 4 c + 3] = M, N, K, batch (M = 0:
+// no launch in that class since the last clear); clear != 0 resets the record.
+extern "C" int mnpbem_gemm_tuning_seen(int* shapes, int ncells, int clear) {
+  MNP_REQUIRE(ncells == GC * GC * GC, "gemm_tuning: expected 343 cells");
+  for (int c = 0; c < ncells; ++c)
+    for (int q = 0; q < 4; ++q) {
+      if (shapes) shapes[4 * c + q] = g_tune.seen[c][q];
+      if (clear) g_tune.seen[c][q] = 0;
+    }
+  return 0;
 }
 
 extern "C" int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA, int* ipiv, long sP, int* info,

@@ -3,6 +3,8 @@ function declared in include/mnpbem_amd.h, and the ctypes table covers them.
 No device compute is called here."""
 
 import os
+
+import numpy as np
 import re
 
 import pytest
@@ -72,3 +74,28 @@ def test_invalid_arguments_are_rejected_without_device_work():
                      None)
     with pytest.raises(RuntimeError, match="mode"):
         _native.call("mnpbem_hmat_diag", 2, 1, None, None, None, None, None, 3, None)
+
+
+def test_gemm_tuning_table_roundtrip():
+    """The bf16x3 tile table is host state: set / get round-trip, bad tables
+    are refused, and the class of a shape follows x <= 64 * 2^c."""
+    from mnpbem_amd import _native
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("libmnpbem_amd.so not built")
+    from mnpbem_amd import autotune as at
+    saved = at.get_table()
+    try:
+        t = np.zeros(at.NCELL, dtype=np.uint8)
+        t[at.cell_of(64, 3000, 64)] = 1
+        t[at.cell_of(65, 129, 4096)] = 1
+        at.set_table(t)
+        assert np.array_equal(at.get_table(), t)
+        assert at.cell_of(64, 64, 64) == 0 and at.cell_of(65, 64, 64) == 49 and at.cell_of(10 ** 6, 1, 1) == 6 * 49
+        with pytest.raises(RuntimeError, match="0 or 1"):
+            at.set_table(np.full(at.NCELL, 2, dtype=np.uint8))
+        with pytest.raises(ValueError):
+            at.set_table(np.zeros(10, dtype=np.uint8))
+        with pytest.raises(RuntimeError, match="variant"):
+            _native.call("mnpbem_cgemm_sub_variant", 2, 1, 1, 1, None, 1, 1, None, 1, 1, None, 1, 1, 1, None)
+    finally:
+        at.set_table(saved)

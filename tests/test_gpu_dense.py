@@ -41,6 +41,69 @@ def test_cgemm_sub(cuda, dtype, tol, split, M, N, K):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("M,N,K", [(64, 300, 64), (130, 129, 200), (17, 64, 33), (200, 70, 96), (1, 5, 1)])
+def test_bf16x3_tile_variants_identical(cuda, M, N, K):
+    """The two tile variants of the bf16x3 GEMM (and the table-routed call)
+    give bit-identical results: each output element is accumulated over K in
+    the same order.  Both agree with numpy to the split-product accuracy."""
+    nat = _lib()
+    from mnpbem_amd import autotune as at
+    rng = np.random.default_rng(M + 7 * N + 13 * K)
+    B = 2
+    A = _rand(rng, (B, M, K), np.complex64)
+    Bm = _rand(rng, (B, K, N), np.complex64)
+    C = _rand(rng, (B, M, N), np.complex64)
+    ref = C.astype(np.complex128) - A.astype(np.complex128) @ Bm.astype(np.complex128)
+    tA, tB = torch.from_numpy(A).to(cuda), torch.from_numpy(Bm).to(cuda)
+    st = nat.stream_handle(cuda)
+    print("bf16x3 variants", M, N, K, flush=True)
+    outs = []
+    for v in (0, 1):
+        tC = torch.from_numpy(C).to(cuda)
+        nat.call("mnpbem_cgemm_sub_variant", v, M, N, K, nat.ptr(tA), K, M * K, nat.ptr(tB), N, K * N, nat.ptr(tC),
+                 N, M * N, B, st)
+        outs.append(tC.cpu().numpy())
+    saved = at.get_table()
+    try:
+        for v in (0, 1):
+            at.set_table(np.full(at.NCELL, v, dtype=np.uint8))
+            tC = torch.from_numpy(C).to(cuda)
+            nat.call("mnpbem_cgemm_sub_batched", nat.DTYPE_C64X, M, N, K, nat.ptr(tA), K, M * K, nat.ptr(tB), N,
+                     K * N, nat.ptr(tC), N, M * N, B, st)
+            outs.append(tC.cpu().numpy())
+    finally:
+        at.set_table(saved)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    assert np.abs(outs[0] - ref).max() / np.abs(ref).max() < 1e-4
+
+
+def test_autotune_records_and_installs(cuda):
+    """tune() times both variants on the recorded shapes and installs the
+    faster one per class; the recorded shape is the launched one."""
+    nat = _lib()
+    from mnpbem_amd import autotune as at
+    M, N, K, B = 64, 320, 64, 4
+    tA = torch.randn((B, M, K), dtype=torch.complex64, device=cuda)
+    tB = torch.randn((B, K, N), dtype=torch.complex64, device=cuda)
+    tC = torch.zeros((B, M, N), dtype=torch.complex64, device=cuda)
+    st = nat.stream_handle(cuda)
+
+    def run():
+        nat.call("mnpbem_cgemm_sub_batched", nat.DTYPE_C64X, M, N, K, nat.ptr(tA), K, M * K, nat.ptr(tB), N, K * N,
+                 nat.ptr(tC), N, M * N, B, st)
+
+    saved = at.get_table()
+    try:
+        table, report = at.tune(run, reps=1, verbose=False)
+        assert [r[1] for r in report] == [(M, N, K, B)]
+        cell, _, t, choice = report[0]
+        assert cell == at.cell_of(M, N, K) and choice == int(np.argmin(t)) and all(x > 0 for x in t)
+        assert at.get_table()[cell] == choice
+    finally:
+        at.set_table(saved)
+
+
 @pytest.mark.parametrize("dtype,tol,split", [(np.complex128, 1e-12, False), (np.complex64, 1e-4, False),
                                              (np.complex64, 1e-3, True)])
 @pytest.mark.parametrize("n", [1, 7, 64, 100, 300, 1031])
