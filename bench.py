@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "mixed-fp32"],
                     help="mixed: complex64 GEMMs on bf16 matrix cores (3 split products); "
                          "mixed-fp32: on FP32 matrix cores")
+    ap.add_argument("--residual", choices=["auto", "stored", "matrix_free"], default="auto",
+                    help="FP64 refinement residual of the mixed path: complex128 copies of G/H or entries "
+                         "recomputed on the fly")
     ap.add_argument("--refine-tol", type=float, default=1e-12,
                     help="FP64 refinement stops when ||R - A X|| <= tol ||R|| for every wavelength")
     ap.add_argument("--workload", default="sphere", choices=["sphere", "uneven", "dimer"],
@@ -133,7 +136,7 @@ def main():
     gemm_tiles = ("autotuned table %s" % os.path.relpath(nat.TUNING_PATH, ROOT)
                   if os.path.exists(nat.TUNING_PATH) and os.environ.get("MNPBEM_GEMM_TUNING", "") != "default"
                   else "built-in default table")
-    bem = m.BEMRet(cp, precision=args.precision, refine_tol=args.refine_tol)
+    bem = m.BEMRet(cp, precision=args.precision, refine_tol=args.refine_tol, residual=args.residual)
     torch.cuda.synchronize()
     log("setup (mesh + GPU refinement coefficients): %.2f s, N = %d" % (time.time() - t0, N))
 
@@ -228,7 +231,7 @@ def main():
         "config": {"workload": "Drude-gold sphere d=40nm, trisphere(1444) = %d curved faces, 64 wavelengths "
                                "400-800 nm, 2 plane waves (x,y pol, +z), mixed precision" % N,
                    "faces": N, "wavelengths": len(lams), "rhs": 2, "chunk": args.chunk,
-                   "gemm_tiles": gemm_tiles},
+                   "gemm_tiles": gemm_tiles, "residual": args.residual},
         "fp64_value": None if fp64 is None else round(fp64, 4),
         "speedup_vs_fp64": None if fp64 is None else round(value / fp64, 3),
         "max_rel_ext_err_vs_fp64_gpu": acc_vs_fp64,

@@ -201,6 +201,17 @@ int mnpbem_surface_field(int n, int P, int side, double hsign, const double* pos
                          const void* X, long ldx, long sx, void* E, void* H, long lde, long se, int batch,
                          void* stream);
 
+/* Matrix-free Green products of one side for the FP64 residual of the mixed
+ * solve (replaces the four complex128 matvecs of the BEM operator,
+ * bem_ret_iter.py:1362-1501 afun, with the entries of CompGreenRet.eval
+ * recomputed instead of stored): Y[:, yg + c] = (G X)[:, xoff + c],
+ * Y[:, yh + c] = (H X)[:, xoff + c], c < cols <= 16, complex128; mat, kmat,
+ * rowptr, pcol, g, f as for mnpbem_surface_field; hsign +1 (H1) / -1 (H2). */
+int mnpbem_green_apply(int n, int cols, double hsign, const double* pos, const double* nvec, const double* area,
+                       const int* mat, const double* kmat, int nmat, int order, const long* rowptr,
+                       const int* pcol, const double* g, const double* f, const void* X, long ldx, long sx,
+                       int xoff, void* Y, long ldy, long sy, int yg, int yh, int batch, void* stream);
+
 /* ---- permittivity derivatives (the gradient path; no counterpart in the
  *      reference, which has no derivatives) ----
  * d/dk of the Green matrices of one side (dG = i a e^{ikd}, dF = -k (n.r) a

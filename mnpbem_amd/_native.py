@@ -69,6 +69,8 @@ SIGNATURES = {
     "mnpbem_refine_classify": [_i, _p, _p, _d, _d, _i, _p, _p, _p, _p, _p, _p],
     "mnpbem_surface_field": [_i, _i, _i, _d, _p, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _p, _p, _l, _l, _p, _p,
                              _l, _l, _i, _p],
+    "mnpbem_green_apply": [_i, _i, _d, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p, _p, _l, _l, _i, _p, _l, _l, _i,
+                           _i, _i, _p],
     "mnpbem_green_dk": [_i, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p, _p, _l, _l, _i, _p],
     "mnpbem_deps_rhs": [_i, _i, _p, _l, _l, _p, _l, _l, _p, _p, _p, _i, _p],
     "mnpbem_prof_begin": [_i],

@@ -77,11 +77,18 @@ class BEMRet:
     needs = {"sim": "ret"}
 
     def __init__(self, p, enei=None, precision="double", green=None, refine_tol=1e-12, refine_maxit=8,
-                 **options):
+                 residual="auto", **options):
         if precision not in ("double", "mixed", "mixed-fp32"):
             raise ValueError("BEMRet: precision must be 'double', 'mixed' or 'mixed-fp32'")
+        if residual not in ("auto", "matrix_free", "stored"):
+            raise ValueError("BEMRet: residual must be 'auto', 'matrix_free' or 'stored'")
         self.p = p
         self.precision = precision
+        # FP64 residual of the mixed path: complex128 copies kept from the
+        # assembly ('stored', faster: 4 N^2 x 16 B per wavelength) or Green
+        # entries recomputed on the fly ('matrix_free', no copies); 'auto'
+        # stores when the copies take under 40 % of the free device memory
+        self.residual = residual
         self.refine_tol = refine_tol
         self.refine_maxit = refine_maxit
         self.g = green if green is not None else CompGreenRet(p, p, **options)
@@ -127,11 +134,19 @@ class BEMRet:
         # GG = [G1^T ; G2^T], HH = [H1^T ; H2^T] stacked so both sides factor in one batch
         GG = torch.empty((2, B, n, n), dtype=cdt, device=dev)
         HH = torch.empty((2, B, n, n), dtype=cdt, device=dev)
-        if mixed:
+        F.kmat = kmat
+        F.G64 = F.H64 = None
+        store = self.residual == "stored"
+        if mixed and self.residual == "auto":
+            store = 4 * B * nn * 16 < 0.4 * torch.cuda.mem_get_info(dev)[0]
+        if mixed and store:
             F.G64 = torch.empty((2, B, n, n), dtype=torch.complex128, device=dev)
             F.H64 = torch.empty((2, B, n, n), dtype=torch.complex128, device=dev)
             for s, hs in ((0, 1.0), (1, -1.0)):
                 g.assemble(s, kmat, F.G64[s], F.H64[s], hs, nat.DTYPE_C128, GG[s], HH[s])
+        elif mixed:
+            for s, hs in ((0, 1.0), (1, -1.0)):
+                g.assemble(s, kmat, GG[s], HH[s], hs, nat.DTYPE_C64)
         else:
             for s, hs in ((0, 1.0), (1, -1.0)):
                 g.assemble(s, kmat, GG[s], HH[s], hs, nat.DTYPE_C128)
@@ -204,10 +219,7 @@ class BEMRet:
             self._solve_direct(F, R32, X32, W32, P)
             nat.call("mnpbem_accumulate", nat.DTYPE_C64, cnt, nat.ptr(X32), nat.ptr(X), st)
             it += 1
-            nat.call("mnpbem_bem_residual_batched", n, P, nat.ptr(F.G64[0]), nat.ptr(F.H64[0]),
-                     nat.ptr(F.G64[1]), nat.ptr(F.H64[1]), n, n * n, nat.ptr(R), nat.ptr(X), nat.ptr(Rres),
-                     8 * P, n * 8 * P, nat.ptr(Y), 16 * P, n * 16 * P, nat.ptr(self.g.nvec), nat.ptr(F.par),
-                     nat.ptr(nrm), B, st)
+            self._residual(F, R, X, Rres, Y, nrm, P)
             h = nrm.cpu().numpy()
             rel = np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300))
             if np.all(rel <= self.refine_tol) or it > self.refine_maxit:
@@ -216,6 +228,26 @@ class BEMRet:
         self.last_refine_iters = it
         self.last_residual = rel
         return X
+
+    def _residual(self, F, R, X, Rres, Y, nrm, P):
+        """Rres = R - A X in complex128 and the squared norms per wavelength;
+        Y = [G1 X1 | H1 X1 | G2 X2 | H2 X2] work space."""
+        g, n, B = self.g, F.n, F.B
+        st = nat.stream_handle(self.device)
+        if F.G64 is not None:
+            nat.call("mnpbem_bem_residual_batched", n, P, nat.ptr(F.G64[0]), nat.ptr(F.H64[0]),
+                     nat.ptr(F.G64[1]), nat.ptr(F.H64[1]), n, n * n, nat.ptr(R), nat.ptr(X), nat.ptr(Rres),
+                     8 * P, n * 8 * P, nat.ptr(Y), 16 * P, n * 16 * P, nat.ptr(g.nvec), nat.ptr(F.par),
+                     nat.ptr(nrm), B, st)
+            return
+        P4 = 4 * P
+        for s, hs in ((0, 1.0), (1, -1.0)):
+            nat.call("mnpbem_green_apply", n, P4, hs, nat.ptr(g.pos), nat.ptr(g.nvec), nat.ptr(g.area),
+                     nat.ptr(g.mat[s]), nat.ptr(F.kmat), g.nmat, g.order, nat.ptr(g._rowptr()), nat.ptr(g.pcol),
+                     nat.ptr(g.g), nat.ptr(g.f), nat.ptr(X), 8 * P, n * 8 * P, s * P4, nat.ptr(Y), 16 * P,
+                     n * 16 * P, 2 * s * P4, 2 * s * P4 + P4, B, st)
+        nat.call("mnpbem_bem_combine_batched", n, P, nat.ptr(Y), 16 * P, n * 16 * P, nat.ptr(R), nat.ptr(Rres),
+                 8 * P, n * 8 * P, nat.ptr(g.nvec), nat.ptr(F.par), nat.ptr(nrm), B, st)
 
     # ------------------------------------------------------------------
     def init(self, enei):

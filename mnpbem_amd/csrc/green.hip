@@ -761,3 +761,213 @@ extern "C" int mnpbem_green_dk(int n, const double* pos, const double* nvec, con
   return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Matrix-free Green products for the FP64 refinement residual of the mixed
+// path: Y[:, yg + c] = (G X)[:, c], Y[:, yh + c] = (H X)[:, c] for the
+// columns X[:, xoff + c], c < cols, of one side and every wavelength of the
+// batch.  The entries are recomputed with the assembly's formulas (same
+// operations in the same order as k_green_assemble / k_green_overlay) instead
+// of being read from stored complex128 matrices: 16 bytes per entry and per
+// matrix are traded for one sincos and one exp per pair.
+//   k_green_apply      plain entries of every pair j != i: thread per row i,
+//                      faces j staged through LDS in tiles of GA_T
+//   k_green_apply_ref  the refined pairs of a row (one wave per row, fixed
+//                      order wave reduction): refined - plain off the
+//                      diagonal, refined (+ hsign 2 pi for H) on it
+// ---------------------------------------------------------------------------
+constexpr int GA_T = 64;
+
+template <int C>
+__global__ __launch_bounds__(256) void k_green_apply(int n, int cols, const double* __restrict__ pos,
+                                                     const double* __restrict__ nvec, const double* __restrict__ area,
+                                                     const int* __restrict__ mat, const double* __restrict__ kmat,
+                                                     int nmat, const double* __restrict__ X, long ldx, long sx,
+                                                     int xoff, double* __restrict__ Y, long ldy, long sy, int yg,
+                                                     int yh) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  __shared__ double sp[GA_T][4];  // x, y, z, area
+  __shared__ int sm[GA_T];
+  __shared__ cplx<double> sxv[GA_T][C];
+  const bool act = i < n;
+  const int ii = act ? i : 0;
+  const double xi = pos[3 * ii], yi = pos[3 * ii + 1], zi = pos[3 * ii + 2];
+  const double nxi = nvec[3 * ii], nyi = nvec[3 * ii + 1], nzi = nvec[3 * ii + 2];
+  const int mi = mat[ii];
+  const double kr = kmat[2 * (b * nmat + mi)], ki = kmat[2 * (b * nmat + mi) + 1];
+  const double* xb = X + 2 * (b * sx + xoff);
+  cplx<double> ag[C], ah[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) ag[c] = ah[c] = cmake(0.0, 0.0);
+  for (int j0 = 0; j0 < n; j0 += GA_T) {
+    const int nj = min(GA_T, n - j0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nj; e += 256) {
+      const int j = j0 + e;
+      sp[e][0] = pos[3 * j];
+      sp[e][1] = pos[3 * j + 1];
+      sp[e][2] = pos[3 * j + 2];
+      sp[e][3] = area[j];
+      sm[e] = mat[j];
+    }
+    for (int e = threadIdx.x; e < nj * C; e += 256) {
+      const int jj = e / C, c = e % C;
+      sxv[jj][c] = c < cols ? cld(xb + 2 * ((long)(j0 + jj) * ldx + c)) : cmake(0.0, 0.0);
+    }
+    __syncthreads();
+    if (!act) continue;
+    for (int jj = 0; jj < nj; ++jj) {
+      if (j0 + jj == i || sm[jj] != mi) continue;
+      const double x = xi - sp[jj][0], y = yi - sp[jj][1], z = zi - sp[jj][2];
+      const double d = fmax(sqrt(x * x + y * y + z * z), 2.220446049250313e-16);
+      const double id = 1.0 / d, aj = sp[jj][3];
+      double s, c;
+      sincos(kr * d, &s, &c);
+      const double e = exp(-ki * d);
+      const double phr = e * c, phi = e * s;
+      const double ga = aj * id;
+      const cplx<double> G = cmake(ga * phr, ga * phi);
+      const double ndr = nxi * x + nyi * y + nzi * z;
+      const double sc = ndr * id * id * aj;
+      const double ar = (-ki - id) * sc, ai = kr * sc;
+      const cplx<double> H = cmake(ar * phr - ai * phi, ar * phi + ai * phr);
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        ag[q] = cfma(ag[q], G, sxv[jj][q]);
+        ah[q] = cfma(ah[q], H, sxv[jj][q]);
+      }
+    }
+  }
+  if (!act) return;
+  double* yb = Y + 2 * (b * sy + (long)i * ldy);
+  for (int c = 0; c < cols; ++c) {
+    yb[2 * (yg + c)] = ag[c].re;
+    yb[2 * (yg + c) + 1] = ag[c].im;
+    yb[2 * (yh + c)] = ah[c].re;
+    yb[2 * (yh + c) + 1] = ah[c].im;
+  }
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void k_green_apply_ref(
+    int n, int cols, double hsign, const double* __restrict__ pos, const double* __restrict__ nvec,
+    const double* __restrict__ area, const int* __restrict__ mat, const double* __restrict__ kmat, int nmat,
+    int ord1, const long* __restrict__ rowptr, const int* __restrict__ pcol, const double* __restrict__ g,
+    const double* __restrict__ f, const double* __restrict__ X, long ldx, long sx, int xoff, double* __restrict__ Y,
+    long ldy, long sy, int yg, int yh) {
+  const long b = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;  // whole wave
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  const double nx = nvec[3 * i], ny = nvec[3 * i + 1], nz = nvec[3 * i + 2];
+  const int mi = mat[i];
+  const double kr = kmat[2 * (b * nmat + mi)], ki = kmat[2 * (b * nmat + mi) + 1];
+  const double* xb = X + 2 * (b * sx + xoff);
+  cplx<double> ag[C], ah[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) ag[c] = ah[c] = cmake(0.0, 0.0);
+  for (long p = rowptr[i] + lane; p < rowptr[i + 1]; p += 64) {
+    const int j = pcol[p];
+    if (mat[j] != mi) continue;
+    const double x = xi - pos[3 * j], y = yi - pos[3 * j + 1], z = zi - pos[3 * j + 2];
+    const double d = fmax(sqrt(x * x + y * y + z * z), 2.220446049250313e-16);
+    double s, c;
+    sincos(kr * d, &s, &c);
+    const double e = exp(-ki * d);
+    const double phr = e * c, phi = e * s;
+    // refined value (k_green_overlay)
+    double pr = 1.0, pim = 0.0;
+    double sgr = 0, sgi = 0, sfr = 0, sfi = 0;
+    for (int q = 0; q < ord1; ++q) {
+      const double gq = g[p * ord1 + q];
+      const double fq = nx * f[(p * 3 + 0) * ord1 + q] + ny * f[(p * 3 + 1) * ord1 + q] + nz * f[(p * 3 + 2) * ord1 + q];
+      sgr += gq * pr;
+      sgi += gq * pim;
+      sfr += fq * pr;
+      sfi += fq * pim;
+      const double tr = pr * (-ki) - pim * kr;
+      pim = pr * kr + pim * (-ki);
+      pr = tr;
+    }
+    double gr = sgr * phr - sgi * phi, gi = sgr * phi + sgi * phr;
+    double hr = sfr * phr - sfi * phi, hi = sfr * phi + sfi * phr;
+    if (j == i) {
+      hr += hsign * 2.0 * M_PI;
+    } else {  // minus the plain entry k_green_apply added (k_green_assemble)
+      const double id = 1.0 / d, aj = area[j];
+      const double ga = aj * id;
+      gr -= ga * phr;
+      gi -= ga * phi;
+      const double ndr = nx * x + ny * y + nz * z;
+      const double sc = ndr * id * id * aj;
+      const double ar = (-ki - id) * sc, ai = kr * sc;
+      hr -= ar * phr - ai * phi;
+      hi -= ar * phi + ai * phr;
+    }
+    const double* xj = xb + 2 * ((long)j * ldx);
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      if (q < cols) {
+        const cplx<double> v = cld(xj + 2 * q);
+        ag[q] = cfma(ag[q], cmake(gr, gi), v);
+        ah[q] = cfma(ah[q], cmake(hr, hi), v);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ag[q].re += __shfl_xor(ag[q].re, o);
+      ag[q].im += __shfl_xor(ag[q].im, o);
+      ah[q].re += __shfl_xor(ah[q].re, o);
+      ah[q].im += __shfl_xor(ah[q].im, o);
+    }
+  }
+  if (lane == 0) {
+    double* yb = Y + 2 * (b * sy + (long)i * ldy);
+    for (int q = 0; q < cols; ++q) {
+      yb[2 * (yg + q)] += ag[q].re;
+      yb[2 * (yg + q) + 1] += ag[q].im;
+      yb[2 * (yh + q)] += ah[q].re;
+      yb[2 * (yh + q) + 1] += ah[q].im;
+    }
+  }
+}
+
+// Y[:, yg:yg+cols] = G X[:, xoff:xoff+cols], Y[:, yh:yh+cols] = H X[...] for
+// the Green functions of the side whose per-face material indices are `mat`
+// (hsign +1: H1 = F + 2 pi, -1: H2 = F - 2 pi), complex128, batch of
+// wavelengths (kmat as for mnpbem_green_assemble; rowptr, pcol, g, f as for
+// mnpbem_surface_field).  Same values as mnpbem_green_assemble followed by a
+// product with the assembled matrices, without storing them.  cols <= 16.
+extern "C" int mnpbem_green_apply(int n, int cols, double hsign, const double* pos, const double* nvec,
+                                  const double* area, const int* mat, const double* kmat, int nmat, int order,
+                                  const long* rowptr, const int* pcol, const double* g, const double* f,
+                                  const void* X, long ldx, long sx, int xoff, void* Y, long ldy, long sy, int yg,
+                                  int yh, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && cols >= 1 && cols <= 16 && xoff >= 0 && xoff + cols <= ldx && yg >= 0 && yh >= 0 &&
+                  yg + cols <= ldy && yh + cols <= ldy,
+              "green_apply: bad size");
+  if (batch <= 0) return 0;
+  dim3 g1(mnp_ceil_div(n, 256), batch), g2(mnp_ceil_div(n, 4), batch);
+  const double* x = (const double*)X;
+  double* y = (double*)Y;
+#define MNP_GA(CC)                                                                                                  \
+  {                                                                                                                 \
+    hipLaunchKernelGGL(k_green_apply<CC>, g1, dim3(256), 0, st, n, cols, pos, nvec, area, mat, kmat, nmat, x, ldx,  \
+                       sx, xoff, y, ldy, sy, yg, yh);                                                               \
+    MNP_LAUNCH_CHECK();                                                                                             \
+    hipLaunchKernelGGL(k_green_apply_ref<CC>, g2, dim3(256), 0, st, n, cols, hsign, pos, nvec, area, mat, kmat,     \
+                       nmat, order + 1, rowptr, pcol, g, f, x, ldx, sx, xoff, y, ldy, sy, yg, yh);                  \
+    MNP_LAUNCH_CHECK();                                                                                             \
+    return 0;                                                                                                       \
+  }
+  if (cols <= 4) MNP_GA(4)
+  if (cols <= 8) MNP_GA(8)
+  MNP_GA(16)
+#undef MNP_GA
+}

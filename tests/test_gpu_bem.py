@@ -63,6 +63,64 @@ def test_green_matrices_match_oracle(setup284, lam):
     assert g.eval(0, 1, "G", lam) == 0
 
 
+def test_matrix_free_green_products(setup284):
+    """mnpbem_green_apply (entries recomputed, not stored) against the
+    assembled complex128 matrices times the same columns (round-off: same
+    formulas, different summation order) and against the oracle's matrices
+    (their parity bound)."""
+    from mnpbem_amd import _native as nat
+    from mnpbem_amd.greenfun import CompGreenRet
+    cp, comp, ref = setup284
+    g = CompGreenRet(cp, cp)
+    n, P, lams = g.n, 2, np.array([450.0, 640.0])
+    B, C = len(lams), 8 * P
+    print("matrix-free green products N=284", flush=True)
+    rng = np.random.default_rng(5)
+    X = torch.from_numpy(rng.standard_normal((B, n, C)) + 1j * rng.standard_normal((B, n, C))).to(g.device)
+    Y = torch.zeros((B, n, 16 * P), dtype=torch.complex128, device=g.device)
+    kmat = g.wavenumbers(lams)
+    st = nat.stream_handle(g.device)
+    P4 = 4 * P
+    for s, hs in ((0, 1.0), (1, -1.0)):
+        nat.call("mnpbem_green_apply", n, P4, hs, nat.ptr(g.pos), nat.ptr(g.nvec), nat.ptr(g.area), nat.ptr(g.mat[s]),
+                 nat.ptr(kmat), g.nmat, g.order, nat.ptr(g._rowptr()), nat.ptr(g.pcol), nat.ptr(g.g), nat.ptr(g.f),
+                 nat.ptr(X), C, n * C, s * P4, nat.ptr(Y), 16 * P, n * 16 * P, 2 * s * P4, 2 * s * P4 + P4, B, st)
+    Yh = Y.cpu().numpy()
+    Xh = X.cpu().numpy()
+    for s, hs in ((0, 1.0), (1, -1.0)):
+        G = torch.empty((B, n, n), dtype=torch.complex128, device=g.device)
+        H = torch.empty_like(G)
+        g.assemble(s, kmat, G, H, hs, nat.DTYPE_C128)
+        Gt, Ht = G.cpu().numpy(), H.cpu().numpy()  # buffers hold M^T
+        for b in range(B):
+            xs = Xh[b][:, s * P4:(s + 1) * P4]
+            for M, off in ((Gt[b].T, 2 * s * P4), (Ht[b].T, 2 * s * P4 + P4)):
+                want = M @ xs
+                got = Yh[b][:, off:off + P4]
+                assert np.abs(got - want).max() < 1e-13 * np.abs(want).max(), (s, b, off)
+        # the oracle's matrices at the same wavelength
+        G1, H1, G2, H2 = o.green_matrices(ref, lams[0])
+        Mo = (G1, H1) if s == 0 else (G2, H2)
+        for M, off in zip(Mo, (2 * s * P4, 2 * s * P4 + P4)):
+            want = M @ Xh[0][:, s * P4:(s + 1) * P4]
+            assert np.abs(Yh[0][:, off:off + P4] - want).max() < 1e-10 * np.abs(want).max()
+
+
+def test_mixed_residual_paths_agree(setup284):
+    """The mixed solve with the matrix-free FP64 residual and with the stored
+    complex128 copies reach the same extinction (both refine to ~1e-14)."""
+    import mnpbem_amd as m
+    cp, comp, ref = setup284
+    lams = np.array([420.0, 530.0, 610.0])
+    exc = m.PlaneWaveRet([[1, 0, 0], [0, 1, 0]], [[0, 0, 1], [0, 0, 1]])
+    print("mixed residual paths", flush=True)
+    e_mf = m.BEMRet(cp, precision="mixed", residual="matrix_free").solve_sweep(lams, exc)
+    b_st = m.BEMRet(cp, precision="mixed", residual="stored")
+    e_st = b_st.solve_sweep(lams, exc)
+    assert np.max(np.abs(e_mf / e_st - 1)) < 1e-12
+    assert np.max(b_st.last_residual) < 1e-12
+
+
 @pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("mixed", 1e-9), ("mixed-fp32", 1e-9)])
 def test_bemret_extinction_matches_oracle(setup284, precision, tol):
     import mnpbem_amd as m
