@@ -8,10 +8,10 @@ ACACompGreenRet), PlaneWaveRet and CompStruct.  Device work runs in libmnpbem_am
 """
 
 from .materials import EV2NM, EpsConst, EpsDrude, EpsTable
-from .geometry import ComParticle, Particle, triellipsoid, trisphere, trisphere_fibonacci
+from .geometry import ComParticle, Particle, triellipsoid, trirod, trisphere, trisphere_fibonacci
 
 __all__ = ["EV2NM", "EpsConst", "EpsDrude", "EpsTable", "ComParticle", "Particle", "trisphere",
-           "trisphere_fibonacci", "triellipsoid",
+           "trisphere_fibonacci", "triellipsoid", "trirod",
            "CompGreenRet", "BEMRet", "CompStruct", "PlaneWaveRet", "BEMIter", "BEMRetIter", "ClusterTree",
            "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss", "DrudeExtinction",
            "extinction_with_drude_gradient"]

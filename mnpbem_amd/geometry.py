@@ -196,6 +196,94 @@ def triellipsoid(n, axes):
     return Particle(verts * axes, faces, interp="flat")
 
 
+def _mlinspace(a, b, n):
+    """a + i (b - a) / (n - 1) with the last point exactly b (the MATLAB
+    linspace of mnpbem/utils/matlab_compat.py mlinspace)."""
+    a, b = float(a), float(b)
+    if n == 1:
+        return np.array([b])
+    y = a + np.arange(n, dtype=np.float64) * (b - a) / (n - 1)
+    y[-1] = b
+    return y
+
+
+def _round_digits(x, k):
+    """Round half away from zero to k decimals: round(x 10^k) / 10^k."""
+    y = np.asarray(x, dtype=np.float64) * (10.0 ** k)
+    return np.trunc(y + np.copysign(0.5, y)) / (10.0 ** k)
+
+
+def _grid_triangles(x, y, z):
+    """Vertices (column-major order of the (m, n) grids) and the two triangles
+    of every grid cell: all [v00, v01, v11] first, then all [v00, v11, v10]
+    (cell (i, j) with v00 = j m + i, v10 = v00 + 1, v01 = v00 + m)."""
+    m, n = x.shape
+    verts = np.column_stack([x.ravel(order="F"), y.ravel(order="F"), z.ravel(order="F")])
+    jj, ii = np.meshgrid(np.arange(n - 1), np.arange(m - 1), indexing="ij")
+    v00 = (jj * m + ii).ravel()
+    v10, v01, v11 = v00 + 1, v00 + m, v00 + m + 1
+    faces = np.concatenate([np.stack([v00, v01, v11], 1), np.stack([v00, v11, v10], 1)])
+    return verts, faces
+
+
+def _merge_clean(verts, faces):
+    """Merge vertices equal after rounding to 8 decimals (the merged vertex
+    takes the rounded coordinates, vertices sorted) and drop faces whose area
+    falls below 1e-10 of the mean (collapsed faces), keeping the face order
+    (Particle.clean, particle.py:2331-2388)."""
+    vr = _round_digits(verts, 8)
+    uniq, inv = np.unique(vr, axis=0, return_inverse=True)
+    faces = inv.reshape(-1)[faces]
+    c = uniq[faces]
+    area = 0.5 * np.linalg.norm(np.cross(c[:, 0] - c[:, 1], c[:, 2] - c[:, 1]), axis=1)
+    return uniq, faces[area > 1e-10 * area.mean()]
+
+
+def trirod(diameter, height, n=None, triangles=True, **kwargs):
+    """Rod (cylinder with hemispherical caps) of flat triangles, the face
+    order and vertex arithmetic of mesh_generators.py:592-640:
+    n = [nphi, ntheta, nz]; upper cap = sphere segment over phi x theta in
+    [0, 2 pi] x [0, pi/2] (grid triangles, merged, vertices projected onto
+    the sphere) shifted up by (height - diameter)/2; lower cap = its mirror
+    image in z with reversed faces; cylinder = phi x z grid triangles with
+    reversed corner order; the three are merged (vertices rounded to 8
+    decimals) and collapsed faces dropped.  Quadrilateral faces
+    (triangles=False, the reference's default) are not supported."""
+    if not triangles:
+        raise NotImplementedError("trirod: quadrilateral faces are not supported; use triangles=True")
+    n = [15, 20, 20] if n is None else list(n)
+    if len(n) != 3:
+        raise ValueError("trirod: n must be [nphi, ntheta, nz]")
+    nphi, ntheta, nz = (int(v) for v in n)
+    d, h = float(diameter), float(height)
+    if not (d > 0 and h >= d):
+        raise ValueError("trirod: need diameter > 0 and height >= diameter")
+    phi = _mlinspace(0, 2 * np.pi, nphi)
+    theta = _mlinspace(0, 0.5 * np.pi, ntheta)
+    # cap: sphere segment (trispheresegment, mesh_generators.py:549-589)
+    pg, tg = np.meshgrid(phi, theta)
+    cx = d / 2.0 * np.sin(tg) * np.cos(pg)
+    cy = d / 2.0 * np.sin(tg) * np.sin(pg)
+    cz = d / 2.0 * np.cos(tg)
+    cv, cf = _grid_triangles(cx, cy, cz)
+    cv, cf = _merge_clean(cv, cf)
+    norms = np.maximum(np.sqrt(np.sum(cv ** 2, axis=1, keepdims=True)), 1e-30)
+    cv = 0.5 * d * (cv / norms)
+    cap1 = cv + np.array([0.0, 0.0, 0.5 * (h - d)])
+    cap2 = cap1 * np.array([1.0, 1.0, -1.0])
+    # cylinder (fvgrid, mesh_generators.py:515-546, corner order reversed)
+    zv = 0.5 * _mlinspace(-1, 1, nz) * (h - d)
+    gx, gy = np.meshgrid(phi, zv)
+    gv, gf = _grid_triangles(gx, gy, np.zeros_like(gx))
+    gf = gf[:, ::-1]
+    cyl = np.column_stack([0.5 * d * np.cos(gv[:, 0]), 0.5 * d * np.sin(gv[:, 0]), gv[:, 1]])
+    verts = np.concatenate([cap1, cap2, cyl])
+    o2, o3 = len(cap1), 2 * len(cap1)
+    faces = np.concatenate([cf, cf[:, ::-1] + o2, gf + o3])
+    verts, faces = _merge_clean(verts, faces)
+    return Particle(verts, faces, interp="flat")
+
+
 def trisphere(n, diameter=1.0, **kwargs):
     """Sphere of curved triangles from the saved point set closest to ``n``
     vertices: scale to the diameter, add one node per edge, project all

@@ -38,7 +38,7 @@ __all__ = [
     "lglnodes", "rule18", "polar_rule_tri", "tri6_shape", "tri6_deriv",
     "green_hp", "surface_potential", "surface_field",
     "read_tri_bin", "trisphere_mesh", "golden_spiral", "spherical_delaunay", "fibonacci_sphere_mesh",
-    "triellipsoid_mesh", "CurvedSurface", "Composite",
+    "triellipsoid_mesh", "flat_mesh_from_corners", "CurvedSurface", "Composite",
     "refinematrix", "refine_coefficients", "closed_surface_correction",
     "GreenSetup", "green_matrices", "BEMRetOracle", "planewave_potential",
     "farfield", "extinction", "scattering", "read_pinfty",
@@ -236,6 +236,16 @@ def _edge_nodes(verts, faces):
     uniq, inv = np.unique(e, axis=0, return_inverse=True)
     nodes = np.vstack([verts, 0.5 * (verts[uniq[:, 0]] + verts[uniq[:, 1]])])
     return nodes, np.hstack([faces, nv + inv.reshape(3, -1).T])
+
+
+def flat_mesh_from_corners(corners):
+    """(F, 3, 3) corner coordinates of flat triangles (in face order) ->
+    (nodes, tri6) with shared vertices and straight-edge midpoints, the
+    form CurvedSurface integrates exactly as flat triangles (particle.py
+    _norm_flat / _quad_flat)."""
+    c = np.asarray(corners, dtype=float)
+    verts, inv = np.unique(c.reshape(-1, 3), axis=0, return_inverse=True)
+    return _edge_nodes(verts, inv.reshape(-1, 3))
 
 
 def golden_spiral(n):

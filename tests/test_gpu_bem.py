@@ -345,3 +345,24 @@ def test_drude_gradient_matches_finite_differences(cuda):
             want = (wts.numpy()[:, :, None] * grad).sum(axis=(0, 1))
             assert np.allclose(params.grad.numpy(), want, rtol=1e-12)
             assert np.allclose(e.detach().numpy(), ext, rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("mixed", 1e-9)])
+def test_trirod_flat_matches_oracle_on_reference_mesh(cuda, precision, tol):
+    """Flat-triangle rod (config 4's geometry: medium eps 1.77, polarisation
+    along the axis): the product's trirod on the GPU against the oracle on
+    the mesh written by the reference's own trirod (tests/golden)."""
+    import mnpbem_amd as m
+    z = np.load(os.path.join(GOLD, "trirod_tri.npz"))
+    key = "d20_h60_n15_8_20"
+    print("trirod", precision, flush=True)
+    p = m.trirod(20.0, 60.0, [15, 8, 20])
+    cp = m.ComParticle([m.EpsConst(1.77), m.EpsDrude(*DRUDE)], [p], [[2, 1]], 1)
+    nodes, tri6 = o.flat_mesh_from_corners(z[key + "_corners"])
+    comp = o.Composite([o.EpsConst(1.77), o.EpsDrude(*DRUDE)], [o.CurvedSurface(nodes, tri6)], [[2, 1]])
+    lams = np.array([560.0, 720.0])
+    pol, dr = [[0, 0, 1], [1, 0, 0]], [[1, 0, 0], [0, 0, 1]]
+    ext = m.BEMRet(cp, precision=precision).solve_sweep(lams, m.PlaneWaveRet(pol, dr))
+    ext_ref = o.solve_spectrum(comp, lams, pol, dr)
+    err = np.max(np.abs(ext / ext_ref - 1))
+    assert err < tol, err

@@ -122,3 +122,58 @@ def test_product_meshes_equal_oracle():
         t = g.trisphere(n, 10.0)
         nodes, tri6 = o.trisphere_mesh(n, 10.0)
         assert t.n == 2 * n - 4 and np.array_equal(t.verts2[t.faces6], nodes[tri6])
+
+
+ROD_CASES = {"d20_h60_n15_8_20": (20.0, 60.0, [15, 8, 20]), "d10_h40_n21_11_31": (10.0, 40.0, [21, 11, 31])}
+
+
+def _golden_rods():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "trirod_tri.npz"))
+
+
+@pytest.mark.parametrize("key", sorted(ROD_CASES))
+def test_trirod_matches_reference_golden(key):
+    """trirod(..., triangles=True) against golden vectors written by the
+    reference's own mesh code (tests/golden/make_trirod_golden.py): face order
+    and corner coordinates bit-identical; centroids, normals, areas (the
+    reference's flat _norm) to round-off."""
+    g = pytest.importorskip("mnpbem_amd.geometry")
+    z = _golden_rods()
+    d, h, n = ROD_CASES[key]
+    p = g.trirod(d, h, n)
+    assert np.array_equal(p.verts2[p.faces6[:, :3]], z[key + "_corners"])
+    assert np.abs(p.pos - z[key + "_pos"]).max() < 1e-13 * d
+    assert np.abs(p.nvec - z[key + "_nvec"]).max() < 1e-13
+    assert np.abs(p.area / z[key + "_area"] - 1).max() < 1e-13
+    # the oracle builds the same surface from the golden corners
+    nodes, tri6 = o.flat_mesh_from_corners(z[key + "_corners"])
+    s = o.CurvedSurface(nodes, tri6)
+    assert np.allclose(s.pos, z[key + "_pos"], rtol=0, atol=1e-13 * d)
+    assert np.allclose(s.area, z[key + "_area"], rtol=1e-13)
+
+
+def test_trirod_closed_outward_and_converging():
+    """Closed 2-manifold with outward normals; area and enclosed volume
+    converge to the capsule's pi d (h - d) + pi d^2 and
+    pi (d/2)^2 (h - d) + 4/3 pi (d/2)^3."""
+    g = pytest.importorskip("mnpbem_amd.geometry")
+    d, h = 20.0, 60.0
+    errs = []
+    for n in ([11, 6, 10], [21, 11, 20], [41, 21, 40]):
+        p = g.trirod(d, h, n)
+        f = p.faces6[:, :3]
+        directed = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        assert len({tuple(e) for e in directed}) == len(directed)
+        _, cnt = np.unique(np.sort(directed, axis=1), axis=0, return_counts=True)
+        assert np.all(cnt == 2)
+        c = p.pos
+        zc = np.clip(c[:, 2], -(h - d) / 2, (h - d) / 2)
+        out = c - np.stack([0 * zc, 0 * zc, zc], 1)   # from the rod axis segment
+        assert np.all(np.sum(out * p.nvec, axis=1) > 0)
+        vol = np.sum(np.sum(c * p.nvec, axis=1) * p.area) / 3
+        a_ref = np.pi * d * (h - d) + np.pi * d ** 2
+        v_ref = np.pi * (d / 2) ** 2 * (h - d) + 4 / 3 * np.pi * (d / 2) ** 3
+        errs.append((abs(p.area.sum() / a_ref - 1), abs(vol / v_ref - 1)))
+    assert errs[0][0] > 3 * errs[1][0] > 9 * errs[2][0] and errs[2][0] < 5e-3
+    assert errs[0][1] > 3 * errs[1][1] > 9 * errs[2][1] and errs[2][1] < 5e-3
