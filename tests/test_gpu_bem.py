@@ -366,3 +366,37 @@ def test_trirod_flat_matches_oracle_on_reference_mesh(cuda, precision, tol):
     ext_ref = o.solve_spectrum(comp, lams, pol, dr)
     err = np.max(np.abs(ext / ext_ref - 1))
     assert err < tol, err
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("mixed", 1e-9)])
+def test_drude_spectrum_matches_reference_golden(cuda, precision, tol):
+    """Config 1 in full (Drude gold, trisphere(144, 20), 41 wavelengths, x and
+    y polarisation): the GPU spectrum against the reference BEMRet's own
+    extinction and scattering (tests/golden/reference_n284.npz)."""
+    import mnpbem_amd as m
+    z = np.load(os.path.join(GOLD, "reference_n284.npz"))
+    print("drude spectrum vs reference", precision, flush=True)
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [m.trisphere(144, 20.0)], [[2, 1]], 1, interp="curv")
+    exc = m.PlaneWaveRet([[1, 0, 0], [0, 1, 0]], [[0, 0, 1], [0, 0, 1]])
+    ext, sca = m.BEMRet(cp, precision=precision).solve_sweep(z["lams"], exc, chunk=41, with_sca=True)
+    assert np.max(np.abs(ext / z["ext"] - 1)) < tol
+    assert np.max(np.abs(sca / z["sca"] - 1)) < tol
+
+
+def test_green_and_refinement_match_reference_golden(cuda):
+    """The GPU refinement classification against the reference's pair list,
+    and rows of the four GPU Green matrices against the reference's
+    CompGreenRet.eval (tests/golden/reference_n284.npz)."""
+    import mnpbem_amd as m
+    from mnpbem_amd.greenfun import CompGreenRet
+    z = np.load(os.path.join(GOLD, "reference_n284.npz"))
+    print("green / refinement vs reference", flush=True)
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [m.trisphere(144, 20.0)], [[2, 1]], 1, interp="curv")
+    g = CompGreenRet(cp, cp)
+    assert np.array_equal(g.row, z["ref_row"]) and np.array_equal(g.col, z["ref_col"])
+    assert np.array_equal(np.asarray(g.kind).astype(np.int8), z["ref_kind"])
+    rows = z["green_rows"]
+    for name, (i, key) in {"G1": (0, "G"), "H1": (0, "H1"), "G2": (1, "G"), "H2": (1, "H2")}.items():
+        M = g.eval(i, i, key, 520.0).cpu().numpy()
+        want = z[name]
+        assert np.abs(M[rows] - want).max() < 1e-10 * np.abs(want).max(), name
