@@ -278,12 +278,14 @@ int mnpbem_hmat_dense(int dtype, int nb, const long* blk, const int* bids, const
 /* packed U, V of the low-rank blocks bids[0:nb] (ranks in blk) from the ACA workspaces */
 int mnpbem_hmat_compact(int dtype, int nb, const long* blk, const int* bids, const void* Uw, const void* Vw,
                         const long* woff, void* U, void* V, void* stream);
-/* Y (+)= M X, X and Y complex128 (n x ldx / ldy, face order), ncol <= 16;
- * T: complex128 work space of (sum of ranks) * ncol; maxchunks =
+/* Y (+)= M X for the n x n H-matrix, X and Y complex128 (n x ldx / ldy, face
+ * order), ncol <= 16; T: complex128 work space of (sum of ranks) * ncol; XW:
+ * complex128 work space of n * ncol (X in cluster order); maxchunks =
  * ceil(largest leaf / 256). */
-int mnpbem_hmat_matvec(int dtype, int nblk, int nleaf, int maxchunks, const long* blk, const long* leafs,
+int mnpbem_hmat_matvec(int dtype, int n, int nblk, int nleaf, int maxchunks, const long* blk, const long* leafs,
                        const int* lst, const int* perm, const void* val, const void* U, const void* V, int ncol,
-                       const void* X, long ldx, void* Y, long ldy, int accumulate, void* T, void* stream);
+                       const void* X, long ldx, void* Y, long ldy, int accumulate, void* T, void* XW,
+                       void* stream);
 /* Out = M as an n x n complex128 row-major matrix (face order, leading dimension ldo) */
 int mnpbem_hmat_full(int dtype, int nblk, const long* blk, const int* perm, const void* val, const void* U,
                      const void* V, void* Out, long ldo, void* stream);

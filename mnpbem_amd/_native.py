@@ -58,7 +58,7 @@ SIGNATURES = {
     "mnpbem_hmat_aca": [_i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _i, _d, _p, _p],
     "mnpbem_hmat_dense": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p],
     "mnpbem_hmat_compact": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p],
-    "mnpbem_hmat_matvec": [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p, _l, _p, _l, _i, _p, _p],
+    "mnpbem_hmat_matvec": [_i, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p, _l, _p, _l, _i, _p, _p, _p],
     "mnpbem_hmat_full": [_i, _i, _p, _p, _p, _p, _p, _p, _l, _p],
     "mnpbem_hmat_truncate": [_i, _i, _p, _p, _p, _p, _p, _i, _d, _p, _p, _p, _p, _p],
     "mnpbem_hmat_trunc_form": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p],

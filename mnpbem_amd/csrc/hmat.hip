@@ -259,21 +259,50 @@ __global__ __launch_bounds__(256) void k_hmat_compact(const long* blk, const int
   for (long e = threadIdx.x; e < nv; e += blockDim.x) cst(V + 2 * (B.ob + e), cmake<TS>((TS)v[2 * e], (TS)v[2 * e + 1]));
 }
 
-// matvec phase 1: t_b = V_b^T x[cols of b] for every low-rank block (one wave per rank column)
+// matvec phase 0: XW = X in cluster order (row c of XW = row perm[c] of X),
+// so that the block products below read their columns contiguously
+__global__ __launch_bounds__(256) void k_hmat_xperm(int n, int ncol, const int* perm, const double* X, long ldx,
+                                                    double* XW) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long)n * ncol) return;
+  const long c = e / ncol;
+  const int q = (int)(e % ncol);
+  const double* x = X + 2 * ((long)perm[c] * ldx + q);
+  XW[2 * e] = x[0];
+  XW[2 * e + 1] = x[1];
+}
+
+// matvec phase 1: t_b = V_b^T x[cols of b] for every low-rank block (one
+// wave per rank column, four column loads in flight per lane; the x rows of
+// the block are contiguous in XW and come from L2 for the later columns)
 template <typename TS, int NC>
-__global__ __launch_bounds__(256) void k_hmat_vt(const long* blk, const int* perm, const TS* Vv, const double* X,
-                                                 long ldx, int ncol, double* T) {
+__global__ __launch_bounds__(256) void k_hmat_vt(const long* blk, const TS* Vv, const double* XW, int ncol,
+                                                 double* T) {
   const Blk B = load_blk(blk, blockIdx.x);
   if (B.rank <= 0) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double* xb = XW + 2 * (B.c0 * (long)ncol);
   for (long r = wv; r < B.rank; r += 4) {
     cplx<double> acc[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[c] = cmake(0.0, 0.0);
     const TS* v = Vv + 2 * (B.ob + r * B.n);
-    for (long j = lane; j < B.n; j += 64) {
+    long j = lane;
+    for (; j + 192 < B.n; j += 256) {
+      cplx<double> a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = ldw(v + 2 * (j + 64 * u));
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double* x = xb + 2 * ((j + 64 * u) * ncol);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (c < ncol) acc[c] = cfma(acc[c], a[u], ld2(x + 2 * c));
+      }
+    }
+    for (; j < B.n; j += 64) {
       const cplx<double> a = ldw(v + 2 * j);
-      const double* x = X + 2 * ((long)perm[B.c0 + j] * ldx);
+      const double* x = xb + 2 * (j * ncol);
 #pragma unroll
       for (int c = 0; c < NC; ++c)
         if (c < ncol) acc[c] = cfma(acc[c], a, ld2(x + 2 * c));
@@ -295,12 +324,20 @@ __global__ __launch_bounds__(256) void k_hmat_vt(const long* blk, const int* per
 // matvec phase 2: one workgroup per (leaf row cluster, 256-row chunk); the
 // waves share the leaf's rows in groups of 64 and split its block list; the
 // group's partial sums meet in LDS and row i of the leaf is written once, to
-// Y[perm[r0 + i]] (deterministic, no atomics).
+// Y[perm[r0 + i]] (deterministic, no atomics).  Per block a wave stages the
+// right-hand side it multiplies (the block's x rows, or its t slice) in LDS,
+// 64 rows at a time, and keeps 16 matrix loads per lane in flight.
+template <int NC>
+struct RowsCfg {
+  static constexpr int nw = 4;  // waves per workgroup (8 measured no faster)
+};
+
 template <typename TS, int NC>
-__global__ __launch_bounds__(256) void k_hmat_rows(const long* leafs, const int* lst, const long* blk,
-                                                   const int* perm, const TS* val, const TS* Uv, const double* T,
-                                                   const double* X, long ldx, int ncol, double* Y, long ldy,
-                                                   int accumulate) {
+__global__ __launch_bounds__(64 * RowsCfg<NC>::nw) void k_hmat_rows(const long* leafs, const int* lst,
+                                                                    const long* blk, const int* perm, const TS* val,
+                                                                    const TS* Uv, const double* T, const double* XW,
+                                                                    int ncol, double* Y, long ldy, int accumulate) {
+  constexpr int NW = RowsCfg<NC>::nw;
   const long* lf = leafs + 4 * blockIdx.x;
   const long lr0 = lf[0], lm = lf[1], first = lf[2], last = lf[3];
   const long i0 = (long)blockIdx.y * 256;
@@ -308,44 +345,57 @@ __global__ __launch_bounds__(256) void k_hmat_rows(const long* leafs, const int*
   const int mc = (int)(lm - i0 < 256 ? lm - i0 : 256);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int RG = (mc + 63) / 64;  // row groups of 64
-  const int WPG = 4 / RG;         // waves per row group
+  const int WPG = NW / RG;        // waves per row group
   const int g = wv / WPG, sub = wv % WPG;
   const bool active = g < RG;
   const long li = i0 + (long)g * 64 + lane;  // row within the leaf
   const bool rowok = active && (g * 64 + lane) < mc;
+  __shared__ cplx<double> rhs[NW][64][NC];  // per wave: the right-hand side rows of the current block
   cplx<double> acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = cmake(0.0, 0.0);
   if (active) {
     for (long e = first + sub; e < last; e += WPG) {
       const Blk B = load_blk(blk, lst[e]);
-      if (B.rank < 0) {
-        const TS* a = val + 2 * (B.oa + li);
-        for (long j = 0; j < B.n; ++j) {
-          const double* x = X + 2 * ((long)perm[B.c0 + j] * ldx);
-          if (rowok) {
-            const cplx<double> v = ldw(a + 2 * j * B.m);
+      const bool dense = B.rank < 0;
+      // matrix column q of the block: dense values val[oa + q m + row], or U[oa + q m + row]
+      const long row = dense ? li : lr0 + li - B.r0;
+      const TS* a = (dense ? val : Uv) + 2 * (B.oa + row);
+      const long nq = dense ? B.n : B.rank;
+      for (long q0 = 0; q0 < nq; q0 += 64) {
+        const int nb = (int)(nq - q0 < 64 ? nq - q0 : 64);
+        // stage: x rows of columns c0 + q0.. (dense) or t rows ot + q0.. (low rank)
+        const double* src = dense ? XW + 2 * ((B.c0 + q0) * (long)ncol) : T + 2 * ((B.ot + q0) * (long)ncol);
+        if (lane < nb) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            rhs[wv][lane][c] = c < ncol ? ld2(src + 2 * ((long)lane * ncol + c)) : cmake(0.0, 0.0);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        int q = 0;
+        for (; q + 16 <= nb; q += 16) {
+          cplx<double> v[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) v[u] = rowok ? ldw(a + 2 * (q0 + q + u) * B.m) : cmake(0.0, 0.0);
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
 #pragma unroll
             for (int c = 0; c < NC; ++c)
-              if (c < ncol) acc[c] = cfma(acc[c], v, ld2(x + 2 * c));
-          }
+              if (c < ncol) acc[c] = cfma(acc[c], v[u], rhs[wv][q + u][c]);
         }
-      } else {
-        const long lrow = lr0 + li - B.r0;  // row inside the low-rank block
-        const TS* u = Uv + 2 * (B.oa + lrow);
-        for (long r = 0; r < B.rank; ++r) {
-          const double* t = T + 2 * ((B.ot + r) * ncol);
-          if (rowok) {
-            const cplx<double> v = ldw(u + 2 * r * B.m);
+        for (; q < nb; ++q) {
+          const cplx<double> v = rowok ? ldw(a + 2 * (q0 + q) * B.m) : cmake(0.0, 0.0);
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
-              if (c < ncol) acc[c] = cfma(acc[c], v, ld2(t + 2 * c));
-          }
+          for (int c = 0; c < NC; ++c)
+            if (c < ncol) acc[c] = cfma(acc[c], v, rhs[wv][q][c]);
         }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
     }
   }
-  __shared__ double red[4][64][NC][2];
+  __shared__ double red[NW][64][NC][2];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     red[wv][lane][c][0] = acc[c].re;
@@ -713,15 +763,20 @@ __global__ __launch_bounds__(256) void k_hmat_diag(const long* blk, const int* b
 }
 
 template <typename TS, int NC>
-int matvec_launch(int nblk, int nleaf, int maxchunks, const long* blk, const long* leafs, const int* lst,
+int matvec_launch(int n, int nblk, int nleaf, int maxchunks, const long* blk, const long* leafs, const int* lst,
                   const int* perm, const void* val, const void* U, const void* V, int ncol, const double* X,
-                  long ldx, double* Y, long ldy, int accumulate, double* T, hipStream_t st) {
+                  long ldx, double* Y, long ldy, int accumulate, double* T, double* XW, hipStream_t st) {
+  hipLaunchKernelGGL(k_hmat_xperm, dim3(mnp_ceil_div((long)n * ncol, 256)), dim3(256), 0, st, n, ncol, perm, X, ldx,
+                     XW);
+  MNP_LAUNCH_CHECK();
   if (nblk > 0) {
-    hipLaunchKernelGGL((k_hmat_vt<TS, NC>), dim3(nblk), dim3(256), 0, st, blk, perm, (const TS*)V, X, ldx, ncol, T);
+    hipLaunchKernelGGL((k_hmat_vt<TS, NC>), dim3(nblk), dim3(256), 0, st, blk, (const TS*)V, (const double*)XW, ncol,
+                       T);
     MNP_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL((k_hmat_rows<TS, NC>), dim3(nleaf, maxchunks), dim3(256), 0, st, leafs, lst, blk, perm,
-                     (const TS*)val, (const TS*)U, (const double*)T, X, ldx, ncol, Y, ldy, accumulate);
+  hipLaunchKernelGGL((k_hmat_rows<TS, NC>), dim3(nleaf, maxchunks), dim3(64 * RowsCfg<NC>::nw), 0, st, leafs, lst,
+                     blk, perm,
+                     (const TS*)val, (const TS*)U, (const double*)T, (const double*)XW, ncol, Y, ldy, accumulate);
   MNP_LAUNCH_CHECK();
   return 0;
 }
@@ -783,23 +838,26 @@ extern "C" int mnpbem_hmat_compact(int dtype, int nb, const long* blk, const int
   return 0;
 }
 
-// Y (+)= M X for the H-matrix (hmatrix.py:450-541).  X, Y: (n, ldx / ldy)
-// complex128 row-major in face order, ncol <= 16 columns used.  T: complex128
-// workspace of (sum of ranks) * ncol elements.  maxchunks = ceil(max leaf
-// size / 256).
-extern "C" int mnpbem_hmat_matvec(int dtype, int nblk, int nleaf, int maxchunks, const long* blk, const long* leafs,
-                                  const int* lst, const int* perm, const void* val, const void* U, const void* V,
-                                  int ncol, const void* X, long ldx, void* Y, long ldy, int accumulate, void* T,
-                                  void* stream) {
+// Y (+)= M X for the H-matrix (hmatrix.py:450-541) of n rows.  X, Y: (n, ldx
+// / ldy) complex128 row-major in face order, ncol <= 16 columns used.  T:
+// complex128 workspace of (sum of ranks) * ncol elements; XW: complex128
+// workspace of n * ncol elements (X in cluster order).  maxchunks =
+// ceil(max leaf size / 256).
+extern "C" int mnpbem_hmat_matvec(int dtype, int n, int nblk, int nleaf, int maxchunks, const long* blk,
+                                  const long* leafs, const int* lst, const int* perm, const void* val, const void* U,
+                                  const void* V, int ncol, const void* X, long ldx, void* Y, long ldy, int accumulate,
+                                  void* T, void* XW, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(ncol >= 1 && ncol <= 16 && ldx >= ncol && ldy >= ncol, "hmat_matvec: 1 <= ncol <= 16");
-  MNP_REQUIRE(nleaf >= 1 && maxchunks >= 1 && maxchunks <= 65535, "hmat_matvec: bad leaf table");
+  MNP_REQUIRE(n >= 1 && nleaf >= 1 && maxchunks >= 1 && maxchunks <= 65535, "hmat_matvec: bad leaf table");
+  MNP_REQUIRE(XW != nullptr, "hmat_matvec: XW workspace required");
   const double* x = (const double*)X;
   double* y = (double*)Y;
   double* t = (double*)T;
+  double* xw = (double*)XW;
 #define MNP_MV(TS, NC)                                                                                          \
-  return matvec_launch<TS, NC>(nblk, nleaf, maxchunks, blk, leafs, lst, perm, val, U, V, ncol, x, ldx, y, ldy, \
-                               accumulate, t, st)
+  return matvec_launch<TS, NC>(n, nblk, nleaf, maxchunks, blk, leafs, lst, perm, val, U, V, ncol, x, ldx, y, ldy, \
+                               accumulate, t, xw, st)
   if (dtype == MNP_DTYPE_C128) {
     if (ncol <= 4) MNP_MV(double, 4);
     if (ncol <= 8) MNP_MV(double, 8);

@@ -329,10 +329,12 @@ class HMatrix:
         (n, ldy) row-major, ncol <= 16 columns."""
         self._check()
         st = nat.stream_handle(self.device) if stream is None else stream
-        nat.call("mnpbem_hmat_matvec", _DT[self.dtype][1], len(self._blk_host), len(self._leafs_host),
+        if getattr(self, "_XW", None) is None or self._XW.numel() < self.tree.n * 16:
+            self._XW = torch.empty((self.tree.n * 16,), dtype=torch.complex128, device=self.device)
+        nat.call("mnpbem_hmat_matvec", _DT[self.dtype][1], self.tree.n, len(self._blk_host), len(self._leafs_host),
                  self._maxchunks, nat.ptr(self._blk), nat.ptr(self._leafs), nat.ptr(self._lst), nat.ptr(self._perm),
                  nat.ptr(self._val), nat.ptr(self._U), nat.ptr(self._V), ncol, X, ldx, Y, ldy,
-                 1 if accumulate else 0, nat.ptr(self._T), st)
+                 1 if accumulate else 0, nat.ptr(self._T), nat.ptr(self._XW), st)
 
     def mtimes_vec(self, v):
         """M v for v (n,) or (n, k), numpy or torch; the result has the type

@@ -59,7 +59,8 @@ def test_invalid_arguments_are_rejected_without_device_work():
     with pytest.raises(RuntimeError, match="kmax"):
         _native.call("mnpbem_hmat_aca", 1, None, None, None, None, 1, 1, None, None, None, 600, 1e-6, None, None)
     with pytest.raises(RuntimeError, match="ncol"):
-        _native.call("mnpbem_hmat_matvec", 2, 1, 1, 1, *([None] * 7), 17, None, 17, None, 17, 0, None, None)
+        _native.call("mnpbem_hmat_matvec", 2, 10, 1, 1, 1, *([None] * 7), 17, None, 17, None, 17, 0, None, None,
+                     None)
     with pytest.raises(RuntimeError, match="order"):
         _native.call("mnpbem_refine_coeffs", 9, 0, *([None] * 10), 28, None, 144, None, None, None, None)
     # recompression rank limits and tolerance
