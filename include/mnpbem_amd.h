@@ -167,6 +167,36 @@ int mnpbem_bem_residual_batched(int n, int P, const void* G1t, const void* H1t, 
                                 void* Y, long ldy, long sy, const double* nvec, const double* par, double* nrm,
                                 int batch, void* stream);
 
+/* ---- adjoint solve for the extinction gradients (no counterpart in the
+ *      reference, which has no derivatives; it transposes the solve of
+ *      bem_ret.py:1694-1810) ----
+ * Y = A^-T C with the factors of mnpbem_bem_solve_batched (same arguments;
+ * C in the unknowns' layout [sig1 | h1 | sig2 | h2], Y in the right-side
+ * layout [phi | a | De | alpha]); W is work space shaped like C. */
+int mnpbem_bem_solve_adjoint_batched(int dtype, int n, int P, const void* G1t, const int* piv1, const void* G2t,
+                                     const int* piv2, const void* S1t, const void* Et, const void* St,
+                                     const int* pivS, long ldm, long smat, long spiv, const void* D1, const void* D2,
+                                     const void* DS, long sD, const void* C, void* W, void* Y, long ldv, long sv,
+                                     const double* nvec, const double* par, int batch, void* stream);
+
+/* Out = C - A^T Y (complex128; C NULL: Out = A^T Y) with unfactored
+ * transposed G1, H1, G2, H2; Z (n x 8P) and T (n x 16P, ldt >= 16P) are work
+ * space; nrm[b] = {||C - A^T Y||^2, ||C||^2} (may be NULL). */
+int mnpbem_bem_residual_adjoint_batched(int n, int P, const void* G1t, const void* H1t, const void* G2t,
+                                        const void* H2t, long ldm, long smat, const void* C, const void* Y, void* Out,
+                                        long ldv, long sv, void* Z, void* T, long ldt, long stt, const double* nvec,
+                                        const double* par, double* nrm, int batch, void* stream);
+
+/* Y = beta Y + buf X, y_i = sum_j buf[i*ldm + j] x_j: the product with M^T
+ * for the transposed buffer of M (the transpose of mnpbem_gemv_t_batched). */
+int mnpbem_gemv_n_batched(int dtype, int n, int C, const void* M, long ldm, long sM, const void* X, long ldx,
+                          long sX, void* Y, long ldy, long sY, double beta, int batch, void* stream);
+
+/* V <- M^-T V with the LU of M^T (the transpose of mnpbem_lu_solve_t_batched). */
+int mnpbem_lu_solve_n_batched(int dtype, int n, int C, const void* LU, long ld, long sLU, const int* ipiv,
+                              long spiv, void* V, long ldv, long sV, void* Z, long ldz, long sZ, const void* Dinv,
+                              long sD, int batch, void* stream);
+
 /* dst (double) += src (complex64 or complex128 as reals); dst = (dtype) src. */
 int mnpbem_accumulate(int src_dtype, long count, const void* src, double* dst, void* stream);
 int mnpbem_convert(int dst_dtype, long count, const double* src, void* dst, void* stream);

@@ -46,6 +46,12 @@ SIGNATURES = {
                                  _p, _p, _l, _l, _p, _p, _i, _p],
     "mnpbem_bem_residual_batched": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _l, _l, _p, _l, _l, _p, _p,
                                     _p, _i, _p],
+    "mnpbem_bem_solve_adjoint_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l,
+                                         _p, _p, _p, _l, _l, _p, _p, _i, _p],
+    "mnpbem_bem_residual_adjoint_batched": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _l, _l, _p, _p, _l, _l,
+                                            _p, _p, _p, _i, _p],
+    "mnpbem_gemv_n_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _l, _p, _l, _l, _d, _i, _p],
+    "mnpbem_lu_solve_n_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _p, _l, _l, _p, _l, _i, _p],
     "mnpbem_accumulate": [_i, _l, _p, _p, _p],
     "mnpbem_convert": [_i, _l, _p, _p, _p],
     "mnpbem_planewave_batched": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _p, _l, _i, _p],

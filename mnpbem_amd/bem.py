@@ -229,6 +229,98 @@ class BEMRet:
         self.last_residual = rel
         return X
 
+    def green64(self, F):
+        """complex128 transposed G1, H1, G2, H2 of the batch, each (2, B, n, n)
+        stacked as (G, H): the copies kept by the mixed path, else assembled
+        (the complex128 path factors its copies in place)."""
+        if getattr(F, "G64", None) is not None:
+            return F.G64, F.H64
+        g, n, B = self.g, F.n, F.B
+        G = torch.empty((2, B, n, n), dtype=torch.complex128, device=self.device)
+        H = torch.empty_like(G)
+        for s, hs in ((0, 1.0), (1, -1.0)):
+            g.assemble(s, F.kmat, G[s], H[s], hs, nat.DTYPE_C128)
+        return G, H
+
+    def solve_adjoint_batch(self, F, C):
+        """Solve A^T Y = C for the batch with the factors of ``factor``
+        (C: (B, n, 8P) complex128 in the unknowns' layout [sig1 | h1 | sig2 |
+        h2]); returns Y (B, n, 8P) in the right-side layout [phi | a | De |
+        alpha], so that Y . R = C . X for A X = R.  The mixed path refines in
+        FP64 against the complex128 transposed operator like ``solve_batch``."""
+        B, n = F.B, F.n
+        P = C.shape[2] // 8
+        dev = self.device
+        st = nat.stream_handle(dev)
+        nn = n * n
+
+        def direct(Cin, Yout, Wk):
+            nat.call("mnpbem_bem_solve_adjoint_batched", F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.piv[0]),
+                     nat.ptr(F.GG[1]), nat.ptr(F.piv[1]), nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]),
+                     nat.ptr(F.piv[3]), n, nn, n, nat.ptr(F.DG[0]), nat.ptr(F.DG[1]), nat.ptr(F.DS), F.sD,
+                     nat.ptr(Cin), nat.ptr(Wk), nat.ptr(Yout), 8 * P, n * 8 * P, nat.ptr(self.g.nvec),
+                     nat.ptr(F.par), B, st)
+
+        if F.dtype == nat.DTYPE_C128:
+            Y = torch.empty_like(C)
+            direct(C, Y, torch.empty_like(C))
+            return Y
+        G64, H64 = self.green64(F)
+        C32 = torch.empty(C.shape, dtype=torch.complex64, device=dev)
+        Y32 = torch.empty_like(C32)
+        W32 = torch.empty_like(C32)
+        Y = torch.zeros_like(C)
+        Cres = torch.empty_like(C)
+        Z = torch.empty_like(C)
+        T = torch.empty((B, n, 16 * P), dtype=torch.complex128, device=dev)
+        nrm = torch.empty((B, 2), dtype=torch.float64, device=dev)
+        cnt = 2 * C.numel()
+        src = C
+        it = 0
+        while True:
+            nat.call("mnpbem_convert", nat.DTYPE_C64, cnt, nat.ptr(src), nat.ptr(C32), st)
+            direct(C32, Y32, W32)
+            nat.call("mnpbem_accumulate", nat.DTYPE_C64, cnt, nat.ptr(Y32), nat.ptr(Y), st)
+            it += 1
+            nat.call("mnpbem_bem_residual_adjoint_batched", n, P, nat.ptr(G64[0]), nat.ptr(H64[0]),
+                     nat.ptr(G64[1]), nat.ptr(H64[1]), n, nn, nat.ptr(C), nat.ptr(Y), nat.ptr(Cres), 8 * P,
+                     n * 8 * P, nat.ptr(Z), nat.ptr(T), 16 * P, n * 16 * P, nat.ptr(self.g.nvec), nat.ptr(F.par),
+                     nat.ptr(nrm), B, st)
+            h = nrm.cpu().numpy()
+            rel = np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300))
+            if np.all(rel <= self.refine_tol) or it > self.refine_maxit:
+                break
+            src = Cres
+        self.last_adjoint_iters = it
+        self.last_adjoint_residual = rel
+        return Y
+
+    def apply_transpose(self, F, Y):
+        """A^T Y in complex128 (B, n, 8P), from the complex128 Green matrices."""
+        B, n = F.B, F.n
+        P = Y.shape[2] // 8
+        G64, H64 = self.green64(F)
+        out = torch.empty_like(Y)
+        Z = torch.empty_like(Y)
+        T = torch.empty((B, n, 16 * P), dtype=torch.complex128, device=self.device)
+        nat.call("mnpbem_bem_residual_adjoint_batched", n, P, nat.ptr(G64[0]), nat.ptr(H64[0]), nat.ptr(G64[1]),
+                 nat.ptr(H64[1]), n, n * n, None, nat.ptr(Y), nat.ptr(out), 8 * P, n * 8 * P, nat.ptr(Z),
+                 nat.ptr(T), 16 * P, n * 16 * P, nat.ptr(self.g.nvec), nat.ptr(F.par), None, B,
+                 nat.stream_handle(self.device))
+        return out
+
+    def apply(self, F, X):
+        """A X in complex128 (B, n, 8P), from the complex128 Green matrices."""
+        B, n = F.B, F.n
+        P = X.shape[2] // 8
+        G64, H64 = self.green64(F)
+        out = torch.empty_like(X)
+        Yw = torch.empty((B, n, 16 * P), dtype=torch.complex128, device=self.device)
+        nat.call("mnpbem_bem_residual_batched", n, P, nat.ptr(G64[0]), nat.ptr(H64[0]), nat.ptr(G64[1]),
+                 nat.ptr(H64[1]), n, n * n, None, nat.ptr(X), nat.ptr(out), 8 * P, n * 8 * P, nat.ptr(Yw), 16 * P,
+                 n * 16 * P, nat.ptr(self.g.nvec), nat.ptr(F.par), None, B, nat.stream_handle(self.device))
+        return out
+
     def _residual(self, F, R, X, Rres, Y, nrm, P):
         """Rres = R - A X in complex128 and the squared norms per wavelength;
         Y = [G1 X1 | H1 X1 | G2 X2 | H2 X2] work space."""

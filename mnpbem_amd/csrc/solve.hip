@@ -1110,3 +1110,572 @@ extern "C" int mnpbem_lu_blockinv_batched(int dtype, int n, const void* LU, long
   MNP_LAUNCH_CHECK();
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Adjoint (transposed) solve A^T y = c with the factors of the direct solve.
+//
+// From the operator of GdA&H Eqs. (10, 11, 14, 17) (see k_residual),
+//   (A^T y)_s1 =  G1^T (y_phi - ik e1 n.y_al) + e1 H1^T y_De
+//   (A^T y)_h1 =  G1^T (y_a - ik e1 n y_De)   +    H1^T y_al
+//   (A^T y)_s2 = -G2^T (y_phi - ik e2 n.y_al) - e2 H2^T y_De
+//   (A^T y)_h2 = -G2^T (y_a - ik e2 n y_De)   -    H2^T y_al
+// Multiplying by G^-T and eliminating as in the forward solve
+// (bem_ret.py:1694-1810 transposed) gives, with g1 = G1^-T c1, g2 = G2^-T c2,
+// gs = g1_s + g2_s, gh = g1_h + g2_h and L = e1 - e2:
+//   y_De = Sigma^-T (gs + ik L n.(Delta^-T gh))
+//   y_al = Delta^-T (gh + ik L n y_De)
+//   y_phi = g1_s + ik e1 n.y_al - e1 (Sigma1^T y_De)
+//   y_a   = g1_h + ik e1 n y_De - Sigma1^T y_al
+// The transposed products are plain row-major GEMVs over the stored buffers
+// (buf = M^T), and M^-T v is the ordinary LU solve with the stored LU of M^T.
+// y is returned in the right-side layout [phi | a | De | alpha], c is given in
+// the unknowns' layout [sig1 | h1 | sig2 | h2].
+// ---------------------------------------------------------------------------
+namespace {
+
+// Y[b] = beta Y[b] + buf[b] X[b], y_i = sum_j buf[i*ld + j] x_j (the product
+// with M^T for the stored buffer of M).  One wavefront per row with the lanes
+// along the row (coalesced); X chunks of 64 rows staged column-major in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_n(int n, int C, const T* __restrict__ M, long ldm, long sM,
+                                                const T* __restrict__ X, long ldx, long sX, T* __restrict__ Y,
+                                                long ldy, long sY, double beta) {
+  const long b = blockIdx.y;
+  M += 2 * b * sM;
+  X += 2 * b * sX;
+  Y += 2 * b * sY;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int i = blockIdx.x * 4 + (tid >> 6);
+  __shared__ cplx<T> xs[MAXC][64];
+  cplx<T> acc[MAXC];
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int nj = min(64, n - j0);
+    __syncthreads();
+    for (int e = tid; e < nj * C; e += 256) {
+      const int jj = e / C, c = e % C;
+      xs[c][jj] = cld(X + 2 * ((long)(j0 + jj) * ldx + c));
+    }
+    __syncthreads();
+    if (i < n && lane < nj) {
+      const cplx<T> m = cld(M + 2 * ((long)i * ldm + j0 + lane));
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c)
+        if (c < C) acc[c] = cfma(acc[c], m, xs[c][lane]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c)
+    if (c < C) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        acc[c].re += __shfl_xor(acc[c].re, off);
+        acc[c].im += __shfl_xor(acc[c].im, off);
+      }
+    }
+  if (lane == 0 && i < n) {
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c)
+      if (c < C) {
+        T* py = Y + 2 * ((long)i * ldy + c);
+        double yr = (double)acc[c].re, yi = (double)acc[c].im;
+        if (beta != 0.0) {
+          const cplx<T> y0 = cld(py);
+          yr += beta * (double)y0.re;
+          yi += beta * (double)y0.im;
+        }
+        cst(py, cmake<T>((T)yr, (T)yi));
+      }
+  }
+}
+
+// One 64-row block step of the solve B x = v with the row-major LU of the
+// buffer B (P B = L U, x = U^-1 L^-1 P v; P applied beforehand by k_perm_n).
+//   FWD (L z = v, unit lower): block rhs in V; z_I -> Z; V rows below -= L_KI z_I.
+//   BWD (U x = z, upper): block rhs in Z; x_I -> V; Z rows above -= U_KI x_I.
+// Dinv, when given, holds the inverse diagonal blocks [U^-1 | L^-1].
+template <typename T, bool FWD>
+__global__ __launch_bounds__(256) void k_trsv_n_step(int n, int C, int i0, const T* __restrict__ LU, long ld,
+                                                     long sLU, T* V, long ldv, long sV, T* Z, long ldz, long sZ,
+                                                     const T* __restrict__ Dinv, long sD) {
+  const long b = blockIdx.y;
+  LU += 2 * b * sLU;
+  V += 2 * b * sV;
+  Z += 2 * b * sZ;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int nb = min(64, n - i0), i1 = i0 + nb;
+  __shared__ cplx<T> blk[64][65];
+  __shared__ cplx<T> sol[64][MAXC];
+  const T* src = FWD ? V : Z;
+  const long lds = FWD ? ldv : ldz;
+  if (Dinv) {
+    const T* D = Dinv + 2 * (b * sD + ((long)(i0 / 64) * 2 + (FWD ? 1 : 0)) * 64 * 64);
+    for (int e = tid; e < 64 * C; e += 256) {
+      const int r = e / C, c = e % C;
+      blk[r][c] = r < nb ? cld(src + 2 * ((long)(i0 + r) * lds + c)) : cmake<T>(0, 0);
+    }
+    __syncthreads();
+    // thread (row r, column group cg): row r of the inverse block
+    const int r = tid & 63, cg = tid >> 6;
+    cplx<T> drow[64];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) drow[q] = (q < nb && r < nb) ? cld(D + 2 * ((long)r * 64 + q)) : cmake<T>(0, 0);
+    for (int c = cg; c < C; c += 4) {
+      cplx<T> acc = cmake<T>(0, 0);
+#pragma unroll
+      for (int q = 0; q < 64; ++q) acc = cfma(acc, drow[q], blk[q][c]);
+      sol[r][c] = acc;
+    }
+    __syncthreads();
+  } else {
+    for (int e = tid; e < nb * nb; e += 256) {
+      const int r = e / nb, q = e % nb;
+      blk[r][q] = cld(LU + 2 * ((long)(i0 + r) * ld + i0 + q));
+    }
+    __syncthreads();
+    if (tid < 64) {
+      cplx<T> val[MAXC];
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c)
+        val[c] = (c < C && lane < nb) ? cld(src + 2 * ((long)(i0 + lane) * lds + c)) : cmake<T>(0, 0);
+      if (FWD) {
+        for (int i = 0; i < nb; ++i) {
+          const cplx<T> l = blk[lane < nb ? lane : 0][i];  // L[r][i]
+#pragma unroll
+          for (int c = 0; c < MAXC; ++c)
+            if (c < C) {
+              cplx<T> zi;
+              zi.re = __shfl(val[c].re, i);
+              zi.im = __shfl(val[c].im, i);
+              if (lane > i && lane < nb) val[c] = cfms(val[c], l, zi);
+            }
+        }
+      } else {
+        for (int i = nb - 1; i >= 0; --i) {
+          if (lane == i) {
+            const cplx<T> inv = crecip(blk[i][i]);
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) val[c] = cmul(val[c], inv);
+          }
+          const cplx<T> u = blk[lane < nb ? lane : 0][i];  // U[r][i], r < i
+#pragma unroll
+          for (int c = 0; c < MAXC; ++c)
+            if (c < C) {
+              cplx<T> xi;
+              xi.re = __shfl(val[c].re, i);
+              xi.im = __shfl(val[c].im, i);
+              if (lane < i) val[c] = cfms(val[c], u, xi);
+            }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c) sol[lane][c] = val[c];
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0) {
+    T* dst = FWD ? Z : V;
+    const long ldd_ = FWD ? ldz : ldv;
+    for (int e = tid; e < nb * C; e += 256) {
+      const int rr = e / C, c = e % C;
+      cst(dst + 2 * ((long)(i0 + rr) * ldd_ + c), sol[rr][c]);
+    }
+  }
+  // tile update: FWD rows k >= i1, BWD rows k < i0; row k reads its own
+  // 64-entry segment LU[k][i0:i1]
+  const int k = (FWD ? i1 : 0) + blockIdx.x * 256 + tid;
+  const int kend = FWD ? n : i0;
+  if (k < kend) {
+    T* upd = FWD ? V : Z;
+    const long ldu = FWD ? ldv : ldz;
+    const T* row = LU + 2 * ((long)k * ld + i0);
+    cplx<T> acc[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
+    int l = 0;
+    for (; l + 16 <= nb; l += 16) {
+      cplx<T> m[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) m[u] = cld(row + 2 * (l + u));
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < C) acc[c] = cfma(acc[c], m[u], sol[l + u][c]);
+    }
+    for (; l < nb; ++l) {
+      const cplx<T> m = cld(row + 2 * l);
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c)
+        if (c < C) acc[c] = cfma(acc[c], m, sol[l][c]);
+    }
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c)
+      if (c < C) {
+        T* pu = upd + 2 * ((long)k * ldu + c);
+        cst(pu, csub(cld(pu), acc[c]));
+      }
+  }
+}
+
+// v = P v: row interchanges ipiv[0], ..., ipiv[n-1] in order; one workgroup
+// per matrix, one thread per column.
+template <typename T>
+__global__ __launch_bounds__(256) void k_perm_n(int n, int C, const int* __restrict__ ipiv, long sP, T* V, long ldv,
+                                                long sV) {
+  const long b = blockIdx.x;
+  ipiv += b * sP;
+  V += 2 * b * sV;
+  const int tid = threadIdx.x;
+  __shared__ int sp[256];
+  for (int lo = 0; lo < n; lo += 256) {
+    const int hi = min(n, lo + 256);
+    __syncthreads();
+    if (tid < hi - lo) sp[tid] = ipiv[lo + tid];
+    __syncthreads();
+    if (tid < C) {
+      for (int i = lo; i < hi; ++i) {
+        const int p = sp[i - lo];
+        if (p != i) {
+          T* x = V + 2 * ((long)i * ldv + tid);
+          T* y = V + 2 * ((long)p * ldv + tid);
+          const cplx<T> u = cld(x), v = cld(y);
+          cst(x, v);
+          cst(y, u);
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+int gemv_n(int n, int C, const T* M, long ldm, long sM, const T* X, long ldx, long sX, T* Y, long ldy, long sY,
+           double beta, int batch, hipStream_t st) {
+  for (int c0 = 0; c0 < C; c0 += MAXC) {
+    const int cc = std::min(MAXC, C - c0);
+    hipLaunchKernelGGL(k_gemv_n<T>, dim3(mnp_ceil_div(n, 4), batch), dim3(256), 0, st, n, cc, M, ldm, sM,
+                       X + 2 * c0, ldx, sX, Y + 2 * c0, ldy, sY, beta);
+    MNP_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+// V <- M^-T V (n x C) with the LU of M^T; Z (n x C) is work space.
+template <typename T>
+int lu_solve_n(int n, int C, const T* LU, long ld, long sLU, const int* ipiv, long sP, T* V, long ldv, long sV,
+               T* Z, long ldz, long sZ, int batch, hipStream_t st, const T* Dinv = nullptr, long sD = 0) {
+  const int nblk = (n + 63) / 64;
+  for (int c0 = 0; c0 < C; c0 += MAXC) {
+    const int cc = std::min(MAXC, C - c0);
+    T* v = V + 2 * c0;
+    T* z = Z + 2 * c0;
+    hipLaunchKernelGGL(k_perm_n<T>, dim3(batch), dim3(256), 0, st, n, cc, ipiv, sP, v, ldv, sV);
+    MNP_LAUNCH_CHECK();
+    for (int ib = 0; ib < nblk; ++ib) {
+      const int i0 = ib * 64, i1 = std::min(n, i0 + 64);
+      dim3 grid(std::max(1, mnp_ceil_div(n - i1, 256)), batch);
+      hipLaunchKernelGGL((k_trsv_n_step<T, true>), grid, dim3(256), 0, st, n, cc, i0, LU, ld, sLU, v, ldv, sV, z,
+                         ldz, sZ, Dinv, sD);
+      MNP_LAUNCH_CHECK();
+    }
+    for (int ib = nblk - 1; ib >= 0; --ib) {
+      const int i0 = ib * 64;
+      dim3 grid(std::max(1, mnp_ceil_div(i0, 256)), batch);
+      hipLaunchKernelGGL((k_trsv_n_step<T, false>), grid, dim3(256), 0, st, n, cc, i0, LU, ld, sLU, v, ldv, sV, z,
+                         ldz, sZ, Dinv, sD);
+      MNP_LAUNCH_CHECK();
+    }
+  }
+  return 0;
+}
+
+// Y = C (n x cols per wavelength, same layout)
+template <typename T>
+__global__ void k_adj_copy(int n, int cols, const T* C, T* Y, long ld, long stride) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long o = 2 * (b * stride + (long)i * ld);
+  for (int c = 0; c < cols; ++c) cst(Y + o + 2 * c, cld(C + o + 2 * c));
+}
+
+// a2: W[:, 0:P] = g1_s + g2_s, W[:, P:4P] = g1_h + g2_h  (Y holds [g1 | g2])
+template <typename T>
+__global__ void k_adj_sum(int n, int P, const T* Y, T* W, long ld, long stride) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T* y = Y + 2 * (b * stride + (long)i * ld);
+  T* w = W + 2 * (b * stride + (long)i * ld);
+  for (int c = 0; c < 4 * P; ++c) cst(w + 2 * c, cadd(cld(y + 2 * c), cld(y + 2 * (4 * P + c))));
+}
+
+// a4: W[:, 4P:5P] = gs + ik L n.u,  u = W[:, 5P:8P]
+template <typename T>
+__global__ void k_adj_rhs(int n, int P, T* W, long ld, long stride, const double* nvec, const double* par) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Par q = load_par(par, b);
+  const cplx<double> L = csub(q.e1, q.e2);
+  T* w = W + 2 * (b * stride + (long)i * ld);
+  for (int p = 0; p < P; ++p) {
+    cplx<double> s = cmake(0.0, 0.0);
+    for (int c = 0; c < 3; ++c) s = cadd(s, cscale(ldd(w + 2 * (5 * P + c * P + p)), nvec[3 * i + c]));
+    std_(w + 2 * (4 * P + p), cadd(ldd(w + 2 * p), ik_times(q.k, cmul(L, s))));
+  }
+}
+
+// a6: W[:, P:4P] = gh + ik L n y_De,  y_De = W[:, 4P:5P]
+template <typename T>
+__global__ void k_adj_t(int n, int P, T* W, long ld, long stride, const double* nvec, const double* par) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Par q = load_par(par, b);
+  const cplx<double> L = csub(q.e1, q.e2);
+  T* w = W + 2 * (b * stride + (long)i * ld);
+  for (int p = 0; p < P; ++p) {
+    const cplx<double> ly = cmul(L, ldd(w + 2 * (4 * P + p)));
+    for (int c = 0; c < 3; ++c) {
+      T* o = w + 2 * (P + c * P + p);
+      std_(o, cadd(ldd(o), ik_times(q.k, cscale(ly, nvec[3 * i + c]))));
+    }
+  }
+}
+
+// a8: Y = [g1_s + ik e1 n.y_al - e1 w_De | g1_h + ik e1 n y_De - w_al | y_De | y_al]
+// with W = [w_De | w_al | y_De | y_al]
+template <typename T>
+__global__ void k_adj_final(int n, int P, T* Y, const T* W, long ld, long stride, const double* nvec,
+                            const double* par) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Par q = load_par(par, b);
+  T* y = Y + 2 * (b * stride + (long)i * ld);
+  const T* w = W + 2 * (b * stride + (long)i * ld);
+  for (int p = 0; p < P; ++p) {
+    const cplx<double> yde = ldd(w + 2 * (4 * P + p));
+    cplx<double> nya = cmake(0.0, 0.0);
+    for (int c = 0; c < 3; ++c) {
+      const double nc = nvec[3 * i + c];
+      const cplx<double> yal = ldd(w + 2 * (5 * P + c * P + p));
+      nya = cadd(nya, cscale(yal, nc));
+      T* o = y + 2 * (P + c * P + p);
+      std_(o, csub(cadd(ldd(o), ik_times(q.k, cmul(q.e1, cscale(yde, nc)))), ldd(w + 2 * (P + c * P + p))));
+      std_(y + 2 * (5 * P + c * P + p), yal);
+    }
+    std_(y + 2 * p, csub(cadd(ldd(y + 2 * p), ik_times(q.k, cmul(q.e1, nya))), cmul(q.e1, ldd(w + 2 * p))));
+    std_(y + 2 * (4 * P + p), yde);
+  }
+}
+
+// Z = [y_phi - ik e1 n.y_al | y_a - ik e1 n y_De | y_phi - ik e2 n.y_al | y_a - ik e2 n y_De]
+__global__ void k_adj_z(int n, int P, const double* Yv, double* Z, long ld, long stride, const double* nvec,
+                        const double* par) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Par q = load_par(par, b);
+  const double* y = Yv + 2 * (b * stride + (long)i * ld);
+  double* z = Z + 2 * (b * stride + (long)i * ld);
+  for (int p = 0; p < P; ++p) {
+    const cplx<double> yphi = ldd(y + 2 * p), yde = ldd(y + 2 * (4 * P + p));
+    cplx<double> nya = cmake(0.0, 0.0);
+    for (int c = 0; c < 3; ++c) nya = cadd(nya, cscale(ldd(y + 2 * (5 * P + c * P + p)), nvec[3 * i + c]));
+    for (int s = 0; s < 2; ++s) {
+      const cplx<double> e = s == 0 ? q.e1 : q.e2;
+      const int off = 4 * P * s;
+      std_(z + 2 * (off + p), csub(yphi, ik_times(q.k, cmul(e, nya))));
+      for (int c = 0; c < 3; ++c)
+        std_(z + 2 * (off + P + c * P + p),
+             csub(ldd(y + 2 * (P + c * P + p)), ik_times(q.k, cmul(e, cscale(yde, nvec[3 * i + c])))));
+    }
+  }
+}
+
+// T = [G1^T Z1 | H1^T Wy | G2^T Z2 | H2^T Wy] (n x 16P), Wy = [y_De | y_al];
+// Out = C - A^T y (norms as in k_residual), or A^T y when C is null.
+__global__ void k_adj_combine(int n, int P, const double* C, const double* T, double* Out, long ldv, long sv,
+                              long ldt, long stt, const double* par, double* nrm) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc_r = 0.0, acc_b = 0.0;
+  if (i < n) {
+    const Par q = load_par(par, b);
+    const double* t = T + 2 * (b * stt + (long)i * ldt);
+    const double* c = C ? C + 2 * (b * sv + (long)i * ldv) : nullptr;
+    double* o = Out + 2 * (b * sv + (long)i * ldv);
+    const int P4 = 4 * P;
+    auto put = [&](int col, cplx<double> v) {
+      if (c == nullptr) {
+        std_(o + 2 * col, v);
+        return;
+      }
+      const cplx<double> cv = ldd(c + 2 * col);
+      const cplx<double> d = csub(cv, v);
+      acc_r += d.re * d.re + d.im * d.im;
+      acc_b += cv.re * cv.re + cv.im * cv.im;
+      std_(o + 2 * col, d);
+    };
+    for (int col = 0; col < P4; ++col) {
+      const cplx<double> g1 = ldd(t + 2 * col), h1 = ldd(t + 2 * (P4 + col));
+      const cplx<double> g2 = ldd(t + 2 * (2 * P4 + col)), h2 = ldd(t + 2 * (3 * P4 + col));
+      if (col < P) {
+        put(col, cadd(g1, cmul(q.e1, h1)));
+        put(P4 + col, cmake(-(g2.re + cmul(q.e2, h2).re), -(g2.im + cmul(q.e2, h2).im)));
+      } else {
+        put(col, cadd(g1, h1));
+        put(P4 + col, cmake(-(g2.re + h2.re), -(g2.im + h2.im)));
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    acc_r += __shfl_xor(acc_r, off);
+    acc_b += __shfl_xor(acc_b, off);
+  }
+  if (nrm != nullptr && (threadIdx.x & 63) == 0) {
+    atomicAdd(nrm + 2 * b, acc_r);
+    atomicAdd(nrm + 2 * b + 1, acc_b);
+  }
+}
+
+}  // namespace
+
+// Adjoint solve Y = A^-T C for ``batch`` wavelengths with the factors of
+// mnpbem_bem_solve_batched (same arguments; C in the unknowns' layout, Y in
+// the right-side layout).  W is work space of the shape of C.
+extern "C" int mnpbem_bem_solve_adjoint_batched(int dtype, int n, int P, const void* G1t, const int* piv1,
+                                                const void* G2t, const int* piv2, const void* S1t, const void* Et,
+                                                const void* St, const int* pivS, long ldm, long smat, long spiv,
+                                                const void* D1, const void* D2, const void* DS, long sD,
+                                                const void* C, void* W, void* Y, long ldv, long sv,
+                                                const double* nvec, const double* par, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldv >= 8 * P, "bem_solve_adjoint: bad size (P <= 4)");
+  if (batch <= 0) return 0;
+  dim3 eg(mnp_ceil_div(n, 256), batch);
+#define MNP_ADJ(T)                                                                                                 \
+  {                                                                                                                \
+    T* y = (T*)Y;                                                                                                  \
+    T* w = (T*)W;                                                                                                  \
+    int rc;                                                                                                        \
+    hipLaunchKernelGGL(k_adj_copy<T>, eg, dim3(256), 0, st, n, 8 * P, (const T*)C, y, ldv, sv);                    \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    /* a1: Y = [G1^-T c1 | G2^-T c2] */                                                                            \
+    if ((rc = lu_solve_n<T>(n, 4 * P, (const T*)G1t, ldm, smat, piv1, spiv, y, ldv, sv, w, ldv, sv, batch, st,    \
+                            (const T*)D1, sD)))                                                                    \
+      return rc;                                                                                                   \
+    if ((rc = lu_solve_n<T>(n, 4 * P, (const T*)G2t, ldm, smat, piv2, spiv, y + 2 * 4 * P, ldv, sv,                \
+                            w + 2 * 4 * P, ldv, sv, batch, st, (const T*)D2, sD)))                                 \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_adj_sum<T>, eg, dim3(256), 0, st, n, P, y, w, ldv, sv);                                   \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    /* a3: u = Delta^-T gh -> W[:, 5P:8P] */                                                                       \
+    if ((rc = gemv_n<T>(n, 3 * P, (const T*)Et, ldm, smat, w + 2 * P, ldv, sv, w + 2 * 5 * P, ldv, sv, 0.0,      \
+                        batch, st)))                                                                               \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_adj_rhs<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                           \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    /* a5: y_De = Sigma^-T rhs in W[:, 4P:5P] */                                                                   \
+    if ((rc = lu_solve_n<T>(n, P, (const T*)St, ldm, smat, pivS, spiv, w + 2 * 4 * P, ldv, sv, w, ldv, sv, batch, \
+                            st, (const T*)DS, sD)))                                                                \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_adj_t<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                             \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    /* a6: y_al = Delta^-T t -> W[:, 5P:8P] */                                                                     \
+    if ((rc = gemv_n<T>(n, 3 * P, (const T*)Et, ldm, smat, w + 2 * P, ldv, sv, w + 2 * 5 * P, ldv, sv, 0.0,      \
+                        batch, st)))                                                                               \
+      return rc;                                                                                                   \
+    /* a7: W[:, 0:4P] = Sigma1^T [y_De | y_al] */                                                                  \
+    if ((rc = gemv_n<T>(n, 4 * P, (const T*)S1t, ldm, smat, w + 2 * 4 * P, ldv, sv, w, ldv, sv, 0.0, batch, st))) \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_adj_final<T>, eg, dim3(256), 0, st, n, P, y, w, ldv, sv, nvec, par);                      \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    return 0;                                                                                                      \
+  }
+  if (dtype == MNP_DTYPE_C128) MNP_ADJ(double)
+  if (mnp_is_c64(dtype)) MNP_ADJ(float)
+#undef MNP_ADJ
+  mnp_set_error("bem_solve_adjoint: unknown dtype");
+  return 1;
+}
+
+// Out = C - A^T Y in complex128 (nrm: ||C - A^T Y||^2 and ||C||^2 per
+// wavelength, may be NULL; with C NULL, Out = A^T Y).  G1t.. are the
+// complex128 transposed Green matrices; Z (batch, n, 8P) and T (batch, n,
+// 16P) are work space.
+extern "C" int mnpbem_bem_residual_adjoint_batched(int n, int P, const void* G1t, const void* H1t, const void* G2t,
+                                                   const void* H2t, long ldm, long smat, const void* C,
+                                                   const void* Yv, void* Out, long ldv, long sv, void* Z, void* T,
+                                                   long ldt, long stt, const double* nvec, const double* par,
+                                                   double* nrm, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldt >= 16 * P && ldv >= 8 * P, "bem_residual_adjoint: bad size");
+  if (batch <= 0) return 0;
+  if (nrm != nullptr) {
+    MNP_REQUIRE(C != nullptr, "bem_residual_adjoint: norms need C");
+    hipLaunchKernelGGL(k_fill_double, dim3(mnp_ceil_div(2L * batch, 256)), dim3(256), 0, st, 2L * batch, nrm, 0.0);
+    MNP_LAUNCH_CHECK();
+  }
+  const double* y = (const double*)Yv;
+  double* z = (double*)Z;
+  double* t = (double*)T;
+  const int P4 = 4 * P;
+  dim3 eg(mnp_ceil_div(n, 256), batch);
+  hipLaunchKernelGGL(k_adj_z, eg, dim3(256), 0, st, n, P, y, z, ldv, sv, nvec, par);
+  MNP_LAUNCH_CHECK();
+  int rc;
+  if ((rc = gemv_n<double>(n, P4, (const double*)G1t, ldm, smat, z, ldv, sv, t, ldt, stt, 0.0, batch, st)))
+    return rc;
+  if ((rc = gemv_n<double>(n, P4, (const double*)H1t, ldm, smat, y + 2 * P4, ldv, sv, t + 2 * P4, ldt, stt, 0.0,
+                           batch, st)))
+    return rc;
+  if ((rc = gemv_n<double>(n, P4, (const double*)G2t, ldm, smat, z + 2 * P4, ldv, sv, t + 2 * 2 * P4, ldt, stt,
+                           0.0, batch, st)))
+    return rc;
+  if ((rc = gemv_n<double>(n, P4, (const double*)H2t, ldm, smat, y + 2 * P4, ldv, sv, t + 2 * 3 * P4, ldt, stt,
+                           0.0, batch, st)))
+    return rc;
+  hipLaunchKernelGGL(k_adj_combine, eg, dim3(256), 0, st, n, P, (const double*)C, t, (double*)Out, ldv, sv, ldt,
+                     stt, par, nrm);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Y[b] = buf[b] X[b] (the product with M^T for the stored buffer of M), beta
+// as in mnpbem_gemv_t_batched.
+extern "C" int mnpbem_gemv_n_batched(int dtype, int n, int C, const void* M, long ldm, long sM, const void* X,
+                                     long ldx, long sX, void* Y, long ldy, long sY, double beta, int batch,
+                                     void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (n <= 0 || C <= 0 || batch <= 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    return gemv_n<double>(n, C, (const double*)M, ldm, sM, (const double*)X, ldx, sX, (double*)Y, ldy, sY, beta,
+                          batch, st);
+  if (mnp_is_c64(dtype))
+    return gemv_n<float>(n, C, (const float*)M, ldm, sM, (const float*)X, ldx, sX, (float*)Y, ldy, sY, beta, batch,
+                         st);
+  mnp_set_error("gemv_n: unknown dtype");
+  return 1;
+}
+
+// V <- M^-T V with the LU of M^T (mnpbem_lu_solve_t_batched's transpose).
+extern "C" int mnpbem_lu_solve_n_batched(int dtype, int n, int C, const void* LU, long ld, long sLU,
+                                         const int* ipiv, long sP, void* V, long ldv, long sV, void* Z, long ldz,
+                                         long sZ, const void* Dinv, long sD, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (n <= 0 || C <= 0 || batch <= 0) return 0;
+  MNP_REQUIRE(ldz >= C, "lu_solve_n: work space narrower than the right-hand side");
+  if (dtype == MNP_DTYPE_C128)
+    return lu_solve_n<double>(n, C, (const double*)LU, ld, sLU, ipiv, sP, (double*)V, ldv, sV, (double*)Z, ldz,
+                              sZ, batch, st, (const double*)Dinv, sD);
+  if (mnp_is_c64(dtype))
+    return lu_solve_n<float>(n, C, (const float*)LU, ld, sLU, ipiv, sP, (float*)V, ldv, sV, (float*)Z, ldz, sZ,
+                             batch, st, (const float*)Dinv, sD);
+  mnp_set_error("lu_solve_n: unknown dtype");
+  return 1;
+}
