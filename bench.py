@@ -97,16 +97,22 @@ def main():
                          "recomputed on the fly")
     ap.add_argument("--refine-tol", type=float, default=1e-12,
                     help="FP64 refinement stops when ||R - A X|| <= tol ||R|| for every wavelength")
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "uneven", "dimer"],
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "uneven", "dimer", "adjoint"],
                     help="sphere: the headline 64-wavelength sweep of the 2884-face gold sphere (default); "
                          "uneven: 32 seeded spheres/ellipsoids of 284-2884 faces, 64 wavelengths each; "
-                         "dimer: two 12500-face spheres, 2 nm gap, H-matrix GMRES at 500/600/700 nm")
+                         "dimer: two 12500-face spheres, 2 nm gap, H-matrix GMRES at 500/600/700 nm; "
+                         "adjoint: extinction and its vertex / Drude gradients of a 2880-face gold rod, "
+                         "21 wavelengths 600-800 nm")
+    ap.add_argument("--fd-vertices", type=int, default=20,
+                    help="adjoint workload: random vertices checked against central differences")
     ap.add_argument("--seed", type=int, default=0, help="seed of the uneven-particle set")
     args = ap.parse_args()
     if args.gpus == 1 and args.workload == "uneven":
         return bench_uneven(args)
     if args.gpus == 1 and args.workload == "dimer":
         return bench_dimer(args)
+    if args.gpus == 1 and args.workload == "adjoint":
+        return bench_adjoint(args)
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the sweep shards by wavelength across GPUs "
               "(see DESIGN.md) but no multi-GPU figure is produced", file=sys.stderr)
@@ -384,6 +390,86 @@ def bench_dimer(args):
                         "bytes_per_product": round(mv_bytes), "avg_ms": round(mv_ms, 4),
                         "achieved_GBps": round(mv_gbs, 1), "peak_GBps": 8000.0,
                         "frac": round(mv_gbs / 8000.0, 4)}}
+    print(json.dumps(line), flush=True)
+
+
+def bench_adjoint(args):
+    """Gold nanorod (trirod(20, 60, [31, 13, 26]) = 2880 flat triangles,
+    hemispherical caps) in eps = 1.77, plane wave polarised along the rod axis
+    (z), propagating along x, 21 wavelengths 600-800 nm across the
+    longitudinal resonance.  One step = extinction plus its gradient with
+    respect to every vertex (forward solve, adjoint solve with the same
+    factors, derivative kernels, chain rule), mixed precision.  Also timed:
+    the forward sweep alone and the Drude-parameter gradient.  Outside the
+    timed region the vertex gradient is checked against central differences
+    (step 1e-4 nm, complex128 spectra) on --fd-vertices random vertices at
+    the resonance wavelength."""
+    import torch
+    import mnpbem_amd as m
+    from mnpbem_amd.adjoint import extinction_with_drude_gradient, extinction_with_shape_gradient
+    rod = m.trirod(20.0, 60.0, [31, 13, 26])
+    mk = lambda verts: m.ComParticle([m.EpsConst(1.77), m.EpsDrude(*DRUDE)],
+                                     [m.geometry.Particle(verts, rod.faces, interp="flat")], [[2, 1]], 1)
+    cp = mk(rod.verts)
+    lams = np.linspace(600.0, 800.0, 21)
+    pw = m.PlaneWaveRet([[0.0, 0.0, 1.0]], [[1.0, 0.0, 0.0]])
+    bem = m.BEMRet(cp, precision="mixed")
+    log("adjoint workload: rod N = %d, %d vertices" % (cp.n, len(rod.verts)))
+    for _ in range(max(args.warmup, 1)):
+        extinction_with_shape_gradient(bem, lams, pw, chunk=len(lams))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ext, grads = extinction_with_shape_gradient(bem, lams, pw, chunk=len(lams))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    gv = grads[0]["verts"]
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ext_f = bem.solve_sweep(lams, pw, chunk=len(lams))
+    torch.cuda.synchronize()
+    dt_fwd = (time.perf_counter() - t1) / args.steps
+    t2 = time.perf_counter()
+    _, gd = extinction_with_drude_gradient(bem, lams, pw, chunk=len(lams))
+    torch.cuda.synchronize()
+    dt_drude = time.perf_counter() - t2
+    k_res = int(np.argmax(ext[:, 0]))
+    lam_res = lams[k_res]
+    log("sweep + vertex gradient %.3f s, forward only %.3f s, Drude gradient %.3f s; resonance %.0f nm"
+        % (dt, dt_fwd, dt_drude, lam_res))
+    # central differences on random vertices at the resonance (complex128)
+    rng = np.random.default_rng(0)
+    h = 1e-4
+    scale = np.abs(gv[k_res, 0]).max()
+    worst = 0.0
+    t3 = time.perf_counter()
+    for vi in rng.choice(len(rod.verts), args.fd_vertices, replace=False):
+        for c in range(3):
+            vals = []
+            for sg in (1.0, -1.0):
+                v = rod.verts.copy()
+                v[vi, c] += sg * h
+                vals.append(m.BEMRet(mk(v)).solve_sweep([lam_res], pw)[0, 0])
+            fd = (vals[0] - vals[1]) / (2 * h)
+            worst = max(worst, abs(gv[k_res, 0, vi, c] - fd) / scale)
+    log("finite-difference check: %d vertices x 3, max |grad - fd| / max|grad| = %.2e (%.1f s)"
+        % (args.fd_vertices, worst, time.perf_counter() - t3))
+    line = {"metric": "wavelengths/s with d(ext)/d(vertex xyz), 2880-face gold rod (config 4)",
+            "value": round(len(lams) / dt, 4), "unit": "wavelengths/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": max(args.warmup, 1), "ms_per_step": round(1000.0 * dt, 1), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "c64(bf16x3 GEMM)+f64-refine, f64 gradient kernels",
+            "data": "synthetic",
+            "config": {"workload": "trirod(20, 60, [31, 13, 26]) gold rod in eps 1.77, pol z, dir x, "
+                                   "21 wavelengths 600-800 nm, mixed precision",
+                       "faces": cp.n, "vertices": len(rod.verts), "wavelengths": len(lams)},
+            "forward_only_value": round(len(lams) / dt_fwd, 4),
+            "gradient_overhead": round(dt / dt_fwd, 3),
+            "drude_gradient_value": round(len(lams) / dt_drude, 4),
+            "resonance_nm": float(lam_res),
+            "fd_check": {"vertices": args.fd_vertices, "step_nm": h, "wavelength_nm": float(lam_res),
+                         "max_abs_err_over_max_grad": worst},
+            "max_rel_ext_err_grad_vs_forward": float(np.max(np.abs(ext / ext_f - 1))),
+            "drude_grad_at_resonance": [float(x) for x in gd[k_res, 0]]}
     print(json.dumps(line), flush=True)
 
 

@@ -187,6 +187,42 @@ int mnpbem_bem_residual_adjoint_batched(int n, int P, const void* G1t, const voi
                                         long ldv, long sv, void* Z, void* T, long ldt, long stt, const double* nvec,
                                         const double* par, double* nrm, int batch, void* stream);
 
+/* ---- geometry derivatives of the extinction (no counterpart in the
+ *      reference; they differentiate greenret_refined.py:122-810,
+ *      compgreen_ret.py:192-400, planewave_ret.py:257-504 and
+ *      spectrum_ret.py:231-392 term by term) ----
+ * Per-face gradient buffer Gr (batch, P, n, 31) float64, zeroed by the caller:
+ * [pos 3 | nvec 3 | tvec1 3 | tvec2 3 | area 1 | nodes 18].  L (batch, n, 2, P,
+ * 8), R (batch, n, 2, P, 4), Wc (batch, n, P) complex128 weights from
+ * mnpbem_grad_prep.  P <= 4. */
+/* C = kappa c, the optical-theorem functional of mnpbem_extinction_batched
+ * written as a vector in the unknowns' layout (ext = Im(C . X)). */
+int mnpbem_ext_functional(int n, int P, const double* pos, const double* area, const int* in_med,
+                          const int* out_med, const double* pol, const double* dir, const double* pw, void* C,
+                          long ldv, long sv, int batch, void* stream);
+/* weights of the four Green matrices in -Y . A X from the solution X and the adjoint Y */
+int mnpbem_grad_prep(int n, int P, const void* X, const void* Y, long ldv, long sv, const double* nvec,
+                     const double* par, void* L, void* R, void* Wc, int batch, void* stream);
+/* far-field pairs and the far terms of the closed-surface correction; rowptr
+ * (int64, n+1) / pcol and colptr (n+1) / crow: refined pairs by row and by column */
+int mnpbem_grad_far(int n, int P, const double* pos, const double* nvec, const double* area, const int* mat0,
+                    const int* mat1, const int* surf, const double* kmat, int nmat, const long* rowptr,
+                    const int* pcol, const int* colptr, const int* crow, const void* L, const void* R,
+                    const void* Wc, double* Gr, int batch, void* stream);
+/* refined (near and self) pairs: the refinement quadrature in dual numbers */
+int mnpbem_grad_refined(int order, int n, int P, int npairs, const int* prow, const int* pcol, const int* pkind,
+                        const double* pos, const double* nvec, const double* tvec1, const double* tvec2,
+                        const double* area, const double* nodes, const int* mat0, const int* mat1, const int* surf,
+                        const double* kmat, int nmat, const double* qnear, int nq_near, const double* qself,
+                        int nq_self, const void* L, const void* R, const void* Wc, double* Gr, int batch,
+                        void* stream);
+/* per-face terms: explicit normals of A (Q = [G1 X1 | G2 X2]), the plane-wave
+ * right side Rb and the functional; corr: closed-surface correction per face */
+int mnpbem_grad_local(int n, int P, const void* X, const void* Y, const void* Q, const void* Rb, long ldv, long sv,
+                      const double* pos, const double* nvec, const double* area, const int* in_med,
+                      const int* out_med, const double* pol, const double* dir, const double* pw,
+                      const double* corr, const void* Wc, double* Gr, int batch, void* stream);
+
 /* Y = beta Y + buf X, y_i = sum_j buf[i*ldm + j] x_j: the product with M^T
  * for the transposed buffer of M (the transpose of mnpbem_gemv_t_batched). */
 int mnpbem_gemv_n_batched(int dtype, int n, int C, const void* M, long ldm, long sM, const void* X, long ldx,

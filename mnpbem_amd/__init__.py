@@ -14,7 +14,7 @@ __all__ = ["EV2NM", "EpsConst", "EpsDrude", "EpsTable", "ComParticle", "Particle
            "trisphere_fibonacci", "triellipsoid", "trirod",
            "CompGreenRet", "BEMRet", "CompStruct", "PlaneWaveRet", "BEMIter", "BEMRetIter", "ClusterTree",
            "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss", "DrudeExtinction",
-           "extinction_with_drude_gradient"]
+           "extinction_with_drude_gradient", "ShapeExtinction", "extinction_with_shape_gradient"]
 
 
 def __getattr__(name):
@@ -32,7 +32,8 @@ def __getattr__(name):
     if name in ("ClusterTree", "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss"):
         from . import hmatrix
         return getattr(hmatrix, name)
-    if name in ("DrudeExtinction", "extinction_with_drude_gradient"):
+    if name in ("DrudeExtinction", "extinction_with_drude_gradient", "ShapeExtinction",
+                "extinction_with_shape_gradient"):
         from . import adjoint
         return getattr(adjoint, name)
     if name == "PlaneWaveRet":

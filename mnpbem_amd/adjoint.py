@@ -123,3 +123,168 @@ class DrudeExtinction(torch.autograd.Function):
     def backward(ctx, grad_out):
         (grad,) = ctx.saved_tensors
         return (grad_out[:, :, None] * grad).sum(dim=(0, 1)), None, None, None
+
+
+# ---------------------------------------------------------------------------
+# Vertex-position gradient
+# ---------------------------------------------------------------------------
+
+_GR_SLOTS = 31  # per-face gradient layout of the kernels: pos 3 | nvec 3 | tvec1 3 | tvec2 3 | area 1 | nodes 18
+
+
+def _face_geometry_torch(X, flat):
+    """pos, nvec, tvec1, tvec2, area of 6-node faces X (n, 6, 3) as torch
+    functions of X, the same formulas as Particle._geometry (particle.py
+    _norm / _norm_flat; area by the 28-point rule of the surface Jacobian)."""
+    from .quadrature import rule18, tri6_tables
+    S, Dx, Dy, w = (torch.as_tensor(a, dtype=X.dtype, device=X.device) for a in tri6_tables(*rule18()))
+    tx = torch.einsum("qc,ncd->nqd", Dx, X)
+    ty = torch.einsum("qc,ncd->nqd", Dy, X)
+    jac = torch.linalg.vector_norm(torch.cross(tx, ty, dim=2), dim=2)
+    area = (0.5 * w[None, :] * jac).sum(dim=1)
+    S0, Dx0, Dy0, _ = (torch.as_tensor(a, dtype=X.dtype, device=X.device)
+                       for a in tri6_tables(np.array([1 / 3]), np.array([1 / 3]), np.array([1.0])))
+    pos = torch.einsum("c,ncd->nd", S0[0], X)
+    a = torch.einsum("c,ncd->nd", Dy0[0], X)
+    b = torch.einsum("c,ncd->nd", Dx0[0], X)
+    nv = torch.cross(a, b, dim=1)
+    nv = nv / torch.linalg.vector_norm(nv, dim=1, keepdim=True)
+    if flat:
+        t1 = X[:, 0, :] - X[:, 1, :]
+    else:
+        t1 = a
+    t1 = t1 / torch.linalg.vector_norm(t1, dim=1, keepdim=True)
+    t2 = torch.cross(nv, t1, dim=1)
+    return pos, nv, t1, t2, area
+
+
+def _chain_to_vertices(cp, Gr):
+    """Per-face gradients Gr (L, P, n, 31) -> per particle the gradient with
+    respect to its nodes verts2 (L, P, nverts2, 3) and, for a particle built
+    from corner triangles (edge midpoints derived), with respect to its
+    corners (L, P, nverts, 3)."""
+    L, P, n, _ = Gr.shape
+    out = []
+    G = torch.as_tensor(Gr, dtype=torch.float64).reshape(L * P, n, _GR_SLOTS)
+    for ip, part in enumerate(cp.p):
+        lo, hi = int(cp.offsets[ip]), int(cp.offsets[ip + 1])
+        g = G[:, lo:hi]
+        # grad mode on: this also runs inside autograd.Function.forward
+        with torch.enable_grad():
+            X = torch.as_tensor(part.nodes, dtype=torch.float64).requires_grad_(True)
+            outs = _face_geometry_torch(X, part.interp == "flat")
+            gouts = [g[:, :, 0:3], g[:, :, 3:6], g[:, :, 6:9], g[:, :, 9:12], g[:, :, 12]]
+            (gX,) = torch.autograd.grad(outs, X, grad_outputs=gouts, is_grads_batched=True)
+        gX = gX + g[:, :, 13:31].reshape(L * P, hi - lo, 6, 3)
+        nv2 = len(part.verts2)
+        gv2 = torch.zeros((L * P, nv2, 3), dtype=torch.float64)
+        idx = torch.as_tensor(part.faces6.reshape(-1), dtype=torch.int64)
+        gv2.index_add_(1, idx, gX.reshape(L * P, -1, 3))
+        gv2 = gv2.reshape(L, P, nv2, 3).numpy()
+        res = {"nodes": gv2}
+        if part.mid_edges is not None:
+            nv = len(part.verts)
+            gc = gv2[:, :, :nv].copy()
+            gm = 0.5 * gv2[:, :, nv:]
+            np.add.at(gc, (slice(None), slice(None), part.mid_edges[:, 0]), gm)
+            np.add.at(gc, (slice(None), slice(None), part.mid_edges[:, 1]), gm)
+            res["verts"] = gc
+        out.append(res)
+    return out
+
+
+def extinction_with_shape_gradient(bem, enei_list, planewave, chunk=16, per_face=False):
+    """Extinction (n_lambda, npol) and its derivatives with respect to the
+    mesh: one adjoint solve A^T y = c per wavelength with the factors of the
+    forward solve (c the optical-theorem functional), then the HIP kernels
+    contract y and x with the derivatives of the Green entries (far-field
+    pairs, refined near-field and self pairs, the closed-surface correction),
+    of the plane-wave right side and of c, per face geometric input; the
+    chain rule to the nodes runs in torch.  Returns (ext, grads) with grads a
+    list per particle of dicts: 'nodes' (n_lambda, npol, nverts2, 3) and, for
+    particles built from corner triangles, 'verts' (n_lambda, npol, nverts, 3).
+    With ``per_face`` the per-face gradients (n_lambda, npol, n, 31) are
+    returned as a third value."""
+    if not isinstance(bem, BEMRet):
+        raise TypeError("extinction_with_shape_gradient: expects a BEMRet solver")
+    p, g = bem.p, bem.g
+    enei_list = np.atleast_1d(np.asarray(enei_list, dtype=float))
+    P = planewave.npol
+    if P > 4:
+        raise ValueError("extinction_with_shape_gradient: at most 4 polarisations")
+    dev = g.device
+    st = nat.stream_handle(dev)
+    n = g.n
+    inm, outm, pol, dr = planewave._side_flags(p, g)
+    exts, grs = [], []
+    for c0 in range(0, len(enei_list), chunk):
+        lam = enei_list[c0:c0 + chunk]
+        B = len(lam)
+        F = bem.factor(lam)
+        R = planewave.rhs(p, lam, g)
+        X = bem.solve_batch(F, R)
+        ext = planewave.extinction_batch(p, lam, X, g)
+        pw = planewave._pw(p, lam, dev)
+        C = torch.empty_like(X)
+        nat.call("mnpbem_ext_functional", n, P, nat.ptr(g.pos), nat.ptr(g.area), nat.ptr(inm), nat.ptr(outm),
+                 nat.ptr(pol), nat.ptr(dr), nat.ptr(pw), nat.ptr(C), 8 * P, n * 8 * P, B, st)
+        Y = bem.solve_adjoint_batch(F, C)
+        G64, _ = bem.green64(F)
+        Q = torch.empty_like(X)
+        for s in range(2):
+            nat.call("mnpbem_gemv_t_batched", nat.DTYPE_C128, n, 4 * P, nat.ptr(G64[s]), n, n * n,
+                     nat.ptr_off(X, s * 4 * P), 8 * P, n * 8 * P, nat.ptr_off(Q, s * 4 * P), 8 * P, n * 8 * P,
+                     1.0, 0.0, 0.0, B, st)
+        del G64
+        Lw = torch.empty((B, n, 2, P, 8), dtype=torch.complex128, device=dev)
+        Rw = torch.empty((B, n, 2, P, 4), dtype=torch.complex128, device=dev)
+        Wc = torch.empty((B, n, P), dtype=torch.complex128, device=dev)
+        Gr = torch.zeros((B, P, n, _GR_SLOTS), dtype=torch.float64, device=dev)
+        nat.call("mnpbem_grad_prep", n, P, nat.ptr(X), nat.ptr(Y), 8 * P, n * 8 * P, nat.ptr(g.nvec), nat.ptr(F.par),
+                 nat.ptr(Lw), nat.ptr(Rw), nat.ptr(Wc), B, st)
+        nat.call("mnpbem_grad_far", n, P, nat.ptr(g.pos), nat.ptr(g.nvec), nat.ptr(g.area), nat.ptr(g.mat[0]),
+                 nat.ptr(g.mat[1]), nat.ptr(g._surf), nat.ptr(F.kmat), g.nmat, nat.ptr(g._rowptr()),
+                 nat.ptr(g.pcol), nat.ptr(g._colptr), nat.ptr(g._crow), nat.ptr(Lw), nat.ptr(Rw), nat.ptr(Wc),
+                 nat.ptr(Gr), B, st)
+        nat.call("mnpbem_grad_refined", g.order, n, P, g.npairs, nat.ptr(g.prow), nat.ptr(g.pcol),
+                 nat.ptr(g._pkind), nat.ptr(g.pos), nat.ptr(g.nvec), nat.ptr(g._tv1), nat.ptr(g._tv2),
+                 nat.ptr(g.area), nat.ptr(g._nodes), nat.ptr(g.mat[0]), nat.ptr(g.mat[1]), nat.ptr(g._surf),
+                 nat.ptr(F.kmat), g.nmat, nat.ptr(g._qn), g._qn.shape[0], nat.ptr(g._qs), g._qs.shape[0],
+                 nat.ptr(Lw), nat.ptr(Rw), nat.ptr(Wc), nat.ptr(Gr), B, st)
+        nat.call("mnpbem_grad_local", n, P, nat.ptr(X), nat.ptr(Y), nat.ptr(Q), nat.ptr(R), 8 * P, n * 8 * P,
+                 nat.ptr(g.pos), nat.ptr(g.nvec), nat.ptr(g.area), nat.ptr(inm), nat.ptr(outm), nat.ptr(pol),
+                 nat.ptr(dr), nat.ptr(pw), nat.ptr(g.corr), nat.ptr(Wc), nat.ptr(Gr), B, st)
+        exts.append(ext.cpu().numpy())
+        grs.append(Gr.cpu().numpy())
+        del F, X, Y, C, Q, Lw, Rw, Wc, Gr, R
+    ext = np.concatenate(exts)
+    Gr = np.concatenate(grs)
+    grads = _chain_to_vertices(p, Gr)
+    return (ext, grads, Gr) if per_face else (ext, grads)
+
+
+class ShapeExtinction(torch.autograd.Function):
+    """Extinction spectrum as a differentiable function of the vertex
+    positions of a one-particle ComParticle: ``ext = ShapeExtinction.apply(
+    verts, build, enei_list, planewave, precision)`` with ``verts`` a float64
+    tensor (nverts, 3) and ``build(verts_numpy) -> ComParticle`` rebuilding
+    the particle (corner triangles or 6-node faces); ``ext`` is (n_lambda,
+    npol).  The backward pass uses the gradient of the adjoint computed with
+    the forward solve."""
+
+    @staticmethod
+    def forward(ctx, verts, build, enei_list, planewave, precision="double"):
+        cp = build(verts.detach().cpu().numpy().astype(float))
+        if cp.np != 1:
+            raise NotImplementedError("ShapeExtinction: one particle")
+        ext, grads = extinction_with_shape_gradient(BEMRet(cp, precision=precision), enei_list, planewave)
+        gv = grads[0]["verts"] if grads[0].get("verts") is not None else grads[0]["nodes"]
+        if gv.shape[2] != verts.shape[0]:
+            raise ValueError("ShapeExtinction: build() must keep the vertex count")
+        ctx.save_for_backward(torch.as_tensor(gv, dtype=verts.dtype, device=verts.device))
+        return torch.as_tensor(ext, dtype=verts.dtype, device=verts.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (gv,) = ctx.saved_tensors
+        return (grad_out[:, :, None, None] * gv).sum(dim=(0, 1)), None, None, None, None

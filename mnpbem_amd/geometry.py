@@ -53,11 +53,13 @@ class Particle:
         faces6 = np.asarray(faces6, dtype=np.int64)
         if verts2.ndim != 2 or verts2.shape[1] != 3:
             raise ValueError("Particle: 'verts' must have shape (N, 3); got %s." % (verts2.shape,))
+        self.verts = self.faces = self.mid_edges = None
         if faces6.ndim == 2 and faces6.shape[1] == 3:
             # plain triangles: edge midpoints on the straight edges, so the
             # quadratic element is the flat triangle itself (its shape
             # functions reproduce the linear map exactly)
-            verts2, faces6 = add_midpoints(verts2, faces6)
+            self.verts, self.faces = verts2, faces6
+            verts2, faces6, self.mid_edges = add_midpoints(verts2, faces6, with_edges=True)
         if faces6.ndim != 2 or faces6.shape[1] != 6:
             raise ValueError("Particle: faces must list 3 corners or 6 nodes per triangle")
         self.verts2 = verts2
@@ -119,10 +121,11 @@ class Particle:
         return self._brad
 
 
-def add_midpoints(verts, faces):
+def add_midpoints(verts, faces, with_edges=False):
     """Append one node per edge at the edge midpoint: returns (nodes,
     faces6) with faces6 = (v0, v1, v2, m01, m12, m20)
-    (mesh_generators.py:133-196)."""
+    (mesh_generators.py:133-196); with ``with_edges`` also the (nedges, 2)
+    corner pair of each appended node."""
     verts = np.asarray(verts, dtype=float)
     faces = np.asarray(faces, dtype=np.int64)
     edges = np.sort(np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]]), axis=1)
@@ -130,7 +133,7 @@ def add_midpoints(verts, faces):
     mid = 0.5 * (verts[uniq[:, 0]] + verts[uniq[:, 1]])
     nodes = np.vstack([verts, mid])
     faces6 = np.hstack([faces, len(verts) + inv.reshape(3, -1).T])
-    return nodes, faces6
+    return (nodes, faces6, uniq) if with_edges else (nodes, faces6)
 
 
 def fibonacci_sphere(n):

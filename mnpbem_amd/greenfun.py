@@ -98,6 +98,11 @@ class CompGreenRet:
         hi = p.offsets[surf + 1]
         self.corr = torch.empty((n,), dtype=f64, device=dev)
         keep = [_dev(a, i32, dev) for a in (lo, hi, colptr, crow, cpair)]
+        # kept for the geometry derivatives: refined rows per column, the
+        # closed surface of each face and the quadrature tables
+        self._colptr, self._crow = keep[2], keep[3]
+        self._surf = _dev(surf, i32, dev)
+        self._qn, self._qs = qn, qs
         nat.call("mnpbem_closed_correction", n, nat.ptr(self.pos), nat.ptr(self.nvec), nat.ptr(self.area),
                  *[nat.ptr(t) for t in keep], nat.ptr(self.fstat), 1.0, self.order, nat.ptr(self.f),
                  nat.ptr(self.corr), st)
