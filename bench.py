@@ -8,7 +8,10 @@ One step = the whole 64-wavelength sweep: Green assembly, factorisation,
 solve and extinction for every wavelength, batched.
 
 ``value`` is the mixed-precision path (complex64 factorisation on MFMA +
-FP64 iterative refinement) in wavelengths/s; the same sweep in complex128
+FP64 iterative refinement) in wavelengths/s, timed as replays of the
+hipGraph-captured chunk step (``SweepGraph``; the eager launches of the same
+step are timed beside it as ``eager_value`` and give the GEMM roofline
+figures; ``--eager`` times them alone); the same sweep in complex128
 ("FP64 plain") is timed too and reported as ``fp64_value``.  The accuracy of
 the mixed path is stated against the complex128 GPU path (all 64 wavelengths)
 and against the FP64 CPU oracle (the baseline sample wavelengths).
@@ -106,6 +109,10 @@ def main():
     ap.add_argument("--fd-vertices", type=int, default=20,
                     help="adjoint workload: random vertices checked against central differences")
     ap.add_argument("--seed", type=int, default=0, help="seed of the uneven-particle set")
+    ap.add_argument("--eager", action="store_true",
+                    help="time the headline sweep with eager launches only (default: value is the replayed "
+                         "hipGraph of the 64-wavelength chunk step, eager steps timed beside it)")
+    ap.add_argument("--graph-iters", type=int, default=3, help="refinement steps inside the captured step")
     args = ap.parse_args()
     if args.gpus == 1 and args.workload == "uneven":
         return bench_uneven(args)
@@ -173,6 +180,25 @@ def main():
     path_flops = 34.67 * float(N) ** 3 * len(lams) * args.steps
     log("mixed: %.2f wavelengths/s, %.1f ms/step; GEMM %d launches, %.3f ms avg, %.1f TFLOP/s"
         % (value, ms_per_step, nl.value, ms.value / max(nl.value, 1), gemm_achieved / 1e12))
+
+    graph = None
+    if not args.eager and args.chunk >= len(lams):
+        from mnpbem_amd.bem import SweepGraph
+        sg = SweepGraph(bem, pw, len(lams), iters=args.graph_iters)
+        sg.run(lams)  # capture
+        torch.cuda.synchronize()
+        tg0 = time.perf_counter()
+        for _ in range(args.steps):
+            ext_g = sg.run(lams)
+        torch.cuda.synchronize()
+        tg = (time.perf_counter() - tg0) / args.steps
+        graph = {"graph_value": round(len(lams) / tg, 4), "graph_ms_per_step": round(1000.0 * tg, 1),
+                 "graph_refine_steps": args.graph_iters,
+                 "graph_max_rel_diff_vs_eager": float(np.max(np.abs(ext_g / ext_mixed - 1))),
+                 "graph_final_residual": float(np.max(sg.last_residual))}
+        log("graph: %.2f wavelengths/s (%.1f ms/step), vs eager max rel diff %.2e"
+            % (graph["graph_value"], graph["graph_ms_per_step"], graph["graph_max_rel_diff_vs_eager"]))
+        del sg
 
     fp64 = None
     acc_vs_fp64 = None
@@ -251,6 +277,20 @@ def main():
                      "avg_launch_ms": round(ms.value / max(nl.value, 1), 4)},
         "cpu_baseline": cpu,
     }
+    if graph is not None:
+        # value: the same sweep as replays of the captured chunk step (all
+        # kernels of the eager step, residual checked after every replay)
+        line["eager_value"] = line["value"]
+        line["eager_ms_per_step"] = line["ms_per_step"]
+        line["value"] = graph["graph_value"]
+        line["ms_per_step"] = graph["graph_ms_per_step"]
+        line["speedup_vs_fp64"] = None if fp64 is None else round(line["value"] / fp64, 3)
+        line["config"]["execution"] = ("hipGraph replay of the 64-wavelength chunk step (Green assembly, "
+                                       "factorisations, %d refinement solves, extinction); roofline from the "
+                                       "eager steps" % args.graph_iters)
+        line["graph"] = graph
+    else:
+        line["config"]["execution"] = "eager launches"
     print(json.dumps(line), flush=True)
 
 

@@ -113,8 +113,19 @@ class BEMRet:
         e_out = np.array([complex(p.eps[m_out - 1](lam)[0]) for lam in enei_list])
         return e_in, e_out
 
-    def factor(self, enei_list):
-        """Assemble and factor the BEM matrices for a batch of wavelengths."""
+    def _store_copies(self, B, n):
+        """Whether the mixed path keeps complex128 copies for its residual."""
+        if self.residual == "stored":
+            return True
+        if self.residual == "matrix_free":
+            return False
+        return 4 * B * n * n * 16 < 0.4 * torch.cuda.mem_get_info(self.device)[0]
+
+    def factor(self, enei_list, kmat=None, par=None, store=None):
+        """Assemble and factor the BEM matrices for a batch of wavelengths.
+        ``kmat``, ``par`` (device tensors of ``g.wavenumbers`` and the
+        per-wavelength permittivities) and ``store`` may be given so that no
+        host transfer or device query happens (graph capture)."""
         enei_list = np.atleast_1d(np.asarray(enei_list, dtype=float))
         g, dev = self.g, self.device
         B, n = len(enei_list), g.n
@@ -125,10 +136,10 @@ class BEMRet:
         F = _Factorization()
         F.enei = enei_list
         F.B, F.n, F.dtype, F.cdt = B, n, dt, cdt
-        kmat = g.wavenumbers(enei_list)
+        kmat = g.wavenumbers(enei_list) if kmat is None else kmat
         e_in, e_out = self._eps_sides(enei_list)
         k0 = 2.0 * np.pi / enei_list
-        F.par = _par_tensor(e_in, e_out, k0, dev)
+        F.par = _par_tensor(e_in, e_out, k0, dev) if par is None else par
         F.e_in, F.e_out, F.k0 = e_in, e_out, k0
         nn = n * n
         # GG = [G1^T ; G2^T], HH = [H1^T ; H2^T] stacked so both sides factor in one batch
@@ -136,9 +147,8 @@ class BEMRet:
         HH = torch.empty((2, B, n, n), dtype=cdt, device=dev)
         F.kmat = kmat
         F.G64 = F.H64 = None
-        store = self.residual == "stored"
-        if mixed and self.residual == "auto":
-            store = 4 * B * nn * 16 < 0.4 * torch.cuda.mem_get_info(dev)[0]
+        if store is None:
+            store = mixed and self._store_copies(B, n)
         if mixed and store:
             F.G64 = torch.empty((2, B, n, n), dtype=torch.complex128, device=dev)
             F.H64 = torch.empty((2, B, n, n), dtype=torch.complex128, device=dev)
@@ -189,9 +199,12 @@ class BEMRet:
                  nat.ptr(R), nat.ptr(W), nat.ptr(X), 8 * P, n * 8 * P,
                  nat.ptr(self.g.nvec), nat.ptr(F.par), F.B, nat.stream_handle(self.device))
 
-    def solve_batch(self, F, R):
+    def solve_batch(self, F, R, fixed_iters=None):
         """Solve A X = R for the batch (R: (B, n, 8P) complex128 on device).
-        Returns X (B, n, 8P) complex128."""
+        Returns X (B, n, 8P) complex128.  With ``fixed_iters`` the mixed path
+        runs exactly that many direct solves (refinement steps) without
+        reading anything back; the last residual norms stay on the device in
+        ``self.last_norms`` ((B, 2): ||R - A X||^2, ||R||^2)."""
         B, n = F.B, F.n
         P = R.shape[2] // 8
         dev = self.device
@@ -220,6 +233,12 @@ class BEMRet:
             nat.call("mnpbem_accumulate", nat.DTYPE_C64, cnt, nat.ptr(X32), nat.ptr(X), st)
             it += 1
             self._residual(F, R, X, Rres, Y, nrm, P)
+            if fixed_iters is not None:
+                if it >= fixed_iters:
+                    self.last_norms = nrm
+                    return X
+                src = Rres
+                continue
             h = nrm.cpu().numpy()
             rel = np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300))
             if np.all(rel <= self.refine_tol) or it > self.refine_maxit:
@@ -398,6 +417,77 @@ class BEMRet:
         if with_x:
             res.append(torch.cat(xs))
         return res[0] if len(res) == 1 else tuple(res)
+
+
+class SweepGraph:
+    """One wavelength-chunk step of the mixed-precision sweep (Green assembly,
+    factorisation, right side, ``iters`` refinement steps, extinction)
+    captured once as a HIP graph and replayed for any ``B`` wavelengths.
+
+    The step reads its per-wavelength inputs (wavenumbers, permittivities,
+    plane-wave parameters) from device buffers that ``run`` fills before each
+    replay, and performs no host transfer, allocation outside the graph's
+    memory pool or device query while captured; every tensor whose address the
+    graph holds is owned by the graph's pool, the solver or this object.  The
+    refinement runs a fixed
+    number of steps; ``run`` reads the final residual norms after the replay
+    and raises if any wavelength misses ``bem.refine_tol`` (more ``iters`` are
+    then needed).  The first ``run`` captures; later calls only replay."""
+
+    def __init__(self, bem, planewave, B, iters=3):
+        if bem.precision == "double":
+            raise ValueError("SweepGraph: the captured step is the mixed-precision path")
+        self.bem, self.pw, self.B, self.iters = bem, planewave, int(B), int(iters)
+        g, dev = bem.g, bem.device
+        self.kmat = torch.empty((self.B, g.nmat, 2), dtype=torch.float64, device=dev)
+        self.par = torch.empty((self.B, 5), dtype=torch.float64, device=dev)
+        self.pwt = torch.empty((self.B, 7), dtype=torch.float64, device=dev)
+        self.store = bem._store_copies(self.B, g.n)
+        self.graph = None
+        self.captures = 0
+
+    def _inputs(self, lams):
+        bem, g = self.bem, self.bem.g
+        e_in, e_out = bem._eps_sides(lams)
+        self.kmat.copy_(g.wavenumbers(lams))
+        self.par.copy_(_par_tensor(e_in, e_out, 2.0 * np.pi / lams, bem.device))
+        self.pwt.copy_(self.pw._pw(bem.p, lams, bem.device))
+
+    def _step(self, lams):
+        bem, p, g = self.bem, self.bem.p, self.bem.g
+        # the graph holds the addresses of the plane wave's side flags and
+        # directions; keep them alive (the plane wave's cache may drop them)
+        self._keep = self.pw._side_flags(p, g)
+        F = bem.factor(lams, kmat=self.kmat, par=self.par, store=self.store)
+        R = self.pw.rhs(p, lams, g, pw=self.pwt)
+        X = bem.solve_batch(F, R, fixed_iters=self.iters)
+        self.ext = self.pw.extinction_batch(p, lams, X, g, pw=self.pwt)
+        self.norms = bem.last_norms
+
+    def run(self, enei_list):
+        lams = np.atleast_1d(np.asarray(enei_list, dtype=float))
+        if len(lams) != self.B:
+            raise ValueError("SweepGraph: captured for %d wavelengths, got %d" % (self.B, len(lams)))
+        self._inputs(lams)
+        if self.graph is None:
+            dev = self.bem.device
+            side = torch.cuda.Stream(device=dev)
+            side.wait_stream(torch.cuda.current_stream(dev))
+            with torch.cuda.stream(side):
+                self._step(lams)   # eager warm-up on a side stream (library, caches)
+            torch.cuda.current_stream(dev).wait_stream(side)
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self._step(lams)
+            self.captures += 1
+        self.graph.replay()
+        h = self.norms.cpu().numpy()
+        rel = np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300))
+        self.last_residual = rel
+        if np.any(rel > self.bem.refine_tol):
+            raise RuntimeError("SweepGraph: residual %.2e above refine_tol after %d steps; use more iters"
+                               % (rel.max(), self.iters))
+        return self.ext.cpu().numpy()
 
 
 def _unpack(p, enei, X, P):

@@ -55,13 +55,14 @@ class PlaneWaveRet:
             self._flag_cache = (key, (inm, outm, pol, dr))
         return self._flag_cache[1]
 
-    def rhs(self, p, enei_list, g, potentials=False):
+    def rhs(self, p, enei_list, g, potentials=False, pw=None):
         """BEM right side R (B, n, 8P) = [phi | a | De | alpha] (bem_ret.py:1360-1429)
-        and optionally the potentials (B, n, 12P) = [a1 | a1' | a2 | a2']."""
+        and optionally the potentials (B, n, 12P) = [a1 | a1' | a2 | a2'].
+        ``pw``: the per-wavelength parameters already on the device (``_pw``)."""
         enei_list = np.atleast_1d(enei_list)
         B, n, P = len(enei_list), p.n, self.npol
         inm, outm, pol, dr = self._side_flags(p, g)
-        pw = self._pw(p, enei_list, g.device)
+        pw = self._pw(p, enei_list, g.device) if pw is None else pw
         R = torch.empty((B, n, 8 * P), dtype=torch.complex128, device=g.device)
         pot = torch.empty((B, n, 12 * P), dtype=torch.complex128, device=g.device) if potentials else None
         nat.call("mnpbem_planewave_batched", n, P, nat.ptr(g.pos), nat.ptr(g.nvec), nat.ptr(inm), nat.ptr(outm),
@@ -84,12 +85,12 @@ class PlaneWaveRet:
 
     __call__ = potential
 
-    def extinction_batch(self, p, enei_list, X, g):
+    def extinction_batch(self, p, enei_list, X, g, pw=None):
         """Extinction (B, npol) from solutions X (B, n, 8P)."""
         enei_list = np.atleast_1d(enei_list)
         B, n, P = len(enei_list), p.n, self.npol
         inm, outm, pol, dr = self._side_flags(p, g)
-        pw = self._pw(p, enei_list, g.device)
+        pw = self._pw(p, enei_list, g.device) if pw is None else pw
         ext = torch.empty((B, P), dtype=torch.float64, device=g.device)
         nat.call("mnpbem_extinction_batched", n, P, nat.ptr(X), 8 * P, n * 8 * P, nat.ptr(g.pos), nat.ptr(g.area),
                  nat.ptr(inm), nat.ptr(outm), nat.ptr(pol), nat.ptr(dr), nat.ptr(pw), nat.ptr(ext), B,
