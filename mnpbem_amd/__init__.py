@@ -12,7 +12,7 @@ from .geometry import ComParticle, Particle, triellipsoid, trirod, trisphere, tr
 
 __all__ = ["EV2NM", "EpsConst", "EpsDrude", "EpsTable", "ComParticle", "Particle", "trisphere",
            "trisphere_fibonacci", "triellipsoid", "trirod",
-           "CompGreenRet", "BEMRet", "CompStruct", "PlaneWaveRet", "BEMIter", "BEMRetIter", "ClusterTree",
+           "CompGreenRet", "BEMRet", "CompStruct", "SweepGraph", "PlaneWaveRet", "BEMIter", "BEMRetIter", "ClusterTree",
            "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss", "DrudeExtinction",
            "extinction_with_drude_gradient", "ShapeExtinction", "extinction_with_shape_gradient"]
 
@@ -23,7 +23,7 @@ def __getattr__(name):
     if name == "CompGreenRet":
         from .greenfun import CompGreenRet
         return CompGreenRet
-    if name in ("BEMRet", "CompStruct"):
+    if name in ("BEMRet", "CompStruct", "SweepGraph"):
         from . import bem
         return getattr(bem, name)
     if name in ("BEMIter", "BEMRetIter"):

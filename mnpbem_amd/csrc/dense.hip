@@ -11,6 +11,8 @@
 // 1-based column is then reported in ``info``).
 #include "common.h"
 
+#include <cstdlib>
+
 #include <vector>
 
 namespace {
@@ -203,16 +205,54 @@ __device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
   }
 }
 
+// Output tile of a workgroup.  The dispatcher hands consecutive workgroups
+// of the linearised grid to the 8 XCDs in turn, and each XCD has its own L2;
+// with remap, XCD x instead gets one contiguous range of the tiles, ordered
+// matrix by matrix and, inside a matrix, in groups of 4 tile rows walked
+// column by column, so the workgroups resident on one XCD share A row panels
+// and B column panels in its L2.  A bijection on the tiles: results do not
+// depend on it.
+struct Tile {
+  int x, y;
+  long z;
+};
+__device__ __forceinline__ Tile gemm_tile(int remap) {
+  Tile t;
+  if (!remap) {
+    t.x = blockIdx.x;
+    t.y = blockIdx.y;
+    t.z = blockIdx.z;
+    return t;
+  }
+  const long gx = gridDim.x, gy = gridDim.y;
+  const long T = gx * gy * gridDim.z;
+  const long L = blockIdx.x + gx * (blockIdx.y + gy * (long)blockIdx.z);
+  const long q = T / 8, r = T % 8, xcd = L % 8;
+  const long id = xcd * q + (xcd < r ? xcd : r) + L / 8;
+  const long per = gx * gy;
+  t.z = id / per;
+  const long rem = id - t.z * per;
+  constexpr long GROUP = 4;
+  const long g = rem / (GROUP * gx);
+  const long y0 = g * GROUP;
+  const long gs = (gy - y0) < GROUP ? (gy - y0) : GROUP;
+  const long w = rem - g * GROUP * gx;
+  t.y = (int)(y0 + w % gs);
+  t.x = (int)(w / gs);
+  return t;
+}
+
 template <int BM, int BN, int WM>
 __global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3(int M, int N, int K, const float* __restrict__ A, long lda,
                                                           long sA, const float* __restrict__ B, long ldb, long sB,
-                                                          float* __restrict__ C, long ldc, long sC) {
-  const long bz = blockIdx.z;
+                                                          float* __restrict__ C, long ldc, long sC, int remap) {
+  const Tile tl = gemm_tile(remap);
+  const long bz = tl.z;
   A += 2 * bz * sA;
   B += 2 * bz * sB;
   C += 2 * bz * sC;
   constexpr int WN = 32, FI = WM / 16, FJ = WN / 16, NWN = BN / WN;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int m0 = tl.y * BM, n0 = tl.x * BN;
   // two stages x four planes (re_hi, re_lo, im_hi, im_lo) x 128 rows x 32
   __shared__ __attribute__((aligned(16))) __bf16 As[2][4][BM * XK];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[2][4][BN * XK];
@@ -627,16 +667,25 @@ inline int gclass(int x) {
 }
 inline int gcell(int M, int N, int K) { return (gclass(M) * GC + gclass(N)) * GC + gclass(K); }
 
+// XCD-aware tile order of the bf16x3 GEMM (MNPBEM_GEMM_XCD=0: dispatch order)
+int gemm_xcd_remap() {
+  static const int v = [] {
+    const char* e = getenv("MNPBEM_GEMM_XCD");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  return v;
+}
+
 int launch_bf16x3(int variant, int M, int N, int K, const float* A, long lda, long sA, const float* B, long ldb,
                   long sB, float* C, long ldc, long sC, int batch, hipStream_t st) {
   if (variant == 1) {
     dim3 grid(mnp_ceil_div(N, 64), mnp_ceil_div(M, 64), batch);
     hipLaunchKernelGGL((k_cgemm_sub_bf16x3<64, 64, 32>), grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB,
-                       C, ldc, sC);
+                       C, ldc, sC, gemm_xcd_remap());
   } else {
     dim3 grid(mnp_ceil_div(N, 128), mnp_ceil_div(M, 128), batch);
     hipLaunchKernelGGL((k_cgemm_sub_bf16x3<128, 128, 64>), grid, dim3(512), 0, st, M, N, K, A, lda, sA, B, ldb, sB,
-                       C, ldc, sC);
+                       C, ldc, sC, gemm_xcd_remap());
   }
   MNP_LAUNCH_CHECK();
   return 0;
