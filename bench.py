@@ -109,6 +109,9 @@ def main():
     ap.add_argument("--fd-vertices", type=int, default=20,
                     help="adjoint workload: random vertices checked against central differences")
     ap.add_argument("--seed", type=int, default=0, help="seed of the uneven-particle set")
+    ap.add_argument("--dimer-precond", default="block", choices=["block", "auto"],
+                    help="dimer workload: GMRES preconditioner (block: each sphere's direct solve; auto: the "
+                         "direct solve of the whole dimer)")
     ap.add_argument("--eager", action="store_true",
                     help="time the headline sweep with eager launches only (default: value is the replayed "
                          "hipGraph of the 64-wavelength chunk step, eager steps timed beside it)")
@@ -358,8 +361,10 @@ def bench_dimer(args):
     """Gold sphere dimer, two 12500-face golden-spiral spheres of 100 nm with a
     2 nm gap (25000 faces): BEMRetIter on ACA H-matrices (leaf 64, htol 1e-6,
     rank <= 30, blocks in complex64), GMRES restart 50, tol 1e-6, FP64
-    Arnoldi, preconditioned by the mixed-precision direct solve; 500, 600,
-    700 nm; x polarisation along the dimer axis, propagation along z."""
+    Arnoldi, preconditioned by the mixed-precision direct solve of each
+    sphere (--dimer-precond block, default) or of the whole dimer (auto);
+    500, 600, 700 nm; x polarisation along the dimer axis, propagation
+    along z."""
     import torch
     import mnpbem_amd as m
     from mnpbem_amd import _native as nat
@@ -370,7 +375,7 @@ def bench_dimer(args):
     lams = np.array([500.0, 600.0, 700.0])
     pw = m.PlaneWaveRet([[1.0, 0.0, 0.0]], [[0.0, 0.0, 1.0]])
     bem = m.BEMRetIter(cp, hmatrix=True, htol=1e-6, kmax=30, cleaf=64, tol=1e-6, restart=50, maxit=20,
-                       precision="mixed", hmat_dtype="complex64")
+                       precision="mixed", hmat_dtype="complex64", preconditioner=args.dimer_precond)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     log("dimer set-up %.1f s, N = %d" % (t_setup, cp.n))
@@ -421,7 +426,8 @@ def bench_dimer(args):
             "ms_per_step": round(1000.0 * dt, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64 GMRES, c64 H-matrix blocks, c64 preconditioner", "data": "synthetic",
             "config": {"workload": "2 x trisphere_fibonacci(6252, 100 nm), 2 nm gap, 3 wavelengths",
-                       "faces": cp.n, "cleaf": 64, "htol": 1e-6, "kmax": 30, "restart": 50, "tol": 1e-6},
+                       "faces": cp.n, "cleaf": 64, "htol": 1e-6, "kmax": 30, "restart": 50, "tol": 1e-6,
+                       "preconditioner": args.dimer_precond},
             "setup_s": round(t_setup, 2), "direct_mixed_value": round(len(lams) / dt_direct, 4),
             "max_rel_ext_err_vs_direct": err,
             "hmatrix": [{"compression": round(x["compression"], 4), "rank_max": x["rank_max"],

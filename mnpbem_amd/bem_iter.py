@@ -99,7 +99,7 @@ class BEMRetIter(BEMIter):
                 raise NotImplementedError("BEMRetIter: option '%s' is not provided" % unsupported)
         if self._hmatrix and "precond" not in options:
             options["precond"] = None
-        if self._hlu_mode not in ("auto", "none", "hlu_dense", "hlu_tree"):
+        if self._hlu_mode not in ("auto", "none", "hlu_dense", "hlu_tree", "block"):
             raise ValueError("BEMRetIter: unknown preconditioner '%s'" % self._hlu_mode)
         super().__init__(**options)
         if self.solver not in self.SOLVER_MAP:
@@ -112,6 +112,17 @@ class BEMRetIter(BEMIter):
         self.g = green if green is not None else CompGreenRet(p, p)
         self.device = self.g.device
         self._direct = BEMRet(p, precision=precision, green=self.g)
+        # preconditioner='block': the direct solve of each particle on its own
+        # faces (the diagonal blocks of the 8N operator: a particle's Green
+        # functions, refinement and closed-surface correction do not involve
+        # the other particles); GMRES carries the coupling between particles
+        self._blocks = None
+        if self._hlu_mode == "block":
+            from .geometry import ComParticle
+            self._blocks = []
+            for k, part in enumerate(p.p):
+                sub = ComParticle(p.eps, [part], [list(p.inout[k])], 1)
+                self._blocks.append((int(p.offsets[k]), BEMRet(sub, precision=precision, residual="matrix_free")))
         self._aca = None
         if self._hmatrix:
             from .hmatrix import ACACompGreenRet
@@ -140,7 +151,10 @@ class BEMRetIter(BEMIter):
         S = _Batch()
         S.enei, S.B, S.n = enei_list, B, n
         S.F = None
-        if self._use_precond:
+        S.Fk = None
+        if self._blocks is not None:
+            S.Fk = [bem.factor(enei_list) for _, bem in self._blocks]
+        if self._use_precond and self._blocks is None:
             S.F = self._direct.factor(enei_list)
             par = S.F.par
         else:
@@ -189,8 +203,33 @@ class BEMRetIter(BEMIter):
         nat.call("mnpbem_bem_combine_batched", n, P, nat.ptr(Y), 16 * P, n * 16 * P, None, nat.ptr(Out), 8 * P,
                  n * 8 * P, nat.ptr(self.g.nvec), nat.ptr(S.par), None, B, st)
 
+    def _precond_blocks(self, S, V, Z, ws, P):
+        """Z = blockdiag(A_kk)^-1 V: each particle's direct solve on its rows
+        of the packed vectors (row stride 8P, wavelength stride n 8P)."""
+        n, B = S.n, S.B
+        st = nat.stream_handle(self.device)
+        c128 = S.Fk[0].dtype == nat.DTYPE_C128
+        if c128:
+            R, X, W = V, Z, ws["W"]
+        else:
+            R, X, W = ws["R32"], ws["X32"], ws["W32"]
+            nat.call("mnpbem_convert", nat.DTYPE_C64, 2 * V.numel(), nat.ptr(V), nat.ptr(R), st)
+        for (off, bem), F in zip(self._blocks, S.Fk):
+            nn = F.n * F.n
+            o = off * 8 * P
+            nat.call("mnpbem_bem_solve_batched", F.dtype, F.n, P, nat.ptr(F.GG[0]), nat.ptr(F.piv[0]),
+                     nat.ptr(F.GG[1]), nat.ptr(F.piv[1]), nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]),
+                     nat.ptr(F.piv[3]), F.n, nn, F.n, nat.ptr(F.DG[0]), nat.ptr(F.DG[1]), nat.ptr(F.DS), F.sD,
+                     nat.ptr_off(R, o), nat.ptr_off(W, o), nat.ptr_off(X, o), 8 * P, n * 8 * P,
+                     nat.ptr(bem.g.nvec), nat.ptr(F.par), B, st)
+        if not c128:
+            nat.call("mnpbem_widen", nat.DTYPE_C64, 2 * V.numel(), nat.ptr(X), nat.ptr(Z), st)
+
     def _precond(self, S, V, Z, ws, P):
         """Z = M^-1 V with the direct BEMRet solve (or Z = V without one)."""
+        if S.Fk is not None:
+            self._precond_blocks(S, V, Z, ws, P)
+            return
         if S.F is None:
             Z.copy_(V)
             return
@@ -221,7 +260,8 @@ class BEMRetIter(BEMIter):
         m = max(1, int(m))
         tol = float(self.tol)
         ws = {"W": torch.empty_like(b)}
-        if S.F is not None and S.F.dtype != nat.DTYPE_C128:
+        F0 = S.F if S.F is not None else (S.Fk[0] if S.Fk else None)
+        if F0 is not None and F0.dtype != nat.DTYPE_C128:
             ws["R32"] = torch.empty(b.shape, dtype=torch.complex64, device=dev)
             ws["X32"] = torch.empty_like(ws["R32"])
             ws["W32"] = torch.empty_like(ws["R32"])
@@ -345,7 +385,7 @@ class BEMRetIter(BEMIter):
         """X solving A X = R for the batch (R (B, n, 8P) complex128)."""
         P = R.shape[2] // 8
         if self.maxit == 0 or self.solver is None:
-            if S.F is None:
+            if S.F is None and S.Fk is None:
                 raise ValueError("[error] preconditioner must be hmat when maxit=0")
             X = torch.empty_like(R)
             self._precond(S, R, X, {"W": torch.empty_like(R), "R32": torch.empty(R.shape, dtype=torch.complex64,

@@ -202,3 +202,26 @@ def test_iter_hmatrix_matches_direct(cuda, hdt):
     # factors cost about as much as the dense blocks they replace
     c = bi.hinfo()
     assert 0 < c < 1.5
+
+
+@pytest.mark.parametrize("precision,hmatrix", [("double", False), ("mixed", False), ("mixed", True)])
+def test_iter_block_preconditioner_dimer(cuda, precision, hmatrix):
+    """preconditioner='block' (each particle's direct solve on its own faces):
+    on a 1-nm-gap dimer of two 284-face spheres GMRES needs more than one
+    iteration (the coupling is not in the preconditioner) and converges to
+    the dense direct solution."""
+    import mnpbem_amd as m
+    print("BEMRetIter block preconditioner", precision, hmatrix, flush=True)
+    p1 = m.trisphere(144, 10.0)
+    p2 = p1.shift([11.0, 0.0, 0.0])
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [p1, p2], [[2, 1], [2, 1]], 1, 2)
+    lams = np.array([520.0, 600.0])
+    exc = m.PlaneWaveRet(*POL2)
+    opts = dict(hmatrix=True, htol=1e-10, kmax=80, cleaf=32) if hmatrix else {}
+    bi = m.BEMRetIter(cp, precision=precision, preconditioner="block", tol=1e-10, maxit=60, restart=60, **opts)
+    e_b = bi.solve_sweep(lams, exc, chunk=2)
+    e_d = m.BEMRet(cp).solve_sweep(lams, exc)
+    flags, rel, its = bi.info()
+    assert all(f == 0 for f in flags)
+    assert max(int(i[1]) for i in its) > 1
+    assert np.max(np.abs(e_b / e_d - 1)) < (1e-8 if not hmatrix else 1e-6)
