@@ -14,6 +14,8 @@ solve + HIP derivative kernels + torch chain rule to the vertices).
    (h / size)^2, stays below 1e-8, their round-off below 1e-8 of the
    derivative; the bar is 1e-6 of the largest derivative component.
 3. The mixed path's gradient vs the complex128 one, and the autograd wrapper.
+4. A dimer of two curved spheres 1 nm apart (refined pairs across the gap,
+   one closed-surface correction per sphere): node gradients of both.
 """
 
 import copy
@@ -174,3 +176,39 @@ def test_shape_gradient_mixed_and_autograd(cuda):
     (ext * w).sum().backward()
     want = (w.numpy()[:, :, None, None] * g64[0]["verts"]).sum(axis=(0, 1))
     assert np.abs(verts.grad.numpy() - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_shape_gradient_two_particles(cuda):
+    """Dimer of two curved 60-face spheres 1 nm apart (refined pairs across
+    the gap, one closed-surface correction per sphere): node gradients of
+    both particles against central differences of the complex128 spectrum."""
+    import mnpbem_amd as m
+    from mnpbem_amd.adjoint import extinction_with_shape_gradient
+    s1 = m.trisphere(32, 8.0)
+    s2 = s1.shift([9.0, 0.0, 0.0])
+    pw = m.PlaneWaveRet([[1, 0, 0]], [[0, 0, 1]])
+    lams = np.array([520.0, 560.0])
+
+    def mk(n1, n2):
+        parts = [m.geometry.Particle(n1, s1.faces6, interp="curv"), m.geometry.Particle(n2, s2.faces6, interp="curv")]
+        return m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], parts, [[2, 1], [2, 1]], 1, 2)
+
+    cp = mk(s1.verts2, s2.verts2)
+    print("shape gradient dimer", cp.n, flush=True)
+    _, grads = extinction_with_shape_gradient(m.BEMRet(cp), lams, pw)
+    scale = max(np.abs(g["nodes"]).max() for g in grads)
+    h = 1e-4
+    rng = np.random.default_rng(3)
+    for k, base in enumerate((s1.verts2, s2.verts2)):
+        for vi in rng.choice(len(base), 2, replace=False):
+            for c in range(3):
+                vals = []
+                for sg in (1.0, -1.0):
+                    v = base.copy()
+                    v[vi, c] += sg * h
+                    cp2 = mk(v, s2.verts2) if k == 0 else mk(s1.verts2, v)
+                    vals.append(_spectrum(m, cp2, lams, pw))
+                fd = (vals[0] - vals[1]) / (2 * h)
+                got = grads[k]["nodes"][:, :, vi, c]
+                print("particle %d node %3d %d kernel %+.8e fd %+.8e" % (k, vi, c, got[0, 0], fd[0, 0]), flush=True)
+                assert np.abs(got - fd).max() < 1e-6 * scale, (k, vi, c)
