@@ -84,6 +84,13 @@ int mnpbem_lu_solve_t_batched(int dtype, int n, int C, const void* LU, long ld, 
                               long spiv, void* V, long ldv, long sV, void* Z, long ldz, long sZ, const void* Dinv,
                               long sD, int batch, void* stream);
 
+/* Rows above which mnpbem_getrf_batched factors its panels with the
+ * multi-workgroup column kernels (k_tall_pivot / k_tall_update) instead of
+ * the register-resident leaf; rows = 0 restores the built-in thresholds (12288
+ * complex64, 6144 complex128).  Returns the previous setting.  A tuning knob
+ * of this library (the reference's LAPACK/cuSOLVER getrf has none). */
+int mnpbem_getrf_tall_rows(int rows);
+
 /* Inverses of the 64 x 64 diagonal blocks of getrf factors ([U_II^-1 | L_II^-1]
  * per block, sD >= ceil(n/64)*2*64*64 elements) so that the block steps of
  * mnpbem_lu_solve_t_batched are parallel matrix-vector products (Dinv = NULL

@@ -850,16 +850,201 @@ int leaf_launch(int W, int N, int c0, int w, T* A, long lda, long sA, int* ipiv,
   return 1;
 }
 
+// ---------------------------------------------------------------------------
+// Tall panels (more rows than the register-resident leaf holds with 8
+// columns): the panel of width w <= TALL_W is factored column by column by
+// many workgroups at once, two launches per column and no grid-wide barrier:
+//   k_tall_pivot (one workgroup per matrix): reduce the per-chunk candidates
+//     of column c to the pivot row p (largest |re|+|im|, lowest row on ties,
+//     the rule of k_lu_leaf), exchange rows c and p inside the panel, record
+//     ipiv and 1 / pivot;
+//   k_tall_update (row chunks x matrices): scale column c below the diagonal,
+//     rank-1 update of the panel columns right of c, and the per-chunk
+//     candidates of column c+1.
+// The same pivots and the same arithmetic per entry as the leaf, so the
+// factors do not depend on which path ran.  Candidates live in a small static
+// device buffer (no allocation; concurrent tall factorisations on one device
+// would share it).
+constexpr int TALL_W = 32;       // panel width
+constexpr int TALL_CH = 512;     // rows per workgroup chunk (256 threads x 2)
+constexpr int TALL_MAXC = 64;    // chunks per matrix (32768 rows)
+constexpr int TALL_MAXB = 512;   // matrices per launch
+__device__ double g_tall_val[TALL_MAXB * TALL_MAXC];
+__device__ int g_tall_idx[TALL_MAXB * TALL_MAXC];
+__device__ double g_tall_inv[2 * TALL_MAXB];
+
+// per-chunk argmax of |re|+|im| of column col over the rows >= col of chunk
+// blockIdx.x (rows [r0 + k TALL_CH, r0 + (k+1) TALL_CH) of [r0, N)); each
+// thread reads the rows it updated itself in k_tall_update
+template <typename T>
+__device__ void tall_candidates(int N, int r0, int col, const T* A, long lda, long b, int nch) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lo = r0 + blockIdx.x * TALL_CH;
+  T best = T(-1);
+  int bi = 0x7fffffff;
+  for (int r = lo + tid; r < min(N, lo + TALL_CH); r += 256) {
+    if (r < col) continue;
+    const T v = cabs1(cld(A + 2 * ((long)r * lda + col)));
+    if (v > best) {
+      best = v;
+      bi = r;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const T ov = __shfl_xor(best, off);
+    const int oi = __shfl_xor(bi, off);
+    if (ov > best || (ov == best && oi < bi)) {
+      best = ov;
+      bi = oi;
+    }
+  }
+  __shared__ T sv[4];
+  __shared__ int si[4];
+  if (lane == 0) {
+    sv[wv] = best;
+    si[wv] = bi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int q = 1; q < 4; ++q)
+      if (sv[q] > best || (sv[q] == best && si[q] < bi)) {
+        best = sv[q];
+        bi = si[q];
+      }
+    g_tall_val[b * TALL_MAXC + blockIdx.x] = (double)best;
+    g_tall_idx[b * TALL_MAXC + blockIdx.x] = bi;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tall_first(int N, int c0, T* A, long lda, long sA, int nch) {
+  const long b = blockIdx.y;
+  tall_candidates<T>(N, c0, c0, A + 2 * b * sA, lda, b, nch);
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_tall_pivot(int N, int c0, int w, int c, T* A, long lda, long sA, int* ipiv,
+                                                   long sP, int* info, int nch) {
+  const long b = blockIdx.x;
+  A += 2 * b * sA;
+  const int tid = threadIdx.x;
+  const int r = c0 + c;
+  __shared__ int sp;
+  if (tid == 0) {
+    double best = -1.0;
+    int bi = 0x7fffffff;
+    for (int q = 0; q < nch; ++q) {
+      const double v = g_tall_val[b * TALL_MAXC + q];
+      const int iq = g_tall_idx[b * TALL_MAXC + q];
+      if (v > best || (v == best && iq < bi)) {
+        best = v;
+        bi = iq;
+      }
+    }
+    if (bi < r || bi >= N) bi = r;  // only NaN entries left: keep the diagonal
+    sp = bi;
+    ipiv[b * sP + r] = bi;
+  }
+  __syncthreads();
+  const int p = sp;
+  if (p != r)
+    for (int t = tid; t < w; t += 64) {
+      T* x = A + 2 * ((long)r * lda + c0 + t);
+      T* y = A + 2 * ((long)p * lda + c0 + t);
+      const cplx<T> u = cld(x), v = cld(y);
+      cst(x, v);
+      cst(y, u);
+    }
+  __syncthreads();
+  if (tid == 0) {
+    const cplx<T> pv = cld(A + 2 * ((long)r * lda + r));
+    const bool zero = (pv.re == T(0) && pv.im == T(0));
+    if (zero && info[b] == 0) info[b] = r + 1;
+    const cplx<T> inv = zero ? cmake<T>(1, 0) : crecip(pv);
+    g_tall_inv[2 * b] = (double)inv.re;
+    g_tall_inv[2 * b + 1] = (double)inv.im;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tall_update(int N, int c0, int w, int c, T* A, long lda, long sA, int nch) {
+  const long b = blockIdx.y;
+  A += 2 * b * sA;
+  const int r0 = c0 + c;  // pivot row
+  __shared__ cplx<T> prow[TALL_W];
+  const int tid = threadIdx.x;
+  if (tid < w) prow[tid] = cld(A + 2 * ((long)r0 * lda + c0 + tid));
+  __syncthreads();
+  const cplx<T> inv = cmake<T>((T)g_tall_inv[2 * b], (T)g_tall_inv[2 * b + 1]);
+  // chunks cover rows [c0, N) in TALL_CH blocks; only rows below the pivot change
+  const int lo = c0 + blockIdx.x * TALL_CH;
+  for (int r = lo + tid; r < min(N, lo + TALL_CH); r += 256) {
+    if (r <= r0) continue;
+    T* row = A + 2 * ((long)r * lda + c0);
+    const cplx<T> l = cmul(cld(row + 2 * c), inv);
+    cst(row + 2 * c, l);
+    for (int cc = c + 1; cc < w; ++cc) cst(row + 2 * cc, cfms(cld(row + 2 * cc), l, prow[cc]));
+  }
+  if (c + 1 < w) {
+    __syncthreads();
+    tall_candidates<T>(N, c0, c0 + c + 1, A, lda, b, nch);
+  }
+}
+
+int tall_lu_enabled() {
+  static const int v = [] {
+    const char* e = getenv("MNPBEM_TALL_LU");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  return v;
+}
+
+// rows per matrix above which the panel is factored by the multi-workgroup
+// column kernels: where the register leaf is down to one column (measured on
+// gfx950, complex64 LU: 0.82 s instead of 1.22 s at 25000 rows, 0.33 instead
+// of 0.36 at 12500, but slower than the 2- and 4-column leaves below);
+// 0 = the built-in thresholds (set by mnpbem_getrf_tall_rows)
+int g_tall_rows = 0;
+
+template <typename T>
+bool tall_ok(int m, int batch) {
+  const int thr = g_tall_rows > 0 ? g_tall_rows : (sizeof(T) == 4 ? 12288 : 6144);
+  return tall_lu_enabled() && m > thr && batch <= TALL_MAXB && (m + TALL_CH - 1) / TALL_CH <= TALL_MAXC;
+}
+
+template <typename T>
+int tall_panel(int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
+               hipStream_t st) {
+  const int m = N - c0;
+  // chunks of the rows [c0, N): the candidates of column c are searched in
+  // rows >= c0 + c, so a chunk may hold none (its candidate is then empty)
+  const int nch = (m + TALL_CH - 1) / TALL_CH;
+  hipLaunchKernelGGL(k_tall_first<T>, dim3(nch, batch), dim3(256), 0, st, N, c0, A, lda, sA, nch);
+  MNP_LAUNCH_CHECK();
+  for (int c = 0; c < w; ++c) {
+    hipLaunchKernelGGL(k_tall_pivot<T>, dim3(batch), dim3(64), 0, st, N, c0, w, c, A, lda, sA, ipiv, sP, info, nch);
+    MNP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tall_update<T>, dim3(nch, batch), dim3(256), 0, st, N, c0, w, c, A, lda, sA, nch);
+    MNP_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
 // recursive LU of columns [c0, c0+n) (rows c0..N-1)
 template <typename T>
 int getrf_rec(int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
               hipStream_t st, int gmode) {
-  const int W = leaf_width<T>(N - c0);
+  const bool tall = tall_ok<T>(N - c0, batch);
+  const int W = tall ? TALL_W : leaf_width<T>(N - c0);
   if (W == 0) {
     mnp_set_error("getrf: matrix too tall for the register-resident panel");
     return 1;
   }
-  if (n <= W) return leaf_launch<T>(W, N, c0, n, A, lda, sA, ipiv, sP, info, batch, st);
+  if (n <= W) {
+    if (tall) return tall_panel<T>(N, c0, n, A, lda, sA, ipiv, sP, info, batch, st);
+    return leaf_launch<T>(W, N, c0, n, A, lda, sA, ipiv, sP, info, batch, st);
+  }
   int n1 = ((n / 2 + W - 1) / W) * W;
   if (n1 >= n) n1 = n - W;
   const int n2 = n - n1;
@@ -907,6 +1092,16 @@ extern "C" int mnpbem_cgemm_sub_variant(int variant, int M, int N, int K, const 
   if (M == 0 || N == 0 || K == 0 || batch == 0) return 0;
   return launch_bf16x3(variant, M, N, K, (const float*)A, lda, sA, (const float*)B, ldb, sB, (float*)C, ldc, sC,
                        batch, (hipStream_t)stream);
+}
+
+// Rows above which getrf factors its panels with the multi-workgroup column
+// kernels instead of the register-resident leaf (rows > 0), or the built-in
+// thresholds (rows = 0: 12288 for complex64, 6144 for complex128).  Returns
+// the previous setting.
+extern "C" int mnpbem_getrf_tall_rows(int rows) {
+  const int prev = g_tall_rows;
+  g_tall_rows = rows > 0 ? rows : 0;
+  return prev;
 }
 
 // Shape-class table of the bf16x3 GEMM (7 x 7 x 7 cells, index (cm * 7 + cn)

@@ -224,3 +224,52 @@ def test_getrf_tall_panels(cuda, dtype, n, tol):
     del A, LU, x, b
 
 
+
+
+@pytest.mark.parametrize("dtype,n", [(np.complex128, 3500), (np.complex64, 4100)])
+def test_getrf_tall_panels_match_lapack(cuda, dtype, n):
+    """With the threshold lowered to 3072 rows (built in: 12288 complex64,
+    6144 complex128) the panels are factored by the multi-workgroup column
+    kernels (k_tall_pivot / k_tall_update): same pivots as LAPACK and factors
+    within complex128 rounding; complex64 a valid pivoted LU; a zero column
+    is reported in info."""
+    import scipy.linalg as sla
+    nat = _lib()
+    # exercise the column kernels above 3072 rows (built in: 12288 / 6144)
+    nat.load().mnpbem_getrf_tall_rows(3072)
+    rng = np.random.default_rng(5)
+    A = _rand(rng, (1, n, n), dtype)
+    tA = torch.from_numpy(A.copy()).to(cuda)
+    piv = torch.empty((1, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((1,), dtype=torch.int32, device=cuda)
+    dt = nat.DTYPE_C128 if dtype == np.complex128 else nat.DTYPE_C64
+    print("getrf tall", dtype.__name__, n, flush=True)
+    nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), 1,
+             nat.stream_handle(cuda))
+    out = tA.cpu().numpy()[0]
+    lu, p = sla.lu_factor(A[0].astype(np.complex128))
+    got = piv[0].cpu().numpy()
+    if dtype == np.complex128:
+        assert np.array_equal(p, got)
+        assert np.abs(out - lu).max() < 1e-10 * np.abs(lu).max()
+    else:
+        # complex64 rounding flips near-ties, after which the pivot sequence
+        # departs from LAPACK's complex128 one: check the factorisation itself,
+        # P A = L U to complex64 accuracy, and the partial-pivoting bound
+        # |l| <= sqrt(2) (pivots chosen by the largest |re|+|im|)
+        L = np.tril(out.astype(np.complex128), -1) + np.eye(n)
+        U = np.triu(out.astype(np.complex128))
+        PA = A[0].astype(np.complex128).copy()
+        for i, q in enumerate(got):
+            PA[[i, q]] = PA[[q, i]]
+        assert np.abs(PA - L @ U).max() < 1e-4 * np.abs(A[0]).max()
+        assert np.abs(L).max() <= np.sqrt(2.0) * (1 + 1e-5)
+    assert int(info.cpu()[0]) == 0
+    # singular: column 300 (inside the tall-panel range, c0 < n - 3072) zero -> info 301
+    Z = np.eye(n, dtype=dtype)[None].copy()
+    Z[0, :, 300] = 0.0
+    tZ = torch.from_numpy(Z).to(cuda)
+    nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(tZ), n, n * n, nat.ptr(piv), n, nat.ptr(info), 1,
+             nat.stream_handle(cuda))
+    nat.load().mnpbem_getrf_tall_rows(0)
+    assert int(info.cpu()[0]) == 301
