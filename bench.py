@@ -109,9 +109,9 @@ def main():
     ap.add_argument("--fd-vertices", type=int, default=20,
                     help="adjoint workload: random vertices checked against central differences")
     ap.add_argument("--seed", type=int, default=0, help="seed of the uneven-particle set")
-    ap.add_argument("--dimer-precond", default="block", choices=["block", "auto"],
-                    help="dimer workload: GMRES preconditioner (block: each sphere's direct solve; auto: the "
-                         "direct solve of the whole dimer)")
+    ap.add_argument("--dimer-precond", default="auto", choices=["block", "auto"],
+                    help="dimer workload: GMRES preconditioner (auto: the direct solve of the whole dimer; "
+                         "block: each sphere's direct solve)")
     ap.add_argument("--eager", action="store_true",
                     help="time the headline sweep with eager launches only (default: value is the replayed "
                          "hipGraph of the 64-wavelength chunk step, eager steps timed beside it)")
@@ -361,8 +361,8 @@ def bench_dimer(args):
     """Gold sphere dimer, two 12500-face golden-spiral spheres of 100 nm with a
     2 nm gap (25000 faces): BEMRetIter on ACA H-matrices (leaf 64, htol 1e-6,
     rank <= 30, blocks in complex64), GMRES restart 50, tol 1e-6, FP64
-    Arnoldi, preconditioned by the mixed-precision direct solve of each
-    sphere (--dimer-precond block, default) or of the whole dimer (auto);
+    Arnoldi, preconditioned by the mixed-precision direct solve of the whole
+    dimer (--dimer-precond auto, default) or of each sphere (block);
     500, 600, 700 nm; x polarisation along the dimer axis, propagation
     along z."""
     import torch
