@@ -72,6 +72,14 @@ int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA, int* ipiv
 int mnpbem_getrs_batched(int dtype, int n, int nrhs, const void* A, long lda, long sA, const int* ipiv, long spiv,
                          void* B, long ldb, long sB, int batch, void* stream);
 
+/* A <- A^-1 from its mnpbem_getrf_batched factors (LAPACK getri): A^-1 =
+ * U^-1 L^-1 P with L^-1 formed by a recursion that skips the zero triangle of
+ * the identity; W (n x n per matrix) and Q (n ints per matrix, n <= 16384) are
+ * work space.  Replaces the explicit inverse that bem_ret.py:437-523 forms by
+ * a solve against the identity. */
+int mnpbem_getri_batched(int dtype, int n, void* A, long lda, long sA, const int* ipiv, long spiv, void* W, long ldw,
+                         long sW, int* Q, long sQ, int batch, void* stream);
+
 /* Y[b] = beta Y[b] + alpha M[b]^T X[b] for n x C blocks X, Y (any C). */
 int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, long ldm, long sM, const void* X, long ldx,
                           long sX, void* Y, long ldy, long sY, double alpha_re, double alpha_im, double beta,

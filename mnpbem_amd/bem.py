@@ -171,11 +171,13 @@ class BEMRet:
         nat.call("mnpbem_sigma_batched", dt, 0, n, nat.ptr(S1), nat.ptr(S2), nat.ptr(Dt), None, n, nn,
                  nat.ptr(g.nvec), nat.ptr(F.par), B, st)
         nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(info[2]), B, st)
-        Et = torch.empty((B, n, n), dtype=cdt, device=dev)
-        nat.call("mnpbem_identity_batched", dt, n, nat.ptr(Et), n, nn, B, st)
-        nat.call("mnpbem_getrs_batched", dt, n, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(Et), n, nn, B,
-                 st)
-        del Dt
+        # (Delta^-1)^T in place of the factors (LAPACK getri's operation count)
+        Wk = torch.empty((B, n, n), dtype=cdt, device=dev)
+        Q = torch.empty((B, n), dtype=torch.int32, device=dev)
+        nat.call("mnpbem_getri_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(Wk), n, nn,
+                 nat.ptr(Q), n, B, st)
+        Et = Dt
+        del Wk, Q
         # Sigma^T overwrites Sigma2^T
         nat.call("mnpbem_sigma_batched", dt, 1, n, nat.ptr(S1), nat.ptr(S2), None, nat.ptr(Et), n, nn,
                  nat.ptr(g.nvec), nat.ptr(F.par), B, st)

@@ -1061,6 +1061,100 @@ int getrf_rec(int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP,
   return laswp<T>(c0 + n1, c0 + n, c0, c0 + n1, A, lda, sA, ipiv, sP, batch, false, st);
 }
 
+// ---------------------------------------------------------------------------
+// Inverse from the LU factors (LAPACK getri's operation count): with P A = L U,
+// A^-1 = U^-1 L^-1 P.  L^-1 is formed by a recursion that keeps the zero
+// upper triangle of the identity out of the work (n^3/6 complex MACs instead
+// of the n^3/2 of a triangular solve against a full identity), the upper
+// solve against it is dense (n^3/2), and P becomes a column scatter.
+// ---------------------------------------------------------------------------
+// X (n x n, identity on entry) <- L^-1, L unit lower triangular
+template <typename T>
+int trsm_lower_ident(int n, const T* L, long ldl, long sL, T* X, long ldx, long sX, int batch, hipStream_t st,
+                     int gmode) {
+  constexpr int TL = TLeaf<T>::n;
+  if (n <= 2 * TL) return trsm_rec<T>(true, n, n, L, ldl, sL, X, ldx, sX, batch, st, gmode);
+  int n1 = ((n / 2 + TL - 1) / TL) * TL;
+  if (n1 >= n) n1 = n - TL;
+  const int n2 = n - n1;
+  int rc;
+  if ((rc = trsm_lower_ident<T>(n1, L, ldl, sL, X, ldx, sX, batch, st, gmode))) return rc;
+  // X21 = -L22^-1 L21 X11 (X21 is zero on entry)
+  if ((rc = gemm_sub<T>(n2, n1, n1, L + 2 * ((long)n1 * ldl), ldl, sL, X, ldx, sX, X + 2 * ((long)n1 * ldx), ldx,
+                        sX, batch, st, gmode)))
+    return rc;
+  if ((rc = trsm_rec<T>(true, n2, n1, L + 2 * ((long)n1 * ldl + n1), ldl, sL, X + 2 * ((long)n1 * ldx), ldx, sX,
+                        batch, st, gmode)))
+    return rc;
+  return trsm_lower_ident<T>(n2, L + 2 * ((long)n1 * ldl + n1), ldl, sL, X + 2 * ((long)n1 * ldx + n1), ldx, sX,
+                             batch, st, gmode);
+}
+
+// W[b] = identity (n x n, leading dimension ld)
+template <typename T>
+__global__ void k_eye(int n, T* W, long ld, long sW) {
+  const long b = blockIdx.y;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  cst(W + 2 * (b * sW + (long)r * ld + c), cmake<T>(r == c ? T(1) : T(0), T(0)));
+}
+
+// perm[b][j]: the row interchanges ipiv[0..n) applied in order to 0..n-1
+// (P I = I[perm]); one workgroup per matrix, the swaps run in LDS.
+__global__ __launch_bounds__(256) void k_perm_array(int n, const int* __restrict__ ipiv, long sP, int* perm,
+                                                    long sQ) {
+  extern __shared__ int arr[];
+  const long b = blockIdx.x;
+  ipiv += b * sP;
+  perm += b * sQ;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) arr[j] = j;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int i = 0; i < n; ++i) {
+      const int p = ipiv[i];
+      if (p != i) {
+        const int t = arr[i];
+        arr[i] = arr[p];
+        arr[p] = t;
+      }
+    }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) perm[j] = arr[j];
+}
+
+// Out[r][perm[i]] = W[r][i]  (A^-1 = W P: column i of W lands in column perm[i])
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter_cols(int n, const T* __restrict__ W, long ldw, long sW,
+                                                      const int* __restrict__ perm, long sQ, T* Out, long ldo,
+                                                      long sO) {
+  const long b = blockIdx.z;
+  const int r = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int c = perm[b * sQ + i];
+  cst(Out + 2 * (b * sO + (long)r * ldo + c), cld(W + 2 * (b * sW + (long)r * ldw + i)));
+}
+
+// A (holding its LU and pivots from getrf) <- A^-1; W: n x n work per matrix
+// (the identity fill, L^-1 and U^-1 L^-1 live there); Q: n ints per matrix.
+template <typename T>
+int getri(int n, T* A, long lda, long sA, const int* ipiv, long sP, T* W, long ldw, long sW, int* Q, long sQ,
+          int batch, hipStream_t st, int gmode) {
+  dim3 gi(mnp_ceil_div((long)n * n, 256), batch);
+  hipLaunchKernelGGL(k_eye<T>, gi, dim3(256), 0, st, n, W, ldw, sW);
+  MNP_LAUNCH_CHECK();
+  int rc;
+  if ((rc = trsm_lower_ident<T>(n, A, lda, sA, W, ldw, sW, batch, st, gmode))) return rc;
+  if ((rc = trsm_rec<T>(false, n, n, A, lda, sA, W, ldw, sW, batch, st, gmode))) return rc;
+  hipLaunchKernelGGL(k_perm_array, dim3(batch), dim3(256), (size_t)n * sizeof(int), st, n, ipiv, sP, Q, sQ);
+  MNP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scatter_cols<T>, dim3(mnp_ceil_div(n, 256), n, batch), dim3(256), 0, st, n, W, ldw, sW, Q,
+                     sQ, A, lda, sA);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1171,6 +1265,25 @@ extern "C" int mnpbem_getrs_batched(int dtype, int n, int nrhs, const void* A, l
     return trsm_rec<float>(false, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st, gm);
   }
   mnp_set_error("getrs: unknown dtype");
+  return 1;
+}
+
+// A <- A^-1 from its getrf factors (LU in A, pivots ipiv), i.e. LAPACK getri;
+// W (n x n per matrix, leading dimension ldw) and Q (n ints per matrix) are
+// work space.  Replaces the inverse that bem_ret.py forms with a solve against
+// the identity.
+extern "C" int mnpbem_getri_batched(int dtype, int n, void* A, long lda, long sA, const int* ipiv, long sP,
+                                    void* W, long ldw, long sW, int* Q, long sQ, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n >= 0 && batch >= 0 && lda >= n && ldw >= n && sQ >= n, "getri: bad size");
+  MNP_REQUIRE((size_t)n * sizeof(int) <= 64 * 1024, "getri: n above 16384");
+  if (n == 0 || batch == 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    return getri<double>(n, (double*)A, lda, sA, ipiv, sP, (double*)W, ldw, sW, Q, sQ, batch, st, 0);
+  if (mnp_is_c64(dtype))
+    return getri<float>(n, (float*)A, lda, sA, ipiv, sP, (float*)W, ldw, sW, Q, sQ, batch, st,
+                        dtype == MNP_DTYPE_C64X ? 1 : 0);
+  mnp_set_error("getri: unknown dtype");
   return 1;
 }
 
