@@ -180,7 +180,7 @@ def main():
     value = len(lams) * args.steps / elapsed
     ms_per_step = 1000.0 * elapsed / args.steps
     gemm_achieved = fl.value / (ms.value * 1e-3) if ms.value > 0 else 0.0
-    path_flops = 34.67 * float(N) ** 3 * len(lams) * args.steps
+    path_flops = 32.0 * float(N) ** 3 * len(lams) * args.steps
     log("mixed: %.2f wavelengths/s, %.1f ms/step; GEMM %d launches, %.3f ms avg, %.1f TFLOP/s"
         % (value, ms_per_step, nl.value, ms.value / max(nl.value, 1), gemm_achieved / 1e12))
 
