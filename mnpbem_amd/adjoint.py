@@ -169,13 +169,19 @@ def _chain_to_vertices(cp, Gr):
     for ip, part in enumerate(cp.p):
         lo, hi = int(cp.offsets[ip]), int(cp.offsets[ip + 1])
         g = G[:, lo:hi]
+        # per-face Jacobian of the 13 geometric outputs (pos, nvec, tvec1,
+        # tvec2, area) with respect to the 18 node coordinates, one backward
+        # pass per output component (faces are independent), then one
+        # contraction with the cotangents of every wavelength and polarisation;
         # grad mode on: this also runs inside autograd.Function.forward
         with torch.enable_grad():
             X = torch.as_tensor(part.nodes, dtype=torch.float64).requires_grad_(True)
-            outs = _face_geometry_torch(X, part.interp == "flat")
-            gouts = [g[:, :, 0:3], g[:, :, 3:6], g[:, :, 6:9], g[:, :, 9:12], g[:, :, 12]]
-            (gX,) = torch.autograd.grad(outs, X, grad_outputs=gouts, is_grads_batched=True)
-        gX = gX + g[:, :, 13:31].reshape(L * P, hi - lo, 6, 3)
+            pos, nv, t1, t2, area = _face_geometry_torch(X, part.interp == "flat")
+            comps = [pos[:, k] for k in range(3)] + [nv[:, k] for k in range(3)] + \
+                    [t1[:, k] for k in range(3)] + [t2[:, k] for k in range(3)] + [area]
+            J = torch.stack([torch.autograd.grad(cpt.sum(), X, retain_graph=(k < 12))[0]
+                             for k, cpt in enumerate(comps)])                # (13, nf, 6, 3)
+        gX = torch.einsum("kfcd,bfk->bfcd", J, g[:, :, 0:13]) + g[:, :, 13:31].reshape(L * P, hi - lo, 6, 3)
         nv2 = len(part.verts2)
         gv2 = torch.zeros((L * P, nv2, 3), dtype=torch.float64)
         idx = torch.as_tensor(part.faces6.reshape(-1), dtype=torch.int64)
