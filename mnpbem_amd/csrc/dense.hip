@@ -12,6 +12,7 @@
 #include "common.h"
 
 #include <cstdlib>
+#include <type_traits>
 
 #include <vector>
 
@@ -242,61 +243,86 @@ __device__ __forceinline__ Tile gemm_tile(int remap) {
   return t;
 }
 
+// ---------------------------------------------------------------------------
+// Software pipeline of the K loop.  A loop that runs the MFMAs of a K slice,
+// then (after waiting for the global loads) the bf16 split and LDS stores of
+// the next slice, then the barrier, sends both waves of a SIMD into the split
+// together, and the matrix pipe idles through its ~150 vector instructions
+// (measured: 283-307 TFLOP/s on large shapes, against 331-347 for this form).
+// Here slice k+1 is already in registers when the MFMAs of slice k start: its
+// split and LDS stores sit between the MFMAs of the first column block, the
+// global loads of slice k+2 after them.  While slice k+2 lies inside K the
+// body is one basic block with compile-time stages (the loop is unrolled by
+// two), so the compiler may interleave the split with the MFMAs and move LDS
+// writes of one stage past reads of the other: rows of A beyond M and
+// columns of B beyond N are clamped to the last row / column instead of
+// zero-filled (they only feed outputs that are not stored).  The slices near
+// the end of K go through a generic loop with zero fill.  Explicit
+// interleaving with sched_group_barrier (split among the MFMAs, loads
+// after them or right after the split of their registers) was measured
+// no faster than the compiler's own order.
+// ---------------------------------------------------------------------------
 template <int BM, int BN, int WM>
-__global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3(int M, int N, int K, const float* __restrict__ A, long lda,
-                                                          long sA, const float* __restrict__ B, long ldb, long sB,
-                                                          float* __restrict__ C, long ldc, long sC, int remap) {
+__global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3(
+    int M, int N, int K, const float* __restrict__ A, long lda, long sA, const float* __restrict__ B, long ldb,
+    long sB, float* __restrict__ C, long ldc, long sC, int remap) {
   const Tile tl = gemm_tile(remap);
   const long bz = tl.z;
   A += 2 * bz * sA;
   B += 2 * bz * sB;
   C += 2 * bz * sC;
   constexpr int WN = 32, FI = WM / 16, FJ = WN / 16, NWN = BN / WN;
+  static_assert(FJ == 2, "two column blocks per wave");
   const int m0 = tl.y * BM, n0 = tl.x * BN;
-  // two stages x four planes (re_hi, re_lo, im_hi, im_lo) x 128 rows x 32
   __shared__ __attribute__((aligned(16))) __bf16 As[2][4][BM * XK];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[2][4][BN * XK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave / NWN) * WM, wn = (wave % NWN) * WN;
-  // staging: each thread owns 8 consecutive k of one A row and of one B column
   const int la_r = tid >> 2, la_c = tid & 3;
   const int lb_n = tid % BN, lb_c = tid / BN;
+  const int ar = min(m0 + la_r, M - 1), bc = min(n0 + lb_n, N - 1);
+  const float* Ap = A + 2 * ((long)ar * lda + la_c * 8);
+  const float* Bp = B + 2 * ((long)(lb_c * 8) * ldb + bc);
   float ra[16], rb[16];
-  auto gload = [&](int k0) {
-    const int gr = m0 + la_r;
-    const int gk = k0 + la_c * 8;
-    if (gr < M && gk + 8 <= K) {
-      const float4* src = reinterpret_cast<const float4*>(A + 2 * ((long)gr * lda + gk));
+  // slice k0 .. k0 + 31 inside K
+  auto gload_full = [&](int k0) {
+    const float4* src = reinterpret_cast<const float4*>(Ap + 2 * k0);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float4 v = src[q];
-        ra[4 * q] = v.x;
-        ra[4 * q + 1] = v.y;
-        ra[4 * q + 2] = v.z;
-        ra[4 * q + 3] = v.w;
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const bool ok = gr < M && gk + q < K;
-        const float2 v = ok ? *reinterpret_cast<const float2*>(A + 2 * ((long)gr * lda + gk + q)) : make_float2(0.f, 0.f);
-        ra[2 * q] = v.x;
-        ra[2 * q + 1] = v.y;
-      }
+    for (int q = 0; q < 4; ++q) {
+      const float4 v = src[q];
+      ra[4 * q] = v.x;
+      ra[4 * q + 1] = v.y;
+      ra[4 * q + 2] = v.z;
+      ra[4 * q + 3] = v.w;
     }
-    const int gc = n0 + lb_n;
+    const float* b = Bp + 2 * (long)k0 * ldb;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const int gk2 = k0 + lb_c * 8 + q;
-      const float2 v = (gc < N && gk2 < K) ? *reinterpret_cast<const float2*>(B + 2 * ((long)gk2 * ldb + gc))
-                                           : make_float2(0.f, 0.f);
+      const float2 v = *reinterpret_cast<const float2*>(b + 2 * (long)q * ldb);
       rb[2 * q] = v.x;
       rb[2 * q + 1] = v.y;
     }
   };
+  // slice at k0 with k beyond K zero-filled
+  auto gload_edge = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int gk = k0 + la_c * 8 + q;
+      const float2 v = gk < K ? *reinterpret_cast<const float2*>(A + 2 * ((long)ar * lda + gk)) : make_float2(0.f, 0.f);
+      ra[2 * q] = v.x;
+      ra[2 * q + 1] = v.y;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int gk = k0 + lb_c * 8 + q;
+      const float2 v = gk < K ? *reinterpret_cast<const float2*>(B + 2 * ((long)gk * ldb + bc)) : make_float2(0.f, 0.f);
+      rb[2 * q] = v.x;
+      rb[2 * q + 1] = v.y;
+    }
+  };
+  const int oa = xsw(la_r, la_c), ob = xsw(lb_n, lb_c);
   auto sstore = [&](int stg) {
     bf16x8 h, l;
-    const int oa = xsw(la_r, la_c), ob = xsw(lb_n, lb_c);
     split8(ra, h, l);
     *reinterpret_cast<bf16x8*>(&As[stg][0][oa]) = h;
     *reinterpret_cast<bf16x8*>(&As[stg][1][oa]) = l;
@@ -318,58 +344,94 @@ __global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3
       accR[i][j] = f32x4{0, 0, 0, 0};
       accI[i][j] = f32x4{0, 0, 0, 0};
     }
-  gload(0);
-  sstore(0);
-  __syncthreads();
   const int fr = lane & 15, fc = lane >> 4;
-  int stg = 0;
-  for (int k0 = 0; k0 < K; k0 += XK) {
-    const bool more = (k0 + XK) < K;
-    if (more) gload(k0 + XK);
-    bf16x8 a[FI][4];  // [row block][plane]
+  int oaf[FI], obf[FJ];
 #pragma unroll
-    for (int i = 0; i < FI; ++i) {
-      const int o = xsw(wm + 16 * i + fr, fc);
+  for (int i = 0; i < FI; ++i) oaf[i] = xsw(wm + 16 * i + fr, fc);
 #pragma unroll
-      for (int pl = 0; pl < 4; ++pl) a[i][pl] = *reinterpret_cast<const bf16x8*>(&As[stg][pl][o]);
-    }
+  for (int j = 0; j < FJ; ++j) obf[j] = xsw(wn + 16 * j + fr, fc);
+  // the 12 MFMAs of column block j, in the order of k_cgemm_sub_bf16x3
+  auto mma = [&](int j, const bf16x8 (&a)[FI][4], const bf16x8 (&b)[4]) {
+    const bf16x8 brh = b[0], brl = b[1], bih = b[2], bil = b[3];
+    const bf16x8 nbih = bneg(bih), nbil = bneg(bil);
 #pragma unroll
-    for (int j = 0; j < FJ; ++j) {
-      const int o = xsw(wn + 16 * j + fr, fc);
-      const bf16x8 brh = *reinterpret_cast<const bf16x8*>(&Bs[stg][0][o]);
-      const bf16x8 brl = *reinterpret_cast<const bf16x8*>(&Bs[stg][1][o]);
-      const bf16x8 bih = *reinterpret_cast<const bf16x8*>(&Bs[stg][2][o]);
-      const bf16x8 bil = *reinterpret_cast<const bf16x8*>(&Bs[stg][3][o]);
-      const bf16x8 nbih = bneg(bih), nbil = bneg(bil);
+    for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][0], brh, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][0], brh, accR[i][j]);
+    for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][0], bih, accI[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][0], bih, accI[i][j]);
+    for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][0], brl, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][0], brl, accR[i][j]);
+    for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][0], bil, accI[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][0], bil, accI[i][j]);
+    for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][1], brh, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][1], brh, accR[i][j]);
+    for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][1], bih, accI[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][1], bih, accI[i][j]);
+    for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][2], nbih, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][2], nbih, accR[i][j]);
+    for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][2], brh, accI[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][2], brh, accI[i][j]);
+    for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][2], nbil, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][2], nbil, accR[i][j]);
+    for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][2], brl, accI[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][2], brl, accI[i][j]);
+    for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][3], nbih, accR[i][j]);
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accR[i][j] = mfma_bf16(a[i][3], nbih, accR[i][j]);
+    for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][3], brh, accI[i][j]);
+  };
+  auto frags = [&](int stg, bf16x8 (&a)[FI][4], bf16x8 (&b)[FJ][4]) {
 #pragma unroll
-      for (int i = 0; i < FI; ++i) accI[i][j] = mfma_bf16(a[i][3], brh, accI[i][j]);
-    }
-    // the other stage is free: every wave passed the barrier after reading it
-    if (more) sstore(stg ^ 1);
+    for (int i = 0; i < FI; ++i)
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) a[i][pl] = *reinterpret_cast<const bf16x8*>(&As[stg][pl][oaf[i]]);
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) b[j][pl] = *reinterpret_cast<const bf16x8*>(&Bs[stg][pl][obf[j]]);
+  };
+  // steady state: slice s from stage S, slice s+1 (registers) to stage S^1,
+  // slice s+2 (inside K) loaded into the registers
+  auto body = [&](auto stc, int s) {
+    constexpr int S = decltype(stc)::value;
+    bf16x8 a[FI][4], b[FJ][4];
+    frags(S, a, b);
+    mma(0, a, b[0]);
+    sstore(S ^ 1);
+    mma(1, a, b[1]);
+    gload_full((s + 2) * XK);
     __syncthreads();
-    stg ^= 1;
+  };
+  const int nk = (K + XK - 1) / XK, nfull = K / XK;
+  if (nfull >= 1)
+    gload_full(0);
+  else
+    gload_edge(0);
+  sstore(0);
+  if (nk > 1) {
+    if (nfull >= 2)
+      gload_full(XK);
+    else
+      gload_edge(XK);
+  }
+  __syncthreads();
+  int s = 0;
+  for (; s + 3 < nfull; s += 2) {
+    body(std::integral_constant<int, 0>(), s);
+    body(std::integral_constant<int, 1>(), s + 1);
+  }
+  if (s + 2 < nfull) {
+    body(std::integral_constant<int, 0>(), s);
+    ++s;
+  }
+  for (; s < nk; ++s) {
+    const int stg = s & 1;
+    bf16x8 a[FI][4], b[FJ][4];
+    frags(stg, a, b);
+    mma(0, a, b[0]);
+    mma(1, a, b[1]);
+    if (s + 1 < nk) sstore(stg ^ 1);
+    if (s + 2 < nk) gload_edge((s + 2) * XK);
+    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < FI; ++i)

@@ -41,7 +41,34 @@ def test_cgemm_sub(cuda, dtype, tol, split, M, N, K):
     assert err < tol, err
 
 
-@pytest.mark.parametrize("M,N,K", [(64, 300, 64), (130, 129, 200), (17, 64, 33), (200, 70, 96), (1, 5, 1)])
+@pytest.mark.parametrize("M,N,K", [(130, 140, 200), (300, 129, 96), (64, 64, 33)])
+def test_bf16x3_strided_operands(cuda, M, N, K):
+    """Operands that are views into larger matrices (leading dimensions above
+    the row length, odd ones included, batch strides above the matrix size):
+    the bf16x3 GEMM against numpy, with the padding of C left untouched."""
+    nat = _lib()
+    rng = np.random.default_rng(M + N + K)
+    B = 2
+    lda, ldb, ldc = K + 3, N + 5, N + 7
+    A = _rand(rng, (B, M + 1, lda), np.complex64)
+    Bm = _rand(rng, (B, K + 2, ldb), np.complex64)
+    C = _rand(rng, (B, M + 2, ldc), np.complex64)
+    ref = C.astype(np.complex128).copy()
+    ref[:, :M, :N] -= A[:, :M, :K].astype(np.complex128) @ Bm[:, :K, :N].astype(np.complex128)
+    tA, tB, tC = (torch.from_numpy(x).to(cuda) for x in (A, Bm, C))
+    print("bf16x3 strided", M, N, K, flush=True)
+    for v in (0, 1):
+        tC.copy_(torch.from_numpy(C))
+        nat.call("mnpbem_cgemm_sub_variant", v, M, N, K, nat.ptr(tA), lda, (M + 1) * lda, nat.ptr(tB), ldb,
+                 (K + 2) * ldb, nat.ptr(tC), ldc, (M + 2) * ldc, B, nat.stream_handle(cuda))
+        out = tC.cpu().numpy()
+        err = np.abs(out[:, :M, :N] - ref[:, :M, :N]).max() / np.abs(ref[:, :M, :N]).max()
+        assert err < 1e-4, (v, err)
+        assert np.array_equal(out[:, M:, :], C[:, M:, :]) and np.array_equal(out[:, :, N:], C[:, :, N:])
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 300, 64), (130, 129, 200), (17, 64, 33), (200, 70, 96), (1, 5, 1),
+                                   (257, 300, 161), (129, 131, 224), (65, 66, 97), (40, 50, 31), (128, 128, 128)])
 def test_bf16x3_tile_variants_identical(cuda, M, N, K):
     """The two tile variants of the bf16x3 GEMM (and the table-routed call)
     give bit-identical results: each output element is accumulated over K in
