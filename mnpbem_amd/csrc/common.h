@@ -85,15 +85,19 @@ template <typename T>
 __device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) {
   return cmake(a.re * s, a.im * s);
 }
-// a - b*c
+// fused multiply-add x*y + z, one rounding
+__device__ __forceinline__ float fmadd(float x, float y, float z) { return __builtin_fmaf(x, y, z); }
+__device__ __forceinline__ double fmadd(double x, double y, double z) { return __builtin_fma(x, y, z); }
+// a - b*c as four fused multiply-adds per complex value (the accumulator
+// enters the first one, so a chain of updates costs 4 instructions each)
 template <typename T>
 __device__ __forceinline__ cplx<T> cfms(cplx<T> a, cplx<T> b, cplx<T> c) {
-  return cmake(a.re - (b.re * c.re - b.im * c.im), a.im - (b.re * c.im + b.im * c.re));
+  return cmake(fmadd(b.im, c.im, fmadd(-b.re, c.re, a.re)), fmadd(-b.im, c.re, fmadd(-b.re, c.im, a.im)));
 }
 // a + b*c
 template <typename T>
 __device__ __forceinline__ cplx<T> cfma(cplx<T> a, cplx<T> b, cplx<T> c) {
-  return cmake(a.re + (b.re * c.re - b.im * c.im), a.im + (b.re * c.im + b.im * c.re));
+  return cmake(fmadd(-b.im, c.im, fmadd(b.re, c.re, a.re)), fmadd(b.im, c.re, fmadd(b.re, c.im, a.im)));
 }
 // 1 / a, scaled to avoid overflow (Smith's algorithm)
 template <typename T>
