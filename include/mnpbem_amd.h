@@ -99,6 +99,14 @@ int mnpbem_lu_solve_t_batched(int dtype, int n, int C, const void* LU, long ld, 
  * of this library (the reference's LAPACK/cuSOLVER getrf has none). */
 int mnpbem_getrf_tall_rows(int rows);
 
+/* Columns (right-hand sides x batch) up to which the leaves of the triangular
+ * solves (the diagonal blocks of 33..64 rows complex64, 17..32 complex128
+ * inside getrf, getrs and getri) run a row-blocked kernel (four waves over
+ * four row blocks of 64 columns) instead of one thread per column; cols < 0
+ * restores the built-in 32768, 0 disables it.  Returns the previous setting.
+ * A tuning knob of this library (no reference counterpart). */
+int mnpbem_trsm_rows_max(int cols);
+
 /* Inverses of the 64 x 64 diagonal blocks of getrf factors ([U_II^-1 | L_II^-1]
  * per block, sD >= ceil(n/64)*2*64*64 elements) so that the block steps of
  * mnpbem_lu_solve_t_batched are parallel matrix-vector products (Dinv = NULL

@@ -58,6 +58,7 @@ SIGNATURES = {
     "mnpbem_grad_refined": [_i, _i, _i, _i] + [_p] * 13 + [_i, _p, _i, _p, _i, _p, _p, _p, _p, _i, _p],
     "mnpbem_grad_local": [_i, _i, _p, _p, _p, _p, _l, _l] + [_p] * 11 + [_i, _p],
     "mnpbem_getrf_tall_rows": [_i],
+    "mnpbem_trsm_rows_max": [_i],
     "mnpbem_getri_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _p, _l, _i, _p],
     "mnpbem_accumulate": [_i, _l, _p, _p, _p],
     "mnpbem_convert": [_i, _l, _p, _p, _p],

@@ -673,6 +673,74 @@ __global__ __launch_bounds__(256) void k_trsm_leaf(int n, int nrhs, const T* Tm,
   }
 }
 
+// The same leaf for few right-hand sides, where one thread per column leaves
+// the substitution chain of TLN steps as the whole launch time.  A workgroup
+// takes 64 columns of one matrix (lane = column) and its four waves take four
+// blocks of RB = TLN / 4 rows (upper-triangular form as above).  Block k (k =
+// 3 .. 0): wave k solves its diagonal block in registers, writes the solved
+// rows to LDS; after the barrier, waves q < k subtract T[block q][block k]
+// times them from their rows.  Four barriers per solve; the result equals
+// the substitution up to rounding (the sums run in another order).
+template <typename T, bool LOWER, int TLN>
+__global__ __launch_bounds__(256) void k_trsm_leaf_rows(int n, int nrhs, const T* Tm, long ldt, long sT, T* B,
+                                                        long ldb, long sB) {
+  constexpr int RB = TLN / 4;
+  const long b = blockIdx.y;
+  Tm += 2 * b * sT;
+  B += 2 * b * sB;
+  __shared__ cplx<T> Ts[TLN][TLN];
+  __shared__ cplx<T> rd[TLN];
+  __shared__ cplx<T> xs[TLN][64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int e = tid; e < TLN * TLN; e += 256) {
+    const int i = e / TLN, j = e % TLN;
+    const int si = LOWER ? n - 1 - i : i, sj = LOWER ? n - 1 - j : j;
+    Ts[i][j] = (i < n && j < n) ? cld(Tm + 2 * ((long)si * ldt + sj)) : cmake<T>(0, 0);
+  }
+  __syncthreads();
+  if (tid < TLN) rd[tid] = (tid >= n) ? cmake<T>(0, 0) : (LOWER ? cmake<T>(1, 0) : crecip(Ts[tid][tid]));
+  const int gc = blockIdx.x * 64 + lane;
+  const bool live = gc < nrhs;
+  cplx<T> x[RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    const int r = wv * RB + i;
+    const int si = LOWER ? n - 1 - r : r;
+    x[i] = (live && r < n) ? cld(B + 2 * ((long)si * ldb + gc)) : cmake<T>(0, 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    if (wv == k) {
+#pragma unroll
+      for (int i = RB - 1; i >= 0; --i) {
+#pragma unroll
+        for (int l = i + 1; l < RB; ++l) x[i] = cfms(x[i], Ts[k * RB + i][k * RB + l], x[l]);
+        x[i] = cmul(x[i], rd[k * RB + i]);
+      }
+#pragma unroll
+      for (int i = 0; i < RB; ++i) xs[k * RB + i][lane] = x[i];
+    }
+    __syncthreads();
+    if (wv < k) {
+#pragma unroll
+      for (int l = 0; l < RB; ++l) {
+        const cplx<T> xl = xs[k * RB + l][lane];
+#pragma unroll
+        for (int i = 0; i < RB; ++i) x[i] = cfms(x[i], Ts[wv * RB + i][k * RB + l], xl);
+      }
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int r = wv * RB + i;
+      const int si = LOWER ? n - 1 - r : r;
+      if (r < n) cst(B + 2 * ((long)si * ldb + gc), x[i]);
+    }
+  }
+}
+
 __global__ void k_fill_int(int* p, long n, int v) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -789,6 +857,15 @@ int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, lon
   return 0;
 }
 
+// Columns (right-hand sides x batch) up to which a triangular leaf of more
+// than half the leaf size runs k_trsm_leaf_rows instead of one thread per
+// column (set by mnpbem_trsm_rows_max; 0 disables).  Measured on the leaves
+// of 64 rows: 50 against 69-82 us per getrs (both leaves) up to 500 x 64
+// columns, 178 against 102 us at 760 x 128; the row-blocked form has the
+// shorter chain but less throughput.
+constexpr int TRSM_ROWS_DEFAULT = 32768;
+long g_trsm_rows_max = TRSM_ROWS_DEFAULT;
+
 // X <- T^-1 X, T n x n at Tm (lower unit or upper), X n x nrhs
 template <typename T>
 int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, long ldx, long sX, int batch,
@@ -796,6 +873,16 @@ int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, 
   if (n <= 0 || nrhs <= 0) return 0;
   constexpr int TL = TLeaf<T>::n;
   if (n <= TL) {
+    if (n > TL / 2 && (long)nrhs * batch <= g_trsm_rows_max) {
+      dim3 grid(mnp_ceil_div(nrhs, 64), batch);
+      if (lower)
+        hipLaunchKernelGGL((k_trsm_leaf_rows<T, true, TL>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx, sX);
+      else
+        hipLaunchKernelGGL((k_trsm_leaf_rows<T, false, TL>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx,
+                           sX);
+      MNP_LAUNCH_CHECK();
+      return 0;
+    }
     dim3 grid(mnp_ceil_div(nrhs, 256), batch);
     // smallest unrolled size that holds the triangle
 #define MNP_TRSM_LEAF(SZ)                                                                                    \
@@ -1254,6 +1341,15 @@ extern "C" int mnpbem_cgemm_sub_variant(int variant, int M, int N, int K, const 
 // kernels instead of the register-resident leaf (rows > 0), or the built-in
 // thresholds (rows = 0: 12288 for complex64, 6144 for complex128).  Returns
 // the previous setting.
+// Columns (right-hand sides x batch) up to which the triangular-solve leaves
+// (64 rows complex64, 32 complex128) use the row-blocked kernel; cols < 0
+// restores the built-in 32768, 0 disables it.  Returns the previous setting.
+extern "C" int mnpbem_trsm_rows_max(int cols) {
+  const int prev = (int)g_trsm_rows_max;
+  g_trsm_rows_max = cols < 0 ? TRSM_ROWS_DEFAULT : cols;
+  return prev;
+}
+
 extern "C" int mnpbem_getrf_tall_rows(int rows) {
   const int prev = g_tall_rows;
   g_tall_rows = rows > 0 ? rows : 0;

@@ -155,6 +155,44 @@ def test_getrf_getrs(cuda, dtype, tol, split, n):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("dtype,tol,split", [(np.complex128, 1e-12, False), (np.complex64, 1e-4, False),
+                                             (np.complex64, 1e-3, True)])
+@pytest.mark.parametrize("n,nrhs", [(64, 5), (61, 64), (200, 130), (200, 1100), (45, 300), (300, 1)])
+def test_trsm_row_blocked_leaves(cuda, dtype, tol, split, n, nrhs):
+    """Triangular leaves with few right-hand sides run the row-blocked kernel
+    (inside getrf's recursion, and in getrs up to 32768 columns over the
+    batch): the factors and solutions match numpy's solve, and the solution
+    matches the one-thread-per-column leaves (knob at 0) to rounding."""
+    nat = _lib()
+    lib = nat.load()
+    rng = np.random.default_rng(n + nrhs)
+    B = 2
+    A = (_rand(rng, (B, n, n), dtype) + (2.0 * np.sqrt(n)) * np.eye(n)[None]).astype(dtype)
+    X = _rand(rng, (B, n, nrhs), dtype)
+    ref = np.linalg.solve(A.astype(np.complex128), X.astype(np.complex128))
+    dt = nat.DTYPE_C128 if dtype == np.complex128 else (nat.DTYPE_C64X if split else nat.DTYPE_C64)
+    st = nat.stream_handle(cuda)
+    outs = []
+    prev = lib.mnpbem_trsm_rows_max(-1)
+    try:
+        for knob in (-1, 0):
+            lib.mnpbem_trsm_rows_max(knob)
+            tA, tX = torch.from_numpy(A.copy()).to(cuda), torch.from_numpy(X.copy()).to(cuda)
+            piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
+            info = torch.empty((B,), dtype=torch.int32, device=cuda)
+            print("trsm rows leaf", dtype.__name__, split, n, nrhs, knob, flush=True)
+            nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
+            nat.call("mnpbem_getrs_batched", dt, n, nrhs, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(tX), nrhs,
+                     n * nrhs, B, st)
+            assert np.all(info.cpu().numpy() == 0)
+            outs.append(tX.cpu().numpy())
+    finally:
+        lib.mnpbem_trsm_rows_max(prev)
+    for o in outs:
+        assert np.abs(o - ref).max() / np.abs(ref).max() < tol
+    assert np.abs(outs[0] - outs[1]).max() / np.abs(ref).max() < tol
+
+
 def test_getrf_pivoting_matches_lapack(cuda):
     """Random (non-dominant) matrices force row interchanges; pivots must
     equal LAPACK zgetrf's (largest |re|+|im|, first on ties)."""
