@@ -746,6 +746,39 @@ __global__ void k_fill_int(int* p, long n, int v) {
   if (i < n) p[i] = v;
 }
 
+// C[b] -= A[b] B[b] for thin complex64 products (N <= 16, K <= 64: the
+// updates at the bottom of the LU recursion, M up to the full height, N = K
+// = 8 or 16) in FP32 on the vector units.  On the 64 x 64 bf16x3 tile such a
+// product is mostly padding and its launch time is latency; here one thread
+// computes one output element (K fused complex multiply-adds in order of k,
+// then C - sum), rows of 256 / N per workgroup, B staged in LDS.  More
+// accurate than the split-bf16 product.
+constexpr int THIN_N = 16, THIN_K = 64;
+__global__ __launch_bounds__(256) void k_cgemm_thin(int M, int N, int K, const float* __restrict__ A, long lda,
+                                                    long sA, const float* __restrict__ B, long ldb, long sB,
+                                                    float* __restrict__ C, long ldc, long sC) {
+  const long b = blockIdx.y;
+  A += 2 * b * sA;
+  B += 2 * b * sB;
+  C += 2 * b * sC;
+  __shared__ cplx<float> Bs[THIN_K][THIN_N];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < K * N; e += 256) {
+    const int k = e / N, c = e % N;
+    Bs[k][c] = cld(B + 2 * ((long)k * ldb + c));
+  }
+  __syncthreads();
+  const int rpb = 256 / N;
+  const int rr = tid / N, c = tid % N;
+  const int r = blockIdx.x * rpb + rr;
+  if (rr >= rpb || r >= M) return;
+  const float* a = A + 2 * (long)r * lda;
+  cplx<float> acc = cmake<float>(0.f, 0.f);
+  for (int k = 0; k < K; ++k) acc = cfma(acc, cld(a + 2 * k), Bs[k][c]);
+  float* p = C + 2 * ((long)r * ldc + c);
+  cst(p, csub(cld(p), acc));
+}
+
 // ---------------------------------------------------------------------------
 // host-side recursion
 // ---------------------------------------------------------------------------
@@ -826,6 +859,14 @@ template <typename T>
 int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, long ldb, long sB, T* C, long ldc,
              long sC, int batch, hipStream_t st, int gmode) {
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return 0;
+  if constexpr (sizeof(T) == 4) {
+    if (gmode == 1 && N <= THIN_N && K <= THIN_K) {
+      dim3 grid(mnp_ceil_div(M, 256 / N), batch);
+      hipLaunchKernelGGL(k_cgemm_thin, grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
+      MNP_LAUNCH_CHECK();
+      return 0;
+    }
+  }
   const bool timed = g_prof.on && g_prof.used + 2 <= g_prof.ev.size();
   if (timed) MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used], st));
   if constexpr (sizeof(T) == 4) {

@@ -41,6 +41,30 @@ def test_cgemm_sub(cuda, dtype, tol, split, M, N, K):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("M,N,K", [(2884, 8, 8), (1000, 16, 16), (300, 1, 64), (5, 16, 3), (700, 13, 40)])
+def test_thin_products_fp32(cuda, M, N, K):
+    """Complex64 products with N <= 16 and K <= 64 (the bottom of the LU
+    recursion) run on the FP32 vector kernel: against numpy to FP32 accuracy,
+    with strided operands and the padding of C untouched."""
+    nat = _lib()
+    rng = np.random.default_rng(M + 3 * N + 7 * K)
+    B = 3
+    lda, ldb, ldc = K + 2, N + 1, N + 5
+    A = _rand(rng, (B, M, lda), np.complex64)
+    Bm = _rand(rng, (B, K, ldb), np.complex64)
+    C = _rand(rng, (B, M, ldc), np.complex64)
+    ref = C.astype(np.complex128).copy()
+    ref[:, :, :N] -= A[:, :, :K].astype(np.complex128) @ Bm[:, :, :N].astype(np.complex128)
+    tA, tB, tC = (torch.from_numpy(x).to(cuda) for x in (A, Bm, C))
+    print("thin product", M, N, K, flush=True)
+    nat.call("mnpbem_cgemm_sub_batched", nat.DTYPE_C64X, M, N, K, nat.ptr(tA), lda, M * lda, nat.ptr(tB), ldb,
+             K * ldb, nat.ptr(tC), ldc, M * ldc, B, nat.stream_handle(cuda))
+    out = tC.cpu().numpy()
+    scale = np.abs(ref[:, :, :N]).max() + np.sqrt(K) * 2.0
+    assert np.abs(out[:, :, :N] - ref[:, :, :N]).max() / scale < 3e-6
+    assert np.array_equal(out[:, :, N:], C[:, :, N:])
+
+
 @pytest.mark.parametrize("M,N,K", [(130, 140, 200), (300, 129, 96), (64, 64, 33)])
 def test_bf16x3_strided_operands(cuda, M, N, K):
     """Operands that are views into larger matrices (leading dimensions above
@@ -91,8 +115,10 @@ def test_bf16x3_tile_variants_identical(cuda, M, N, K):
                  N, M * N, B, st)
         outs.append(tC.cpu().numpy())
     saved = at.get_table()
+    # products with N <= 16 and K <= 64 leave the table for the FP32 kernel
+    thin = N <= 16 and K <= 64
     try:
-        for v in (0, 1):
+        for v in ((0, 1) if not thin else ()):
             at.set_table(np.full(at.NCELL, v, dtype=np.uint8))
             tC = torch.from_numpy(C).to(cuda)
             nat.call("mnpbem_cgemm_sub_batched", nat.DTYPE_C64X, M, N, K, nat.ptr(tA), K, M * K, nat.ptr(tB), N,
