@@ -566,7 +566,9 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int N, int c0, int w, T* A, long
         for (int cc = c + 1; cc < W; ++cc) a[s][cc] = cfms(a[s][cc], l, rowp[cc]);
       }
     }
-    __syncthreads();
+    // no barrier here: the next column writes rowp / rowc only after its
+    // first barrier, which every thread reaches after finishing this column,
+    // and its candidates go to the other parity of sval / sidx
     }
   }
 #pragma unroll
@@ -746,14 +748,16 @@ __global__ void k_fill_int(int* p, long n, int v) {
   if (i < n) p[i] = v;
 }
 
-// C[b] -= A[b] B[b] for thin complex64 products (N <= 16, K <= 64: the
+// C[b] -= A[b] B[b] for thin complex64 products (N <= 32, K <= 64: the
 // updates at the bottom of the LU recursion, M up to the full height, N = K
-// = 8 or 16) in FP32 on the vector units.  On the 64 x 64 bf16x3 tile such a
-// product is mostly padding and its launch time is latency; here one thread
-// computes one output element (K fused complex multiply-adds in order of k,
-// then C - sum), rows of 256 / N per workgroup, B staged in LDS.  More
-// accurate than the split-bf16 product.
-constexpr int THIN_N = 16, THIN_K = 64;
+// = 8, 16 or 32) in FP32 on the vector units.  On the 64 x 64 bf16x3 tile
+// such a product is mostly padding and its launch time is latency; here one
+// thread computes one output element (K fused complex multiply-adds in order
+// of k, then C - sum), rows of 256 / N per workgroup, B staged in LDS.  More
+// accurate than the split-bf16 product.  Measured on the 2884-face sweep:
+// 149.5 wavelengths/s with every product on the bf16x3 tiles, 152.5 with this
+// kernel up to N = 16, 154.5 up to N = 32, 148.9 up to N = 64.
+constexpr int THIN_N = 32, THIN_K = 64;
 __global__ __launch_bounds__(256) void k_cgemm_thin(int M, int N, int K, const float* __restrict__ A, long lda,
                                                     long sA, const float* __restrict__ B, long ldb, long sB,
                                                     float* __restrict__ C, long ldc, long sC) {

@@ -41,9 +41,10 @@ def test_cgemm_sub(cuda, dtype, tol, split, M, N, K):
     assert err < tol, err
 
 
-@pytest.mark.parametrize("M,N,K", [(2884, 8, 8), (1000, 16, 16), (300, 1, 64), (5, 16, 3), (700, 13, 40)])
+@pytest.mark.parametrize("M,N,K", [(2884, 8, 8), (1000, 16, 16), (300, 1, 64), (5, 16, 3), (700, 13, 40),
+                                   (2884, 32, 32), (129, 31, 64)])
 def test_thin_products_fp32(cuda, M, N, K):
-    """Complex64 products with N <= 16 and K <= 64 (the bottom of the LU
+    """Complex64 products with N <= 32 and K <= 64 (the bottom of the LU
     recursion) run on the FP32 vector kernel: against numpy to FP32 accuracy,
     with strided operands and the padding of C untouched."""
     nat = _lib()
@@ -115,8 +116,8 @@ def test_bf16x3_tile_variants_identical(cuda, M, N, K):
                  N, M * N, B, st)
         outs.append(tC.cpu().numpy())
     saved = at.get_table()
-    # products with N <= 16 and K <= 64 leave the table for the FP32 kernel
-    thin = N <= 16 and K <= 64
+    # products with N <= 32 and K <= 64 leave the table for the FP32 kernel
+    thin = N <= 32 and K <= 64
     try:
         for v in ((0, 1) if not thin else ()):
             at.set_table(np.full(at.NCELL, v, dtype=np.uint8))
