@@ -89,6 +89,63 @@ __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restric
   }
 }
 
+// The four products of the complex128 residual, Y = [G1^T.. buffers applied:
+// G1 X1 | H1 X1 | G2 X2 | H2 X2], in one launch (blockIdx.z = product), with
+// accumulators for exactly the CC = 4P columns.  One product per launch with
+// accumulators for 16 columns kept three waves per SIMD in flight and read
+// the matrices at 4.2 TB/s.  Same loop and summation order as k_gemv_t.
+template <int CC>
+__global__ __launch_bounds__(256) void k_gemv4_t(int n, const double* __restrict__ M0, const double* __restrict__ M1,
+                                                 const double* __restrict__ M2, const double* __restrict__ M3,
+                                                 long ldm, long sM, const double* __restrict__ X, long ldx, long sX,
+                                                 double* __restrict__ Y, long ldy, long sY) {
+  const int q = blockIdx.z;
+  const double* M = q == 0 ? M0 : q == 1 ? M1 : q == 2 ? M2 : M3;
+  const long b = blockIdx.y;
+  M += 2 * b * sM;
+  const double* Xq = X + 2 * (b * sX + (q >= 2 ? CC : 0));
+  double* Yq = Y + 2 * (b * sY + (long)q * CC);
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * 256 + tid;
+  __shared__ cplx<double> xs[64][CC];
+  cplx<double> acc[CC];
+#pragma unroll
+  for (int c = 0; c < CC; ++c) acc[c] = cmake<double>(0, 0);
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int nj = min(64, n - j0);
+    __syncthreads();
+    for (int e = tid; e < nj * CC; e += 256) {
+      const int jj = e / CC, c = e % CC;
+      xs[jj][c] = cld(Xq + 2 * ((long)(j0 + jj) * ldx + c));
+    }
+    __syncthreads();
+    if (i < n) {
+      int jj = 0;
+      for (; jj + 16 <= nj; jj += 16) {
+        cplx<double> m[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) m[u] = cld(M + 2 * ((long)(j0 + jj + u) * ldm + i));
+        // keep the 16 row loads issued ahead of the arithmetic (left to itself
+        // the compiler interleaves them with the FMAs, fewer bytes in flight)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+#pragma unroll
+          for (int c = 0; c < CC; ++c) acc[c] = cfma(acc[c], m[u], xs[jj + u][c]);
+      }
+      for (; jj < nj; ++jj) {
+        const cplx<double> m = cld(M + 2 * ((long)(j0 + jj) * ldm + i));
+#pragma unroll
+        for (int c = 0; c < CC; ++c) acc[c] = cfma(acc[c], m, xs[jj][c]);
+      }
+    }
+  }
+  if (i < n) {
+#pragma unroll
+    for (int c = 0; c < CC; ++c) cst(Yq + 2 * ((long)i * ldy + c), acc[c]);
+  }
+}
+
 // One 64-row block step of the solve M x = v with the LU of M^T in the
 // buffer (P M^T = L U, so x = P^T L^-T U^-T v).
 //   FWD (U^T z = v, U^T lower non-unit): the block rhs is in V (already
@@ -921,19 +978,26 @@ extern "C" int mnpbem_bem_residual_batched(int n, int P, const void* G1t, const 
   if (batch <= 0) return 0;
   const double* x = (const double*)X;
   double* y = (double*)Y;
-  const int P4 = 4 * P;
-  int rc;
-  if ((rc = gemv_t<double>(n, P4, (const double*)G1t, ldm, smat, x, ldv, sv, y, ldy, sy, 1, 0, 0, batch, st)))
-    return rc;
-  if ((rc = gemv_t<double>(n, P4, (const double*)H1t, ldm, smat, x, ldv, sv, y + 2 * P4, ldy, sy, 1, 0, 0, batch,
-                           st)))
-    return rc;
-  if ((rc = gemv_t<double>(n, P4, (const double*)G2t, ldm, smat, x + 2 * P4, ldv, sv, y + 2 * 2 * P4, ldy, sy, 1,
-                           0, 0, batch, st)))
-    return rc;
-  if ((rc = gemv_t<double>(n, P4, (const double*)H2t, ldm, smat, x + 2 * P4, ldv, sv, y + 2 * 3 * P4, ldy, sy, 1,
-                           0, 0, batch, st)))
-    return rc;
+  const double *g1 = (const double*)G1t, *h1 = (const double*)H1t, *g2 = (const double*)G2t,
+               *h2 = (const double*)H2t;
+  dim3 grid(mnp_ceil_div(n, 256), batch, 4);
+  switch (P) {
+    case 1:
+      hipLaunchKernelGGL(k_gemv4_t<4>, grid, dim3(256), 0, st, n, g1, h1, g2, h2, ldm, smat, x, ldv, sv, y, ldy, sy);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_gemv4_t<8>, grid, dim3(256), 0, st, n, g1, h1, g2, h2, ldm, smat, x, ldv, sv, y, ldy, sy);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_gemv4_t<12>, grid, dim3(256), 0, st, n, g1, h1, g2, h2, ldm, smat, x, ldv, sv, y, ldy,
+                         sy);
+      break;
+    default:
+      hipLaunchKernelGGL(k_gemv4_t<16>, grid, dim3(256), 0, st, n, g1, h1, g2, h2, ldm, smat, x, ldv, sv, y, ldy,
+                         sy);
+      break;
+  }
+  MNP_LAUNCH_CHECK();
   return mnpbem_bem_combine_batched(n, P, Y, ldy, sy, R, Rout, ldv, sv, nvec, par, nrm, batch, stream);
 }
 
