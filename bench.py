@@ -174,8 +174,9 @@ def main():
     import ctypes
     ms, fl, nl = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
     nat.call("mnpbem_prof_end", ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(nl))
-    by = ctypes.c_double()
+    by, cby = ctypes.c_double(), ctypes.c_double()
     nat.call("mnpbem_prof_bytes", ctypes.byref(by))
+    nat.call("mnpbem_prof_cbytes", ctypes.byref(cby))
     elapsed = t1 - t0
     value = len(lams) * args.steps / elapsed
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -234,15 +235,25 @@ def main():
         log("cpu oracle: %.3f wavelengths/s on %d threads; mixed vs oracle max rel ext err %.2e"
             % (cpu_rate, cores, acc_vs_oracle))
 
-    traffic = None
+    # HBM bytes per launch of the dominant kernel: FETCH_SIZE from the counter
+    # pass summarised in the traffic file; writes from its WRITE_SIZE line, or,
+    # where the file has none, the compulsory writes of C of this run (the
+    # kernel stores each output element once)
+    traffic, traffic_write = None, None
     if args.traffic and args.precision == "mixed" and os.path.exists(args.traffic):
         tr = {}
         with open(args.traffic) as fh:
             for ln in fh:
                 if ":" in ln and ln.split(":")[0].strip() in ("fetch_bytes_per_launch", "write_bytes_per_launch"):
                     tr[ln.split(":")[0].strip()] = int(ln.split(":")[1])
-        if len(tr) == 2:
-            traffic = tr["fetch_bytes_per_launch"] + tr["write_bytes_per_launch"]
+        if "fetch_bytes_per_launch" in tr:
+            if "write_bytes_per_launch" in tr:
+                traffic_write = "WRITE_SIZE"
+                wr = tr["write_bytes_per_launch"]
+            else:
+                traffic_write = "compulsory C writes of this run (no WRITE_SIZE pass in the traffic file)"
+                wr = round(cby.value / max(nl.value, 1))
+            traffic = tr["fetch_bytes_per_launch"] + wr
     if args.precision == "mixed":
         kname = ("k_cgemm_sub_bf16x3 (complex64 GEMM on bf16 MFMA, 3 split products per real product; "
                  "achieved = 8MNK algorithmic flops / time, peak = bf16 dense peak / 3)")
@@ -275,6 +286,7 @@ def main():
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": round(gemm_achieved / 1e12, 3),
                      "peak": round(peak / 1e12, 1), "unit": "TFLOP/s", "frac": round(gemm_achieved / peak, 4),
                      "traffic": traffic,
+                     "traffic_write": traffic_write,
                      "compulsory_bytes_per_launch": round(by.value / max(nl.value, 1)),
                      "launches": nl.value,
                      "avg_launch_ms": round(ms.value / max(nl.value, 1), 4)},

@@ -426,6 +426,8 @@ int mnpbem_prof_begin(int capacity);
 int mnpbem_prof_end(double* total_ms, double* total_flops, long* launches);
 /* compulsory bytes of the timed launches: A, B read once, C read and written once */
 int mnpbem_prof_bytes(double* total_bytes);
+/* the part of those bytes that writes C (each output element once) */
+int mnpbem_prof_cbytes(double* c_write_bytes);
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,7 @@ SIGNATURES = {
     "mnpbem_prof_begin": [_i],
     "mnpbem_prof_end": [_p, _p, _p],
     "mnpbem_prof_bytes": [_p],
+    "mnpbem_prof_cbytes": [_p],
 }
 
 _lib = None

@@ -802,7 +802,8 @@ struct GemmProf {
   bool on = false;
   std::vector<hipEvent_t> ev;
   std::vector<double> flops;
-  double bytes = 0.0;  // compulsory bytes: A and B read once, C read and written once
+  double bytes = 0.0;   // compulsory bytes: A and B read once, C read and written once
+  double cbytes = 0.0;  // of which the writes of C
   size_t used = 0;
 };
 GemmProf g_prof;
@@ -898,6 +899,7 @@ int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, lon
     g_prof.used += 2;
     g_prof.flops.push_back(8.0 * M * (double)N * K * batch);
     g_prof.bytes += (double)batch * 2.0 * sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N);
+    g_prof.cbytes += (double)batch * 2.0 * sizeof(T) * ((double)M * N);
   }
   return 0;
 }
@@ -1503,6 +1505,7 @@ extern "C" int mnpbem_prof_begin(int capacity) {
   g_prof.used = 0;
   g_prof.flops.clear();
   g_prof.bytes = 0.0;
+  g_prof.cbytes = 0.0;
   g_prof.on = true;
   return 0;
 }
@@ -1529,5 +1532,11 @@ extern "C" int mnpbem_prof_end(double* total_ms, double* total_flops, long* laun
 // operand read once, C read and written once).
 extern "C" int mnpbem_prof_bytes(double* total_bytes) {
   *total_bytes = g_prof.bytes;
+  return 0;
+}
+
+// The part of those bytes that writes C (each output element once).
+extern "C" int mnpbem_prof_cbytes(double* c_write_bytes) {
+  *c_write_bytes = g_prof.cbytes;
   return 0;
 }
