@@ -100,7 +100,7 @@ int mnpbem_lu_solve_t_batched(int dtype, int n, int C, const void* LU, long ld, 
 int mnpbem_getrf_tall_rows(int rows);
 
 /* Columns (right-hand sides x batch) up to which the leaves of the triangular
- * solves (the diagonal blocks of 33..64 rows complex64, 17..32 complex128
+ * solves (the diagonal blocks of 17..64 rows complex64, 17..32 complex128
  * inside getrf, getrs and getri) run a row-blocked kernel (four waves over
  * four row blocks of 64 columns) instead of one thread per column; cols < 0
  * restores the built-in 32768, 0 disables it.  Returns the previous setting.

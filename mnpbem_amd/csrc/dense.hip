@@ -930,6 +930,20 @@ int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, 
       MNP_LAUNCH_CHECK();
       return 0;
     }
+    if constexpr (TL > 32) {
+      // 17 .. 32 rows (the 32-column level of the LU recursion)
+      if (n > 16 && n <= 32 && (long)nrhs * batch <= g_trsm_rows_max) {
+        dim3 grid(mnp_ceil_div(nrhs, 64), batch);
+        if (lower)
+          hipLaunchKernelGGL((k_trsm_leaf_rows<T, true, 32>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx,
+                             sX);
+        else
+          hipLaunchKernelGGL((k_trsm_leaf_rows<T, false, 32>), grid, dim3(256), 0, st, n, nrhs, Tm, ldt, sT, X, ldx,
+                             sX);
+        MNP_LAUNCH_CHECK();
+        return 0;
+      }
+    }
     dim3 grid(mnp_ceil_div(nrhs, 256), batch);
     // smallest unrolled size that holds the triangle
 #define MNP_TRSM_LEAF(SZ)                                                                                    \
@@ -1389,7 +1403,7 @@ extern "C" int mnpbem_cgemm_sub_variant(int variant, int M, int N, int K, const 
 // thresholds (rows = 0: 12288 for complex64, 6144 for complex128).  Returns
 // the previous setting.
 // Columns (right-hand sides x batch) up to which the triangular-solve leaves
-// (64 rows complex64, 32 complex128) use the row-blocked kernel; cols < 0
+// (17 to 64 rows complex64, 17 to 32 complex128) use the row-blocked kernel; cols < 0
 // restores the built-in 32768, 0 disables it.  Returns the previous setting.
 extern "C" int mnpbem_trsm_rows_max(int cols) {
   const int prev = (int)g_trsm_rows_max;

@@ -184,7 +184,8 @@ def test_getrf_getrs(cuda, dtype, tol, split, n):
 
 @pytest.mark.parametrize("dtype,tol,split", [(np.complex128, 1e-12, False), (np.complex64, 1e-4, False),
                                              (np.complex64, 1e-3, True)])
-@pytest.mark.parametrize("n,nrhs", [(64, 5), (61, 64), (200, 130), (200, 1100), (45, 300), (300, 1)])
+@pytest.mark.parametrize("n,nrhs", [(64, 5), (61, 64), (200, 130), (200, 1100), (45, 300), (300, 1), (30, 40),
+                                    (17, 3)])
 def test_trsm_row_blocked_leaves(cuda, dtype, tol, split, n, nrhs):
     """Triangular leaves with few right-hand sides run the row-blocked kernel
     (inside getrf's recursion, and in getrs up to 32768 columns over the
