@@ -74,7 +74,7 @@ int mnpbem_getrs_batched(int dtype, int n, int nrhs, const void* A, long lda, lo
 
 /* A <- A^-1 from its mnpbem_getrf_batched factors (LAPACK getri): A^-1 =
  * U^-1 L^-1 P with L^-1 formed by a recursion that skips the zero triangle of
- * the identity; W (n x n per matrix) and Q (n ints per matrix, n <= 16384) are
+ * the identity; W (n x n per matrix) and Q (n ints per matrix, n <= 32768) are
  * work space.  Replaces the explicit inverse that bem_ret.py:437-523 forms by
  * a solve against the identity. */
 int mnpbem_getri_batched(int dtype, int n, void* A, long lda, long sA, const int* ipiv, long spiv, void* W, long ldw,

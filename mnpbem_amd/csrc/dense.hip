@@ -1311,10 +1311,12 @@ __global__ void k_eye(int n, T* W, long ld, long sW) {
 }
 
 // perm[b][j]: the row interchanges ipiv[0..n) applied in order to 0..n-1
-// (P I = I[perm]); one workgroup per matrix, the swaps run in LDS.
+// (P I = I[perm]); one workgroup per matrix, the swaps run in LDS (128 KB of
+// the 160 KB per CU: n up to GETRI_MAXN, the row limit of the complex64 getrf).
+constexpr int GETRI_MAXN = 32768;
 __global__ __launch_bounds__(256) void k_perm_array(int n, const int* __restrict__ ipiv, long sP, int* perm,
                                                     long sQ) {
-  extern __shared__ int arr[];
+  __shared__ int arr[GETRI_MAXN];
   const long b = blockIdx.x;
   ipiv += b * sP;
   perm += b * sQ;
@@ -1357,7 +1359,7 @@ int getri(int n, T* A, long lda, long sA, const int* ipiv, long sP, T* W, long l
   int rc;
   if ((rc = trsm_lower_ident<T>(n, A, lda, sA, W, ldw, sW, batch, st, gmode))) return rc;
   if ((rc = trsm_rec<T>(false, n, n, A, lda, sA, W, ldw, sW, batch, st, gmode))) return rc;
-  hipLaunchKernelGGL(k_perm_array, dim3(batch), dim3(256), (size_t)n * sizeof(int), st, n, ipiv, sP, Q, sQ);
+  hipLaunchKernelGGL(k_perm_array, dim3(batch), dim3(256), 0, st, n, ipiv, sP, Q, sQ);
   MNP_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scatter_cols<T>, dim3(mnp_ceil_div(n, 256), n, batch), dim3(256), 0, st, n, W, ldw, sW, Q,
                      sQ, A, lda, sA);
@@ -1495,7 +1497,7 @@ extern "C" int mnpbem_getri_batched(int dtype, int n, void* A, long lda, long sA
                                     void* W, long ldw, long sW, int* Q, long sQ, int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n >= 0 && batch >= 0 && lda >= n && ldw >= n && sQ >= n, "getri: bad size");
-  MNP_REQUIRE((size_t)n * sizeof(int) <= 64 * 1024, "getri: n above 16384");
+  MNP_REQUIRE(n <= GETRI_MAXN, "getri: n above 32768");
   if (n == 0 || batch == 0) return 0;
   if (dtype == MNP_DTYPE_C128)
     return getri<double>(n, (double*)A, lda, sA, ipiv, sP, (double*)W, ldw, sW, Q, sQ, batch, st, 0);

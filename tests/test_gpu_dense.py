@@ -368,6 +368,34 @@ def test_getrf_tall_panels_match_lapack(cuda, dtype, n):
     assert int(info.cpu()[0]) == 301
 
 
+def test_getri_above_16384_rows(cuda):
+    """getri on one complex64 matrix of 17000 rows (the permutation of the
+    pivots no longer fits 64 KB of LDS; the 25 000-face dimer needs such
+    sizes): A^-1 (A x) against x for random vectors, with the products taken
+    by torch on the GPU."""
+    nat = _lib()
+    n = 17000
+    g = torch.Generator(device=cuda).manual_seed(5)
+    A = torch.randn((n, n), dtype=torch.complex64, device=cuda, generator=g)
+    A.diagonal().add_(2.0 * np.sqrt(n))
+    tA = A.clone()
+    piv = torch.empty((1, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((1,), dtype=torch.int32, device=cuda)
+    W = torch.empty_like(tA)
+    Q = torch.empty((1, n), dtype=torch.int32, device=cuda)
+    st = nat.stream_handle(cuda)
+    print("getri", n, flush=True)
+    nat.call("mnpbem_getrf_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), 1, st)
+    nat.call("mnpbem_getri_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(W), n,
+             n * n, nat.ptr(Q), n, 1, st)
+    del W
+    assert int(info.cpu()[0]) == 0
+    x = torch.randn((n, 3), dtype=torch.complex64, device=cuda, generator=g)
+    y = tA @ (A @ x)
+    err = ((y - x).abs().max() / x.abs().max()).item()
+    assert err < 1e-3, err
+
+
 @pytest.mark.parametrize("dtype,n", [(np.complex128, 300), (np.complex128, 517), (np.complex64, 300)])
 def test_getri_matches_inverse(cuda, dtype, n):
     """mnpbem_getri_batched (triangular inverse of L, dense upper solve, column
