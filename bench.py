@@ -124,8 +124,9 @@ def main():
     if args.gpus == 1 and args.workload == "adjoint":
         return bench_adjoint(args)
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; the sweep shards by wavelength across GPUs "
-              "(see DESIGN.md) but no multi-GPU figure is produced", file=sys.stderr)
+        print("bench.py: this build runs on one GPU only; the sweep would shard by wavelength across GPUs "
+              "(one process per GPU, one gather of the spectra) but no multi-GPU figure is produced",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
