@@ -63,7 +63,11 @@ int mnpbem_gemm_tuning_seen(int* shapes, int ncells, int clear);
 
 /* In-place LU with partial pivoting, P A = L U (largest |re|+|im| pivot,
  * LAPACK convention); info[b] = 1-based column of the first exactly-zero
- * pivot or 0.  Replaces lu_factor_dispatch (mnpbem/utils/gpu.py:145-191). */
+ * pivot or 0.  Replaces lu_factor_dispatch (mnpbem/utils/gpu.py:145-191).
+ * Matrices above the tall-panel threshold (mnpbem_getrf_tall_rows) keep their
+ * pivot candidates in one device-wide buffer of this library: two such
+ * factorisations must not run concurrently on one device (different streams
+ * or overlapping graph replays); smaller matrices have no such restriction. */
 int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA, int* ipiv, long spiv, int* info, int batch,
                          void* stream);
 

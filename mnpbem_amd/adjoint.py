@@ -115,6 +115,8 @@ class DrudeExtinction(torch.autograd.Function):
         _, drude = _inside_drude(bem.p)
         vals = params.detach().cpu().numpy().astype(float)
         drude.eps0, drude.wp, drude.gammad = float(vals[0]), float(vals[1]), float(vals[2])
+        # factors held by the solver belong to the old material
+        bem._fac, bem.enei, bem._key = None, None, None
         ext, grad = extinction_with_drude_gradient(bem, enei_list, planewave)
         ctx.save_for_backward(torch.as_tensor(grad, dtype=params.dtype, device=params.device))
         return torch.as_tensor(ext, dtype=params.dtype, device=params.device)

@@ -95,6 +95,7 @@ class BEMRet:
         self.device = self.g.device
         self.enei = None
         self._fac = None
+        self._key = None
         self.last_refine_iters = None
         self.last_residual = None
         io = p.inout
@@ -227,28 +228,32 @@ class BEMRet:
         nrm = torch.empty((B, 2), dtype=torch.float64, device=dev)
         cnt = 2 * R.numel()
         src = R
-        it = 0
-        rel = None
+        if fixed_iters is not None:
+            for _ in range(int(fixed_iters)):
+                nat.call("mnpbem_convert", nat.DTYPE_C64, cnt, nat.ptr(src), nat.ptr(R32), st)
+                self._solve_direct(F, R32, X32, W32, P)
+                nat.call("mnpbem_accumulate", nat.DTYPE_C64, cnt, nat.ptr(X32), nat.ptr(X), st)
+                self._residual(F, R, X, Rres, Y, nrm, P)
+                src = Rres
+            self.last_norms = nrm
+            return X
+        mon = RefinementMonitor(self.refine_tol, self.refine_maxit)
+        Xbest = None
         while True:
             nat.call("mnpbem_convert", nat.DTYPE_C64, cnt, nat.ptr(src), nat.ptr(R32), st)
             self._solve_direct(F, R32, X32, W32, P)
             nat.call("mnpbem_accumulate", nat.DTYPE_C64, cnt, nat.ptr(X32), nat.ptr(X), st)
-            it += 1
             self._residual(F, R, X, Rres, Y, nrm, P)
-            if fixed_iters is not None:
-                if it >= fixed_iters:
-                    self.last_norms = nrm
-                    return X
-                src = Rres
-                continue
             h = nrm.cpu().numpy()
-            rel = np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300))
-            if np.all(rel <= self.refine_tol) or it > self.refine_maxit:
+            improved, done = mon.step(np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300)))
+            Xbest = _keep_best(Xbest, X, improved)
+            if done:
                 break
             src = Rres
-        self.last_refine_iters = it
-        self.last_residual = rel
-        return X
+        self.last_refine_iters = mon.it
+        self.last_residual = mon.best
+        mon.check("BEMRet.solve")
+        return Xbest
 
     def green64(self, F):
         """complex128 transposed G1, H1, G2, H2 of the batch, each (2, B, n, n)
@@ -297,24 +302,26 @@ class BEMRet:
         nrm = torch.empty((B, 2), dtype=torch.float64, device=dev)
         cnt = 2 * C.numel()
         src = C
-        it = 0
+        mon = RefinementMonitor(self.refine_tol, self.refine_maxit)
+        Ybest = None
         while True:
             nat.call("mnpbem_convert", nat.DTYPE_C64, cnt, nat.ptr(src), nat.ptr(C32), st)
             direct(C32, Y32, W32)
             nat.call("mnpbem_accumulate", nat.DTYPE_C64, cnt, nat.ptr(Y32), nat.ptr(Y), st)
-            it += 1
             nat.call("mnpbem_bem_residual_adjoint_batched", n, P, nat.ptr(G64[0]), nat.ptr(H64[0]),
                      nat.ptr(G64[1]), nat.ptr(H64[1]), n, nn, nat.ptr(C), nat.ptr(Y), nat.ptr(Cres), 8 * P,
                      n * 8 * P, nat.ptr(Z), nat.ptr(T), 16 * P, n * 16 * P, nat.ptr(self.g.nvec), nat.ptr(F.par),
                      nat.ptr(nrm), B, st)
             h = nrm.cpu().numpy()
-            rel = np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300))
-            if np.all(rel <= self.refine_tol) or it > self.refine_maxit:
+            improved, done = mon.step(np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300)))
+            Ybest = _keep_best(Ybest, Y, improved)
+            if done:
                 break
             src = Cres
-        self.last_adjoint_iters = it
-        self.last_adjoint_residual = rel
-        return Y
+        self.last_adjoint_iters = mon.it
+        self.last_adjoint_residual = mon.best
+        mon.check("BEMRet adjoint solve")
+        return Ybest
 
     def apply_transpose(self, F, Y):
         """A^T Y in complex128 (B, n, 8P), from the complex128 Green matrices."""
@@ -363,11 +370,19 @@ class BEMRet:
                  8 * P, n * 8 * P, nat.ptr(g.nvec), nat.ptr(F.par), nat.ptr(nrm), B, st)
 
     # ------------------------------------------------------------------
+    def _state_key(self, enei):
+        """What the factors of one wavelength depend on besides the mesh: the
+        wavelength and every material's permittivity there (so that a
+        material changed in place, e.g. by a gradient step, is refactored)."""
+        return (float(enei),) + tuple(complex(e(enei)[0]) for e in self.p.eps)
+
     def init(self, enei):
-        if self.enei is not None and np.isclose(self.enei, enei):
+        key = self._state_key(enei)
+        if self._fac is not None and self._key == key:
             return self
         self._fac = self.factor([enei])
         self.enei = enei
+        self._key = key
         return self
 
     def __call__(self, enei):
@@ -490,6 +505,55 @@ class SweepGraph:
             raise RuntimeError("SweepGraph: residual %.2e above refine_tol after %d steps; use more iters"
                                % (rel.max(), self.iters))
         return self.ext.cpu().numpy()
+
+
+class RefinementMonitor:
+    """Stopping rule of the FP64 iterative refinement of a batch of systems.
+
+    ``step(rel)`` records the relative residuals ||R - A X|| / ||R|| after one
+    more correction and returns (improved, done): which systems reached their
+    smallest residual so far (their iterate is the one to keep) and whether to
+    stop.  Refinement stops when every system is at or below ``tol``, after
+    ``maxit`` + 1 solves, or when no system still above ``tol`` gained at
+    least a factor ``stall`` in the last step (it has stagnated or diverges,
+    as it can for nearly singular systems).  ``check`` raises when a system
+    missed ``tol``."""
+
+    def __init__(self, tol, maxit, stall=0.5):
+        self.tol, self.maxit, self.stall = float(tol), int(maxit), float(stall)
+        self.best = None
+        self.it = 0
+
+    def step(self, rel):
+        rel = np.asarray(rel, dtype=float)
+        self.it += 1
+        if self.best is None:
+            self.best = rel.copy()
+            return np.ones(rel.shape, dtype=bool), bool(np.all(rel <= self.tol)) or self.it > self.maxit
+        improved = rel < self.best
+        still_open = self.best > self.tol
+        gaining = np.any(rel[still_open] < self.stall * self.best[still_open])
+        self.best = np.where(improved, rel, self.best)
+        done = bool(np.all(self.best <= self.tol)) or self.it > self.maxit or not gaining
+        return improved, done
+
+    def check(self, what):
+        if self.best is not None and np.any(self.best > self.tol):
+            raise RuntimeError("%s: mixed-precision refinement stopped at relative residual %.2e > refine_tol "
+                               "%.1e after %d solves (the system is too ill-conditioned for the complex64 "
+                               "factors; use precision='double')" % (what, float(np.max(self.best)), self.tol,
+                                                                     self.it))
+
+
+def _keep_best(best, X, improved):
+    """The iterate with the smallest residual per system (batch index 0)."""
+    if best is None or np.all(improved):
+        return X.clone() if best is None else best.copy_(X)
+    idx = np.nonzero(improved)[0]
+    if len(idx):
+        t = torch.as_tensor(idx, device=X.device)
+        best[t] = X[t]
+    return best
 
 
 def _unpack(p, enei, X, P):

@@ -372,10 +372,14 @@ class BEMRetIter(BEMIter):
 
     # ------------------------------------------------------------------
     def init(self, enei):
-        if self.enei is not None and np.isclose(self.enei, enei):
+        # keyed on the wavelength and every material's permittivity there, so
+        # that a material changed in place is prepared again
+        key = (float(enei),) + tuple(complex(e(enei)[0]) for e in self.p.eps)
+        if getattr(self, "_state", None) is not None and getattr(self, "_key", None) == key:
             return self
         self._state = self._prepare([enei])
         self.enei = enei
+        self._key = key
         return self
 
     def __call__(self, enei):
@@ -438,6 +442,7 @@ class BEMRetIter(BEMIter):
     def clear(self):
         self._state = None
         self.enei = None
+        self._key = None
         return self
 
     def hinfo(self):

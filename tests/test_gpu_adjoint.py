@@ -61,3 +61,30 @@ def test_adjoint_solve(cuda, precision):
             Xs = bem.solve_batch(F, R)
             a, b = _dot(Y, R), _dot(C, Xs)
             assert torch.max(torch.abs(a - b) / torch.abs(b)).item() < tol, (name, P)
+
+
+def test_material_change_refactors(cuda):
+    """A solver whose factors are cached for a wavelength factors again when
+    a material is changed in place (as DrudeExtinction does with the new
+    parameters), so bem.solve at that wavelength sees the new material."""
+    import mnpbem_amd as m
+    from mnpbem_amd.adjoint import DrudeExtinction
+    drude = m.EpsDrude(*DRUDE)
+    cp = m.ComParticle([m.EpsConst(1.0), drude], [m.trisphere(60, 20.0)], [[2, 1]], 1)
+    pw = m.PlaneWaveRet([[1, 0, 0]], [[0, 0, 1]])
+    bem = m.BEMRet(cp)
+    lam = 540.0
+    print("material change, cached factors", flush=True)
+    e_old = pw.extinction(bem.solve(pw.potential(cp, lam))[0])
+    params = torch.tensor([9.0, 8.9, 0.09], dtype=torch.float64, requires_grad=True)
+    DrudeExtinction.apply(params, bem, [lam], pw)
+    e_new = pw.extinction(bem.solve(pw.potential(cp, lam))[0])
+    fresh = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(9.0, 8.9, 0.09)], [m.trisphere(60, 20.0)], [[2, 1]], 1)
+    e_fresh = pw.extinction(m.BEMRet(fresh).solve(pw.potential(fresh, lam))[0])
+    assert abs(e_new / e_fresh - 1) < 1e-12
+    assert abs(e_new / e_old - 1) > 1e-3
+    # an in-place change without the wrapper is caught as well
+    drude.gammad = 0.2
+    e_g = pw.extinction(bem.solve(pw.potential(cp, lam))[0])
+    fresh2 = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(9.0, 8.9, 0.2)], [m.trisphere(60, 20.0)], [[2, 1]], 1)
+    assert abs(e_g / pw.extinction(m.BEMRet(fresh2).solve(pw.potential(fresh2, lam))[0]) - 1) < 1e-12

@@ -328,8 +328,16 @@ def test_getrf_tall_panels_match_lapack(cuda, dtype, n):
     is reported in info."""
     import scipy.linalg as sla
     nat = _lib()
-    # exercise the column kernels above 3072 rows (built in: 12288 / 6144)
-    nat.load().mnpbem_getrf_tall_rows(3072)
+    # exercise the column kernels above 3072 rows (built in: 12288 / 6144);
+    # the knob is process-wide, so it is restored whatever the outcome
+    prev = nat.load().mnpbem_getrf_tall_rows(3072)
+    try:
+        _tall_panels_case(nat, sla, cuda, dtype, n)
+    finally:
+        nat.load().mnpbem_getrf_tall_rows(prev)
+
+
+def _tall_panels_case(nat, sla, cuda, dtype, n):
     rng = np.random.default_rng(5)
     A = _rand(rng, (1, n, n), dtype)
     tA = torch.from_numpy(A.copy()).to(cuda)
@@ -364,7 +372,6 @@ def test_getrf_tall_panels_match_lapack(cuda, dtype, n):
     tZ = torch.from_numpy(Z).to(cuda)
     nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(tZ), n, n * n, nat.ptr(piv), n, nat.ptr(info), 1,
              nat.stream_handle(cuda))
-    nat.load().mnpbem_getrf_tall_rows(0)
     assert int(info.cpu()[0]) == 301
 
 
@@ -372,12 +379,15 @@ def test_getri_above_16384_rows(cuda):
     """getri on one complex64 matrix of 17000 rows (the permutation of the
     pivots no longer fits 64 KB of LDS; the 25 000-face dimer needs such
     sizes): A^-1 (A x) against x for random vectors, with the products taken
-    by torch on the GPU."""
+    by torch on the GPU.  The rows of a diagonally dominant matrix are
+    permuted at random, so the pivots undo that permutation and the column
+    scatter moves entries across the whole 0..16999 range."""
     nat = _lib()
     n = 17000
     g = torch.Generator(device=cuda).manual_seed(5)
     A = torch.randn((n, n), dtype=torch.complex64, device=cuda, generator=g)
-    A.diagonal().add_(2.0 * np.sqrt(n))
+    A.diagonal().add_(4.0 * np.sqrt(n))
+    A = A[torch.randperm(n, device=cuda, generator=g)].contiguous()
     tA = A.clone()
     piv = torch.empty((1, n), dtype=torch.int32, device=cuda)
     info = torch.empty((1,), dtype=torch.int32, device=cuda)
@@ -390,6 +400,8 @@ def test_getri_above_16384_rows(cuda):
              n * n, nat.ptr(Q), n, 1, st)
     del W
     assert int(info.cpu()[0]) == 0
+    pv = piv[0].cpu().numpy()
+    assert (pv != np.arange(n)).sum() > n // 2 and pv.max() > 16384
     x = torch.randn((n, 3), dtype=torch.complex64, device=cuda, generator=g)
     y = tA @ (A @ x)
     err = ((y - x).abs().max() / x.abs().max()).item()
