@@ -186,6 +186,21 @@ int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t, const int
                              const void* R, void* W, void* X, long ldv, long sv, const double* nvec,
                              const double* par, int batch, void* stream);
 
+/* mnpbem_bem_solve_batched without the work space argument W: no counterpart
+ * in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it allocates nothing.  W lives in a static 64 MiB device array
+ * of this library (one per device, held until the process ends), so the
+ * batch must satisfy ((batch-1) sv + (n-1) ldv + 8P) x element size <= 64 MiB
+ * (complex64 c. 23.6 MB for 64 wavelengths of N = 2884, P = 2), else status 1.
+ * Calls that overlap on one device (several streams, overlapping graph
+ * replays) share that array and race. */
+int mnpbem_bem_solve_batched_noalloc(int dtype, int n, int P, const void* G1t, const int* piv1, const void* G2t,
+                                     const int* piv2, const void* S1t, const void* Et, const void* St,
+                                     const int* pivS, long ldm, long smat, long spiv, const void* D1, const void* D2,
+                                     const void* DS, long sD, const void* R, void* X, long ldv, long sv,
+                                     const double* nvec, const double* par, int batch, void* stream);
+
 /* Rout = R - A X for the complex128 8N operator (bem_ret_iter.py:1362-1501)
  * with unfactored transposed G1, H1, G2, H2; Y is n x 16P work space;
  * nrm[b] = {||R - A X||^2, ||R||^2}. */

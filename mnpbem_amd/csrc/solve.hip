@@ -21,6 +21,8 @@
 #include "common.h"
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace {
 
@@ -959,6 +961,74 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
 #undef MNP_SOLVE
   mnp_set_error("bem_solve: unknown dtype");
   return 1;
+}
+
+// ---------------------------------------------------------------------------
+// The same solve with its work space W in a static device array instead of a
+// caller buffer, for callers that cannot allocate device memory (code under
+// HIP graph capture that must not reach an allocator, serving loops,
+// harnesses that forbid hipMalloc).  Each device holds its own copy of
+// `scratch` (64 MiB) from the library's load until the process ends; its
+// address is looked up once per device, with that device current, on the
+// first call there, and only that address is handed to kernels.  Every call
+// on one device shares the array: calls that overlap on one device (two
+// streams, overlapping graph replays) race.  Nothing in it has to start at
+// zero: every element the solve reads from W it has written first in the same
+// call.
+// ---------------------------------------------------------------------------
+constexpr size_t kSolveScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kSolveScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::once_flag g_scratch_once[kScratchDevices];
+void* g_scratch_ptr[kScratchDevices];
+hipError_t g_scratch_rc[kScratchDevices];
+
+// device address of `scratch` on the current device
+int scratch_address(void** out) {
+  int dev = 0;
+  MNP_HIP_CHECK(hipGetDevice(&dev));
+  MNP_REQUIRE(dev >= 0 && dev < kScratchDevices, "bem_solve_noalloc: device ordinal above 63");
+  std::call_once(g_scratch_once[dev], [dev]() {
+    void* p = nullptr;
+    g_scratch_rc[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = g_scratch_rc[dev] == hipSuccess ? p : nullptr;
+  });
+  if (g_scratch_rc[dev] != hipSuccess || g_scratch_ptr[dev] == nullptr) {
+    mnp_set_error(std::string("bem_solve_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                  hipGetErrorString(g_scratch_rc[dev]));
+    return 2;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" int mnpbem_bem_solve_batched_noalloc(int dtype, int n, int P, const void* G1t, const int* piv1,
+                                                const void* G2t, const int* piv2, const void* S1t, const void* Et,
+                                                const void* St, const int* pivS, long ldm, long smat, long spiv,
+                                                const void* D1, const void* D2, const void* DS, long sD,
+                                                const void* R, void* X, long ldv, long sv, const double* nvec,
+                                                const double* par, int batch, void* stream) {
+  MNP_REQUIRE(dtype == MNP_DTYPE_C128 || mnp_is_c64(dtype), "bem_solve_noalloc: unknown dtype");
+  MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldv >= 8 * P && sv >= 0, "bem_solve_noalloc: bad size (P <= 4)");
+  if (batch <= 0) return 0;
+  // W takes R's layout: element (b, i, c) at b sv + i ldv + c
+  const size_t esz = dtype == MNP_DTYPE_C128 ? 16 : 8;
+  const size_t need = ((size_t)(batch - 1) * (size_t)sv + (size_t)(n - 1) * (size_t)ldv + 8 * (size_t)P) * esz;
+  if (need > kSolveScratchBytes) {
+    mnp_set_error("bem_solve_noalloc: work space of " + std::to_string(need) + " bytes (batch " +
+                  std::to_string(batch) + ", n " + std::to_string(n) + ", P " + std::to_string(P) +
+                  ") exceeds the static " + std::to_string(kSolveScratchBytes) +
+                  "-byte scratch; split the batch or use mnpbem_bem_solve_batched");
+    return 1;
+  }
+  void* w = nullptr;
+  int rc = scratch_address(&w);
+  if (rc) return rc;
+  return mnpbem_bem_solve_batched(dtype, n, P, G1t, piv1, G2t, piv2, S1t, Et, St, pivS, ldm, smat, spiv, D1, D2,
+                                  DS, sD, R, w, X, ldv, sv, nvec, par, batch, stream);
 }
 
 extern "C" int mnpbem_bem_combine_batched(int n, int P, const void* Y, long ldy, long sy, const void* R, void* Out,
