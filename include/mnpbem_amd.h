@@ -22,7 +22,14 @@
  *     calling thread;
  *   - nothing here allocates device memory except mnpbem_prof_begin (HIP
  *     events); calls on one stream are ordered, calls that share output
- *     buffers on different streams race.
+ *     buffers on different streams race;
+ *   - geometry slots (gdiv): calls with a gdiv argument can batch several
+ *     particles of one face count n.  Their per-face inputs are then stacked
+ *     per particle (pos, nvec: slots x n x 3; area, mat, in_med, out_med:
+ *     slots x n) and the batch is ordered particle-major with gdiv wavelengths
+ *     per particle, so element b reads slot b / gdiv.  gdiv = 0: one geometry
+ *     for the whole batch.  (No counterpart in the reference, which loops over
+ *     particles and wavelengths in Python.)
  */
 #ifndef MNPBEM_AMD_H
 #define MNPBEM_AMD_H
@@ -158,12 +165,16 @@ int mnpbem_closed_correction(int n, const double* pos, const double* nvec, const
  * hsign = +1; outside: -1) for `batch` wavelengths with wavenumbers
  * kmat[b][m] (complex, doubles), written transposed: entry (i, j) at
  * out[b*stride + j*ld + i].  dual = 1 (dtype complex128) also writes a
- * complex64 copy to G32/H32.  Replaces CompGreenRet.eval(i, i, 'G'/'H1'/'H2')
- * (compgreen_ret.py:507-858, greenret_refined.py:654-810). */
+ * complex64 copy to G32/H32.  With geometry slots (gdiv > 0) the refined pairs
+ * and coefficients of the slots are concatenated, slot s owning pairs
+ * [poff[s], poff[s+1]) (int64, device) with face indices local to the slot,
+ * and npairs is the longest slot's count.  Replaces CompGreenRet.eval(i, i,
+ * 'G'/'H1'/'H2') (compgreen_ret.py:507-858, greenret_refined.py:654-810). */
 int mnpbem_green_assemble(int dtype, int dual, int n, const double* pos, const double* nvec, const double* area,
                           const int* mat, const double* kmat, int nmat, int order, int npairs, const int* prow,
                           const int* pcol, const double* g, const double* f, double hsign, void* G, void* H,
-                          long ld, long stride, void* G32, void* H32, int batch, void* stream);
+                          long ld, long stride, void* G32, void* H32, int gdiv, const long* poff, int batch,
+                          void* stream);
 
 /* ---- BEM solver (replaces mnpbem/bem/bem_ret.py:293-525 and 1360-1831) ----
  * Vectors per wavelength are n x 8P row-major (P polarisations):
@@ -174,7 +185,7 @@ int mnpbem_green_assemble(int dtype, int dual, int n, const double* pos, const d
 /* mode 0: D = S1 - S2 (Delta^T); mode 1: S2 <- eps1 S1 - eps2 S2
  * + k0^2 (eps1 - eps2)^2 E o (n n^T)  (Sigma^T, bem_ret.py:476-495). */
 int mnpbem_sigma_batched(int dtype, int mode, int n, const void* S1, void* S2, void* D, const void* E, long ld,
-                         long stride, const double* nvec, const double* par, int batch, void* stream);
+                         long stride, const double* nvec, const double* par, int gdiv, int batch, void* stream);
 
 /* X = A^-1 R with the factors of BEMRet.init: LU(G1^T), LU(G2^T), Sigma1^T,
  * (Delta^-1)^T, LU(Sigma^T) (Eqs. 19-20 of Garcia de Abajo & Howie,
@@ -184,7 +195,7 @@ int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t, const int
                              const int* piv2, const void* S1t, const void* Et, const void* St, const int* pivS,
                              long ldm, long smat, long spiv, const void* D1, const void* D2, const void* DS, long sD,
                              const void* R, void* W, void* X, long ldv, long sv, const double* nvec,
-                             const double* par, int batch, void* stream);
+                             const double* par, int gdiv, int batch, void* stream);
 
 /* mnpbem_bem_solve_batched without the work space argument W: no counterpart
  * in the reference.  For callers that cannot allocate device memory (code
@@ -199,15 +210,16 @@ int mnpbem_bem_solve_batched_noalloc(int dtype, int n, int P, const void* G1t, c
                                      const int* piv2, const void* S1t, const void* Et, const void* St,
                                      const int* pivS, long ldm, long smat, long spiv, const void* D1, const void* D2,
                                      const void* DS, long sD, const void* R, void* X, long ldv, long sv,
-                                     const double* nvec, const double* par, int batch, void* stream);
+                                     const double* nvec, const double* par, int gdiv, int batch, void* stream);
 
 /* Rout = R - A X for the complex128 8N operator (bem_ret_iter.py:1362-1501)
  * with unfactored transposed G1, H1, G2, H2; Y is n x 16P work space;
- * nrm[b] = {||R - A X||^2, ||R||^2}. */
+ * nrm[b] = {||R - A X||^2, ||R||^2} (NULL to skip; Rout must not alias R
+ * then), summed in a fixed order so the values repeat bit for bit. */
 int mnpbem_bem_residual_batched(int n, int P, const void* G1t, const void* H1t, const void* G2t, const void* H2t,
                                 long ldm, long smat, const void* R, const void* X, void* Rout, long ldv, long sv,
                                 void* Y, long ldy, long sy, const double* nvec, const double* par, double* nrm,
-                                int batch, void* stream);
+                                int gdiv, int batch, void* stream);
 
 /* ---- adjoint solve for the extinction gradients (no counterpart in the
  *      reference, which has no derivatives; it transposes the solve of
@@ -288,12 +300,12 @@ int mnpbem_convert(int dst_dtype, long count, const double* src, void* dst, void
  * [a1 | a1' | a2 | a2'] as n x 12P per wavelength. */
 int mnpbem_planewave_batched(int n, int P, const double* pos, const double* nvec, const int* in_med,
                              const int* out_med, const double* pol, const double* dir, const double* pw, double* R,
-                             long ldv, long sv, double* pot, long spot, int batch, void* stream);
+                             long ldv, long sv, double* pot, long spot, int gdiv, int batch, void* stream);
 
 /* ext[b][p] = 4 pi / k Im(pol_p . e_p(dir_p)) from the solution X (complex128). */
 int mnpbem_extinction_batched(int n, int P, const double* X, long ldv, long sv, const double* pos,
                               const double* area, const int* in_med, const int* out_med, const double* pol,
-                              const double* dir, const double* pw, double* ext, int batch, void* stream);
+                              const double* dir, const double* pw, double* ext, int gdiv, int batch, void* stream);
 
 /* Surface fields (replaces CompGreenRet.field, compgreen_ret.py:912-969):
  * e_i = i k0 sum_j G_ij h_j - sum_j Hp_ij sig_j, h_i = sum_j Hp_ij x h_j on
@@ -340,7 +352,8 @@ int mnpbem_deps_rhs(int n, int P, const void* Y, long ldy, long sy, void* R, lon
  * mnpbem_bem_residual_batched or NULL (the tail of BEMRetIter._afun,
  * bem_ret_iter.py:1440-1501, after its four Green matvecs). */
 int mnpbem_bem_combine_batched(int n, int P, const void* Y, long ldy, long sy, const void* R, void* Out, long ldv,
-                               long sv, const double* nvec, const double* par, double* nrm, int batch, void* stream);
+                               long sv, const double* nvec, const double* par, double* nrm, int gdiv, int batch,
+                               void* stream);
 
 /* ---- GMRES vector kernels (replace the Arnoldi / Givens vector algebra
  *      of scipy.sparse.linalg.gmres called at mnpbem/bem/bem_iter.py:79-89) ----

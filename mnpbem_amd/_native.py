@@ -36,18 +36,18 @@ SIGNATURES = {
     "mnpbem_refine_coeffs": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _p],
     "mnpbem_closed_correction": [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _i, _p, _p, _p],
     "mnpbem_green_assemble": [_i, _i, _i, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _d, _p, _p, _l, _l,
-                              _p, _p, _i, _p],
+                              _p, _p, _i, _p, _i, _p],
     "mnpbem_gemv_t_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _l, _p, _l, _l, _d, _d, _d, _i, _p],
     "mnpbem_lu_solve_t_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _p, _l, _l, _p, _l, _i, _p],
     "mnpbem_lu_blockinv_batched": [_i, _i, _p, _l, _l, _p, _l, _i, _p],
     "mnpbem_identity_batched": [_i, _i, _p, _l, _l, _i, _p],
-    "mnpbem_sigma_batched": [_i, _i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _i, _p],
+    "mnpbem_sigma_batched": [_i, _i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _i, _i, _p],
     "mnpbem_bem_solve_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l, _p,
-                                 _p, _p, _l, _l, _p, _p, _i, _p],
+                                 _p, _p, _l, _l, _p, _p, _i, _i, _p],
     "mnpbem_bem_solve_batched_noalloc": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l,
-                                         _p, _p, _l, _l, _p, _p, _i, _p],
+                                         _p, _p, _l, _l, _p, _p, _i, _i, _p],
     "mnpbem_bem_residual_batched": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _l, _l, _p, _l, _l, _p, _p,
-                                    _p, _i, _p],
+                                    _p, _i, _i, _p],
     "mnpbem_bem_solve_adjoint_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l,
                                          _p, _p, _p, _l, _l, _p, _p, _i, _p],
     "mnpbem_bem_residual_adjoint_batched": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _l, _l, _p, _p, _l, _l,
@@ -64,9 +64,9 @@ SIGNATURES = {
     "mnpbem_getri_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _p, _l, _i, _p],
     "mnpbem_accumulate": [_i, _l, _p, _p, _p],
     "mnpbem_convert": [_i, _l, _p, _p, _p],
-    "mnpbem_planewave_batched": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _p, _l, _i, _p],
-    "mnpbem_extinction_batched": [_i, _i, _p, _l, _l, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p],
-    "mnpbem_bem_combine_batched": [_i, _i, _p, _l, _l, _p, _p, _l, _l, _p, _p, _p, _i, _p],
+    "mnpbem_planewave_batched": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _p, _l, _i, _i, _p],
+    "mnpbem_extinction_batched": [_i, _i, _p, _l, _l, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p],
+    "mnpbem_bem_combine_batched": [_i, _i, _p, _l, _l, _p, _p, _l, _l, _p, _p, _p, _i, _i, _p],
     "mnpbem_widen": [_i, _l, _p, _p, _p],
     "mnpbem_mv_dot_batched": [_l, _i, _i, _p, _l, _l, _p, _l, _p, _p, _i, _p],
     "mnpbem_mv_update_batched": [_l, _i, _i, _p, _l, _l, _p, _d, _p, _l, _i, _p],

@@ -169,8 +169,9 @@ class BEMRet:
                  2 * B, st)
         S1, S2 = HH[0], HH[1]
         Dt = torch.empty((B, n, n), dtype=cdt, device=dev)
+        gd = getattr(g, "gdiv", 0)
         nat.call("mnpbem_sigma_batched", dt, 0, n, nat.ptr(S1), nat.ptr(S2), nat.ptr(Dt), None, n, nn,
-                 nat.ptr(g.nvec), nat.ptr(F.par), B, st)
+                 nat.ptr(g.nvec), nat.ptr(F.par), gd, B, st)
         nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(info[2]), B, st)
         # (Delta^-1)^T in place of the factors (LAPACK getri's operation count)
         Wk = torch.empty((B, n, n), dtype=cdt, device=dev)
@@ -181,7 +182,7 @@ class BEMRet:
         del Wk, Q
         # Sigma^T overwrites Sigma2^T
         nat.call("mnpbem_sigma_batched", dt, 1, n, nat.ptr(S1), nat.ptr(S2), None, nat.ptr(Et), n, nn,
-                 nat.ptr(g.nvec), nat.ptr(F.par), B, st)
+                 nat.ptr(g.nvec), nat.ptr(F.par), gd, B, st)
         nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(piv[3]), n, nat.ptr(info[3]), B, st)
         # inverse diagonal blocks of the factors used by the small-RHS solves
         nblk = (n + 63) // 64
@@ -200,7 +201,8 @@ class BEMRet:
                  nat.ptr(F.GG[1]), nat.ptr(F.piv[1]), nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]),
                  nat.ptr(F.piv[3]), n, nn, n, nat.ptr(F.DG[0]), nat.ptr(F.DG[1]), nat.ptr(F.DS), F.sD,
                  nat.ptr(R), nat.ptr(W), nat.ptr(X), 8 * P, n * 8 * P,
-                 nat.ptr(self.g.nvec), nat.ptr(F.par), F.B, nat.stream_handle(self.device))
+                 nat.ptr(self.g.nvec), nat.ptr(F.par), getattr(self.g, "gdiv", 0), F.B,
+                 nat.stream_handle(self.device))
 
     def solve_batch(self, F, R, fixed_iters=None):
         """Solve A X = R for the batch (R: (B, n, 8P) complex128 on device).
@@ -346,7 +348,8 @@ class BEMRet:
         Yw = torch.empty((B, n, 16 * P), dtype=torch.complex128, device=self.device)
         nat.call("mnpbem_bem_residual_batched", n, P, nat.ptr(G64[0]), nat.ptr(H64[0]), nat.ptr(G64[1]),
                  nat.ptr(H64[1]), n, n * n, None, nat.ptr(X), nat.ptr(out), 8 * P, n * 8 * P, nat.ptr(Yw), 16 * P,
-                 n * 16 * P, nat.ptr(self.g.nvec), nat.ptr(F.par), None, B, nat.stream_handle(self.device))
+                 n * 16 * P, nat.ptr(self.g.nvec), nat.ptr(F.par), None, getattr(self.g, "gdiv", 0), B,
+                 nat.stream_handle(self.device))
         return out
 
     def _residual(self, F, R, X, Rres, Y, nrm, P):
@@ -358,8 +361,10 @@ class BEMRet:
             nat.call("mnpbem_bem_residual_batched", n, P, nat.ptr(F.G64[0]), nat.ptr(F.H64[0]),
                      nat.ptr(F.G64[1]), nat.ptr(F.H64[1]), n, n * n, nat.ptr(R), nat.ptr(X), nat.ptr(Rres),
                      8 * P, n * 8 * P, nat.ptr(Y), 16 * P, n * 16 * P, nat.ptr(g.nvec), nat.ptr(F.par),
-                     nat.ptr(nrm), B, st)
+                     nat.ptr(nrm), getattr(g, "gdiv", 0), B, st)
             return
+        if getattr(g, "gdiv", 0):
+            raise NotImplementedError("BEMRet: the matrix-free residual needs one geometry per batch")
         P4 = 4 * P
         for s, hs in ((0, 1.0), (1, -1.0)):
             nat.call("mnpbem_green_apply", n, P4, hs, nat.ptr(g.pos), nat.ptr(g.nvec), nat.ptr(g.area),
@@ -367,7 +372,7 @@ class BEMRet:
                      nat.ptr(g.g), nat.ptr(g.f), nat.ptr(X), 8 * P, n * 8 * P, s * P4, nat.ptr(Y), 16 * P,
                      n * 16 * P, 2 * s * P4, 2 * s * P4 + P4, B, st)
         nat.call("mnpbem_bem_combine_batched", n, P, nat.ptr(Y), 16 * P, n * 16 * P, nat.ptr(R), nat.ptr(Rres),
-                 8 * P, n * 8 * P, nat.ptr(g.nvec), nat.ptr(F.par), nat.ptr(nrm), B, st)
+                 8 * P, n * 8 * P, nat.ptr(g.nvec), nat.ptr(F.par), nat.ptr(nrm), 0, B, st)
 
     # ------------------------------------------------------------------
     def _state_key(self, enei):

@@ -191,7 +191,7 @@ class BEMRetIter(BEMIter):
             G1, H1, G2, H2 = S.ops
             nat.call("mnpbem_bem_residual_batched", n, P, nat.ptr(G1), nat.ptr(H1), nat.ptr(G2), nat.ptr(H2), n,
                      n * n, None, nat.ptr(X), nat.ptr(Out), 8 * P, n * 8 * P, nat.ptr(Y), 16 * P, n * 16 * P,
-                     nat.ptr(self.g.nvec), nat.ptr(S.par), None, B, st)
+                     nat.ptr(self.g.nvec), nat.ptr(S.par), None, 0, B, st)
             return
         P4 = 4 * P
         for b in range(B):
@@ -201,7 +201,7 @@ class BEMRetIter(BEMIter):
                 hm.matvec_device(nat.ptr_off(X, xo + (0 if q < 2 else P4)), 8 * P, nat.ptr_off(Y, yo + q * P4),
                                  16 * P, P4, stream=st)
         nat.call("mnpbem_bem_combine_batched", n, P, nat.ptr(Y), 16 * P, n * 16 * P, None, nat.ptr(Out), 8 * P,
-                 n * 8 * P, nat.ptr(self.g.nvec), nat.ptr(S.par), None, B, st)
+                 n * 8 * P, nat.ptr(self.g.nvec), nat.ptr(S.par), None, 0, B, st)
 
     def _precond_blocks(self, S, V, Z, ws, P):
         """Z = blockdiag(A_kk)^-1 V: each particle's direct solve on its rows
@@ -221,7 +221,7 @@ class BEMRetIter(BEMIter):
                      nat.ptr(F.GG[1]), nat.ptr(F.piv[1]), nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]),
                      nat.ptr(F.piv[3]), F.n, nn, F.n, nat.ptr(F.DG[0]), nat.ptr(F.DG[1]), nat.ptr(F.DS), F.sD,
                      nat.ptr_off(R, o), nat.ptr_off(W, o), nat.ptr_off(X, o), 8 * P, n * 8 * P,
-                     nat.ptr(bem.g.nvec), nat.ptr(F.par), B, st)
+                     nat.ptr(bem.g.nvec), nat.ptr(F.par), 0, B, st)
         if not c128:
             nat.call("mnpbem_widen", nat.DTYPE_C64, 2 * V.numel(), nat.ptr(X), nat.ptr(Z), st)
 

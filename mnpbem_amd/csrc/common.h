@@ -43,6 +43,13 @@ void mnp_set_error(const std::string& msg);
 
 #define MNP_LAUNCH_CHECK() MNP_HIP_CHECK(hipGetLastError())
 
+// Geometry slot of batch element b.  Batched calls over several particles of
+// one face count stack the per-face geometry of `slots` particles (pos, nvec:
+// slots x n x 3; area, material and side flags: slots x n) and order the batch
+// particle-major, gdiv wavelengths per particle: element b reads slot b / gdiv.
+// gdiv <= 0: every element reads slot 0 (one particle for the whole batch).
+__host__ __device__ __forceinline__ long mnp_geo_slot(long b, int gdiv) { return gdiv > 0 ? b / gdiv : 0; }
+
 // ---------------------------------------------------------------------------
 // complex arithmetic (T = float or double)
 // ---------------------------------------------------------------------------

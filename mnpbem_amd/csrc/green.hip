@@ -239,11 +239,16 @@ template <typename TO, bool DUAL>
 __global__ __launch_bounds__(AS_I) void k_green_assemble(
     int n, const double* __restrict__ pos, const double* __restrict__ nvec, const double* __restrict__ area,
     const int* __restrict__ mat, const double* __restrict__ kmat, int nmat, TO* __restrict__ G, TO* __restrict__ H,
-    long ld, long stride, float* __restrict__ G2, float* __restrict__ H2) {
+    long ld, long stride, float* __restrict__ G2, float* __restrict__ H2, int gdiv) {
   const long b = blockIdx.z;
   const int i = blockIdx.x * AS_I + threadIdx.x;
   const int j0 = blockIdx.y * AS_J;
   if (i >= n) return;
+  const long gs = mnp_geo_slot(b, gdiv);
+  pos += 3 * n * gs;
+  nvec += 3 * n * gs;
+  area += n * gs;
+  mat += n * gs;
   const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   const double nxi = nvec[3 * i], nyi = nvec[3 * i + 1], nzi = nvec[3 * i + 2];
   const int mi = mat[i];
@@ -288,17 +293,29 @@ __global__ __launch_bounds__(AS_I) void k_green_assemble(
 }
 
 // refined entries: value = (sum_n c_n (ik)^n) e^{ikd}; H gets +/-2 pi on the
-// self diagonal after the phase (greenret_refined.py:698-810).
+// self diagonal after the phase (greenret_refined.py:698-810).  With several
+// geometry slots (gdiv > 0) the pair lists of the slots are concatenated:
+// slot s owns pairs [poff[s], poff[s+1]) with face indices local to the slot,
+// and npairs is the longest list (the grid's extent).
 template <typename TO, bool DUAL>
 __global__ __launch_bounds__(256) void k_green_overlay(
-    int npairs, int ord1, const int* __restrict__ prow, const int* __restrict__ pcol,
+    int n, int npairs, int ord1, const int* __restrict__ prow, const int* __restrict__ pcol,
     const double* __restrict__ g, const double* __restrict__ f, const double* __restrict__ pos,
     const double* __restrict__ nvec, const int* __restrict__ mat, const double* __restrict__ kmat, int nmat,
     double hsign, TO* __restrict__ G, TO* __restrict__ H, long ld, long stride, float* __restrict__ G2,
-    float* __restrict__ H2) {
+    float* __restrict__ H2, int gdiv, const long* __restrict__ poff) {
   const long b = blockIdx.y;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npairs) return;
+  if (gdiv > 0) {
+    const long gs = mnp_geo_slot(b, gdiv);
+    const long q = poff[gs] + p;
+    if (q >= poff[gs + 1]) return;
+    p = (int)q;
+    pos += 3 * n * gs;
+    nvec += 3 * n * gs;
+    mat += n * gs;
+  }
   const int i = prow[p], j = pcol[p];
   const int mi = mat[i];
   if (mat[j] != mi) return;
@@ -435,9 +452,11 @@ extern "C" int mnpbem_green_assemble(int dtype, int dual, int n, const double* p
                                      const double* area, const int* mat, const double* kmat, int nmat,
                                      int order, int npairs, const int* prow, const int* pcol, const double* g,
                                      const double* f, double hsign, void* G, void* H, long ld, long stride,
-                                     void* G32, void* H32, int batch, void* stream) {
+                                     void* G32, void* H32, int gdiv, const long* poff, int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n >= 0 && ld >= n && batch >= 0 && nmat >= 1, "green_assemble: bad size");
+  MNP_REQUIRE(gdiv <= 0 || poff != nullptr, "green_assemble: several geometry slots need the pair offsets poff");
+  if (gdiv <= 0) poff = nullptr;
   MNP_REQUIRE(!(dual && dtype != MNP_DTYPE_C128), "green_assemble: dual output needs complex128 primary");
   if (n == 0 || batch == 0) return 0;
   dim3 g1(mnp_ceil_div(n, AS_I), mnp_ceil_div(n, AS_J), batch);
@@ -445,28 +464,28 @@ extern "C" int mnpbem_green_assemble(int dtype, int dual, int n, const double* p
   const int ord1 = order + 1;
   if (dtype == MNP_DTYPE_C128 && dual) {
     hipLaunchKernelGGL((k_green_assemble<double, true>), g1, dim3(AS_I), 0, st, n, pos, nvec, area, mat, kmat, nmat,
-                       (double*)G, (double*)H, ld, stride, (float*)G32, (float*)H32);
+                       (double*)G, (double*)H, ld, stride, (float*)G32, (float*)H32, gdiv);
     MNP_LAUNCH_CHECK();
     if (npairs > 0)
-      hipLaunchKernelGGL((k_green_overlay<double, true>), g2, dim3(256), 0, st, npairs, ord1, prow, pcol, g, f, pos,
-                         nvec, mat, kmat, nmat, hsign, (double*)G, (double*)H, ld, stride, (float*)G32,
-                         (float*)H32);
+      hipLaunchKernelGGL((k_green_overlay<double, true>), g2, dim3(256), 0, st, n, npairs, ord1, prow, pcol, g, f,
+                         pos, nvec, mat, kmat, nmat, hsign, (double*)G, (double*)H, ld, stride, (float*)G32,
+                         (float*)H32, gdiv, poff);
   } else if (dtype == MNP_DTYPE_C128) {
     hipLaunchKernelGGL((k_green_assemble<double, false>), g1, dim3(AS_I), 0, st, n, pos, nvec, area, mat, kmat,
-                       nmat, (double*)G, (double*)H, ld, stride, (float*)nullptr, (float*)nullptr);
+                       nmat, (double*)G, (double*)H, ld, stride, (float*)nullptr, (float*)nullptr, gdiv);
     MNP_LAUNCH_CHECK();
     if (npairs > 0)
-      hipLaunchKernelGGL((k_green_overlay<double, false>), g2, dim3(256), 0, st, npairs, ord1, prow, pcol, g, f,
+      hipLaunchKernelGGL((k_green_overlay<double, false>), g2, dim3(256), 0, st, n, npairs, ord1, prow, pcol, g, f,
                          pos, nvec, mat, kmat, nmat, hsign, (double*)G, (double*)H, ld, stride, (float*)nullptr,
-                         (float*)nullptr);
+                         (float*)nullptr, gdiv, poff);
   } else if (mnp_is_c64(dtype)) {
     hipLaunchKernelGGL((k_green_assemble<float, false>), g1, dim3(AS_I), 0, st, n, pos, nvec, area, mat, kmat,
-                       nmat, (float*)G, (float*)H, ld, stride, (float*)nullptr, (float*)nullptr);
+                       nmat, (float*)G, (float*)H, ld, stride, (float*)nullptr, (float*)nullptr, gdiv);
     MNP_LAUNCH_CHECK();
     if (npairs > 0)
-      hipLaunchKernelGGL((k_green_overlay<float, false>), g2, dim3(256), 0, st, npairs, ord1, prow, pcol, g, f,
+      hipLaunchKernelGGL((k_green_overlay<float, false>), g2, dim3(256), 0, st, n, npairs, ord1, prow, pcol, g, f,
                          pos, nvec, mat, kmat, nmat, hsign, (float*)G, (float*)H, ld, stride, (float*)nullptr,
-                         (float*)nullptr);
+                         (float*)nullptr, gdiv, poff);
   } else {
     mnp_set_error("green_assemble: unknown dtype");
     return 1;

@@ -374,8 +374,9 @@ __global__ void k_identity(int n, T* A, long ld, long sA) {
 // per-wavelength scalars in par[b] = {e1r, e1i, e2r, e2i, k}
 template <typename T>
 __global__ void k_sigma(int mode, int n, const T* S1, T* S2, T* D, const T* E, long ld, long stride,
-                        const double* __restrict__ nvec, const double* __restrict__ par) {
+                        const double* __restrict__ nvec, const double* __restrict__ par, int gdiv) {
   const long b = blockIdx.y;
+  nvec += 3L * n * mnp_geo_slot(b, gdiv);
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (e >= (long)n * n) return;
   const int r = e / n, c = e % n;
@@ -429,8 +430,9 @@ __device__ __forceinline__ cplx<double> ik_times(double k, cplx<double> z) { ret
 //   De'    = De - e1 Sigma1 phi + ik n.(e1 a)        -> W[:, 4P:5P]
 template <typename T>
 __global__ void k_step_mod(int n, int P, const T* R, T* W, long ld, long stride, const double* nvec,
-                           const double* par) {
+                           const double* par, int gdiv) {
   const long b = blockIdx.y;
+  nvec += 3L * n * mnp_geo_slot(b, gdiv);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Par q = load_par(par, b);
@@ -457,8 +459,10 @@ __global__ void k_step_mod(int n, int P, const T* R, T* W, long ld, long stride,
 
 // step 2: U = Delta^-1 alpha' in W[:, P:4P].  rhs = De' + ik n.((e1-e2) U) -> W[:, 0:P]
 template <typename T>
-__global__ void k_step_rhs(int n, int P, T* W, long ld, long stride, const double* nvec, const double* par) {
+__global__ void k_step_rhs(int n, int P, T* W, long ld, long stride, const double* nvec, const double* par,
+                           int gdiv) {
   const long b = blockIdx.y;
+  nvec += 3L * n * mnp_geo_slot(b, gdiv);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Par q = load_par(par, b);
@@ -474,8 +478,10 @@ __global__ void k_step_rhs(int n, int P, T* W, long ld, long stride, const doubl
 
 // step 3: sig2 = Sigma^-1 rhs in W[:, 0:P].  T = ik n ((e1-e2) sig2) + alpha' -> W[:, P:4P]
 template <typename T>
-__global__ void k_step_h2pre(int n, int P, T* W, long ld, long stride, const double* nvec, const double* par) {
+__global__ void k_step_h2pre(int n, int P, T* W, long ld, long stride, const double* nvec, const double* par,
+                             int gdiv) {
   const long b = blockIdx.y;
+  nvec += 3L * n * mnp_geo_slot(b, gdiv);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Par q = load_par(par, b);
@@ -517,13 +523,13 @@ __global__ void k_step_pack(int n, int P, const T* R, const T* W, T* X, long ld,
 //   phi = G1 s1 - G2 s2;  a = G1 h1 - G2 h2
 //   alpha = H1 h1 - H2 h2 - ik n (e1 G1 s1 - e2 G2 s2)
 //   De = e1 H1 s1 - e2 H2 s2 - ik n.(e1 G1 h1 - e2 G2 h2)
-// Rout = R - A X; nrm[b] += |Rout|^2 and |R|^2 (atomic per wavelength) when nrm
-// is given; with R == nullptr, Rout = A X.
+// Rout = R - A X; with R == nullptr, Rout = A X.  The norms of the refinement
+// come from k_norms2 afterwards.
 __global__ void k_residual(int n, int P, const double* R, const double* Y, double* Rout, long ldr, long sr,
-                           long ldy, long sy, const double* nvec, const double* par, double* nrm) {
+                           long ldy, long sy, const double* nvec, const double* par, int gdiv) {
   const long b = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc_r = 0.0, acc_b = 0.0;
+  nvec += 3L * n * mnp_geo_slot(b, gdiv);
   if (i < n) {
     const Par q = load_par(par, b);
     const double* r = R ? R + 2 * (b * sr + (long)i * ldr) : nullptr;
@@ -539,11 +545,7 @@ __global__ void k_residual(int n, int P, const double* R, const double* Y, doubl
         std_(o + 2 * col, v);
         return;
       }
-      const cplx<double> rv = ldd(r + 2 * col);
-      const cplx<double> d = csub(rv, v);
-      acc_r += d.re * d.re + d.im * d.im;
-      acc_b += rv.re * rv.re + rv.im * rv.im;
-      std_(o + 2 * col, d);
+      std_(o + 2 * col, csub(ldd(r + 2 * col), v));
     };
     for (int p = 0; p < P; ++p) {
       put(p, csub(G1(p), G2(p)));
@@ -560,14 +562,47 @@ __global__ void k_residual(int n, int P, const double* R, const double* Y, doubl
       put(4 * P + p, de);
     }
   }
+}
+
+// nrm[b] = {sum |Out|^2, sum |Rin|^2} over the n x C block of wavelength b in a
+// fixed order: one workgroup per wavelength, each thread summing a fixed
+// strided set of elements, then a fixed butterfly per wave and a sum over the
+// waves in index order.  The refinement's stopping decision therefore does
+// not change from run to run.
+__global__ __launch_bounds__(1024) void k_norms2(int n, int C, const double* __restrict__ Rin,
+                                                 const double* __restrict__ Out, long ld, long stride,
+                                                 double* __restrict__ nrm) {
+  const long b = blockIdx.x;
+  const double* r = Rin + 2 * b * stride;
+  const double* o = Out + 2 * b * stride;
+  double ar = 0.0, ab = 0.0;
+  const long total = (long)n * C;
+  for (long e = threadIdx.x; e < total; e += blockDim.x) {
+    const long at = 2 * ((e / C) * ld + e % C);
+    const double o0 = o[at], o1 = o[at + 1], r0 = r[at], r1 = r[at + 1];
+    ar += o0 * o0 + o1 * o1;
+    ab += r0 * r0 + r1 * r1;
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    acc_r += __shfl_xor(acc_r, off);
-    acc_b += __shfl_xor(acc_b, off);
+    ar += __shfl_xor(ar, off);
+    ab += __shfl_xor(ab, off);
   }
-  if (nrm != nullptr && (threadIdx.x & 63) == 0) {
-    atomicAdd(nrm + 2 * b, acc_r);
-    atomicAdd(nrm + 2 * b + 1, acc_b);
+  __shared__ double sr[16], sb[16];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sr[w] = ar;
+    sb[w] = ab;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tr = 0.0, tb = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+      tr += sr[k];
+      tb += sb[k];
+    }
+    nrm[2 * b] = tr;
+    nrm[2 * b + 1] = tb;
   }
 }
 
@@ -594,10 +629,15 @@ __global__ void k_fill_double(long count, double* p, double v) {
 // pw[b] = {k0, kr, ki, e1r, e1i, e2r, e2i}; pol, dir (P, 3) real.
 __global__ void k_planewave(int n, int P, const double* pos, const double* nvec, const int* in_med,
                             const int* out_med, const double* pol, const double* dir, const double* pw,
-                            double* R, long ld, long stride, double* pot, long spot) {
+                            double* R, long ld, long stride, double* pot, long spot, int gdiv) {
   const long b = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const long gs = mnp_geo_slot(b, gdiv);
+  pos += 3 * n * gs;
+  nvec += 3 * n * gs;
+  in_med += n * gs;
+  out_med += n * gs;
   const double* q = pw + 7 * b;
   const double k0 = q[0], kr = q[1], ki = q[2];
   const cplx<double> e1 = cmake(q[3], q[4]), e2 = cmake(q[5], q[6]);
@@ -641,9 +681,14 @@ __global__ void k_planewave(int n, int P, const double* pos, const double* nvec,
 __global__ __launch_bounds__(256) void k_extinction(int n, int P, const double* X, long ld, long stride,
                                                     const double* pos, const double* area, const int* in_med,
                                                     const int* out_med, const double* pol, const double* dir,
-                                                    const double* pw, double* ext) {
+                                                    const double* pw, double* ext, int gdiv) {
   const long b = blockIdx.y;
   const int p = blockIdx.x;
+  const long gs = mnp_geo_slot(b, gdiv);
+  pos += 3 * n * gs;
+  area += n * gs;
+  in_med += n * gs;
+  out_med += n * gs;
   const double* q = pw + 7 * b;
   const double k0 = q[0], kr = q[1], ki = q[2];
   const double* dp = dir + 3 * p;
@@ -888,17 +933,17 @@ extern "C" int mnpbem_identity_batched(int dtype, int n, void* A, long ld, long 
 
 // mode 0: D = S1 - S2 ; mode 1: S2 <- Sigma (needs E)
 extern "C" int mnpbem_sigma_batched(int dtype, int mode, int n, const void* S1, void* S2, void* D, const void* E,
-                                    long ld, long stride, const double* nvec, const double* par, int batch,
-                                    void* stream) {
+                                    long ld, long stride, const double* nvec, const double* par, int gdiv,
+                                    int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   if (n <= 0 || batch <= 0) return 0;
   dim3 grid(mnp_ceil_div((long)n * n, 256), batch);
   if (dtype == MNP_DTYPE_C128)
     hipLaunchKernelGGL(k_sigma<double>, grid, dim3(256), 0, st, mode, n, (const double*)S1, (double*)S2,
-                       (double*)D, (const double*)E, ld, stride, nvec, par);
+                       (double*)D, (const double*)E, ld, stride, nvec, par, gdiv);
   else if (mnp_is_c64(dtype))
     hipLaunchKernelGGL(k_sigma<float>, grid, dim3(256), 0, st, mode, n, (const float*)S1, (float*)S2, (float*)D,
-                       (const float*)E, ld, stride, nvec, par);
+                       (const float*)E, ld, stride, nvec, par, gdiv);
   else {
     mnp_set_error("sigma: unknown dtype");
     return 1;
@@ -914,8 +959,8 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
                                         const int* piv2, const void* S1t, const void* Et, const void* St,
                                         const int* pivS, long ldm, long smat, long spiv, const void* D1,
                                         const void* D2, const void* DS, long sD, const void* R, void* W, void* X,
-                                        long ldv, long sv, const double* nvec, const double* par, int batch,
-                                        void* stream) {
+                                        long ldv, long sv, const double* nvec, const double* par, int gdiv,
+                                        int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldv >= 8 * P, "bem_solve: bad size (P <= 4)");
   if (batch <= 0) return 0;
@@ -929,18 +974,18 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
     /* W[:,0:4P] = Sigma1 [phi | a] */                                                                             \
     if ((rc = gemv_t<T>(n, 4 * P, (const T*)S1t, ldm, smat, r, ldv, sv, w, ldv, sv, 1.0, 0.0, 0.0, batch, st)))   \
       return rc;                                                                                                   \
-    hipLaunchKernelGGL(k_step_mod<T>, eg, dim3(256), 0, st, n, P, r, w, ldv, sv, nvec, par);                       \
+    hipLaunchKernelGGL(k_step_mod<T>, eg, dim3(256), 0, st, n, P, r, w, ldv, sv, nvec, par, gdiv);                 \
     MNP_LAUNCH_CHECK();                                                                                            \
     /* W[:,P:4P] = Delta^-1 alpha' */                                                                              \
     if ((rc = gemv_t<T>(n, 3 * P, (const T*)Et, ldm, smat, w + 2 * 5 * P, ldv, sv, w + 2 * P, ldv, sv, 1.0, 0.0, \
                         0.0, batch, st)))                                                                          \
       return rc;                                                                                                   \
-    hipLaunchKernelGGL(k_step_rhs<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                          \
+    hipLaunchKernelGGL(k_step_rhs<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par, gdiv);                    \
     MNP_LAUNCH_CHECK();                                                                                            \
     if ((rc = lu_solve_t<T>(n, P, (const T*)St, ldm, smat, pivS, spiv, w, ldv, sv, w + 2 * P, ldv, sv, batch,    \
                             st, (const T*)DS, sD)))                                                                \
       return rc;                                                                                                   \
-    hipLaunchKernelGGL(k_step_h2pre<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                        \
+    hipLaunchKernelGGL(k_step_h2pre<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par, gdiv);                  \
     MNP_LAUNCH_CHECK();                                                                                            \
     /* W[:,4P:7P] = Delta^-1 T */                                                                                  \
     if ((rc = gemv_t<T>(n, 3 * P, (const T*)Et, ldm, smat, w + 2 * P, ldv, sv, w + 2 * 4 * P, ldv, sv, 1.0, 0.0, \
@@ -1010,7 +1055,7 @@ extern "C" int mnpbem_bem_solve_batched_noalloc(int dtype, int n, int P, const v
                                                 const void* St, const int* pivS, long ldm, long smat, long spiv,
                                                 const void* D1, const void* D2, const void* DS, long sD,
                                                 const void* R, void* X, long ldv, long sv, const double* nvec,
-                                                const double* par, int batch, void* stream) {
+                                                const double* par, int gdiv, int batch, void* stream) {
   MNP_REQUIRE(dtype == MNP_DTYPE_C128 || mnp_is_c64(dtype), "bem_solve_noalloc: unknown dtype");
   MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldv >= 8 * P && sv >= 0, "bem_solve_noalloc: bad size (P <= 4)");
   if (batch <= 0) return 0;
@@ -1028,12 +1073,12 @@ extern "C" int mnpbem_bem_solve_batched_noalloc(int dtype, int n, int P, const v
   int rc = scratch_address(&w);
   if (rc) return rc;
   return mnpbem_bem_solve_batched(dtype, n, P, G1t, piv1, G2t, piv2, S1t, Et, St, pivS, ldm, smat, spiv, D1, D2,
-                                  DS, sD, R, w, X, ldv, sv, nvec, par, batch, stream);
+                                  DS, sD, R, w, X, ldv, sv, nvec, par, gdiv, batch, stream);
 }
 
 extern "C" int mnpbem_bem_combine_batched(int n, int P, const void* Y, long ldy, long sy, const void* R, void* Out,
                                           long ldv, long sv, const double* nvec, const double* par, double* nrm,
-                                          int batch, void* stream);
+                                          int gdiv, int batch, void* stream);
 
 // Rout = R - A X (complex128), Y: (batch, n, 16P) work buffer; nrm: (batch, 2)
 // receives ||R - A X||^2 and ||R||^2 (nrm may be NULL; with R NULL, Rout =
@@ -1041,7 +1086,7 @@ extern "C" int mnpbem_bem_combine_batched(int n, int P, const void* Y, long ldy,
 extern "C" int mnpbem_bem_residual_batched(int n, int P, const void* G1t, const void* H1t, const void* G2t,
                                            const void* H2t, long ldm, long smat, const void* R, const void* X,
                                            void* Rout, long ldv, long sv, void* Y, long ldy, long sy,
-                                           const double* nvec, const double* par, double* nrm, int batch,
+                                           const double* nvec, const double* par, double* nrm, int gdiv, int batch,
                                            void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldy >= 16 * P, "bem_residual: bad size");
@@ -1068,7 +1113,7 @@ extern "C" int mnpbem_bem_residual_batched(int n, int P, const void* G1t, const 
       break;
   }
   MNP_LAUNCH_CHECK();
-  return mnpbem_bem_combine_batched(n, P, Y, ldy, sy, R, Rout, ldv, sv, nvec, par, nrm, batch, stream);
+  return mnpbem_bem_combine_batched(n, P, Y, ldy, sy, R, Rout, ldv, sv, nvec, par, nrm, gdiv, batch, stream);
 }
 
 // Combination step of the 8N operator from precomputed products
@@ -1078,18 +1123,19 @@ extern "C" int mnpbem_bem_residual_batched(int n, int P, const void* G1t, const 
 // (mnpbem/bem/bem_ret_iter.py:1362-1501 after its four Green matvecs).
 extern "C" int mnpbem_bem_combine_batched(int n, int P, const void* Y, long ldy, long sy, const void* R, void* Out,
                                           long ldv, long sv, const double* nvec, const double* par, double* nrm,
-                                          int batch, void* stream) {
+                                          int gdiv, int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldy >= 16 * P && ldv >= 8 * P, "bem_combine: bad size");
+  MNP_REQUIRE(nrm == nullptr || (R != nullptr && R != Out), "bem_combine: norms need R, distinct from Out");
   if (batch <= 0) return 0;
+  hipLaunchKernelGGL(k_residual, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, P, (const double*)R,
+                     (const double*)Y, (double*)Out, ldv, sv, ldy, sy, nvec, par, gdiv);
+  MNP_LAUNCH_CHECK();
   if (nrm != nullptr) {
-    MNP_REQUIRE(R != nullptr, "bem_combine: norms need R");
-    hipLaunchKernelGGL(k_fill_double, dim3(mnp_ceil_div(2L * batch, 256)), dim3(256), 0, st, 2L * batch, nrm, 0.0);
+    hipLaunchKernelGGL(k_norms2, dim3(batch), dim3(1024), 0, st, n, 8 * P, (const double*)R, (const double*)Out, ldv,
+                       sv, nrm);
     MNP_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_residual, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, P, (const double*)R,
-                     (const double*)Y, (double*)Out, ldv, sv, ldy, sy, nvec, par, nrm);
-  MNP_LAUNCH_CHECK();
   return 0;
 }
 
@@ -1175,13 +1221,13 @@ extern "C" int mnpbem_convert(int dst_dtype, long count, const double* src, void
 
 extern "C" int mnpbem_planewave_batched(int n, int P, const double* pos, const double* nvec, const int* in_med,
                                         const int* out_med, const double* pol, const double* dir, const double* pw,
-                                        double* R, long ldv, long sv, double* pot, long spot, int batch,
+                                        double* R, long ldv, long sv, double* pot, long spot, int gdiv, int batch,
                                         void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n > 0 && P > 0 && ldv >= 8 * P, "planewave: bad size");
   if (batch <= 0) return 0;
   hipLaunchKernelGGL(k_planewave, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, P, pos, nvec, in_med,
-                     out_med, pol, dir, pw, R, ldv, sv, pot, spot);
+                     out_med, pol, dir, pw, R, ldv, sv, pot, spot, gdiv);
   MNP_LAUNCH_CHECK();
   return 0;
 }
@@ -1189,12 +1235,12 @@ extern "C" int mnpbem_planewave_batched(int n, int P, const double* pos, const d
 extern "C" int mnpbem_extinction_batched(int n, int P, const double* X, long ldv, long sv, const double* pos,
                                          const double* area, const int* in_med, const int* out_med,
                                          const double* pol, const double* dir, const double* pw, double* ext,
-                                         int batch, void* stream) {
+                                         int gdiv, int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n > 0 && P > 0, "extinction: bad size");
   if (batch <= 0) return 0;
   hipLaunchKernelGGL(k_extinction, dim3(P, batch), dim3(256), 0, st, n, P, X, ldv, sv, pos, area, in_med, out_med,
-                     pol, dir, pw, ext);
+                     pol, dir, pw, ext, gdiv);
   MNP_LAUNCH_CHECK();
   return 0;
 }
@@ -1633,10 +1679,9 @@ __global__ void k_adj_z(int n, int P, const double* Yv, double* Z, long ld, long
 // T = [G1^T Z1 | H1^T Wy | G2^T Z2 | H2^T Wy] (n x 16P), Wy = [y_De | y_al];
 // Out = C - A^T y (norms as in k_residual), or A^T y when C is null.
 __global__ void k_adj_combine(int n, int P, const double* C, const double* T, double* Out, long ldv, long sv,
-                              long ldt, long stt, const double* par, double* nrm) {
+                              long ldt, long stt, const double* par) {
   const long b = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc_r = 0.0, acc_b = 0.0;
   if (i < n) {
     const Par q = load_par(par, b);
     const double* t = T + 2 * (b * stt + (long)i * ldt);
@@ -1648,11 +1693,7 @@ __global__ void k_adj_combine(int n, int P, const double* C, const double* T, do
         std_(o + 2 * col, v);
         return;
       }
-      const cplx<double> cv = ldd(c + 2 * col);
-      const cplx<double> d = csub(cv, v);
-      acc_r += d.re * d.re + d.im * d.im;
-      acc_b += cv.re * cv.re + cv.im * cv.im;
-      std_(o + 2 * col, d);
+      std_(o + 2 * col, csub(ldd(c + 2 * col), v));
     };
     for (int col = 0; col < P4; ++col) {
       const cplx<double> g1 = ldd(t + 2 * col), h1 = ldd(t + 2 * (P4 + col));
@@ -1665,15 +1706,6 @@ __global__ void k_adj_combine(int n, int P, const double* C, const double* T, do
         put(P4 + col, cmake(-(g2.re + h2.re), -(g2.im + h2.im)));
       }
     }
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    acc_r += __shfl_xor(acc_r, off);
-    acc_b += __shfl_xor(acc_b, off);
-  }
-  if (nrm != nullptr && (threadIdx.x & 63) == 0) {
-    atomicAdd(nrm + 2 * b, acc_r);
-    atomicAdd(nrm + 2 * b + 1, acc_b);
   }
 }
 
@@ -1750,11 +1782,7 @@ extern "C" int mnpbem_bem_residual_adjoint_batched(int n, int P, const void* G1t
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldt >= 16 * P && ldv >= 8 * P, "bem_residual_adjoint: bad size");
   if (batch <= 0) return 0;
-  if (nrm != nullptr) {
-    MNP_REQUIRE(C != nullptr, "bem_residual_adjoint: norms need C");
-    hipLaunchKernelGGL(k_fill_double, dim3(mnp_ceil_div(2L * batch, 256)), dim3(256), 0, st, 2L * batch, nrm, 0.0);
-    MNP_LAUNCH_CHECK();
-  }
+  MNP_REQUIRE(nrm == nullptr || (C != nullptr && C != Out), "bem_residual_adjoint: norms need C, distinct from Out");
   const double* y = (const double*)Yv;
   double* z = (double*)Z;
   double* t = (double*)T;
@@ -1775,8 +1803,13 @@ extern "C" int mnpbem_bem_residual_adjoint_batched(int n, int P, const void* G1t
                            0.0, batch, st)))
     return rc;
   hipLaunchKernelGGL(k_adj_combine, eg, dim3(256), 0, st, n, P, (const double*)C, t, (double*)Out, ldv, sv, ldt,
-                     stt, par, nrm);
+                     stt, par);
   MNP_LAUNCH_CHECK();
+  if (nrm != nullptr) {
+    hipLaunchKernelGGL(k_norms2, dim3(batch), dim3(1024), 0, st, n, 8 * P, (const double*)C, (const double*)Out, ldv,
+                       sv, nrm);
+    MNP_LAUNCH_CHECK();
+  }
   return 0;
 }
 

@@ -132,7 +132,7 @@ class CompGreenRet:
         nat.call("mnpbem_green_assemble", dtype, dual, n, nat.ptr(self.pos), nat.ptr(self.nvec),
                  nat.ptr(self.area), nat.ptr(self.mat[side]), nat.ptr(kmat), self.nmat, self.order,
                  self.npairs, nat.ptr(self.prow), nat.ptr(self.pcol), nat.ptr(self.g), nat.ptr(self.f),
-                 float(hsign), nat.ptr(G), nat.ptr(H), n, n * n, nat.ptr(G32), nat.ptr(H32), B,
+                 float(hsign), nat.ptr(G), nat.ptr(H), n, n * n, nat.ptr(G32), nat.ptr(H32), 0, None, B,
                  nat.stream_handle(self.device))
 
     def eval(self, i, j, key, enei, ind=None):

@@ -66,8 +66,8 @@ class PlaneWaveRet:
         R = torch.empty((B, n, 8 * P), dtype=torch.complex128, device=g.device)
         pot = torch.empty((B, n, 12 * P), dtype=torch.complex128, device=g.device) if potentials else None
         nat.call("mnpbem_planewave_batched", n, P, nat.ptr(g.pos), nat.ptr(g.nvec), nat.ptr(inm), nat.ptr(outm),
-                 nat.ptr(pol), nat.ptr(dr), nat.ptr(pw), nat.ptr(R), 8 * P, n * 8 * P, nat.ptr(pot), n * 12 * P, B,
-                 nat.stream_handle(g.device))
+                 nat.ptr(pol), nat.ptr(dr), nat.ptr(pw), nat.ptr(R), 8 * P, n * 8 * P, nat.ptr(pot), n * 12 * P, 0,
+                 B, nat.stream_handle(g.device))
         return (R, pot) if potentials else R
 
     def potential(self, p, enei, green=None):
@@ -93,7 +93,7 @@ class PlaneWaveRet:
         pw = self._pw(p, enei_list, g.device) if pw is None else pw
         ext = torch.empty((B, P), dtype=torch.float64, device=g.device)
         nat.call("mnpbem_extinction_batched", n, P, nat.ptr(X), 8 * P, n * 8 * P, nat.ptr(g.pos), nat.ptr(g.area),
-                 nat.ptr(inm), nat.ptr(outm), nat.ptr(pol), nat.ptr(dr), nat.ptr(pw), nat.ptr(ext), B,
+                 nat.ptr(inm), nat.ptr(outm), nat.ptr(pol), nat.ptr(dr), nat.ptr(pw), nat.ptr(ext), 0, B,
                  nat.stream_handle(g.device))
         return ext
 

@@ -34,7 +34,7 @@ def _solve_both(m, nat, nv, lams, precision):
     common = (F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.piv[0]), nat.ptr(F.GG[1]), nat.ptr(F.piv[1]),
               nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]), nat.ptr(F.piv[3]), n, nn, n, nat.ptr(F.DG[0]),
               nat.ptr(F.DG[1]), nat.ptr(F.DS), F.sD, nat.ptr(R))
-    tail = (8 * P, n * 8 * P, nat.ptr(bem.g.nvec), nat.ptr(F.par), F.B, st)
+    tail = (8 * P, n * 8 * P, nat.ptr(bem.g.nvec), nat.ptr(F.par), 0, F.B, st)
     print("solve with caller work space: N=%d B=%d %s" % (n, F.B, precision), flush=True)
     nat.call("mnpbem_bem_solve_batched", *common, nat.ptr(W), nat.ptr(X1), *tail)
     X2 = torch.full_like(R, float("nan"))
@@ -64,5 +64,5 @@ def test_noalloc_rejects_oversized_batch(cuda):
     assert need > 2 * (64 << 20)   # far above the array: the host check must refuse it
     with pytest.raises(RuntimeError, match="exceeds the static"):
         nat.call("mnpbem_bem_solve_batched_noalloc", nat.DTYPE_C128, n, P, None, None, None, None, None, None,
-                 None, None, n, n * n, n, None, None, None, 0, None, None, 8 * P, n * 8 * P, None, None, B,
+                 None, None, n, n * n, n, None, None, None, 0, None, None, 8 * P, n * 8 * P, None, None, 0, B,
                  nat.stream_handle(cuda))
