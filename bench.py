@@ -311,13 +311,18 @@ def main():
 
 
 def bench_uneven(args):
-    """32 particles of uneven size, each swept over 64 wavelengths in one
-    batched factor/solve chunk: seeded kinds (sphere / flat ellipsoid), sizes
-    (144, 256, 400, 900, 1444 vertices -> 284 ... 2884 faces), sphere
+    """32 particles of uneven size, 64 wavelengths each, solved together by
+    BEMRetBatch: seeded kinds (sphere with curved faces / flat ellipsoid),
+    sizes (144, 256, 400, 900, 1444 vertices -> 284 ... 2884 faces), sphere
     diameters U(20, 80) nm, ellipsoid semi-axes U(10, 40) nm, Drude gold with
-    gamma U(0.05, 0.1) eV; mixed precision with FP64 refinement."""
+    gamma U(0.05, 0.1) eV; mixed precision with FP64 refinement.  The
+    particles are grouped by face count and every group's matrices of one
+    wavelength block share each launch; the block step of all groups is one
+    captured HIP graph, replayed 64 / L times per step (L from the free
+    memory).  Set-up (GPU refinement data of all particles) is timed apart."""
     import torch
     import mnpbem_amd as m
+    from mnpbem_amd.batch import BEMRetBatch, BatchSweepGraph
     rng = np.random.default_rng(args.seed)
     sizes = [144, 256, 400, 900, 1444]
     lams = np.linspace(400.0, 800.0, 64)
@@ -330,42 +335,74 @@ def bench_uneven(args):
         gam = float(rng.uniform(0.05, 0.1))
         specs.append((kind, nv, geo, gam))
 
-    def run_one(spec):
+    def build(spec):
         kind, nv, geo, gam = spec
         part = m.trisphere(nv, geo) if kind == "sphere" else m.triellipsoid(nv, geo)
-        cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(10.0, 9.065, gam)], [part], [[2, 1]], 1)
-        bem = m.BEMRet(cp, precision=args.precision)
-        ext = bem.solve_sweep(lams, pw, chunk=64)
-        return cp.n, ext, bem
+        return m.ComParticle([m.EpsConst(1.0), m.EpsDrude(10.0, 9.065, gam)], [part], [[2, 1]], 1)
 
-    run_one(specs[0])  # warm-up: library, caches
-    torch.cuda.synchronize()
+    parts = [build(s) for s in specs]
+    faces = [p.n for p in parts]
     t0 = time.perf_counter()
-    faces = []
-    exts = []
-    for k, spec in enumerate(specs):
-        n, ext, _ = run_one(spec)
-        faces.append(n)
-        exts.append(ext)
-        log("particle %d: %s %d faces done" % (k, spec[0], n))
+    bb = BEMRetBatch(parts, precision=args.precision)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    # accuracy spot check: the largest particle again in complex128
+    t_setup = time.perf_counter() - t0
+    L = bb.auto_block(len(lams))
+    groups = [(g.n, g.S) for g in bb.groups]
+    log("uneven: %d particles, %d faces, groups (faces, particles) %s, block %d wavelengths, set-up %.2f s"
+        % (len(parts), sum(faces), groups, L, t_setup))
+    sg = BatchSweepGraph(bb, pw, L, iters=args.graph_iters)
+    for w in range(max(args.warmup, 1)):
+        sg.solve_sweep(lams)        # the first run captures
+        log("warmup %d done" % w)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for k in range(args.steps):
+        ext = sg.solve_sweep(lams)
+        log("step %d done (final residual %.2e)" % (k, sg.last_residual))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t1) / args.steps
+    del sg
+    torch.cuda.empty_cache()
+    # the same batches with eager launches, and the per-particle loop of
+    # BEMRet sweeps it replaces (64 wavelengths per particle in one chunk)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ext_eager = bb.solve_sweep(lams, pw, block=L)
+    torch.cuda.synchronize()
+    dt_eager = time.perf_counter() - t2
+    t3 = time.perf_counter()
+    seq = []
+    for g in bb.greens:
+        seq.append(m.BEMRet(g.p1, precision=args.precision, green=g).solve_sweep(lams, pw, chunk=64))
+    torch.cuda.synchronize()
+    dt_seq = time.perf_counter() - t3
+    diff_eager = max(float(np.max(np.abs(e / q - 1))) for e, q in zip(ext, ext_eager))
+    diff_seq = max(float(np.max(np.abs(e / q - 1))) for e, q in zip(ext, seq))
+    # accuracy: the largest particle in complex128
     big = int(np.argmax(faces))
-    kind, nv, geo, gam = specs[big]
-    part = m.trisphere(nv, geo) if kind == "sphere" else m.triellipsoid(nv, geo)
-    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(10.0, 9.065, gam)], [part], [[2, 1]], 1)
-    e64 = m.BEMRet(cp, precision="double").solve_sweep(lams, pw, chunk=64)
-    err = float(np.max(np.abs(exts[big] / e64 - 1)))
+    e64 = m.BEMRet(parts[big], precision="double", green=bb.greens[big]).solve_sweep(lams, pw, chunk=32)
+    err = float(np.max(np.abs(ext[big] / e64 - 1)))
     units = len(specs) * len(lams)
+    work = 32.0 * float(np.sum(np.asarray(faces, dtype=float) ** 3)) * len(lams)
+    log("uneven: graph %.1f particle-wavelengths/s (%.3f s/step), eager %.3f s, per-particle loop %.3f s; "
+        "largest vs complex128 %.2e" % (units / dt, dt, dt_eager, dt_seq, err))
     line = {"metric": "particle-wavelengths solved/s, 32 uneven Drude-gold particles (config 3)",
-            "value": round(units / dt, 3), "unit": "particle-wavelengths/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-            "ms_per_step": round(1000.0 * dt, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "value": round(units / dt, 3), "unit": "particle-wavelengths/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": max(args.warmup, 1), "ms_per_step": round(1000.0 * dt, 1), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
             "dtype": "c64(bf16x3 GEMM)+f64-refine" if args.precision == "mixed" else "c64+f64-refine",
             "data": "synthetic",
             "config": {"workload": "32 seeded spheres/ellipsoids, 64 wavelengths each, 2 plane waves, seed %d"
                                    % args.seed, "faces": faces, "total_faces": int(sum(faces)),
-                       "setup_included": True},
+                       "groups_faces_particles": groups, "block_wavelengths": L,
+                       "execution": "hipGraph replay of one wavelength block of all groups (%d replays per step, "
+                                    "%d refinement solves)" % (len(lams) // L, args.graph_iters),
+                       "setup_included": False},
+            "setup_s": round(t_setup, 3),
+            "path_tflops_W": round(work / dt / 1e12, 3),
+            "eager_value": round(units / dt_eager, 3),
+            "per_particle_loop_value": round(units / dt_seq, 3),
+            "max_rel_diff_vs_eager": diff_eager, "max_rel_diff_vs_per_particle": diff_seq,
             "max_rel_ext_err_vs_fp64_largest": err}
     print(json.dumps(line), flush=True)
 

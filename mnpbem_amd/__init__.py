@@ -12,7 +12,7 @@ from .geometry import ComParticle, Particle, triellipsoid, trirod, trisphere, tr
 
 __all__ = ["EV2NM", "EpsConst", "EpsDrude", "EpsTable", "ComParticle", "Particle", "trisphere",
            "trisphere_fibonacci", "triellipsoid", "trirod",
-           "CompGreenRet", "BEMRet", "CompStruct", "SweepGraph", "PlaneWaveRet", "BEMIter", "BEMRetIter", "ClusterTree",
+           "CompGreenRet", "BEMRet", "CompStruct", "SweepGraph", "BEMRetBatch", "BatchSweepGraph", "PlaneWaveRet", "BEMIter", "BEMRetIter", "ClusterTree",
            "HMatrix", "ACACompGreenRet", "make_kaware_fadmiss", "DrudeExtinction",
            "extinction_with_drude_gradient", "ShapeExtinction", "extinction_with_shape_gradient"]
 
@@ -26,6 +26,9 @@ def __getattr__(name):
     if name in ("BEMRet", "CompStruct", "SweepGraph"):
         from . import bem
         return getattr(bem, name)
+    if name in ("BEMRetBatch", "BatchSweepGraph"):
+        from . import batch
+        return getattr(batch, name)
     if name in ("BEMIter", "BEMRetIter"):
         from . import bem_iter
         return getattr(bem_iter, name)
