@@ -287,6 +287,17 @@ int mnpbem_lu_solve_n_batched(int dtype, int n, int C, const void* LU, long ld, 
                               long spiv, void* V, long ldv, long sV, void* Z, long ldz, long sZ, const void* Dinv,
                               long sD, int batch, void* stream);
 
+/* Right side R = [phi | a | De | alpha] (n x 8P, leading dimension ldv) of the
+ * BEM equations for one wavelength from an excitation's potentials: phi1,
+ * phi1p, phi2, phi2p (n x P), a1, a1p, a2, a2p (n x 3 x P, index c P + p),
+ * complex128, any NULL for zero; eps1, eps2: per-face complex permittivities
+ * inside / outside; k0 = 2 pi / lambda.  Replaces BEMRet._excitation
+ * (bem_ret.py:1360-1429; GdA&H Eqs. 10, 11, 15, 18). */
+int mnpbem_excitation(int n, int P, const double* phi1, const double* phi1p, const double* a1, const double* a1p,
+                      const double* phi2, const double* phi2p, const double* a2, const double* a2p,
+                      const double* nvec, const double* eps1, const double* eps2, double k0, double* R, long ldv,
+                      void* stream);
+
 /* dst (double) += src (complex64 or complex128 as reals); dst = (dtype) src. */
 int mnpbem_accumulate(int src_dtype, long count, const void* src, double* dst, void* stream);
 int mnpbem_convert(int dst_dtype, long count, const double* src, void* dst, void* stream);
@@ -331,6 +342,28 @@ int mnpbem_green_apply(int n, int cols, double hsign, const double* pos, const d
                        const int* mat, const double* kmat, int nmat, int order, const long* rowptr,
                        const int* pcol, const double* g, const double* f, const void* X, long ldx, long sx,
                        int xoff, void* Y, long ldy, long sy, int yg, int yh, int batch, void* stream);
+
+/* Cartesian derivative of the Green function of one side, Gp (hsign 0),
+ * H1p = Gp + 2 pi n_i on the diagonal (hsign +1) or H2p (hsign -1), for
+ * `batch` wavelengths as (batch, n, 3, n) complex128 in the layout of
+ * CompGreenRet.eval(i, i, 'Gp' / 'H1p' / 'H2p') (replaces compgreen_ret.py:
+ * 780-851 and greenret_refined.py:812-864); pairs whose materials on this side
+ * differ are 0.  mat, kmat, rowptr, pcol, g, f as for mnpbem_surface_field;
+ * n <= 65535. */
+int mnpbem_green_deriv(int n, double hsign, const double* pos, const double* nvec, const double* area,
+                       const int* mat, const double* kmat, int nmat, int order, const long* rowptr, const int* pcol,
+                       const double* g, const double* f, void* out, int batch, void* stream);
+
+/* Selected entries (rows[e], cols[e]) of one side at one wavelength (kmat: nmat
+ * complex wavenumbers): kind 0 G; kind 1 F + hsign 2 pi on the diagonal (F, H1,
+ * H2); kind 2 Gp + hsign 2 pi n_i on the diagonal (Gp, H1p, H2p), three values
+ * per entry; out: count (x 3) complex128.  Replaces CompGreenRet.eval(i, j,
+ * key, enei, ind) (compgreen_ret.py:860-908, greenret_refined.py via
+ * compgreen_ret.py:1770-1857; MATLAB Greenfun/@greenret/private/eval2.m). */
+int mnpbem_green_select(int n, int kind, double hsign, const double* pos, const double* nvec, const double* area,
+                        const int* mat, const double* kmat, int nmat, int order, const long* rowptr, const int* pcol,
+                        const double* g, const double* f, long count, const int* rows, const int* cols, void* out,
+                        void* stream);
 
 /* ---- permittivity derivatives (the gradient path; no counterpart in the
  *      reference, which has no derivatives) ----

@@ -87,6 +87,9 @@ SIGNATURES = {
                              _l, _l, _i, _p],
     "mnpbem_green_apply": [_i, _i, _d, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p, _p, _l, _l, _i, _p, _l, _l, _i,
                            _i, _i, _p],
+    "mnpbem_green_deriv": [_i, _d, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p, _p, _i, _p],
+    "mnpbem_green_select": [_i, _i, _d, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p, _l, _p, _p, _p, _p],
+    "mnpbem_excitation": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _p, _l, _p],
     "mnpbem_green_dk": [_i, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p, _p, _l, _l, _i, _p],
     "mnpbem_deps_rhs": [_i, _i, _p, _l, _l, _p, _l, _l, _p, _p, _p, _i, _p],
     "mnpbem_prof_begin": [_i],

@@ -394,13 +394,14 @@ class BEMRet:
         return self.init(enei)
 
     def solve(self, exc):
-        """Surface charges and currents sig1, sig2, h1, h2 for a plane-wave
-        excitation from PlaneWaveRet.potential (bem_ret.py:1569-1831)."""
+        """Surface charges and currents sig1, sig2, h1, h2 (bem_ret.py:1569-1831)
+        for an excitation CompStruct: the one PlaneWaveRet.potential returns,
+        or any with potentials phi1, phi1p, a1, a1p, phi2, phi2p, a2, a2p
+        (missing fields are 0; phi (n,) or (n, npol), a (n, 3) or (n, 3, npol),
+        numpy or torch)."""
         enei = exc.enei
         self.init(enei)
-        R = exc.get("_R")
-        if R is None:
-            raise NotImplementedError("BEMRet.solve: excitations other than PlaneWaveRet.potential")
+        R = excitation_rhs(self.p, self.g, exc)
         X = self.solve_batch(self._fac, R)
         return _unpack(self.p, enei, X[0], R.shape[2] // 8), self
 
@@ -510,6 +511,53 @@ class SweepGraph:
             raise RuntimeError("SweepGraph: residual %.2e above refine_tol after %d steps; use more iters"
                                % (rel.max(), self.iters))
         return self.ext.cpu().numpy()
+
+
+_POTENTIALS = ("phi1", "phi1p", "a1", "a1p", "phi2", "phi2p", "a2", "a2p")
+
+
+def excitation_rhs(p, g, exc):
+    """Right side R (1, n, 8P) = [phi | a | De | alpha] of the BEM equations
+    for an excitation CompStruct (BEMRet._excitation, bem_ret.py:1360-1429):
+    the plane wave's precomputed right side, or the one the HIP kernel forms
+    from the potentials phi1, phi1p, a1, a1p, phi2, phi2p, a2, a2p."""
+    R = exc.get("_R")
+    if R is not None:
+        return R
+    n, dev = p.n, g.device
+    vals, P = {}, None
+    for name in _POTENTIALS:
+        v = exc.get(name)
+        if v is None or (np.isscalar(v) and v == 0):
+            vals[name] = None
+            continue
+        t = (v if torch.is_tensor(v) else torch.as_tensor(np.asarray(v))).to(device=dev, dtype=torch.complex128)
+        if name.startswith("phi"):
+            if t.ndim not in (1, 2) or t.shape[0] != n:
+                raise ValueError("BEMRet.solve: %s must have shape (n,) or (n, npol) with n = %d" % (name, n))
+            t = t.reshape(n, -1)
+            q = t.shape[1]
+        else:
+            if t.ndim not in (2, 3) or tuple(t.shape[:2]) != (n, 3):
+                raise ValueError("BEMRet.solve: %s must have shape (n, 3) or (n, 3, npol) with n = %d" % (name, n))
+            t = t.reshape(n, 3, -1)
+            q = t.shape[2]
+        if P is not None and q != P:
+            raise ValueError("BEMRet.solve: the potentials have different numbers of polarisations")
+        P = q
+        vals[name] = t.contiguous()
+    if P is None:
+        raise ValueError("BEMRet.solve: the excitation has no potentials (phi1, a1, ..., a2p)")
+    if P > 4:
+        raise ValueError("BEMRet.solve: at most 4 polarisations per solve")
+    enei = exc.enei
+    eps = lambda side: torch.as_tensor(np.array([complex(p.eps[m - 1](enei)[0]) for m in p.inout_faces[:, side]]),
+                                       dtype=torch.complex128).to(dev)
+    e1, e2 = eps(0), eps(1)
+    R = torch.empty((1, n, 8 * P), dtype=torch.complex128, device=dev)
+    nat.call("mnpbem_excitation", n, P, *[nat.ptr(vals[k]) for k in _POTENTIALS], nat.ptr(g.nvec), nat.ptr(e1),
+             nat.ptr(e2), 2.0 * np.pi / enei, nat.ptr(R), 8 * P, nat.stream_handle(dev))
+    return R
 
 
 class RefinementMonitor:

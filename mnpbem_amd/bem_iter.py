@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .bem import BEMRet, _unpack
+from .bem import BEMRet, _unpack, excitation_rhs
 from .greenfun import CompGreenRet
 
 
@@ -404,13 +404,12 @@ class BEMRetIter(BEMIter):
         return X
 
     def solve(self, exc):
-        """Surface charges and currents for a PlaneWaveRet excitation
-        (bem_ret_iter.py:1636-1700)."""
+        """Surface charges and currents (bem_ret_iter.py:1636-1700) for an
+        excitation CompStruct: a plane wave's, or any with potentials phi1,
+        phi1p, a1, a1p, phi2, phi2p, a2, a2p (as BEMRet.solve)."""
         enei = exc.enei
         self.init(enei)
-        R = exc.get("_R")
-        if R is None:
-            raise NotImplementedError("BEMRetIter.solve: excitations other than PlaneWaveRet.potential")
+        R = excitation_rhs(self.p, self.g, exc)
         X = self.solve_batch(self._state, R)
         return _unpack(self.p, enei, X[0], R.shape[2] // 8), self
 

@@ -654,6 +654,125 @@ extern "C" int mnpbem_surface_field(int n, int P, int side, double hsign, const 
 }
 
 // ---------------------------------------------------------------------------
+// Single Green entries of one side with refinement: G_ij and Gp_ij = grad_i G
+// (deriv='cart'), e^{ikd} included; refined pairs (binary search in row i's
+// column-sorted slice of the pair list) take sum_n (ik)^n {g_n, f_n} before
+// the phase (greenret_refined.py:698-844, MATLAB Greenfun/@greenret/private/
+// eval2.m).  F_ij = n_i . Gp_ij.  The diagonal terms of H and Hp are added by
+// the callers.
+// ---------------------------------------------------------------------------
+struct GreenEntry {
+  cplx<double> G, Gp[3];
+};
+
+__device__ __forceinline__ GreenEntry green_entry(int i, int j, double kr, double ki, const double* __restrict__ pos,
+                                                  const double* __restrict__ area, const long* __restrict__ rowptr,
+                                                  const int* __restrict__ pcol, const double* __restrict__ g,
+                                                  const double* __restrict__ f, int ord1) {
+  GreenEntry e;
+  const double x = pos[3 * i] - pos[3 * j], y = pos[3 * i + 1] - pos[3 * j + 1], z = pos[3 * i + 2] - pos[3 * j + 2];
+  const double d = fmax(sqrt(x * x + y * y + z * z), 2.220446049250313e-16);
+  double s, c;
+  sincos(kr * d, &s, &c);
+  const double ex = exp(-ki * d);
+  const cplx<double> ph = cmake(ex * c, ex * s);
+  const long p1 = rowptr[i + 1];
+  long lo = rowptr[i], hi = p1;
+  while (lo < hi) {
+    const long mid = (lo + hi) >> 1;
+    if (pcol[mid] < j) lo = mid + 1; else hi = mid;
+  }
+  if (lo < p1 && pcol[lo] == j) {
+    double pr = 1.0, pim = 0.0;  // (ik)^q, ik = -ki + i kr
+    cplx<double> sg = cmake(0.0, 0.0), sf[3] = {cmake(0.0, 0.0), cmake(0.0, 0.0), cmake(0.0, 0.0)};
+    for (int q = 0; q < ord1; ++q) {
+      const double gq = g[lo * ord1 + q];
+      sg = cadd(sg, cmake(gq * pr, gq * pim));
+      for (int cc = 0; cc < 3; ++cc) {
+        const double fq = f[(lo * 3 + cc) * ord1 + q];
+        sf[cc] = cadd(sf[cc], cmake(fq * pr, fq * pim));
+      }
+      const double tr = pr * (-ki) - pim * kr;
+      pim = pr * kr + pim * (-ki);
+      pr = tr;
+    }
+    e.G = cmul(sg, ph);
+    for (int cc = 0; cc < 3; ++cc) e.Gp[cc] = cmul(sf[cc], ph);
+  } else {
+    const double id = 1.0 / d, aj = area[j];
+    e.G = cscale(ph, aj * id);
+    const cplx<double> fa = cmul(cmake((-ki - id) * id * id * aj, kr * id * id * aj), ph);
+    e.Gp[0] = cscale(fa, x);
+    e.Gp[1] = cscale(fa, y);
+    e.Gp[2] = cscale(fa, z);
+  }
+  return e;
+}
+
+// Gp (hsign 0), H1p = Gp + 2 pi n_i on the diagonal (hsign +1) or H2p (-1) of
+// one side as a (batch, n, 3, n) array: out[((b n + i) 3 + c) n + j]
+// (compgreen_ret.py:780-851, greenret_refined.py:812-864).  Pairs whose side
+// materials differ are 0.  One thread per (i, j), threads along j.
+__global__ __launch_bounds__(256) void k_green_deriv(int n, double hsign, const double* __restrict__ pos,
+                                                     const double* __restrict__ nvec, const double* __restrict__ area,
+                                                     const int* __restrict__ mat, const double* __restrict__ kmat,
+                                                     int nmat, const long* __restrict__ rowptr,
+                                                     const int* __restrict__ pcol, const double* __restrict__ g,
+                                                     const double* __restrict__ f, int ord1,
+                                                     double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  const long b = blockIdx.z;
+  if (j >= n) return;
+  const int mi = mat[i];
+  cplx<double> v[3] = {cmake(0.0, 0.0), cmake(0.0, 0.0), cmake(0.0, 0.0)};
+  if (mat[j] == mi) {
+    const double kr = kmat[2 * (b * nmat + mi)], ki = kmat[2 * (b * nmat + mi) + 1];
+    const GreenEntry e = green_entry(i, j, kr, ki, pos, area, rowptr, pcol, g, f, ord1);
+    for (int cc = 0; cc < 3; ++cc) v[cc] = e.Gp[cc];
+    if (i == j)
+      for (int cc = 0; cc < 3; ++cc) v[cc].re += hsign * 2.0 * M_PI * nvec[3 * i + cc];
+  }
+  double* o = out + 2 * (((b * n + i) * 3) * (long)n + j);
+  for (int cc = 0; cc < 3; ++cc) store_c(o + 2 * (long)cc * n, v[cc].re, v[cc].im);
+}
+
+// Selected entries (i_e, j_e) of one side at one wavelength: kind 0 G; kind 1
+// F + hsign 2 pi [i == j] (F, H1, H2); kind 2 Gp + hsign 2 pi n_i [i == j]
+// (Gp, H1p, H2p), three values per entry (compgreen_ret.py:860-908, MATLAB
+// Greenfun/@greenret/private/eval2.m).
+__global__ __launch_bounds__(256) void k_green_select(long count, int kind, double hsign,
+                                                      const int* __restrict__ rows, const int* __restrict__ cols,
+                                                      const double* __restrict__ pos, const double* __restrict__ nvec,
+                                                      const double* __restrict__ area, const int* __restrict__ mat,
+                                                      const double* __restrict__ kmat, const long* __restrict__ rowptr,
+                                                      const int* __restrict__ pcol, const double* __restrict__ g,
+                                                      const double* __restrict__ f, int ord1,
+                                                      double* __restrict__ out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  const int i = rows[e], j = cols[e];
+  const int mi = mat[i];
+  cplx<double> v[3] = {cmake(0.0, 0.0), cmake(0.0, 0.0), cmake(0.0, 0.0)};
+  if (mat[j] == mi) {
+    const GreenEntry ge = green_entry(i, j, kmat[2 * mi], kmat[2 * mi + 1], pos, area, rowptr, pcol, g, f, ord1);
+    if (kind == 0) {
+      v[0] = ge.G;
+    } else if (kind == 1) {
+      v[0] = cadd(cadd(cscale(ge.Gp[0], nvec[3 * i]), cscale(ge.Gp[1], nvec[3 * i + 1])),
+                  cscale(ge.Gp[2], nvec[3 * i + 2]));
+      if (i == j) v[0].re += hsign * 2.0 * M_PI;
+    } else {
+      for (int cc = 0; cc < 3; ++cc) v[cc] = ge.Gp[cc];
+      if (i == j)
+        for (int cc = 0; cc < 3; ++cc) v[cc].re += hsign * 2.0 * M_PI * nvec[3 * i + cc];
+    }
+  }
+  const int w = kind == 2 ? 3 : 1;
+  for (int cc = 0; cc < w; ++cc) store_c(out + 2 * (e * w + cc), v[cc].re, v[cc].im);
+}
+
+// ---------------------------------------------------------------------------
 // d/dk of the Green matrices of one side (for derivatives with respect to
 // the material's permittivity, k = k0 sqrt(eps)): from G = a_j e^{ikd}/d and
 // F = (n.r) a_j (ik - 1/d)/d^2 e^{ikd},
@@ -989,4 +1108,38 @@ extern "C" int mnpbem_green_apply(int n, int cols, double hsign, const double* p
   if (cols <= 8) MNP_GA(8)
   MNP_GA(16)
 #undef MNP_GA
+}
+
+// Gp / H1p / H2p of one side (hsign 0 / +1 / -1) for `batch` wavelengths as
+// (batch, n, 3, n) complex128, the layout of CompGreenRet.eval(i, i, 'Gp' /
+// 'H1p' / 'H2p') (compgreen_ret.py:780-851).  rowptr (int64, n + 1): offsets
+// of each row's refined pairs in the (row, col)-sorted list pcol / f.
+extern "C" int mnpbem_green_deriv(int n, double hsign, const double* pos, const double* nvec, const double* area,
+                                  const int* mat, const double* kmat, int nmat, int order, const long* rowptr,
+                                  const int* pcol, const double* g, const double* f, void* out, int batch,
+                                  void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && n <= 65535 && nmat >= 1 && batch >= 0, "green_deriv: bad size (n <= 65535)");
+  if (batch == 0) return 0;
+  hipLaunchKernelGGL(k_green_deriv, dim3(mnp_ceil_div(n, 256), n, batch), dim3(256), 0, st, n, hsign, pos, nvec,
+                     area, mat, kmat, nmat, rowptr, pcol, g, f, order + 1, (double*)out);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// Selected entries of one side at one wavelength (kmat: nmat complex
+// wavenumbers): kind 0 G, 1 F / H (hsign), 2 Gp / Hp (three values per
+// entry); rows, cols: count int32 face indices (< n, checked by the caller);
+// out: count (kinds 0, 1) or count x 3 (kind 2) complex128.
+extern "C" int mnpbem_green_select(int n, int kind, double hsign, const double* pos, const double* nvec,
+                                   const double* area, const int* mat, const double* kmat, int nmat, int order,
+                                   const long* rowptr, const int* pcol, const double* g, const double* f, long count,
+                                   const int* rows, const int* cols, void* out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && nmat >= 1 && count >= 0 && kind >= 0 && kind <= 2, "green_select: bad arguments");
+  if (count == 0) return 0;
+  hipLaunchKernelGGL(k_green_select, dim3(mnp_ceil_div(count, 256)), dim3(256), 0, st, count, kind, hsign, rows,
+                     cols, pos, nvec, area, mat, kmat, rowptr, pcol, g, f, order + 1, (double*)out);
+  MNP_LAUNCH_CHECK();
+  return 0;
 }

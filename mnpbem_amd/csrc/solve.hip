@@ -1846,3 +1846,59 @@ extern "C" int mnpbem_lu_solve_n_batched(int dtype, int n, int C, const void* LU
   mnp_set_error("lu_solve_n: unknown dtype");
   return 1;
 }
+
+// ---------------------------------------------------------------------------
+// Right side of the BEM equations from an excitation's potentials (GdA&H
+// Eqs. 10, 11, 15, 18; bem_ret.py:1360-1429):
+//   phi = phi2 - phi1,  a = a2 - a1,
+//   alpha = a2' - a1' - i k0 n (eps2 phi2 - eps1 phi1),
+//   De = eps2 phi2' - eps1 phi1' - i k0 (eps2 n.a2 - eps1 n.a1),
+// written as R = [phi | a | De | alpha] (n x 8P).  phi*, phi*p: n x P, a*, a*p:
+// n x 3 x P complex128 (row-major, index c P + p), any of them NULL for zero;
+// eps1, eps2: per-face permittivities inside / outside (complex128).
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void k_excitation(int n, int P, const double* phi1, const double* phi1p, const double* a1,
+                             const double* a1p, const double* phi2, const double* phi2p, const double* a2,
+                             const double* a2p, const double* nvec, const double* eps1, const double* eps2, double k0,
+                             double* R, long ldv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  auto ld = [&](const double* q, long e) { return q ? cmake(q[2 * e], q[2 * e + 1]) : cmake(0.0, 0.0); };
+  const cplx<double> e1 = cmake(eps1[2 * i], eps1[2 * i + 1]), e2 = cmake(eps2[2 * i], eps2[2 * i + 1]);
+  const double nx[3] = {nvec[3 * i], nvec[3 * i + 1], nvec[3 * i + 2]};
+  double* r = R + 2 * (long)i * ldv;
+  for (int p = 0; p < P; ++p) {
+    const long sp = (long)i * P + p;
+    const cplx<double> f1 = ld(phi1, sp), f2 = ld(phi2, sp);
+    std_(r + 2 * p, csub(f2, f1));
+    const cplx<double> t = csub(cmul(e2, f2), cmul(e1, f1));   // eps2 phi2 - eps1 phi1
+    cplx<double> na1 = cmake(0.0, 0.0), na2 = cmake(0.0, 0.0);
+    for (int c = 0; c < 3; ++c) {
+      const long sa = (long)i * 3 * P + c * P + p;
+      const cplx<double> va1 = ld(a1, sa), va2 = ld(a2, sa);
+      std_(r + 2 * (P + c * P + p), csub(va2, va1));
+      na1 = cadd(na1, cscale(va1, nx[c]));
+      na2 = cadd(na2, cscale(va2, nx[c]));
+      // alpha_c = a2p - a1p - i k0 n_c t
+      const cplx<double> al = csub(csub(ld(a2p, sa), ld(a1p, sa)), ik_times(k0, cscale(t, nx[c])));
+      std_(r + 2 * (5 * P + c * P + p), al);
+    }
+    const cplx<double> dp = csub(cmul(e2, ld(phi2p, sp)), cmul(e1, ld(phi1p, sp)));
+    const cplx<double> na = csub(cmul(e2, na2), cmul(e1, na1));
+    std_(r + 2 * (4 * P + p), csub(dp, ik_times(k0, na)));
+  }
+}
+}  // namespace
+
+extern "C" int mnpbem_excitation(int n, int P, const double* phi1, const double* phi1p, const double* a1,
+                                 const double* a1p, const double* phi2, const double* phi2p, const double* a2,
+                                 const double* a2p, const double* nvec, const double* eps1, const double* eps2,
+                                 double k0, double* R, long ldv, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && ldv >= 8 * P && k0 > 0, "excitation: bad size");
+  hipLaunchKernelGGL(k_excitation, dim3(mnp_ceil_div(n, 256)), dim3(256), 0, st, n, P, phi1, phi1p, a1, a1p, phi2,
+                     phi2p, a2, a2p, nvec, eps1, eps2, k0, R, ldv);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}

@@ -135,28 +135,55 @@ class CompGreenRet:
                  float(hsign), nat.ptr(G), nat.ptr(H), n, n * n, nat.ptr(G32), nat.ptr(H32), 0, None, B,
                  nat.stream_handle(self.device))
 
+    _KEYS = {"G": (0, 0.0), "F": (1, 0.0), "H1": (1, 1.0), "H2": (1, -1.0),
+             "Gp": (2, 0.0), "H1p": (2, 1.0), "H2p": (2, -1.0)}
+
     def eval(self, i, j, key, enei, ind=None):
         """Green function of regions (i, j) (0 = inside, 1 = outside) at
-        wavelength ``enei``: an (n, n) complex128 device tensor, or the int 0
-        when no particle connects the two regions (compgreen_ret.py:507-553)."""
-        if ind is not None:
-            raise NotImplementedError("CompGreenRet.eval: element selection is not supported")
-        if key not in ("G", "F", "H1", "H2"):
+        wavelength ``enei`` (compgreen_ret.py:507-553): G, F, H1 = F + 2 pi,
+        H2 = F - 2 pi as (n, n) complex128 device tensors; the Cartesian
+        derivatives Gp, H1p = Gp + 2 pi n, H2p = Gp - 2 pi n as (n, 3, n); the
+        int 0 when no particle connects the two regions.  With ``ind``
+        (row-major linear indices i * n + j into the (n, n) matrix, as
+        compgreen_ret.py:1927-1966 reads them) only those entries: (len(ind),)
+        for G, F, H1, H2 and (len(ind), 3) for Gp, H1p, H2p, the three
+        derivative components of each selected entry (MATLAB
+        Greenfun/@greenret/private/eval2.m)."""
+        if key not in self._KEYS:
             raise ValueError("Unknown key: %s" % key)
         con = self.con[i][j]
         if not np.any(con):
             return 0
         if i != j:
             raise NotImplementedError("CompGreenRet.eval: cross-region blocks (nested particles)")
+        kind, hs = self._KEYS[key]
         n = self.n
         kmat = self.wavenumbers([enei])
+        st = nat.stream_handle(self.device)
+        if ind is not None:
+            idx = np.asarray(ind, dtype=np.int64).ravel()
+            if idx.size and (idx.min() < 0 or idx.max() >= n * n):
+                raise IndexError("CompGreenRet.eval: ind outside the %d x %d matrix" % (n, n))
+            rows = torch.as_tensor((idx // n).astype(np.int32)).to(self.device)
+            cols = torch.as_tensor((idx % n).astype(np.int32)).to(self.device)
+            out = torch.empty((len(idx), 3) if kind == 2 else (len(idx),), dtype=torch.complex128,
+                              device=self.device)
+            nat.call("mnpbem_green_select", n, kind, hs, nat.ptr(self.pos), nat.ptr(self.nvec), nat.ptr(self.area),
+                     nat.ptr(self.mat[i]), nat.ptr(kmat), self.nmat, self.order, nat.ptr(self._rowptr()),
+                     nat.ptr(self.pcol), nat.ptr(self.g), nat.ptr(self.f), len(idx), nat.ptr(rows), nat.ptr(cols),
+                     nat.ptr(out), st)
+            return out
+        if kind == 2:
+            out = torch.empty((1, n, 3, n), dtype=torch.complex128, device=self.device)
+            nat.call("mnpbem_green_deriv", n, hs, nat.ptr(self.pos), nat.ptr(self.nvec), nat.ptr(self.area),
+                     nat.ptr(self.mat[i]), nat.ptr(kmat), self.nmat, self.order, nat.ptr(self._rowptr()),
+                     nat.ptr(self.pcol), nat.ptr(self.g), nat.ptr(self.f), nat.ptr(out), 1, st)
+            return out[0]
         G = torch.empty((1, n, n), dtype=torch.complex128, device=self.device)
         H = torch.empty_like(G)
-        hs = {"G": 0.0, "F": 0.0, "H1": 1.0, "H2": -1.0}[key]
         self.assemble(i, kmat, G, H, hs, nat.DTYPE_C128)
         M = G if key == "G" else H
         return M[0].transpose(0, 1)
-
 
     # ------------------------------------------------------------------
     def _rowptr(self):
