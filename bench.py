@@ -151,8 +151,7 @@ def main():
     pw = m.PlaneWaveRet(pol, dr)
     t0 = time.time()
     gemm_tiles = ("autotuned table %s" % os.path.relpath(nat.TUNING_PATH, ROOT)
-                  if os.path.exists(nat.TUNING_PATH) and os.environ.get("MNPBEM_GEMM_TUNING", "") != "default"
-                  else "built-in default table")
+                  if os.path.exists(nat.TUNING_PATH) else "built-in default table")
     bem = m.BEMRet(cp, precision=args.precision, refine_tol=args.refine_tol, residual=args.residual)
     torch.cuda.synchronize()
     log("setup (mesh + GPU refinement coefficients): %.2f s, N = %d" % (time.time() - t0, N))

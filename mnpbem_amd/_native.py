@@ -13,7 +13,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmnpbem_amd.so")
 # GEMM tile table written by mnpbem_amd.autotune (applied at load when present;
-# MNPBEM_GEMM_TUNING=default keeps the library's built-in table)
+# both tiles accumulate in the same order, so the table changes speed only)
 TUNING_PATH = os.path.join(_HERE, "data", "gemm_tuning_gfx950.json")
 
 DTYPE_C64 = 1
@@ -123,7 +123,7 @@ def load(path=None):
 
 
 def _apply_tuning(lib):
-    if os.environ.get("MNPBEM_GEMM_TUNING", "") == "default" or not os.path.exists(TUNING_PATH):
+    if not os.path.exists(TUNING_PATH):
         return
     import json
     with open(TUNING_PATH) as fh:

@@ -457,12 +457,16 @@ __global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3
 // workgroup per matrix.  Thread t holds rows t, t+NT, ... (S rows) of the
 // panel in registers; per column: workgroup argmax of |re|+|im|, row
 // exchange through LDS, scaling and rank-1 update of the remaining columns.
-// ipiv receives global (0-based) row indices.  Swaps are applied only inside
-// the panel columns; the callers apply them to the other columns.
+// ipiv receives global (0-based) row indices.  Swaps are applied inside the
+// panel columns; with fullswap the leaf also exchanges the two rows in every
+// column outside the panel as soon as the pivot is known (the interchanges
+// LAPACK's recursive getrf applies later with laswp: the same permutation,
+// applied before any later use of those rows), otherwise the callers apply
+// them to the other columns.
 // ---------------------------------------------------------------------------
 template <typename T, int W, int S, int NT>
 __global__ __launch_bounds__(NT) void k_lu_leaf(int N, int c0, int w, T* A, long lda, long sA,
-                                                int* ipiv, long sP, int* info) {
+                                                int* ipiv, long sP, int* info, int fullswap) {
   const long b = blockIdx.x;
   A += 2 * b * sA;
   ipiv += b * sP;
@@ -529,6 +533,17 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int N, int c0, int w, T* A, long
       p = ii;
     }
     if (tid == 0) ipiv[c0 + c] = c0 + p;
+    if (fullswap && p != c) {
+      // columns outside the panel: memory that no other part of this launch touches
+      T* rc_ = A + 2 * ((long)(c0 + c) * lda);
+      T* rp_ = A + 2 * ((long)(c0 + p) * lda);
+      for (int j = tid; j < N; j += NT) {
+        if (j >= c0 && j < c0 + w) continue;
+        const cplx<T> x = cld(rc_ + 2 * j), y = cld(rp_ + 2 * j);
+        cst(rc_ + 2 * j, y);
+        cst(rp_ + 2 * j, x);
+      }
+    }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int r = tid + s * NT;
@@ -835,14 +850,9 @@ inline int gclass(int x) {
 }
 inline int gcell(int M, int N, int K) { return (gclass(M) * GC + gclass(N)) * GC + gclass(K); }
 
-// XCD-aware tile order of the bf16x3 GEMM (MNPBEM_GEMM_XCD=0: dispatch order)
-int gemm_xcd_remap() {
-  static const int v = [] {
-    const char* e = getenv("MNPBEM_GEMM_XCD");
-    return (e && e[0] == '0') ? 0 : 1;
-  }();
-  return v;
-}
+// XCD-aware tile order of the bf16x3 GEMM (each XCD takes a contiguous range
+// of tiles; see gemm_tile)
+inline int gemm_xcd_remap() { return 1; }
 
 int launch_bf16x3(int variant, int M, int N, int K, const float* A, long lda, long sA, const float* B, long ldb,
                   long sB, float* C, long ldc, long sC, int batch, hipStream_t st) {
@@ -1015,12 +1025,12 @@ int leaf_width(int m) {
 
 template <typename T, int W, int SMAX, int NT>
 int leaf_launch_w(int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
-                  hipStream_t st) {
+                  hipStream_t st, int fullswap) {
   const int m = N - c0;
 #define MNP_LEAF(S_)                                                                                       \
   if (m <= (S_)*NT) {                                                                                      \
     hipLaunchKernelGGL((k_lu_leaf<T, W, S_, NT>), dim3(batch), dim3(NT), 0, st, N, c0, w, A, lda, sA, ipiv,  \
-                       sP, info);                                                                          \
+                       sP, info, fullswap);                                                                \
     MNP_LAUNCH_CHECK();                                                                                    \
     return 0;                                                                                              \
   }
@@ -1047,14 +1057,14 @@ int leaf_launch_w(int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long
 
 template <typename T>
 int leaf_launch(int W, int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
-                hipStream_t st) {
-  if (W == 8) return leaf_launch_w<T, 8, 6, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
-  if (W == 4) return leaf_launch_w<T, 4, 12, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+                hipStream_t st, int fullswap) {
+  if (W == 8) return leaf_launch_w<T, 8, 6, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st, fullswap);
+  if (W == 4) return leaf_launch_w<T, 4, 12, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st, fullswap);
   if constexpr (sizeof(T) == 4) {
-    if (W == 2) return leaf_launch_w<T, 2, 24, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
-    if (W == 1) return leaf_launch_w<T, 1, 32, 1024>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+    if (W == 2) return leaf_launch_w<T, 2, 24, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st, fullswap);
+    if (W == 1) return leaf_launch_w<T, 1, 32, 1024>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st, fullswap);
   } else {
-    if (W == 1) return leaf_launch_w<T, 1, 40, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st);
+    if (W == 1) return leaf_launch_w<T, 1, 40, 512>(N, c0, w, A, lda, sA, ipiv, sP, info, batch, st, fullswap);
   }
   mnp_set_error("getrf: matrix too tall for the register-resident panel");
   return 1;
@@ -1202,14 +1212,6 @@ __global__ __launch_bounds__(256) void k_tall_update(int N, int c0, int w, int c
   }
 }
 
-int tall_lu_enabled() {
-  static const int v = [] {
-    const char* e = getenv("MNPBEM_TALL_LU");
-    return (e && e[0] == '0') ? 0 : 1;
-  }();
-  return v;
-}
-
 // rows per matrix above which the panel is factored by the multi-workgroup
 // column kernels: where the register leaf is down to one column (measured on
 // gfx950, complex64 LU: 0.82 s instead of 1.22 s at 25000 rows, 0.33 instead
@@ -1220,7 +1222,7 @@ int g_tall_rows = 0;
 template <typename T>
 bool tall_ok(int m, int batch) {
   const int thr = g_tall_rows > 0 ? g_tall_rows : (sizeof(T) == 4 ? 12288 : 6144);
-  return tall_lu_enabled() && m > thr && batch <= TALL_MAXB && (m + TALL_CH - 1) / TALL_CH <= TALL_MAXC;
+  return m > thr && batch <= TALL_MAXB && (m + TALL_CH - 1) / TALL_CH <= TALL_MAXC;
 }
 
 template <typename T>
@@ -1241,10 +1243,13 @@ int tall_panel(int N, int c0, int w, T* A, long lda, long sA, int* ipiv, long sP
   return 0;
 }
 
-// recursive LU of columns [c0, c0+n) (rows c0..N-1)
+// recursive LU of columns [c0, c0+n) (rows c0..N-1).  fullswap: the leaves
+// exchange whole rows (every factorisation whose first panel is not a tall
+// one, so that no tall panel follows either), and the interchanges need no
+// laswp launches.
 template <typename T>
 int getrf_rec(int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP, int* info, int batch,
-              hipStream_t st, int gmode) {
+              hipStream_t st, int gmode, bool fullswap) {
   const bool tall = tall_ok<T>(N - c0, batch);
   const int W = tall ? TALL_W : leaf_width<T>(N - c0);
   if (W == 0) {
@@ -1253,21 +1258,22 @@ int getrf_rec(int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP,
   }
   if (n <= W) {
     if (tall) return tall_panel<T>(N, c0, n, A, lda, sA, ipiv, sP, info, batch, st);
-    return leaf_launch<T>(W, N, c0, n, A, lda, sA, ipiv, sP, info, batch, st);
+    return leaf_launch<T>(W, N, c0, n, A, lda, sA, ipiv, sP, info, batch, st, fullswap ? 1 : 0);
   }
   int n1 = ((n / 2 + W - 1) / W) * W;
   if (n1 >= n) n1 = n - W;
   const int n2 = n - n1;
   int rc;
-  if ((rc = getrf_rec(N, c0, n1, A, lda, sA, ipiv, sP, info, batch, st, gmode))) return rc;
-  if ((rc = laswp<T>(c0, c0 + n1, c0 + n1, c0 + n, A, lda, sA, ipiv, sP, batch, false, st))) return rc;
+  if ((rc = getrf_rec(N, c0, n1, A, lda, sA, ipiv, sP, info, batch, st, gmode, fullswap))) return rc;
+  if (!fullswap && (rc = laswp<T>(c0, c0 + n1, c0 + n1, c0 + n, A, lda, sA, ipiv, sP, batch, false, st))) return rc;
   T* A11 = A + 2 * ((long)c0 * lda + c0);
   T* A12 = A + 2 * ((long)c0 * lda + c0 + n1);
   T* A21 = A + 2 * ((long)(c0 + n1) * lda + c0);
   T* A22 = A + 2 * ((long)(c0 + n1) * lda + c0 + n1);
   if ((rc = trsm_rec<T>(true, n1, n2, A11, lda, sA, A12, lda, sA, batch, st, gmode))) return rc;
   if ((rc = gemm_sub<T>(N - c0 - n1, n2, n1, A21, lda, sA, A12, lda, sA, A22, lda, sA, batch, st, gmode))) return rc;
-  if ((rc = getrf_rec(N, c0 + n1, n2, A, lda, sA, ipiv, sP, info, batch, st, gmode))) return rc;
+  if ((rc = getrf_rec(N, c0 + n1, n2, A, lda, sA, ipiv, sP, info, batch, st, gmode, fullswap))) return rc;
+  if (fullswap) return 0;
   return laswp<T>(c0 + n1, c0 + n, c0, c0 + n1, A, lda, sA, ipiv, sP, batch, false, st);
 }
 
@@ -1458,9 +1464,10 @@ extern "C" int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA
   hipLaunchKernelGGL(k_fill_int, dim3(mnp_ceil_div(batch, 256)), dim3(256), 0, st, info, (long)batch, 0);
   MNP_LAUNCH_CHECK();
   if (dtype == MNP_DTYPE_C128)
-    return getrf_rec<double>(n, 0, n, (double*)A, lda, sA, ipiv, sP, info, batch, st, 0);
+    return getrf_rec<double>(n, 0, n, (double*)A, lda, sA, ipiv, sP, info, batch, st, 0, !tall_ok<double>(n, batch));
   if (mnp_is_c64(dtype))
-    return getrf_rec<float>(n, 0, n, (float*)A, lda, sA, ipiv, sP, info, batch, st, dtype == MNP_DTYPE_C64X ? 1 : 0);
+    return getrf_rec<float>(n, 0, n, (float*)A, lda, sA, ipiv, sP, info, batch, st, dtype == MNP_DTYPE_C64X ? 1 : 0,
+                            !tall_ok<float>(n, batch));
   mnp_set_error("getrf: unknown dtype");
   return 1;
 }

@@ -156,14 +156,17 @@ __global__ __launch_bounds__(256) void k_gemv4_t(int n, const double* __restrict
 //     the block -= L_IK^T w_I.
 // Every workgroup solves the 64 x 64 triangle (one wavefront, shuffles) and
 // updates its own 256-row tile; workgroup 0 stores the block solution.
+// Systems b >= half take their vectors hoff columns to the right of system
+// b - half's (two factorisations stacked in one batch solving different
+// column ranges of the same vectors).
 template <typename T, bool FWD>
 __global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const T* __restrict__ LU, long ld,
                                                      long sLU, T* V, long ldv, long sV, T* Z, long ldz, long sZ,
-                                                     const T* __restrict__ Dinv, long sD) {
+                                                     const T* __restrict__ Dinv, long sD, int half, long hoff) {
   const long b = blockIdx.y;
   LU += 2 * b * sLU;
-  V += 2 * b * sV;
-  Z += 2 * b * sZ;
+  V += 2 * ((b % half) * sV + (b / half) * hoff);
+  Z += 2 * ((b % half) * sZ + (b / half) * hoff);
   const int tid = threadIdx.x, lane = tid & 63;
   const int nb = min(64, n - i0), i1 = i0 + nb;
   __shared__ cplx<T> blk[64][65];
@@ -333,10 +336,10 @@ __global__ __launch_bounds__(64) void k_lu_blockinv(int n, const T* __restrict__
 // one workgroup per matrix, one thread per column.
 template <typename T>
 __global__ __launch_bounds__(256) void k_perm_t(int n, int C, const int* __restrict__ ipiv, long sP, T* V, long ldv,
-                                                long sV) {
+                                                long sV, int half, long hoff) {
   const long b = blockIdx.x;
   ipiv += b * sP;
-  V += 2 * b * sV;
+  V += 2 * ((b % half) * sV + (b / half) * hoff);
   const int tid = threadIdx.x;
   __shared__ int sp[256];
   for (int base = n; base > 0; base -= 256) {
@@ -853,10 +856,13 @@ int gemv_t(int n, int C, const T* M, long ldm, long sM, const T* X, long ldx, lo
   return 0;
 }
 
-// V <- M^-1 V (n x C) with the LU of M^T; Z (n x C) is work space.
+// V <- M^-1 V (n x C) with the LU of M^T; Z (n x C) is work space.  Systems
+// b >= half solve the columns hoff to the right in the vectors of b - half.
 template <typename T>
 int lu_solve_t(int n, int C, const T* LU, long ld, long sLU, const int* ipiv, long sP, T* V, long ldv, long sV,
-               T* Z, long ldz, long sZ, int batch, hipStream_t st, const T* Dinv = nullptr, long sD = 0) {
+               T* Z, long ldz, long sZ, int batch, hipStream_t st, const T* Dinv = nullptr, long sD = 0,
+               int half = 0, long hoff = 0) {
+  if (half <= 0) half = batch;
   const int nblk = (n + 63) / 64;
   for (int c0 = 0; c0 < C; c0 += MAXC) {
     const int cc = std::min(MAXC, C - c0);
@@ -866,17 +872,17 @@ int lu_solve_t(int n, int C, const T* LU, long ld, long sLU, const int* ipiv, lo
       const int i0 = ib * 64, i1 = std::min(n, i0 + 64);
       dim3 grid(std::max(1, mnp_ceil_div(n - i1, 256)), batch);
       hipLaunchKernelGGL((k_trsv_t_step<T, true>), grid, dim3(256), 0, st, n, cc, i0, LU, ld, sLU, v, ldv, sV, z,
-                         ldz, sZ, Dinv, sD);
+                         ldz, sZ, Dinv, sD, half, hoff);
       MNP_LAUNCH_CHECK();
     }
     for (int ib = nblk - 1; ib >= 0; --ib) {
       const int i0 = ib * 64;
       dim3 grid(std::max(1, mnp_ceil_div(i0, 256)), batch);
       hipLaunchKernelGGL((k_trsv_t_step<T, false>), grid, dim3(256), 0, st, n, cc, i0, LU, ld, sLU, v, ldv, sV, z,
-                         ldz, sZ, Dinv, sD);
+                         ldz, sZ, Dinv, sD, half, hoff);
       MNP_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_perm_t<T>, dim3(batch), dim3(256), 0, st, n, cc, ipiv, sP, v, ldv, sV);
+    hipLaunchKernelGGL(k_perm_t<T>, dim3(batch), dim3(256), 0, st, n, cc, ipiv, sP, v, ldv, sV, half, hoff);
     MNP_LAUNCH_CHECK();
   }
   return 0;
@@ -965,6 +971,13 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
   MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldv >= 8 * P, "bem_solve: bad size (P <= 4)");
   if (batch <= 0) return 0;
   dim3 eg(mnp_ceil_div(n, 256), batch);
+  // the factors of G1 and G2 as one stack (the solver's layout): both
+  // triangular solves of the last step run as one batch of 2 batch systems
+  const long esz = dtype == MNP_DTYPE_C128 ? 16 : 8;
+  const bool stacked = (const char*)G2t == (const char*)G1t + (long)batch * smat * esz &&
+                       piv2 == piv1 + (long)batch * spiv &&
+                       ((D1 == nullptr && D2 == nullptr) ||
+                        (D1 != nullptr && (const char*)D2 == (const char*)D1 + (long)batch * sD * esz));
 #define MNP_SOLVE(T)                                                                                               \
   {                                                                                                                \
     const T* r = (const T*)R;                                                                                      \
@@ -993,6 +1006,13 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
       return rc;                                                                                                   \
     hipLaunchKernelGGL(k_step_pack<T>, eg, dim3(256), 0, st, n, P, r, w, x, ldv, sv);                              \
     MNP_LAUNCH_CHECK();                                                                                            \
+    if (stacked) {                                                                                                 \
+      /* G1 and G2 factors stacked: one solve over 2 batch systems */                                            \
+      if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G1t, ldm, smat, piv1, spiv, x, ldv, sv, w, ldv, sv, 2 * batch,  \
+                              st, (const T*)D1, sD, batch, 4 * P)))                                                \
+        return rc;                                                                                                 \
+      return 0;                                                                                                    \
+    }                                                                                                              \
     if ((rc = lu_solve_t<T>(n, 4 * P, (const T*)G1t, ldm, smat, piv1, spiv, x, ldv, sv, w, ldv, sv, batch, st,    \
                             (const T*)D1, sD)))                                                                    \
       return rc;                                                                                                   \
