@@ -502,7 +502,7 @@ def bench_adjoint(args):
     import torch
     import mnpbem_amd as m
     from mnpbem_amd.adjoint import extinction_with_drude_gradient, extinction_with_shape_gradient
-    rod = m.trirod(20.0, 60.0, [31, 13, 26])
+    rod = m.trirod(20.0, 60.0, [31, 13, 26], triangles=True)
     mk = lambda verts: m.ComParticle([m.EpsConst(1.77), m.EpsDrude(*DRUDE)],
                                      [m.geometry.Particle(verts, rod.faces, interp="flat")], [[2, 1]], 1)
     cp = mk(rod.verts)

@@ -145,13 +145,19 @@ int mnpbem_refine_classify(int n, const double* pos, const double* brad, double 
  *   g[p][n], f[p][c][n] (c = x,y,z) and the quasistatic normal derivative
  *   fstat[p] = -sum_q w_q n_i.(r_i - x_q)/|r_i - x_q|^3.
  * Field faces: pos1, nvec1, tvec1, tvec2 (n1 x 3 doubles); source faces:
- * pos2 (n2 x 3) and nodes2 (n2 x 6 x 3, quadratic triangles); qnear / qself:
- * host-built tables of nq rows [shape(6), d/dx(6), d/dy(6), weight] for the
- * 28-point rule and the polar rule.  (greenret_refined.py:122-516) */
+ * pos2 (n2 x 3), nodes9 (n2 x 9 x 3: corners 0-3, edge midpoints 4-7, centre
+ * 8 of a quadrilateral; corners 0-2 and midpoints 4-6 of a triangle) and
+ * nvert2 (n2 ints, 3 or 4); qnear / qself: host-built tables of nq rows
+ * [shape(6), d/dx(6), d/dy(6), weight] for the 28-point rule and the
+ * triangle polar rule; qselfq: nq_selfq rows [shape(9), d/dx(9), d/dy(9),
+ * weight] of the quadrilateral polar rule (may be NULL with 0 rows when no
+ * face is a quadrilateral).  Quadrilaterals integrate near pairs as their two
+ * 6-node triangles.  (greenret_refined.py:122-516, particle.py:1364-1575) */
 int mnpbem_refine_coeffs(int order, int npairs, const int* prow, const int* pcol, const int* pkind,
                          const double* pos1, const double* nvec1, const double* tvec1, const double* tvec2,
-                         const double* pos2, const double* nodes2, const double* qnear, int nq_near,
-                         const double* qself, int nq_self, double* g, double* f, double* fstat, void* stream);
+                         const double* pos2, const double* nodes9, const int* nvert2, const double* qnear,
+                         int nq_near, const double* qself, int nq_self, const double* qselfq, int nq_selfq,
+                         double* g, double* f, double* fstat, void* stream);
 
 /* Fuchs-Liu closed-surface correction: corr_j = -2 pi dir - sum_i a_i F_ij / a_j
  * over rows lo[j] <= i < hi[j] (j's closed surface), with the refined pairs of

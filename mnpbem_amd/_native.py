@@ -33,7 +33,8 @@ SIGNATURES = {
     "mnpbem_gemm_tuning_seen": [_p, _i, _i],
     "mnpbem_getrf_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _i, _p],
     "mnpbem_getrs_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _i, _p],
-    "mnpbem_refine_coeffs": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _p],
+    "mnpbem_refine_coeffs": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p, _i, _p, _i, _p, _p, _p,
+                             _p],
     "mnpbem_closed_correction": [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _i, _p, _p, _p],
     "mnpbem_green_assemble": [_i, _i, _i, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _d, _p, _p, _l, _l,
                               _p, _p, _i, _p, _i, _p],

@@ -216,6 +216,9 @@ def extinction_with_shape_gradient(bem, enei_list, planewave, chunk=16, per_face
     if not isinstance(bem, BEMRet):
         raise TypeError("extinction_with_shape_gradient: expects a BEMRet solver")
     p, g = bem.p, bem.g
+    if getattr(g, "has_quads", False):
+        raise NotImplementedError("extinction_with_shape_gradient: surfaces of 6-node triangles only "
+                                  "(this particle has quadrilateral faces)")
     enei_list = np.atleast_1d(np.asarray(enei_list, dtype=float))
     P = planewave.npol
     if P > 4:

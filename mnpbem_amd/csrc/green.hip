@@ -55,12 +55,53 @@ __device__ __forceinline__ QPoint face_point(const double* __restrict__ X, const
   return q;
 }
 
+constexpr int QSTRIDE9 = 28;  // quadrilateral table row: shape[9], d/dx[9], d/dy[9], weight
+
+// point and weight of quadrature node q on the curved 9-node quadrilateral
+// X[9][3] over [-1, 1]^2: w = w_q |dX/dx x dX/dy| (particle.py:1405-1429)
+__device__ __forceinline__ QPoint face_point9(const double* __restrict__ X, const double* __restrict__ tq) {
+  double p[3] = {0, 0, 0}, tx[3] = {0, 0, 0}, ty[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double x = X[3 * c + d];
+      p[d] += tq[c] * x;
+      tx[d] += tq[9 + c] * x;
+      ty[d] += tq[18 + c] * x;
+    }
+  }
+  const double nx = tx[1] * ty[2] - tx[2] * ty[1];
+  const double ny = tx[2] * ty[0] - tx[0] * ty[2];
+  const double nz = tx[0] * ty[1] - tx[1] * ty[0];
+  QPoint q;
+  q.p[0] = p[0];
+  q.p[1] = p[1];
+  q.p[2] = p[2];
+  q.w = tq[27] * sqrt(nx * nx + ny * ny + nz * nz);
+  return q;
+}
+
+// the 6 nodes of sub-triangle `sub` of a face in the 9-node layout: a
+// triangle is (v0, v1, v2, m01, m12, m20); a quadrilateral integrates as
+// (v0, v1, v2, m01, m12, c) and (v2, v3, v0, m23, m30, c)
+// (particle.py:695-720)
+__device__ __forceinline__ void sub_triangle(const double* __restrict__ X9, int nv, int sub, double* X) {
+  const int tri[6] = {0, 1, 2, 4, 5, 6}, q1[6] = {0, 1, 2, 4, 5, 8}, q2[6] = {2, 3, 0, 6, 7, 8};
+  const int* ix = nv == 3 ? tri : (sub == 0 ? q1 : q2);
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) X[3 * c + d] = X9[3 * ix[c] + d];
+}
+
 template <int ORD>
 __global__ __launch_bounds__(256) void k_refine_coeffs(
     int npairs, const int* __restrict__ prow, const int* __restrict__ pcol, const int* __restrict__ pkind,
     const double* __restrict__ pos1, const double* __restrict__ nvec1, const double* __restrict__ tvec1,
-    const double* __restrict__ tvec2, const double* __restrict__ pos2, const double* __restrict__ nodes2,
-    const double* __restrict__ qnear, int nq_near, const double* __restrict__ qself, int nq_self,
+    const double* __restrict__ tvec2, const double* __restrict__ pos2, const double* __restrict__ nodes9,
+    const int* __restrict__ nvert2, const double* __restrict__ qnear, int nq_near,
+    const double* __restrict__ qself, int nq_self, const double* __restrict__ qselfq, int nq_selfq,
     double* __restrict__ g, double* __restrict__ f, double* __restrict__ fstat) {
   extern __shared__ double qs[];
   double* qn = qs;
@@ -71,9 +112,17 @@ __global__ __launch_bounds__(256) void k_refine_coeffs(
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npairs) return;
   const int i = prow[p], j = pcol[p], kind = pkind[p];
-  double X[18];
+  double X9[27], X[18];
 #pragma unroll
-  for (int e = 0; e < 18; ++e) X[e] = nodes2[18 * (long)j + e];
+  for (int e = 0; e < 27; ++e) X9[e] = nodes9[27 * (long)j + e];
+  const int nv = nvert2[j];
+  sub_triangle(X9, nv, 0, X);
+  // self terms of a quadrilateral: the 9-node shape functions with the
+  // quadrilateral polar rule (read from global memory); of a triangle: the
+  // 6-node shape functions with the triangle polar rule
+  const bool quad = nv == 4;
+  const int nqs = quad ? nq_selfq : nq_self;
+  auto self_point = [&](int q) { return quad ? face_point9(X9, qselfq + q * QSTRIDE9) : face_point(X, qp + q * QSTRIDE); };
   const double pi_[3] = {pos1[3 * i], pos1[3 * i + 1], pos1[3 * i + 2]};
   const double ni[3] = {nvec1[3 * i], nvec1[3 * i + 1], nvec1[3 * i + 2]};
   const double tiny = 2.220446049250313e-16;
@@ -92,16 +141,16 @@ __global__ __launch_bounds__(256) void k_refine_coeffs(
     const double t1[3] = {tvec1[3 * i], tvec1[3 * i + 1], tvec1[3 * i + 2]};
     const double t2[3] = {tvec2[3 * i], tvec2[3 * i + 1], tvec2[3 * i + 2]};
     double rmax = 0.0;
-    for (int q = 0; q < nq_self; ++q) {
-      const QPoint Q = face_point(X, qp + q * QSTRIDE);
+    for (int q = 0; q < nqs; ++q) {
+      const QPoint Q = self_point(q);
       const double vx = pi_[0] - Q.p[0], vy = pi_[1] - Q.p[1], vz = pi_[2] - Q.p[2];
       rmax = fmax(rmax, fmax(sqrt(vx * vx + vy * vy + vz * vz), tiny));
     }
     double f1[ORD + 1], f2[ORD + 1], f3[ORD + 1];
 #pragma unroll
     for (int n = 0; n <= ORD; ++n) f1[n] = f2[n] = f3[n] = 0.0;
-    for (int q = 0; q < nq_self; ++q) {
-      const QPoint Q = face_point(X, qp + q * QSTRIDE);
+    for (int q = 0; q < nqs; ++q) {
+      const QPoint Q = self_point(q);
       const double vx = pi_[0] - Q.p[0], vy = pi_[1] - Q.p[1], vz = pi_[2] - Q.p[2];
       const double r = fmax(sqrt(vx * vx + vy * vy + vz * vz), tiny);
       const double rr = fmax(r, 1e-4 * rmax);
@@ -138,7 +187,9 @@ __global__ __launch_bounds__(256) void k_refine_coeffs(
     // centroid distance r0 (greenret_refined.py:357-449)
     const double dx0 = pos2[3 * j] - pi_[0], dy0 = pos2[3 * j + 1] - pi_[1], dz0 = pos2[3 * j + 2] - pi_[2];
     const double r0 = sqrt(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
-    for (int q = 0; q < nq_near; ++q) {
+    for (int qq = 0; qq < (quad ? 2 : 1) * nq_near; ++qq) {
+      const int q = qq % nq_near;
+      if (quad && qq == nq_near) sub_triangle(X9, nv, 1, X);   // second triangle of the quadrilateral
       const QPoint Q = face_point(X, qn + q * QSTRIDE);
       const double vx = pi_[0] - Q.p[0], vy = pi_[1] - Q.p[1], vz = pi_[2] - Q.p[2];
       const double r = fmax(sqrt(vx * vx + vy * vy + vz * vz), tiny);
@@ -409,20 +460,23 @@ __global__ __launch_bounds__(256) void k_refine_classify(int n, const double* __
 
 extern "C" int mnpbem_refine_coeffs(int order, int npairs, const int* prow, const int* pcol, const int* pkind,
                                     const double* pos1, const double* nvec1, const double* tvec1,
-                                    const double* tvec2, const double* pos2, const double* nodes2,
-                                    const double* qnear, int nq_near, const double* qself, int nq_self,
-                                    double* g, double* f, double* fstat, void* stream) {
+                                    const double* tvec2, const double* pos2, const double* nodes9,
+                                    const int* nvert2, const double* qnear, int nq_near, const double* qself,
+                                    int nq_self, const double* qselfq, int nq_selfq, double* g, double* f,
+                                    double* fstat, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(order >= 0 && order <= 7, "refine_coeffs: order must be in 0..7");
   MNP_REQUIRE(nq_near > 0 && nq_self > 0 && (nq_near + nq_self) * QSTRIDE * 8 <= 60000,
               "refine_coeffs: quadrature tables too large");
+  MNP_REQUIRE(nq_selfq >= 0 && (nq_selfq == 0 || qselfq != nullptr), "refine_coeffs: bad quadrilateral table");
   if (npairs == 0) return 0;
   const size_t shm = (size_t)(nq_near + nq_self) * QSTRIDE * sizeof(double);
   dim3 grid(mnp_ceil_div(npairs, 256));
 #define MNP_RC(O)                                                                                             \
   case O:                                                                                                     \
     hipLaunchKernelGGL(k_refine_coeffs<O>, grid, dim3(256), shm, st, npairs, prow, pcol, pkind, pos1, nvec1, \
-                       tvec1, tvec2, pos2, nodes2, qnear, nq_near, qself, nq_self, g, f, fstat);            \
+                       tvec1, tvec2, pos2, nodes9, nvert2, qnear, nq_near, qself, nq_self, qselfq, nq_selfq, \
+                       g, f, fstat);                                                                          \
     break;
   switch (order) {
     MNP_RC(0) MNP_RC(1) MNP_RC(2) MNP_RC(3) MNP_RC(4) MNP_RC(5) MNP_RC(6) MNP_RC(7)

@@ -110,6 +110,11 @@ class Particle:
         self._t2 = np.cross(nv, a)
         corners = X[:, :3, :]
         self._brad = np.sqrt(((corners - self.pos[:, None, :]) ** 2).sum(axis=2)).max(axis=1)
+        # the 9-node layout of mixed triangle / quadrilateral surfaces
+        # (corners 0-2, midpoints 4-6 for a triangle)
+        self.nodes9 = np.zeros((self.n, 9, 3))
+        self.nodes9[:, [0, 1, 2, 4, 5, 6]] = X
+        self.nvert = np.full(self.n, 3, dtype=np.int32)
 
     def shift(self, vec):
         """Copy of the particle translated by ``vec`` (particle.py shift)."""
@@ -119,6 +124,150 @@ class Particle:
         """Radius of the smallest centroid-centred sphere through the corners
         (particle.py:2848-2880)."""
         return self._brad
+
+
+class MixedParticle(Particle):
+    """Curved surface of 6-node triangles and 9-node quadrilaterals
+    (particle.py:551-613, 1234-1429): ``nodes9`` (n, 9, 3) holds the corners
+    (0-3), edge midpoints (4-7) and centre (8) of a quadrilateral, or the
+    corners (0-2) and midpoints (4-6) of a triangle; ``nvert`` (n,) is 3 or
+    4; ``corners`` (n, 4, 3) the corner vertices of the cleaned mesh (NaN in
+    slot 3 of a triangle), from which the element radii are taken.
+
+    A quadrilateral integrates as its two 6-node triangles (v0, v1, v2, m01,
+    m12, c) and (v2, v3, v0, m23, m30, c) with the 28-point rule
+    (particle.py:695-720, 1520-1575); its centroid is the centre node and its
+    tangents come from the first triangle (particle.py:582-596); its self
+    term uses the 9-node shape functions with the quadrilateral polar rule."""
+
+    def __init__(self, nodes9, nvert, corners, interp="curv"):
+        self.nodes9 = np.ascontiguousarray(np.asarray(nodes9, dtype=float))
+        self.nvert = np.ascontiguousarray(np.asarray(nvert, dtype=np.int32))
+        self.corners = np.asarray(corners, dtype=float)
+        self.interp = interp
+        self.verts = self.faces = self.mid_edges = None
+        self.faces6 = np.zeros((len(self.nvert), 6), dtype=np.int64)   # face count only
+        self._geometry()
+
+    def _geometry(self):
+        N9, q = self.nodes9, self.nvert == 4
+        sub1 = N9[:, [0, 1, 2, 4, 5, 6]].copy()
+        sub1[q] = N9[q][:, [0, 1, 2, 4, 5, 8]]
+        sub2 = N9[q][:, [2, 3, 0, 6, 7, 8]]
+        self.nodes = np.ascontiguousarray(sub1)
+        S, Dx, Dy, w = tri6_tables(*rule18())
+        jac = lambda X: np.linalg.norm(np.cross(np.einsum("qc,ncd->nqd", Dx, X),
+                                                np.einsum("qc,ncd->nqd", Dy, X)), axis=2)
+        self.area = (0.5 * w[None, :] * jac(sub1)).sum(axis=1)
+        self.area[q] += (0.5 * w[None, :] * jac(sub2)).sum(axis=1)
+        S0, Dx0, Dy0, _ = tri6_tables(np.array([1 / 3]), np.array([1 / 3]), np.array([1.0]))
+        pos = sub1.transpose(0, 2, 1) @ S0[0]
+        a = sub1.transpose(0, 2, 1) @ Dy0[0]
+        b = sub1.transpose(0, 2, 1) @ Dx0[0]
+        pos[q] = N9[q, 8]
+        a[q] = sub1[q].transpose(0, 2, 1) @ np.array([0.0, -1.0, -1.0, 2.0, 2.0, -2.0])
+        b[q] = sub1[q].transpose(0, 2, 1) @ np.array([1.0, 0.0, -1.0, 0.0, 0.0, 0.0])
+        nv = np.cross(a, b)
+        nv /= np.maximum(np.linalg.norm(nv, axis=1, keepdims=True), 1e-14)
+        a /= np.maximum(np.linalg.norm(a, axis=1, keepdims=True), 1e-14)
+        self.pos, self._nvec, self._t1 = pos, nv, a
+        self._t2 = np.cross(nv, a)
+        d = np.sqrt(((self.corners - pos[:, None, :]) ** 2).sum(axis=2))
+        self._brad = np.nanmax(d, axis=1)
+
+    @property
+    def n(self):
+        return len(self.nvert)
+
+    nfaces = n
+
+    def shift(self, vec):
+        v = np.asarray(vec, dtype=float).reshape(1, 1, 3)
+        return MixedParticle(self.nodes9 + v, self.nvert, self.corners + v, interp=self.interp)
+
+
+def _quad_grid(m, n):
+    """Cells of an (m, n) vertex grid (vertices in column-major order) as
+    quadrilaterals [v00, v01, v11, v10], columns outer, rows inner."""
+    jj, ii = np.meshgrid(np.arange(n - 1), np.arange(m - 1), indexing="ij")
+    v00 = (jj * m + ii).ravel()
+    return np.stack([v00, v00 + m, v00 + m + 1, v00 + 1], axis=1)
+
+
+def _nodes9(verts, faces4):
+    """9-node layout of triangles (faces4[:, 3] < 0) and quadrilaterals:
+    corners, edge midpoints 0.5 (a + b), centre 0.25 (v0 + v1 + v2 + v3)
+    (mesh_generators.py:133-196)."""
+    tri = faces4[:, 3] < 0
+    N = np.zeros((len(faces4), 9, 3))
+    for k in range(4):
+        N[:, k] = verts[np.where(faces4[:, k] < 0, 0, faces4[:, k])]
+    N[tri, 3] = 0.0
+    for e, (a, b) in enumerate(((0, 1), (1, 2), (2, 3), (3, 0))):
+        N[:, 4 + e] = 0.5 * (N[:, a] + N[:, b])
+    N[tri, 6] = 0.5 * (N[tri, 2] + N[tri, 0])
+    N[tri, 7] = 0.0
+    N[:, 8] = 0.25 * (N[:, 0] + N[:, 1] + N[:, 2] + N[:, 3])
+    N[tri, 8] = 0.0
+    return N
+
+
+def _clean_quads(verts, faces):
+    """Particle.clean (particle.py:2331-2388) for quadrilateral grids: merge
+    vertices equal after rounding to 8 decimals (they take the rounded,
+    sorted coordinates); a quadrilateral left with three distinct corners
+    becomes the triangle of those corners in their order (slot 3 = -1)."""
+    vr = _round_digits(verts, 8)
+    uniq, inv = np.unique(vr, axis=0, return_inverse=True)
+    f = inv.reshape(-1)[faces]
+    out = np.full((len(f), 4), -1, dtype=np.int64)
+    for i, row in enumerate(f):
+        _, first = np.unique(row, return_index=True)
+        keep = row[np.sort(first)]
+        out[i, :len(keep)] = keep
+    return uniq, out
+
+
+def _trirod_quads(d, h, phi, theta, nz):
+    """The reference's default rod: quadrilateral faces with triangles at the
+    poles (mesh_generators.py:549-640).  Cap = sphere segment over (theta,
+    phi): grid cells merged (the pole row collapses to triangles), flat
+    midpoints and centres from the merged corners, every node projected onto
+    the sphere, shifted up by (h - d)/2; lower cap = its mirror image in z
+    with reversed faces; cylinder = (phi, z) grid cells in reversed corner
+    order with midpoints and centres taken in the (phi, z) parameters and
+    mapped onto the cylinder.  Corners of the three parts are merged once
+    more (rounded to 8 decimals) for the element radii."""
+    pg, tg = np.meshgrid(phi, theta)
+    cx, cy, cz = d / 2.0 * np.sin(tg) * np.cos(pg), d / 2.0 * np.sin(tg) * np.sin(pg), d / 2.0 * np.cos(tg)
+    cv = np.column_stack([cx.ravel(order="F"), cy.ravel(order="F"), cz.ravel(order="F")])
+    cv, cf = _clean_quads(cv, _quad_grid(*cx.shape))
+    N1 = _nodes9(cv, cf)
+    tri = cf[:, 3] < 0
+    nrm = np.maximum(np.sqrt(np.sum(N1 ** 2, axis=2, keepdims=True)), 1e-30)
+    N1 = 0.5 * d * (N1 / nrm)
+    N1[tri, 3] = N1[tri, 7] = N1[tri, 8] = 0.0
+    N1 = N1 + np.array([0.0, 0.0, 0.5 * (h - d)])
+    N2 = N1 * np.array([1.0, 1.0, -1.0])
+    N2q = N2[~tri][:, [3, 2, 1, 0, 6, 5, 4, 7, 8]]
+    N2t = N2[tri][:, [2, 1, 0, 3, 5, 4, 6, 7, 8]]
+    N2 = N2.copy()
+    N2[~tri], N2[tri] = N2q, N2t
+    zv = 0.5 * _mlinspace(-1, 1, nz) * (h - d)
+    gx, gy = np.meshgrid(phi, zv)
+    gv = np.column_stack([gx.ravel(order="F"), gy.ravel(order="F"), np.zeros(gx.size)])
+    gf = _quad_grid(*gx.shape)[:, ::-1]
+    Ng = _nodes9(gv, gf)
+    N3 = np.stack([0.5 * d * np.cos(Ng[:, :, 0]), 0.5 * d * np.sin(Ng[:, :, 0]), Ng[:, :, 1]], axis=2)
+    nodes = np.concatenate([N1, N2, N3])
+    nvert = np.concatenate([np.where(tri, 3, 4), np.where(tri, 3, 4), np.full(len(gf), 4)]).astype(np.int32)
+    # corners of the merged mesh (element radii)
+    cn = nodes[:, :4].copy()
+    cn[nvert == 3, 3] = np.nan
+    flat = cn.reshape(-1, 3)
+    ok = ~np.isnan(flat[:, 0])
+    flat[ok] = _round_digits(flat[ok], 8)
+    return MixedParticle(nodes, nvert, flat.reshape(-1, 4, 3))
 
 
 def add_midpoints(verts, faces, with_edges=False):
@@ -242,7 +391,7 @@ def _merge_clean(verts, faces):
     return uniq, faces[area > 1e-10 * area.mean()]
 
 
-def trirod(diameter, height, n=None, triangles=True, **kwargs):
+def trirod(diameter, height, n=None, triangles=False, **kwargs):
     """Rod (cylinder with hemispherical caps) of flat triangles, the face
     order and vertex arithmetic of mesh_generators.py:592-640:
     n = [nphi, ntheta, nz]; upper cap = sphere segment over phi x theta in
@@ -250,10 +399,8 @@ def trirod(diameter, height, n=None, triangles=True, **kwargs):
     the sphere) shifted up by (height - diameter)/2; lower cap = its mirror
     image in z with reversed faces; cylinder = phi x z grid triangles with
     reversed corner order; the three are merged (vertices rounded to 8
-    decimals) and collapsed faces dropped.  Quadrilateral faces
-    (triangles=False, the reference's default) are not supported."""
-    if not triangles:
-        raise NotImplementedError("trirod: quadrilateral faces are not supported; use triangles=True")
+    decimals) and collapsed faces dropped.  triangles=False (the reference's
+    default) gives the curved quadrilateral rod of ``_trirod_quads``."""
     n = [15, 20, 20] if n is None else list(n)
     if len(n) != 3:
         raise ValueError("trirod: n must be [nphi, ntheta, nz]")
@@ -263,6 +410,8 @@ def trirod(diameter, height, n=None, triangles=True, **kwargs):
         raise ValueError("trirod: need diameter > 0 and height >= diameter")
     phi = _mlinspace(0, 2 * np.pi, nphi)
     theta = _mlinspace(0, 0.5 * np.pi, ntheta)
+    if not triangles:
+        return _trirod_quads(d, h, phi, theta, nz)
     # cap: sphere segment (trispheresegment, mesh_generators.py:549-589)
     pg, tg = np.meshgrid(phi, theta)
     cx = d / 2.0 * np.sin(tg) * np.cos(pg)
@@ -345,6 +494,8 @@ class ComParticle:
         self.tvec2 = cat(lambda q: q.tvec2)
         self.area = cat(lambda q: q.area)
         self.nodes = cat(lambda q: q.nodes)
+        self.nodes9 = cat(lambda q: q.nodes9)
+        self.nvert = cat(lambda q: q.nvert)
         self.brad = cat(lambda q: q.bradius())
         self.inout_faces = np.concatenate([np.tile(self.inout[i], (q.n, 1)) for i, q in enumerate(self.p)])
         self.surface = np.concatenate([np.full(q.n, i) for i, q in enumerate(self.p)])

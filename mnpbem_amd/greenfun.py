@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .quadrature import kernel_table, polar_rule, rule18
+from .quadrature import kernel_table, kernel_table9, polar_rule, polar_rule_quad, rule18
 
 
 def refinement_pairs(pos, brad, rel_cutoff=3.0, abs_cutoff=0.0, device="cuda"):
@@ -75,11 +75,25 @@ class CompGreenRet:
         self._tv1 = _dev(p.tvec1, f64, dev)
         self._tv2 = _dev(p.tvec2, f64, dev)
         self._nodes = _dev(p.nodes, f64, dev)
+        # nodes in the 9-node layout of mixed surfaces (triangles: corners
+        # 0-2, midpoints 4-6), taken from the 6-node array when every face is
+        # a triangle
+        nvert = getattr(p, "nvert", None)
+        self.has_quads = nvert is not None and bool(np.any(nvert == 4))
+        if self.has_quads:
+            n9 = p.nodes9
+        else:
+            n9 = np.zeros((n, 9, 3))
+            n9[:, [0, 1, 2, 4, 5, 6]] = p.nodes
+            nvert = np.full(n, 3, dtype=np.int32)
+        self._nodes9 = _dev(n9, f64, dev)
+        self._nvert = _dev(nvert, i32, dev)
         # per side: 0-based material index of each face
         self.mat = [_dev(p.inout_faces[:, s] - 1, i32, dev) for s in (0, 1)]
         self.nmat = len(p.eps)
         qn = _dev(kernel_table(*rule18()), f64, dev)
         qs = _dev(kernel_table(*polar_rule((7, 5))), f64, dev)
+        qq = _dev(kernel_table9(*polar_rule_quad((7, 5))), f64, dev) if self.has_quads else None
         o1 = self.order + 1
         self.g = torch.empty((self.npairs, o1), dtype=f64, device=dev)
         self.f = torch.empty((self.npairs, 3, o1), dtype=f64, device=dev)
@@ -87,8 +101,9 @@ class CompGreenRet:
         st = nat.stream_handle(dev)
         nat.call("mnpbem_refine_coeffs", self.order, self.npairs, nat.ptr(self.prow), nat.ptr(self.pcol),
                  nat.ptr(self._pkind), nat.ptr(self.pos), nat.ptr(self.nvec), nat.ptr(self._tv1),
-                 nat.ptr(self._tv2), nat.ptr(self.pos), nat.ptr(self._nodes), nat.ptr(qn), qn.shape[0],
-                 nat.ptr(qs), qs.shape[0], nat.ptr(self.g), nat.ptr(self.f), nat.ptr(self.fstat), st)
+                 nat.ptr(self._tv2), nat.ptr(self.pos), nat.ptr(self._nodes9), nat.ptr(self._nvert), nat.ptr(qn),
+                 qn.shape[0], nat.ptr(qs), qs.shape[0], nat.ptr(qq), 0 if qq is None else qq.shape[0],
+                 nat.ptr(self.g), nat.ptr(self.f), nat.ptr(self.fstat), st)
         # closed-surface correction (each particle closed, outward normals)
         oc = np.lexsort((row, col))
         crow, cpair = row[oc], oc

@@ -7,6 +7,10 @@
   (mnpbem/utils/quadface.py:106-182).
 * ``tri6_tables``: quadratic shape functions and their x/y derivatives at the
   nodes, packed as the 19-column rows the GPU refinement kernel reads.
+* ``polar_rule_quad`` / ``quad9_tables``: the polar rule of quadrilaterals
+  (four rotated 45-degree sectors on [-1, 1]^2, weights summing to 4;
+  mnpbem/utils/quadface.py:184-249) and the biquadratic 9-node shape
+  functions (particle.py:1684-1750), packed as 28-column rows.
 """
 
 import numpy as np
@@ -88,4 +92,56 @@ def tri6_tables(x, y, w):
 def kernel_table(x, y, w):
     """(m, 19) rows [S(6), Dx(6), Dy(6), w] for the GPU kernel."""
     S, Dx, Dy, w = tri6_tables(x, y, w)
+    return np.ascontiguousarray(np.hstack([S, Dx, Dy, w[:, None]]))
+
+
+def polar_rule_quad(npol=(7, 5)):
+    """Polar rule about the centre of [-1, 1]^2: LGL nodes in radius (mapped
+    to (0, 1]) and angle (one 45-degree sector at 90-135 degrees, rotated
+    four times by 90 degrees), weight w_rho w_phi rho r_edge^2 with r_edge =
+    1 / |sin phi|, normalised to the square's area 4."""
+    nr, na = (npol, npol) if np.isscalar(npol) else npol
+    xr, wr = lgl(nr)
+    xa, wa = lgl(na)
+    rho = 0.5 * (xr + 1.0 + 1e-6)
+    phi = np.deg2rad(90.0 + 45.0 * xa)
+    rho2 = np.repeat(rho, len(phi))
+    phi2 = np.tile(phi, len(rho))
+    w2 = np.outer(wr, wa).ravel()
+    edge = 1.0 / np.abs(np.sin(phi2))
+    xs, ys, ws = [], [], []
+    for s in range(4):
+        ang = phi2 + s * np.deg2rad(90.0)
+        xs.append(np.cos(ang) * rho2 * edge)
+        ys.append(np.sin(ang) * rho2 * edge)
+        ws.append(w2 * rho2 * edge ** 2)
+    w = np.concatenate(ws)
+    return np.concatenate(xs), np.concatenate(ys), 4.0 * w / w.sum()
+
+
+def quad9_tables(x, y, w):
+    """Biquadratic shape functions S (m, 9) and derivatives Dx, Dy of the
+    9-node quadrilateral on [-1, 1]^2 with nodes (v0, v1, v2, v3) at
+    (-1,-1), (1,-1), (1,1), (-1,1), edge midpoints (m01, m12, m23, m30) and
+    the centre."""
+    x, y = np.asarray(x, float), np.asarray(y, float)
+    S = np.stack([0.25 * x * (x - 1) * y * (y - 1), 0.25 * x * (x + 1) * y * (y - 1),
+                  0.25 * x * (x + 1) * y * (y + 1), 0.25 * x * (x - 1) * y * (y + 1),
+                  0.5 * (1 - x ** 2) * y * (y - 1), 0.5 * x * (x + 1) * (1 - y ** 2),
+                  0.5 * (1 - x ** 2) * y * (y + 1), 0.5 * x * (x - 1) * (1 - y ** 2),
+                  (1 - x ** 2) * (1 - y ** 2)], axis=1)
+    Dx = np.stack([0.25 * (2 * x - 1) * y * (y - 1), 0.25 * (2 * x + 1) * y * (y - 1),
+                   0.25 * (2 * x + 1) * y * (y + 1), 0.25 * (2 * x - 1) * y * (y + 1),
+                   -x * y * (y - 1), 0.5 * (2 * x + 1) * (1 - y ** 2), -x * y * (y + 1),
+                   0.5 * (2 * x - 1) * (1 - y ** 2), -2 * x * (1 - y ** 2)], axis=1)
+    Dy = np.stack([0.25 * x * (x - 1) * (2 * y - 1), 0.25 * x * (x + 1) * (2 * y - 1),
+                   0.25 * x * (x + 1) * (2 * y + 1), 0.25 * x * (x - 1) * (2 * y + 1),
+                   0.5 * (1 - x ** 2) * (2 * y - 1), -x * (x + 1) * y, 0.5 * (1 - x ** 2) * (2 * y + 1),
+                   -x * (x - 1) * y, -2 * (1 - x ** 2) * y], axis=1)
+    return S, Dx, Dy, np.asarray(w, float)
+
+
+def kernel_table9(x, y, w):
+    """(m, 28) rows [S(9), Dx(9), Dy(9), w] of the quadrilateral polar rule."""
+    S, Dx, Dy, w = quad9_tables(x, y, w)
     return np.ascontiguousarray(np.hstack([S, Dx, Dy, w[:, None]]))

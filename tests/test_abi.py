@@ -48,7 +48,7 @@ def test_invalid_arguments_are_rejected_without_device_work():
     rc = lib.mnpbem_getrf_batched(_native.DTYPE_C128, 10, None, 5, 100, None, 10, None, 1, None)
     assert rc == 1
     assert b"getrf" in lib.mnpbem_last_error()
-    rc = lib.mnpbem_refine_coeffs(9, 0, *([None] * 10), 28, None, 144, None, None, None, None)
+    rc = lib.mnpbem_refine_coeffs(9, 0, *([None] * 11), 28, None, 144, None, 0, None, None, None, None)
     assert rc == 1
     # panels beyond the register-resident limits are refused up front
     with pytest.raises(RuntimeError, match="limited to 20480"):
@@ -62,7 +62,7 @@ def test_invalid_arguments_are_rejected_without_device_work():
         _native.call("mnpbem_hmat_matvec", 2, 10, 1, 1, 1, *([None] * 7), 17, None, 17, None, 17, 0, None, None,
                      None)
     with pytest.raises(RuntimeError, match="order"):
-        _native.call("mnpbem_refine_coeffs", 9, 0, *([None] * 10), 28, None, 144, None, None, None, None)
+        _native.call("mnpbem_refine_coeffs", 9, 0, *([None] * 11), 28, None, 144, None, 0, None, None, None, None)
     # recompression rank limits and tolerance
     with pytest.raises(RuntimeError, match="above 128"):
         _native.call("mnpbem_hmat_truncate", 2, 1, None, None, None, None, None, 129, 1e-6, None, None, None, None,

@@ -30,7 +30,7 @@ def _dot(a, b):
 
 def _particles(m):
     yield "sphere284", m.trisphere(144, 20.0)
-    yield "rod", m.trirod(10.0, 24.0, [10, 5, 6])
+    yield "rod", m.trirod(10.0, 24.0, [10, 5, 6], triangles=True)
 
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])

@@ -356,7 +356,7 @@ def test_trirod_flat_matches_oracle_on_reference_mesh(cuda, precision, tol):
     z = np.load(os.path.join(GOLD, "trirod_tri.npz"))
     key = "d20_h60_n15_8_20"
     print("trirod", precision, flush=True)
-    p = m.trirod(20.0, 60.0, [15, 8, 20])
+    p = m.trirod(20.0, 60.0, [15, 8, 20], triangles=True)
     cp = m.ComParticle([m.EpsConst(1.77), m.EpsDrude(*DRUDE)], [p], [[2, 1]], 1)
     nodes, tri6 = o.flat_mesh_from_corners(z[key + "_corners"])
     comp = o.Composite([o.EpsConst(1.77), o.EpsDrude(*DRUDE)], [o.CurvedSurface(nodes, tri6)], [[2, 1]])

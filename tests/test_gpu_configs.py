@@ -117,7 +117,7 @@ ROD_LAM = 720.0   # longitudinal resonance of the rod on the 600-800 nm scan
 
 
 def _rod(m, verts=None, drude=DRUDE):
-    part = m.trirod(20.0, 60.0, [31, 13, 26])
+    part = m.trirod(20.0, 60.0, [31, 13, 26], triangles=True)
     if verts is not None:
         part = m.geometry.Particle(verts, part.faces, interp="flat")
     return m.ComParticle([m.EpsConst(1.77), m.EpsDrude(*drude)], [part], [[2, 1]], 1)

@@ -32,7 +32,7 @@ EPS_OUT = 1.77
 
 
 def _rod_cp(m, verts=None):
-    part = m.trirod(10.0, 24.0, [10, 5, 6])
+    part = m.trirod(10.0, 24.0, [10, 5, 6], triangles=True)
     if verts is not None:
         part = m.geometry.Particle(verts, part.faces, interp="flat")
     return m.ComParticle([m.EpsConst(EPS_OUT), m.EpsDrude(*DRUDE)], [part], [[2, 1]], 1)

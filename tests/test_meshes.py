@@ -141,7 +141,7 @@ def test_trirod_matches_reference_golden(key):
     g = pytest.importorskip("mnpbem_amd.geometry")
     z = _golden_rods()
     d, h, n = ROD_CASES[key]
-    p = g.trirod(d, h, n)
+    p = g.trirod(d, h, n, triangles=True)
     assert np.array_equal(p.verts2[p.faces6[:, :3]], z[key + "_corners"])
     assert np.abs(p.pos - z[key + "_pos"]).max() < 1e-13 * d
     assert np.abs(p.nvec - z[key + "_nvec"]).max() < 1e-13
@@ -161,7 +161,7 @@ def test_trirod_closed_outward_and_converging():
     d, h = 20.0, 60.0
     errs = []
     for n in ([11, 6, 10], [21, 11, 20], [41, 21, 40]):
-        p = g.trirod(d, h, n)
+        p = g.trirod(d, h, n, triangles=True)
         f = p.faces6[:, :3]
         directed = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
         assert len({tuple(e) for e in directed}) == len(directed)
@@ -177,3 +177,27 @@ def test_trirod_closed_outward_and_converging():
         errs.append((abs(p.area.sum() / a_ref - 1), abs(vol / v_ref - 1)))
     assert errs[0][0] > 3 * errs[1][0] > 9 * errs[2][0] and errs[2][0] < 5e-3
     assert errs[0][1] > 3 * errs[1][1] > 9 * errs[2][1] and errs[2][1] < 5e-3
+
+
+def test_trirod_quadrilaterals_match_reference():
+    """trirod's default quadrilateral rod (triangles=False; the reference's
+    rod validation case trirod(10, 40, [15, 15, 15]) with interp='curv'):
+    face order, nodes of every face and the corners after clean() bit for bit
+    equal to the reference's (tests/golden/make_trirod_quad_golden.py),
+    centroids, normals, tangents, areas and element radii to rounding."""
+    import os
+    import mnpbem_amd as m
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "trirod_quad.npz"))
+    rod = m.trirod(10.0, 40.0, [15, 15, 15], triangles=False)
+    gn = d["nodes"]
+    assert rod.n == len(gn) == 588
+    assert np.array_equal(rod.nvert == 4, ~np.isnan(gn[:, 3, 0]))
+    ok = ~np.isnan(gn)
+    assert np.array_equal(rod.nodes9[ok], gn[ok])
+    oc = ~np.isnan(d["corners"])
+    assert np.array_equal(rod.corners[oc], d["corners"][oc])
+    for key, val in (("pos", rod.pos), ("nvec", rod.nvec), ("tvec1", rod.tvec1), ("tvec2", rod.tvec2),
+                     ("area", rod.area), ("brad", rod.bradius())):
+        assert np.abs(val - d[key]).max() < 1e-13, key
+    # a closed surface: the curved area approaches the rod's 2 pi r (h - d) + pi d^2
+    assert abs(rod.area.sum() / (2 * np.pi * 5 * 30 + np.pi * 100) - 1) < 1e-4
