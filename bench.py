@@ -109,9 +109,10 @@ def main():
     ap.add_argument("--fd-vertices", type=int, default=20,
                     help="adjoint workload: random vertices checked against central differences")
     ap.add_argument("--seed", type=int, default=0, help="seed of the uneven-particle set")
-    ap.add_argument("--dimer-precond", default="auto", choices=["block", "auto"],
-                    help="dimer workload: GMRES preconditioner (auto: the direct solve of the whole dimer; "
-                         "block: each sphere's direct solve)")
+    ap.add_argument("--dimer-precond", default="block", choices=["block", "whole"],
+                    help="dimer workload: GMRES preconditioner (block: each sphere's direct solve, one "
+                         "factorisation shared by the two translated spheres; whole: the direct solve of the "
+                         "whole dimer, an n x n factorisation)")
     ap.add_argument("--eager", action="store_true",
                     help="time the headline sweep with eager launches only (default: value is the replayed "
                          "hipGraph of the 64-wavelength chunk step, eager steps timed beside it)")
@@ -408,12 +409,12 @@ def bench_uneven(args):
 
 def bench_dimer(args):
     """Gold sphere dimer, two 12500-face golden-spiral spheres of 100 nm with a
-    2 nm gap (25000 faces): BEMRetIter on ACA H-matrices (leaf 64, htol 1e-6,
-    rank <= 30, blocks in complex64), GMRES restart 50, tol 1e-6, FP64
-    Arnoldi, preconditioned by the mixed-precision direct solve of the whole
-    dimer (--dimer-precond auto, default) or of each sphere (block);
-    500, 600, 700 nm; x polarisation along the dimer axis, propagation
-    along z."""
+    2 nm gap (25000 faces): BEMRetIter on ACA H-matrices whose entries the
+    kernels generate in place (leaf 64, htol 1e-6, rank <= 30, blocks in
+    complex64), GMRES restart 50, tol 1e-6, FP64 Arnoldi, preconditioned by
+    each sphere's mixed-precision direct solve (--dimer-precond block,
+    default) or by that of the whole dimer (whole); 500, 600, 700 nm; x
+    polarisation along the dimer axis, propagation along z."""
     import torch
     import mnpbem_amd as m
     from mnpbem_amd import _native as nat
@@ -424,7 +425,9 @@ def bench_dimer(args):
     lams = np.array([500.0, 600.0, 700.0])
     pw = m.PlaneWaveRet([[1.0, 0.0, 0.0]], [[0.0, 0.0, 1.0]])
     bem = m.BEMRetIter(cp, hmatrix=True, htol=1e-6, kmax=30, cleaf=64, tol=1e-6, restart=50, maxit=20,
-                       precision="mixed", hmat_dtype="complex64", preconditioner=args.dimer_precond)
+                       precision="mixed", hmat_dtype="complex64",
+                       preconditioner="block" if args.dimer_precond == "block" else "hlu_dense")
+    torch.cuda.reset_peak_memory_stats()
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     log("dimer set-up %.1f s, N = %d" % (t_setup, cp.n))
@@ -439,6 +442,36 @@ def bench_dimer(args):
              [x["rank_max"] for x in st]))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t1
+    peak_gb = torch.cuda.max_memory_allocated() / 1e9
+    # phases at 600 nm, outside the timed region: the preconditioner's
+    # factorisation, the in-place ACA of the four Green functions, GMRES
+    phase = {}
+    lam = np.array([600.0])
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    if bem._block_bems is not None:
+        for b in bem._block_bems:
+            b.factor(lam)
+    else:
+        bem._direct.factor(lam)
+    torch.cuda.synchronize()
+    phase["precond_factor_s"] = round(time.perf_counter() - ta, 3)
+    ta = time.perf_counter()
+    kmat = bem.g.wavenumbers(lam)
+    for s_, key in ((0, "G"), (0, "H1"), (1, "G"), (1, "H2")):
+        bem._aca.generate(s_, key, kmat[0], 600.0)
+    torch.cuda.synchronize()
+    phase["aca_in_place_s"] = round(time.perf_counter() - ta, 3)
+    S = bem._prepare(lam)
+    R = pw.rhs(cp, lam, bem.g)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    bem.solve_batch(S, R)
+    torch.cuda.synchronize()
+    phase["gmres_s"] = round(time.perf_counter() - ta, 3)
+    phase["gmres_iterations"] = int(bem.info()[2][-1][1])
+    del S, R
+    log("phases at 600 nm: %s; peak device memory of the timed solves %.1f GB" % (phase, peak_gb))
     hstat = [h.stat() for h in bem.last_hmats[0]]
     # H-matrix matvec as GMRES calls it (4 columns per side for one plane
     # wave): algorithmic bytes = stored blocks read once per product, timed with
@@ -477,7 +510,8 @@ def bench_dimer(args):
             "config": {"workload": "2 x trisphere_fibonacci(6252, 100 nm), 2 nm gap, 3 wavelengths",
                        "faces": cp.n, "cleaf": 64, "htol": 1e-6, "kmax": 30, "restart": 50, "tol": 1e-6,
                        "preconditioner": args.dimer_precond},
-            "setup_s": round(t_setup, 2), "direct_mixed_value": round(len(lams) / dt_direct, 4),
+            "setup_s": round(t_setup, 2), "phases_600nm": phase, "peak_device_GB": round(peak_gb, 2),
+            "direct_mixed_value": round(len(lams) / dt_direct, 4),
             "max_rel_ext_err_vs_direct": err,
             "hmatrix": [{"compression": round(x["compression"], 4), "rank_max": x["rank_max"],
                          "bytes": x["bytes"]} for x in hstat],

@@ -435,6 +435,26 @@ int mnpbem_hmat_aca(int nb, const long* blk, const int* bids, const int* perm, c
 /* dense blocks bids[0:nb] copied from src into val */
 int mnpbem_hmat_dense(int dtype, int nb, const long* blk, const int* bids, const int* perm, const void* src, long si,
                       long sj, void* val, void* stream);
+/* The same two fills with entries generated in place from one side's
+ * geometry and refinement tables (no assembled matrix; replaces the
+ * entry functions greenret::getrow / getcol of
+ * mex/acagreen/acagreen.cpp:79-127, here with the refined near-field
+ * entries of CompGreenRet.eval, compgreen_ret.py:507-553): kind 0 G, kind 1
+ * F + hsign 2 pi [i == j] (H1: hsign +1, H2: -1).  pos, nvec (n x 3), area
+ * (n), mat (int32 n, the side's material per face; faces of different
+ * materials give 0), kmat (nmat complex wavenumbers of one wavelength as
+ * float64 pairs), rowptr (int64 n + 1), pcol, g, f: the (row, col)-sorted
+ * refined pairs and their order + 1 coefficients, as for
+ * mnpbem_green_select. */
+int mnpbem_hmat_aca_green(int nb, const long* blk, const int* bids, const int* perm, int kind, double hsign,
+                          const double* pos, const double* nvec, const double* area, const int* mat,
+                          const double* kmat, int order, const long* rowptr, const int* pcol, const double* g,
+                          const double* f, void* Uw, void* Vw, const long* woff, int kmax, double htol,
+                          int* rank_out, void* stream);
+int mnpbem_hmat_dense_green(int dtype, int nb, const long* blk, const int* bids, const int* perm, int kind,
+                            double hsign, const double* pos, const double* nvec, const double* area, const int* mat,
+                            const double* kmat, int order, const long* rowptr, const int* pcol, const double* g,
+                            const double* f, void* val, void* stream);
 /* packed U, V of the low-rank blocks bids[0:nb] (ranks in blk) from the ACA workspaces */
 int mnpbem_hmat_compact(int dtype, int nb, const long* blk, const int* bids, const void* Uw, const void* Vw,
                         const long* woff, void* U, void* V, void* stream);

@@ -74,6 +74,9 @@ SIGNATURES = {
     "mnpbem_mv_scale_batched": [_l, _i, _i, _p, _p, _l, _p, _l, _i, _p],
     "mnpbem_hmat_aca": [_i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _i, _d, _p, _p],
     "mnpbem_hmat_dense": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p],
+    "mnpbem_hmat_aca_green": [_i, _p, _p, _p, _i, _d, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _d,
+                              _p, _p],
+    "mnpbem_hmat_dense_green": [_i, _i, _p, _p, _p, _i, _d, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p],
     "mnpbem_hmat_compact": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p],
     "mnpbem_hmat_matvec": [_i, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p, _l, _p, _l, _i, _p, _p, _p],
     "mnpbem_hmat_full": [_i, _i, _p, _p, _p, _p, _p, _p, _l, _p],

@@ -101,6 +101,10 @@ class BEMRetIter(BEMIter):
             options["precond"] = None
         if self._hlu_mode not in ("auto", "none", "hlu_dense", "hlu_tree", "block"):
             raise ValueError("BEMRetIter: unknown preconditioner '%s'" % self._hlu_mode)
+        if self._hlu_mode == "auto" and self._hmatrix and len(getattr(p, "p", [])) > 1:
+            # H-matrix operator over several particles: the per-particle
+            # direct solves, so that no matrix spans all faces
+            self._hlu_mode = "block"
         super().__init__(**options)
         if self.solver not in self.SOLVER_MAP:
             raise ValueError("[error] iterative solver not known: <{}>".format(self.solver))
@@ -116,13 +120,23 @@ class BEMRetIter(BEMIter):
         # faces (the diagonal blocks of the 8N operator: a particle's Green
         # functions, refinement and closed-surface correction do not involve
         # the other particles); GMRES carries the coupling between particles
+        # Particles that are translated copies of an earlier one (same faces
+        # and materials, nodes equal up to a shift) reuse its solver: their
+        # diagonal blocks agree to rounding, which GMRES absorbs.
         self._blocks = None
+        self._block_bems = None
         if self._hlu_mode == "block":
             from .geometry import ComParticle
-            self._blocks = []
+            self._blocks, self._block_bems, firsts = [], [], []
             for k, part in enumerate(p.p):
-                sub = ComParticle(p.eps, [part], [list(p.inout[k])], 1)
-                self._blocks.append((int(p.offsets[k]), BEMRet(sub, precision=precision, residual="matrix_free")))
+                u = next((q for q, (kk, _) in enumerate(firsts)
+                          if _translated_copy(p.p[kk], p.inout[kk], part, p.inout[k])), None)
+                if u is None:
+                    u = len(self._block_bems)
+                    firsts.append((k, part))
+                    sub = ComParticle(p.eps, [part], [list(p.inout[k])], 1)
+                    self._block_bems.append(BEMRet(sub, precision=precision, residual="matrix_free"))
+                self._blocks.append((int(p.offsets[k]), u))
         self._aca = None
         if self._hmatrix:
             from .hmatrix import ACACompGreenRet
@@ -153,7 +167,7 @@ class BEMRetIter(BEMIter):
         S.F = None
         S.Fk = None
         if self._blocks is not None:
-            S.Fk = [bem.factor(enei_list) for _, bem in self._blocks]
+            S.Fk = [bem.factor(enei_list) for bem in self._block_bems]
         if self._use_precond and self._blocks is None:
             S.F = self._direct.factor(enei_list)
             par = S.F.par
@@ -162,6 +176,18 @@ class BEMRetIter(BEMIter):
             from .bem import _par_tensor
             par = _par_tensor(e_in, e_out, 2.0 * np.pi / enei_list, dev)
         S.par = par
+        if self._hmatrix:
+            # H-matrices of G1, H1, G2, H2 per wavelength, entries generated
+            # in place by the ACA kernels (no n x n matrix)
+            kmat = g.wavenumbers(enei_list)
+            S.hm = [[self._aca.generate(s, key, kmat[b], lam) for s, key in ((0, "G"), (0, "H1"), (1, "G"),
+                                                                              (1, "H2"))]
+                    for b, lam in enumerate(enei_list)]
+            S.ops = None
+            if S.F is not None:
+                S.F.G64 = S.F.H64 = None
+            self.last_hmats = S.hm
+            return S
         if S.F is not None and getattr(S.F, "G64", None) is not None:
             ops = (S.F.G64[0], S.F.H64[0], S.F.G64[1], S.F.H64[1])
         else:
@@ -171,16 +197,8 @@ class BEMRetIter(BEMIter):
             for s, hs in ((0, 1.0), (1, -1.0)):
                 g.assemble(s, kmat, G[s], H[s], hs, nat.DTYPE_C128)
             ops = (G[0], H[0], G[1], H[1])
-        if self._hmatrix:
-            # H-matrices of G1, H1, G2, H2 per wavelength (the buffers hold M^T)
-            S.hm = [[self._aca.compress(M[b].transpose(0, 1), lam) for M in ops] for b, lam in enumerate(enei_list)]
-            S.ops = None
-            if S.F is not None:
-                S.F.G64 = S.F.H64 = None
-            self.last_hmats = S.hm
-        else:
-            S.ops = ops
-            S.hm = None
+        S.ops = ops
+        S.hm = None
         return S
 
     def _apply(self, S, X, Out, Y, P):
@@ -214,7 +232,8 @@ class BEMRetIter(BEMIter):
         else:
             R, X, W = ws["R32"], ws["X32"], ws["W32"]
             nat.call("mnpbem_convert", nat.DTYPE_C64, 2 * V.numel(), nat.ptr(V), nat.ptr(R), st)
-        for (off, bem), F in zip(self._blocks, S.Fk):
+        for off, u in self._blocks:
+            F, bem = S.Fk[u], self._block_bems[u]
             nn = F.n * F.n
             o = off * 8 * P
             nat.call("mnpbem_bem_solve_batched", F.dtype, F.n, P, nat.ptr(F.GG[0]), nat.ptr(F.piv[0]),
@@ -455,6 +474,22 @@ class BEMRetIter(BEMIter):
     def __repr__(self):
         status = "enei={:.1f}nm".format(self.enei) if self.enei is not None else "not initialized"
         return "BEMRetIter(p: {} faces, solver={}, {})".format(self.p.n, self.solver, status)
+
+
+def _translated_copy(a, io_a, b, io_b, rel=1e-12):
+    """True when particle b is particle a shifted: the same element layout,
+    materials and interpolation, and nodes equal up to one translation (to
+    rel of a's extent)."""
+    if type(a) is not type(b) or a.interp != b.interp or a.n != b.n or list(io_a) != list(io_b):
+        return False
+    if not np.array_equal(a.nvert, b.nvert):
+        return False
+    used = np.ones(a.nodes9.shape[:2], dtype=bool)   # a triangle leaves slots 3, 7, 8 empty
+    used[np.ix_(a.nvert == 3, [3, 7, 8])] = False
+    x, y = a.nodes9[used], b.nodes9[used]
+    d = y - x
+    ext = max(float(np.ptp(x, axis=0).max()), 1e-300)
+    return bool(np.abs(d - d[0]).max() <= rel * ext)
 
 
 def _back_substitute(R, g):

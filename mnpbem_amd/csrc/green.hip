@@ -19,6 +19,7 @@
 // row-major LU of the buffer factors the transpose (the LU solve then yields
 // H G^-1 directly; see mnpbem_getrs_batched).
 #include "common.h"
+#include "green_entry.h"
 
 namespace {
 
@@ -705,62 +706,6 @@ extern "C" int mnpbem_surface_field(int n, int P, int side, double hsign, const 
                      (double*)H, lde, se);
   MNP_LAUNCH_CHECK();
   return 0;
-}
-
-// ---------------------------------------------------------------------------
-// Single Green entries of one side with refinement: G_ij and Gp_ij = grad_i G
-// (deriv='cart'), e^{ikd} included; refined pairs (binary search in row i's
-// column-sorted slice of the pair list) take sum_n (ik)^n {g_n, f_n} before
-// the phase (greenret_refined.py:698-844, MATLAB Greenfun/@greenret/private/
-// eval2.m).  F_ij = n_i . Gp_ij.  The diagonal terms of H and Hp are added by
-// the callers.
-// ---------------------------------------------------------------------------
-struct GreenEntry {
-  cplx<double> G, Gp[3];
-};
-
-__device__ __forceinline__ GreenEntry green_entry(int i, int j, double kr, double ki, const double* __restrict__ pos,
-                                                  const double* __restrict__ area, const long* __restrict__ rowptr,
-                                                  const int* __restrict__ pcol, const double* __restrict__ g,
-                                                  const double* __restrict__ f, int ord1) {
-  GreenEntry e;
-  const double x = pos[3 * i] - pos[3 * j], y = pos[3 * i + 1] - pos[3 * j + 1], z = pos[3 * i + 2] - pos[3 * j + 2];
-  const double d = fmax(sqrt(x * x + y * y + z * z), 2.220446049250313e-16);
-  double s, c;
-  sincos(kr * d, &s, &c);
-  const double ex = exp(-ki * d);
-  const cplx<double> ph = cmake(ex * c, ex * s);
-  const long p1 = rowptr[i + 1];
-  long lo = rowptr[i], hi = p1;
-  while (lo < hi) {
-    const long mid = (lo + hi) >> 1;
-    if (pcol[mid] < j) lo = mid + 1; else hi = mid;
-  }
-  if (lo < p1 && pcol[lo] == j) {
-    double pr = 1.0, pim = 0.0;  // (ik)^q, ik = -ki + i kr
-    cplx<double> sg = cmake(0.0, 0.0), sf[3] = {cmake(0.0, 0.0), cmake(0.0, 0.0), cmake(0.0, 0.0)};
-    for (int q = 0; q < ord1; ++q) {
-      const double gq = g[lo * ord1 + q];
-      sg = cadd(sg, cmake(gq * pr, gq * pim));
-      for (int cc = 0; cc < 3; ++cc) {
-        const double fq = f[(lo * 3 + cc) * ord1 + q];
-        sf[cc] = cadd(sf[cc], cmake(fq * pr, fq * pim));
-      }
-      const double tr = pr * (-ki) - pim * kr;
-      pim = pr * kr + pim * (-ki);
-      pr = tr;
-    }
-    e.G = cmul(sg, ph);
-    for (int cc = 0; cc < 3; ++cc) e.Gp[cc] = cmul(sf[cc], ph);
-  } else {
-    const double id = 1.0 / d, aj = area[j];
-    e.G = cscale(ph, aj * id);
-    const cplx<double> fa = cmul(cmake((-ki - id) * id * id * aj, kr * id * id * aj), ph);
-    e.Gp[0] = cscale(fa, x);
-    e.Gp[1] = cscale(fa, y);
-    e.Gp[2] = cscale(fa, z);
-  }
-  return e;
 }
 
 // Gp (hsign 0), H1p = Gp + 2 pi n_i on the diagonal (hsign +1) or H2p (-1) of

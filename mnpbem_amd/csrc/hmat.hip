@@ -20,6 +20,7 @@
 // low-rank block over it).  Values are complex64 (dtype 1) or complex128
 // (dtype 2); vectors and all arithmetic are complex128.
 #include "common.h"
+#include "green_entry.h"
 
 namespace {
 
@@ -52,6 +53,35 @@ __device__ __forceinline__ cplx<double> ldw(const T* p) {
   const cplx<T> v = cld(p);
   return cmake((double)v.re, (double)v.im);
 }
+
+// Entry sources of the ACA and dense-block kernels: M[i][j] for face indices
+// (i, j).  DenseSrc reads an assembled complex128 matrix with element strides
+// (si, sj); GreenSrc evaluates the refined Green function of one side in
+// place (G for kind 0, F + hsign 2 pi [i == j] for kind 1), so no n x n
+// matrix exists (the approach of mex/acagreen/acagreen.cpp:79-127, which
+// fills ACA rows and columns from positions, normals and areas).
+struct DenseSrc {
+  const double* m;
+  long si, sj;
+  __device__ __forceinline__ cplx<double> operator()(int i, int j) const {
+    const double2 v = *reinterpret_cast<const double2*>(m + 2 * ((long)i * si + (long)j * sj));
+    return cmake(v.x, v.y);
+  }
+};
+struct GreenSrc {
+  int kind;
+  double hsign;
+  const double *pos, *nvec, *area;
+  const int* mat;
+  const double* kmat;
+  const long* rowptr;
+  const int* pcol;
+  const double *g, *f;
+  int ord1;
+  __device__ __forceinline__ cplx<double> operator()(int i, int j) const {
+    return green_value(kind, hsign, i, j, pos, nvec, area, mat, kmat, rowptr, pcol, g, f, ord1);
+  }
+};
 
 // (|v|, index, v) candidate for the pivot searches; ties keep the lower index
 struct Cand {
@@ -99,13 +129,13 @@ __device__ __forceinline__ cplx<double> wave_sum(cplx<double> v) {
 // argmax |row| with used columns read as 0, stop when |pivot| < 1e-15 or
 // ||u_k|| ||v_k|| < htol ||sum_r u_r v_r^T||_F (norm updated with the cross
 // terms 2 Re sum_s (u_s^H u_k)(v_s^H v_k)), next row = argmax |u_k| with
-// used rows read as 0.  Entries come from an assembled complex128 matrix
-// with element strides (si, sj): M[i][j] = src[i si + j sj] (the solver's
-// transposed buffers have si = 1, sj = n).  U, V go to a workspace with
-// per-block offsets woff[2 b], woff[2 b + 1] and column stride m (n).
-__global__ __launch_bounds__(256) void k_aca(const long* blk, const int* bids, const int* perm,
-                                             const double* src, long si, long sj, double* Uw, double* Vw,
-                                             const long* woff, int kmax, double htol, int* rank_out) {
+// used rows read as 0.  Entries M[i][j] come from the source functor (an
+// assembled matrix or the Green function evaluated in place).  U, V go to a
+// workspace with per-block offsets woff[2 b], woff[2 b + 1] and column
+// stride m (n).
+template <class SRC>
+__global__ __launch_bounds__(256) void k_aca(const long* blk, const int* bids, const int* perm, SRC src, double* Uw,
+                                             double* Vw, const long* woff, int kmax, double htol, int* rank_out) {
   const int tid = threadIdx.x;
   const int lb = blockIdx.x;
   const Blk B = load_blk(blk, bids[lb]);
@@ -140,13 +170,13 @@ __global__ __launch_bounds__(256) void k_aca(const long* blk, const int* bids, c
     }
     __syncthreads();
     // residual row p -> V[:, k]
-    const long rowoff = (long)prow[p] * si;
+    const int fi = prow[p];
     Cand c;
     c.a = -1.0;
     c.idx = n;
     c.re = c.im = 0.0;
     for (int j = tid; j < n; j += blockDim.x) {
-      cplx<double> v = ld2(src + 2 * ((long)pcol[j] * sj + rowoff));
+      cplx<double> v = src(fi, pcol[j]);
       for (int r = 0; r < k; ++r) v = cfms(v, cmake(uk[2 * r], uk[2 * r + 1]), ld2(V + 2 * ((long)r * n + j)));
       double* o = V + 2 * ((long)k * n + j);
       o[0] = v.re;
@@ -174,13 +204,13 @@ __global__ __launch_bounds__(256) void k_aca(const long* blk, const int* bids, c
     __syncthreads();
     // residual column q / pivot -> U[:, k]
     const cplx<double> inv = crecip(cmake(pc.re, pc.im));
-    const long coloff = (long)pcol[q] * sj;
+    const int fj = pcol[q];
     Cand cr;
     cr.a = -1.0;
     cr.idx = m;
     cr.re = cr.im = 0.0;
     for (int i = tid; i < m; i += blockDim.x) {
-      cplx<double> v = ld2(src + 2 * (coloff + (long)prow[i] * si));
+      cplx<double> v = src(prow[i], fj);
       for (int r = 0; r < k; ++r) v = cfms(v, ld2(U + 2 * ((long)r * m + i)), cmake(vq[2 * r], vq[2 * r + 1]));
       const cplx<double> u = cmul(v, inv);
       double* o = U + 2 * ((long)k * m + i);
@@ -233,14 +263,14 @@ __global__ __launch_bounds__(256) void k_aca(const long* blk, const int* bids, c
 }
 
 // dense blocks: val[off_a + j m + i] = M[perm[r0+i]][perm[c0+j]]
-template <typename TS>
-__global__ __launch_bounds__(256) void k_hmat_dense(const long* blk, const int* bids, const int* perm,
-                                                    const double* src, long si, long sj, TS* val) {
+template <typename TS, class SRC>
+__global__ __launch_bounds__(256) void k_hmat_dense(const long* blk, const int* bids, const int* perm, SRC src,
+                                                    TS* val) {
   const Blk B = load_blk(blk, bids[blockIdx.x]);
   const long mn = B.m * B.n;
   for (long e = threadIdx.x; e < mn; e += blockDim.x) {
     const long j = e / B.m, i = e - j * B.m;
-    const cplx<double> v = ld2(src + 2 * ((long)perm[B.c0 + j] * sj + (long)perm[B.r0 + i] * si));
+    const cplx<double> v = src(perm[B.r0 + i], perm[B.c0 + j]);
     cst(val + 2 * (B.oa + e), cmake<TS>((TS)v.re, (TS)v.im));
   }
 }
@@ -794,8 +824,9 @@ extern "C" int mnpbem_hmat_aca(int nb, const long* blk, const int* bids, const i
   MNP_REQUIRE(kmax >= 1 && kmax <= ACA_KMAX, "hmat_aca: kmax must be in [1, 512]");
   MNP_REQUIRE(htol >= 0.0, "hmat_aca: htol < 0");
   if (nb <= 0) return 0;
-  hipLaunchKernelGGL(k_aca, dim3(nb), dim3(256), 0, st, blk, bids, perm, (const double*)src, si, sj, (double*)Uw,
-                     (double*)Vw, woff, kmax, htol, rank_out);
+  const DenseSrc s{(const double*)src, si, sj};
+  hipLaunchKernelGGL(k_aca<DenseSrc>, dim3(nb), dim3(256), 0, st, blk, bids, perm, s, (double*)Uw, (double*)Vw,
+                     woff, kmax, htol, rank_out);
   MNP_LAUNCH_CHECK();
   return 0;
 }
@@ -805,14 +836,64 @@ extern "C" int mnpbem_hmat_dense(int dtype, int nb, const long* blk, const int* 
                                  const void* src, long si, long sj, void* val, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   if (nb <= 0) return 0;
+  const DenseSrc s{(const double*)src, si, sj};
   if (dtype == MNP_DTYPE_C128)
-    hipLaunchKernelGGL(k_hmat_dense<double>, dim3(nb), dim3(256), 0, st, blk, bids, perm, (const double*)src, si,
-                       sj, (double*)val);
+    hipLaunchKernelGGL((k_hmat_dense<double, DenseSrc>), dim3(nb), dim3(256), 0, st, blk, bids, perm, s,
+                       (double*)val);
   else if (dtype == MNP_DTYPE_C64)
-    hipLaunchKernelGGL(k_hmat_dense<float>, dim3(nb), dim3(256), 0, st, blk, bids, perm, (const double*)src, si,
-                       sj, (float*)val);
+    hipLaunchKernelGGL((k_hmat_dense<float, DenseSrc>), dim3(nb), dim3(256), 0, st, blk, bids, perm, s,
+                       (float*)val);
   else {
     mnp_set_error("hmat_dense: dtype must be 1 or 2");
+    return 1;
+  }
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+// The two fills above with entries generated in place from the geometry and
+// the refinement tables of one side, never from an assembled matrix: kind 0
+// G, kind 1 F + hsign 2 pi [i == j] (H1 with hsign +1, H2 with -1).  kmat:
+// the nmat complex wavenumbers of one wavelength (float64 pairs); mat: the
+// side's material index per face (pairs of different materials are 0);
+// rowptr (int64, n + 1), pcol, g, f: the (row, col)-sorted refined pairs and
+// their order + 1 expansion coefficients, as for mnpbem_green_select.
+extern "C" int mnpbem_hmat_aca_green(int nb, const long* blk, const int* bids, const int* perm, int kind,
+                                     double hsign, const double* pos, const double* nvec, const double* area,
+                                     const int* mat, const double* kmat, int order, const long* rowptr,
+                                     const int* pcol, const double* g, const double* f, void* Uw, void* Vw,
+                                     const long* woff, int kmax, double htol, int* rank_out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(kmax >= 1 && kmax <= ACA_KMAX, "hmat_aca_green: kmax must be in [1, 512]");
+  MNP_REQUIRE(htol >= 0.0, "hmat_aca_green: htol < 0");
+  MNP_REQUIRE(kind == 0 || kind == 1, "hmat_aca_green: kind must be 0 (G) or 1 (F / H)");
+  MNP_REQUIRE(order >= 0, "hmat_aca_green: order < 0");
+  if (nb <= 0) return 0;
+  const GreenSrc s{kind, hsign, pos, nvec, area, mat, kmat, rowptr, pcol, g, f, order + 1};
+  hipLaunchKernelGGL(k_aca<GreenSrc>, dim3(nb), dim3(256), 0, st, blk, bids, perm, s, (double*)Uw, (double*)Vw,
+                     woff, kmax, htol, rank_out);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int mnpbem_hmat_dense_green(int dtype, int nb, const long* blk, const int* bids, const int* perm,
+                                       int kind, double hsign, const double* pos, const double* nvec,
+                                       const double* area, const int* mat, const double* kmat, int order,
+                                       const long* rowptr, const int* pcol, const double* g, const double* f,
+                                       void* val, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(kind == 0 || kind == 1, "hmat_dense_green: kind must be 0 (G) or 1 (F / H)");
+  MNP_REQUIRE(order >= 0, "hmat_dense_green: order < 0");
+  if (nb <= 0) return 0;
+  const GreenSrc s{kind, hsign, pos, nvec, area, mat, kmat, rowptr, pcol, g, f, order + 1};
+  if (dtype == MNP_DTYPE_C128)
+    hipLaunchKernelGGL((k_hmat_dense<double, GreenSrc>), dim3(nb), dim3(256), 0, st, blk, bids, perm, s,
+                       (double*)val);
+  else if (dtype == MNP_DTYPE_C64)
+    hipLaunchKernelGGL((k_hmat_dense<float, GreenSrc>), dim3(nb), dim3(256), 0, st, blk, bids, perm, s,
+                       (float*)val);
+  else {
+    mnp_set_error("hmat_dense_green: dtype must be 1 or 2");
     return 1;
   }
   MNP_LAUNCH_CHECK();

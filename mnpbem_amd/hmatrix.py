@@ -260,6 +260,50 @@ class HMatrix:
         if tuple(src.shape) != (n, n) or src.device.type != dev.type:
             raise ValueError("HMatrix.aca: expected an (n, n) matrix on %s" % dev)
         si, sj = src.stride()
+
+        def dense(code, nb, dblk, bids, val, st):
+            nat.call("mnpbem_hmat_dense", code, nb, dblk, bids, nat.ptr(self._perm), nat.ptr(src), si, sj, val, st)
+
+        def aca(nb, dblk, bids, Uw, Vw, woff, rk, st):
+            nat.call("mnpbem_hmat_aca", nb, dblk, bids, nat.ptr(self._perm), nat.ptr(src), si, sj, Uw, Vw, woff,
+                     self.kmax, float(self.htol), rk, st)
+
+        return self._fill(dense, aca)
+
+    def aca_green(self, green, side, key, kmat):
+        """Fill the blocks with entries generated in place from the geometry
+        and refinement tables of CompGreenRet ``green`` (side 0 inside, 1
+        outside; key 'G', 'F', 'H1' or 'H2'; kmat: the (nmat, 2) float64
+        device wavenumbers of one wavelength, a row of green.wavenumbers):
+        the dense leaf blocks and the ACA rows and columns of the admissible
+        blocks are computed by the kernels, so no (n, n) matrix is assembled
+        (the entry functions of mex/acagreen/acagreen.cpp:79-127, with the
+        refined near-field entries of compgreen_ret.py:507-553)."""
+        kinds = {"G": (0, 0.0), "F": (1, 0.0), "H1": (1, 1.0), "H2": (1, -1.0)}
+        if key not in kinds:
+            raise ValueError("HMatrix.aca_green: key must be one of %s" % sorted(kinds))
+        if green.n != self.tree.n:
+            raise ValueError("HMatrix.aca_green: the tree and the Green function have different face counts")
+        kind, hs = kinds[key]
+        kmat = kmat.contiguous()
+        geo = (nat.ptr(green.pos), nat.ptr(green.nvec), nat.ptr(green.area), nat.ptr(green.mat[side]),
+               nat.ptr(kmat), green.order, nat.ptr(green._rowptr()), nat.ptr(green.pcol), nat.ptr(green.g),
+               nat.ptr(green.f))
+
+        def dense(code, nb, dblk, bids, val, st):
+            nat.call("mnpbem_hmat_dense_green", code, nb, dblk, bids, nat.ptr(self._perm), kind, hs, *geo, val, st)
+
+        def aca(nb, dblk, bids, Uw, Vw, woff, rk, st):
+            nat.call("mnpbem_hmat_aca_green", nb, dblk, bids, nat.ptr(self._perm), kind, hs, *geo, Uw, Vw, woff,
+                     self.kmax, float(self.htol), rk, st)
+
+        return self._fill(dense, aca)
+
+    def _fill(self, dense, aca):
+        """Dense leaf blocks through ``dense`` and ACA of the admissible
+        blocks through ``aca`` (launchers of one entry source), then the
+        packed factors."""
+        dev = self.device
         self._tables()
         blk = self._blk_host
         nd, nl = self._nd, self._nl
@@ -269,8 +313,7 @@ class HMatrix:
         if nd:
             dblk = torch.as_tensor(blk).to(dev)
             bids = torch.arange(nd, dtype=torch.int32, device=dev)
-            nat.call("mnpbem_hmat_dense", code, nd, nat.ptr(dblk), nat.ptr(bids), nat.ptr(self._perm),
-                     nat.ptr(src), si, sj, nat.ptr(self._val), st)
+            dense(code, nd, nat.ptr(dblk), nat.ptr(bids), nat.ptr(self._val), st)
         ranks = np.zeros(nl, dtype=np.int64)
         if nl:
             if self.kmax > 512:
@@ -287,8 +330,7 @@ class HMatrix:
             rk = torch.empty((nl,), dtype=torch.int32, device=dev)
             dblk = torch.as_tensor(blk).to(dev)
             bids = torch.arange(nd, nd + nl, dtype=torch.int32, device=dev)
-            nat.call("mnpbem_hmat_aca", nl, nat.ptr(dblk), nat.ptr(bids), nat.ptr(self._perm), nat.ptr(src), si, sj,
-                     nat.ptr(Uw), nat.ptr(Vw), nat.ptr(woff), self.kmax, float(self.htol), nat.ptr(rk), st)
+            aca(nl, nat.ptr(dblk), nat.ptr(bids), nat.ptr(Uw), nat.ptr(Vw), nat.ptr(woff), nat.ptr(rk), st)
             ranks = rk.cpu().numpy().astype(np.int64)
             if np.any(ranks < 0):
                 raise RuntimeError("HMatrix.aca: block outside the kernel's capacity")
@@ -747,8 +789,11 @@ class HMatrix:
 class ACACompGreenRet:
     """Green functions of a ComParticle compressed to H-matrices
     (aca_compgreen_ret.py:96-208).  ``eval(i, j, key, enei)`` returns an
-    HMatrix of the dense CompGreenRet.eval matrix (or the int 0 for regions
-    that no particle connects); the default admissibility is the
+    HMatrix of the CompGreenRet.eval matrix (or the int 0 for regions that no
+    particle connects) whose entries the kernels generate in place, dense
+    leaf blocks and ACA rows and columns alike, so no (n, n) matrix is
+    assembled (mex/acagreen/acagreen.cpp:79-127); ``compress`` takes an
+    assembled matrix instead.  The default admissibility is the
     wavenumber-aware rule of make_kaware_fadmiss(2 pi / enei, eta)."""
 
     def __init__(self, p, htol=1e-6, kmax=100, cleaf=32, fadmiss=None, eta=2.5, dtype="complex128", green=None,
@@ -779,14 +824,25 @@ class ACACompGreenRet:
                     device=self.g.device)
         return h.aca(M)
 
+    def generate(self, side, key, kmat, enei):
+        """HMatrix of side ``side``'s key ('G', 'F', 'H1', 'H2') at wavelength
+        enei with entries generated in place; kmat: that wavelength's (nmat,
+        2) row of CompGreenRet.wavenumbers."""
+        h = HMatrix(self.tree, htol=self.htol, kmax=self.kmax, fadmiss=self.fadmiss_for(enei), dtype=self.dtype,
+                    device=self.g.device)
+        return h.aca_green(self.g, side, key, kmat)
+
     def eval(self, i, j, key, enei):
+        if key not in ("G", "F", "H1", "H2"):
+            raise ValueError("ACACompGreenRet.eval: key must be G, F, H1 or H2")
         ck = (i, j, key, float(enei))
         if ck in self._cache:
             return self._cache[ck]
-        M = self.g.eval(i, j, key, enei)
-        if isinstance(M, int):
+        if not np.any(self.g.con[i][j]):
             return 0
-        h = self.compress(M, enei)
+        if i != j:
+            raise NotImplementedError("ACACompGreenRet.eval: cross-region blocks (nested particles)")
+        h = self.generate(i, key, self.g.wavenumbers([enei])[0], enei)
         self._cache[ck] = h
         return h
 

@@ -192,13 +192,28 @@ def test_dimer25k_hmatrix_gmres_against_direct(cuda):
     assert cp.n == 25000
     pw = m.PlaneWaveRet([[1.0, 0.0, 0.0]], [[0.0, 0.0, 1.0]])
     lam = np.array([600.0])
+    _free()
+    torch.cuda.reset_peak_memory_stats()
     bi = m.BEMRetIter(cp, hmatrix=True, htol=1e-6, kmax=30, cleaf=64, tol=1e-6, restart=50, maxit=20,
                       precision="mixed", hmat_dtype="complex64")
+    # several particles: the default preconditioner is the per-particle
+    # direct solve, shared by the two translated spheres
+    assert len(bi._blocks) == 2 and len(bi._block_bems) == 1
     print("dimer 25000: H-matrix GMRES at 600 nm", flush=True)
     e_it = bi.solve_sweep(lam, pw, chunk=1)
     flag, rel, it = bi.info()
-    print("GMRES flag %s relres %s it %s" % (flag, rel, it), flush=True)
+    peak = torch.cuda.max_memory_allocated()
+    seg = max(sg["total_size"] for sg in torch.cuda.memory_snapshot())
+    print("GMRES flag %s relres %s it %s; peak device memory %.2f GB, largest segment %.2f GB"
+          % (flag, rel, it, peak / 1e9, seg / 1e9), flush=True)
     assert all(f == 0 for f in flag) and np.max(rel) <= 1e-6
+    # the H-matrix entries are generated in place and the preconditioner
+    # factors 12500-face blocks: no allocation reaches one 25000 x 25000
+    # complex64 matrix (5 GB; the four complex128 Green matrices the ACA
+    # used to read took 40 GB), and the whole solve stays near one
+    # complex128 matrix's 10 GB
+    assert seg < 8 * cp.n * cp.n, seg
+    assert peak < 1.25 * 16 * cp.n * cp.n, peak
     st = [h.stat() for h in bi.last_hmats[0]]
     assert all(s["rank_max"] <= 30 for s in st)
     assert all(s["compression"] < 0.3 for s in st)

@@ -225,3 +225,37 @@ def test_iter_block_preconditioner_dimer(cuda, precision, hmatrix):
     assert all(f == 0 for f in flags)
     assert max(int(i[1]) for i in its) > 1
     assert np.max(np.abs(e_b / e_d - 1)) < (1e-8 if not hmatrix else 1e-6)
+
+
+@pytest.mark.parametrize("dimer", [False, True])
+def test_aca_generated_in_place_matches_assembled(sphere284, dimer):
+    """ACACompGreenRet.eval generates every entry in the kernels (dense leaf
+    blocks and ACA rows / columns, no n x n matrix): its dense blocks equal
+    the assembled matrix's, the ranks agree within one with the ACA of the
+    assembled matrix, and both reproduce the matrix to htol (G, H1 inside;
+    G, H2 outside; a one-particle sphere and a 1-nm-gap dimer whose inside
+    Green function couples the two particles)."""
+    import mnpbem_amd as m
+    from mnpbem_amd.hmatrix import ACACompGreenRet, HMatrix
+    if dimer:
+        p1 = m.trisphere(144, 10.0)
+        cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [p1, p1.shift([11.0, 0.0, 0.0])],
+                           [[2, 1], [2, 1]], 1, 2)
+    else:
+        cp = sphere284[0]
+    lam = 580.0
+    htol = 1e-8
+    aca = ACACompGreenRet(cp, htol=htol, kmax=60, cleaf=16)
+    for side, key in [(0, "G"), (0, "H1"), (1, "G"), (1, "H2")]:
+        print("in-place ACA", "dimer" if dimer else "sphere", side, key, flush=True)
+        h = aca.eval(side, side, key, lam)
+        M = aca.g.eval(side, side, key, lam)
+        ha = HMatrix(aca.tree, htol=htol, kmax=60, fadmiss=aca.fadmiss_for(lam)).aca(M)
+        Mh = M.cpu().numpy()
+        assert np.array_equal(h._blk_host[:h._nd], ha._blk_host[:ha._nd])
+        assert _rel(h._val, ha._val.cpu().numpy()) < 1e-14, (side, key)
+        assert np.max(np.abs(h.ranks - ha.ranks)) <= 1
+        for hm in (h, ha):
+            assert np.linalg.norm(hm.full(np) - Mh) / np.linalg.norm(Mh) < 10 * htol, (side, key)
+    # inside and outside share no material here: no particle connects them
+    assert aca.eval(0, 1, "G", lam) == 0

@@ -201,3 +201,20 @@ def test_trirod_quadrilaterals_match_reference():
         assert np.abs(val - d[key]).max() < 1e-13, key
     # a closed surface: the curved area approaches the rod's 2 pi r (h - d) + pi d^2
     assert abs(rod.area.sum() / (2 * np.pi * 5 * 30 + np.pi * 100) - 1) < 1e-4
+
+
+def test_translated_copy_detection():
+    """The iterative solver's block preconditioner shares one factorisation
+    between particles that are shifted copies of each other: shifted spheres
+    and rods qualify, a scaled or rotated copy or other materials do not."""
+    import mnpbem_amd as m
+    from mnpbem_amd.bem_iter import _translated_copy
+    p1 = m.trisphere_fibonacci(500, 100.0)
+    assert _translated_copy(p1, [2, 1], p1.shift([102.0, 0.0, 0.0]), [2, 1])
+    assert not _translated_copy(p1, [2, 1], p1.shift([102.0, 0.0, 0.0]), [3, 1])
+    assert not _translated_copy(p1, [2, 1], m.trisphere_fibonacci(500, 90.0), [2, 1])
+    rot = m.Particle(p1.verts2[:, [1, 0, 2]], p1.faces6)
+    assert not _translated_copy(p1, [2, 1], rot, [2, 1])
+    rod = m.trirod(10.0, 40.0, n=[15, 15, 15])
+    assert _translated_copy(rod, [2, 1], rod.shift([0.0, 50.0, 0.0]), [2, 1])
+    assert not _translated_copy(rod, [2, 1], p1, [2, 1])
