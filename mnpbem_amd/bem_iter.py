@@ -223,7 +223,11 @@ class BEMRetIter(BEMIter):
 
     def _precond_blocks(self, S, V, Z, ws, P):
         """Z = blockdiag(A_kk)^-1 V: each particle's direct solve on its rows
-        of the packed vectors (row stride 8P, wavelength stride n 8P)."""
+        of the packed vectors (row stride 8P, wavelength stride n 8P).  The
+        particles sharing one factorisation at equally spaced offsets (the
+        translated copies) are solved as one batch whose systems read the
+        same factors (factor strides 0) and whose vectors are the copies'
+        row ranges."""
         n, B = S.n, S.B
         st = nat.stream_handle(self.device)
         c128 = S.Fk[0].dtype == nat.DTYPE_C128
@@ -232,17 +236,41 @@ class BEMRetIter(BEMIter):
         else:
             R, X, W = ws["R32"], ws["X32"], ws["W32"]
             nat.call("mnpbem_convert", nat.DTYPE_C64, 2 * V.numel(), nat.ptr(V), nat.ptr(R), st)
-        for off, u in self._blocks:
+        for u, offs, step, par in self._block_groups(S):
             F, bem = S.Fk[u], self._block_bems[u]
             nn = F.n * F.n
-            o = off * 8 * P
-            nat.call("mnpbem_bem_solve_batched", F.dtype, F.n, P, nat.ptr(F.GG[0]), nat.ptr(F.piv[0]),
-                     nat.ptr(F.GG[1]), nat.ptr(F.piv[1]), nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]),
-                     nat.ptr(F.piv[3]), F.n, nn, F.n, nat.ptr(F.DG[0]), nat.ptr(F.DG[1]), nat.ptr(F.DS), F.sD,
-                     nat.ptr_off(R, o), nat.ptr_off(W, o), nat.ptr_off(X, o), 8 * P, n * 8 * P,
-                     nat.ptr(bem.g.nvec), nat.ptr(F.par), 0, B, st)
+            cnt = len(offs)
+            for b in range(B):
+                o = (b * n + offs[0]) * 8 * P
+                fo = lambda t, stride: nat.ptr_off(t, b * stride)
+                nat.call("mnpbem_bem_solve_batched", F.dtype, F.n, P, fo(F.GG[0], nn), fo(F.piv[0], F.n),
+                         fo(F.GG[1], nn), fo(F.piv[1], F.n), fo(F.HH[0], nn), fo(F.Et, nn), fo(F.HH[1], nn),
+                         fo(F.piv[3], F.n), F.n, 0, 0, fo(F.DG[0], F.sD), fo(F.DG[1], F.sD), fo(F.DS, F.sD), 0,
+                         nat.ptr_off(R, o), nat.ptr_off(W, o), nat.ptr_off(X, o), 8 * P, step * 8 * P,
+                         nat.ptr(bem.g.nvec), nat.ptr_off(par, 5 * cnt * b), 0, cnt, st)
         if not c128:
             nat.call("mnpbem_widen", nat.DTYPE_C64, 2 * V.numel(), nat.ptr(X), nat.ptr(Z), st)
+
+    def _block_groups(self, S):
+        """(factor index, row offsets, offset step, per-system parameters) of
+        each batch of block solves; cached on the wavelength batch."""
+        groups = getattr(S, "block_groups", None)
+        if groups is not None:
+            return groups
+        groups = []
+        for u in range(len(self._block_bems)):
+            offs = [off for off, uu in self._blocks if uu == u]
+            steps = np.diff(offs)
+            if len(offs) > 1 and np.all(steps == steps[0]) and steps[0] >= self._block_bems[u].p.n:
+                runs = [offs]
+            else:
+                runs = [[off] for off in offs]
+            for r in runs:
+                step = int(r[1] - r[0]) if len(r) > 1 else 0
+                par = S.Fk[u].par.reshape(S.B, 1, 5).expand(S.B, len(r), 5).contiguous()
+                groups.append((u, r, step, par))
+        S.block_groups = groups
+        return groups
 
     def _precond(self, S, V, Z, ws, P):
         """Z = M^-1 V with the direct BEMRet solve (or Z = V without one)."""
