@@ -77,6 +77,20 @@ def blas_threads():
     return len(os.sched_getaffinity(0))
 
 
+def copy_probe(mib):
+    """One plain device copy of known size through the library's copy kernel
+    (k_copy_cast<double>), the calibration of the HBM byte counters."""
+    import torch
+    from mnpbem_amd import _native as nat
+    dev = torch.device("cuda:0")
+    cnt = mib * (1 << 20) // 8
+    src = torch.ones(cnt, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    nat.call("mnpbem_convert", nat.DTYPE_C128, cnt, nat.ptr(src), nat.ptr(dst), nat.stream_handle(dev))
+    torch.cuda.synchronize()
+    log("copy probe: %d bytes read, %d bytes written by k_copy_cast<double>" % (cnt * 8, cnt * 8))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +132,8 @@ def main():
                          "hipGraph of the 64-wavelength chunk step, eager steps timed beside it)")
     ap.add_argument("--graph-iters", type=int, default=3, help="refinement steps inside the captured step")
     args = ap.parse_args()
+    if args.gpus == 1 and args.copy_probe > 0:
+        copy_probe(args.copy_probe)
     if args.gpus == 1 and args.workload == "uneven":
         return bench_uneven(args)
     if args.gpus == 1 and args.workload == "dimer":
@@ -135,14 +151,6 @@ def main():
     from mnpbem_amd import _native as nat
 
     dev = torch.device("cuda:0")
-    if args.copy_probe > 0:
-        cnt = args.copy_probe * (1 << 20) // 8
-        src = torch.ones(cnt, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        nat.call("mnpbem_convert", nat.DTYPE_C128, cnt, nat.ptr(src), nat.ptr(dst), nat.stream_handle(dev))
-        torch.cuda.synchronize()
-        log("copy probe: %d bytes read, %d bytes written by k_copy_cast<double>" % (cnt * 8, cnt * 8))
-        del src, dst
     lams = np.linspace(400.0, 800.0, 64)
     pol = [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]]
     dr = [[0.0, 0.0, 1.0], [0.0, 0.0, 1.0]]

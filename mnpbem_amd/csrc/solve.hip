@@ -91,6 +91,80 @@ __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restric
   }
 }
 
+// The same product for few systems: 64 output rows per workgroup and the
+// buffer rows j split over the four waves (four times the workgroups of
+// k_gemv_t), each wave staging its own X rows in LDS; the wave partial sums
+// are added in wave order (deterministic).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_t_split(int n, int C, const T* __restrict__ M, long ldm, long sM,
+                                                      const T* __restrict__ X, long ldx, long sX, T* __restrict__ Y,
+                                                      long ldy, long sY, double alr, double ali, double beta) {
+  const long b = blockIdx.y;
+  M += 2 * b * sM;
+  X += 2 * b * sX;
+  Y += 2 * b * sY;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const int q = (((n + 3) / 4) + 31) & ~31;  // rows per wave, a multiple of the 32-row chunk
+  const int jb = wv * q, je = min(n, jb + q);
+  __shared__ cplx<T> xs[4][32][MAXC];
+  __shared__ cplx<T> part[3][64][MAXC];
+  cplx<T> acc[MAXC];
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
+  for (int t = 0; t < q; t += 32) {
+    const int j0 = jb + t;
+    const int nj = max(0, min(32, je - j0));
+    __syncthreads();
+    for (int e = lane; e < nj * C; e += 64) {
+      const int jj = e / C, c = e % C;
+      xs[wv][jj][c] = cld(X + 2 * ((long)(j0 + jj) * ldx + c));
+    }
+    __syncthreads();
+    if (i < n) {
+      int jj = 0;
+      for (; jj + 16 <= nj; jj += 16) {
+        cplx<T> m[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) m[u] = cld(M + 2 * ((long)(j0 + jj + u) * ldm + i));
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+#pragma unroll
+          for (int c = 0; c < MAXC; ++c)
+            if (c < C) acc[c] = cfma(acc[c], m[u], xs[wv][jj + u][c]);
+      }
+      for (; jj < nj; ++jj) {
+        const cplx<T> m = cld(M + 2 * ((long)(j0 + jj) * ldm + i));
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < C) acc[c] = cfma(acc[c], m, xs[wv][jj][c]);
+      }
+    }
+  }
+  if (wv > 0)
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) part[wv - 1][lane][c] = acc[c];
+  __syncthreads();
+  if (wv > 0 || i >= n) return;
+#pragma unroll
+  for (int w = 0; w < 3; ++w)
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) acc[c] = cadd(acc[c], part[w][lane][c]);
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c)
+    if (c < C) {
+      T* py = Y + 2 * ((long)i * ldy + c);
+      const double sr = (double)acc[c].re, si = (double)acc[c].im;
+      double yr = alr * sr - ali * si, yi = alr * si + ali * sr;
+      if (beta != 0.0) {
+        const cplx<T> y0 = cld(py);
+        yr += beta * (double)y0.re;
+        yi += beta * (double)y0.im;
+      }
+      cst(py, cmake<T>((T)yr, (T)yi));
+    }
+}
+
 // The four products of the complex128 residual, Y = [G1^T.. buffers applied:
 // G1 X1 | H1 X1 | G2 X2 | H2 X2], in one launch (blockIdx.z = product), with
 // accumulators for exactly the CC = 4P columns.  One product per launch with
@@ -290,6 +364,141 @@ __global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const
         cst(pu, csub(cld(pu), acc[c]));
       }
   }
+}
+
+// Window step of the same solve with the inverse diagonal blocks: the
+// window's rows [w0, w1) (up to TRSV_WIN rows, 64-row blocks) are solved
+// block by block (block solution = inverse block^T * rhs, then the window's
+// own off-diagonal block applied to the blocks still to solve, all in LDS)
+// by every workgroup, so no grid-wide synchronisation is needed; then each
+// workgroup updates its tile of the rows outside the window (FWD rows >= w1,
+// BWD rows < w0) with all the window's rows.  Fewer launches than one
+// 64-row step each, for the same bytes of the triangle.
+//   TILE 256: one row per thread, window rows summed in order;
+//   TILE 64: the window rows split over the four waves (four times the
+//     workgroups, for few systems), wave partial sums added in wave order.
+// CM >= C: right-hand-side columns held per row.  Deterministic.
+constexpr int TRSV_WIN = 256;
+template <typename T, bool FWD, int CM, int TILE>
+__global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1, const T* __restrict__ LU, long ld,
+                                                    long sLU, T* V, long ldv, long sV, T* Z, long ldz, long sZ,
+                                                    const T* __restrict__ Dinv, long sD, int half, long hoff) {
+  const long b = blockIdx.y;
+  LU += 2 * b * sLU;
+  Dinv += 2 * b * sD;
+  V += 2 * ((b % half) * sV + (b / half) * hoff);
+  Z += 2 * ((b % half) * sZ + (b / half) * hoff);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int W = w1 - w0;
+  __shared__ cplx<T> rs[TRSV_WIN][CM];  // window right-hand side, then its solution
+  __shared__ cplx<T> part[TILE == 64 ? 3 : 1][64][CM];
+  {
+    const T* src = FWD ? V : Z;
+    const long lds = FWD ? ldv : ldz;
+    for (int e = tid; e < TRSV_WIN * CM; e += 256) {
+      const int r = e / CM, c = e % CM;
+      rs[r][c] = (r < W && c < C) ? cld(src + 2 * ((long)(w0 + r) * lds + c)) : cmake<T>(0, 0);
+    }
+  }
+  const int nbk = (W + 63) / 64;
+  for (int s = 0; s < nbk; ++s) {
+    const int r = FWD ? s : nbk - 1 - s;
+    const int i0 = w0 + 64 * r, nb = min(64, w1 - i0);
+    const T* D = Dinv + 2 * (((long)(i0 / 64) * 2 + (FWD ? 0 : 1)) * 64 * 64);
+    __syncthreads();  // rhs of block r complete
+    // row lane of the block, columns wv, wv + 4, ...
+    cplx<T> sl[CM / 4];
+#pragma unroll
+    for (int v = 0; v < CM / 4; ++v) sl[v] = cmake<T>(0, 0);
+    for (int q0 = 0; q0 < nb; q0 += 16) {
+      cplx<T> d[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        d[u] = (q0 + u < nb && lane < nb) ? cld(D + 2 * ((long)(q0 + u) * 64 + lane)) : cmake<T>(0, 0);
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+#pragma unroll
+        for (int v = 0; v < CM / 4; ++v) sl[v] = cfma(sl[v], d[u], rs[64 * r + q0 + u][wv + 4 * v]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < CM / 4; ++v) rs[64 * r + lane][wv + 4 * v] = sl[v];
+    __syncthreads();
+    // the window's blocks still to solve: FWD rows [i0 + 64, w1), BWD rows [w0, i0)
+    const int k0 = FWD ? i0 + 64 : w0;
+    const int cnt = FWD ? w1 - k0 : i0 - w0;
+    if (tid < cnt) {
+      const int k = k0 + tid;
+      cplx<T> acc[CM];
+#pragma unroll
+      for (int c = 0; c < CM; ++c) acc[c] = cmake<T>(0, 0);
+      for (int l0 = 0; l0 < nb; l0 += 16) {
+        cplx<T> m[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          m[u] = l0 + u < nb ? cld(LU + 2 * ((long)(i0 + l0 + u) * ld + k)) : cmake<T>(0, 0);
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+#pragma unroll
+          for (int c = 0; c < CM; ++c) acc[c] = cfma(acc[c], m[u], rs[64 * r + l0 + u][c]);
+      }
+#pragma unroll
+      for (int c = 0; c < CM; ++c) rs[k - w0][c] = csub(rs[k - w0][c], acc[c]);
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    T* dst = FWD ? Z : V;
+    const long ldd_ = FWD ? ldz : ldv;
+    for (int e = tid; e < W * C; e += 256) {
+      const int rr = e / C, c = e % C;
+      cst(dst + 2 * ((long)(w0 + rr) * ldd_ + c), rs[rr][c]);
+    }
+  }
+  // tile update with the whole window
+  T* upd = FWD ? V : Z;
+  const long ldu = FWD ? ldv : ldz;
+  const int kend = FWD ? n : w0;
+  cplx<T> acc[CM];
+#pragma unroll
+  for (int c = 0; c < CM; ++c) acc[c] = cmake<T>(0, 0);
+  const int k = (FWD ? w1 : 0) + blockIdx.x * TILE + (TILE == 64 ? lane : tid);
+  const int la = TILE == 64 ? 64 * wv : 0, lb = TILE == 64 ? min(W, 64 * wv + 64) : W;
+  if (k < kend) {
+    int l = la;
+    for (; l + 16 <= lb; l += 16) {
+      cplx<T> m[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) m[u] = cld(LU + 2 * ((long)(w0 + l + u) * ld + k));
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+#pragma unroll
+        for (int c = 0; c < CM; ++c) acc[c] = cfma(acc[c], m[u], rs[l + u][c]);
+    }
+    for (; l < lb; ++l) {
+      const cplx<T> m = cld(LU + 2 * ((long)(w0 + l) * ld + k));
+#pragma unroll
+      for (int c = 0; c < CM; ++c) acc[c] = cfma(acc[c], m, rs[l][c]);
+    }
+  }
+  if (TILE == 64) {
+    if (wv > 0)
+#pragma unroll
+      for (int c = 0; c < CM; ++c) part[wv - 1][lane][c] = acc[c];
+    __syncthreads();
+    if (wv > 0) return;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int c = 0; c < CM; ++c) acc[c] = cadd(acc[c], part[q][lane][c]);
+  }
+  if (k < kend)
+#pragma unroll
+    for (int c = 0; c < CM; ++c)
+      if (c < C) {
+        T* pu = upd + 2 * ((long)k * ldu + c);
+        cst(pu, csub(cld(pu), acc[c]));
+      }
 }
 
 // Inverses of the 64 x 64 diagonal blocks of the LU factors (U_II^-1 upper,
@@ -847,13 +1056,59 @@ __global__ void k_sca_reduce(int ndir, int P, int total, const double* w, const 
 template <typename T>
 int gemv_t(int n, int C, const T* M, long ldm, long sM, const T* X, long ldx, long sX, T* Y, long ldy, long sY,
            double alr, double ali, double beta, int batch, hipStream_t st) {
+  // few systems: split the rows of the buffer over the waves
+  const bool split = (long)mnp_ceil_div(n, 256) * batch < 512;
   for (int c0 = 0; c0 < C; c0 += MAXC) {
     const int cc = std::min(MAXC, C - c0);
-    hipLaunchKernelGGL(k_gemv_t<T>, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, cc, M, ldm, sM,
-                       X + 2 * c0, ldx, sX, Y + 2 * c0, ldy, sY, alr, ali, beta);
+    if (split)
+      hipLaunchKernelGGL(k_gemv_t_split<T>, dim3(mnp_ceil_div(n, 64), batch), dim3(256), 0, st, n, cc, M, ldm, sM,
+                         X + 2 * c0, ldx, sX, Y + 2 * c0, ldy, sY, alr, ali, beta);
+    else
+      hipLaunchKernelGGL(k_gemv_t<T>, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, cc, M, ldm, sM,
+                         X + 2 * c0, ldx, sX, Y + 2 * c0, ldy, sY, alr, ali, beta);
     MNP_LAUNCH_CHECK();
   }
   return 0;
+}
+
+// Both triangular solves of lu_solve_t by window steps (TRSV_WIN rows per
+// launch; tiles of 64 rows with the window split over the waves when the
+// 256-row tiles would give fewer than 512 workgroups).
+template <typename T, int CM, int TILE>
+int trsv_t_windows_cm(int n, int C, const T* LU, long ld, long sLU, T* V, long ldv, long sV, T* Z, long ldz,
+                      long sZ, const T* Dinv, long sD, int half, long hoff, int batch, hipStream_t st) {
+  const int nblk = (n + 63) / 64, R = TRSV_WIN / 64;
+  for (int b0 = 0; b0 < nblk; b0 += R) {
+    const int w0 = 64 * b0, w1 = std::min(n, 64 * (b0 + R));
+    dim3 grid(std::max(1, mnp_ceil_div(n - w1, TILE)), batch);
+    hipLaunchKernelGGL((k_trsv_t_win<T, true, CM, TILE>), grid, dim3(256), 0, st, n, C, w0, w1, LU, ld, sLU, V, ldv,
+                       sV, Z, ldz, sZ, Dinv, sD, half, hoff);
+    MNP_LAUNCH_CHECK();
+  }
+  for (int b1 = nblk; b1 > 0; b1 -= R) {
+    const int b0 = std::max(0, b1 - R);
+    const int w0 = 64 * b0, w1 = std::min(n, 64 * b1);
+    dim3 grid(std::max(1, mnp_ceil_div(w0, TILE)), batch);
+    hipLaunchKernelGGL((k_trsv_t_win<T, false, CM, TILE>), grid, dim3(256), 0, st, n, C, w0, w1, LU, ld, sLU, V,
+                       ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
+    MNP_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+template <typename T>
+int trsv_t_windows(int n, int C, const T* LU, long ld, long sLU, T* V, long ldv, long sV, T* Z, long ldz, long sZ,
+                   const T* Dinv, long sD, int half, long hoff, int batch, hipStream_t st) {
+  const bool wide = (long)mnp_ceil_div(n, 256) * batch >= 512;
+#define MNP_WIN(CM)                                                                                             \
+  return wide ? trsv_t_windows_cm<T, CM, 256>(n, C, LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff, \
+                                              batch, st)                                                     \
+              : trsv_t_windows_cm<T, CM, 64>(n, C, LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff,  \
+                                             batch, st);
+  if (C <= 4) MNP_WIN(4)
+  if (C <= 8) MNP_WIN(8)
+  MNP_WIN(16)
+#undef MNP_WIN
 }
 
 // V <- M^-1 V (n x C) with the LU of M^T; Z (n x C) is work space.  Systems
@@ -868,6 +1123,13 @@ int lu_solve_t(int n, int C, const T* LU, long ld, long sLU, const int* ipiv, lo
     const int cc = std::min(MAXC, C - c0);
     T* v = V + 2 * c0;
     T* z = Z + 2 * c0;
+    if (Dinv) {
+      int rc = trsv_t_windows<T>(n, cc, LU, ld, sLU, v, ldv, sV, z, ldz, sZ, Dinv, sD, half, hoff, batch, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_perm_t<T>, dim3(batch), dim3(256), 0, st, n, cc, ipiv, sP, v, ldv, sV, half, hoff);
+      MNP_LAUNCH_CHECK();
+      continue;
+    }
     for (int ib = 0; ib < nblk; ++ib) {
       const int i0 = ib * 64, i1 = std::min(n, i0 + 64);
       dim3 grid(std::max(1, mnp_ceil_div(n - i1, 256)), batch);
