@@ -78,6 +78,20 @@ int mnpbem_gemm_tuning_seen(int* shapes, int ncells, int clear);
 int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA, int* ipiv, long spiv, int* info, int batch,
                          void* stream);
 
+/* The same factorisation without row interchanges (ipiv = identity), for
+ * matrices whose partial pivoting makes none; flag[b] = 1 when it would have
+ * made one (some multiplier has |re|+|im| > 1), and then the factors differ
+ * from mnpbem_getrf_batched's; with flag[b] = 0 they are the partial-
+ * pivoting factors.  info as above.  Panels of 64 (complex64) or 32
+ * (complex128) columns: the diagonal block factored in LDS, the multipliers
+ * below as the product with its inverse.  No counterpart in the reference
+ * (an alternative path to lu_factor_dispatch, mnpbem/utils/gpu.py:145-191,
+ * for the Green matrices).  The inverse diagonal blocks live in one
+ * device-wide buffer of this library: two such factorisations must not run
+ * concurrently on one device. */
+int mnpbem_getrf_nopiv_batched(int dtype, int n, void* A, long lda, long sA, int* ipiv, long spiv, int* info,
+                               int* flag, int batch, void* stream);
+
 /* B <- A^-1 B for A factored by mnpbem_getrf_batched, nrhs columns.
  * Replaces lu_solve_dispatch / lu_solve_native (mnpbem/utils/gpu.py:194-279). */
 int mnpbem_getrs_batched(int dtype, int n, int nrhs, const void* A, long lda, long sA, const int* ipiv, long spiv,

@@ -32,6 +32,7 @@ SIGNATURES = {
     "mnpbem_gemm_tuning_get": [_p, _i],
     "mnpbem_gemm_tuning_seen": [_p, _i, _i],
     "mnpbem_getrf_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _i, _p],
+    "mnpbem_getrf_nopiv_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _p, _i, _p],
     "mnpbem_getrs_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _i, _p],
     "mnpbem_refine_coeffs": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p, _i, _p, _i, _p, _p, _p,
                              _p],

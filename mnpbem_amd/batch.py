@@ -278,7 +278,7 @@ class BatchSweepGraph:
             buf["pw"].copy_(gr.pw_params(self.pw, lb))
 
     def _step(self, lams):
-        ext, norms, keep = [], [], []
+        ext, norms, keep, flags = [], [], [], []
         for gr, buf in zip(self.batch.groups, self.inputs):
             gr.green.gdiv = self.L
             keep.append(gr.side_flags(self.pw))
@@ -288,8 +288,9 @@ class BatchSweepGraph:
             X = gr.bem.solve_batch(F, R, fixed_iters=self.iters)
             ext.append(gr.extinction(self.pw, X, buf["pw"]))
             norms.append(gr.bem.last_norms)
+            flags.append(F.npflag)
             del F, R, X
-        self.ext, self.norms, self._keep = ext, norms, keep
+        self.ext, self.norms, self._keep, self.npflags = ext, norms, keep, flags
 
     def run(self, lams):
         """Extinction (S, L, npol) per group for the block ``lams``."""
@@ -308,6 +309,11 @@ class BatchSweepGraph:
             with torch.cuda.graph(self.graph):
                 self._step(lams)
         self.graph.replay()
+        if any(f is not None and int(f.max()) > 0 for f in self.npflags):
+            # partial pivoting interchanges rows of some G or Delta of this
+            # block, which the captured factorisation did not: the eager path
+            # factors it with interchanges
+            return self.batch.solve_block(lams, self.pw)
         worst = 0.0
         for nrm in self.norms:
             h = nrm.cpu().numpy()

@@ -122,11 +122,19 @@ class BEMRet:
             return False
         return 4 * B * n * n * 16 < 0.4 * torch.cuda.mem_get_info(self.device)[0]
 
-    def factor(self, enei_list, kmat=None, par=None, store=None):
+    def factor(self, enei_list, kmat=None, par=None, store=None, interchanges=False):
         """Assemble and factor the BEM matrices for a batch of wavelengths.
         ``kmat``, ``par`` (device tensors of ``g.wavenumbers`` and the
         per-wavelength permittivities) and ``store`` may be given so that no
-        host transfer or device query happens (graph capture)."""
+        host transfer or device query happens (graph capture).
+
+        G1, G2 and Delta are factored without row interchanges, recording
+        per matrix whether partial pivoting would have made one (a
+        multiplier above 1 in |re| + |im|); when none would, the factors are
+        those of partial pivoting (LAPACK getrf).  Outside graph capture a
+        recorded interchange makes the whole batch factor again with partial
+        pivoting (``interchanges=True``, always used for Sigma); a captured
+        step leaves the record in ``F.npflag`` for its caller."""
         enei_list = np.atleast_1d(np.asarray(enei_list, dtype=float))
         g, dev = self.g, self.device
         B, n = len(enei_list), g.n
@@ -163,8 +171,17 @@ class BEMRet:
                 g.assemble(s, kmat, GG[s], HH[s], hs, nat.DTYPE_C128)
         piv = torch.empty((4, B, n), dtype=torch.int32, device=dev)  # G1, G2, Delta, Sigma
         info = torch.empty((4, B), dtype=torch.int32, device=dev)
+        npflag = torch.empty((3, B), dtype=torch.int32, device=dev)  # G1, G2, Delta
+
+        def lu_gd(A, pv, inf, flg, nb):
+            if interchanges:
+                nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(A), n, nn, nat.ptr(pv), n, nat.ptr(inf), nb, st)
+            else:
+                nat.call("mnpbem_getrf_nopiv_batched", dt, n, nat.ptr(A), n, nn, nat.ptr(pv), n, nat.ptr(inf),
+                         nat.ptr(flg), nb, st)
+
         # LU(G^T), then Sigma^T = (G^T)^-1 H^T for both sides in one batch of 2B
-        nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(GG), n, nn, nat.ptr(piv), n, nat.ptr(info), 2 * B, st)
+        lu_gd(GG, piv, info, npflag, 2 * B)
         nat.call("mnpbem_getrs_batched", dt, n, n, nat.ptr(GG), n, nn, nat.ptr(piv), n, nat.ptr(HH), n, nn,
                  2 * B, st)
         S1, S2 = HH[0], HH[1]
@@ -172,7 +189,7 @@ class BEMRet:
         gd = getattr(g, "gdiv", 0)
         nat.call("mnpbem_sigma_batched", dt, 0, n, nat.ptr(S1), nat.ptr(S2), nat.ptr(Dt), None, n, nn,
                  nat.ptr(g.nvec), nat.ptr(F.par), gd, B, st)
-        nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(info[2]), B, st)
+        lu_gd(Dt, piv[2], info[2], npflag[2], B)
         # (Delta^-1)^T in place of the factors (LAPACK getri's operation count)
         Wk = torch.empty((B, n, n), dtype=cdt, device=dev)
         Q = torch.empty((B, n), dtype=torch.int32, device=dev)
@@ -192,6 +209,10 @@ class BEMRet:
         nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(GG), n, nn, nat.ptr(F.DG), F.sD, 2 * B, st)
         nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(F.DS), F.sD, B, st)
         F.GG, F.HH, F.Et, F.piv, F.info = GG, HH, Et, piv, info
+        F.npflag = None if interchanges else npflag
+        if not interchanges and not torch.cuda.is_current_stream_capturing() and int(npflag.max()) > 0:
+            del F
+            return self.factor(enei_list, kmat=kmat, par=par, store=store, interchanges=True)
         return F
 
     def _solve_direct(self, F, R, X, W, P):
@@ -486,6 +507,7 @@ class SweepGraph:
         X = bem.solve_batch(F, R, fixed_iters=self.iters)
         self.ext = self.pw.extinction_batch(p, lams, X, g, pw=self.pwt)
         self.norms = bem.last_norms
+        self.npflag = F.npflag
 
     def run(self, enei_list):
         lams = np.atleast_1d(np.asarray(enei_list, dtype=float))
@@ -504,6 +526,15 @@ class SweepGraph:
                 self._step(lams)
             self.captures += 1
         self.graph.replay()
+        if self.npflag is not None and int(self.npflag.max()) > 0:
+            # partial pivoting interchanges rows of some G or Delta of these
+            # wavelengths, which the captured factorisation did not: solve
+            # them eagerly with interchanges
+            bem = self.bem
+            F = bem.factor(lams, interchanges=True)
+            X = bem.solve_batch(F, self.pw.rhs(bem.p, lams, bem.g))
+            self.last_residual = None
+            return self.pw.extinction_batch(bem.p, lams, X, bem.g).cpu().numpy()
         h = self.norms.cpu().numpy()
         rel = np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300))
         self.last_residual = rel

@@ -1278,6 +1278,198 @@ int getrf_rec(int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP,
 }
 
 // ---------------------------------------------------------------------------
+// LU without row interchanges, for matrices whose partial pivoting makes none
+// (the Green matrices G1, G2 and Delta of the BEM system are diagonally
+// dominant in this sense: every multiplier has |re| + |im| <= 0.51 on the
+// test geometries).  Partial pivoting takes the entry of largest |re| + |im|
+// of the remaining column (lowest row on ties); it keeps the diagonal at
+// every step exactly when no multiplier exceeds 1 in |re| + |im|, and then
+// its factors are these.  The factorisation records that test per matrix
+// (flag[b] = 1 when some multiplier exceeds 1) so that the caller can factor
+// again with interchanges.
+//
+// Panel of w <= W columns [c0, c0 + w): k_lu_np_diag (one workgroup per
+// matrix) factors the w x w diagonal block in LDS, stores it and leaves
+// U11^-1 in a static device buffer; k_lu_np_rows then forms the multipliers
+// below as a product, L21 = A21 U11^-1, 64 rows per workgroup, every output
+// a sum over k <= j in order.  No per-column reduction or row exchange: two
+// launches per W columns against W / 8 for the pivoting leaf.  The buffer
+// holds NP_MAXB matrices (the entry point runs larger batches in chunks);
+// concurrent factorisations on one device would share it.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct NpW;
+template <>
+struct NpW<float> {
+  static constexpr int w = 64;
+};
+template <>
+struct NpW<double> {
+  static constexpr int w = 32;
+};
+constexpr int NP_MAXB = 512;
+__device__ float g_np_uinv_f[NP_MAXB * 64 * 64 * 2];
+__device__ double g_np_uinv_d[NP_MAXB * 32 * 32 * 2];
+template <typename T>
+__device__ __forceinline__ T* np_uinv();
+template <>
+__device__ __forceinline__ float* np_uinv<float>() {
+  return g_np_uinv_f;
+}
+template <>
+__device__ __forceinline__ double* np_uinv<double>() {
+  return g_np_uinv_d;
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(256) void k_lu_np_diag(int c0, int w, T* A, long lda, long sA, int* info, int* flag) {
+  const long b = blockIdx.x;
+  A += 2 * b * sA;
+  const int tid = threadIdx.x;
+  __shared__ cplx<T> U[W][W + 1];
+  __shared__ cplx<T> X[W][W + 1];
+  for (int e = tid; e < W * W; e += 256) {
+    const int i = e / W, j = e % W;
+    U[i][j] = (i < w && j < w) ? cld(A + 2 * ((long)(c0 + i) * lda + c0 + j)) : cmake<T>(0, 0);
+  }
+  __syncthreads();
+  bool big = false;
+  for (int k = 0; k < w; ++k) {
+    const cplx<T> pv = U[k][k];
+    const bool zero = pv.re == T(0) && pv.im == T(0);
+    const cplx<T> inv = zero ? cmake<T>(1, 0) : crecip(pv);
+    if (tid == 0 && zero && info[b] == 0) info[b] = c0 + k + 1;
+    if (tid > k && tid < w) {
+      const cplx<T> l = cmul(U[tid][k], inv);
+      U[tid][k] = l;
+      big = big || cabs1(l) > T(1);
+    }
+    __syncthreads();
+    const int r = w - k - 1;
+    for (int e = tid; e < r * r; e += 256) {
+      const int i = k + 1 + e / r, j = k + 1 + e % r;
+      U[i][j] = cfms(U[i][j], U[i][k], U[k][j]);
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < w * w; e += 256) {
+    const int i = e / w, j = e % w;
+    cst(A + 2 * ((long)(c0 + i) * lda + c0 + j), U[i][j]);
+  }
+  // X = U11^-1 (upper), column j by back substitution in thread j; zero
+  // pivots (reported in info) are taken as 1, as in the factorisation
+  if (tid < W) {
+    const int j = tid;
+    for (int i = W - 1; i >= 0; --i) {
+      cplx<T> sv = cmake<T>((i == j && i < w) ? T(1) : T(0), T(0));
+      if (i < w && i <= j && j < w) {
+        for (int l = i + 1; l <= j; ++l) sv = cfms(sv, U[i][l], X[l][j]);
+        const cplx<T> pv = U[i][i];
+        sv = (pv.re == T(0) && pv.im == T(0)) ? sv : cdiv(sv, pv);
+      } else {
+        sv = cmake<T>(0, 0);
+      }
+      X[i][j] = sv;
+    }
+  }
+  __syncthreads();
+  T* Xg = np_uinv<T>() + 2 * b * W * W;
+  for (int e = tid; e < W * W; e += 256) cst(Xg + 2 * e, X[e / W][e % W]);
+  if (big) flag[b] = 1;  // every writer stores the same value
+}
+
+// 64 rows per workgroup; thread (row group tr, column group tc) computes
+// RW rows x CW columns of L21 = A21 U11^-1
+template <typename T, int W>
+__global__ __launch_bounds__(256) void k_lu_np_rows(int N, int c0, int w, T* A, long lda, long sA, int* flag) {
+  constexpr int RW = 4, CW = W * 64 / (256 * RW);  // outputs per thread: RW x CW
+  constexpr int NTC = W / CW;                      // column groups
+  const long b = blockIdx.y;
+  A += 2 * b * sA;
+  const int tid = threadIdx.x;
+  __shared__ cplx<T> As[64][W + 1];
+  __shared__ cplx<T> Xs[W][W + 1];
+  const int r0 = c0 + w + blockIdx.x * 64;
+  const int nr = min(64, N - r0);
+  for (int e = tid; e < 64 * W; e += 256) {
+    const int i = e / W, j = e % W;
+    As[i][j] = (i < nr && j < w) ? cld(A + 2 * ((long)(r0 + i) * lda + c0 + j)) : cmake<T>(0, 0);
+  }
+  const T* Xg = np_uinv<T>() + 2 * b * W * W;
+  for (int e = tid; e < W * W; e += 256) Xs[e / W][e % W] = cld(Xg + 2 * e);
+  __syncthreads();
+  const int tc = tid % NTC, tr = tid / NTC;
+  cplx<T> acc[RW][CW];
+#pragma unroll
+  for (int i = 0; i < RW; ++i)
+#pragma unroll
+    for (int j = 0; j < CW; ++j) acc[i][j] = cmake<T>(0, 0);
+  // column j = tc + NTC * jj; U11^-1 is upper, so k runs to the largest column
+  const int kmax = min(w, tc + NTC * (CW - 1) + 1);
+  for (int k = 0; k < kmax; ++k) {
+    cplx<T> a[RW], x[CW];
+#pragma unroll
+    for (int i = 0; i < RW; ++i) a[i] = As[tr + 16 * i][k];
+#pragma unroll
+    for (int j = 0; j < CW; ++j) x[j] = Xs[k][tc + NTC * j];
+#pragma unroll
+    for (int i = 0; i < RW; ++i)
+#pragma unroll
+      for (int j = 0; j < CW; ++j) acc[i][j] = cfma(acc[i][j], a[i], x[j]);
+  }
+  bool big = false;
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+    const int r = tr + 16 * i;
+    if (r < nr)
+#pragma unroll
+      for (int j = 0; j < CW; ++j) {
+        const int c = tc + NTC * j;
+        if (c < w) {
+          cst(A + 2 * ((long)(r0 + r) * lda + c0 + c), acc[i][j]);
+          big = big || cabs1(acc[i][j]) > T(1);
+        }
+      }
+  }
+  if (big) flag[b] = 1;
+}
+
+__global__ void k_iota_rows(int n, int* ipiv, long sP) {
+  const long b = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) ipiv[b * sP + i] = i;
+}
+
+// recursive LU without interchanges of columns [c0, c0 + n) (rows c0..N-1)
+template <typename T>
+int getrf_np_rec(int N, int c0, int n, T* A, long lda, long sA, int* info, int* flag, int batch, hipStream_t st,
+                 int gmode) {
+  constexpr int W = NpW<T>::w;
+  if (n <= W) {
+    hipLaunchKernelGGL((k_lu_np_diag<T, W>), dim3(batch), dim3(256), 0, st, c0, n, A, lda, sA, info, flag);
+    MNP_LAUNCH_CHECK();
+    if (N - c0 - n > 0) {
+      dim3 grid(mnp_ceil_div(N - c0 - n, 64), batch);
+      hipLaunchKernelGGL((k_lu_np_rows<T, W>), grid, dim3(256), 0, st, N, c0, n, A, lda, sA, flag);
+      MNP_LAUNCH_CHECK();
+    }
+    return 0;
+  }
+  int n1 = ((n / 2 + W - 1) / W) * W;
+  if (n1 >= n) n1 = n - W;
+  const int n2 = n - n1;
+  int rc;
+  if ((rc = getrf_np_rec(N, c0, n1, A, lda, sA, info, flag, batch, st, gmode))) return rc;
+  T* A11 = A + 2 * ((long)c0 * lda + c0);
+  T* A12 = A + 2 * ((long)c0 * lda + c0 + n1);
+  T* A21 = A + 2 * ((long)(c0 + n1) * lda + c0);
+  T* A22 = A + 2 * ((long)(c0 + n1) * lda + c0 + n1);
+  if ((rc = trsm_rec<T>(true, n1, n2, A11, lda, sA, A12, lda, sA, batch, st, gmode))) return rc;
+  if ((rc = gemm_sub<T>(N - c0 - n1, n2, n1, A21, lda, sA, A12, lda, sA, A22, lda, sA, batch, st, gmode))) return rc;
+  return getrf_np_rec(N, c0 + n1, n2, A, lda, sA, info, flag, batch, st, gmode);
+}
+
+// ---------------------------------------------------------------------------
 // Inverse from the LU factors (LAPACK getri's operation count): with P A = L U,
 // A^-1 = U^-1 L^-1 P.  L^-1 is formed by a recursion that keeps the zero
 // upper triangle of the identity out of the work (n^3/6 complex MACs instead
@@ -1470,6 +1662,33 @@ extern "C" int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA
                             !tall_ok<float>(n, batch));
   mnp_set_error("getrf: unknown dtype");
   return 1;
+}
+
+// LU without row interchanges (ipiv = identity), flag[b] = 1 when partial
+// pivoting would have interchanged rows of matrix b
+extern "C" int mnpbem_getrf_nopiv_batched(int dtype, int n, void* A, long lda, long sA, int* ipiv, long sP,
+                                          int* info, int* flag, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n >= 0 && batch >= 0 && lda >= n && sP >= n, "getrf_nopiv: bad size");
+  MNP_REQUIRE(dtype == MNP_DTYPE_C128 || mnp_is_c64(dtype), "getrf_nopiv: unknown dtype");
+  if (n == 0 || batch == 0) return 0;
+  hipLaunchKernelGGL(k_fill_int, dim3(mnp_ceil_div(batch, 256)), dim3(256), 0, st, info, (long)batch, 0);
+  MNP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fill_int, dim3(mnp_ceil_div(batch, 256)), dim3(256), 0, st, flag, (long)batch, 0);
+  MNP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_iota_rows, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, ipiv, sP);
+  MNP_LAUNCH_CHECK();
+  const long esz = dtype == MNP_DTYPE_C128 ? 16 : 8;
+  for (int b0 = 0; b0 < batch; b0 += NP_MAXB) {
+    const int nb = std::min(NP_MAXB, batch - b0);
+    void* Ab = (char*)A + (long)b0 * sA * esz;
+    int rc = dtype == MNP_DTYPE_C128
+                 ? getrf_np_rec<double>(n, 0, n, (double*)Ab, lda, sA, info + b0, flag + b0, nb, st, 0)
+                 : getrf_np_rec<float>(n, 0, n, (float*)Ab, lda, sA, info + b0, flag + b0, nb, st,
+                                       dtype == MNP_DTYPE_C64X ? 1 : 0);
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 // B <- A^-1 B with A = P^T L U from mnpbem_getrf_batched (no transpose)

@@ -176,7 +176,8 @@ class HMatrix:
     (from a device matrix, a host array or a host callable fun(row, col)) or
     ``HMatrix.from_func``."""
 
-    def __init__(self, tree=None, htol=1e-6, kmax=100, fadmiss=None, dtype="complex128", device="cuda"):
+    def __init__(self, tree=None, htol=1e-6, kmax=100, fadmiss=None, dtype="complex128", device="cuda",
+                 partition=None):
         if dtype not in _DT:
             raise ValueError("HMatrix: dtype must be 'complex128' or 'complex64'")
         self.tree = tree
@@ -191,8 +192,17 @@ class HMatrix:
         self.ranks = np.zeros(0, dtype=np.int64)
         self._built = False
         self._lu = None
-        if tree is not None:
+        if tree is not None and partition is not None:
+            # the (dense, low-rank) block lists of another H-matrix over the
+            # same tree and admissibility
+            self.row1, self.col1, self.row2, self.col2 = partition
+        elif tree is not None:
             self._init(tree, fadmiss)
+
+    @property
+    def partition(self):
+        """(row1, col1, row2, col2): the dense and low-rank cluster pairs."""
+        return self.row1, self.col1, self.row2, self.col2
 
     def _init(self, tree, fadmiss):
         self.tree = tree
@@ -806,6 +816,7 @@ class ACACompGreenRet:
         self._user_fadmiss = fadmiss
         self.htol, self.kmax, self.cleaf, self.dtype = htol, kmax, cleaf, dtype
         self._cache = {}
+        self._partitions = {}
 
     @staticmethod
     def _ipart(p):
@@ -828,8 +839,15 @@ class ACACompGreenRet:
         """HMatrix of side ``side``'s key ('G', 'F', 'H1', 'H2') at wavelength
         enei with entries generated in place; kmat: that wavelength's (nmat,
         2) row of CompGreenRet.wavenumbers."""
+        # the four Green functions of one wavelength share the partition
+        pkey = None if self._user_fadmiss is not None else float(enei)
+        part = self._partitions.get(pkey)
         h = HMatrix(self.tree, htol=self.htol, kmax=self.kmax, fadmiss=self.fadmiss_for(enei), dtype=self.dtype,
-                    device=self.g.device)
+                    device=self.g.device, partition=part)
+        if part is None:
+            if len(self._partitions) >= 8:
+                self._partitions.clear()
+            self._partitions[pkey] = h.partition
         return h.aca_green(self.g, side, key, kmat)
 
     def eval(self, i, j, key, enei):
@@ -855,3 +873,4 @@ class ACACompGreenRet:
 
     def clear_cache(self):
         self._cache = {}
+        self._partitions = {}

@@ -446,3 +446,135 @@ def test_getri_matches_inverse(cuda, dtype, n):
             # complex64 / bf16x3: the condition number (~n for random
             # matrices) times the product accuracy; no worse than before
             assert err < 3 * err_old + 1e-6, (b, err, err_old)
+
+
+@pytest.mark.parametrize("dtype,tol,split", [(np.complex128, 1e-12, False), (np.complex64, 2e-5, False),
+                                             (np.complex64, 2e-4, True)])
+@pytest.mark.parametrize("n", [1, 7, 33, 64, 65, 100, 300, 1031])
+def test_getrf_nopiv_equals_partial_pivoting(cuda, dtype, tol, split, n):
+    """Diagonally dominant matrices (partial pivoting keeps the diagonal):
+    the factorisation without interchanges reports no interchange, returns
+    the identity permutation and the factors of LAPACK's zgetrf and of the
+    pivoting getrf, and its getrs solves the system."""
+    import scipy.linalg as sla
+    nat = _lib()
+    rng = np.random.default_rng(100 + n)
+    B = 3
+    A = (_rand(rng, (B, n, n), dtype) + (2.0 * np.sqrt(n) + 2.0) * np.eye(n)[None]).astype(dtype)
+    dt = nat.DTYPE_C128 if dtype == np.complex128 else (nat.DTYPE_C64X if split else nat.DTYPE_C64)
+    st = nat.stream_handle(cuda)
+    out = []
+    for entry in ("mnpbem_getrf_nopiv_batched", "mnpbem_getrf_batched"):
+        tA = torch.from_numpy(A.copy()).to(cuda)
+        piv = torch.full((B, n), -7, dtype=torch.int32, device=cuda)
+        info = torch.full((B,), -7, dtype=torch.int32, device=cuda)
+        flag = torch.full((B,), -7, dtype=torch.int32, device=cuda)
+        print(entry, dtype.__name__, split, n, flush=True)
+        if entry == "mnpbem_getrf_nopiv_batched":
+            nat.call(entry, dt, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), nat.ptr(flag), B, st)
+            assert np.all(flag.cpu().numpy() == 0)
+        else:
+            nat.call(entry, dt, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
+        assert np.all(info.cpu().numpy() == 0)
+        assert np.array_equal(piv.cpu().numpy(), np.tile(np.arange(n), (B, 1)))
+        out.append(tA.cpu().numpy().astype(np.complex128))
+    for b in range(B):
+        lu, p = sla.lu_factor(A[b].astype(np.complex128))
+        assert np.array_equal(p, np.arange(n))
+        scale = np.abs(lu).max()
+        assert np.abs(out[0][b] - lu).max() < tol * scale, b
+        assert np.abs(out[0][b] - out[1][b]).max() < tol * scale, b
+    X = _rand(rng, (B, n, 5), dtype)
+    tA = torch.from_numpy(A.copy()).to(cuda)
+    tX = torch.from_numpy(X.copy()).to(cuda)
+    piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((B,), dtype=torch.int32, device=cuda)
+    flag = torch.empty((B,), dtype=torch.int32, device=cuda)
+    nat.call("mnpbem_getrf_nopiv_batched", dt, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info),
+             nat.ptr(flag), B, st)
+    nat.call("mnpbem_getrs_batched", dt, n, 5, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(tX), 5, n * 5, B, st)
+    ref = np.linalg.solve(A.astype(np.complex128), X.astype(np.complex128))
+    assert np.abs(tX.cpu().numpy() - ref).max() / np.abs(ref).max() < 10 * tol
+
+
+def test_getrf_nopiv_flags_interchanges_and_zero_pivots(cuda):
+    """A general random matrix needs interchanges (flag 1, as LAPACK's
+    pivots show); a dominant one does not (flag 0); an exactly zero pivot is
+    reported in info like getrf; batches above the 512 matrices of the
+    inverse-block buffer run in chunks."""
+    import scipy.linalg as sla
+    nat = _lib()
+    rng = np.random.default_rng(5)
+    st = nat.stream_handle(cuda)
+    n = 150
+    A = _rand(rng, (2, n, n), np.complex128)
+    A[1] += 3.0 * np.sqrt(n) * np.eye(n)
+    assert not np.array_equal(sla.lu_factor(A[0])[1], np.arange(n))
+    tA = torch.from_numpy(A.copy()).to(cuda)
+    piv = torch.empty((2, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((2,), dtype=torch.int32, device=cuda)
+    flag = torch.empty((2,), dtype=torch.int32, device=cuda)
+    nat.call("mnpbem_getrf_nopiv_batched", nat.DTYPE_C128, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info),
+             nat.ptr(flag), 2, st)
+    assert flag.cpu().numpy().tolist() == [1, 0]
+    Z = np.eye(40, dtype=np.complex128)
+    Z[17, 17] = 0.0
+    tZ = torch.from_numpy(Z[None].copy()).to(cuda)
+    p1 = torch.empty((1, 40), dtype=torch.int32, device=cuda)
+    i1 = torch.empty((1,), dtype=torch.int32, device=cuda)
+    f1 = torch.empty((1,), dtype=torch.int32, device=cuda)
+    nat.call("mnpbem_getrf_nopiv_batched", nat.DTYPE_C128, 40, nat.ptr(tZ), 40, 1600, nat.ptr(p1), 40, nat.ptr(i1),
+             nat.ptr(f1), 1, st)
+    assert int(i1.cpu()[0]) == 18
+    B, n = 520, 70
+    A = (_rand(rng, (B, n, n), np.complex64) + (2.0 * np.sqrt(n) + 2.0) * np.eye(n)[None]).astype(np.complex64)
+    outs = []
+    for entry in ("mnpbem_getrf_nopiv_batched", "mnpbem_getrf_batched"):
+        tA = torch.from_numpy(A.copy()).to(cuda)
+        piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
+        info = torch.empty((B,), dtype=torch.int32, device=cuda)
+        flag = torch.empty((B,), dtype=torch.int32, device=cuda)
+        extra = (nat.ptr(flag),) if entry == "mnpbem_getrf_nopiv_batched" else ()
+        nat.call(entry, nat.DTYPE_C64, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), *extra, B, st)
+        if extra:
+            assert np.all(flag.cpu().numpy() == 0)
+        outs.append(tA.cpu().numpy())
+    assert np.abs(outs[0] - outs[1]).max() < 2e-5 * np.abs(outs[1]).max()
+
+
+def test_factor_interchanges_fallback(cuda, monkeypatch):
+    """BEMRet.factor runs G1, G2 and Delta without interchanges; when the
+    factorisation records one it factors again with partial pivoting.  The
+    record is forced from the host here (the flag words set to 1 after the
+    call) and the solution must equal the one of the plain pivoting path."""
+    import mnpbem_amd as m
+    from mnpbem_amd import _native as nat
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(10.0, 9.065, 0.0708)], [m.trisphere(144, 20.0)], [[2, 1]], 1)
+    pw = m.PlaneWaveRet([[1, 0, 0]], [[0, 0, 1]])
+    lam = [520.0, 610.0]
+    bem = m.BEMRet(cp, precision="double")
+    ref = bem.solve_sweep(lam, pw)
+    F = bem.factor(lam)
+    assert F.npflag is not None and int(F.npflag.max()) == 0
+    Fp = bem.factor(lam, interchanges=True)
+    assert Fp.npflag is None
+    ones = torch.ones((2,), dtype=torch.int32, device=cuda)
+    orig = nat.call
+    seen = []
+
+    def call(name, *args):
+        rc = orig(name, *args)
+        if name == "mnpbem_getrf_nopiv_batched":
+            seen.append(name)
+            # flag pointer (argument 8): copy the bits of two int32 ones over the first two words
+            orig("mnpbem_convert", nat.DTYPE_C128, 1, nat.ptr(ones), args[8], args[-1])
+        return rc
+
+    monkeypatch.setattr(nat, "call", call)
+    F2 = bem.factor(lam)
+    monkeypatch.setattr(nat, "call", orig)
+    assert seen and F2.npflag is None  # factored again with interchanges
+    lams = np.asarray(lam)
+    X = bem.solve_batch(F2, pw.rhs(cp, lams, bem.g))
+    out = pw.extinction_batch(cp, lams, X, bem.g).cpu().numpy()
+    assert np.abs(out / ref - 1).max() < 1e-12
