@@ -208,9 +208,11 @@ def main():
         graph = {"graph_value": round(len(lams) / tg, 4), "graph_ms_per_step": round(1000.0 * tg, 1),
                  "graph_refine_steps": args.graph_iters,
                  "graph_max_rel_diff_vs_eager": float(np.max(np.abs(ext_g / ext_mixed - 1))),
-                 "graph_final_residual": float(np.max(sg.last_residual))}
-        log("graph: %.2f wavelengths/s (%.1f ms/step), vs eager max rel diff %.2e"
-            % (graph["graph_value"], graph["graph_ms_per_step"], graph["graph_max_rel_diff_vs_eager"]))
+                 "graph_final_residual": float(np.max(sg.last_residual)),
+                 "graph_eager_fallbacks": sg.fallbacks}
+        log("graph: %.2f wavelengths/s (%.1f ms/step), vs eager max rel diff %.2e, %d eager fallbacks"
+            % (graph["graph_value"], graph["graph_ms_per_step"], graph["graph_max_rel_diff_vs_eager"],
+               sg.fallbacks))
         del sg
 
     fp64 = None

@@ -81,8 +81,8 @@ int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA, int* ipiv
 /* The same factorisation without row interchanges (ipiv = identity), for
  * matrices whose partial pivoting makes none; flag[b] = 1 when it would have
  * made one (some multiplier has |re|+|im| > 1), and then the factors differ
- * from mnpbem_getrf_batched's; with flag[b] = 0 they are the partial-
- * pivoting factors.  info as above.  Panels of 64 (complex64) or 32
+ * from mnpbem_getrf_batched's, 2 when a multiplier exceeds 1024 (growth guard);
+ * with flag[b] = 0 they are the partial-pivoting factors.  info as above.  Panels of 64 (complex64) or 32
  * (complex128) columns: the diagonal block factored in LDS, the multipliers
  * below as the product with its inverse.  No counterpart in the reference
  * (an alternative path to lu_factor_dispatch, mnpbem/utils/gpu.py:145-191,

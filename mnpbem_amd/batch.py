@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .bem import BEMRet, _par_tensor
+from .bem import BEMRet, _par_tensor, needs_interchanges
 from .greenfun import CompGreenRet
 
 
@@ -309,16 +309,18 @@ class BatchSweepGraph:
             with torch.cuda.graph(self.graph):
                 self._step(lams)
         self.graph.replay()
-        if any(f is not None and int(f.max()) > 0 for f in self.npflags):
-            # partial pivoting interchanges rows of some G or Delta of this
-            # block, which the captured factorisation did not: the eager path
-            # factors it with interchanges
+        if any(needs_interchanges(f) for f in self.npflags):
+            # the captured factorisation ran without the row interchanges
+            # that this block needs: the eager path factors it with them
             return self.batch.solve_block(lams, self.pw)
         worst = 0.0
         for nrm in self.norms:
             h = nrm.cpu().numpy()
             worst = max(worst, float(np.max(np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300)))))
         self.last_residual = worst
+        if worst > self.batch.refine_tol and any(f is not None for f in self.npflags):
+            # Sigma without interchanges needs more steps than captured
+            return self.batch.solve_block(lams, self.pw)
         if worst > self.batch.refine_tol:
             raise RuntimeError("BatchSweepGraph: residual %.2e above refine_tol after %d steps; use more iters"
                                % (worst, self.iters))

@@ -131,10 +131,18 @@ class BEMRet:
         G1, G2 and Delta are factored without row interchanges, recording
         per matrix whether partial pivoting would have made one (a
         multiplier above 1 in |re| + |im|); when none would, the factors are
-        those of partial pivoting (LAPACK getrf).  Outside graph capture a
-        recorded interchange makes the whole batch factor again with partial
-        pivoting (``interchanges=True``, always used for Sigma); a captured
-        step leaves the record in ``F.npflag`` for its caller."""
+        those of partial pivoting (LAPACK getrf).  Sigma, whose partial
+        pivoting does interchange rows, is factored with interchanges for
+        precision='double'; the mixed precisions factor it without them
+        unless a multiplier exceeds 1024 in |re| + |im| (its factors are the
+        inner solve of the FP64 refinement, which restores the accuracy and
+        checks it: a refinement that misses refine_tol factors again with
+        interchanges; the factorisation without interchanges has a backward
+        error of ~1e-15 on Sigma in FP64 and a growth below 5 on the test
+        geometries).  Outside graph capture a recorded case makes the whole
+        batch factor again with partial pivoting (``interchanges=True``); a
+        captured step leaves the record in ``F.npflag`` for its caller
+        (``needs_interchanges``)."""
         enei_list = np.atleast_1d(np.asarray(enei_list, dtype=float))
         g, dev = self.g, self.device
         B, n = len(enei_list), g.n
@@ -171,7 +179,7 @@ class BEMRet:
                 g.assemble(s, kmat, GG[s], HH[s], hs, nat.DTYPE_C128)
         piv = torch.empty((4, B, n), dtype=torch.int32, device=dev)  # G1, G2, Delta, Sigma
         info = torch.empty((4, B), dtype=torch.int32, device=dev)
-        npflag = torch.empty((3, B), dtype=torch.int32, device=dev)  # G1, G2, Delta
+        npflag = torch.zeros((4, B), dtype=torch.int32, device=dev)  # G1, G2, Delta, Sigma
 
         def lu_gd(A, pv, inf, flg, nb):
             if interchanges:
@@ -200,7 +208,11 @@ class BEMRet:
         # Sigma^T overwrites Sigma2^T
         nat.call("mnpbem_sigma_batched", dt, 1, n, nat.ptr(S1), nat.ptr(S2), None, nat.ptr(Et), n, nn,
                  nat.ptr(g.nvec), nat.ptr(F.par), gd, B, st)
-        nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(piv[3]), n, nat.ptr(info[3]), B, st)
+        if mixed and not interchanges:
+            nat.call("mnpbem_getrf_nopiv_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(piv[3]), n,
+                     nat.ptr(info[3]), nat.ptr(npflag[3]), B, st)
+        else:
+            nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(piv[3]), n, nat.ptr(info[3]), B, st)
         # inverse diagonal blocks of the factors used by the small-RHS solves
         nblk = (n + 63) // 64
         F.sD = nblk * 2 * 64 * 64
@@ -210,7 +222,7 @@ class BEMRet:
         nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(F.DS), F.sD, B, st)
         F.GG, F.HH, F.Et, F.piv, F.info = GG, HH, Et, piv, info
         F.npflag = None if interchanges else npflag
-        if not interchanges and not torch.cuda.is_current_stream_capturing() and int(npflag.max()) > 0:
+        if not interchanges and not torch.cuda.is_current_stream_capturing() and needs_interchanges(npflag):
             del F
             return self.factor(enei_list, kmat=kmat, par=par, store=store, interchanges=True)
         return F
@@ -275,6 +287,10 @@ class BEMRet:
             src = Rres
         self.last_refine_iters = mon.it
         self.last_residual = mon.best
+        if F.npflag is not None and np.any(mon.best > mon.tol):
+            # Sigma factored without interchanges was not good enough for the
+            # refinement: factor the batch again with partial pivoting
+            return self.solve_batch(self.factor(F.enei, interchanges=True), R)
         mon.check("BEMRet.solve")
         return Xbest
 
@@ -463,6 +479,16 @@ class BEMRet:
         return res[0] if len(res) == 1 else tuple(res)
 
 
+def needs_interchanges(npflag):
+    """Whether a factorisation without row interchanges must be redone with
+    them: a multiplier of G1, G2 or Delta above 1 in |re| + |im| (partial
+    pivoting would interchange rows), or one of Sigma above the growth guard
+    (records of mnpbem_getrf_nopiv_batched, rows G1, G2, Delta, Sigma)."""
+    if npflag is None:
+        return False
+    return int(npflag[:3].max()) > 0 or int(npflag[3].max()) > 1
+
+
 class SweepGraph:
     """One wavelength-chunk step of the mixed-precision sweep (Green assembly,
     factorisation, right side, ``iters`` refinement steps, extinction)
@@ -489,6 +515,7 @@ class SweepGraph:
         self.store = bem._store_copies(self.B, g.n)
         self.graph = None
         self.captures = 0
+        self.fallbacks = 0
 
     def _inputs(self, lams):
         bem, g = self.bem, self.bem.g
@@ -509,6 +536,15 @@ class SweepGraph:
         self.norms = bem.last_norms
         self.npflag = F.npflag
 
+    def _eager(self, lams):
+        """The chunk by the eager path with partial pivoting throughout."""
+        bem = self.bem
+        F = bem.factor(lams, interchanges=True)
+        X = bem.solve_batch(F, self.pw.rhs(bem.p, lams, bem.g))
+        self.last_residual = bem.last_residual
+        self.fallbacks += 1
+        return self.pw.extinction_batch(bem.p, lams, X, bem.g).cpu().numpy()
+
     def run(self, enei_list):
         lams = np.atleast_1d(np.asarray(enei_list, dtype=float))
         if len(lams) != self.B:
@@ -526,18 +562,16 @@ class SweepGraph:
                 self._step(lams)
             self.captures += 1
         self.graph.replay()
-        if self.npflag is not None and int(self.npflag.max()) > 0:
-            # partial pivoting interchanges rows of some G or Delta of these
-            # wavelengths, which the captured factorisation did not: solve
-            # them eagerly with interchanges
-            bem = self.bem
-            F = bem.factor(lams, interchanges=True)
-            X = bem.solve_batch(F, self.pw.rhs(bem.p, lams, bem.g))
-            self.last_residual = None
-            return self.pw.extinction_batch(bem.p, lams, X, bem.g).cpu().numpy()
+        if needs_interchanges(self.npflag):
+            # the captured factorisation ran without the row interchanges
+            # that these wavelengths need: solve them eagerly with them
+            return self._eager(lams)
         h = self.norms.cpu().numpy()
         rel = np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300))
         self.last_residual = rel
+        if np.any(rel > self.bem.refine_tol) and self.npflag is not None:
+            # Sigma without interchanges needs more steps than captured
+            return self._eager(lams)
         if np.any(rel > self.bem.refine_tol):
             raise RuntimeError("SweepGraph: residual %.2e above refine_tol after %d steps; use more iters"
                                % (rel.max(), self.iters))

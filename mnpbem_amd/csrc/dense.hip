@@ -1284,9 +1284,11 @@ int getrf_rec(int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP,
 // test geometries).  Partial pivoting takes the entry of largest |re| + |im|
 // of the remaining column (lowest row on ties); it keeps the diagonal at
 // every step exactly when no multiplier exceeds 1 in |re| + |im|, and then
-// its factors are these.  The factorisation records that test per matrix
-// (flag[b] = 1 when some multiplier exceeds 1) so that the caller can factor
-// again with interchanges.
+// its factors are these.  The factorisation records that test per matrix,
+// flag[b] = 1 when some multiplier exceeds 1 and 2 when one exceeds
+// NP_GUARD (a growth guard for callers that accept a factorisation without
+// the interchanges, e.g. as the inner solve of iterative refinement), so
+// that the caller can factor again with interchanges.
 //
 // Panel of w <= W columns [c0, c0 + w): k_lu_np_diag (one workgroup per
 // matrix) factors the w x w diagonal block in LDS, stores it and leaves
@@ -1308,6 +1310,13 @@ struct NpW<double> {
   static constexpr int w = 32;
 };
 constexpr int NP_MAXB = 512;
+constexpr float NP_GUARD = 1024.0f;
+
+template <typename T>
+__device__ __forceinline__ int np_level(cplx<T> l) {
+  const T a = cabs1(l);
+  return a > T(NP_GUARD) ? 2 : (a > T(1) ? 1 : 0);
+}
 __device__ float g_np_uinv_f[NP_MAXB * 64 * 64 * 2];
 __device__ double g_np_uinv_d[NP_MAXB * 32 * 32 * 2];
 template <typename T>
@@ -1333,7 +1342,7 @@ __global__ __launch_bounds__(256) void k_lu_np_diag(int c0, int w, T* A, long ld
     U[i][j] = (i < w && j < w) ? cld(A + 2 * ((long)(c0 + i) * lda + c0 + j)) : cmake<T>(0, 0);
   }
   __syncthreads();
-  bool big = false;
+  int lev = 0;
   for (int k = 0; k < w; ++k) {
     const cplx<T> pv = U[k][k];
     const bool zero = pv.re == T(0) && pv.im == T(0);
@@ -1342,7 +1351,7 @@ __global__ __launch_bounds__(256) void k_lu_np_diag(int c0, int w, T* A, long ld
     if (tid > k && tid < w) {
       const cplx<T> l = cmul(U[tid][k], inv);
       U[tid][k] = l;
-      big = big || cabs1(l) > T(1);
+      lev = max(lev, np_level(l));
     }
     __syncthreads();
     const int r = w - k - 1;
@@ -1375,7 +1384,7 @@ __global__ __launch_bounds__(256) void k_lu_np_diag(int c0, int w, T* A, long ld
   __syncthreads();
   T* Xg = np_uinv<T>() + 2 * b * W * W;
   for (int e = tid; e < W * W; e += 256) cst(Xg + 2 * e, X[e / W][e % W]);
-  if (big) flag[b] = 1;  // every writer stores the same value
+  if (lev) atomicMax(flag + b, lev);
 }
 
 // 64 rows per workgroup; thread (row group tr, column group tc) computes
@@ -1417,7 +1426,7 @@ __global__ __launch_bounds__(256) void k_lu_np_rows(int N, int c0, int w, T* A, 
 #pragma unroll
       for (int j = 0; j < CW; ++j) acc[i][j] = cfma(acc[i][j], a[i], x[j]);
   }
-  bool big = false;
+  int lev = 0;
 #pragma unroll
   for (int i = 0; i < RW; ++i) {
     const int r = tr + 16 * i;
@@ -1427,11 +1436,11 @@ __global__ __launch_bounds__(256) void k_lu_np_rows(int N, int c0, int w, T* A, 
         const int c = tc + NTC * j;
         if (c < w) {
           cst(A + 2 * ((long)(r0 + r) * lda + c0 + c), acc[i][j]);
-          big = big || cabs1(acc[i][j]) > T(1);
+          lev = max(lev, np_level(acc[i][j]));
         }
       }
   }
-  if (big) flag[b] = 1;
+  if (lev) atomicMax(flag + b, lev);
 }
 
 __global__ void k_iota_rows(int n, int* ipiv, long sP) {
@@ -1664,8 +1673,9 @@ extern "C" int mnpbem_getrf_batched(int dtype, int n, void* A, long lda, long sA
   return 1;
 }
 
-// LU without row interchanges (ipiv = identity), flag[b] = 1 when partial
-// pivoting would have interchanged rows of matrix b
+// LU without row interchanges (ipiv = identity); flag[b] = 1 when partial
+// pivoting would have interchanged rows of matrix b, 2 when a multiplier
+// exceeds NP_GUARD in |re| + |im|
 extern "C" int mnpbem_getrf_nopiv_batched(int dtype, int n, void* A, long lda, long sA, int* ipiv, long sP,
                                           int* info, int* flag, int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;

@@ -516,7 +516,17 @@ def test_getrf_nopiv_flags_interchanges_and_zero_pivots(cuda):
     flag = torch.empty((2,), dtype=torch.int32, device=cuda)
     nat.call("mnpbem_getrf_nopiv_batched", nat.DTYPE_C128, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info),
              nat.ptr(flag), 2, st)
-    assert flag.cpu().numpy().tolist() == [1, 0]
+    f = flag.cpu().numpy().tolist()
+    assert f[0] >= 1 and f[1] == 0, f
+    # levels: multipliers 2 (interchange, within the growth guard of 1024) and 1e4
+    T = np.array([[[1.0, 0.5], [2.0, 1.0]], [[1e-4, 1.0], [1.0, 1.0]], [[4.0, 1.0], [1.0, 3.0]]], np.complex128)
+    tT = torch.from_numpy(T.copy()).to(cuda)
+    p3 = torch.empty((3, 2), dtype=torch.int32, device=cuda)
+    i3 = torch.empty((3,), dtype=torch.int32, device=cuda)
+    f3 = torch.empty((3,), dtype=torch.int32, device=cuda)
+    nat.call("mnpbem_getrf_nopiv_batched", nat.DTYPE_C128, 2, nat.ptr(tT), 2, 4, nat.ptr(p3), 2, nat.ptr(i3),
+             nat.ptr(f3), 3, st)
+    assert f3.cpu().numpy().tolist() == [1, 2, 0]
     Z = np.eye(40, dtype=np.complex128)
     Z[17, 17] = 0.0
     tZ = torch.from_numpy(Z[None].copy()).to(cuda)
