@@ -1303,11 +1303,11 @@ template <typename T>
 struct NpW;
 template <>
 struct NpW<float> {
-  static constexpr int w = 64;
+  static constexpr int w = 32;
 };
 template <>
 struct NpW<double> {
-  static constexpr int w = 32;
+  static constexpr int w = 16;
 };
 constexpr int NP_MAXB = 512;
 constexpr float NP_GUARD = 1024.0f;
@@ -1330,113 +1330,173 @@ __device__ __forceinline__ double* np_uinv<double>() {
   return g_np_uinv_d;
 }
 
+// Diagonal block: thread (row i = tid / 4, column group cg = tid % 4) holds
+// row i's columns cg, cg + 4, ... in registers; per column k the pivot row
+// and the column go through LDS (double-buffered, one barrier per column)
+// and every row below updates its own entries.  Then X = U11^-1 column by
+// column, four lanes per column splitting each sum (adjacent lanes, added
+// by shuffles in a fixed order).
+// 1 / a with one real division (|a|^2 well inside the range: pivots of
+// diagonally dominant, O(1)-scaled blocks)
+template <typename T>
+__device__ __forceinline__ cplx<T> crecip_np(cplx<T> a) {
+  const T sc = T(1) / (a.re * a.re + a.im * a.im);
+  return cmake(a.re * sc, -a.im * sc);
+}
+
 template <typename T, int W>
 __global__ __launch_bounds__(256) void k_lu_np_diag(int c0, int w, T* A, long lda, long sA, int* info, int* flag) {
+  constexpr int CPT = W / 4;
   const long b = blockIdx.x;
   A += 2 * b * sA;
   const int tid = threadIdx.x;
+  const int i = tid >> 2, cg = tid & 3;
+  const bool own = i < w;
+  __shared__ cplx<T> rowb[2][W], colb[2][W];
   __shared__ cplx<T> U[W][W + 1];
-  __shared__ cplx<T> X[W][W + 1];
-  for (int e = tid; e < W * W; e += 256) {
-    const int i = e / W, j = e % W;
-    U[i][j] = (i < w && j < w) ? cld(A + 2 * ((long)(c0 + i) * lda + c0 + j)) : cmake<T>(0, 0);
+  __shared__ cplx<T> dinv[W];
+  cplx<T> a[CPT];
+#pragma unroll
+  for (int jj = 0; jj < CPT; ++jj) {
+    const int j = cg + 4 * jj;
+    a[jj] = (own && j < w) ? cld(A + 2 * ((long)(c0 + i) * lda + c0 + j)) : cmake<T>(0, 0);
   }
-  __syncthreads();
   int lev = 0;
   for (int k = 0; k < w; ++k) {
-    const cplx<T> pv = U[k][k];
+    const int pb = k & 1;
+    // branch-free selects of the thread's entry in column k
+    cplx<T> ck = a[0];
+#pragma unroll
+    for (int jj = 1; jj < CPT; ++jj) {
+      const bool hit = cg + 4 * jj == k;
+      ck.re = hit ? a[jj].re : ck.re;
+      ck.im = hit ? a[jj].im : ck.im;
+    }
+    if (i == k)
+#pragma unroll
+      for (int jj = 0; jj < CPT; ++jj) rowb[pb][cg + 4 * jj] = a[jj];
+    if (own && i > k && cg == (k & 3)) colb[pb][i] = ck;
+    __syncthreads();
+    const cplx<T> pv = rowb[pb][k];
     const bool zero = pv.re == T(0) && pv.im == T(0);
-    const cplx<T> inv = zero ? cmake<T>(1, 0) : crecip(pv);
     if (tid == 0 && zero && info[b] == 0) info[b] = c0 + k + 1;
-    if (tid > k && tid < w) {
-      const cplx<T> l = cmul(U[tid][k], inv);
-      U[tid][k] = l;
+    if (own && i > k) {
+      const cplx<T> l = cmul(colb[pb][i], zero ? cmake<T>(1, 0) : crecip_np(pv));
       lev = max(lev, np_level(l));
-    }
-    __syncthreads();
-    const int r = w - k - 1;
-    for (int e = tid; e < r * r; e += 256) {
-      const int i = k + 1 + e / r, j = k + 1 + e % r;
-      U[i][j] = cfms(U[i][j], U[i][k], U[k][j]);
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < w * w; e += 256) {
-    const int i = e / w, j = e % w;
-    cst(A + 2 * ((long)(c0 + i) * lda + c0 + j), U[i][j]);
-  }
-  // X = U11^-1 (upper), column j by back substitution in thread j; zero
-  // pivots (reported in info) are taken as 1, as in the factorisation
-  if (tid < W) {
-    const int j = tid;
-    for (int i = W - 1; i >= 0; --i) {
-      cplx<T> sv = cmake<T>((i == j && i < w) ? T(1) : T(0), T(0));
-      if (i < w && i <= j && j < w) {
-        for (int l = i + 1; l <= j; ++l) sv = cfms(sv, U[i][l], X[l][j]);
-        const cplx<T> pv = U[i][i];
-        sv = (pv.re == T(0) && pv.im == T(0)) ? sv : cdiv(sv, pv);
-      } else {
-        sv = cmake<T>(0, 0);
+#pragma unroll
+      for (int jj = 0; jj < CPT; ++jj) {
+        const int j = cg + 4 * jj;
+        const cplx<T> nv = cfms(a[jj], l, rowb[pb][j]);
+        const bool upd = j > k, put = j == k;
+        a[jj].re = put ? l.re : (upd ? nv.re : a[jj].re);
+        a[jj].im = put ? l.im : (upd ? nv.im : a[jj].im);
       }
-      X[i][j] = sv;
     }
+  }
+#pragma unroll
+  for (int jj = 0; jj < CPT; ++jj) {
+    const int j = cg + 4 * jj;
+    if (own && j < w) cst(A + 2 * ((long)(c0 + i) * lda + c0 + j), a[jj]);
+    if (i < W) U[i][j] = a[jj];
   }
   __syncthreads();
-  T* Xg = np_uinv<T>() + 2 * b * W * W;
-  for (int e = tid; e < W * W; e += 256) cst(Xg + 2 * e, X[e / W][e % W]);
+  if (tid < W) {
+    const cplx<T> pv = U[tid][tid];
+    dinv[tid] = (tid >= w || (pv.re == T(0) && pv.im == T(0))) ? cmake<T>(1, 0) : crecip_np(pv);
+  }
+  __syncthreads();
+  // X = U11^-1: thread (column jc, part q) keeps rows q, q + 4, ... of column jc
+  {
+    const int jc = tid >> 2, q = tid & 3;
+    cplx<T> xr[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) xr[m] = cmake<T>(0, 0);
+#pragma unroll
+    for (int ii = W - 1; ii >= 0; --ii) {
+      cplx<T> sm = cmake<T>(0, 0);
+#pragma unroll
+      for (int m = 0; m < CPT; ++m) {
+        const int l = q + 4 * m;
+        if (l > ii) {  // compile-time test; xr[m] is zero for rows beyond jc
+          const cplx<T> t = cfma(sm, U[ii][l], xr[m]);
+          sm = t;
+        }
+      }
+      sm.re += __shfl_xor(sm.re, 1);
+      sm.im += __shfl_xor(sm.im, 1);
+      sm.re += __shfl_xor(sm.re, 2);
+      sm.im += __shfl_xor(sm.im, 2);
+      const cplx<T> d = cmake<T>(ii == jc ? T(1) : T(0), T(0));
+      cplx<T> xv = cmul(csub(d, sm), dinv[ii]);
+      const bool live = ii <= jc && jc < w;
+      xv.re = live ? xv.re : T(0);
+      xv.im = live ? xv.im : T(0);
+#pragma unroll
+      for (int m = 0; m < CPT; ++m) {
+        const bool hit = q + 4 * m == ii;
+        xr[m].re = hit ? xv.re : xr[m].re;
+        xr[m].im = hit ? xv.im : xr[m].im;
+      }
+    }
+    T* Xg = np_uinv<T>() + 2 * b * W * W;
+    if (jc < W)
+#pragma unroll
+      for (int m = 0; m < CPT; ++m) cst(Xg + 2 * ((long)(q + 4 * m) * W + jc), xr[m]);
+  }
   if (lev) atomicMax(flag + b, lev);
 }
 
-// 64 rows per workgroup; thread (row group tr, column group tc) computes
-// RW rows x CW columns of L21 = A21 U11^-1
+// 64 rows per workgroup; wave v forms columns [v W/4, (v+1) W/4) of
+// L21 = A21 U11^-1 (sums over k below the wave's last column: U11^-1 is
+// upper), lane (row group, column group) 4 rows x W/16 columns, every output
+// summed over k in order
 template <typename T, int W>
 __global__ __launch_bounds__(256) void k_lu_np_rows(int N, int c0, int w, T* A, long lda, long sA, int* flag) {
-  constexpr int RW = 4, CW = W * 64 / (256 * RW);  // outputs per thread: RW x CW
-  constexpr int NTC = W / CW;                      // column groups
+  constexpr int CPW = W / 4, CW = CPW / 4, RW = 4;
   const long b = blockIdx.y;
   A += 2 * b * sA;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   __shared__ cplx<T> As[64][W + 1];
   __shared__ cplx<T> Xs[W][W + 1];
   const int r0 = c0 + w + blockIdx.x * 64;
   const int nr = min(64, N - r0);
   for (int e = tid; e < 64 * W; e += 256) {
-    const int i = e / W, j = e % W;
-    As[i][j] = (i < nr && j < w) ? cld(A + 2 * ((long)(r0 + i) * lda + c0 + j)) : cmake<T>(0, 0);
+    const int ii = e / W, j = e % W;
+    As[ii][j] = (ii < nr && j < w) ? cld(A + 2 * ((long)(r0 + ii) * lda + c0 + j)) : cmake<T>(0, 0);
   }
   const T* Xg = np_uinv<T>() + 2 * b * W * W;
   for (int e = tid; e < W * W; e += 256) Xs[e / W][e % W] = cld(Xg + 2 * e);
   __syncthreads();
-  const int tc = tid % NTC, tr = tid / NTC;
+  const int rg = lane >> 2, cs = lane & 3;
+  const int jb = wv * CPW + cs * CW;
+  const int kend = min(w, (wv + 1) * CPW);
   cplx<T> acc[RW][CW];
 #pragma unroll
-  for (int i = 0; i < RW; ++i)
+  for (int ii = 0; ii < RW; ++ii)
 #pragma unroll
-    for (int j = 0; j < CW; ++j) acc[i][j] = cmake<T>(0, 0);
-  // column j = tc + NTC * jj; U11^-1 is upper, so k runs to the largest column
-  const int kmax = min(w, tc + NTC * (CW - 1) + 1);
-  for (int k = 0; k < kmax; ++k) {
-    cplx<T> a[RW], x[CW];
+    for (int j = 0; j < CW; ++j) acc[ii][j] = cmake<T>(0, 0);
+  for (int k = 0; k < kend; ++k) {
+    cplx<T> av[RW], xv[CW];
 #pragma unroll
-    for (int i = 0; i < RW; ++i) a[i] = As[tr + 16 * i][k];
+    for (int ii = 0; ii < RW; ++ii) av[ii] = As[RW * rg + ii][k];
 #pragma unroll
-    for (int j = 0; j < CW; ++j) x[j] = Xs[k][tc + NTC * j];
+    for (int j = 0; j < CW; ++j) xv[j] = Xs[k][jb + j];
 #pragma unroll
-    for (int i = 0; i < RW; ++i)
+    for (int ii = 0; ii < RW; ++ii)
 #pragma unroll
-      for (int j = 0; j < CW; ++j) acc[i][j] = cfma(acc[i][j], a[i], x[j]);
+      for (int j = 0; j < CW; ++j) acc[ii][j] = cfma(acc[ii][j], av[ii], xv[j]);
   }
   int lev = 0;
 #pragma unroll
-  for (int i = 0; i < RW; ++i) {
-    const int r = tr + 16 * i;
+  for (int ii = 0; ii < RW; ++ii) {
+    const int r = RW * rg + ii;
     if (r < nr)
 #pragma unroll
       for (int j = 0; j < CW; ++j) {
-        const int c = tc + NTC * j;
+        const int c = jb + j;
         if (c < w) {
-          cst(A + 2 * ((long)(r0 + r) * lda + c0 + c), acc[i][j]);
-          lev = max(lev, np_level(acc[i][j]));
+          cst(A + 2 * ((long)(r0 + r) * lda + c0 + c), acc[ii][j]);
+          lev = max(lev, np_level(acc[ii][j]));
         }
       }
   }
