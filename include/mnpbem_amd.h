@@ -105,6 +105,19 @@ int mnpbem_getrs_batched(int dtype, int n, int nrhs, const void* A, long lda, lo
 int mnpbem_getri_batched(int dtype, int n, void* A, long lda, long sA, const int* ipiv, long spiv, void* W, long ldw,
                          long sW, int* Q, long sQ, int batch, void* stream);
 
+/* mnpbem_getrs_batched and mnpbem_getri_batched given the inverse diagonal
+ * blocks of the factors (Dinv from mnpbem_lu_blockinv_batched on the same
+ * factors, per-matrix stride sD).  For MNPBEM_DTYPE_C64X the 64-row leaves of
+ * the triangular solves become products with those inverses on the bf16x3
+ * kernel instead of substitutions; every other dtype, or Dinv = NULL, runs the
+ * plain entry point.  Same contract as those (bem_ret.py:437-523). */
+int mnpbem_getrs_blockinv_batched(int dtype, int n, int nrhs, const void* A, long lda, long sA, const int* ipiv,
+                                  long spiv, const void* Dinv, long sD, void* B, long ldb, long sB, int batch,
+                                  void* stream);
+int mnpbem_getri_blockinv_batched(int dtype, int n, void* A, long lda, long sA, const int* ipiv, long spiv,
+                                  const void* Dinv, long sD, void* W, long ldw, long sW, int* Q, long sQ, int batch,
+                                  void* stream);
+
 /* Y[b] = beta Y[b] + alpha M[b]^T X[b] for n x C blocks X, Y (any C). */
 int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, long ldm, long sM, const void* X, long ldx,
                           long sX, void* Y, long ldy, long sY, double alpha_re, double alpha_im, double beta,

@@ -188,21 +188,30 @@ class BEMRet:
                 nat.call("mnpbem_getrf_nopiv_batched", dt, n, nat.ptr(A), n, nn, nat.ptr(pv), n, nat.ptr(inf),
                          nat.ptr(flg), nb, st)
 
+        # inverse diagonal blocks of the factors: the leaves of the many-column
+        # triangular solves below and the small-right-hand-side solves
+        nblk = (n + 63) // 64
+        F.sD = nblk * 2 * 64 * 64
+        F.DG = torch.empty((2, B, F.sD), dtype=cdt, device=dev)
+        F.DS = torch.empty((B, F.sD), dtype=cdt, device=dev)
         # LU(G^T), then Sigma^T = (G^T)^-1 H^T for both sides in one batch of 2B
         lu_gd(GG, piv, info, npflag, 2 * B)
-        nat.call("mnpbem_getrs_batched", dt, n, n, nat.ptr(GG), n, nn, nat.ptr(piv), n, nat.ptr(HH), n, nn,
-                 2 * B, st)
+        nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(GG), n, nn, nat.ptr(F.DG), F.sD, 2 * B, st)
+        nat.call("mnpbem_getrs_blockinv_batched", dt, n, n, nat.ptr(GG), n, nn, nat.ptr(piv), n, nat.ptr(F.DG),
+                 F.sD, nat.ptr(HH), n, nn, 2 * B, st)
         S1, S2 = HH[0], HH[1]
         Dt = torch.empty((B, n, n), dtype=cdt, device=dev)
         gd = getattr(g, "gdiv", 0)
         nat.call("mnpbem_sigma_batched", dt, 0, n, nat.ptr(S1), nat.ptr(S2), nat.ptr(Dt), None, n, nn,
                  nat.ptr(g.nvec), nat.ptr(F.par), gd, B, st)
         lu_gd(Dt, piv[2], info[2], npflag[2], B)
-        # (Delta^-1)^T in place of the factors (LAPACK getri's operation count)
+        # (Delta^-1)^T in place of the factors (LAPACK getri's operation count);
+        # F.DS holds the inverse diagonal blocks of Delta's factors until Sigma's
+        nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(F.DS), F.sD, B, st)
         Wk = torch.empty((B, n, n), dtype=cdt, device=dev)
         Q = torch.empty((B, n), dtype=torch.int32, device=dev)
-        nat.call("mnpbem_getri_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(Wk), n, nn,
-                 nat.ptr(Q), n, B, st)
+        nat.call("mnpbem_getri_blockinv_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(F.DS),
+                 F.sD, nat.ptr(Wk), n, nn, nat.ptr(Q), n, B, st)
         Et = Dt
         del Wk, Q
         # Sigma^T overwrites Sigma2^T
@@ -213,12 +222,6 @@ class BEMRet:
                      nat.ptr(info[3]), nat.ptr(npflag[3]), B, st)
         else:
             nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(piv[3]), n, nat.ptr(info[3]), B, st)
-        # inverse diagonal blocks of the factors used by the small-RHS solves
-        nblk = (n + 63) // 64
-        F.sD = nblk * 2 * 64 * 64
-        F.DG = torch.empty((2, B, F.sD), dtype=cdt, device=dev)
-        F.DS = torch.empty((B, F.sD), dtype=cdt, device=dev)
-        nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(GG), n, nn, nat.ptr(F.DG), F.sD, 2 * B, st)
         nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(S2), n, nn, nat.ptr(F.DS), F.sD, B, st)
         F.GG, F.HH, F.Et, F.piv, F.info = GG, HH, Et, piv, info
         F.npflag = None if interchanges else npflag

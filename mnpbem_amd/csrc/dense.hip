@@ -265,7 +265,7 @@ __device__ __forceinline__ Tile gemm_tile(int remap) {
 template <int BM, int BN, int WM>
 __global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3(
     int M, int N, int K, const float* __restrict__ A, long lda, long sA, const float* __restrict__ B, long ldb,
-    long sB, float* __restrict__ C, long ldc, long sC, int remap) {
+    long sB, float* __restrict__ C, long ldc, long sC, int remap, int assign) {
   const Tile tl = gemm_tile(remap);
   const long bz = tl.z;
   A += 2 * bz * sA;
@@ -407,7 +407,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3
   // K <= 256; the 128 x 128 tile has no registers to spare for it).
   constexpr bool CPRE = BM == 64;
   float2 cpre[CPRE ? FI * FJ * 4 : 1];
-  if constexpr (CPRE) {
+  if (CPRE && !assign) {
 #pragma unroll
     for (int i = 0; i < FI; ++i)
 #pragma unroll
@@ -463,12 +463,14 @@ __global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3
         if (gr < M && gc < N) {
           float* p = C + 2 * ((long)gr * ldc + gc);
           float2 c;
-          if constexpr (CPRE)
+          if (assign)
+            c = make_float2(0.f, 0.f);
+          else if constexpr (CPRE)
             c = cpre[(i * FJ + j) * 4 + r];
           else
             c = *reinterpret_cast<float2*>(p);
-          c.x -= accR[i][j][r];
-          c.y -= accI[i][j][r];
+          c.x = assign ? accR[i][j][r] : c.x - accR[i][j][r];
+          c.y = assign ? accI[i][j][r] : c.y - accI[i][j][r];
           *reinterpret_cast<float2*>(p) = c;
         }
       }
@@ -877,28 +879,30 @@ inline int gcell(int M, int N, int K) { return (gclass(M) * GC + gclass(N)) * GC
 // of tiles; see gemm_tile)
 inline int gemm_xcd_remap() { return 1; }
 
+// assign = 1: C = A B instead of C -= A B
 int launch_bf16x3(int variant, int M, int N, int K, const float* A, long lda, long sA, const float* B, long ldb,
-                  long sB, float* C, long ldc, long sC, int batch, hipStream_t st) {
+                  long sB, float* C, long ldc, long sC, int batch, hipStream_t st, int assign = 0) {
   if (variant == 1) {
     dim3 grid(mnp_ceil_div(N, 64), mnp_ceil_div(M, 64), batch);
     hipLaunchKernelGGL((k_cgemm_sub_bf16x3<64, 64, 32>), grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB,
-                       C, ldc, sC, gemm_xcd_remap());
+                       C, ldc, sC, gemm_xcd_remap(), assign);
   } else {
     dim3 grid(mnp_ceil_div(N, 128), mnp_ceil_div(M, 128), batch);
     hipLaunchKernelGGL((k_cgemm_sub_bf16x3<128, 128, 64>), grid, dim3(512), 0, st, M, N, K, A, lda, sA, B, ldb, sB,
-                       C, ldc, sC, gemm_xcd_remap());
+                       C, ldc, sC, gemm_xcd_remap(), assign);
   }
   MNP_LAUNCH_CHECK();
   return 0;
 }
 
 // gmode 1 (complex64 only): bf16 split-product GEMM
+// assign = 1 (bf16x3 kernel only): C = A B
 template <typename T>
 int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, long ldb, long sB, T* C, long ldc,
-             long sC, int batch, hipStream_t st, int gmode) {
+             long sC, int batch, hipStream_t st, int gmode, int assign = 0) {
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return 0;
   if constexpr (sizeof(T) == 4) {
-    if (gmode == 1 && N <= THIN_N && K <= THIN_K) {
+    if (gmode == 1 && !assign && N <= THIN_N && K <= THIN_K) {
       dim3 grid(mnp_ceil_div(M, 256 / N), batch);
       hipLaunchKernelGGL(k_cgemm_thin, grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
       MNP_LAUNCH_CHECK();
@@ -916,7 +920,7 @@ int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, lon
         g_tune.seen[cell][2] = K;
         g_tune.seen[cell][3] = batch;
       }
-      int rc = launch_bf16x3(g_tune.var[cell], M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, st);
+      int rc = launch_bf16x3(g_tune.var[cell], M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, st, assign);
       if (rc) return rc;
     } else {
       dim3 grid(mnp_ceil_div(N, GT), mnp_ceil_div(M, GT), batch);
@@ -946,12 +950,40 @@ int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, lon
 constexpr int TRSM_ROWS_DEFAULT = 32768;
 long g_trsm_rows_max = TRSM_ROWS_DEFAULT;
 
-// X <- T^-1 X, T n x n at Tm (lower unit or upper), X n x nrhs
+// Inverse diagonal blocks of a triangular factor for the leaves of trsm_rec:
+// per matrix (stride sD) and 64-row block, [U^-1 | L^-1] as two 64 x 64
+// row-major blocks (the layout of k_lu_blockinv in solve.hip); blk is the
+// block index of the node's first row.
+template <typename T>
+struct LeafInv {
+  const T* D = nullptr;
+  long sD = 0;
+  int blk = 0;
+  LeafInv at(int rows) const {
+    LeafInv r = *this;
+    r.blk += rows / 64;
+    return r;
+  }
+};
+
+// X <- T^-1 X, T n x n at Tm (lower unit or upper), X n x nrhs.  With the
+// inverse diagonal blocks (complex64 on the bf16x3 kernel), a leaf of up to
+// 64 rows is the product X <- D X with its block's inverse D instead of a
+// substitution: one row tile covers the leaf, so every workgroup reads its
+// whole column strip of X before writing it back.  The recursion splits at
+// multiples of 64 rows from the first row, so leaves start on block
+// boundaries and only the last leaf of the matrix is shorter than 64.
 template <typename T>
 int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, long ldx, long sX, int batch,
-             hipStream_t st, int gmode) {
+             hipStream_t st, int gmode, LeafInv<T> inv = LeafInv<T>()) {
   if (n <= 0 || nrhs <= 0) return 0;
   constexpr int TL = TLeaf<T>::n;
+  if constexpr (sizeof(T) == 4) {
+    if (n <= 64 && inv.D != nullptr && gmode == 1) {
+      const T* D = inv.D + 2 * ((long)inv.blk * 2 * 64 * 64 + (lower ? 64 * 64 : 0));
+      return gemm_sub<T>(n, nrhs, n, D, 64, inv.sD, X, ldx, sX, X, ldx, sX, batch, st, gmode, 1);
+    }
+  }
   if (n <= TL) {
     if (n > TL / 2 && (long)nrhs * batch <= g_trsm_rows_max) {
       dim3 grid(mnp_ceil_div(nrhs, 64), batch);
@@ -1002,20 +1034,20 @@ int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, 
   const int n2 = n - n1;
   int rc;
   if (lower) {
-    if ((rc = trsm_rec(true, n1, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st, gmode))) return rc;
+    if ((rc = trsm_rec(true, n1, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st, gmode, inv))) return rc;
     if ((rc = gemm_sub(n2, nrhs, n1, Tm + 2 * ((long)n1 * ldt), ldt, sT, X, ldx, sX, X + 2 * ((long)n1 * ldx), ldx,
                        sX, batch, st, gmode)))
       return rc;
     return trsm_rec(true, n2, nrhs, Tm + 2 * ((long)n1 * ldt + n1), ldt, sT, X + 2 * ((long)n1 * ldx), ldx, sX,
-                    batch, st, gmode);
+                    batch, st, gmode, inv.at(n1));
   } else {
     if ((rc = trsm_rec(false, n2, nrhs, Tm + 2 * ((long)n1 * ldt + n1), ldt, sT, X + 2 * ((long)n1 * ldx), ldx, sX,
-                       batch, st, gmode)))
+                       batch, st, gmode, inv.at(n1))))
       return rc;
     if ((rc = gemm_sub(n1, nrhs, n2, Tm + 2 * (long)n1, ldt, sT, X + 2 * ((long)n1 * ldx), ldx, sX, X, ldx, sX,
                        batch, st, gmode)))
       return rc;
-    return trsm_rec(false, n1, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st, gmode);
+    return trsm_rec(false, n1, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st, gmode, inv);
   }
 }
 
@@ -1571,23 +1603,23 @@ int getrf_np_rec(int N, int c0, int n, T* A, long lda, long sA, int* info, int* 
 // X (n x n, identity on entry) <- L^-1, L unit lower triangular
 template <typename T>
 int trsm_lower_ident(int n, const T* L, long ldl, long sL, T* X, long ldx, long sX, int batch, hipStream_t st,
-                     int gmode) {
+                     int gmode, LeafInv<T> inv = LeafInv<T>()) {
   constexpr int TL = TLeaf<T>::n;
-  if (n <= 2 * TL) return trsm_rec<T>(true, n, n, L, ldl, sL, X, ldx, sX, batch, st, gmode);
+  if (n <= 2 * TL) return trsm_rec<T>(true, n, n, L, ldl, sL, X, ldx, sX, batch, st, gmode, inv);
   int n1 = ((n / 2 + TL - 1) / TL) * TL;
   if (n1 >= n) n1 = n - TL;
   const int n2 = n - n1;
   int rc;
-  if ((rc = trsm_lower_ident<T>(n1, L, ldl, sL, X, ldx, sX, batch, st, gmode))) return rc;
+  if ((rc = trsm_lower_ident<T>(n1, L, ldl, sL, X, ldx, sX, batch, st, gmode, inv))) return rc;
   // X21 = -L22^-1 L21 X11 (X21 is zero on entry)
   if ((rc = gemm_sub<T>(n2, n1, n1, L + 2 * ((long)n1 * ldl), ldl, sL, X, ldx, sX, X + 2 * ((long)n1 * ldx), ldx,
                         sX, batch, st, gmode)))
     return rc;
   if ((rc = trsm_rec<T>(true, n2, n1, L + 2 * ((long)n1 * ldl + n1), ldl, sL, X + 2 * ((long)n1 * ldx), ldx, sX,
-                        batch, st, gmode)))
+                        batch, st, gmode, inv.at(n1))))
     return rc;
   return trsm_lower_ident<T>(n2, L + 2 * ((long)n1 * ldl + n1), ldl, sL, X + 2 * ((long)n1 * ldx + n1), ldx, sX,
-                             batch, st, gmode);
+                             batch, st, gmode, inv.at(n1));
 }
 
 // W[b] = identity (n x n, leading dimension ld)
@@ -1642,13 +1674,13 @@ __global__ __launch_bounds__(256) void k_scatter_cols(int n, const T* __restrict
 // (the identity fill, L^-1 and U^-1 L^-1 live there); Q: n ints per matrix.
 template <typename T>
 int getri(int n, T* A, long lda, long sA, const int* ipiv, long sP, T* W, long ldw, long sW, int* Q, long sQ,
-          int batch, hipStream_t st, int gmode) {
+          int batch, hipStream_t st, int gmode, LeafInv<T> inv = LeafInv<T>()) {
   dim3 gi(mnp_ceil_div((long)n * n, 256), batch);
   hipLaunchKernelGGL(k_eye<T>, gi, dim3(256), 0, st, n, W, ldw, sW);
   MNP_LAUNCH_CHECK();
   int rc;
-  if ((rc = trsm_lower_ident<T>(n, A, lda, sA, W, ldw, sW, batch, st, gmode))) return rc;
-  if ((rc = trsm_rec<T>(false, n, n, A, lda, sA, W, ldw, sW, batch, st, gmode))) return rc;
+  if ((rc = trsm_lower_ident<T>(n, A, lda, sA, W, ldw, sW, batch, st, gmode, inv))) return rc;
+  if ((rc = trsm_rec<T>(false, n, n, A, lda, sA, W, ldw, sW, batch, st, gmode, inv))) return rc;
   hipLaunchKernelGGL(k_perm_array, dim3(batch), dim3(256), 0, st, n, ipiv, sP, Q, sQ);
   MNP_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scatter_cols<T>, dim3(mnp_ceil_div(n, 256), n, batch), dim3(256), 0, st, n, W, ldw, sW, Q,
@@ -1825,6 +1857,45 @@ extern "C" int mnpbem_getri_batched(int dtype, int n, void* A, long lda, long sA
                         dtype == MNP_DTYPE_C64X ? 1 : 0);
   mnp_set_error("getri: unknown dtype");
   return 1;
+}
+
+// getrs and getri with the inverse diagonal blocks of the factors (Dinv from
+// mnpbem_lu_blockinv_batched on the same factors, per-matrix stride sD): for
+// dtype MNPBEM_DTYPE_C64X the 64-row leaves of the triangular solves are
+// products with those inverses on the bf16x3 kernel; other dtypes ignore Dinv.
+extern "C" int mnpbem_getrs_blockinv_batched(int dtype, int n, int nrhs, const void* A, long lda, long sA,
+                                             const int* ipiv, long sP, const void* Dinv, long sD, void* B, long ldb,
+                                             long sB, int batch, void* stream) {
+  if (dtype != MNP_DTYPE_C64X || Dinv == nullptr)
+    return mnpbem_getrs_batched(dtype, n, nrhs, A, lda, sA, ipiv, sP, B, ldb, sB, batch, stream);
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n >= 0 && nrhs >= 0 && batch >= 0 && lda >= n && ldb >= nrhs, "getrs: bad size");
+  MNP_REQUIRE(sD >= (long)((n + 63) / 64) * 2 * 64 * 64, "getrs: Dinv stride too small");
+  if (n == 0 || nrhs == 0 || batch == 0) return 0;
+  LeafInv<float> inv;
+  inv.D = (const float*)Dinv;
+  inv.sD = sD;
+  int rc;
+  if ((rc = laswp<float>(0, n, 0, nrhs, (float*)B, ldb, sB, ipiv, sP, batch, false, st))) return rc;
+  if ((rc = trsm_rec<float>(true, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st, 1, inv)))
+    return rc;
+  return trsm_rec<float>(false, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st, 1, inv);
+}
+
+extern "C" int mnpbem_getri_blockinv_batched(int dtype, int n, void* A, long lda, long sA, const int* ipiv, long sP,
+                                             const void* Dinv, long sD, void* W, long ldw, long sW, int* Q, long sQ,
+                                             int batch, void* stream) {
+  if (dtype != MNP_DTYPE_C64X || Dinv == nullptr)
+    return mnpbem_getri_batched(dtype, n, A, lda, sA, ipiv, sP, W, ldw, sW, Q, sQ, batch, stream);
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n >= 0 && batch >= 0 && lda >= n && ldw >= n && sQ >= n, "getri: bad size");
+  MNP_REQUIRE(n <= GETRI_MAXN, "getri: n above 32768");
+  MNP_REQUIRE(sD >= (long)((n + 63) / 64) * 2 * 64 * 64, "getri: Dinv stride too small");
+  if (n == 0 || batch == 0) return 0;
+  LeafInv<float> inv;
+  inv.D = (const float*)Dinv;
+  inv.sD = sD;
+  return getri<float>(n, (float*)A, lda, sA, ipiv, sP, (float*)W, ldw, sW, Q, sQ, batch, st, 1, inv);
 }
 
 // Start timing complex-GEMM launches (up to ``capacity`` launches).

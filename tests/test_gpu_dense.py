@@ -448,6 +448,85 @@ def test_getri_matches_inverse(cuda, dtype, n):
             assert err < 3 * err_old + 1e-6, (b, err, err_old)
 
 
+@pytest.mark.parametrize("n,nrhs", [(1, 3), (7, 70), (64, 64), (65, 129), (300, 303), (1031, 200), (2884, 64)])
+def test_getrs_blockinv_leaves(cuda, n, nrhs):
+    """mnpbem_getrs_blockinv_batched: the 64-row leaves as products with the
+    inverse diagonal blocks (bf16x3) against numpy's solve, and no less
+    accurate than the substitution leaves of mnpbem_getrs_batched; the
+    partial last leaf (n mod 64 rows) included.  Other dtypes fall through
+    to the plain entry point (bit-identical)."""
+    nat = _lib()
+    rng = np.random.default_rng(n + nrhs)
+    B = 2
+    A = (_rand(rng, (B, n, n), np.complex64) + (2.0 * np.sqrt(n)) * np.eye(n)[None]).astype(np.complex64)
+    X = _rand(rng, (B, n, nrhs), np.complex64)
+    ref = np.linalg.solve(A.astype(np.complex128), X.astype(np.complex128))
+    tA = torch.from_numpy(A.copy()).to(cuda)
+    piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((B,), dtype=torch.int32, device=cuda)
+    st = nat.stream_handle(cuda)
+    print("getrs_blockinv", n, nrhs, flush=True)
+    nat.call("mnpbem_getrf_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
+    sD = ((n + 63) // 64) * 2 * 64 * 64
+    D = torch.empty((B, sD), dtype=torch.complex64, device=cuda)
+    nat.call("mnpbem_lu_blockinv_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(D), sD, B, st)
+    out = {}
+    for name in ("plain", "blockinv", "fp32"):
+        tX = torch.from_numpy(X.copy()).to(cuda)
+        if name == "plain":
+            nat.call("mnpbem_getrs_batched", nat.DTYPE_C64X, n, nrhs, nat.ptr(tA), n, n * n, nat.ptr(piv), n,
+                     nat.ptr(tX), nrhs, n * nrhs, B, st)
+        else:
+            dt = nat.DTYPE_C64X if name == "blockinv" else nat.DTYPE_C64
+            nat.call("mnpbem_getrs_blockinv_batched", dt, n, nrhs, nat.ptr(tA), n, n * n, nat.ptr(piv), n,
+                     nat.ptr(D), sD, nat.ptr(tX), nrhs, n * nrhs, B, st)
+        out[name] = tX.cpu().numpy()
+    assert np.all(info.cpu().numpy() == 0)
+    scale = np.abs(ref).max()
+    err = np.abs(out["blockinv"] - ref).max() / scale
+    err_plain = np.abs(out["plain"] - ref).max() / scale
+    # a leaf product in bf16x3 is accurate to ~2^-16 of |D| |X| (the
+    # substitution it replaces runs in FP32: exact for n = 1)
+    assert err < 3 * err_plain + 1e-4, (err, err_plain)
+    # complex64 without the split products ignores the inverses
+    tX = torch.from_numpy(X.copy()).to(cuda)
+    nat.call("mnpbem_getrs_batched", nat.DTYPE_C64, n, nrhs, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(tX),
+             nrhs, n * nrhs, B, st)
+    assert np.array_equal(tX.cpu().numpy(), out["fp32"])
+
+
+@pytest.mark.parametrize("n", [65, 300, 1031])
+def test_getri_blockinv_leaves(cuda, n):
+    """mnpbem_getri_blockinv_batched against numpy's inverse, no less
+    accurate than mnpbem_getri_batched on the same factors."""
+    nat = _lib()
+    rng = np.random.default_rng(7 * n)
+    B = 2
+    A = _rand(rng, (B, n, n), np.complex64)
+    tA = torch.from_numpy(A.copy()).to(cuda)
+    piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((B,), dtype=torch.int32, device=cuda)
+    st = nat.stream_handle(cuda)
+    print("getri_blockinv", n, flush=True)
+    nat.call("mnpbem_getrf_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
+    sD = ((n + 63) // 64) * 2 * 64 * 64
+    D = torch.empty((B, sD), dtype=torch.complex64, device=cuda)
+    nat.call("mnpbem_lu_blockinv_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(D), sD, B, st)
+    W = torch.empty_like(tA)
+    Q = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    t1, t2 = tA.clone(), tA.clone()
+    nat.call("mnpbem_getri_blockinv_batched", nat.DTYPE_C64X, n, nat.ptr(t1), n, n * n, nat.ptr(piv), n, nat.ptr(D),
+             sD, nat.ptr(W), n, n * n, nat.ptr(Q), n, B, st)
+    nat.call("mnpbem_getri_batched", nat.DTYPE_C64X, n, nat.ptr(t2), n, n * n, nat.ptr(piv), n, nat.ptr(W), n, n * n,
+             nat.ptr(Q), n, B, st)
+    o1, o2 = t1.cpu().numpy(), t2.cpu().numpy()
+    for b in range(B):
+        ref = np.linalg.inv(A[b].astype(np.complex128))
+        sc = np.abs(ref).max()
+        e1, e2 = np.abs(o1[b] - ref).max() / sc, np.abs(o2[b] - ref).max() / sc
+        assert e1 < 3 * e2 + 1e-4, (b, e1, e2)
+
+
 @pytest.mark.parametrize("dtype,tol,split", [(np.complex128, 1e-12, False), (np.complex64, 2e-5, False),
                                              (np.complex64, 2e-4, True)])
 @pytest.mark.parametrize("n", [1, 7, 33, 64, 65, 100, 300, 1031])

@@ -42,14 +42,16 @@ def test_iter_dense_matches_direct_oracle(sphere284, precision):
     import mnpbem_amd as m
     cp, comp, ref = sphere284
     print("BEMRetIter dense", precision, flush=True)
-    bem = m.BEMRetIter(cp, precision=precision, tol=1e-10, maxit=50)
+    # GMRES at 1e-11: the distance to the direct solution is bounded by the
+    # tolerance times the condition of the system (~100-1000 here)
+    bem = m.BEMRetIter(cp, precision=precision, tol=1e-11, maxit=50)
     exc = m.PlaneWaveRet(*POL2)
     sig, _ = bem.solve(exc.potential(cp, 550.0))
     d = o.BEMRetOracle(ref).solve(o.planewave_potential(comp, *POL2, 550.0))
     for f in ("sig1", "sig2", "h1", "h2"):
         assert _rel(sig.get(f), d[f]) < 1e-8, f
     flag, relres, it = bem.info()
-    assert flag[-1] == 0 and relres[-1] < 1e-10
+    assert flag[-1] == 0 and relres[-1] < 1e-11
     # the direct preconditioner makes GMRES converge in a few iterations
     assert it[-1][1] <= (2 if precision == "double" else 6)
 
