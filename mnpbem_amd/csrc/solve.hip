@@ -28,130 +28,63 @@ namespace {
 
 constexpr int MAXC = 16;  // right-hand-side columns handled per pass
 
-// Y[b] = beta*Y[b] + alpha * M[b]^T X[b]  (M n x n buffer, X, Y n x C).
-// One thread per output row i (a column of the buffer), so every load of M
-// is a coalesced row segment; X rows are staged through LDS.
-template <typename T>
+// Y[b] = beta*Y[b] + alpha * M[b]^T X[b]  (M n x n buffer, X, Y n x C, C <= CM).
+// A workgroup takes 64 output rows i (lane = row: every load of M is a
+// coalesced 512-byte row segment) and its four waves take four quarters of
+// the buffer rows j, partial sums added in wave order (deterministic).  The
+// rows of X are read with wave-uniform loads, not staged through LDS: with
+// LDS staging the workgroup barriers every 64 rows broke the stream of
+// loads (2.9 TB/s at C = 8 over the complex64 matrices of a 2884-row batch
+// of 64, against 4.7 TB/s in this form); 8 row loads in flight per lane.
+template <typename T, int CM>
 __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restrict__ M, long ldm, long sM,
                                                 const T* __restrict__ X, long ldx, long sX, T* __restrict__ Y,
                                                 long ldy, long sY, double alr, double ali, double beta) {
+  constexpr int UNR = 8;
   const long b = blockIdx.y;
   M += 2 * b * sM;
   X += 2 * b * sX;
   Y += 2 * b * sY;
-  const int tid = threadIdx.x;
-  const int i = blockIdx.x * 256 + tid;
-  __shared__ cplx<T> xs[64][MAXC];
-  cplx<T> acc[MAXC];
-#pragma unroll
-  for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
-  for (int j0 = 0; j0 < n; j0 += 64) {
-    const int nj = min(64, n - j0);
-    __syncthreads();
-    for (int e = tid; e < nj * C; e += 256) {
-      const int jj = e / C, c = e % C;
-      xs[jj][c] = cld(X + 2 * ((long)(j0 + jj) * ldx + c));
-    }
-    __syncthreads();
-    if (i < n) {
-      // 16 independent row loads in flight per thread
-      int jj = 0;
-      for (; jj + 16 <= nj; jj += 16) {
-        cplx<T> m[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) m[u] = cld(M + 2 * ((long)(j0 + jj + u) * ldm + i));
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-#pragma unroll
-          for (int c = 0; c < MAXC; ++c)
-            if (c < C) acc[c] = cfma(acc[c], m[u], xs[jj + u][c]);
-      }
-      for (; jj < nj; ++jj) {
-        const cplx<T> m = cld(M + 2 * ((long)(j0 + jj) * ldm + i));
-#pragma unroll
-        for (int c = 0; c < MAXC; ++c)
-          if (c < C) acc[c] = cfma(acc[c], m, xs[jj][c]);
-      }
-    }
-  }
-  if (i < n) {
-#pragma unroll
-    for (int c = 0; c < MAXC; ++c)
-      if (c < C) {
-        T* py = Y + 2 * ((long)i * ldy + c);
-        const double sr = (double)acc[c].re, si = (double)acc[c].im;
-        double yr = alr * sr - ali * si, yi = alr * si + ali * sr;
-        if (beta != 0.0) {
-          const cplx<T> y0 = cld(py);
-          yr += beta * (double)y0.re;
-          yi += beta * (double)y0.im;
-        }
-        cst(py, cmake<T>((T)yr, (T)yi));
-      }
-  }
-}
-
-// The same product for few systems: 64 output rows per workgroup and the
-// buffer rows j split over the four waves (four times the workgroups of
-// k_gemv_t), each wave staging its own X rows in LDS; the wave partial sums
-// are added in wave order (deterministic).
-template <typename T>
-__global__ __launch_bounds__(256) void k_gemv_t_split(int n, int C, const T* __restrict__ M, long ldm, long sM,
-                                                      const T* __restrict__ X, long ldx, long sX, T* __restrict__ Y,
-                                                      long ldy, long sY, double alr, double ali, double beta) {
-  const long b = blockIdx.y;
-  M += 2 * b * sM;
-  X += 2 * b * sX;
-  Y += 2 * b * sY;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int i = blockIdx.x * 64 + lane;
-  const int q = (((n + 3) / 4) + 31) & ~31;  // rows per wave, a multiple of the 32-row chunk
+  const int q = (((n + 3) / 4) + UNR - 1) / UNR * UNR;  // buffer rows per wave
   const int jb = wv * q, je = min(n, jb + q);
-  __shared__ cplx<T> xs[4][32][MAXC];
-  __shared__ cplx<T> part[3][64][MAXC];
-  cplx<T> acc[MAXC];
+  const int ic = min(i, n - 1);
+  __shared__ cplx<T> part[3][64][CM];
+  cplx<T> acc[CM];
 #pragma unroll
-  for (int c = 0; c < MAXC; ++c) acc[c] = cmake<T>(0, 0);
-  for (int t = 0; t < q; t += 32) {
-    const int j0 = jb + t;
-    const int nj = max(0, min(32, je - j0));
-    __syncthreads();
-    for (int e = lane; e < nj * C; e += 64) {
-      const int jj = e / C, c = e % C;
-      xs[wv][jj][c] = cld(X + 2 * ((long)(j0 + jj) * ldx + c));
-    }
-    __syncthreads();
-    if (i < n) {
-      int jj = 0;
-      for (; jj + 16 <= nj; jj += 16) {
-        cplx<T> m[16];
+  for (int c = 0; c < CM; ++c) acc[c] = cmake<T>(0, 0);
+  auto xval = [&](int j, int c) {
+    const T* xr = X + 2 * ((long)j * ldx + c);
+    return c < C ? cmake<T>(xr[0], xr[1]) : cmake<T>(0, 0);
+  };
+  int j = jb;
+  for (; j + UNR <= je; j += UNR) {
+    cplx<T> m[UNR];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) m[u] = cld(M + 2 * ((long)(j0 + jj + u) * ldm + i));
+    for (int u = 0; u < UNR; ++u) m[u] = cld(M + 2 * ((long)(j + u) * ldm + ic));
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
+    for (int u = 0; u < UNR; ++u)
 #pragma unroll
-          for (int c = 0; c < MAXC; ++c)
-            if (c < C) acc[c] = cfma(acc[c], m[u], xs[wv][jj + u][c]);
-      }
-      for (; jj < nj; ++jj) {
-        const cplx<T> m = cld(M + 2 * ((long)(j0 + jj) * ldm + i));
+      for (int c = 0; c < CM; ++c) acc[c] = cfma(acc[c], m[u], xval(j + u, c));
+  }
+  for (; j < je; ++j) {
+    const cplx<T> m = cld(M + 2 * ((long)j * ldm + ic));
 #pragma unroll
-        for (int c = 0; c < MAXC; ++c)
-          if (c < C) acc[c] = cfma(acc[c], m, xs[wv][jj][c]);
-      }
-    }
+    for (int c = 0; c < CM; ++c) acc[c] = cfma(acc[c], m, xval(j, c));
   }
   if (wv > 0)
 #pragma unroll
-    for (int c = 0; c < MAXC; ++c) part[wv - 1][lane][c] = acc[c];
+    for (int c = 0; c < CM; ++c) part[wv - 1][lane][c] = acc[c];
   __syncthreads();
   if (wv > 0 || i >= n) return;
 #pragma unroll
   for (int w = 0; w < 3; ++w)
 #pragma unroll
-    for (int c = 0; c < MAXC; ++c) acc[c] = cadd(acc[c], part[w][lane][c]);
+    for (int c = 0; c < CM; ++c) acc[c] = cadd(acc[c], part[w][lane][c]);
 #pragma unroll
-  for (int c = 0; c < MAXC; ++c)
+  for (int c = 0; c < CM; ++c)
     if (c < C) {
       T* py = Y + 2 * ((long)i * ldy + c);
       const double sr = (double)acc[c].re, si = (double)acc[c].im;
@@ -1056,16 +989,19 @@ __global__ void k_sca_reduce(int ndir, int P, int total, const double* w, const 
 template <typename T>
 int gemv_t(int n, int C, const T* M, long ldm, long sM, const T* X, long ldx, long sX, T* Y, long ldy, long sY,
            double alr, double ali, double beta, int batch, hipStream_t st) {
-  // few systems: split the rows of the buffer over the waves
-  const bool split = (long)mnp_ceil_div(n, 256) * batch < 512;
   for (int c0 = 0; c0 < C; c0 += MAXC) {
     const int cc = std::min(MAXC, C - c0);
-    if (split)
-      hipLaunchKernelGGL(k_gemv_t_split<T>, dim3(mnp_ceil_div(n, 64), batch), dim3(256), 0, st, n, cc, M, ldm, sM,
-                         X + 2 * c0, ldx, sX, Y + 2 * c0, ldy, sY, alr, ali, beta);
+    const dim3 grid(mnp_ceil_div(n, 64), batch);
+#define MNP_GEMV(CM)                                                                                            \
+  hipLaunchKernelGGL((k_gemv_t<T, CM>), grid, dim3(256), 0, st, n, cc, M, ldm, sM, X + 2 * c0, ldx, sX, Y + 2 * c0, \
+                     ldy, sY, alr, ali, beta)
+    if (cc <= 4)
+      MNP_GEMV(4);
+    else if (cc <= 8)
+      MNP_GEMV(8);
     else
-      hipLaunchKernelGGL(k_gemv_t<T>, dim3(mnp_ceil_div(n, 256), batch), dim3(256), 0, st, n, cc, M, ldm, sM,
-                         X + 2 * c0, ldx, sX, Y + 2 * c0, ldy, sY, alr, ali, beta);
+      MNP_GEMV(16);
+#undef MNP_GEMV
     MNP_LAUNCH_CHECK();
   }
   return 0;
