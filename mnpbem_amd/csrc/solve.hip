@@ -302,20 +302,28 @@ __global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const
 // Window step of the same solve with the inverse diagonal blocks: the
 // window's rows [w0, w1) (up to TRSV_WIN rows, 64-row blocks) are solved
 // block by block (block solution = inverse block^T * rhs, then the window's
-// own off-diagonal block applied to the blocks still to solve, all in LDS)
-// by every workgroup, so no grid-wide synchronisation is needed; then each
-// workgroup updates its tile of the rows outside the window (FWD rows >= w1,
-// BWD rows < w0) with all the window's rows.  Fewer launches than one
-// 64-row step each, for the same bytes of the triangle.
-//   TILE 256: one row per thread, window rows summed in order;
+// own off-diagonal block applied to the blocks still to solve, all in LDS);
+// then each workgroup updates its tile of the rows outside the window (FWD
+// rows >= w1, BWD rows < w0) with all the window's rows.  Fewer launches than
+// one 64-row step each, for the same bytes of the triangle.
+//   TILE 256: one row per thread, window rows summed in order.  The tiles are
+//     laid out from the window outwards (FWD upwards from w1, BWD downwards
+//     from w0), so workgroup 0 updates exactly the next window's rows; with
+//     `ahead` it then solves that window once and writes its solution, and
+//     the next launch (`have`) reads the solution instead of every workgroup
+//     solving the window again (the redundant solves took ~40 % of the
+//     time).  Only the first window is solved by every workgroup.
 //   TILE 64: the window rows split over the four waves (four times the
-//     workgroups, for few systems), wave partial sums added in wave order.
-// CM >= C: right-hand-side columns held per row.  Deterministic.
+//     workgroups, for few systems), wave partial sums added in wave order;
+//     every workgroup solves its window.
+// CM >= C: right-hand-side columns held per row.  Deterministic: the window
+// solve runs the same operations in the same order whichever workgroup runs it.
 constexpr int TRSV_WIN = 256;
 template <typename T, bool FWD, int CM, int TILE>
-__global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1, const T* __restrict__ LU, long ld,
-                                                    long sLU, T* V, long ldv, long sV, T* Z, long ldz, long sZ,
-                                                    const T* __restrict__ Dinv, long sD, int half, long hoff) {
+__global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1, int have, int ahead,
+                                                    const T* __restrict__ LU, long ld, long sLU, T* V, long ldv,
+                                                    long sV, T* Z, long ldz, long sZ, const T* __restrict__ Dinv,
+                                                    long sD, int half, long hoff) {
   const long b = blockIdx.y;
   LU += 2 * b * sLU;
   Dinv += 2 * b * sD;
@@ -325,79 +333,94 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
   const int W = w1 - w0;
   __shared__ cplx<T> rs[TRSV_WIN][CM];  // window right-hand side, then its solution
   __shared__ cplx<T> part[TILE == 64 ? 3 : 1][64][CM];
+  T* dst = FWD ? Z : V;
+  const long ldd_ = FWD ? ldz : ldv;
+  // solve the window [s0, s1) whose right-hand side is in rs[0 .. s1 - s0)
+  auto solve_window = [&](int s0, int s1) {
+    const int nbk = (s1 - s0 + 63) / 64;
+    for (int s = 0; s < nbk; ++s) {
+      const int r = FWD ? s : nbk - 1 - s;
+      const int i0 = s0 + 64 * r, nb = min(64, s1 - i0);
+      const T* D = Dinv + 2 * (((long)(i0 / 64) * 2 + (FWD ? 0 : 1)) * 64 * 64);
+      __syncthreads();  // rhs of block r complete
+      // row lane of the block, columns wv, wv + 4, ...
+      cplx<T> sl[CM / 4];
+#pragma unroll
+      for (int v = 0; v < CM / 4; ++v) sl[v] = cmake<T>(0, 0);
+      for (int q0 = 0; q0 < nb; q0 += 16) {
+        cplx<T> d[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          d[u] = (q0 + u < nb && lane < nb) ? cld(D + 2 * ((long)(q0 + u) * 64 + lane)) : cmake<T>(0, 0);
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+#pragma unroll
+          for (int v = 0; v < CM / 4; ++v) sl[v] = cfma(sl[v], d[u], rs[64 * r + q0 + u][wv + 4 * v]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int v = 0; v < CM / 4; ++v) rs[64 * r + lane][wv + 4 * v] = sl[v];
+      __syncthreads();
+      // the window's blocks still to solve: FWD rows [i0 + 64, s1), BWD rows [s0, i0)
+      const int k0 = FWD ? i0 + 64 : s0;
+      const int cnt = FWD ? s1 - k0 : i0 - s0;
+      if (tid < cnt) {
+        const int k = k0 + tid;
+        cplx<T> acc[CM];
+#pragma unroll
+        for (int c = 0; c < CM; ++c) acc[c] = cmake<T>(0, 0);
+        for (int l0 = 0; l0 < nb; l0 += 16) {
+          cplx<T> m[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            m[u] = l0 + u < nb ? cld(LU + 2 * ((long)(i0 + l0 + u) * ld + k)) : cmake<T>(0, 0);
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+#pragma unroll
+            for (int c = 0; c < CM; ++c) acc[c] = cfma(acc[c], m[u], rs[64 * r + l0 + u][c]);
+        }
+#pragma unroll
+        for (int c = 0; c < CM; ++c) rs[k - s0][c] = csub(rs[k - s0][c], acc[c]);
+      }
+    }
+    __syncthreads();
+  };
   {
-    const T* src = FWD ? V : Z;
-    const long lds = FWD ? ldv : ldz;
+    // the window's solution (have) or its right-hand side
+    const T* src = have ? dst : (FWD ? V : Z);
+    const long lds = have ? ldd_ : (FWD ? ldv : ldz);
     for (int e = tid; e < TRSV_WIN * CM; e += 256) {
       const int r = e / CM, c = e % CM;
       rs[r][c] = (r < W && c < C) ? cld(src + 2 * ((long)(w0 + r) * lds + c)) : cmake<T>(0, 0);
     }
   }
-  const int nbk = (W + 63) / 64;
-  for (int s = 0; s < nbk; ++s) {
-    const int r = FWD ? s : nbk - 1 - s;
-    const int i0 = w0 + 64 * r, nb = min(64, w1 - i0);
-    const T* D = Dinv + 2 * (((long)(i0 / 64) * 2 + (FWD ? 0 : 1)) * 64 * 64);
-    __syncthreads();  // rhs of block r complete
-    // row lane of the block, columns wv, wv + 4, ...
-    cplx<T> sl[CM / 4];
-#pragma unroll
-    for (int v = 0; v < CM / 4; ++v) sl[v] = cmake<T>(0, 0);
-    for (int q0 = 0; q0 < nb; q0 += 16) {
-      cplx<T> d[16];
-#pragma unroll
-      for (int u = 0; u < 16; ++u)
-        d[u] = (q0 + u < nb && lane < nb) ? cld(D + 2 * ((long)(q0 + u) * 64 + lane)) : cmake<T>(0, 0);
-#pragma unroll
-      for (int u = 0; u < 16; ++u)
-#pragma unroll
-        for (int v = 0; v < CM / 4; ++v) sl[v] = cfma(sl[v], d[u], rs[64 * r + q0 + u][wv + 4 * v]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int v = 0; v < CM / 4; ++v) rs[64 * r + lane][wv + 4 * v] = sl[v];
-    __syncthreads();
-    // the window's blocks still to solve: FWD rows [i0 + 64, w1), BWD rows [w0, i0)
-    const int k0 = FWD ? i0 + 64 : w0;
-    const int cnt = FWD ? w1 - k0 : i0 - w0;
-    if (tid < cnt) {
-      const int k = k0 + tid;
-      cplx<T> acc[CM];
-#pragma unroll
-      for (int c = 0; c < CM; ++c) acc[c] = cmake<T>(0, 0);
-      for (int l0 = 0; l0 < nb; l0 += 16) {
-        cplx<T> m[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-          m[u] = l0 + u < nb ? cld(LU + 2 * ((long)(i0 + l0 + u) * ld + k)) : cmake<T>(0, 0);
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-#pragma unroll
-          for (int c = 0; c < CM; ++c) acc[c] = cfma(acc[c], m[u], rs[64 * r + l0 + u][c]);
+  if (!have) {
+    solve_window(w0, w1);
+    if (blockIdx.x == 0)
+      for (int e = tid; e < W * C; e += 256) {
+        const int rr = e / C, c = e % C;
+        cst(dst + 2 * ((long)(w0 + rr) * ldd_ + c), rs[rr][c]);
       }
-#pragma unroll
-      for (int c = 0; c < CM; ++c) rs[k - w0][c] = csub(rs[k - w0][c], acc[c]);
-    }
-  }
-  __syncthreads();
-  if (blockIdx.x == 0) {
-    T* dst = FWD ? Z : V;
-    const long ldd_ = FWD ? ldz : ldv;
-    for (int e = tid; e < W * C; e += 256) {
-      const int rr = e / C, c = e % C;
-      cst(dst + 2 * ((long)(w0 + rr) * ldd_ + c), rs[rr][c]);
-    }
+  } else {
+    __syncthreads();
   }
   // tile update with the whole window
   T* upd = FWD ? V : Z;
   const long ldu = FWD ? ldv : ldz;
-  const int kend = FWD ? n : w0;
   cplx<T> acc[CM];
 #pragma unroll
   for (int c = 0; c < CM; ++c) acc[c] = cmake<T>(0, 0);
-  const int k = (FWD ? w1 : 0) + blockIdx.x * TILE + (TILE == 64 ? lane : tid);
+  int k;
+  bool live;
+  if constexpr (TILE == 256) {
+    k = FWD ? w1 + blockIdx.x * 256 + tid : w0 - 256 * (blockIdx.x + 1) + tid;
+    live = FWD ? k < n : k >= 0;
+  } else {
+    k = (FWD ? w1 : 0) + blockIdx.x * 64 + lane;
+    live = k < (FWD ? n : w0);
+  }
   const int la = TILE == 64 ? 64 * wv : 0, lb = TILE == 64 ? min(W, 64 * wv + 64) : W;
-  if (k < kend) {
+  if (live) {
     int l = la;
     for (; l + 16 <= lb; l += 16) {
       cplx<T> m[16];
@@ -414,7 +437,7 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
       for (int c = 0; c < CM; ++c) acc[c] = cfma(acc[c], m, rs[l][c]);
     }
   }
-  if (TILE == 64) {
+  if constexpr (TILE == 64) {
     if (wv > 0)
 #pragma unroll
       for (int c = 0; c < CM; ++c) part[wv - 1][lane][c] = acc[c];
@@ -425,13 +448,35 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
 #pragma unroll
       for (int c = 0; c < CM; ++c) acc[c] = cadd(acc[c], part[q][lane][c]);
   }
-  if (k < kend)
+  cplx<T> nv[CM];
+  if (live)
 #pragma unroll
     for (int c = 0; c < CM; ++c)
       if (c < C) {
         T* pu = upd + 2 * ((long)k * ldu + c);
-        cst(pu, csub(cld(pu), acc[c]));
+        nv[c] = csub(cld(pu), acc[c]);
+        cst(pu, nv[c]);
       }
+  if constexpr (TILE == 256) {
+    // workgroup 0 holds the next window's updated right-hand side: solve it
+    if (ahead && blockIdx.x == 0) {
+      const int s0 = FWD ? w1 : max(0, w0 - 256), s1 = FWD ? min(n, w1 + 256) : w0;
+      if (s1 > s0) {
+        __syncthreads();  // every thread done with the current window in rs
+        const int r = k - s0;
+#pragma unroll
+        for (int c = 0; c < CM; ++c) {
+          if (live && r >= 0 && r < TRSV_WIN) rs[r][c] = c < C ? nv[c] : cmake<T>(0, 0);
+        }
+        // rows of rs beyond the window (the last window of each direction is shorter) stay unused
+        solve_window(s0, s1);
+        for (int e = tid; e < (s1 - s0) * C; e += 256) {
+          const int rr = e / C, c = e % C;
+          cst(dst + 2 * ((long)(s0 + rr) * ldd_ + c), rs[rr][c]);
+        }
+      }
+    }
+  }
 }
 
 // Inverses of the 64 x 64 diagonal blocks of the LU factors (U_II^-1 upper,
@@ -1014,20 +1059,29 @@ template <typename T, int CM, int TILE>
 int trsv_t_windows_cm(int n, int C, const T* LU, long ld, long sLU, T* V, long ldv, long sV, T* Z, long ldz,
                       long sZ, const T* Dinv, long sD, int half, long hoff, int batch, hipStream_t st) {
   const int nblk = (n + 63) / 64, R = TRSV_WIN / 64;
+  // with 256-row tiles every launch after the first reads its window's
+  // solution, written by workgroup 0 of the launch before
+  const int la = TILE == 256 ? 1 : 0;
+  bool first = true;
   for (int b0 = 0; b0 < nblk; b0 += R) {
     const int w0 = 64 * b0, w1 = std::min(n, 64 * (b0 + R));
+    if (!first && la && w1 >= n) break;  // solved by the previous launch, nothing left to update
     dim3 grid(std::max(1, mnp_ceil_div(n - w1, TILE)), batch);
-    hipLaunchKernelGGL((k_trsv_t_win<T, true, CM, TILE>), grid, dim3(256), 0, st, n, C, w0, w1, LU, ld, sLU, V, ldv,
-                       sV, Z, ldz, sZ, Dinv, sD, half, hoff);
+    hipLaunchKernelGGL((k_trsv_t_win<T, true, CM, TILE>), grid, dim3(256), 0, st, n, C, w0, w1, first ? 0 : la, la,
+                       LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
     MNP_LAUNCH_CHECK();
+    first = false;
   }
+  first = true;
   for (int b1 = nblk; b1 > 0; b1 -= R) {
     const int b0 = std::max(0, b1 - R);
     const int w0 = 64 * b0, w1 = std::min(n, 64 * b1);
+    if (!first && la && w0 <= 0) break;
     dim3 grid(std::max(1, mnp_ceil_div(w0, TILE)), batch);
-    hipLaunchKernelGGL((k_trsv_t_win<T, false, CM, TILE>), grid, dim3(256), 0, st, n, C, w0, w1, LU, ld, sLU, V,
-                       ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
+    hipLaunchKernelGGL((k_trsv_t_win<T, false, CM, TILE>), grid, dim3(256), 0, st, n, C, w0, w1, first ? 0 : la, la,
+                       LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
     MNP_LAUNCH_CHECK();
+    first = false;
   }
   return 0;
 }

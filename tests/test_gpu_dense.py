@@ -286,6 +286,36 @@ def test_lu_solve_transposed(cuda, dtype, tol, n, C, blockinv):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 1e-4)])
+@pytest.mark.parametrize("n,C,B", [(700, 8, 200), (1000, 2, 130), (513, 5, 260)])
+def test_lu_solve_transposed_many_systems(cuda, dtype, tol, n, C, B):
+    """Enough systems for the 256-row tiles of the windowed solve, where
+    workgroup 0 of each launch solves the next window once and the next launch
+    reads that solution (look-ahead), against numpy's solve."""
+    nat = _lib()
+    rng = np.random.default_rng(n + C + B)
+    M = (_rand(rng, (B, n, n), dtype) + (2.0 * np.sqrt(n)) * np.eye(n)[None]).astype(dtype)
+    V = _rand(rng, (B, n, C), dtype)
+    ref = np.linalg.solve(M.astype(np.complex128), V.astype(np.complex128))
+    assert -(-n // 256) * B >= 512
+    tA = torch.from_numpy(np.ascontiguousarray(np.transpose(M, (0, 2, 1)))).to(cuda)
+    tV = torch.from_numpy(V.copy()).to(cuda)
+    tZ = torch.empty_like(tV)
+    piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((B,), dtype=torch.int32, device=cuda)
+    dt = nat.DTYPE_C128 if dtype == np.complex128 else nat.DTYPE_C64
+    st = nat.stream_handle(cuda)
+    print("lu_solve_t many", dtype.__name__, n, C, B, flush=True)
+    nat.call("mnpbem_getrf_batched", dt, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
+    sD = ((n + 63) // 64) * 2 * 64 * 64
+    D = torch.empty((B, sD), dtype=tA.dtype, device=cuda)
+    nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(tA), n, n * n, nat.ptr(D), sD, B, st)
+    nat.call("mnpbem_lu_solve_t_batched", dt, n, C, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(tV), C, n * C,
+             nat.ptr(tZ), C, n * C, nat.ptr(D), sD, B, st)
+    err = np.abs(tV.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < tol, err
+
+
 @pytest.mark.parametrize("dtype,n,tol", [(np.complex64, 14000, 2e-4), (np.complex128, 13000, 1e-11),
                                          (np.complex64, 6500, 1e-4)])
 def test_getrf_tall_panels(cuda, dtype, n, tol):
