@@ -67,6 +67,24 @@ def cpu_baseline(lams, pol, dr, seconds_budget=30.0):
     return np.array(done), np.array(ext), len(done) / dt, t_setup
 
 
+def cpu_config1():
+    """The FP64 numpy/scipy oracle on the reference's own CPU case in full:
+    trisphere(144, 20) = 284 curved faces, Drude gold, x polarisation along
+    +z, 41 wavelengths 400-800 nm (complex128).  Returns wavelengths/s and
+    the seconds taken (one-off set-up not counted)."""
+    import oracle as o
+    nodes, tri6 = o.trisphere_mesh(144, 20.0)
+    comp = o.Composite([o.EpsConst(1.0), o.EpsDrude(*DRUDE)], [o.CurvedSurface(nodes, tri6)], [[2, 1]])
+    bem = o.BEMRetOracle(o.GreenSetup(comp))
+    lams = np.linspace(400.0, 800.0, 41)
+    t0 = time.time()
+    for lam in lams:
+        exc = o.planewave_potential(comp, [[1.0, 0.0, 0.0]], [[0.0, 0.0, 1.0]], lam)
+        o.extinction(comp, bem.solve(exc), [[1.0, 0.0, 0.0]], [[0.0, 0.0, 1.0]], lam)
+    dt = time.time() - t0
+    return len(lams) / dt, dt
+
+
 def blas_threads():
     for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "MKL_NUM_THREADS"):
         if os.environ.get(k):
@@ -99,6 +117,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the complex128 comparison sweep")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the mixed-fp32 sweep (complex64 GEMMs on FP32 matrix cores) timed beside the headline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device with the library's plain copy kernel "
                          "(k_copy_cast<double>), the reference point that scales HBM byte counters")
@@ -230,6 +250,25 @@ def main():
         log("fp64 plain: %.2f wavelengths/s; mixed vs fp64 max rel ext err %.2e" % (fp64, acc_vs_fp64))
         del bem64
 
+    # the same mixed path with its complex64 GEMMs on FP32 matrix cores
+    # instead of the bf16 split products, timed eagerly beside the headline
+    fp32 = None
+    if args.precision == "mixed" and not args.no_fp32:
+        bem32 = m.BEMRet(cp, precision="mixed-fp32", refine_tol=args.refine_tol, residual=args.residual,
+                         green=bem.g)
+        sweep(bem32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ext32 = sweep(bem32)
+        torch.cuda.synchronize()
+        t32 = time.perf_counter() - t0
+        fp32 = {"value": round(len(lams) / t32, 4), "unit": "wavelengths/s", "execution": "eager launches",
+                "refinement_solves": bem32.last_refine_iters,
+                "max_rel_ext_err_vs_fp64_gpu": (None if fp64 is None
+                                                else float(np.max(np.abs(ext32 / ext64 - 1))))}
+        log("mixed-fp32: %.2f wavelengths/s (%s refinement solves)" % (fp32["value"], bem32.last_refine_iters))
+        del bem32
+
     cpu = None
     acc_vs_oracle = None
     if not args.no_cpu:
@@ -245,6 +284,12 @@ def main():
                          % (len(done), ",".join("%.1f" % l for l in done), N, cores, t_setup)}
         log("cpu oracle: %.3f wavelengths/s on %d threads; mixed vs oracle max rel ext err %.2e"
             % (cpu_rate, cores, acc_vs_oracle))
+        c1_rate, c1_s = cpu_config1()
+        cpu["config1"] = {"value": round(c1_rate, 3), "unit": "wavelengths/s", "seconds": round(c1_s, 2),
+                          "sample": "the reference's CPU case in full: trisphere(144, 20) = 284 faces, Drude gold, "
+                                    "x-pol +z, 41 wavelengths, same oracle and threads (the reference publishes "
+                                    "6.683 s for this mesh with tabulated gold on its EPYC host)"}
+        log("cpu oracle, 284 faces x 41 wavelengths: %.2f s (%.2f wavelengths/s)" % (c1_s, c1_rate))
 
     # HBM bytes per launch of the dominant kernel: FETCH_SIZE from the counter
     # pass summarised in the traffic file; writes from its WRITE_SIZE line, or,
@@ -290,6 +335,7 @@ def main():
                    "faces": N, "wavelengths": len(lams), "rhs": 2, "chunk": args.chunk,
                    "gemm_tiles": gemm_tiles, "residual": args.residual},
         "fp64_value": None if fp64 is None else round(fp64, 4),
+        "mixed_fp32": fp32,
         "speedup_vs_fp64": None if fp64 is None else round(value / fp64, 3),
         "max_rel_ext_err_vs_fp64_gpu": acc_vs_fp64,
         "max_rel_ext_err_vs_cpu_oracle": acc_vs_oracle,
