@@ -1353,6 +1353,14 @@ int getrf_rec(int N, int c0, int n, T* A, long lda, long sA, int* ipiv, long sP,
 // launches per W columns against W / 8 for the pivoting leaf.  The buffer
 // holds NP_MAXB matrices (the entry point runs larger batches in chunks);
 // concurrent factorisations on one device would share it.
+//
+// At the bottom nodes of the recursion (W < n <= 2W columns: a panel of W and
+// one of nx = n - W) the left panel's kernels also do the node's triangular
+// solve and update: k_lu_np_diag solves U12 = L11^-1 A12 (the nx columns to
+// the right of its block, in place) and k_lu_np_rows, after forming its rows
+// of L21, subtracts L21 U12 from the same rows' nx columns.  That replaces a
+// triangular leaf launch and a thin product launch per node, and L21 is used
+// from LDS instead of being read back.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct NpW;
@@ -1400,7 +1408,8 @@ __device__ __forceinline__ cplx<T> crecip_np(cplx<T> a) {
 }
 
 template <typename T, int W>
-__global__ __launch_bounds__(256) void k_lu_np_diag(int c0, int w, T* A, long lda, long sA, int* info, int* flag) {
+__global__ __launch_bounds__(256) void k_lu_np_diag(int c0, int w, T* A, long lda, long sA, int* info, int* flag,
+                                                    int nx) {
   constexpr int CPT = W / 4;
   const long b = blockIdx.x;
   A += 2 * b * sA;
@@ -1498,6 +1507,25 @@ __global__ __launch_bounds__(256) void k_lu_np_diag(int c0, int w, T* A, long ld
 #pragma unroll
       for (int m = 0; m < CPT; ++m) cst(Xg + 2 * ((long)(q + 4 * m) * W + jc), xr[m]);
   }
+  // U12 = L11^-1 A12 for the nx columns right of the block (unit lower L11 in
+  // U below the diagonal): the columns in LDS, one thread per column
+  if (nx > 0) {
+    __shared__ cplx<T> X12[W][W + 1];
+    for (int e = tid; e < W * W; e += 256) {
+      const int r = e / W, c = e % W;
+      X12[r][c] = (r < w && c < nx) ? cld(A + 2 * ((long)(c0 + r) * lda + c0 + w + c)) : cmake<T>(0, 0);
+    }
+    __syncthreads();
+    if (tid < nx) {
+#pragma unroll 1
+      for (int r = 1; r < w; ++r) {
+        cplx<T> s = X12[r][tid];
+        for (int k = 0; k < r; ++k) s = cfms(s, U[r][k], X12[k][tid]);
+        X12[r][tid] = s;
+      }
+      for (int r = 0; r < w; ++r) cst(A + 2 * ((long)(c0 + r) * lda + c0 + w + tid), X12[r][tid]);
+    }
+  }
   if (lev) atomicMax(flag + b, lev);
 }
 
@@ -1506,7 +1534,8 @@ __global__ __launch_bounds__(256) void k_lu_np_diag(int c0, int w, T* A, long ld
 // upper), lane (row group, column group) 4 rows x W/16 columns, every output
 // summed over k in order
 template <typename T, int W>
-__global__ __launch_bounds__(256) void k_lu_np_rows(int N, int c0, int w, T* A, long lda, long sA, int* flag) {
+__global__ __launch_bounds__(256) void k_lu_np_rows(int N, int c0, int w, T* A, long lda, long sA, int* flag,
+                                                    int nx) {
   constexpr int CPW = W / 4, CW = CPW / 4, RW = 4;
   const long b = blockIdx.y;
   A += 2 * b * sA;
@@ -1555,6 +1584,50 @@ __global__ __launch_bounds__(256) void k_lu_np_rows(int N, int c0, int w, T* A, 
         }
       }
   }
+  if (nx > 0) {
+    // A22 -= L21 U12 on this workgroup's rows and the nx columns right of the
+    // panel (U12 solved in place by k_lu_np_diag); every output summed over k
+    // in order
+    __syncthreads();  // every wave done with As and Xs
+#pragma unroll
+    for (int ii = 0; ii < RW; ++ii)
+#pragma unroll
+      for (int j = 0; j < CW; ++j) As[RW * rg + ii][jb + j] = acc[ii][j];
+    for (int e = tid; e < W * W; e += 256) {
+      const int k = e / W, c = e % W;
+      Xs[k][c] = (k < w && c < nx) ? cld(A + 2 * ((long)(c0 + k) * lda + c0 + w + c)) : cmake<T>(0, 0);
+    }
+    __syncthreads();
+    cplx<T> up[RW][CW];
+#pragma unroll
+    for (int ii = 0; ii < RW; ++ii)
+#pragma unroll
+      for (int j = 0; j < CW; ++j) up[ii][j] = cmake<T>(0, 0);
+    for (int k = 0; k < w; ++k) {
+      cplx<T> av[RW], xv[CW];
+#pragma unroll
+      for (int ii = 0; ii < RW; ++ii) av[ii] = As[RW * rg + ii][k];
+#pragma unroll
+      for (int j = 0; j < CW; ++j) xv[j] = Xs[k][jb + j];
+#pragma unroll
+      for (int ii = 0; ii < RW; ++ii)
+#pragma unroll
+        for (int j = 0; j < CW; ++j) up[ii][j] = cfma(up[ii][j], av[ii], xv[j]);
+    }
+#pragma unroll
+    for (int ii = 0; ii < RW; ++ii) {
+      const int r = RW * rg + ii;
+      if (r < nr)
+#pragma unroll
+        for (int j = 0; j < CW; ++j) {
+          const int c = jb + j;
+          if (c < nx) {
+            T* pa = A + 2 * ((long)(r0 + r) * lda + c0 + w + c);
+            cst(pa, csub(cld(pa), up[ii][j]));
+          }
+        }
+    }
+  }
   if (lev) atomicMax(flag + b, lev);
 }
 
@@ -1569,14 +1642,18 @@ template <typename T>
 int getrf_np_rec(int N, int c0, int n, T* A, long lda, long sA, int* info, int* flag, int batch, hipStream_t st,
                  int gmode) {
   constexpr int W = NpW<T>::w;
-  if (n <= W) {
-    hipLaunchKernelGGL((k_lu_np_diag<T, W>), dim3(batch), dim3(256), 0, st, c0, n, A, lda, sA, info, flag);
+  if (n <= 2 * W) {
+    // a panel of up to W columns; with n > W the left panel of W columns
+    // also solves and updates the nx = n - W columns of the right one
+    const int w = n <= W ? n : W, nx = n - w;
+    hipLaunchKernelGGL((k_lu_np_diag<T, W>), dim3(batch), dim3(256), 0, st, c0, w, A, lda, sA, info, flag, nx);
     MNP_LAUNCH_CHECK();
-    if (N - c0 - n > 0) {
-      dim3 grid(mnp_ceil_div(N - c0 - n, 64), batch);
-      hipLaunchKernelGGL((k_lu_np_rows<T, W>), grid, dim3(256), 0, st, N, c0, n, A, lda, sA, flag);
+    if (N - c0 - w > 0) {
+      dim3 grid(mnp_ceil_div(N - c0 - w, 64), batch);
+      hipLaunchKernelGGL((k_lu_np_rows<T, W>), grid, dim3(256), 0, st, N, c0, w, A, lda, sA, flag, nx);
       MNP_LAUNCH_CHECK();
     }
+    if (nx > 0) return getrf_np_rec(N, c0 + w, nx, A, lda, sA, info, flag, batch, st, gmode);
     return 0;
   }
   int n1 = ((n / 2 + W - 1) / W) * W;
