@@ -106,17 +106,27 @@ int mnpbem_getri_batched(int dtype, int n, void* A, long lda, long sA, const int
                          long sW, int* Q, long sQ, int batch, void* stream);
 
 /* mnpbem_getrs_batched and mnpbem_getri_batched given the inverse diagonal
- * blocks of the factors (Dinv from mnpbem_lu_blockinv_batched on the same
- * factors, per-matrix stride sD).  For MNPBEM_DTYPE_C64X the 64-row leaves of
- * the triangular solves become products with those inverses on the bf16x3
- * kernel instead of substitutions; every other dtype, or Dinv = NULL, runs the
- * plain entry point.  Same contract as those (bem_ret.py:437-523). */
+ * blocks of the factors, bs = 64 rows (Dinv from mnpbem_lu_blockinv_batched)
+ * or 128 rows (from mnpbem_lu_blockinv128_batched), per-matrix stride sD.  For
+ * MNPBEM_DTYPE_C64X the bs-row leaves of the triangular solves become products
+ * with those inverses on the bf16x3 kernel instead of substitutions; every
+ * other dtype, or Dinv = NULL, runs the plain entry point.  Same contract as
+ * those (bem_ret.py:437-523). */
 int mnpbem_getrs_blockinv_batched(int dtype, int n, int nrhs, const void* A, long lda, long sA, const int* ipiv,
-                                  long spiv, const void* Dinv, long sD, void* B, long ldb, long sB, int batch,
+                                  long spiv, const void* Dinv, long sD, int bs, void* B, long ldb, long sB, int batch,
                                   void* stream);
 int mnpbem_getri_blockinv_batched(int dtype, int n, void* A, long lda, long sA, const int* ipiv, long spiv,
-                                  const void* Dinv, long sD, void* W, long ldw, long sW, int* Q, long sQ, int batch,
-                                  void* stream);
+                                  const void* Dinv, long sD, int bs, void* W, long ldw, long sW, int* Q, long sQ,
+                                  int batch, void* stream);
+
+/* Inverses of the 128 x 128 diagonal blocks of complex64 LU factors (as
+ * [U^-1 | L^-1] per block, two 128 x 128 row-major blocks) from the 64 x 64
+ * ones of mnpbem_lu_blockinv_batched (D64) and the factors' off-diagonal
+ * 64 x 64 blocks; for the 128-row leaves of the blockinv solves above.
+ * Replaces nothing in the reference (its inverses are whole matrices,
+ * bem_ret.py:437-523). */
+int mnpbem_lu_blockinv128_batched(int dtype, int n, const void* LU, long ld, long sLU, const void* D64, long sD64,
+                                  void* D128, long sD128, int batch, void* stream);
 
 /* Y[b] = beta Y[b] + alpha M[b]^T X[b] for n x C blocks X, Y (any C). */
 int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, long ldm, long sM, const void* X, long ldx,

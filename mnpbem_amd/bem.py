@@ -197,8 +197,16 @@ class BEMRet:
         # LU(G^T), then Sigma^T = (G^T)^-1 H^T for both sides in one batch of 2B
         lu_gd(GG, piv, info, npflag, 2 * B)
         nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(GG), n, nn, nat.ptr(F.DG), F.sD, 2 * B, st)
-        nat.call("mnpbem_getrs_blockinv_batched", dt, n, n, nat.ptr(GG), n, nn, nat.ptr(piv), n, nat.ptr(F.DG),
-                 F.sD, nat.ptr(HH), n, nn, 2 * B, st)
+        # 128-row inverse blocks for the leaves of the many-column solves
+        # (complex64 on the split products only; other dtypes ignore them)
+        s128 = ((n + 127) // 128) * 2 * 128 * 128
+        D128 = None
+        if dt == nat.DTYPE_C64X:
+            D128 = torch.empty((2 * B, s128), dtype=cdt, device=dev)
+            nat.call("mnpbem_lu_blockinv128_batched", dt, n, nat.ptr(GG), n, nn, nat.ptr(F.DG), F.sD,
+                     nat.ptr(D128), s128, 2 * B, st)
+        nat.call("mnpbem_getrs_blockinv_batched", dt, n, n, nat.ptr(GG), n, nn, nat.ptr(piv), n, nat.ptr(D128),
+                 s128, 128, nat.ptr(HH), n, nn, 2 * B, st)
         S1, S2 = HH[0], HH[1]
         Dt = torch.empty((B, n, n), dtype=cdt, device=dev)
         gd = getattr(g, "gdiv", 0)
@@ -208,12 +216,16 @@ class BEMRet:
         # (Delta^-1)^T in place of the factors (LAPACK getri's operation count);
         # F.DS holds the inverse diagonal blocks of Delta's factors until Sigma's
         nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(F.DS), F.sD, B, st)
+        if D128 is not None:
+            D128 = D128[:B]
+            nat.call("mnpbem_lu_blockinv128_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(F.DS), F.sD,
+                     nat.ptr(D128), s128, B, st)
         Wk = torch.empty((B, n, n), dtype=cdt, device=dev)
         Q = torch.empty((B, n), dtype=torch.int32, device=dev)
-        nat.call("mnpbem_getri_blockinv_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(F.DS),
-                 F.sD, nat.ptr(Wk), n, nn, nat.ptr(Q), n, B, st)
+        nat.call("mnpbem_getri_blockinv_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(D128),
+                 s128, 128, nat.ptr(Wk), n, nn, nat.ptr(Q), n, B, st)
         Et = Dt
-        del Wk, Q
+        del Wk, Q, D128
         # Sigma^T overwrites Sigma2^T
         nat.call("mnpbem_sigma_batched", dt, 1, n, nat.ptr(S1), nat.ptr(S2), None, nat.ptr(Et), n, nn,
                  nat.ptr(g.nvec), nat.ptr(F.par), gd, B, st)

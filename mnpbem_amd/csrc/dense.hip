@@ -895,6 +895,10 @@ int launch_bf16x3(int variant, int M, int N, int K, const float* A, long lda, lo
   return 0;
 }
 
+// X <- D X in place for n <= 128 rows on the 128 x 128 tile (one row tile), timed like gemm_sub
+int launch_bf16x3_assign128(int n, int nrhs, const float* D, long ldd, long sD, float* X, long ldx, long sX,
+                            int batch, hipStream_t st);
+
 // gmode 1 (complex64 only): bf16 split-product GEMM
 // assign = 1 (bf16x3 kernel only): C = A B
 template <typename T>
@@ -941,6 +945,23 @@ int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, lon
   return 0;
 }
 
+int launch_bf16x3_assign128(int n, int nrhs, const float* D, long ldd, long sD, float* X, long ldx, long sX,
+                            int batch, hipStream_t st) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return 0;
+  const bool timed = g_prof.on && g_prof.used + 2 <= g_prof.ev.size();
+  if (timed) MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used], st));
+  int rc = launch_bf16x3(0, n, nrhs, n, D, ldd, sD, X, ldx, sX, X, ldx, sX, batch, st, 1);
+  if (rc) return rc;
+  if (timed) {
+    MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used + 1], st));
+    g_prof.used += 2;
+    g_prof.flops.push_back(8.0 * n * (double)nrhs * n * batch);
+    g_prof.bytes += (double)batch * 8.0 * ((double)n * n + 2.0 * n * nrhs);
+    g_prof.cbytes += (double)batch * 8.0 * ((double)n * nrhs);
+  }
+  return 0;
+}
+
 // Columns (right-hand sides x batch) up to which a triangular leaf of more
 // than half the leaf size runs k_trsm_leaf_rows instead of one thread per
 // column (set by mnpbem_trsm_rows_max; 0 disables).  Measured on the leaves
@@ -959,9 +980,10 @@ struct LeafInv {
   const T* D = nullptr;
   long sD = 0;
   int blk = 0;
+  int bs = 64;  // block rows: 64 (k_lu_blockinv layout) or 128 (k_blockinv128)
   LeafInv at(int rows) const {
     LeafInv r = *this;
-    r.blk += rows / 64;
+    r.blk += rows / bs;
     return r;
   }
 };
@@ -978,10 +1000,14 @@ int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, 
              hipStream_t st, int gmode, LeafInv<T> inv = LeafInv<T>()) {
   if (n <= 0 || nrhs <= 0) return 0;
   constexpr int TL = TLeaf<T>::n;
+  const bool prod = sizeof(T) == 4 && inv.D != nullptr && gmode == 1;
   if constexpr (sizeof(T) == 4) {
-    if (n <= 64 && inv.D != nullptr && gmode == 1) {
-      const T* D = inv.D + 2 * ((long)inv.blk * 2 * 64 * 64 + (lower ? 64 * 64 : 0));
-      return gemm_sub<T>(n, nrhs, n, D, 64, inv.sD, X, ldx, sX, X, ldx, sX, batch, st, gmode, 1);
+    if (prod && n <= inv.bs) {
+      const int bs = inv.bs;
+      const T* D = inv.D + 2 * ((long)inv.blk * 2 * bs * bs + (lower ? bs * bs : 0));
+      // in place: the tile must cover all n rows (the 128-row tile above 64)
+      if (n > 64) return launch_bf16x3_assign128(n, nrhs, D, bs, inv.sD, X, ldx, sX, batch, st);
+      return gemm_sub<T>(n, nrhs, n, D, bs, inv.sD, X, ldx, sX, X, ldx, sX, batch, st, gmode, 1);
     }
   }
   if (n <= TL) {
@@ -1029,8 +1055,10 @@ int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, 
 #undef MNP_TRSM_LEAF
     return 0;
   }
-  int n1 = ((n / 2 + TL - 1) / TL) * TL;
-  if (n1 >= n) n1 = n - TL;
+  // split at a multiple of the leaf size from the node's first row
+  const int SB = prod ? inv.bs : TL;
+  int n1 = ((n / 2 + SB - 1) / SB) * SB;
+  if (n1 >= n) n1 = n - SB;
   const int n2 = n - n1;
   int rc;
   if (lower) {
@@ -1682,9 +1710,10 @@ template <typename T>
 int trsm_lower_ident(int n, const T* L, long ldl, long sL, T* X, long ldx, long sX, int batch, hipStream_t st,
                      int gmode, LeafInv<T> inv = LeafInv<T>()) {
   constexpr int TL = TLeaf<T>::n;
-  if (n <= 2 * TL) return trsm_rec<T>(true, n, n, L, ldl, sL, X, ldx, sX, batch, st, gmode, inv);
-  int n1 = ((n / 2 + TL - 1) / TL) * TL;
-  if (n1 >= n) n1 = n - TL;
+  const int SB = (sizeof(T) == 4 && inv.D != nullptr && gmode == 1) ? inv.bs : TL;
+  if (n <= 2 * SB) return trsm_rec<T>(true, n, n, L, ldl, sL, X, ldx, sX, batch, st, gmode, inv);
+  int n1 = ((n / 2 + SB - 1) / SB) * SB;
+  if (n1 >= n) n1 = n - SB;
   const int n2 = n - n1;
   int rc;
   if ((rc = trsm_lower_ident<T>(n1, L, ldl, sL, X, ldx, sX, batch, st, gmode, inv))) return rc;
@@ -1941,17 +1970,19 @@ extern "C" int mnpbem_getri_batched(int dtype, int n, void* A, long lda, long sA
 // dtype MNPBEM_DTYPE_C64X the 64-row leaves of the triangular solves are
 // products with those inverses on the bf16x3 kernel; other dtypes ignore Dinv.
 extern "C" int mnpbem_getrs_blockinv_batched(int dtype, int n, int nrhs, const void* A, long lda, long sA,
-                                             const int* ipiv, long sP, const void* Dinv, long sD, void* B, long ldb,
-                                             long sB, int batch, void* stream) {
+                                             const int* ipiv, long sP, const void* Dinv, long sD, int bs, void* B,
+                                             long ldb, long sB, int batch, void* stream) {
   if (dtype != MNP_DTYPE_C64X || Dinv == nullptr)
     return mnpbem_getrs_batched(dtype, n, nrhs, A, lda, sA, ipiv, sP, B, ldb, sB, batch, stream);
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n >= 0 && nrhs >= 0 && batch >= 0 && lda >= n && ldb >= nrhs, "getrs: bad size");
-  MNP_REQUIRE(sD >= (long)((n + 63) / 64) * 2 * 64 * 64, "getrs: Dinv stride too small");
+  MNP_REQUIRE(bs == 64 || bs == 128, "getrs: block size must be 64 or 128");
+  MNP_REQUIRE(sD >= (long)((n + bs - 1) / bs) * 2 * bs * bs, "getrs: Dinv stride too small");
   if (n == 0 || nrhs == 0 || batch == 0) return 0;
   LeafInv<float> inv;
   inv.D = (const float*)Dinv;
   inv.sD = sD;
+  inv.bs = bs;
   int rc;
   if ((rc = laswp<float>(0, n, 0, nrhs, (float*)B, ldb, sB, ipiv, sP, batch, false, st))) return rc;
   if ((rc = trsm_rec<float>(true, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st, 1, inv)))
@@ -1960,19 +1991,117 @@ extern "C" int mnpbem_getrs_blockinv_batched(int dtype, int n, int nrhs, const v
 }
 
 extern "C" int mnpbem_getri_blockinv_batched(int dtype, int n, void* A, long lda, long sA, const int* ipiv, long sP,
-                                             const void* Dinv, long sD, void* W, long ldw, long sW, int* Q, long sQ,
-                                             int batch, void* stream) {
+                                             const void* Dinv, long sD, int bs, void* W, long ldw, long sW, int* Q,
+                                             long sQ, int batch, void* stream) {
   if (dtype != MNP_DTYPE_C64X || Dinv == nullptr)
     return mnpbem_getri_batched(dtype, n, A, lda, sA, ipiv, sP, W, ldw, sW, Q, sQ, batch, stream);
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n >= 0 && batch >= 0 && lda >= n && ldw >= n && sQ >= n, "getri: bad size");
   MNP_REQUIRE(n <= GETRI_MAXN, "getri: n above 32768");
-  MNP_REQUIRE(sD >= (long)((n + 63) / 64) * 2 * 64 * 64, "getri: Dinv stride too small");
+  MNP_REQUIRE(bs == 64 || bs == 128, "getri: block size must be 64 or 128");
+  MNP_REQUIRE(sD >= (long)((n + bs - 1) / bs) * 2 * bs * bs, "getri: Dinv stride too small");
   if (n == 0 || batch == 0) return 0;
   LeafInv<float> inv;
   inv.D = (const float*)Dinv;
   inv.sD = sD;
+  inv.bs = bs;
   return getri<float>(n, (float*)A, lda, sA, ipiv, sP, (float*)W, ldw, sW, Q, sQ, batch, st, 1, inv);
+}
+
+// Inverses of the 128 x 128 diagonal blocks of LU factors, from those of their
+// 64 x 64 halves (k_lu_blockinv in solve.hip) and the off-diagonal 64 x 64
+// blocks of the factors:
+//   L^-1 = [[L0^-1, 0], [-L1^-1 L10 L0^-1, L1^-1]],
+//   U^-1 = [[U0^-1, -U0^-1 U01 U1^-1], [0, U1^-1]].
+// Output per matrix and 128-row block: [U^-1 | L^-1], two 128 x 128 row-major
+// blocks, zero outside the triangle (and beyond n).  One workgroup per
+// (block, matrix); the products in FP32 through LDS, each output summed over
+// k in order.
+namespace {
+__device__ __forceinline__ void mm64(const cplx<float> (&S1)[64][65], const cplx<float> (&S2)[64][65],
+                                     cplx<float> (&acc)[16], int r, int cq) {
+#pragma unroll
+  for (int m = 0; m < 16; ++m) acc[m] = cmake<float>(0.f, 0.f);
+  for (int k = 0; k < 64; ++k) {
+    const cplx<float> a = S1[r][k];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) acc[m] = cfma(acc[m], a, S2[k][cq + 4 * m]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_blockinv128(int n, const float* __restrict__ LU, long ld, long sLU,
+                                                     const float* __restrict__ D64, long sD64, float* D128,
+                                                     long sD128) {
+  const long b = blockIdx.y;
+  const int p = blockIdx.x, i0 = 128 * p;
+  const int nb = min(128, n - i0), n1 = max(0, nb - 64);  // rows of the block and of its second half
+  LU += 2 * b * sLU;
+  const float* d0 = D64 + 2 * (b * sD64 + (long)(2 * p) * 2 * 64 * 64);  // [U0^-1 | L0^-1]
+  const float* d1 = d0 + 2 * 2 * 64 * 64;                                // [U1^-1 | L1^-1] (if n1 > 0)
+  float* o = D128 + 2 * (b * sD128 + (long)p * 2 * 128 * 128);
+  __shared__ cplx<float> S1[64][65], S2[64][65], S3[64][65];
+  const int tid = threadIdx.x, r = tid >> 2, cq = tid & 3;
+  // diagonal halves and the zero blocks (U: lower left, L: upper right)
+  for (int tri = 0; tri < 2; ++tri)
+    for (int e = tid; e < 128 * 128; e += 256) {
+      const int rr = e / 128, cc = e % 128;
+      const bool lo = rr >= 64, le = cc < 64;
+      if ((tri == 0 && !lo && !le) || (tri == 1 && lo && le)) continue;  // computed below
+      cplx<float> v = cmake<float>(0.f, 0.f);
+      if (!lo && le)
+        v = cld(d0 + 2 * ((long)tri * 64 * 64 + rr * 64 + cc));
+      else if (lo && !le && n1 > 0)
+        v = cld(d1 + 2 * ((long)tri * 64 * 64 + (rr - 64) * 64 + (cc - 64)));
+      cst(o + 2 * ((long)tri * 128 * 128 + rr * 128 + cc), v);
+    }
+  cplx<float> acc[16];
+  for (int tri = 0; tri < 2; ++tri) {
+    // tri 1 (L): X = -L1^-1 (L10 L0^-1); tri 0 (U): Y = -U0^-1 (U01 U1^-1)
+    __syncthreads();
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int rr = e / 64, cc = e % 64;
+      if (tri == 1) {
+        S1[rr][cc] = rr < n1 ? cld(LU + 2 * ((long)(i0 + 64 + rr) * ld + i0 + cc)) : cmake<float>(0.f, 0.f);
+        S2[rr][cc] = cld(d0 + 2 * ((long)64 * 64 + rr * 64 + cc));
+      } else {
+        S1[rr][cc] = cc < n1 ? cld(LU + 2 * ((long)(i0 + rr) * ld + i0 + 64 + cc)) : cmake<float>(0.f, 0.f);
+        S2[rr][cc] = n1 > 0 ? cld(d1 + 2 * ((long)rr * 64 + cc)) : cmake<float>(0.f, 0.f);
+      }
+    }
+    __syncthreads();
+    mm64(S1, S2, acc, r, cq);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) S3[r][cq + 4 * m] = acc[m];
+    __syncthreads();
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int rr = e / 64, cc = e % 64;
+      if (tri == 1)
+        S1[rr][cc] = n1 > 0 ? cld(d1 + 2 * ((long)64 * 64 + rr * 64 + cc)) : cmake<float>(0.f, 0.f);
+      else
+        S1[rr][cc] = cld(d0 + 2 * ((long)rr * 64 + cc));
+    }
+    __syncthreads();
+    mm64(S1, S3, acc, r, cq);
+    const int ro = tri == 1 ? 64 : 0, co = tri == 1 ? 0 : 64;
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+      cst(o + 2 * ((long)tri * 128 * 128 + (ro + r) * 128 + co + cq + 4 * m),
+          cmake<float>(-acc[m].re, -acc[m].im));
+  }
+}
+}  // namespace
+
+extern "C" int mnpbem_lu_blockinv128_batched(int dtype, int n, const void* LU, long ld, long sLU, const void* D64,
+                                             long sD64, void* D128, long sD128, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(mnp_is_c64(dtype), "lu_blockinv128: complex64 factors only");
+  if (n <= 0 || batch <= 0) return 0;
+  MNP_REQUIRE(sD64 >= (long)((n + 63) / 64) * 2 * 64 * 64, "lu_blockinv128: D64 stride too small");
+  MNP_REQUIRE(sD128 >= (long)((n + 127) / 128) * 2 * 128 * 128, "lu_blockinv128: D128 stride too small");
+  hipLaunchKernelGGL(k_blockinv128, dim3((n + 127) / 128, batch), dim3(256), 0, st, n, (const float*)LU, ld, sLU,
+                     (const float*)D64, sD64, (float*)D128, sD128);
+  MNP_LAUNCH_CHECK();
+  return 0;
 }
 
 // Start timing complex-GEMM launches (up to ``capacity`` launches).

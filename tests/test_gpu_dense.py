@@ -478,8 +478,10 @@ def test_getri_matches_inverse(cuda, dtype, n):
             assert err < 3 * err_old + 1e-6, (b, err, err_old)
 
 
-@pytest.mark.parametrize("n,nrhs", [(1, 3), (7, 70), (64, 64), (65, 129), (300, 303), (1031, 200), (2884, 64)])
-def test_getrs_blockinv_leaves(cuda, n, nrhs):
+@pytest.mark.parametrize("bs", [64, 128])
+@pytest.mark.parametrize("n,nrhs", [(1, 3), (7, 70), (64, 64), (65, 129), (130, 40), (300, 303), (1031, 200),
+                                    (2884, 64)])
+def test_getrs_blockinv_leaves(cuda, n, nrhs, bs):
     """mnpbem_getrs_blockinv_batched: the 64-row leaves as products with the
     inverse diagonal blocks (bf16x3) against numpy's solve, and no less
     accurate than the substitution leaves of mnpbem_getrs_batched; the
@@ -497,9 +499,7 @@ def test_getrs_blockinv_leaves(cuda, n, nrhs):
     st = nat.stream_handle(cuda)
     print("getrs_blockinv", n, nrhs, flush=True)
     nat.call("mnpbem_getrf_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
-    sD = ((n + 63) // 64) * 2 * 64 * 64
-    D = torch.empty((B, sD), dtype=torch.complex64, device=cuda)
-    nat.call("mnpbem_lu_blockinv_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(D), sD, B, st)
+    D, sD = _block_inverses(nat, tA, n, B, bs, st)
     out = {}
     for name in ("plain", "blockinv", "fp32"):
         tX = torch.from_numpy(X.copy()).to(cuda)
@@ -509,7 +509,7 @@ def test_getrs_blockinv_leaves(cuda, n, nrhs):
         else:
             dt = nat.DTYPE_C64X if name == "blockinv" else nat.DTYPE_C64
             nat.call("mnpbem_getrs_blockinv_batched", dt, n, nrhs, nat.ptr(tA), n, n * n, nat.ptr(piv), n,
-                     nat.ptr(D), sD, nat.ptr(tX), nrhs, n * nrhs, B, st)
+                     nat.ptr(D), sD, bs, nat.ptr(tX), nrhs, n * nrhs, B, st)
         out[name] = tX.cpu().numpy()
     assert np.all(info.cpu().numpy() == 0)
     scale = np.abs(ref).max()
@@ -525,8 +525,56 @@ def test_getrs_blockinv_leaves(cuda, n, nrhs):
     assert np.array_equal(tX.cpu().numpy(), out["fp32"])
 
 
+def _block_inverses(nat, tA, n, B, bs, st):
+    """Inverse diagonal blocks of the factors in tA: 64 rows from
+    mnpbem_lu_blockinv_batched, 128 rows built from those by
+    mnpbem_lu_blockinv128_batched."""
+    s64 = ((n + 63) // 64) * 2 * 64 * 64
+    D64 = torch.empty((B, s64), dtype=torch.complex64, device=tA.device)
+    nat.call("mnpbem_lu_blockinv_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(D64), s64, B, st)
+    if bs == 64:
+        return D64, s64
+    s128 = ((n + 127) // 128) * 2 * 128 * 128
+    D128 = torch.empty((B, s128), dtype=torch.complex64, device=tA.device)
+    nat.call("mnpbem_lu_blockinv128_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(D64), s64,
+             nat.ptr(D128), s128, B, st)
+    return D128, s128
+
+
+@pytest.mark.parametrize("n", [60, 128, 200, 300])
+def test_lu_blockinv128_matches_numpy(cuda, n):
+    """The 128-row inverse blocks against numpy's inverses of the unit-lower
+    and upper triangles of each 128 x 128 diagonal block (the last one partial),
+    zero outside the triangle."""
+    nat = _lib()
+    rng = np.random.default_rng(3 * n)
+    B = 2
+    A = (_rand(rng, (B, n, n), np.complex64) + (2.0 * np.sqrt(n)) * np.eye(n)[None]).astype(np.complex64)
+    tA = torch.from_numpy(A.copy()).to(cuda)
+    piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((B,), dtype=torch.int32, device=cuda)
+    st = nat.stream_handle(cuda)
+    print("blockinv128", n, flush=True)
+    nat.call("mnpbem_getrf_batched", nat.DTYPE_C64, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
+    D, sD = _block_inverses(nat, tA, n, B, 128, st)
+    LU = tA.cpu().numpy().astype(np.complex128)
+    out = D.cpu().numpy().reshape(B, -1, 2, 128, 128)
+    for b in range(B):
+        for p in range((n + 127) // 128):
+            i0, nb = 128 * p, min(128, n - 128 * p)
+            blk = LU[b, i0:i0 + nb, i0:i0 + nb]
+            U = np.triu(blk)
+            L = np.tril(blk, -1) + np.eye(nb)
+            for tri, M in ((0, U), (1, L)):
+                ref = np.zeros((128, 128), np.complex128)
+                ref[:nb, :nb] = np.linalg.inv(M)
+                err = np.abs(out[b, p, tri] - ref).max() / np.abs(ref).max()
+                assert err < 1e-5, (b, p, tri, err)
+
+
+@pytest.mark.parametrize("bs", [64, 128])
 @pytest.mark.parametrize("n", [65, 300, 1031])
-def test_getri_blockinv_leaves(cuda, n):
+def test_getri_blockinv_leaves(cuda, n, bs):
     """mnpbem_getri_blockinv_batched against numpy's inverse, no less
     accurate than mnpbem_getri_batched on the same factors."""
     nat = _lib()
@@ -539,14 +587,12 @@ def test_getri_blockinv_leaves(cuda, n):
     st = nat.stream_handle(cuda)
     print("getri_blockinv", n, flush=True)
     nat.call("mnpbem_getrf_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
-    sD = ((n + 63) // 64) * 2 * 64 * 64
-    D = torch.empty((B, sD), dtype=torch.complex64, device=cuda)
-    nat.call("mnpbem_lu_blockinv_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(D), sD, B, st)
+    D, sD = _block_inverses(nat, tA, n, B, bs, st)
     W = torch.empty_like(tA)
     Q = torch.empty((B, n), dtype=torch.int32, device=cuda)
     t1, t2 = tA.clone(), tA.clone()
     nat.call("mnpbem_getri_blockinv_batched", nat.DTYPE_C64X, n, nat.ptr(t1), n, n * n, nat.ptr(piv), n, nat.ptr(D),
-             sD, nat.ptr(W), n, n * n, nat.ptr(Q), n, B, st)
+             sD, bs, nat.ptr(W), n, n * n, nat.ptr(Q), n, B, st)
     nat.call("mnpbem_getri_batched", nat.DTYPE_C64X, n, nat.ptr(t2), n, n * n, nat.ptr(piv), n, nat.ptr(W), n, n * n,
              nat.ptr(Q), n, B, st)
     o1, o2 = t1.cpu().numpy(), t2.cpu().numpy()
