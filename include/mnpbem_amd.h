@@ -503,6 +503,23 @@ int mnpbem_hmat_matvec(int dtype, int n, int nblk, int nleaf, int maxchunks, con
                        const int* lst, const int* perm, const void* val, const void* U, const void* V, int ncol,
                        const void* X, long ldx, void* Y, long ldy, int accumulate, void* T, void* XW,
                        void* stream);
+/* Packed product layouts (also replaces HMatrix.mtimes_vec, hmatrix.py:450-541,
+ * through mnpbem_hmat_matvec_packed): copies the blocks into column panels
+ * (VS, tix: the V factors of the low-rank blocks sharing a column cluster,
+ * side by side and transposed) and row panels (P, gidx: per leaf row cluster
+ * and 64-row group, the rows of every block over it side by side); ncp / nrp
+ * pieces of 8 int64 each, described in csrc/hmat.hip above k_hmat_pack_cols. */
+int mnpbem_hmat_pack(int dtype, int ncp, const long* cdesc, int nrp, const long* rdesc, const void* val,
+                     const void* U, const void* V, void* VS, int* tix, void* P, int* gidx, void* stream);
+/* Y (+)= M X on the packed layouts (replaces HMatrix.mtimes_vec,
+ * hmatrix.py:450-541): ncitem column-panel items {vs, K, o0, n, c0, og} and
+ * nritem row-panel items {p, mi, K, g, r0, 0} (int64); X, Y complex128
+ * (n x ldx / ldy, face order), 1 <= ncol <= 16; R: complex128 work space of
+ * (n + sum of ranks) * 16 elements. */
+int mnpbem_hmat_matvec_packed(int dtype, int n, int ncitem, const long* citems, const void* VS, const int* tix,
+                              int nritem, const long* ritems, const void* P, const int* gidx, const int* perm,
+                              int ncol, const void* X, long ldx, void* Y, long ldy, int accumulate, void* R,
+                              void* stream);
 /* Out = M as an n x n complex128 row-major matrix (face order, leading dimension ldo) */
 int mnpbem_hmat_full(int dtype, int nblk, const long* blk, const int* perm, const void* val, const void* U,
                      const void* V, void* Out, long ldo, void* stream);

@@ -83,6 +83,8 @@ SIGNATURES = {
     "mnpbem_hmat_dense_green": [_i, _i, _p, _p, _p, _i, _d, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p],
     "mnpbem_hmat_compact": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p],
     "mnpbem_hmat_matvec": [_i, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p, _l, _p, _l, _i, _p, _p, _p],
+    "mnpbem_hmat_pack": [_i, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p],
+    "mnpbem_hmat_matvec_packed": [_i, _i, _i, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p, _l, _p, _l, _i, _p, _p],
     "mnpbem_hmat_full": [_i, _i, _p, _p, _p, _p, _p, _p, _l, _p],
     "mnpbem_hmat_truncate": [_i, _i, _p, _p, _p, _p, _p, _i, _d, _p, _p, _p, _p, _p],
     "mnpbem_hmat_trunc_form": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p],

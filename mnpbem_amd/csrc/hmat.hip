@@ -450,6 +450,209 @@ __global__ __launch_bounds__(64 * RowsCfg<NC>::nw) void k_hmat_rows(const long* 
   }
 }
 
+// Packed product layouts.  The block storage above serves the block
+// operations (densify, recompression, sums); for products the blocks are
+// also copied once into two packed layouts, so that each phase of a product
+// streams long contiguous panels instead of walking 50-100 blocks per leaf
+// with a descriptor, a right-hand-side fetch and a few column loads each:
+//  * column panels (phase 1): the low-rank blocks that share a column
+//    cluster [c0, c0 + n) with their V factors side by side, transposed:
+//    VS[vs + j K + o] = V_b[j][r] for the panel's output o = (block b, rank
+//    column r), o < K; output o is row tix[og + o] of the workspace R.
+//  * row panels (phase 2): per leaf row cluster and 64-row group, the rows of
+//    every block over it side by side, column-major with leading dimension
+//    mi: P[p + q mi + i]; column q multiplies row gidx[g + q] of R.
+// R: (n + sum of ranks) x ncol complex128 rows: X in cluster order in rows
+// [0, n), the t = V^T x values of block b in rows n + off_t + r; allocated
+// with 16 columns per row, so that the kernels may read a full bucket of
+// columns past the last row.
+//
+// Pack pieces.  Column piece (one per low-rank block, 8 int64):
+//   {dst, K, n, ob, rank, tdst, trow, 0}:
+//   VS[dst + j K + r] = V[ob + r n + j], tix[tdst + r] = trow + r.
+// Row piece (one per row group and block over it, 8 int64):
+//   {dst, mi, ncols, src, ld, dense, gdst, gstart}:
+//   P[dst + q mi + i] = (dense ? val : U)[src + q ld + i],
+//   gidx[gdst + q] = gstart + q.
+template <typename TS>
+__global__ __launch_bounds__(256) void k_hmat_pack_cols(const long* desc, const TS* V, TS* VS, int* tix) {
+  const long* d = desc + 8 * (long)blockIdx.x;
+  const long dst = d[0], K = d[1], n = d[2], ob = d[3], rank = d[4], tdst = d[5], trow = d[6];
+  for (long e = threadIdx.x; e < n * rank; e += blockDim.x) {
+    const long r = e / n, j = e - r * n;
+    cst(VS + 2 * (dst + j * K + r), cld(V + 2 * (ob + e)));
+  }
+  for (long r = threadIdx.x; r < rank; r += blockDim.x) tix[tdst + r] = (int)(trow + r);
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_hmat_pack_rows(const long* desc, const TS* val, const TS* U, TS* P,
+                                                        int* gidx) {
+  const long* d = desc + 8 * (long)blockIdx.x;
+  const long dst = d[0], mi = d[1], nc = d[2], src = d[3], ld = d[4], dense = d[5], gdst = d[6], gstart = d[7];
+  const TS* S = dense ? val : U;
+  for (long e = threadIdx.x; e < nc * mi; e += blockDim.x) {
+    const long q = e / mi, i = e - q * mi;
+    cst(P + 2 * (dst + e), cld(S + 2 * (src + q * ld + i)));
+  }
+  for (long q = threadIdx.x; q < nc; q += blockDim.x) gidx[gdst + q] = (int)(gstart + q);
+}
+
+// Sum of the NW waves' accumulators for column c, in wave order, into
+// wave 0 (deterministic); red is NW x 64 complex128 of LDS.
+template <int NW>
+__device__ __forceinline__ cplx<double> wave_order_sum(cplx<double> (*red)[64], cplx<double> a, int wv, int lane) {
+  red[wv][lane] = a;
+  __syncthreads();
+  cplx<double> s = red[0][lane];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) s = cadd(s, red[w][lane]);
+  __syncthreads();
+  return s;
+}
+
+// phase 1 on column panels: one workgroup per (panel, 64 outputs); lane =
+// output (every load of VS is a coalesced row segment), the NW waves take
+// consecutive parts of the panel's n rows, x rows of R by wave-uniform loads,
+// 8 panel loads in flight per lane; writes the outputs' t rows of R.
+template <typename TS, int NC, int NW, int UNR>
+__global__ __launch_bounds__(64 * NW) void k_hmat_vt_pk(const long* items, const TS* __restrict__ VS,
+                                                        const int* __restrict__ tix, const double* __restrict__ RX,
+                                                        int ncol, double* __restrict__ RT) {
+  const long* it = items + 6 * (long)blockIdx.x;
+  const long vs = it[0], K = it[1], o0 = it[2], n = it[3], c0 = it[4], og = it[5];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const long o = o0 + lane;
+  const long oc = o < K ? o : K - 1;
+  const long q = ((n + NW - 1) / NW + UNR - 1) / UNR * UNR;
+  const long jb = wv * q, je = jb + q < n ? jb + q : n;
+  const double* xb = RX + 2 * c0 * ncol;
+  cplx<double> acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = cmake(0.0, 0.0);
+  // columns c >= ncol read the next row's values (in bounds: R holds 16
+  // columns per row) into accumulators that are never written
+  auto xval = [&](long j, int c) { return ld2(xb + 2 * (j * ncol + c)); };
+  long j = jb;
+  for (; j + UNR <= je; j += UNR) {
+    cplx<double> m[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) m[u] = ldw(VS + 2 * (vs + (j + u) * K + oc));
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] = cfma(acc[c], m[u], xval(j + u, c));
+  }
+  for (; j < je; ++j) {
+    const cplx<double> m = ldw(VS + 2 * (vs + j * K + oc));
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = cfma(acc[c], m, xval(j, c));
+  }
+  __shared__ cplx<double> red[NW][64];
+  const long row = o < K ? tix[og + o] : 0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (c < ncol) {
+      const cplx<double> s = wave_order_sum<NW>(red, acc[c], wv, lane);
+      if (wv == 0 && o < K) {
+        double* t = RT + 2 * (row * ncol + c);
+        t[0] = s.re;
+        t[1] = s.im;
+      }
+    }
+  }
+}
+
+// phase 2 on row panels: one workgroup per (leaf row cluster, 64-row
+// group); lane = row, the NW waves take consecutive parts of the panel's K
+// columns; the right-hand-side row of column q (gidx, then the row of R) by
+// wave-uniform loads, 8 panel loads in flight per lane.  Row i of the group
+// is written once, to Y[perm[r0 + i]] (no atomics, deterministic).
+template <typename TS, int NC, int NW, int UNR>
+__global__ __launch_bounds__(64 * NW) void k_hmat_rows_pk(const long* items, const TS* __restrict__ P,
+                                                          const int* __restrict__ gidx, const int* perm,
+                                                          const double* __restrict__ R, int ncol,
+                                                          double* __restrict__ Y, long ldy, int accumulate) {
+  const long* it = items + 6 * (long)blockIdx.x;
+  const long p = it[0], mi = it[1], K = it[2], g = it[3], r0 = it[4];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const long ic = lane < mi ? lane : mi - 1;
+  const long q = ((K + NW - 1) / NW + UNR - 1) / UNR * UNR;
+  const long qb = wv * q, qe = qb + q < K ? qb + q : K;
+  const int* gx = gidx + g;
+  cplx<double> acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = cmake(0.0, 0.0);
+  // unconditional loads as in k_hmat_vt_pk (R holds 16 columns per row)
+  auto xval = [&](long qq, int c) { return ld2(R + 2 * ((long)gx[qq] * ncol + c)); };
+  long k = qb;
+  for (; k + UNR <= qe; k += UNR) {
+    cplx<double> m[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) m[u] = ldw(P + 2 * (p + (k + u) * mi + ic));
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] = cfma(acc[c], m[u], xval(k + u, c));
+  }
+  for (; k < qe; ++k) {
+    const cplx<double> m = ldw(P + 2 * (p + k * mi + ic));
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = cfma(acc[c], m, xval(k, c));
+  }
+  __shared__ cplx<double> red[NW][64];
+  double* y = Y + 2 * ((long)perm[r0 + ic] * ldy);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (c < ncol) {
+      cplx<double> s = wave_order_sum<NW>(red, acc[c], wv, lane);
+      if (wv == 0 && lane < mi) {
+        if (accumulate) s = cadd(s, ld2(y + 2 * c));
+        y[2 * c] = s.re;
+        y[2 * c + 1] = s.im;
+      }
+    }
+  }
+}
+
+// waves per workgroup and panel loads in flight per lane of the two product
+// kernels (measured on the 25 000-face dimer's H-matrices: 8 / 8 for both;
+// 16 loads in flight or 16 waves per row-panel workgroup were slower)
+#ifndef MNP_HM_NW1
+#define MNP_HM_NW1 8
+#endif
+#ifndef MNP_HM_NW2
+#define MNP_HM_NW2 8
+#endif
+#ifndef MNP_HM_UNR1
+#define MNP_HM_UNR1 8
+#endif
+#ifndef MNP_HM_UNR2
+#define MNP_HM_UNR2 8
+#endif
+template <typename TS, int NC>
+int matvec_packed_launch(int n, int ncitem, const long* citems, const void* VS, const int* tix, int nritem,
+                         const long* ritems, const void* P, const int* gidx, const int* perm, int ncol,
+                         const double* X, long ldx, double* Y, long ldy, int accumulate, double* R, hipStream_t st) {
+  constexpr int NW1 = MNP_HM_NW1;
+  constexpr int NW2 = MNP_HM_NW2;
+  constexpr int U1 = MNP_HM_UNR1, U2 = MNP_HM_UNR2;
+  hipLaunchKernelGGL(k_hmat_xperm, dim3(mnp_ceil_div((long)n * ncol, 256)), dim3(256), 0, st, n, ncol, perm, X, ldx,
+                     R);
+  MNP_LAUNCH_CHECK();
+  if (ncitem > 0) {
+    hipLaunchKernelGGL((k_hmat_vt_pk<TS, NC, NW1, U1>), dim3(ncitem), dim3(64 * NW1), 0, st, citems, (const TS*)VS, tix,
+                       (const double*)R, ncol, R);
+    MNP_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL((k_hmat_rows_pk<TS, NC, NW2, U2>), dim3(nritem), dim3(64 * NW2), 0, st, ritems, (const TS*)P, gidx,
+                     perm, (const double*)R, ncol, Y, ldy, accumulate);
+  MNP_LAUNCH_CHECK();
+  return 0;
+}
+
 // densify: Out[perm[r0+i]][perm[c0+j]] (row-major, ldo) for every block
 template <typename TS>
 __global__ __launch_bounds__(256) void k_hmat_full(const long* blk, const int* perm, const TS* val, const TS* Uv,
@@ -951,6 +1154,72 @@ extern "C" int mnpbem_hmat_matvec(int dtype, int n, int nblk, int nleaf, int max
   }
 #undef MNP_MV
   mnp_set_error("hmat_matvec: dtype must be 1 or 2");
+  return 1;
+}
+
+// Copy the blocks into the packed product layouts (the layouts and piece
+// descriptors are described above k_hmat_pack_cols): ncp column pieces
+// building VS and tix, nrp row pieces building P and gidx.
+extern "C" int mnpbem_hmat_pack(int dtype, int ncp, const long* cdesc, int nrp, const long* rdesc, const void* val,
+                                const void* U, const void* V, void* VS, int* tix, void* P, int* gidx, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(ncp >= 0 && nrp >= 0, "hmat_pack: negative piece count");
+  if (dtype != MNP_DTYPE_C128 && dtype != MNP_DTYPE_C64) {
+    mnp_set_error("hmat_pack: dtype must be 1 or 2");
+    return 1;
+  }
+#define MNP_PK(TS)                                                                                               \
+  if (ncp > 0) {                                                                                                 \
+    hipLaunchKernelGGL(k_hmat_pack_cols<TS>, dim3(ncp), dim3(256), 0, st, cdesc, (const TS*)V, (TS*)VS, tix);    \
+    MNP_LAUNCH_CHECK();                                                                                          \
+  }                                                                                                              \
+  if (nrp > 0) {                                                                                                 \
+    hipLaunchKernelGGL(k_hmat_pack_rows<TS>, dim3(nrp), dim3(256), 0, st, rdesc, (const TS*)val, (const TS*)U,   \
+                       (TS*)P, gidx);                                                                            \
+    MNP_LAUNCH_CHECK();                                                                                          \
+  }
+  if (dtype == MNP_DTYPE_C128) {
+    MNP_PK(double)
+  } else {
+    MNP_PK(float)
+  }
+#undef MNP_PK
+  return 0;
+}
+
+// Y (+)= M X on the packed layouts: phase 0 writes X in cluster order into
+// rows [0, n) of the workspace R (rows of ncol values; (n + sum of ranks)
+// * 16 complex128 elements allocated),
+// phase 1 the t rows of R from the ncitem column-panel items (6 int64:
+// {vs, K, o0, n, c0, og}), phase 2 Y from the nritem row-panel items
+// (6 int64: {p, mi, K, g, r0, 0}).  X, Y: (n, ldx / ldy) complex128
+// row-major in face order, 1 <= ncol <= 16.
+extern "C" int mnpbem_hmat_matvec_packed(int dtype, int n, int ncitem, const long* citems, const void* VS,
+                                         const int* tix, int nritem, const long* ritems, const void* P,
+                                         const int* gidx, const int* perm, int ncol, const void* X, long ldx, void* Y,
+                                         long ldy, int accumulate, void* R, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(ncol >= 1 && ncol <= 16 && ldx >= ncol && ldy >= ncol, "hmat_matvec_packed: 1 <= ncol <= 16");
+  MNP_REQUIRE(n >= 1 && nritem >= 1 && ncitem >= 0, "hmat_matvec_packed: bad item tables");
+  MNP_REQUIRE(R != nullptr, "hmat_matvec_packed: R workspace required");
+  const double* x = (const double*)X;
+  double* y = (double*)Y;
+  double* r = (double*)R;
+#define MNP_MV(TS, NC)                                                                                            \
+  return matvec_packed_launch<TS, NC>(n, ncitem, citems, VS, tix, nritem, ritems, P, gidx, perm, ncol, x, ldx, y, \
+                                      ldy, accumulate, r, st)
+  if (dtype == MNP_DTYPE_C128) {
+    if (ncol <= 4) MNP_MV(double, 4);
+    if (ncol <= 8) MNP_MV(double, 8);
+    MNP_MV(double, 16);
+  }
+  if (dtype == MNP_DTYPE_C64) {
+    if (ncol <= 4) MNP_MV(float, 4);
+    if (ncol <= 8) MNP_MV(float, 8);
+    MNP_MV(float, 16);
+  }
+#undef MNP_MV
+  mnp_set_error("hmat_matvec_packed: dtype must be 1 or 2");
   return 1;
 }
 

@@ -168,6 +168,91 @@ class ClusterTree:
         return out
 
 
+def pack_tables(blk, nd, n, leafs, lst):
+    """Piece and item tables of the packed product layouts for a block table
+    ``blk`` (int64, 8 per block: r0, m, c0, n, rank, off_a, off_b, off_t;
+    the first ``nd`` dense) and leaf table ``leafs`` / ``lst``:
+
+    * column panels: the low-rank blocks of rank > 0 that share a column
+      cluster, their V factors side by side and transposed,
+      VS[vs + j K + o] = V_b[j][r] for output o = (b, r) of the panel, whose
+      t value goes to workspace row tix[og + o] = n + off_t + r.  Pieces
+      (one per block) {dst, K, n, off_b, rank, tdst, trow, 0}; items (one per
+      64 outputs of a panel) {vs, K, o0, n, c0, og}.
+    * row panels: per leaf row cluster and 64-row group (mi rows), the rows
+      of every block over it side by side, column-major,
+      P[p + q mi + i]; column q multiplies workspace row gidx[g + q] (X row
+      c0 + j of a dense block, t row n + off_t + r of a low-rank one).
+      Pieces (one per group and block) {dst, mi, ncols, src, ld, dense,
+      gdst, gstart}; items {p, mi, K, g, r0, 0}.
+
+    Returns (cdesc, citems, rdesc, ritems, (len VS, len tix, len P, len gidx))."""
+    i64 = np.int64
+    lr = np.nonzero(blk[nd:, 4] > 0)[0] + nd
+    cdesc = np.zeros((0, 8), i64)
+    citems = np.zeros((0, 6), i64)
+    nvs = ntix = 0
+    if len(lr):
+        lr = lr[np.argsort(blk[lr, 2] * (n + 1) + blk[lr, 3], kind="stable")]
+        key = blk[lr, 2] * (n + 1) + blk[lr, 3]
+        k = blk[lr, 4]
+        newg = np.r_[True, key[1:] != key[:-1]]
+        gid = np.cumsum(newg) - 1
+        gs = np.nonzero(newg)[0]
+        Kg = np.add.reduceat(k, gs)
+        nG, c0G = blk[lr[gs], 3], blk[lr[gs], 2]
+        vsG = np.r_[0, np.cumsum(nG * Kg)[:-1]].astype(i64)
+        ogG = np.r_[0, np.cumsum(Kg)[:-1]].astype(i64)
+        kx = np.cumsum(k) - k
+        ooff = kx - kx[gs][gid]
+        cdesc = np.zeros((len(lr), 8), i64)
+        cdesc[:, 0] = vsG[gid] + ooff
+        cdesc[:, 1] = Kg[gid]
+        cdesc[:, 2] = blk[lr, 3]
+        cdesc[:, 3] = blk[lr, 6]
+        cdesc[:, 4] = k
+        cdesc[:, 5] = ogG[gid] + ooff
+        cdesc[:, 6] = n + blk[lr, 7]
+        nsl = (Kg + 63) // 64
+        gi = np.repeat(np.arange(len(gs)), nsl)
+        sl = np.arange(int(nsl.sum())) - np.repeat(np.cumsum(nsl) - nsl, nsl)
+        citems = np.stack([vsG[gi], Kg[gi], 64 * sl, nG[gi], c0G[gi], ogG[gi]], axis=1).astype(i64)
+        # longest panels first: the workgroups start in item order, so the
+        # long items do not form the tail of the launch
+        citems = citems[np.argsort(-citems[:, 3], kind="stable")]
+        nvs, ntix = int((nG * Kg).sum()), int(Kg.sum())
+    ngr = (leafs[:, 1] + 63) // 64
+    li = np.repeat(np.arange(len(leafs)), ngr)
+    gg = np.arange(int(ngr.sum())) - np.repeat(np.cumsum(ngr) - ngr, ngr)
+    ir0 = leafs[li, 0] + 64 * gg
+    imi = np.minimum(64, leafs[li, 1] - 64 * gg)
+    cnt = leafs[li, 3] - leafs[li, 2]
+    pi = np.repeat(np.arange(len(li)), cnt)
+    pe = np.repeat(leafs[li, 2], cnt) + (np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+    pb = np.asarray(lst)[pe].astype(i64)
+    keep = blk[pb, 4] != 0
+    pi, pb = pi[keep], pb[keep]
+    rk = blk[pb, 4]
+    dense = rk < 0
+    ncols = np.where(dense, blk[pb, 3], rk)
+    K = np.bincount(pi, weights=ncols, minlength=len(li)).astype(i64)
+    gofs = np.r_[0, np.cumsum(K)[:-1]].astype(i64)
+    pcum = np.cumsum(ncols) - ncols
+    pbase = np.r_[0, np.cumsum(imi * K)[:-1]].astype(i64)
+    rdesc = np.zeros((len(pb), 8), i64)
+    rdesc[:, 0] = pbase[pi] + (pcum - gofs[pi]) * imi[pi]
+    rdesc[:, 1] = imi[pi]
+    rdesc[:, 2] = ncols
+    rdesc[:, 3] = blk[pb, 5] + (ir0[pi] - blk[pb, 0])
+    rdesc[:, 4] = blk[pb, 1]
+    rdesc[:, 5] = dense
+    rdesc[:, 6] = pcum
+    rdesc[:, 7] = np.where(dense, blk[pb, 2], n + blk[pb, 7])
+    ritems = np.stack([pbase, imi, K, gofs, ir0, np.zeros_like(K)], axis=1).astype(i64)
+    ritems = ritems[np.argsort(-(imi * K), kind="stable")]
+    return cdesc, citems, rdesc, ritems, (nvs, ntix, int((imi * K).sum()), int(K.sum()))
+
+
 class HMatrix:
     """H-matrix over a ClusterTree with device-resident blocks.
 
@@ -192,6 +277,7 @@ class HMatrix:
         self.ranks = np.zeros(0, dtype=np.int64)
         self._built = False
         self._lu = None
+        self._pk = None
         if tree is not None and partition is not None:
             # the (dense, low-rank) block lists of another H-matrix over the
             # same tree and admissibility
@@ -246,6 +332,7 @@ class HMatrix:
         leafs = np.stack([starts, t.cind[lv, 1] - starts + 1, ptr[:-1], ptr[1:]], axis=1).astype(np.int64)
         self._blk_host = blk
         self._leafs_host = leafs
+        self._lst_host = lst
         self._nd, self._nl = nd, nl
         self._maxchunks = int((leafs[:, 1].max() + 255) // 256) if len(leafs) else 1
         dev = self.device
@@ -365,6 +452,7 @@ class HMatrix:
         self._T = torch.empty((max(self._rsum, 1) * 16,), dtype=torch.complex128, device=dev)
         self._built = True
         self._lu = None
+        self._pk = None
         return self
 
     @staticmethod
@@ -376,9 +464,45 @@ class HMatrix:
         if not self._built:
             raise RuntimeError("HMatrix: blocks not filled (call aca first)")
 
+    def _pack(self):
+        """Packed product layouts, built on the first product after the
+        blocks change (see pack_tables)."""
+        n = self.tree.n
+        dev = self.device
+        tdt, code = _DT[self.dtype]
+        cdesc, citems, rdesc, ritems, (nvs, ntix, nP, ng) = pack_tables(self._blk_host, self._nd, n,
+                                                                       self._leafs_host, self._lst_host)
+        if max(n + self._rsum, ng, ntix) >= 2 ** 31:
+            raise ValueError("HMatrix: product tables above 2^31 entries")
+        pk = {"VS": torch.empty((max(nvs, 1),), dtype=tdt, device=dev),
+              "tix": torch.empty((max(ntix, 1),), dtype=torch.int32, device=dev),
+              "P": torch.empty((max(nP, 1),), dtype=tdt, device=dev),
+              "gidx": torch.empty((max(ng, 1),), dtype=torch.int32, device=dev),
+              "citems": torch.as_tensor(citems).to(dev), "ritems": torch.as_tensor(ritems).to(dev),
+              "nc": len(citems), "nr": len(ritems),
+              "R": torch.empty(((n + self._rsum) * 16,), dtype=torch.complex128, device=dev)}
+        cd, rd = torch.as_tensor(cdesc).to(dev), torch.as_tensor(rdesc).to(dev)
+        nat.call("mnpbem_hmat_pack", code, len(cdesc), nat.ptr(cd), len(rdesc), nat.ptr(rd), nat.ptr(self._val),
+                 nat.ptr(self._U), nat.ptr(self._V), nat.ptr(pk["VS"]), nat.ptr(pk["tix"]), nat.ptr(pk["P"]),
+                 nat.ptr(pk["gidx"]), nat.stream_handle(dev))
+        pk["_desc"] = (cd, rd)  # kept with the layouts: the pack launches read them asynchronously
+        self._pk = pk
+        return pk
+
     def matvec_device(self, X, ldx, Y, ldy, ncol, accumulate=False, stream=None):
         """Y (+)= M X for complex128 device buffers in face order, (n, ldx) and
-        (n, ldy) row-major, ncol <= 16 columns."""
+        (n, ldy) row-major, ncol <= 16 columns (on the packed layouts)."""
+        self._check()
+        st = nat.stream_handle(self.device) if stream is None else stream
+        pk = self._pk if self._pk is not None else self._pack()
+        nat.call("mnpbem_hmat_matvec_packed", _DT[self.dtype][1], self.tree.n, pk["nc"], nat.ptr(pk["citems"]),
+                 nat.ptr(pk["VS"]), nat.ptr(pk["tix"]), pk["nr"], nat.ptr(pk["ritems"]), nat.ptr(pk["P"]),
+                 nat.ptr(pk["gidx"]), nat.ptr(self._perm), ncol, X, ldx, Y, ldy, 1 if accumulate else 0,
+                 nat.ptr(pk["R"]), st)
+
+    def matvec_blocks_device(self, X, ldx, Y, ldy, ncol, accumulate=False, stream=None):
+        """As matvec_device, on the block storage itself (one workgroup per
+        leaf row cluster walking its blocks)."""
         self._check()
         st = nat.stream_handle(self.device) if stream is None else stream
         if getattr(self, "_XW", None) is None or self._XW.numel() < self.tree.n * 16:
@@ -492,6 +616,7 @@ class HMatrix:
         self.ranks = blk[nd:, 4].copy()
         self._rsum = int(r.sum())
         self._T = torch.empty((max(self._rsum, 1) * 16,), dtype=torch.complex128, device=self.device)
+        self._pk = None
 
     def _factor_bufs(self, blk):
         nd = self._nd
@@ -513,6 +638,7 @@ class HMatrix:
         c.ranks = self.ranks.copy()
         c._T = torch.empty_like(self._T)
         c._lu = None
+        c._pk = None
         return c
 
     def truncate(self, htol=None):
@@ -566,6 +692,7 @@ class HMatrix:
         br, bi = complex(b).real, complex(b).imag
         nat.call("mnpbem_hmat_axpby", _DT[self.dtype][1], x.numel(), ar, ai, nat.ptr(x), br, bi,
                  nat.ptr(y) if y is not None else None, nat.ptr(out), nat.stream_handle(self.device))
+        self._pk = None
         return out
 
     def __rmul__(self, scalar):
@@ -667,6 +794,7 @@ class HMatrix:
         bids, nb = c._diag_bids()
         nat.call("mnpbem_hmat_diag", _DT[c.dtype][1], nb, nat.ptr(c._blk), nat.ptr(bids), nat.ptr(c._perm),
                  nat.ptr(c._val), None, 1, nat.stream_handle(c.device))
+        c._pk = None
         return c
 
     def fillval(self, fun):
@@ -686,6 +814,7 @@ class HMatrix:
                 host[b[5]:b[5] + b[1] * b[3]] = np.asarray(fun(rr.ravel(), cc.ravel()))
             self._val = torch.as_tensor(host).to(dev, _DT[self.dtype][0])
             self._lu = None
+            self._pk = None
             return self
         src = torch.as_tensor(fun).to(dev, torch.complex128) if isinstance(fun, np.ndarray) else fun.to(torch.complex128)
         if tuple(src.shape) != (n, n):
@@ -696,6 +825,7 @@ class HMatrix:
         nat.call("mnpbem_hmat_dense", _DT[self.dtype][1], self._nd, nat.ptr(dblk), nat.ptr(bids), nat.ptr(self._perm),
                  nat.ptr(src), si, sj, nat.ptr(self._val), nat.stream_handle(dev))
         self._lu = None
+        self._pk = None
         return self
 
     # ------------------------------------------------------------ dense solves

@@ -143,6 +143,61 @@ def test_hmatrix_kernel_1_over_r(cuda, dtype, tol):
     assert np.linalg.norm(Y - Yf) / np.linalg.norm(Yf) < (1e-13 if dtype == "complex128" else 1e-6)
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_hmatrix_packed_product_matches_block_product(cuda, dtype):
+    """The product on the packed panels (matvec_device) against the product
+    on the block storage (matvec_blocks_device), same stored values, both
+    accumulating in complex128: 1, 4, 7 and 16 columns, row strides above
+    the column count, overwrite and accumulate, leaves of up to 100 rows
+    (two 64-row groups each)."""
+    from mnpbem_amd import _native as nat
+    from mnpbem_amd.hmatrix import ClusterTree, HMatrix
+    print("hmatrix packed vs block product", dtype, flush=True)
+    pos = _points(3000, seed=4)
+    t = ClusterTree(pos, cleaf=100)
+    h = HMatrix.from_func(t, _kernel(pos), htol=1e-6, kmax=64, dtype=dtype)
+    assert (h.ranks > 0).any() and t.n == 3000
+    n, ld = t.n, 17
+    g = torch.Generator(device="cpu").manual_seed(11)
+    for ncol in (1, 4, 7, 16):
+        X = torch.randn(n, ld, dtype=torch.complex128, generator=g).cuda()
+        Y0 = torch.randn(n, ld, dtype=torch.complex128, generator=g).cuda()
+        for acc in (False, True):
+            Y1, Y2 = Y0.clone(), Y0.clone()
+            h.matvec_device(nat.ptr(X), ld, nat.ptr(Y1), ld, ncol, accumulate=acc)
+            h.matvec_blocks_device(nat.ptr(X), ld, nat.ptr(Y2), ld, ncol, accumulate=acc)
+            torch.cuda.synchronize()
+            d = (Y1[:, :ncol] - Y2[:, :ncol]).abs().max().item() / Y2[:, :ncol].abs().max().item()
+            assert d < 1e-13, (ncol, acc, d)
+            assert torch.equal(Y1[:, ncol:], Y0[:, ncol:])  # columns beyond ncol untouched
+
+
+def test_hmatrix_packed_product_follows_block_changes(cuda):
+    """The packed panels are rebuilt after the blocks change in place
+    (fillval, truncate, scaling): each product equals the densified matrix's."""
+    from mnpbem_amd.hmatrix import ClusterTree, HMatrix
+    print("hmatrix packed product after changes", flush=True)
+    pos = _points(900, seed=5)
+    fun = _kernel(pos)
+    n = len(pos)
+    t = ClusterTree(pos, cleaf=16)
+    h = HMatrix.from_func(t, fun, htol=1e-8, kmax=64)
+    X = np.random.RandomState(7).randn(n, 2) + 1j * np.random.RandomState(8).randn(n, 2)
+
+    def check(hm):
+        Y = hm.mtimes_vec(X)
+        Yf = hm.full(np) @ X
+        assert np.linalg.norm(Y - Yf) / np.linalg.norm(Yf) < 1e-13
+
+    check(h)
+    h.fillval(lambda r, c: 3.0 * fun(r, c))
+    check(h)
+    h.truncate(1e-4)
+    check(h)
+    check(2.5 * h)
+    check(h + h)
+
+
 def test_hmatrix_two_particles_known_rank(cuda):
     """Two well-separated particles and a separable kernel: every admissible
     block has exact rank 2, ACA must stop at rank <= 3 with round-off error."""
