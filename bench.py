@@ -514,8 +514,7 @@ def bench_dimer(args):
     phase["precond_factor_s"] = round(time.perf_counter() - ta, 3)
     ta = time.perf_counter()
     kmat = bem.g.wavenumbers(lam)
-    for s_, key in ((0, "G"), (0, "H1"), (1, "G"), (1, "H2")):
-        bem._aca.generate(s_, key, kmat[0], 600.0)
+    hms = [bem._aca.generate(s_, key, kmat[0], 600.0) for s_, key in ((0, "G"), (0, "H1"), (1, "G"), (1, "H2"))]
     torch.cuda.synchronize()
     phase["aca_in_place_s"] = round(time.perf_counter() - ta, 3)
     S = bem._prepare(lam)
@@ -530,9 +529,9 @@ def bench_dimer(args):
     log("phases at 600 nm: %s; peak device memory of the timed solves %.1f GB" % (phase, peak_gb))
     hstat = [h.stat() for h in bem.last_hmats[0]]
     # H-matrix matvec as GMRES calls it (4 columns per side for one plane
-    # wave): algorithmic bytes = stored blocks read once per product, timed with
-    # HIP events on the launching stream
-    hms = bem.last_hmats[0]
+    # wave), on the four H-matrices of the ACA phase above (GMRES's own have
+    # released their block storage): algorithmic bytes = stored blocks read
+    # once per product, timed with HIP events on the launching stream
     n = cp.n
     xv = torch.randn((n, 4), dtype=torch.complex128, device="cuda")
     yv = torch.empty_like(xv)

@@ -573,6 +573,12 @@ int mnpbem_prof_end(double* total_ms, double* total_flops, long* launches);
 int mnpbem_prof_bytes(double* total_bytes);
 /* the part of those bytes that writes C (each output element once) */
 int mnpbem_prof_cbytes(double* c_write_bytes);
+/* per timed launch (after mnpbem_prof_end): shapes[5 i ..] = M, N, K, batch,
+ * kernel (0: 128 x 128 bf16x3 tile, 1: 64 x 64 tile, 2: in-place 128-row
+ * product, 3: FP32/FP64 kernel), ms[i] = its HIP-event time; up to cap
+ * launches; returns the number recorded.  No reference counterpart
+ * (measurement only). */
+int mnpbem_prof_launches(int cap, int* shapes, float* ms);
 
 #ifdef __cplusplus
 }

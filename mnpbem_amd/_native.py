@@ -106,6 +106,7 @@ SIGNATURES = {
     "mnpbem_prof_end": [_p, _p, _p],
     "mnpbem_prof_bytes": [_p],
     "mnpbem_prof_cbytes": [_p],
+    "mnpbem_prof_launches": [_i, _p, _p],
 }
 
 _lib = None

@@ -178,10 +178,12 @@ class BEMRetIter(BEMIter):
         S.par = par
         if self._hmatrix:
             # H-matrices of G1, H1, G2, H2 per wavelength, entries generated
-            # in place by the ACA kernels (no n x n matrix)
+            # in place by the ACA kernels (no n x n matrix); GMRES only
+            # multiplies with them, so each keeps its packed product layouts
+            # and frees its block storage before the next is generated
             kmat = g.wavenumbers(enei_list)
-            S.hm = [[self._aca.generate(s, key, kmat[b], lam) for s, key in ((0, "G"), (0, "H1"), (1, "G"),
-                                                                              (1, "H2"))]
+            S.hm = [[self._aca.generate(s, key, kmat[b], lam).release_blocks()
+                     for s, key in ((0, "G"), (0, "H1"), (1, "G"), (1, "H2"))]
                     for b, lam in enumerate(enei_list)]
             S.ops = None
             if S.F is not None:

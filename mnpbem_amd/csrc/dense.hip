@@ -844,6 +844,7 @@ struct GemmProf {
   std::vector<double> flops;
   double bytes = 0.0;   // compulsory bytes: A and B read once, C read and written once
   double cbytes = 0.0;  // of which the writes of C
+  std::vector<int> shape;  // per timed launch: M, N, K, batch, kernel (0/1: bf16x3 tile, 2: assign128, 3: generic)
   size_t used = 0;
 };
 GemmProf g_prof;
@@ -939,6 +940,9 @@ int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, lon
     MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used + 1], st));
     g_prof.used += 2;
     g_prof.flops.push_back(8.0 * M * (double)N * K * batch);
+    int kern = 3;
+    if constexpr (sizeof(T) == 4) kern = gmode == 1 ? g_tune.var[gcell(M, N, K)] : 3;
+    g_prof.shape.insert(g_prof.shape.end(), {M, N, K, batch, kern});
     g_prof.bytes += (double)batch * 2.0 * sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N);
     g_prof.cbytes += (double)batch * 2.0 * sizeof(T) * ((double)M * N);
   }
@@ -956,6 +960,7 @@ int launch_bf16x3_assign128(int n, int nrhs, const float* D, long ldd, long sD, 
     MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used + 1], st));
     g_prof.used += 2;
     g_prof.flops.push_back(8.0 * n * (double)nrhs * n * batch);
+    g_prof.shape.insert(g_prof.shape.end(), {n, nrhs, n, batch, 2});
     g_prof.bytes += (double)batch * 8.0 * ((double)n * n + 2.0 * n * nrhs);
     g_prof.cbytes += (double)batch * 8.0 * ((double)n * nrhs);
   }
@@ -2116,6 +2121,7 @@ extern "C" int mnpbem_prof_begin(int capacity) {
   }
   g_prof.used = 0;
   g_prof.flops.clear();
+  g_prof.shape.clear();
   g_prof.bytes = 0.0;
   g_prof.cbytes = 0.0;
   g_prof.on = true;
@@ -2145,6 +2151,25 @@ extern "C" int mnpbem_prof_end(double* total_ms, double* total_flops, long* laun
 extern "C" int mnpbem_prof_bytes(double* total_bytes) {
   *total_bytes = g_prof.bytes;
   return 0;
+}
+
+// Per-launch record of the launches timed since mnpbem_prof_begin (after
+// mnpbem_prof_end): shapes[5 i ..] = M, N, K, batch, kernel (0: 128 x 128
+// bf16x3 tile, 1: 64 x 64 bf16x3 tile, 2: in-place 128-row product, 3: the
+// FP32/FP64 kernel) and ms[i] its HIP-event time, for up to cap launches;
+// returns the number of launches recorded.
+extern "C" int mnpbem_prof_launches(int cap, int* shapes, float* ms) {
+  MNP_REQUIRE(cap >= 0, "prof_launches: negative capacity");
+  const int nl = (int)(g_prof.used / 2);
+  const int m = nl < cap ? nl : cap;
+  for (int i = 0; i < m; ++i) {
+    MNP_HIP_CHECK(hipEventSynchronize(g_prof.ev[2 * i + 1]));
+    float t = 0.f;
+    MNP_HIP_CHECK(hipEventElapsedTime(&t, g_prof.ev[2 * i], g_prof.ev[2 * i + 1]));
+    ms[i] = t;
+    for (int q = 0; q < 5; ++q) shapes[5 * i + q] = g_prof.shape[5 * i + q];
+  }
+  return m;
 }
 
 // The part of those bytes that writes C (each output element once).

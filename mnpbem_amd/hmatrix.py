@@ -460,9 +460,23 @@ class HMatrix:
         return HMatrix(tree, htol=htol, kmax=kmax, fadmiss=fadmiss, dtype=dtype, device=device).aca(fun)
 
     # ------------------------------------------------------------------
-    def _check(self):
+    def _check(self, blocks=True):
         if not self._built:
             raise RuntimeError("HMatrix: blocks not filled (call aca first)")
+        if blocks and self._val is None:
+            raise RuntimeError("HMatrix: block storage released (release_blocks); only products remain")
+
+    def release_blocks(self):
+        """Build the packed product layouts and free the block storage (dense
+        leaf values and low-rank factors), for an H-matrix used only as an
+        operator: its device memory halves, products (matvec_device,
+        mtimes_vec) and the host tables (stat, compression) remain, and
+        every method that reads or changes the blocks raises."""
+        self._check()
+        if self._pk is None:
+            self._pack()
+        self._val = self._U = self._V = None
+        return self
 
     def _pack(self):
         """Packed product layouts, built on the first product after the
@@ -492,7 +506,7 @@ class HMatrix:
     def matvec_device(self, X, ldx, Y, ldy, ncol, accumulate=False, stream=None):
         """Y (+)= M X for complex128 device buffers in face order, (n, ldx) and
         (n, ldy) row-major, ncol <= 16 columns (on the packed layouts)."""
-        self._check()
+        self._check(blocks=self._pk is None)
         st = nat.stream_handle(self.device) if stream is None else stream
         pk = self._pk if self._pk is not None else self._pack()
         nat.call("mnpbem_hmat_matvec_packed", _DT[self.dtype][1], self.tree.n, pk["nc"], nat.ptr(pk["citems"]),
@@ -515,7 +529,7 @@ class HMatrix:
     def mtimes_vec(self, v):
         """M v for v (n,) or (n, k), numpy or torch; the result has the type
         and device of v (hmatrix.py:450-541)."""
-        self._check()
+        self._check(blocks=self._pk is None)
         as_np = isinstance(v, np.ndarray)
         x = torch.as_tensor(v) if as_np else v
         one = x.dim() == 1

@@ -172,6 +172,26 @@ def test_hmatrix_packed_product_matches_block_product(cuda, dtype):
             assert torch.equal(Y1[:, ncol:], Y0[:, ncol:])  # columns beyond ncol untouched
 
 
+def test_hmatrix_release_blocks_keeps_products(cuda):
+    """After release_blocks the products on the packed panels are unchanged
+    bit for bit, the host tables still answer, and a method that reads the
+    freed block storage raises instead of reading freed memory."""
+    from mnpbem_amd.hmatrix import ClusterTree, HMatrix
+    print("hmatrix release_blocks", flush=True)
+    pos = _points(900, seed=6)
+    t = ClusterTree(pos, cleaf=16)
+    h = HMatrix.from_func(t, _kernel(pos), htol=1e-6, kmax=64, dtype="complex64")
+    X = np.random.RandomState(3).randn(len(pos), 3) + 1j * np.random.RandomState(4).randn(len(pos), 3)
+    Y0 = h.mtimes_vec(X)
+    st0 = h.stat()
+    assert h.release_blocks() is h
+    assert np.array_equal(h.mtimes_vec(X), Y0)
+    assert h.stat() == st0
+    for call in (lambda: h.full(np), lambda: h.truncate(1e-4), lambda: h.diag(np)):
+        with pytest.raises(RuntimeError, match="released"):
+            call()
+
+
 def test_hmatrix_packed_product_follows_block_changes(cuda):
     """The packed panels are rebuilt after the blocks change in place
     (fillval, truncate, scaling): each product equals the densified matrix's."""
