@@ -155,6 +155,15 @@ int mnpbem_getrf_tall_rows(int rows);
  * A tuning knob of this library (no reference counterpart). */
 int mnpbem_trsm_rows_max(int cols);
 
+/* Right-hand sides from which the many-column triangular solves of
+ * mnpbem_getrs_blockinv_batched and mnpbem_getri_blockinv_batched (complex64
+ * on the bf16x3 products, bs = 128, n > 256) run block row by block row
+ * (left-looking: one product over all solved rows, then the inverse-block
+ * leaf) instead of recursively; rhs < 0 restores the built-in 512, 0 disables
+ * it.  Returns the previous setting.  A tuning knob of this library (no
+ * reference counterpart). */
+int mnpbem_trsm_left_rhs(int rhs);
+
 /* Inverses of the 64 x 64 diagonal blocks of getrf factors ([U_II^-1 | L_II^-1]
  * per block, sD >= ceil(n/64)*2*64*64 elements) so that the block steps of
  * mnpbem_lu_solve_t_batched are parallel matrix-vector products (Dinv = NULL

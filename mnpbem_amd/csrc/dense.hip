@@ -1084,6 +1084,56 @@ int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, 
   }
 }
 
+// Right-hand sides from which the many-column triangular solves with 128-row
+// inverse diagonal blocks run block row by block row (trsm_left) instead of
+// recursively (set by mnpbem_trsm_left_rhs; 0 disables).
+constexpr int TRSM_LEFT_DEFAULT = 512;
+int g_trsm_left_rhs = TRSM_LEFT_DEFAULT;
+
+// X <- T^-1 X block row by block row (left-looking), with the inverse
+// diagonal blocks: for block row k of inv.bs rows (lower: first to last,
+// upper: last to first), X_k -= T_k,s X_s over all rows s solved so far (one
+// product whose K grows with k, so nearly all flops run on long-K products
+// that keep their C tile in registers), then X_k <- D_k X_k.  The recursion
+// of trsm_rec spends its lowest levels on products of K = 128..384 whose C
+// read and write dominate.  Same operation count as trsm_rec with
+// inverse-block leaves.
+template <typename T>
+int trsm_left(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, long ldx, long sX, int batch,
+              hipStream_t st, int gmode, LeafInv<T> inv) {
+  const int bs = inv.bs;
+  const int nb = (n + bs - 1) / bs;
+  for (int q = 0; q < nb; ++q) {
+    const int k = lower ? q : nb - 1 - q;
+    const int r0 = k * bs, m = min(bs, n - r0), r1 = r0 + m;
+    int rc;
+    if (lower && r0 > 0) {
+      if ((rc = gemm_sub<T>(m, nrhs, r0, Tm + 2 * ((long)r0 * ldt), ldt, sT, X, ldx, sX, X + 2 * ((long)r0 * ldx),
+                            ldx, sX, batch, st, gmode)))
+        return rc;
+    } else if (!lower && r1 < n) {
+      if ((rc = gemm_sub<T>(m, nrhs, n - r1, Tm + 2 * ((long)r0 * ldt + r1), ldt, sT, X + 2 * ((long)r1 * ldx), ldx,
+                            sX, X + 2 * ((long)r0 * ldx), ldx, sX, batch, st, gmode)))
+        return rc;
+    }
+    if ((rc = trsm_rec<T>(lower, m, nrhs, Tm + 2 * ((long)r0 * ldt + r0), ldt, sT, X + 2 * ((long)r0 * ldx), ldx, sX,
+                          batch, st, gmode, inv.at(r0))))
+      return rc;
+  }
+  return 0;
+}
+
+// the many-column solves: left-looking with 128-row inverse blocks on the
+// bf16x3 products, else the recursion
+template <typename T>
+int trsm_many(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, long ldx, long sX, int batch,
+              hipStream_t st, int gmode, LeafInv<T> inv) {
+  if (sizeof(T) == 4 && gmode == 1 && inv.D != nullptr && inv.bs == 128 && g_trsm_left_rhs > 0 &&
+      nrhs >= g_trsm_left_rhs && n > 2 * inv.bs)
+    return trsm_left<T>(lower, n, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st, gmode, inv);
+  return trsm_rec<T>(lower, n, nrhs, Tm, ldt, sT, X, ldx, sX, batch, st, gmode, inv);
+}
+
 template <typename T>
 int laswp(int k0, int k1, int clo, int chi, T* A, long lda, long sA, const int* ipiv, long sP, int batch, bool rev,
           hipStream_t st) {
@@ -1791,7 +1841,7 @@ int getri(int n, T* A, long lda, long sA, const int* ipiv, long sP, T* W, long l
   MNP_LAUNCH_CHECK();
   int rc;
   if ((rc = trsm_lower_ident<T>(n, A, lda, sA, W, ldw, sW, batch, st, gmode, inv))) return rc;
-  if ((rc = trsm_rec<T>(false, n, n, A, lda, sA, W, ldw, sW, batch, st, gmode, inv))) return rc;
+  if ((rc = trsm_many<T>(false, n, n, A, lda, sA, W, ldw, sW, batch, st, gmode, inv))) return rc;
   hipLaunchKernelGGL(k_perm_array, dim3(batch), dim3(256), 0, st, n, ipiv, sP, Q, sQ);
   MNP_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scatter_cols<T>, dim3(mnp_ceil_div(n, 256), n, batch), dim3(256), 0, st, n, W, ldw, sW, Q,
@@ -1843,6 +1893,15 @@ extern "C" int mnpbem_cgemm_sub_variant(int variant, int M, int N, int K, const 
 extern "C" int mnpbem_trsm_rows_max(int cols) {
   const int prev = (int)g_trsm_rows_max;
   g_trsm_rows_max = cols < 0 ? TRSM_ROWS_DEFAULT : cols;
+  return prev;
+}
+
+// Right-hand sides from which getrs / getri with 128-row inverse blocks solve
+// block row by block row (left-looking); rhs < 0 restores the built-in 512, 0
+// disables it (recursive solves only).  Returns the previous setting.
+extern "C" int mnpbem_trsm_left_rhs(int rhs) {
+  const int prev = g_trsm_left_rhs;
+  g_trsm_left_rhs = rhs < 0 ? TRSM_LEFT_DEFAULT : rhs;
   return prev;
 }
 
@@ -1990,9 +2049,9 @@ extern "C" int mnpbem_getrs_blockinv_batched(int dtype, int n, int nrhs, const v
   inv.bs = bs;
   int rc;
   if ((rc = laswp<float>(0, n, 0, nrhs, (float*)B, ldb, sB, ipiv, sP, batch, false, st))) return rc;
-  if ((rc = trsm_rec<float>(true, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st, 1, inv)))
+  if ((rc = trsm_many<float>(true, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st, 1, inv)))
     return rc;
-  return trsm_rec<float>(false, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st, 1, inv);
+  return trsm_many<float>(false, n, nrhs, (const float*)A, lda, sA, (float*)B, ldb, sB, batch, st, 1, inv);
 }
 
 extern "C" int mnpbem_getri_blockinv_batched(int dtype, int n, void* A, long lda, long sA, const int* ipiv, long sP,

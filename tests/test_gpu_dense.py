@@ -525,6 +525,55 @@ def test_getrs_blockinv_leaves(cuda, n, nrhs, bs):
     assert np.array_equal(tX.cpu().numpy(), out["fp32"])
 
 
+@pytest.mark.parametrize("n,nrhs", [(300, 512), (1031, 700), (2884, 600)])
+def test_getrs_getri_left_looking(cuda, n, nrhs):
+    """The many-column solves with 128-row inverse blocks block row by block
+    row (mnpbem_trsm_left_rhs, from 512 right-hand sides) against numpy's
+    solve and inverse, as accurate as the recursive solves (knob 0) on the
+    same factors; the partial last block row included."""
+    nat = _lib()
+    rng = np.random.default_rng(5 * n + nrhs)
+    B = 2
+    A = (_rand(rng, (B, n, n), np.complex64) + (2.0 * np.sqrt(n)) * np.eye(n)[None]).astype(np.complex64)
+    X = _rand(rng, (B, n, nrhs), np.complex64)
+    ref = np.linalg.solve(A.astype(np.complex128), X.astype(np.complex128))
+    tA = torch.from_numpy(A.copy()).to(cuda)
+    piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((B,), dtype=torch.int32, device=cuda)
+    st = nat.stream_handle(cuda)
+    print("getrs/getri left-looking", n, nrhs, flush=True)
+    nat.call("mnpbem_getrf_batched", nat.DTYPE_C64X, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info), B, st)
+    D, sD = _block_inverses(nat, tA, n, B, 128, st)
+    W = torch.empty_like(tA)
+    Q = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    lib = nat.load()
+    out, inv = {}, {}
+    prev = lib.mnpbem_trsm_left_rhs(-1)
+    try:
+        for knob in (0, 512):
+            lib.mnpbem_trsm_left_rhs(knob)
+            tX = torch.from_numpy(X.copy()).to(cuda)
+            nat.call("mnpbem_getrs_blockinv_batched", nat.DTYPE_C64X, n, nrhs, nat.ptr(tA), n, n * n, nat.ptr(piv),
+                     n, nat.ptr(D), sD, 128, nat.ptr(tX), nrhs, n * nrhs, B, st)
+            out[knob] = tX.cpu().numpy()
+            t1 = tA.clone()
+            nat.call("mnpbem_getri_blockinv_batched", nat.DTYPE_C64X, n, nat.ptr(t1), n, n * n, nat.ptr(piv), n,
+                     nat.ptr(D), sD, 128, nat.ptr(W), n, n * n, nat.ptr(Q), n, B, st)
+            inv[knob] = t1.cpu().numpy()
+    finally:
+        lib.mnpbem_trsm_left_rhs(prev)
+    assert np.all(info.cpu().numpy() == 0)
+    scale = np.abs(ref).max()
+    e_left = np.abs(out[512] - ref).max() / scale
+    e_rec = np.abs(out[0] - ref).max() / scale
+    assert e_left < 3 * e_rec + 1e-5, (e_left, e_rec)
+    for b in range(B):
+        rinv = np.linalg.inv(A[b].astype(np.complex128))
+        sc = np.abs(rinv).max()
+        e1, e2 = np.abs(inv[512][b] - rinv).max() / sc, np.abs(inv[0][b] - rinv).max() / sc
+        assert e1 < 3 * e2 + 1e-5, (b, e1, e2)
+
+
 def _block_inverses(nat, tA, n, B, bs, st):
     """Inverse diagonal blocks of the factors in tA: 64 rows from
     mnpbem_lu_blockinv_batched, 128 rows built from those by
