@@ -98,6 +98,90 @@ __global__ __launch_bounds__(256) void k_gemv_t(int n, int C, const T* __restric
     }
 }
 
+// complex64 form of k_gemv_t with two adjacent output rows i, i + 1 per lane:
+// one 16-byte load per buffer row (a wave reads a 1 KB segment instead of
+// 512 bytes, half the load instructions per byte).  Every output is the same
+// sum in the same order as in k_gemv_t (bit-identical).  Needs n, ldm and sM
+// even and M 16-byte aligned.
+template <int CM>
+__global__ __launch_bounds__(256) void k_gemv_t2(int n, int C, const float* __restrict__ M, long ldm, long sM,
+                                                 const float* __restrict__ X, long ldx, long sX, float* __restrict__ Y,
+                                                 long ldy, long sY, double alr, double ali, double beta) {
+  constexpr int UNR = 8;
+  const long b = blockIdx.y;
+  M += 2 * b * sM;
+  X += 2 * b * sX;
+  Y += 2 * b * sY;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int i = blockIdx.x * 128 + 2 * lane;
+  const int q = (((n + 3) / 4) + UNR - 1) / UNR * UNR;  // buffer rows per wave
+  const int jb = wv * q, je = min(n, jb + q);
+  const int ic = min(i, n - 2);
+  __shared__ cplx<float> part[3][64][2 * CM];
+  cplx<float> a0[CM], a1[CM];
+#pragma unroll
+  for (int c = 0; c < CM; ++c) a0[c] = a1[c] = cmake<float>(0, 0);
+  auto xval = [&](int j, int c) {
+    const float* xr = X + 2 * ((long)j * ldx + c);
+    return c < C ? cmake<float>(xr[0], xr[1]) : cmake<float>(0, 0);
+  };
+  int j = jb;
+  for (; j + UNR <= je; j += UNR) {
+    float4 m[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) m[u] = *reinterpret_cast<const float4*>(M + 2 * ((long)(j + u) * ldm + ic));
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        const cplx<float> x = xval(j + u, c);
+        a0[c] = cfma(a0[c], cmake<float>(m[u].x, m[u].y), x);
+        a1[c] = cfma(a1[c], cmake<float>(m[u].z, m[u].w), x);
+      }
+  }
+  for (; j < je; ++j) {
+    const float4 m = *reinterpret_cast<const float4*>(M + 2 * ((long)j * ldm + ic));
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      const cplx<float> x = xval(j, c);
+      a0[c] = cfma(a0[c], cmake<float>(m.x, m.y), x);
+      a1[c] = cfma(a1[c], cmake<float>(m.z, m.w), x);
+    }
+  }
+  if (wv > 0)
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      part[wv - 1][lane][c] = a0[c];
+      part[wv - 1][lane][CM + c] = a1[c];
+    }
+  __syncthreads();
+  if (wv > 0 || i >= n) return;
+#pragma unroll
+  for (int w = 0; w < 3; ++w)
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      a0[c] = cadd(a0[c], part[w][lane][c]);
+      a1[c] = cadd(a1[c], part[w][lane][CM + c]);
+    }
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int c = 0; c < CM; ++c)
+      if (c < C) {
+        const cplx<float> a = h ? a1[c] : a0[c];
+        float* py = Y + 2 * ((long)(i + h) * ldy + c);
+        const double sr = (double)a.re, si = (double)a.im;
+        double yr = alr * sr - ali * si, yi = alr * si + ali * sr;
+        if (beta != 0.0) {
+          const cplx<float> y0 = cld(py);
+          yr += beta * (double)y0.re;
+          yi += beta * (double)y0.im;
+        }
+        cst(py, cmake<float>((float)yr, (float)yi));
+      }
+}
+
 // The four products of the complex128 residual, Y = [G1^T.. buffers applied:
 // G1 X1 | H1 X1 | G2 X2 | H2 X2], in one launch (blockIdx.z = product), with
 // accumulators for exactly the CC = 4P columns.  One product per launch with
@@ -1036,6 +1120,23 @@ int gemv_t(int n, int C, const T* M, long ldm, long sM, const T* X, long ldx, lo
            double alr, double ali, double beta, int batch, hipStream_t st) {
   for (int c0 = 0; c0 < C; c0 += MAXC) {
     const int cc = std::min(MAXC, C - c0);
+    if constexpr (sizeof(T) == 4) {
+      if (n % 2 == 0 && ldm % 2 == 0 && sM % 2 == 0 && ((uintptr_t)M & 15) == 0) {
+        const dim3 grid2(mnp_ceil_div(n, 128), batch);
+#define MNP_GEMV2(CM)                                                                                             \
+  hipLaunchKernelGGL((k_gemv_t2<CM>), grid2, dim3(256), 0, st, n, cc, (const float*)M, ldm, sM,                   \
+                     (const float*)(X + 2 * c0), ldx, sX, (float*)(Y + 2 * c0), ldy, sY, alr, ali, beta)
+        if (cc <= 4)
+          MNP_GEMV2(4);
+        else if (cc <= 8)
+          MNP_GEMV2(8);
+        else
+          MNP_GEMV2(16);
+#undef MNP_GEMV2
+        MNP_LAUNCH_CHECK();
+        continue;
+      }
+    }
     const dim3 grid(mnp_ceil_div(n, 64), batch);
 #define MNP_GEMV(CM)                                                                                            \
   hipLaunchKernelGGL((k_gemv_t<T, CM>), grid, dim3(256), 0, st, n, cc, M, ldm, sM, X + 2 * c0, ldx, sX, Y + 2 * c0, \

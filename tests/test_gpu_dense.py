@@ -792,3 +792,35 @@ def test_factor_interchanges_fallback(cuda, monkeypatch):
     X = bem.solve_batch(F2, pw.rhs(cp, lams, bem.g))
     out = pw.extinction_batch(cp, lams, X, bem.g).cpu().numpy()
     assert np.abs(out / ref - 1).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,C", [(2884, 8), (1030, 6), (130, 3), (512, 16), (257, 8)])
+def test_gemv_t_paired_rows_bit_identical(cuda, n, C):
+    """mnpbem_gemv_t_batched (Y = beta Y + alpha M^T X, M the row-major
+    buffer) in complex64 against numpy; with n even and a 16-byte aligned
+    buffer two output rows per lane (16-byte loads), otherwise one: the same
+    buffer at a misaligned address takes the one-row kernel and must give the
+    same bits."""
+    nat = _lib()
+    rng = np.random.default_rng(n + C)
+    B = 3
+    M = _rand(rng, (B, n, n), np.complex64)
+    X = _rand(rng, (B, n, C), np.complex64)
+    Y0 = _rand(rng, (B, n, C), np.complex64)
+    al, beta = 0.7 - 0.2j, 0.5
+    ref = beta * Y0.astype(np.complex128) + al * np.einsum("bji,bjc->bic", M.astype(np.complex128),
+                                                            X.astype(np.complex128))
+    st = nat.stream_handle(cuda)
+    print("gemv_t paired rows", n, C, flush=True)
+    outs = []
+    for off in (0, 1):  # complex elements before the buffer: 0 aligned, 1 misaligned (8 bytes)
+        buf = torch.zeros(B * n * n + 2, dtype=torch.complex64, device=cuda)
+        buf[off:off + B * n * n] = torch.from_numpy(M.reshape(-1)).to(cuda)
+        tX = torch.from_numpy(X.copy()).to(cuda)
+        tY = torch.from_numpy(Y0.copy()).to(cuda)
+        nat.call("mnpbem_gemv_t_batched", nat.DTYPE_C64, n, C, nat.ptr_off(buf, off), n, n * n, nat.ptr(tX), C,
+                 n * C, nat.ptr(tY), C, n * C, al.real, al.imag, beta, B, st)
+        outs.append(tY.cpu().numpy())
+    err = np.abs(outs[0] - ref).max() / np.abs(ref).max()
+    assert err < 1e-5, err
+    assert np.array_equal(outs[0], outs[1])
