@@ -58,8 +58,9 @@ int mnpbem_cgemm_sub_batched(int dtype, int M, int N, int K, const void* A, long
  * reference, whose LU runs in LAPACK / cuSOLVER; this is the tile autotuner
  * of the mixed-precision path).  A 7 x 7 x 7 table over shape classes of
  * (M, N, K) (class c: x <= 64 * 2^c, class 6 unbounded; cell (cm*7+cn)*7+ck)
- * picks variant 0 (128 x 128 tile) or 1 (64 x 64 tile) per launch; both give
- * identical results.  _variant forces a variant (complex64 data). */
+ * picks variant 0 (128 x 128 tile), 1 (64 x 64 tile) or 2 (128 x 128 tile that
+ * reads its C tile before the K loop) per launch; all give identical
+ * results.  _variant forces a variant (complex64 data). */
 int mnpbem_cgemm_sub_variant(int variant, int M, int N, int K, const void* A, long lda, long sA, const void* B,
                              long ldb, long sB, void* C, long ldc, long sC, int batch, void* stream);
 int mnpbem_gemm_tuning_set(const unsigned char* table, int ncells);
@@ -583,8 +584,9 @@ int mnpbem_prof_bytes(double* total_bytes);
 /* the part of those bytes that writes C (each output element once) */
 int mnpbem_prof_cbytes(double* c_write_bytes);
 /* per timed launch (after mnpbem_prof_end): shapes[5 i ..] = M, N, K, batch,
- * kernel (0: 128 x 128 bf16x3 tile, 1: 64 x 64 tile, 2: in-place 128-row
- * product, 3: FP32/FP64 kernel), ms[i] = its HIP-event time; up to cap
+ * kernel (bf16x3 tiles 0: 128 x 128, 1: 64 x 64, 2: 128 x 128 with C
+ * prefetch; 3: FP32/FP64 kernel; 4: in-place 128-row product), ms[i] = its
+ * HIP-event time; up to cap
  * launches; returns the number recorded.  No reference counterpart
  * (measurement only). */
 int mnpbem_prof_launches(int cap, int* shapes, float* ms);

@@ -3,16 +3,17 @@
 The mixed-precision factorisation launches C -= A B products of very
 different shapes: large square updates near the top of the LU / TRSM
 recursion and thin ones (M or N = 64, K = 64) at its bottom.  The library has
-two tile variants of the bf16x3 kernel (0: 128 x 128 with 8 waves, one
-workgroup per CU; 1: 64 x 64 with 4 waves, two workgroups per CU) and routes
+three tile variants of the bf16x3 kernel (0: 128 x 128 with 8 waves, one
+workgroup per CU; 1: 64 x 64 with 4 waves, two workgroups per CU; 2: the
+128 x 128 tile reading its C tile before the K loop) and routes
 every launch through a 7 x 7 x 7 table of shape classes (class c of a
-dimension x: x <= 64 * 2^c, the last class unbounded).  Both variants
+dimension x: x <= 64 * 2^c, the last class unbounded).  All variants
 accumulate every output element in the same order, so the table changes
 speed, never results.
 
 ``tune(run)`` executes ``run()`` once, reads back one launched shape per
-class that the run touched, times both variants on that shape and keeps the
-faster one.  ``save_table`` / ``load_table`` keep the result as JSON
+class that the run touched, times every variant on that shape and keeps the
+fastest one.  ``save_table`` / ``load_table`` keep the result as JSON
 (``data/gemm_tuning_gfx950.json``, applied when the library loads).
 
     python -m mnpbem_amd.autotune             # tune on the 2884-face sweep, print, save
@@ -27,7 +28,7 @@ from . import _native as nat
 
 NCLS = 7
 NCELL = NCLS ** 3
-VARIANTS = {0: "128x128 (8 waves)", 1: "64x64 (4 waves)"}
+VARIANTS = {0: "128x128 (8 waves)", 1: "64x64 (4 waves)", 2: "128x128 C prefetch"}
 TABLE_PATH = nat.TUNING_PATH
 
 
@@ -94,7 +95,7 @@ def time_variant(variant, M, N, K, batch, reps=3, device=None):
 
 
 def tune(run, reps=3, device=None, verbose=True):
-    """Run the workload once, then time both tile variants on one recorded
+    """Run the workload once, then time every tile variant on one recorded
     shape of every class it used and install the faster choice.  Returns the
     table and a list of (cell, shape, seconds per variant, choice)."""
     seen_shapes(clear=True)
@@ -103,14 +104,14 @@ def tune(run, reps=3, device=None, verbose=True):
     table = get_table()
     report = []
     for cell, (M, N, K, batch) in sorted(shapes.items()):
-        t = [time_variant(v, M, N, K, batch, reps, device) for v in (0, 1)]
+        t = [time_variant(v, M, N, K, batch, reps, device) for v in sorted(VARIANTS)]
         choice = int(np.argmin(t))
         table[cell] = choice
         report.append((cell, (M, N, K, batch), t, choice))
         if verbose:
-            print("class %3d  M=%5d N=%5d K=%5d batch=%4d  %s %.3f ms  %s %.3f ms  -> %s"
-                  % (cell, M, N, K, batch, VARIANTS[0], t[0] * 1e3, VARIANTS[1], t[1] * 1e3, VARIANTS[choice]),
-                  flush=True)
+            print("class %3d  M=%5d N=%5d K=%5d batch=%4d  %s  -> %s"
+                  % (cell, M, N, K, batch, "  ".join("%s %.3f ms" % (VARIANTS[v], t[v] * 1e3) for v in sorted(VARIANTS)),
+                     VARIANTS[choice]), flush=True)
     set_table(table)
     return table, report
 

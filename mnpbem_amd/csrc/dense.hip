@@ -262,7 +262,7 @@ __device__ __forceinline__ Tile gemm_tile(int remap) {
 // after them or right after the split of their registers) was measured
 // no faster than the compiler's own order.
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int WM>
+template <int BM, int BN, int WM, bool CPRE = (BM == 64)>
 __global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3(
     int M, int N, int K, const float* __restrict__ A, long lda, long sA, const float* __restrict__ B, long ldb,
     long sB, float* __restrict__ C, long ldc, long sC, int remap, int assign) {
@@ -401,11 +401,12 @@ __global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3
     gload_full((s + 2) * XK);
     __syncthreads();
   };
-  // The 64 x 64 tile reads its C tile into registers before the K loop: its
-  // launches are the few-row, small-K products of the recursions, where the
-  // epilogue's wait for C was a large part of the launch (13-33 % faster at
-  // K <= 256; the 128 x 128 tile has no registers to spare for it).
-  constexpr bool CPRE = BM == 64;
+  // CPRE: the C tile is read into registers before the K loop.  The 64 x 64
+  // tile always does it: its launches are the few-row, small-K products of
+  // the recursions, where the epilogue's wait for C was a large part of the
+  // launch (13-33 % faster at K <= 256).  The 128 x 128 tile has no registers
+  // to spare for it on long K loops; its prefetching instance (tile variant
+  // 2) is for the short ones the shape table assigns to it.
   float2 cpre[CPRE ? FI * FJ * 4 : 1];
   if (CPRE && !assign) {
 #pragma unroll
@@ -844,7 +845,7 @@ struct GemmProf {
   std::vector<double> flops;
   double bytes = 0.0;   // compulsory bytes: A and B read once, C read and written once
   double cbytes = 0.0;  // of which the writes of C
-  std::vector<int> shape;  // per timed launch: M, N, K, batch, kernel (0/1: bf16x3 tile, 2: assign128, 3: generic)
+  std::vector<int> shape;  // per timed launch: M, N, K, batch, kernel (0-2: bf16x3 tile, 3: generic, 4: assign128)
   size_t used = 0;
 };
 GemmProf g_prof;
@@ -887,6 +888,10 @@ int launch_bf16x3(int variant, int M, int N, int K, const float* A, long lda, lo
     dim3 grid(mnp_ceil_div(N, 64), mnp_ceil_div(M, 64), batch);
     hipLaunchKernelGGL((k_cgemm_sub_bf16x3<64, 64, 32>), grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB,
                        C, ldc, sC, gemm_xcd_remap(), assign);
+  } else if (variant == 2) {
+    dim3 grid(mnp_ceil_div(N, 128), mnp_ceil_div(M, 128), batch);
+    hipLaunchKernelGGL((k_cgemm_sub_bf16x3<128, 128, 64, true>), grid, dim3(512), 0, st, M, N, K, A, lda, sA, B, ldb,
+                       sB, C, ldc, sC, gemm_xcd_remap(), assign);
   } else {
     dim3 grid(mnp_ceil_div(N, 128), mnp_ceil_div(M, 128), batch);
     hipLaunchKernelGGL((k_cgemm_sub_bf16x3<128, 128, 64>), grid, dim3(512), 0, st, M, N, K, A, lda, sA, B, ldb, sB,
@@ -960,7 +965,7 @@ int launch_bf16x3_assign128(int n, int nrhs, const float* D, long ldd, long sD, 
     MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used + 1], st));
     g_prof.used += 2;
     g_prof.flops.push_back(8.0 * n * (double)nrhs * n * batch);
-    g_prof.shape.insert(g_prof.shape.end(), {n, nrhs, n, batch, 2});
+    g_prof.shape.insert(g_prof.shape.end(), {n, nrhs, n, batch, 4});
     g_prof.bytes += (double)batch * 8.0 * ((double)n * n + 2.0 * n * nrhs);
     g_prof.cbytes += (double)batch * 8.0 * ((double)n * nrhs);
   }
@@ -1876,7 +1881,7 @@ extern "C" int mnpbem_cgemm_sub_batched(int dtype, int M, int N, int K, const vo
 extern "C" int mnpbem_cgemm_sub_variant(int variant, int M, int N, int K, const void* A, long lda, long sA,
                                         const void* B, long ldb, long sB, void* C, long ldc, long sC, int batch,
                                         void* stream) {
-  MNP_REQUIRE(variant == 0 || variant == 1, "cgemm_variant: variant must be 0 or 1");
+  MNP_REQUIRE(variant >= 0 && variant <= 2, "cgemm_variant: variant must be 0, 1 or 2");
   MNP_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "cgemm_variant: negative size");
   if (M == 0 || N == 0 || K == 0 || batch == 0) return 0;
   return launch_bf16x3(variant, M, N, K, (const float*)A, lda, sA, (const float*)B, ldb, sB, (float*)C, ldc, sC,
@@ -1913,10 +1918,10 @@ extern "C" int mnpbem_getrf_tall_rows(int rows) {
 
 // Shape-class table of the bf16x3 GEMM (7 x 7 x 7 cells, index (cm * 7 + cn)
 // * 7 + ck, class c of a dimension x: x <= 64 * 2^c, last class unbounded).
-// set: table[343] of variants (0 or 1); get: copies it out.
+// set: table[343] of variants (0, 1 or 2); get: copies it out.
 extern "C" int mnpbem_gemm_tuning_set(const unsigned char* table, int ncells) {
   MNP_REQUIRE(ncells == GC * GC * GC && table != nullptr, "gemm_tuning: expected 343 cells");
-  for (int c = 0; c < ncells; ++c) MNP_REQUIRE(table[c] <= 1, "gemm_tuning: variants are 0 or 1");
+  for (int c = 0; c < ncells; ++c) MNP_REQUIRE(table[c] <= 2, "gemm_tuning: variants are 0, 1 or 2");
   for (int c = 0; c < ncells; ++c) g_tune.var[c] = table[c];
   return 0;
 }
@@ -2213,9 +2218,10 @@ extern "C" int mnpbem_prof_bytes(double* total_bytes) {
 }
 
 // Per-launch record of the launches timed since mnpbem_prof_begin (after
-// mnpbem_prof_end): shapes[5 i ..] = M, N, K, batch, kernel (0: 128 x 128
-// bf16x3 tile, 1: 64 x 64 bf16x3 tile, 2: in-place 128-row product, 3: the
-// FP32/FP64 kernel) and ms[i] its HIP-event time, for up to cap launches;
+// mnpbem_prof_end): shapes[5 i ..] = M, N, K, batch, kernel (bf16x3 tiles 0:
+// 128 x 128, 1: 64 x 64, 2: 128 x 128 with C prefetch; 3: the FP32/FP64
+// kernel; 4: in-place 128-row product) and ms[i] its HIP-event time, for up
+// to cap launches;
 // returns the number of launches recorded.
 extern "C" int mnpbem_prof_launches(int cap, int* shapes, float* ms) {
   MNP_REQUIRE(cap >= 0, "prof_launches: negative capacity");

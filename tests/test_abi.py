@@ -92,11 +92,14 @@ def test_gemm_tuning_table_roundtrip():
         at.set_table(t)
         assert np.array_equal(at.get_table(), t)
         assert at.cell_of(64, 64, 64) == 0 and at.cell_of(65, 64, 64) == 49 and at.cell_of(10 ** 6, 1, 1) == 6 * 49
-        with pytest.raises(RuntimeError, match="0 or 1"):
-            at.set_table(np.full(at.NCELL, 2, dtype=np.uint8))
+        t[at.cell_of(300, 300, 128)] = 2
+        at.set_table(t)
+        assert np.array_equal(at.get_table(), t)
+        with pytest.raises(RuntimeError, match="0, 1 or 2"):
+            at.set_table(np.full(at.NCELL, 3, dtype=np.uint8))
         with pytest.raises(ValueError):
             at.set_table(np.zeros(10, dtype=np.uint8))
         with pytest.raises(RuntimeError, match="variant"):
-            _native.call("mnpbem_cgemm_sub_variant", 2, 1, 1, 1, None, 1, 1, None, 1, 1, None, 1, 1, 1, None)
+            _native.call("mnpbem_cgemm_sub_variant", 3, 1, 1, 1, None, 1, 1, None, 1, 1, None, 1, 1, 1, None)
     finally:
         at.set_table(saved)

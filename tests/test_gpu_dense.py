@@ -82,7 +82,7 @@ def test_bf16x3_strided_operands(cuda, M, N, K):
     ref[:, :M, :N] -= A[:, :M, :K].astype(np.complex128) @ Bm[:, :K, :N].astype(np.complex128)
     tA, tB, tC = (torch.from_numpy(x).to(cuda) for x in (A, Bm, C))
     print("bf16x3 strided", M, N, K, flush=True)
-    for v in (0, 1):
+    for v in (0, 1, 2):
         tC.copy_(torch.from_numpy(C))
         nat.call("mnpbem_cgemm_sub_variant", v, M, N, K, nat.ptr(tA), lda, (M + 1) * lda, nat.ptr(tB), ldb,
                  (K + 2) * ldb, nat.ptr(tC), ldc, (M + 2) * ldc, B, nat.stream_handle(cuda))
@@ -95,9 +95,9 @@ def test_bf16x3_strided_operands(cuda, M, N, K):
 @pytest.mark.parametrize("M,N,K", [(64, 300, 64), (130, 129, 200), (17, 64, 33), (200, 70, 96), (1, 5, 1),
                                    (257, 300, 161), (129, 131, 224), (65, 66, 97), (40, 50, 31), (128, 128, 128)])
 def test_bf16x3_tile_variants_identical(cuda, M, N, K):
-    """The two tile variants of the bf16x3 GEMM (and the table-routed call)
+    """The three tile variants of the bf16x3 GEMM (and the table-routed call)
     give bit-identical results: each output element is accumulated over K in
-    the same order.  Both agree with numpy to the split-product accuracy."""
+    the same order.  All agree with numpy to the split-product accuracy."""
     nat = _lib()
     from mnpbem_amd import autotune as at
     rng = np.random.default_rng(M + 7 * N + 13 * K)
@@ -110,7 +110,7 @@ def test_bf16x3_tile_variants_identical(cuda, M, N, K):
     st = nat.stream_handle(cuda)
     print("bf16x3 variants", M, N, K, flush=True)
     outs = []
-    for v in (0, 1):
+    for v in (0, 1, 2):
         tC = torch.from_numpy(C).to(cuda)
         nat.call("mnpbem_cgemm_sub_variant", v, M, N, K, nat.ptr(tA), K, M * K, nat.ptr(tB), N, K * N, nat.ptr(tC),
                  N, M * N, B, st)
@@ -119,7 +119,7 @@ def test_bf16x3_tile_variants_identical(cuda, M, N, K):
     # products with N <= 32 and K <= 64 leave the table for the FP32 kernel
     thin = N <= 32 and K <= 64
     try:
-        for v in ((0, 1) if not thin else ()):
+        for v in ((0, 1, 2) if not thin else ()):
             at.set_table(np.full(at.NCELL, v, dtype=np.uint8))
             tC = torch.from_numpy(C).to(cuda)
             nat.call("mnpbem_cgemm_sub_batched", nat.DTYPE_C64X, M, N, K, nat.ptr(tA), K, M * K, nat.ptr(tB), N,
@@ -133,7 +133,7 @@ def test_bf16x3_tile_variants_identical(cuda, M, N, K):
 
 
 def test_autotune_records_and_installs(cuda):
-    """tune() times both variants on the recorded shapes and installs the
+    """tune() times every variant on the recorded shapes and installs the
     faster one per class; the recorded shape is the launched one."""
     nat = _lib()
     from mnpbem_amd import autotune as at
