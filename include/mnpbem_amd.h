@@ -120,6 +120,15 @@ int mnpbem_getri_blockinv_batched(int dtype, int n, void* A, long lda, long sA, 
                                   const void* Dinv, long sD, int bs, void* W, long ldw, long sW, int* Q, long sQ,
                                   int batch, void* stream);
 
+/* W <- A^-1 for A factored without row interchanges (mnpbem_getrf_nopiv_batched
+ * with no interchange recorded, so P = I): the inverse U^-1 L^-1 is built in
+ * W and left there (A is only read; no permutation work, no copy back into
+ * A).  Dinv / bs as for mnpbem_getri_blockinv_batched (NULL: substitution
+ * leaves).  Replaces, like mnpbem_getri_batched, the explicit inverse of
+ * bem_ret.py:437-523. */
+int mnpbem_getri_nopiv_blockinv_batched(int dtype, int n, const void* A, long lda, long sA, const void* Dinv, long sD,
+                                        int bs, void* W, long ldw, long sW, int batch, void* stream);
+
 /* Inverses of the 128 x 128 diagonal blocks of complex64 LU factors (as
  * [U^-1 | L^-1] per block, two 128 x 128 row-major blocks) from the 64 x 64
  * ones of mnpbem_lu_blockinv_batched (D64) and the factors' off-diagonal

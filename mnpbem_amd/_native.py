@@ -67,6 +67,7 @@ SIGNATURES = {
     "mnpbem_getri_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _p, _l, _i, _p],
     "mnpbem_getrs_blockinv_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _i, _p, _l, _l, _i, _p],
     "mnpbem_getri_blockinv_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _l, _i, _p, _l, _l, _p, _l, _i, _p],
+    "mnpbem_getri_nopiv_blockinv_batched": [_i, _i, _p, _l, _l, _p, _l, _i, _p, _l, _l, _i, _p],
     "mnpbem_lu_blockinv128_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _l, _i, _p],
     "mnpbem_accumulate": [_i, _l, _p, _p, _p],
     "mnpbem_convert": [_i, _l, _p, _p, _p],

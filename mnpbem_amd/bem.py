@@ -213,19 +213,29 @@ class BEMRet:
         nat.call("mnpbem_sigma_batched", dt, 0, n, nat.ptr(S1), nat.ptr(S2), nat.ptr(Dt), None, n, nn,
                  nat.ptr(g.nvec), nat.ptr(F.par), gd, B, st)
         lu_gd(Dt, piv[2], info[2], npflag[2], B)
-        # (Delta^-1)^T in place of the factors (LAPACK getri's operation count);
-        # F.DS holds the inverse diagonal blocks of Delta's factors until Sigma's
+        # (Delta^-1)^T (LAPACK getri's operation count) in a new buffer, or in
+        # place of the factors after interchanges; F.DS holds the inverse
+        # diagonal blocks of Delta's factors until Sigma's
         nat.call("mnpbem_lu_blockinv_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(F.DS), F.sD, B, st)
         if D128 is not None:
             D128 = D128[:B]
             nat.call("mnpbem_lu_blockinv128_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(F.DS), F.sD,
                      nat.ptr(D128), s128, B, st)
         Wk = torch.empty((B, n, n), dtype=cdt, device=dev)
-        Q = torch.empty((B, n), dtype=torch.int32, device=dev)
-        nat.call("mnpbem_getri_blockinv_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(D128),
-                 s128, 128, nat.ptr(Wk), n, nn, nat.ptr(Q), n, B, st)
-        Et = Dt
-        del Wk, Q, D128
+        if interchanges:
+            Q = torch.empty((B, n), dtype=torch.int32, device=dev)
+            nat.call("mnpbem_getri_blockinv_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(piv[2]), n, nat.ptr(D128),
+                     s128, 128, nat.ptr(Wk), n, nn, nat.ptr(Q), n, B, st)
+            Et = Dt
+            del Wk, Q
+        else:
+            # factors without interchanges (P = I): the inverse is built in Wk
+            # and stays there (no permutation, no copy back)
+            nat.call("mnpbem_getri_nopiv_blockinv_batched", dt, n, nat.ptr(Dt), n, nn, nat.ptr(D128), s128, 128,
+                     nat.ptr(Wk), n, nn, B, st)
+            Et = Wk
+            del Dt
+        del D128
         # Sigma^T overwrites Sigma2^T
         nat.call("mnpbem_sigma_batched", dt, 1, n, nat.ptr(S1), nat.ptr(S2), None, nat.ptr(Et), n, nn,
                  nat.ptr(g.nvec), nat.ptr(F.par), gd, B, st)

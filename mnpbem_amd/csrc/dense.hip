@@ -625,7 +625,8 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int N, int c0, int w, T* A, long
 // ---------------------------------------------------------------------------
 // Apply row interchanges ipiv[k0..k1) in order (forward) or reverse order to
 // columns [clo, chi).  Pivots are staged through LDS; interchanges with
-// ipiv[i] == i touch no memory.
+// ipiv[i] == i touch no memory, and chunks of 1024 pivots without any are
+// skipped whole.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_laswp(int k0, int k1, int clo, int chi, T* A, long lda,
@@ -639,11 +640,15 @@ __global__ __launch_bounds__(256) void k_laswp(int k0, int k1, int clo, int chi,
   for (int base = 0; base < n; base += 1024) {
     const int cnt = min(1024, n - base);
     __syncthreads();
+    int moved = 0;
     for (int q = threadIdx.x; q < cnt; q += 256) {
       const int i = reverse ? (k1 - 1 - base - q) : (k0 + base + q);
       sp[q] = ipiv[i];
+      moved |= sp[q] != i;
     }
-    __syncthreads();
+    // a chunk without interchanges (the factorisations without them record
+    // ipiv[i] = i) is skipped instead of walked pivot by pivot
+    if (!__syncthreads_or(moved)) continue;
     for (int q = 0; q < cnt; ++q) {
       const int i = reverse ? (k1 - 1 - base - q) : (k0 + base + q);
       const int p = sp[q];
@@ -1855,6 +1860,20 @@ int getri(int n, T* A, long lda, long sA, const int* ipiv, long sP, T* W, long l
   return 0;
 }
 
+// W <- A^-1 from LU factors made without row interchanges (P = I): the
+// inverse U^-1 L^-1 is formed in W and stays there (no permutation array,
+// no scatter back into A); A is only read.
+template <typename T>
+int getri_np(int n, const T* A, long lda, long sA, T* W, long ldw, long sW, int batch, hipStream_t st, int gmode,
+             LeafInv<T> inv = LeafInv<T>()) {
+  dim3 gi(mnp_ceil_div((long)n * n, 256), batch);
+  hipLaunchKernelGGL(k_eye<T>, gi, dim3(256), 0, st, n, W, ldw, sW);
+  MNP_LAUNCH_CHECK();
+  int rc;
+  if ((rc = trsm_lower_ident<T>(n, A, lda, sA, W, ldw, sW, batch, st, gmode, inv))) return rc;
+  return trsm_many<T>(false, n, n, A, lda, sA, W, ldw, sW, batch, st, gmode, inv);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2075,6 +2094,30 @@ extern "C" int mnpbem_getri_blockinv_batched(int dtype, int n, void* A, long lda
   inv.sD = sD;
   inv.bs = bs;
   return getri<float>(n, (float*)A, lda, sA, ipiv, sP, (float*)W, ldw, sW, Q, sQ, batch, st, 1, inv);
+}
+
+extern "C" int mnpbem_getri_nopiv_blockinv_batched(int dtype, int n, const void* A, long lda, long sA,
+                                                   const void* Dinv, long sD, int bs, void* W, long ldw, long sW,
+                                                   int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n >= 0 && batch >= 0 && lda >= n && ldw >= n, "getri_nopiv: bad size");
+  if (n == 0 || batch == 0) return 0;
+  if (dtype == MNP_DTYPE_C128)
+    return getri_np<double>(n, (const double*)A, lda, sA, (double*)W, ldw, sW, batch, st, 0);
+  if (!mnp_is_c64(dtype)) {
+    mnp_set_error("getri_nopiv: unknown dtype");
+    return 1;
+  }
+  LeafInv<float> inv;
+  if (dtype == MNP_DTYPE_C64X && Dinv != nullptr) {
+    MNP_REQUIRE(bs == 64 || bs == 128, "getri_nopiv: block size must be 64 or 128");
+    MNP_REQUIRE(sD >= (long)((n + bs - 1) / bs) * 2 * bs * bs, "getri_nopiv: Dinv stride too small");
+    inv.D = (const float*)Dinv;
+    inv.sD = sD;
+    inv.bs = bs;
+  }
+  return getri_np<float>(n, (const float*)A, lda, sA, (float*)W, ldw, sW, batch, st,
+                         dtype == MNP_DTYPE_C64X ? 1 : 0, inv);
 }
 
 // Inverses of the 128 x 128 diagonal blocks of LU factors, from those of their

@@ -616,8 +616,14 @@ __global__ __launch_bounds__(256) void k_perm_t(int n, int C, const int* __restr
   for (int base = n; base > 0; base -= 256) {
     const int lo = max(0, base - 256);
     __syncthreads();
-    if (tid < base - lo) sp[tid] = ipiv[lo + tid];
-    __syncthreads();
+    int moved = 0;
+    if (tid < base - lo) {
+      sp[tid] = ipiv[lo + tid];
+      moved = sp[tid] != lo + tid;
+    }
+    // chunks without interchanges (ipiv[i] = i, the factorisations without
+    // them) are skipped instead of walked pivot by pivot
+    if (!__syncthreads_or(moved)) continue;
     if (tid < C) {
       for (int i = base - 1; i >= lo; --i) {
         const int p = sp[i - lo];

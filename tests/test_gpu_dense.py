@@ -824,3 +824,44 @@ def test_gemv_t_paired_rows_bit_identical(cuda, n, C):
     err = np.abs(outs[0] - ref).max() / np.abs(ref).max()
     assert err < 1e-5, err
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("dt_name,bs", [("c64x", 128), ("c64x", 64), ("c64", 0), ("c128", 0)])
+@pytest.mark.parametrize("n", [65, 300, 1031])
+def test_getri_nopiv_leaves_inverse_in_w(cuda, dt_name, bs, n):
+    """mnpbem_getri_nopiv_blockinv_batched on factors without interchanges:
+    the inverse is left in W, A (the factors) is not written, and W holds the
+    same bits that mnpbem_getri_blockinv_batched writes back into A for the
+    identity permutation; both against numpy's inverse."""
+    nat = _lib()
+    dt = {"c64x": nat.DTYPE_C64X, "c64": nat.DTYPE_C64, "c128": nat.DTYPE_C128}[dt_name]
+    npdt = np.complex128 if dt_name == "c128" else np.complex64
+    rng = np.random.default_rng(11 * n + bs)
+    B = 2
+    A = (_rand(rng, (B, n, n), npdt) + (2.0 * np.sqrt(n) + 2.0) * np.eye(n)[None]).astype(npdt)
+    tA = torch.from_numpy(A.copy()).to(cuda)
+    piv = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    info = torch.empty((B,), dtype=torch.int32, device=cuda)
+    flag = torch.zeros((B,), dtype=torch.int32, device=cuda)
+    st = nat.stream_handle(cuda)
+    print("getri_nopiv", dt_name, bs, n, flush=True)
+    nat.call("mnpbem_getrf_nopiv_batched", dt, n, nat.ptr(tA), n, n * n, nat.ptr(piv), n, nat.ptr(info),
+             nat.ptr(flag), B, st)
+    assert np.all(flag.cpu().numpy() == 0) and np.array_equal(piv.cpu().numpy(), np.tile(np.arange(n), (B, 1)))
+    D, sD = (None, 0) if bs == 0 else _block_inverses(nat, tA, n, B, bs, st)
+    LU = tA.clone()
+    W = torch.full_like(tA, complex(np.nan, 0.0))
+    nat.call("mnpbem_getri_nopiv_blockinv_batched", dt, n, nat.ptr(tA), n, n * n, nat.ptr(D), sD, max(bs, 64),
+             nat.ptr(W), n, n * n, B, st)
+    W2 = torch.empty_like(tA)
+    Q = torch.empty((B, n), dtype=torch.int32, device=cuda)
+    t2 = tA.clone()
+    nat.call("mnpbem_getri_blockinv_batched", dt, n, nat.ptr(t2), n, n * n, nat.ptr(piv), n, nat.ptr(D), sD,
+             max(bs, 64), nat.ptr(W2), n, n * n, nat.ptr(Q), n, B, st)
+    assert torch.equal(tA, LU)
+    w, o2 = W.cpu().numpy(), t2.cpu().numpy()
+    assert np.array_equal(w, o2)
+    tol = 1e-12 if dt_name == "c128" else (1e-4 if dt_name == "c64x" else 1e-5)
+    for b in range(B):
+        ref = np.linalg.inv(A[b].astype(np.complex128))
+        assert np.abs(w[b] - ref).max() / np.abs(ref).max() < tol
