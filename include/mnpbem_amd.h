@@ -174,6 +174,13 @@ int mnpbem_trsm_rows_max(int cols);
  * reference counterpart). */
 int mnpbem_trsm_left_rhs(int rhs);
 
+/* 1 (built-in): in those block-row solves every block row after the first is
+ * one launch, C <- D (C - A B), the inverse-block leaf folded into the update
+ * product's epilogue (bit-identical to the two-launch form); 0: update and
+ * leaf as two launches.  Returns the previous setting.  A tuning knob of this
+ * library (no reference counterpart). */
+int mnpbem_trsm_leaf_fused(int on);
+
 /* Inverses of the 64 x 64 diagonal blocks of getrf factors ([U_II^-1 | L_II^-1]
  * per block, sD >= ceil(n/64)*2*64*64 elements) so that the block steps of
  * mnpbem_lu_solve_t_batched are parallel matrix-vector products (Dinv = NULL
@@ -594,7 +601,8 @@ int mnpbem_prof_bytes(double* total_bytes);
 int mnpbem_prof_cbytes(double* c_write_bytes);
 /* per timed launch (after mnpbem_prof_end): shapes[5 i ..] = M, N, K, batch,
  * kernel (bf16x3 tiles 0: 128 x 128, 1: 64 x 64, 2: 128 x 128 with C
- * prefetch; 3: FP32/FP64 kernel; 4: in-place 128-row product), ms[i] = its
+ * prefetch; 3: FP32/FP64 kernel; 4: in-place 128-row product; 5: block-row
+ * update with its leaf), ms[i] = its
  * HIP-event time; up to cap
  * launches; returns the number recorded.  No reference counterpart
  * (measurement only). */

@@ -262,10 +262,11 @@ __device__ __forceinline__ Tile gemm_tile(int remap) {
 // after them or right after the split of their registers) was measured
 // no faster than the compiler's own order.
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int WM, bool CPRE = (BM == 64)>
+template <int BM, int BN, int WM, bool CPRE = (BM == 64), bool LEAF = false>
 __global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3(
     int M, int N, int K, const float* __restrict__ A, long lda, long sA, const float* __restrict__ B, long ldb,
-    long sB, float* __restrict__ C, long ldc, long sC, int remap, int assign) {
+    long sB, float* __restrict__ C, long ldc, long sC, int remap, int assign, const float* __restrict__ Dl,
+    long sDl) {
   const Tile tl = gemm_tile(remap);
   const long bz = tl.z;
   A += 2 * bz * sA;
@@ -452,6 +453,93 @@ __global__ __launch_bounds__((BM / WM) * (BN / 32) * 64) void k_cgemm_sub_bf16x3
     if (s + 1 < nk) sstore(stg ^ 1);
     if (s + 2 < nk) gload_edge((s + 2) * XK);
     __syncthreads();
+  }
+  if constexpr (LEAF) {
+    // Block-row leaf in the epilogue (one row tile, M <= BM = 128): C_new =
+    // C - A B stays on chip as the B operand of X = D C_new, D the block's
+    // 128 x 128 inverse (rows and columns past M zero), and X is stored in
+    // place of C.  C_new goes to LDS as bf16 planes in the B layout, its four
+    // K slices of 32 rows in the two stages of both operand arrays; D's
+    // fragments come straight from memory with the same split.  The splits
+    // and the MFMA order are those of the stand-alone in-place product X <- D
+    // X on this tile, so the result is bit-identical to it.
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    static_assert(BM == 128 && BN == 128, "leaf epilogue on the 128 x 128 tile");
+    __syncthreads();  // every wave done with the last slice's fragments
+    auto bpl = [&](int sl, int pl) -> __bf16* { return sl < 2 ? &Bs[sl][pl][0] : &As[sl - 2][pl][0]; };
+#pragma unroll
+    for (int i = 0; i < FI; ++i)
+#pragma unroll
+      for (int j = 0; j < FJ; ++j) {
+        const int col = wn + 16 * j + fr, gc = n0 + col;
+        const int k0 = wm + 16 * i + 4 * (lane >> 4);
+        bf16x4 rh, rl, ih, il;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gr = m0 + k0 + r;
+          const float2 c = (gr < M && gc < N) ? *reinterpret_cast<const float2*>(C + 2 * ((long)gr * ldc + gc))
+                                              : make_float2(0.f, 0.f);
+          const float xr = c.x - accR[i][j][r], xi = c.y - accI[i][j][r];
+          const __bf16 hr = (__bf16)xr, hi = (__bf16)xi;
+          rh[r] = hr;
+          rl[r] = (__bf16)(xr - (float)hr);
+          ih[r] = hi;
+          il[r] = (__bf16)(xi - (float)hi);
+        }
+        const int sl = k0 >> 5, kk = k0 & 31;
+        const int off = xsw(col, kk >> 3) + (kk & 7);
+        *reinterpret_cast<bf16x4*>(bpl(sl, 0) + off) = rh;
+        *reinterpret_cast<bf16x4*>(bpl(sl, 1) + off) = rl;
+        *reinterpret_cast<bf16x4*>(bpl(sl, 2) + off) = ih;
+        *reinterpret_cast<bf16x4*>(bpl(sl, 3) + off) = il;
+      }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < FI; ++i)
+#pragma unroll
+      for (int j = 0; j < FJ; ++j) {
+        accR[i][j] = f32x4{0, 0, 0, 0};
+        accI[i][j] = f32x4{0, 0, 0, 0};
+      }
+    const float* Dp = Dl + 2 * bz * sDl;
+#pragma unroll 1
+    for (int sl = 0; sl < 4; ++sl) {
+      bf16x8 a[FI][4], b[FJ][4];
+#pragma unroll
+      for (int i = 0; i < FI; ++i) {
+        const float4* src = reinterpret_cast<const float4*>(Dp + 2 * ((long)(wm + 16 * i + fr) * BM + XK * sl + 8 * fc));
+        float v[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float4 t = src[q];
+          v[4 * q] = t.x;
+          v[4 * q + 1] = t.y;
+          v[4 * q + 2] = t.z;
+          v[4 * q + 3] = t.w;
+        }
+        split8(v, a[i][0], a[i][1]);
+        split8(v + 1, a[i][2], a[i][3]);
+      }
+#pragma unroll
+      for (int j = 0; j < FJ; ++j)
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) b[j][pl] = *reinterpret_cast<const bf16x8*>(bpl(sl, pl) + obf[j]);
+      mma(0, a, b[0]);
+      mma(1, a, b[1]);
+    }
+#pragma unroll
+    for (int i = 0; i < FI; ++i)
+#pragma unroll
+      for (int j = 0; j < FJ; ++j) {
+        const int gc = n0 + wn + 16 * j + fr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gr = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+          if (gr < M && gc < N)
+            *reinterpret_cast<float2*>(C + 2 * ((long)gr * ldc + gc)) = make_float2(accR[i][j][r], accI[i][j][r]);
+        }
+      }
+    return;
   }
 #pragma unroll
   for (int i = 0; i < FI; ++i)
@@ -850,7 +938,7 @@ struct GemmProf {
   std::vector<double> flops;
   double bytes = 0.0;   // compulsory bytes: A and B read once, C read and written once
   double cbytes = 0.0;  // of which the writes of C
-  std::vector<int> shape;  // per timed launch: M, N, K, batch, kernel (0-2: bf16x3 tile, 3: generic, 4: assign128)
+  std::vector<int> shape;  // per timed launch: M, N, K, batch, kernel (0-2: bf16x3 tile, 3: generic, 4: assign128, 5: block-row leaf)
   size_t used = 0;
 };
 GemmProf g_prof;
@@ -892,15 +980,15 @@ int launch_bf16x3(int variant, int M, int N, int K, const float* A, long lda, lo
   if (variant == 1) {
     dim3 grid(mnp_ceil_div(N, 64), mnp_ceil_div(M, 64), batch);
     hipLaunchKernelGGL((k_cgemm_sub_bf16x3<64, 64, 32>), grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB,
-                       C, ldc, sC, gemm_xcd_remap(), assign);
+                       C, ldc, sC, gemm_xcd_remap(), assign, nullptr, 0L);
   } else if (variant == 2) {
     dim3 grid(mnp_ceil_div(N, 128), mnp_ceil_div(M, 128), batch);
     hipLaunchKernelGGL((k_cgemm_sub_bf16x3<128, 128, 64, true>), grid, dim3(512), 0, st, M, N, K, A, lda, sA, B, ldb,
-                       sB, C, ldc, sC, gemm_xcd_remap(), assign);
+                       sB, C, ldc, sC, gemm_xcd_remap(), assign, nullptr, 0L);
   } else {
     dim3 grid(mnp_ceil_div(N, 128), mnp_ceil_div(M, 128), batch);
     hipLaunchKernelGGL((k_cgemm_sub_bf16x3<128, 128, 64>), grid, dim3(512), 0, st, M, N, K, A, lda, sA, B, ldb, sB,
-                       C, ldc, sC, gemm_xcd_remap(), assign);
+                       C, ldc, sC, gemm_xcd_remap(), assign, nullptr, 0L);
   }
   MNP_LAUNCH_CHECK();
   return 0;
@@ -973,6 +1061,30 @@ int launch_bf16x3_assign128(int n, int nrhs, const float* D, long ldd, long sD, 
     g_prof.shape.insert(g_prof.shape.end(), {n, nrhs, n, batch, 4});
     g_prof.bytes += (double)batch * 8.0 * ((double)n * n + 2.0 * n * nrhs);
     g_prof.cbytes += (double)batch * 8.0 * ((double)n * nrhs);
+  }
+  return 0;
+}
+
+// One block row of a left-looking solve in one launch: C <- D (C - A B) with
+// M <= 128 rows (one row tile) and D the block's 128 x 128 inverse (per
+// matrix stride sD); timed like gemm_sub (update and leaf flops).
+int launch_bf16x3_leaf(int M, int N, int K, const float* A, long lda, long sA, const float* B, long ldb, long sB,
+                       float* C, long ldc, long sC, const float* D, long sD, int batch, hipStream_t st) {
+  if (M <= 0 || N <= 0 || batch <= 0) return 0;
+  MNP_REQUIRE(M <= 128 && K > 0, "leaf product: M above 128 or K empty");
+  const bool timed = g_prof.on && g_prof.used + 2 <= g_prof.ev.size();
+  if (timed) MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used], st));
+  dim3 grid(mnp_ceil_div(N, 128), 1, batch);
+  hipLaunchKernelGGL((k_cgemm_sub_bf16x3<128, 128, 64, false, true>), grid, dim3(512), 0, st, M, N, K, A, lda, sA, B,
+                     ldb, sB, C, ldc, sC, gemm_xcd_remap(), 0, D, sD);
+  MNP_LAUNCH_CHECK();
+  if (timed) {
+    MNP_HIP_CHECK(hipEventRecord(g_prof.ev[g_prof.used + 1], st));
+    g_prof.used += 2;
+    g_prof.flops.push_back(8.0 * M * (double)N * (K + M) * batch);
+    g_prof.shape.insert(g_prof.shape.end(), {M, N, K, batch, 5});
+    g_prof.bytes += (double)batch * 8.0 * ((double)M * K + (double)K * N + 2.0 * M * N + 128.0 * 128.0);
+    g_prof.cbytes += (double)batch * 8.0 * ((double)M * N);
   }
   return 0;
 }
@@ -1099,6 +1211,10 @@ int trsm_rec(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X, 
 // recursively (set by mnpbem_trsm_left_rhs; 0 disables).
 constexpr int TRSM_LEFT_DEFAULT = 512;
 int g_trsm_left_rhs = TRSM_LEFT_DEFAULT;
+// 1: the block rows after the first fold their leaf into the update's
+// epilogue (launch_bf16x3_leaf); 0: update and leaf as two launches
+// (mnpbem_trsm_leaf_fused).
+int g_trsm_leaf_fused = 1;
 
 // X <- T^-1 X block row by block row (left-looking), with the inverse
 // diagonal blocks: for block row k of inv.bs rows (lower: first to last,
@@ -1117,6 +1233,20 @@ int trsm_left(bool lower, int n, int nrhs, const T* Tm, long ldt, long sT, T* X,
     const int k = lower ? q : nb - 1 - q;
     const int r0 = k * bs, m = min(bs, n - r0), r1 = r0 + m;
     int rc;
+    if constexpr (sizeof(T) == 4) {
+      if (g_trsm_leaf_fused && bs == 128 && (lower ? r0 > 0 : r1 < n)) {
+        const float* D = (const float*)inv.D + 2 * ((long)(inv.blk + k) * 2 * bs * bs + (lower ? bs * bs : 0));
+        if (lower)
+          rc = launch_bf16x3_leaf(m, nrhs, r0, (const float*)Tm + 2 * ((long)r0 * ldt), ldt, sT, (const float*)X, ldx,
+                                  sX, (float*)X + 2 * ((long)r0 * ldx), ldx, sX, D, inv.sD, batch, st);
+        else
+          rc = launch_bf16x3_leaf(m, nrhs, n - r1, (const float*)Tm + 2 * ((long)r0 * ldt + r1), ldt, sT,
+                                  (const float*)X + 2 * ((long)r1 * ldx), ldx, sX, (float*)X + 2 * ((long)r0 * ldx),
+                                  ldx, sX, D, inv.sD, batch, st);
+        if (rc) return rc;
+        continue;
+      }
+    }
     if (lower && r0 > 0) {
       if ((rc = gemm_sub<T>(m, nrhs, r0, Tm + 2 * ((long)r0 * ldt), ldt, sT, X, ldx, sX, X + 2 * ((long)r0 * ldx),
                             ldx, sX, batch, st, gmode)))
@@ -1923,6 +2053,15 @@ extern "C" int mnpbem_trsm_rows_max(int cols) {
 // Right-hand sides from which getrs / getri with 128-row inverse blocks solve
 // block row by block row (left-looking); rhs < 0 restores the built-in 512, 0
 // disables it (recursive solves only).  Returns the previous setting.
+// 1 (built-in): the block rows of the left-looking solves after the first
+// fold their inverse-block leaf into the update product's epilogue; 0: two
+// launches per block row.  Returns the previous setting.
+extern "C" int mnpbem_trsm_leaf_fused(int on) {
+  const int prev = g_trsm_leaf_fused;
+  g_trsm_leaf_fused = on ? 1 : 0;
+  return prev;
+}
+
 extern "C" int mnpbem_trsm_left_rhs(int rhs) {
   const int prev = g_trsm_left_rhs;
   g_trsm_left_rhs = rhs < 0 ? TRSM_LEFT_DEFAULT : rhs;
@@ -2263,7 +2402,8 @@ extern "C" int mnpbem_prof_bytes(double* total_bytes) {
 // Per-launch record of the launches timed since mnpbem_prof_begin (after
 // mnpbem_prof_end): shapes[5 i ..] = M, N, K, batch, kernel (bf16x3 tiles 0:
 // 128 x 128, 1: 64 x 64, 2: 128 x 128 with C prefetch; 3: the FP32/FP64
-// kernel; 4: in-place 128-row product) and ms[i] its HIP-event time, for up
+// kernel; 4: in-place 128-row product; 5: block-row update with its leaf)
+// and ms[i] its HIP-event time, for up
 // to cap launches;
 // returns the number of launches recorded.
 extern "C" int mnpbem_prof_launches(int cap, int* shapes, float* ms) {

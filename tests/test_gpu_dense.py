@@ -530,7 +530,9 @@ def test_getrs_getri_left_looking(cuda, n, nrhs):
     """The many-column solves with 128-row inverse blocks block row by block
     row (mnpbem_trsm_left_rhs, from 512 right-hand sides) against numpy's
     solve and inverse, as accurate as the recursive solves (knob 0) on the
-    same factors; the partial last block row included."""
+    same factors; the partial last block row included.  The leaf folded into
+    the update product (mnpbem_trsm_leaf_fused) gives the bits of the
+    two-launch form."""
     nat = _lib()
     rng = np.random.default_rng(5 * n + nrhs)
     B = 2
@@ -549,9 +551,13 @@ def test_getrs_getri_left_looking(cuda, n, nrhs):
     lib = nat.load()
     out, inv = {}, {}
     prev = lib.mnpbem_trsm_left_rhs(-1)
+    prev_f = lib.mnpbem_trsm_leaf_fused(1)
     try:
-        for knob in (0, 512):
-            lib.mnpbem_trsm_left_rhs(knob)
+        # 0: recursive; 512: block rows with the leaf folded into the update
+        # product; 513: block rows, update and leaf as two launches
+        for knob in (0, 512, 513):
+            lib.mnpbem_trsm_left_rhs(min(knob, 512))
+            lib.mnpbem_trsm_leaf_fused(0 if knob == 513 else 1)
             tX = torch.from_numpy(X.copy()).to(cuda)
             nat.call("mnpbem_getrs_blockinv_batched", nat.DTYPE_C64X, n, nrhs, nat.ptr(tA), n, n * n, nat.ptr(piv),
                      n, nat.ptr(D), sD, 128, nat.ptr(tX), nrhs, n * nrhs, B, st)
@@ -562,7 +568,10 @@ def test_getrs_getri_left_looking(cuda, n, nrhs):
             inv[knob] = t1.cpu().numpy()
     finally:
         lib.mnpbem_trsm_left_rhs(prev)
+        lib.mnpbem_trsm_leaf_fused(prev_f)
     assert np.all(info.cpu().numpy() == 0)
+    # the folded leaf runs the stand-alone leaf's splits and MFMA order
+    assert np.array_equal(out[512], out[513]) and np.array_equal(inv[512], inv[513])
     scale = np.abs(ref).max()
     e_left = np.abs(out[512] - ref).max() / scale
     e_rec = np.abs(out[0] - ref).max() / scale
