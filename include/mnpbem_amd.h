@@ -174,6 +174,12 @@ int mnpbem_trsm_rows_max(int cols);
  * reference counterpart). */
 int mnpbem_trsm_left_rhs(int rhs);
 
+/* Widest complex64 bf16x3-path product (N columns, with K <= 64) that runs on
+ * the FP32 vector kernel for thin products instead of the bf16x3 tiles: 0..32
+ * (0: none), n < 0 restores the built-in 16.  Returns the previous setting.
+ * A tuning knob of this library (no reference counterpart). */
+int mnpbem_gemm_thin_max(int n);
+
 /* 1 (built-in): in those block-row solves every block row after the first is
  * one launch, C <- D (C - A B), the inverse-block leaf folded into the update
  * product's epilogue (bit-identical to the two-launch form); 0: update and

@@ -890,8 +890,13 @@ __global__ void k_fill_int(int* p, long n, int v) {
 // of k, then C - sum), rows of 256 / N per workgroup, B staged in LDS.  More
 // accurate than the split-bf16 product.  Measured on the 2884-face sweep:
 // 149.5 wavelengths/s with every product on the bf16x3 tiles, 152.5 with this
-// kernel up to N = 16, 154.5 up to N = 32, 148.9 up to N = 64.
+// kernel up to N = 16, 154.5 up to N = 32, 148.9 up to N = 64.  Since the
+// 64 x 64 tile reads its C tile ahead of its K loop, it takes the N = 17..32
+// products (M ~2800, K = 64) faster: one of them ran ~300 us here at batch
+// 128; sweep 295-300 -> 290-292 ms with this kernel up to N = 16 only.
 constexpr int THIN_N = 32, THIN_K = 64;
+constexpr int THIN_N_DEFAULT = 16;
+int g_thin_n = THIN_N_DEFAULT;  // products with N <= g_thin_n (and K <= THIN_K) use k_cgemm_thin
 __global__ __launch_bounds__(256) void k_cgemm_thin(int M, int N, int K, const float* __restrict__ A, long lda,
                                                     long sA, const float* __restrict__ B, long ldb, long sB,
                                                     float* __restrict__ C, long ldc, long sC) {
@@ -1005,7 +1010,7 @@ int gemm_sub(int M, int N, int K, const T* A, long lda, long sA, const T* B, lon
              long sC, int batch, hipStream_t st, int gmode, int assign = 0) {
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return 0;
   if constexpr (sizeof(T) == 4) {
-    if (gmode == 1 && !assign && N <= THIN_N && K <= THIN_K) {
+    if (gmode == 1 && !assign && N <= g_thin_n && K <= THIN_K) {
       dim3 grid(mnp_ceil_div(M, 256 / N), batch);
       hipLaunchKernelGGL(k_cgemm_thin, grid, dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC);
       MNP_LAUNCH_CHECK();
@@ -2059,6 +2064,15 @@ extern "C" int mnpbem_trsm_rows_max(int cols) {
 extern "C" int mnpbem_trsm_leaf_fused(int on) {
   const int prev = g_trsm_leaf_fused;
   g_trsm_leaf_fused = on ? 1 : 0;
+  return prev;
+}
+
+// Widest product (N columns, K <= 64) that goes to the FP32 vector kernel
+// k_cgemm_thin instead of the bf16x3 tiles: n in 0..32 (0: none), n < 0
+// restores the built-in 16.  Returns the previous setting.
+extern "C" int mnpbem_gemm_thin_max(int n) {
+  const int prev = g_thin_n;
+  g_thin_n = n < 0 ? THIN_N_DEFAULT : (n > THIN_N ? THIN_N : n);
   return prev;
 }
 

@@ -45,8 +45,9 @@ def test_cgemm_sub(cuda, dtype, tol, split, M, N, K):
                                    (2884, 32, 32), (129, 31, 64)])
 def test_thin_products_fp32(cuda, M, N, K):
     """Complex64 products with N <= 32 and K <= 64 (the bottom of the LU
-    recursion) run on the FP32 vector kernel: against numpy to FP32 accuracy,
-    with strided operands and the padding of C untouched."""
+    recursion) on the FP32 vector kernel (mnpbem_gemm_thin_max, built in for
+    N <= 16): against numpy to FP32 accuracy, with strided operands and the
+    padding of C untouched."""
     nat = _lib()
     rng = np.random.default_rng(M + 3 * N + 7 * K)
     B = 3
@@ -58,8 +59,13 @@ def test_thin_products_fp32(cuda, M, N, K):
     ref[:, :, :N] -= A[:, :, :K].astype(np.complex128) @ Bm[:, :, :N].astype(np.complex128)
     tA, tB, tC = (torch.from_numpy(x).to(cuda) for x in (A, Bm, C))
     print("thin product", M, N, K, flush=True)
-    nat.call("mnpbem_cgemm_sub_batched", nat.DTYPE_C64X, M, N, K, nat.ptr(tA), lda, M * lda, nat.ptr(tB), ldb,
-             K * ldb, nat.ptr(tC), ldc, M * ldc, B, nat.stream_handle(cuda))
+    lib = nat.load()
+    prev = lib.mnpbem_gemm_thin_max(32)  # every N <= 32 on the FP32 kernel (built-in: N <= 16)
+    try:
+        nat.call("mnpbem_cgemm_sub_batched", nat.DTYPE_C64X, M, N, K, nat.ptr(tA), lda, M * lda, nat.ptr(tB), ldb,
+                 K * ldb, nat.ptr(tC), ldc, M * ldc, B, nat.stream_handle(cuda))
+    finally:
+        lib.mnpbem_gemm_thin_max(prev)
     out = tC.cpu().numpy()
     scale = np.abs(ref[:, :, :N]).max() + np.sqrt(K) * 2.0
     assert np.abs(out[:, :, :N] - ref[:, :, :N]).max() / scale < 3e-6
@@ -116,8 +122,8 @@ def test_bf16x3_tile_variants_identical(cuda, M, N, K):
                  N, M * N, B, st)
         outs.append(tC.cpu().numpy())
     saved = at.get_table()
-    # products with N <= 32 and K <= 64 leave the table for the FP32 kernel
-    thin = N <= 32 and K <= 64
+    # products with N <= 16 and K <= 64 leave the table for the FP32 kernel
+    thin = N <= 16 and K <= 64
     try:
         for v in ((0, 1, 2) if not thin else ()):
             at.set_table(np.full(at.NCELL, v, dtype=np.uint8))
