@@ -267,6 +267,7 @@ class BatchSweepGraph:
                                 "pw": torch.empty((B, 7), dtype=torch.float64, device=dev)})
         self.graph = None
         self.ext, self.norms = None, None
+        self.fallbacks = 0  # groups of a replayed block solved again eagerly
 
     def _fill(self, lams):
         for gr, buf in zip(self.batch.groups, self.inputs):
@@ -309,22 +310,34 @@ class BatchSweepGraph:
             with torch.cuda.graph(self.graph):
                 self._step(lams)
         self.graph.replay()
-        if any(needs_interchanges(f) for f in self.npflags):
-            # the captured factorisation ran without the row interchanges
-            # that this block needs: the eager path factors it with them
-            return self.batch.solve_block(lams, self.pw)
-        worst = 0.0
-        for nrm in self.norms:
+        groups = self.batch.groups
+        out = [e.reshape(gr.S, self.L, self.pw.npol) for gr, e in zip(groups, self.ext)]
+        worst, redo = 0.0, []
+        for gi, (nrm, fl) in enumerate(zip(self.norms, self.npflags)):
+            if needs_interchanges(fl):
+                # the captured factorisation ran without the row interchanges
+                # that this group's block needs: the eager path factors it
+                # with them
+                redo.append(gi)
+                continue
             h = nrm.cpu().numpy()
-            worst = max(worst, float(np.max(np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300)))))
+            w = float(np.max(np.sqrt(h[:, 0] / np.maximum(h[:, 1], 1e-300))))
+            if w > self.batch.refine_tol:
+                if fl is None:
+                    raise RuntimeError("BatchSweepGraph: residual %.2e above refine_tol after %d steps; use more "
+                                       "iters" % (w, self.iters))
+                # Sigma without interchanges needs more steps than captured:
+                # this group alone is solved again eagerly (its refinement
+                # runs until refine_tol), the other groups keep the replay's
+                redo.append(gi)
+                continue
+            worst = max(worst, w)
+        for gi in redo:
+            out[gi] = self.batch.solve_block(lams, self.pw, group=groups[gi])
+            worst = max(worst, self.batch.last_residual)
+        self.fallbacks += len(redo)
         self.last_residual = worst
-        if worst > self.batch.refine_tol and any(f is not None for f in self.npflags):
-            # Sigma without interchanges needs more steps than captured
-            return self.batch.solve_block(lams, self.pw)
-        if worst > self.batch.refine_tol:
-            raise RuntimeError("BatchSweepGraph: residual %.2e above refine_tol after %d steps; use more iters"
-                               % (worst, self.iters))
-        return [e.reshape(gr.S, self.L, self.pw.npol) for gr, e in zip(self.batch.groups, self.ext)]
+        return out
 
     def solve_sweep(self, enei_list):
         """Extinction spectra of every particle over ``enei_list`` (a multiple

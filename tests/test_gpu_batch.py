@@ -106,3 +106,20 @@ def test_batched_graph_replay(batch_setup):
         assert np.array_equal(first[k], second[k])
         assert np.max(np.abs(second[k] / eager[k] - 1)) < 1e-12
     assert sg.last_residual <= bb.refine_tol
+
+
+def test_batched_graph_too_few_steps_resolves_groups(batch_setup):
+    """A replay whose captured refinement steps miss refine_tol (one step
+    captured) solves the failing groups again eagerly, group by group, and
+    keeps the replay's results for the rest: the spectra equal the eager
+    solve's and the final residual meets refine_tol."""
+    from mnpbem_amd.batch import BEMRetBatch, BatchSweepGraph
+    specs, parts, pw, single = batch_setup
+    bb = BEMRetBatch(parts, precision="mixed")
+    eager = bb.solve_sweep(LAMS, pw, block=3)
+    print("BatchSweepGraph L=3 iters=1", flush=True)
+    sg = BatchSweepGraph(bb, pw, 3, iters=1)
+    out = sg.solve_sweep(LAMS)
+    assert sg.fallbacks > 0 and sg.last_residual <= bb.refine_tol
+    for k in range(len(parts)):
+        assert np.max(np.abs(out[k] / eager[k] - 1)) < 1e-12
