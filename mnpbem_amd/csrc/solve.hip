@@ -1891,8 +1891,13 @@ __global__ __launch_bounds__(256) void k_perm_n(int n, int C, const int* __restr
   for (int lo = 0; lo < n; lo += 256) {
     const int hi = min(n, lo + 256);
     __syncthreads();
-    if (tid < hi - lo) sp[tid] = ipiv[lo + tid];
-    __syncthreads();
+    int moved = 0;
+    if (tid < hi - lo) {
+      sp[tid] = ipiv[lo + tid];
+      moved = sp[tid] != lo + tid;
+    }
+    // chunks without interchanges are skipped (as in k_perm_t)
+    if (!__syncthreads_or(moved)) continue;
     if (tid < C) {
       for (int i = lo; i < hi; ++i) {
         const int p = sp[i - lo];
