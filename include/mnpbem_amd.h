@@ -333,17 +333,24 @@ int mnpbem_ext_functional(int n, int P, const double* pos, const double* area, c
 int mnpbem_grad_prep(int n, int P, const void* X, const void* Y, long ldv, long sv, const double* nvec,
                      const double* par, void* L, void* R, void* Wc, int batch, void* stream);
 /* far-field pairs and the far terms of the closed-surface correction; rowptr
- * (int64, n+1) / pcol and colptr (n+1) / crow: refined pairs by row and by column */
+ * (int64, n+1) / pcol and colptr (n+1) / crow: refined pairs by row and by column.
+ * Each face's sums are formed by one workgroup in a fixed order (no atomics). */
 int mnpbem_grad_far(int n, int P, const double* pos, const double* nvec, const double* area, const int* mat0,
                     const int* mat1, const int* surf, const double* kmat, int nmat, const long* rowptr,
                     const int* pcol, const int* colptr, const int* crow, const void* L, const void* R,
                     const void* Wc, double* Gr, int batch, void* stream);
-/* refined (near and self) pairs: the refinement quadrature in dual numbers */
+/* refined (near and self) pairs: the refinement quadrature in dual numbers.
+ * Every (pair, derivative group) writes its terms to the work space Ws
+ * (ws_len >= batch * P * 11 * npairs * 5 doubles), which a second kernel adds
+ * per face in a fixed order: by row (rowptr, int64, n+1) and by column
+ * (colptr, n+1, and cpair: the pair indices sorted by column).  Gr receives
+ * the same sums in the same order on every run (no atomics). */
 int mnpbem_grad_refined(int order, int n, int P, int npairs, const int* prow, const int* pcol, const int* pkind,
                         const double* pos, const double* nvec, const double* tvec1, const double* tvec2,
                         const double* area, const double* nodes, const int* mat0, const int* mat1, const int* surf,
                         const double* kmat, int nmat, const double* qnear, int nq_near, const double* qself,
-                        int nq_self, const void* L, const void* R, const void* Wc, double* Gr, int batch,
+                        int nq_self, const void* L, const void* R, const void* Wc, const long* rowptr,
+                        const int* colptr, const int* cpair, double* Ws, long ws_len, double* Gr, int batch,
                         void* stream);
 /* per-face terms: explicit normals of A (Q = [G1 X1 | G2 X2]), the plane-wave
  * right side Rb and the functional; corr: closed-surface correction per face */

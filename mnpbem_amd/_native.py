@@ -59,7 +59,8 @@ SIGNATURES = {
     "mnpbem_ext_functional": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _i, _p],
     "mnpbem_grad_prep": [_i, _i, _p, _p, _l, _l, _p, _p, _p, _p, _p, _i, _p],
     "mnpbem_grad_far": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p],
-    "mnpbem_grad_refined": [_i, _i, _i, _i] + [_p] * 13 + [_i, _p, _i, _p, _i, _p, _p, _p, _p, _i, _p],
+    "mnpbem_grad_refined": [_i, _i, _i, _i] + [_p] * 13 + [_i, _p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _l,
+                                                         _p, _i, _p],
     "mnpbem_grad_local": [_i, _i, _p, _p, _p, _p, _l, _l] + [_p] * 11 + [_i, _p],
     "mnpbem_getrf_tall_rows": [_i],
     "mnpbem_trsm_rows_max": [_i],

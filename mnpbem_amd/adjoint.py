@@ -257,11 +257,14 @@ def extinction_with_shape_gradient(bem, enei_list, planewave, chunk=16, per_face
                  nat.ptr(g.mat[1]), nat.ptr(g._surf), nat.ptr(F.kmat), g.nmat, nat.ptr(g._rowptr()),
                  nat.ptr(g.pcol), nat.ptr(g._colptr), nat.ptr(g._crow), nat.ptr(Lw), nat.ptr(Rw), nat.ptr(Wc),
                  nat.ptr(Gr), B, st)
+        Ws = torch.empty((B * P * 11 * g.npairs * 5,), dtype=torch.float64, device=dev)
         nat.call("mnpbem_grad_refined", g.order, n, P, g.npairs, nat.ptr(g.prow), nat.ptr(g.pcol),
                  nat.ptr(g._pkind), nat.ptr(g.pos), nat.ptr(g.nvec), nat.ptr(g._tv1), nat.ptr(g._tv2),
                  nat.ptr(g.area), nat.ptr(g._nodes), nat.ptr(g.mat[0]), nat.ptr(g.mat[1]), nat.ptr(g._surf),
                  nat.ptr(F.kmat), g.nmat, nat.ptr(g._qn), g._qn.shape[0], nat.ptr(g._qs), g._qs.shape[0],
-                 nat.ptr(Lw), nat.ptr(Rw), nat.ptr(Wc), nat.ptr(Gr), B, st)
+                 nat.ptr(Lw), nat.ptr(Rw), nat.ptr(Wc), nat.ptr(g._rowptr()), nat.ptr(g._colptr),
+                 nat.ptr(g._cpair), nat.ptr(Ws), Ws.numel(), nat.ptr(Gr), B, st)
+        del Ws
         nat.call("mnpbem_grad_local", n, P, nat.ptr(X), nat.ptr(Y), nat.ptr(Q), nat.ptr(R), 8 * P, n * 8 * P,
                  nat.ptr(g.pos), nat.ptr(g.nvec), nat.ptr(g.area), nat.ptr(inm), nat.ptr(outm), nat.ptr(pol),
                  nat.ptr(dr), nat.ptr(pw), nat.ptr(g.corr), nat.ptr(Wc), nat.ptr(Gr), B, st)

@@ -147,11 +147,13 @@ __device__ __forceinline__ void weights(const double* L, const double* R, long b
 //   dG/da_j = e/d, dF/da_j = (n_i.r) h,
 // and the closed-surface correction corr_j = -2 pi - sum_i term_ij with the
 // far terms term_ij = -area_i (n_i.r) / d^3 (i in j's closed surface).
-// PASS 0: thread = row i (pos_i, nvec_i, area_i); PASS 1: thread = column j
-// (pos_j, area_j).  grid (ceil(n/256), nchunk, batch), each chunk a range of
-// the other index; results are added atomically.
+// PASS 0: lane = row i (pos_i, nvec_i, area_i); PASS 1: lane = column j
+// (pos_j, area_j).  grid (ceil(n/64), 1, batch): 64 own faces per workgroup,
+// its four waves take four quarters of the other index, the quarters' sums
+// are added in wave order through LDS and each own face's slots are written
+// by one thread (no atomics: the same sums in the same order on every run).
 template <int PASS>
-__global__ __launch_bounds__(256) void k_grad_far(int n, int P, int chunk, const double* __restrict__ pos,
+__global__ __launch_bounds__(256) void k_grad_far(int n, int P, const double* __restrict__ pos,
                                                   const double* __restrict__ nvec, const double* __restrict__ area,
                                                   const int* __restrict__ mat0, const int* __restrict__ mat1,
                                                   const int* __restrict__ surf, const double* __restrict__ kmat,
@@ -161,9 +163,11 @@ __global__ __launch_bounds__(256) void k_grad_far(int n, int P, int chunk, const
                                                   const double* __restrict__ R, const double* __restrict__ Wc,
                                                   double* Gr) {
   const long b = blockIdx.z;
-  const int own = blockIdx.x * 256 + threadIdx.x;
-  const int o0 = blockIdx.y * chunk, o1 = min(n, o0 + chunk);
-  if (own >= n || o0 >= n) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool act = blockIdx.x * 64 + lane < n;
+  const int own = act ? blockIdx.x * 64 + lane : n - 1;
+  const int quarter = (n + 3) / 4;
+  const int o0 = min(n, wv * quarter), o1 = act ? min(n, o0 + quarter) : o0;
   // sorted refined partners of the own face
   int q, qend;
   const int* part;
@@ -259,14 +263,27 @@ __global__ __launch_bounds__(256) void k_grad_far(int n, int P, int chunk, const
       }
     }
   }
+  __shared__ double wsum[3][64][PM * 7];
+  if (wv > 0)
+#pragma unroll
+    for (int p = 0; p < PM; ++p)
+#pragma unroll
+      for (int t = 0; t < 7; ++t) wsum[wv - 1][lane][p * 7 + t] = acc[p][t];
+  __syncthreads();
+  if (wv > 0 || !act) return;
+  for (int w = 0; w < 3; ++w)
+#pragma unroll
+    for (int p = 0; p < PM; ++p)
+#pragma unroll
+      for (int t = 0; t < 7; ++t) acc[p][t] += wsum[w][lane][p * 7 + t];
   for (int p = 0; p < P && p < PM; ++p) {
     double* gr = Gr + ((b * P + p) * n + own) * (long)NG;
-    for (int c = 0; c < 3; ++c) atomicAdd(gr + S_POS + c, acc[p][c]);
+    for (int c = 0; c < 3; ++c) gr[S_POS + c] += acc[p][c];
     if (PASS == 0) {
-      for (int c = 0; c < 3; ++c) atomicAdd(gr + S_NVEC + c, acc[p][3 + c]);
-      atomicAdd(gr + S_AREA, acc[p][6]);
+      for (int c = 0; c < 3; ++c) gr[S_NVEC + c] += acc[p][3 + c];
+      gr[S_AREA] += acc[p][6];
     } else {
-      atomicAdd(gr + S_AREA, acc[p][3]);
+      gr[S_AREA] += acc[p][3];
     }
   }
 }
@@ -344,7 +361,10 @@ __device__ __forceinline__ QP3 face_point_d3(const double* X, const double* tq, 
 // differentiated and contracted with the weights, plus the closed-surface
 // term area_i fstat / a_j of corr_j.  grid (ceil(npairs/256), 11 groups):
 // group 0 pos_i, 1 nvec_i, 2 tvec1_i, 3 tvec2_i, 4..9 node (g-4) of face j,
-// 10 pos_j.
+// 10 pos_j.  Each (pair, group) writes its 3 derivative values (and, group 0,
+// the two area terms of the closed-surface correction for faces i and j) to
+// the work space Ws[((b P + p) 11 + group) npairs + pair][5]; k_grad_refined_sum
+// adds them per face in a fixed order (no atomics).
 template <int ORD>
 __global__ __launch_bounds__(256) void k_grad_refined(
     int n, int P, int batch, int npairs, const int* __restrict__ prow, const int* __restrict__ pcol,
@@ -353,7 +373,7 @@ __global__ __launch_bounds__(256) void k_grad_refined(
     const double* __restrict__ nodes, const int* __restrict__ mat0, const int* __restrict__ mat1,
     const int* __restrict__ surf, const double* __restrict__ kmat, int nmat, const double* __restrict__ qnear,
     int nq_near, const double* __restrict__ qself, int nq_self, const double* __restrict__ L,
-    const double* __restrict__ R, const double* __restrict__ Wc, double* Gr) {
+    const double* __restrict__ R, const double* __restrict__ Wc, double* __restrict__ Ws) {
   extern __shared__ double qs[];
   double* qn = qs;
   double* qp = qs + nq_near * QSTRIDE;
@@ -492,9 +512,6 @@ __global__ __launch_bounds__(256) void k_grad_refined(
   }
   const bool same_surf = surf[i] == surf[j];
   const double ai = area[i], aj = area[j];
-  const int target = grp <= 3 ? i : j;
-  const int slot = grp == 0 ? S_POS : grp == 1 ? S_NVEC : grp == 2 ? S_T1 : grp == 3 ? S_T2
-                 : grp == 10 ? S_POS : S_NODES + 3 * (grp - 4);
   for (long b = 0; b < batch; ++b) {
     cplx<double> ag[2], af[2], dag[2][3], daf[2][3], E[2];
     bool on[2];
@@ -536,7 +553,8 @@ __global__ __launch_bounds__(256) void k_grad_refined(
         weights(L, R, b, n, P, i, j, s, p, wg, wh);
         for (int t = 0; t < 3; ++t) out[t] += im_mul(wg, dag[s][t]) + im_mul(wh, daf[s][t]);
       }
-      double* gr = Gr + ((b * P + p) * n) * (long)NG;
+      double* ws = Ws + ((((b * P + p) * 11 + grp) * (long)npairs) + pr) * 5;
+      double ar_i = 0.0, ar_j = 0.0;
       if (i == j && grp == 1) {
         // d(n.n)/dn * fstat of the cancelled static part (n.n) (fstat + corr_i);
         // the corr_i half is added with the per-face terms
@@ -547,13 +565,64 @@ __global__ __launch_bounds__(256) void k_grad_refined(
         const double wc = Wc[2 * ((b * n + j) * P + p) + 1];
         for (int t = 0; t < 3; ++t) out[t] -= wc * ai * fq.d[t] / aj;
         if (grp == 0) {
-          atomicAdd(gr + (long)i * NG + S_AREA, -wc * fq.v / aj);
-          atomicAdd(gr + (long)j * NG + S_AREA, wc * ai * fq.v / (aj * aj));
+          ar_i = -wc * fq.v / aj;
+          ar_j = wc * ai * fq.v / (aj * aj);
         }
       }
-      for (int t = 0; t < 3; ++t) atomicAdd(gr + (long)target * NG + slot + t, out[t]);
+      for (int t = 0; t < 3; ++t) ws[t] = out[t];
+      ws[3] = ar_i;
+      ws[4] = ar_j;
     }
   }
+}
+
+// Per face and group, the sums of k_grad_refined's work space into Gr in a
+// fixed order: groups 0..3 (derivatives at face i) over the face's pairs by
+// row, groups 4..9 (nodes of face j) over its pairs by column; the thread of
+// group 0 also adds group 10 (pos_j, by column) to the same position slots and
+// both area terms of the closed-surface correction (by row, then by column).
+// grid (ceil(n/256), 10, batch P).
+__global__ __launch_bounds__(256) void k_grad_refined_sum(int n, int npairs, const long* __restrict__ rowptr,
+                                                          const int* __restrict__ colptr,
+                                                          const int* __restrict__ cpair,
+                                                          const double* __restrict__ Ws, double* Gr) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  const int grp = blockIdx.y;
+  const long bp = blockIdx.z;
+  if (f >= n) return;
+  const double* wg = Ws + (bp * 11 + grp) * (long)npairs * 5;
+  const int slot = grp == 0 ? S_POS : grp == 1 ? S_NVEC : grp == 2 ? S_T1 : grp == 3 ? S_T2
+                 : S_NODES + 3 * (grp - 4);
+  double s[3] = {0.0, 0.0, 0.0}, sa = 0.0;
+  if (grp <= 3) {
+    for (long q = rowptr[f]; q < rowptr[f + 1]; ++q) {
+      const double* w = wg + q * 5;
+      s[0] += w[0];
+      s[1] += w[1];
+      s[2] += w[2];
+      sa += w[3];
+    }
+  } else {
+    for (int q = colptr[f]; q < colptr[f + 1]; ++q) {
+      const double* w = wg + (long)cpair[q] * 5;
+      s[0] += w[0];
+      s[1] += w[1];
+      s[2] += w[2];
+    }
+  }
+  if (grp == 0) {
+    const double* w10 = Ws + (bp * 11 + 10) * (long)npairs * 5;
+    for (int q = colptr[f]; q < colptr[f + 1]; ++q) {
+      const double* w = w10 + (long)cpair[q] * 5;
+      s[0] += w[0];
+      s[1] += w[1];
+      s[2] += w[2];
+      sa += wg[(long)cpair[q] * 5 + 4];
+    }
+  }
+  double* gr = Gr + (bp * n + f) * (long)NG;
+  for (int t = 0; t < 3; ++t) gr[slot + t] += s[t];
+  if (grp == 0) gr[S_AREA] += sa;
 }
 
 // ---------------------------------------------------------------------------
@@ -672,13 +741,12 @@ extern "C" int mnpbem_grad_far(int n, int P, const double* pos, const double* nv
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(n > 0 && P > 0 && P <= 4, "grad_far: bad size (P <= 4)");
   if (batch <= 0) return 0;
-  const int chunk = 128;
-  dim3 grid(mnp_ceil_div(n, 256), mnp_ceil_div(n, chunk), batch);
-  hipLaunchKernelGGL(k_grad_far<0>, grid, dim3(256), 0, st, n, P, chunk, pos, nvec, area, mat0, mat1, surf, kmat,
-                     nmat, rowptr, pcol, colptr, crow, (const double*)L, (const double*)R, (const double*)Wc, Gr);
+  dim3 grid(mnp_ceil_div(n, 64), 1, batch);
+  hipLaunchKernelGGL(k_grad_far<0>, grid, dim3(256), 0, st, n, P, pos, nvec, area, mat0, mat1, surf, kmat, nmat,
+                     rowptr, pcol, colptr, crow, (const double*)L, (const double*)R, (const double*)Wc, Gr);
   MNP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_grad_far<1>, grid, dim3(256), 0, st, n, P, chunk, pos, nvec, area, mat0, mat1, surf, kmat,
-                     nmat, rowptr, pcol, colptr, crow, (const double*)L, (const double*)R, (const double*)Wc, Gr);
+  hipLaunchKernelGGL(k_grad_far<1>, grid, dim3(256), 0, st, n, P, pos, nvec, area, mat0, mat1, surf, kmat, nmat,
+                     rowptr, pcol, colptr, crow, (const double*)L, (const double*)R, (const double*)Wc, Gr);
   MNP_LAUNCH_CHECK();
   return 0;
 }
@@ -688,26 +756,32 @@ extern "C" int mnpbem_grad_refined(int order, int n, int P, int npairs, const in
                                    const double* tvec2, const double* area, const double* nodes, const int* mat0,
                                    const int* mat1, const int* surf, const double* kmat, int nmat,
                                    const double* qnear, int nq_near, const double* qself, int nq_self,
-                                   const void* L, const void* R, const void* Wc, double* Gr, int batch,
-                                   void* stream) {
+                                   const void* L, const void* R, const void* Wc, const long* rowptr,
+                                   const int* colptr, const int* cpair, double* Ws, long ws_len, double* Gr,
+                                   int batch, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   MNP_REQUIRE(order >= 0 && order <= 7, "grad_refined: order must be in 0..7");
   MNP_REQUIRE(n > 0 && P > 0 && P <= 4, "grad_refined: bad size (P <= 4)");
   MNP_REQUIRE(nq_near > 0 && nq_self > 0 && (nq_near + nq_self) * QSTRIDE * 8 <= 60000,
               "grad_refined: quadrature tables too large");
   if (npairs == 0 || batch <= 0) return 0;
+  MNP_REQUIRE(ws_len >= (long)batch * P * 11 * npairs * 5,
+              "grad_refined: work space below batch * P * 11 * npairs * 5 doubles");
   const size_t shm = (size_t)(nq_near + nq_self) * QSTRIDE * sizeof(double);
   dim3 grid(mnp_ceil_div(npairs, 256), 11);
 #define MNP_GR(O)                                                                                                  \
   case O:                                                                                                          \
     hipLaunchKernelGGL(k_grad_refined<O>, grid, dim3(256), shm, st, n, P, batch, npairs, prow, pcol, pkind, pos,   \
                        nvec, tvec1, tvec2, area, nodes, mat0, mat1, surf, kmat, nmat, qnear, nq_near, qself,       \
-                       nq_self, (const double*)L, (const double*)R, (const double*)Wc, Gr);                        \
+                       nq_self, (const double*)L, (const double*)R, (const double*)Wc, Ws);                        \
     break;
   switch (order) {
     MNP_GR(0) MNP_GR(1) MNP_GR(2) MNP_GR(3) MNP_GR(4) MNP_GR(5) MNP_GR(6) MNP_GR(7)
   }
 #undef MNP_GR
+  MNP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_grad_refined_sum, dim3(mnp_ceil_div(n, 256), 10, batch * P), dim3(256), 0, st, n, npairs,
+                     rowptr, colptr, cpair, (const double*)Ws, Gr);
   MNP_LAUNCH_CHECK();
   return 0;
 }

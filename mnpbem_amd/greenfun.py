@@ -115,7 +115,7 @@ class CompGreenRet:
         keep = [_dev(a, i32, dev) for a in (lo, hi, colptr, crow, cpair)]
         # kept for the geometry derivatives: refined rows per column, the
         # closed surface of each face and the quadrature tables
-        self._colptr, self._crow = keep[2], keep[3]
+        self._colptr, self._crow, self._cpair = keep[2], keep[3], keep[4]
         self._surf = _dev(surf, i32, dev)
         self._qn, self._qs = qn, qs
         nat.call("mnpbem_closed_correction", n, nat.ptr(self.pos), nat.ptr(self.nvec), nat.ptr(self.area),
