@@ -272,6 +272,20 @@ int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t, const int
                              const void* R, void* W, void* X, long ldv, long sv, const double* nvec,
                              const double* par, int gdiv, int batch, void* stream);
 
+/* mnpbem_bem_solve_batched with explicit inverses instead of the LU factors of
+ * G1, G2 and Sigma: G1i, G2i, Si are the transposed buffers of G1^-1, G2^-1,
+ * Sigma^-1 (getri on the factors of the transposed buffers), so every solve is
+ * a matrix-vector product.  For a factorisation applied many times to a few
+ * right-hand sides (the block preconditioner inside GMRES), where the
+ * triangular steps of a few systems are latency-bound.  Same vectors, work
+ * space and parameters as mnpbem_bem_solve_batched; replaces, like it, the
+ * solve of bem_ret.py:1569-1831 (with the reference's own explicit G1^-1,
+ * G2^-1 of bem_ret.py:437-523). */
+int mnpbem_bem_solve_inv_batched(int dtype, int n, int P, const void* G1i, const void* G2i, const void* S1t,
+                                 const void* Et, const void* Si, long ldm, long smat, const void* R, void* W, void* X,
+                                 long ldv, long sv, const double* nvec, const double* par, int gdiv, int batch,
+                                 void* stream);
+
 /* mnpbem_bem_solve_batched without the work space argument W: no counterpart
  * in the reference.  For callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -46,6 +46,8 @@ SIGNATURES = {
     "mnpbem_sigma_batched": [_i, _i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _i, _i, _p],
     "mnpbem_bem_solve_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l, _p,
                                  _p, _p, _l, _l, _p, _p, _i, _i, _p],
+    "mnpbem_bem_solve_inv_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _l, _l, _p, _p, _p, _l, _l, _p, _p, _i, _i,
+                                     _p],
     "mnpbem_bem_solve_batched_noalloc": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l,
                                          _p, _p, _l, _l, _p, _p, _i, _i, _p],
     "mnpbem_bem_residual_batched": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _l, _l, _p, _l, _l, _p, _p,

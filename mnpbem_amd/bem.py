@@ -252,7 +252,31 @@ class BEMRet:
             return self.factor(enei_list, kmat=kmat, par=par, store=store, interchanges=True)
         return F
 
+    def invert_factors(self, F):
+        """Replace the LU factors of G1^T, G2^T and Sigma^T in ``F`` by the
+        inverses (getri in place; the transposed buffers of G1^-1, G2^-1,
+        Sigma^-1), for a factorisation that is applied many times to a few
+        right-hand sides (the block preconditioner inside GMRES): each solve
+        is then a matrix-vector product (``solve_inverse``) instead of chains
+        of latency-bound triangular steps.  Work space: one n x n matrix per
+        wavelength while it runs.  ``F`` no longer serves ``_solve_direct``."""
+        if getattr(F, "inverted", False):
+            return F
+        n, B, dt = F.n, F.B, F.dtype
+        nn = n * n
+        st = nat.stream_handle(self.device)
+        W = torch.empty((B, n, n), dtype=F.cdt, device=self.device)
+        Q = torch.empty((B, n), dtype=torch.int32, device=self.device)
+        for A, piv, D in ((F.GG[0], F.piv[0], F.DG[0]), (F.GG[1], F.piv[1], F.DG[1]), (F.HH[1], F.piv[3], F.DS)):
+            nat.call("mnpbem_getri_blockinv_batched", dt, n, nat.ptr(A), n, nn, nat.ptr(piv), n, nat.ptr(D), F.sD, 64,
+                     nat.ptr(W), n, nn, nat.ptr(Q), n, B, st)
+        del W, Q
+        F.inverted = True
+        return F
+
     def _solve_direct(self, F, R, X, W, P):
+        if getattr(F, "inverted", False):
+            raise RuntimeError("BEMRet: the factors were replaced by inverses (invert_factors); use the inverse solve")
         n = F.n
         nn = n * n
         nat.call("mnpbem_bem_solve_batched", F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.piv[0]),

@@ -94,6 +94,9 @@ class BEMRetIter(BEMIter):
         self._eta = options.pop("eta", 2.5)
         self._hlu_mode = options.pop("preconditioner", "auto")
         self._htol_precond = options.pop("htol_precond", 1e-4)
+        # block preconditioner: apply explicit inverses of G1, G2 and Sigma
+        # (matrix-vector products) instead of their triangular factors
+        self._precond_inverse = bool(options.pop("precond_inverse", True))
         for unsupported in ("schur", "refun"):
             if options.pop(unsupported, None):
                 raise NotImplementedError("BEMRetIter: option '%s' is not provided" % unsupported)
@@ -168,6 +171,12 @@ class BEMRetIter(BEMIter):
         S.Fk = None
         if self._blocks is not None:
             S.Fk = [bem.factor(enei_list) for bem in self._block_bems]
+            if self._precond_inverse:
+                # GMRES applies the block factorisations once per iteration to
+                # a few right-hand sides: explicit inverses turn the
+                # latency-bound triangular steps into matrix-vector products
+                for F, bem in zip(S.Fk, self._block_bems):
+                    bem.invert_factors(F)
         if self._use_precond and self._blocks is None:
             S.F = self._direct.factor(enei_list)
             par = S.F.par
@@ -245,6 +254,12 @@ class BEMRetIter(BEMIter):
             for b in range(B):
                 o = (b * n + offs[0]) * 8 * P
                 fo = lambda t, stride: nat.ptr_off(t, b * stride)
+                if getattr(F, "inverted", False):
+                    nat.call("mnpbem_bem_solve_inv_batched", F.dtype, F.n, P, fo(F.GG[0], nn), fo(F.GG[1], nn),
+                             fo(F.HH[0], nn), fo(F.Et, nn), fo(F.HH[1], nn), F.n, 0, nat.ptr_off(R, o),
+                             nat.ptr_off(W, o), nat.ptr_off(X, o), 8 * P, step * 8 * P, nat.ptr(bem.g.nvec),
+                             nat.ptr_off(par, 5 * cnt * b), 0, cnt, st)
+                    continue
                 nat.call("mnpbem_bem_solve_batched", F.dtype, F.n, P, fo(F.GG[0], nn), fo(F.piv[0], F.n),
                          fo(F.GG[1], nn), fo(F.piv[1], F.n), fo(F.HH[0], nn), fo(F.Et, nn), fo(F.HH[1], nn),
                          fo(F.piv[3], F.n), F.n, 0, 0, fo(F.DG[0], F.sD), fo(F.DG[1], F.sD), fo(F.DS, F.sD), 0,

@@ -798,6 +798,18 @@ __global__ void k_step_pack(int n, int P, const T* R, const T* W, T* X, long ld,
   }
 }
 
+// Dst[b][i][0..C) = Src[b][i][0..C) for n rows (row strides ld, system
+// stride stride): the columns a solve with an explicit inverse wrote to its
+// work space, back into the vector it solves in place
+template <typename T>
+__global__ void k_copy_cols(int n, int C, const T* __restrict__ Src, T* __restrict__ Dst, long ld, long stride) {
+  const long b = blockIdx.y;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * C) return;
+  const long i = e / C, c = e % C;
+  cst(Dst + 2 * (b * stride + i * ld + c), cld(Src + 2 * (b * stride + i * ld + c)));
+}
+
 // residual of the 8N operator (GdA&H Eqs. 10, 11, 14, 17; bem_ret_iter.py:1362-1501):
 // Y = [G1 X1 | H1 X1 | G2 X2 | H2 X2], each (N, 4P), in one (N, 16P) buffer.
 //   phi = G1 s1 - G2 s2;  a = G1 h1 - G2 h2
@@ -1384,6 +1396,68 @@ extern "C" int mnpbem_bem_solve_batched(int dtype, int n, int P, const void* G1t
   if (mnp_is_c64(dtype)) MNP_SOLVE(float)
 #undef MNP_SOLVE
   mnp_set_error("bem_solve: unknown dtype");
+  return 1;
+}
+
+// The BEM solve of mnpbem_bem_solve_batched with explicit inverses in place
+// of the LU factors of G1, G2 and Sigma (G1i, G2i, Si: the transposed
+// buffers of G1^-1, G2^-1, Sigma^-1, as getri leaves them for the factors of
+// the transposed buffers): every solve is a matrix-vector product at full
+// bandwidth instead of a chain of triangular steps.  For callers that apply
+// one factorisation many times with a few right-hand sides (a preconditioner
+// inside GMRES), where the triangular steps of a few systems are
+// latency-bound.
+extern "C" int mnpbem_bem_solve_inv_batched(int dtype, int n, int P, const void* G1i, const void* G2i,
+                                            const void* S1t, const void* Et, const void* Si, long ldm, long smat,
+                                            const void* R, void* W, void* X, long ldv, long sv, const double* nvec,
+                                            const double* par, int gdiv, int batch, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldv >= 8 * P, "bem_solve_inv: bad size (P <= 4)");
+  if (batch <= 0) return 0;
+  dim3 eg(mnp_ceil_div(n, 256), batch);
+#define MNP_SOLVE_INV(T)                                                                                           \
+  {                                                                                                                \
+    const T* r = (const T*)R;                                                                                      \
+    T* w = (T*)W;                                                                                                  \
+    T* x = (T*)X;                                                                                                  \
+    int rc;                                                                                                        \
+    if ((rc = gemv_t<T>(n, 4 * P, (const T*)S1t, ldm, smat, r, ldv, sv, w, ldv, sv, 1.0, 0.0, 0.0, batch, st)))   \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_step_mod<T>, eg, dim3(256), 0, st, n, P, r, w, ldv, sv, nvec, par, gdiv);                 \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    if ((rc = gemv_t<T>(n, 3 * P, (const T*)Et, ldm, smat, w + 2 * 5 * P, ldv, sv, w + 2 * P, ldv, sv, 1.0, 0.0, \
+                        0.0, batch, st)))                                                                          \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_step_rhs<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par, gdiv);                    \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    /* Sigma^-1 on columns 0..P-1, through columns P..2P-1 (free here) */                                         \
+    if ((rc = gemv_t<T>(n, P, (const T*)Si, ldm, smat, w, ldv, sv, w + 2 * P, ldv, sv, 1.0, 0.0, 0.0, batch, st))) \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_copy_cols<T>, dim3(mnp_ceil_div((long)n * P, 256), batch), dim3(256), 0, st, n, P,      \
+                       (const T*)(w + 2 * P), w, ldv, sv);                                                         \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    hipLaunchKernelGGL(k_step_h2pre<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par, gdiv);                  \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    if ((rc = gemv_t<T>(n, 3 * P, (const T*)Et, ldm, smat, w + 2 * P, ldv, sv, w + 2 * 4 * P, ldv, sv, 1.0, 0.0, \
+                        0.0, batch, st)))                                                                          \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_step_pack<T>, eg, dim3(256), 0, st, n, P, r, w, x, ldv, sv);                              \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    /* G1^-1, G2^-1 on the two halves of x, into w, then back into x */                                          \
+    if ((rc = gemv_t<T>(n, 4 * P, (const T*)G1i, ldm, smat, x, ldv, sv, w, ldv, sv, 1.0, 0.0, 0.0, batch, st)))   \
+      return rc;                                                                                                   \
+    if ((rc = gemv_t<T>(n, 4 * P, (const T*)G2i, ldm, smat, x + 2 * 4 * P, ldv, sv, w + 2 * 4 * P, ldv, sv, 1.0,  \
+                        0.0, 0.0, batch, st)))                                                                     \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_copy_cols<T>, dim3(mnp_ceil_div((long)n * 8 * P, 256), batch), dim3(256), 0, st, n,     \
+                       8 * P, (const T*)w, x, ldv, sv);                                                            \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    return 0;                                                                                                      \
+  }
+  if (dtype == MNP_DTYPE_C128) MNP_SOLVE_INV(double)
+  if (mnp_is_c64(dtype)) MNP_SOLVE_INV(float)
+#undef MNP_SOLVE_INV
+  mnp_set_error("bem_solve_inv: unknown dtype");
   return 1;
 }
 

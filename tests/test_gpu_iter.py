@@ -336,3 +336,56 @@ def test_aca_generated_in_place_matches_assembled(sphere284, dimer):
             assert np.linalg.norm(hm.full(np) - Mh) / np.linalg.norm(Mh) < 10 * htol, (side, key)
     # inside and outside share no material here: no particle connects them
     assert aca.eval(0, 1, "G", lam) == 0
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-9), ("mixed", 1e-3)])
+def test_bem_solve_with_inverses_matches_factors(cuda, precision, tol):
+    """mnpbem_bem_solve_inv_batched (explicit inverses of G1, G2, Sigma from
+    BEMRet.invert_factors) against mnpbem_bem_solve_batched on the factors of
+    the same factorisation: the same direct solve, to the inverse's accuracy
+    (complex128 here to ~1e-12, the bf16x3 inverses to ~1e-5)."""
+    import mnpbem_amd as m
+    from mnpbem_amd import _native as nat
+    print("bem solve with inverses", precision, flush=True)
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [m.trisphere(144, 20.0)], [[2, 1]], 1)
+    lams = np.array([520.0, 600.0])
+    exc = m.PlaneWaveRet(*POL2)
+    bem = m.BEMRet(cp, precision=precision)
+    F = bem.factor(lams)
+    R = exc.rhs(cp, lams, bem.g)
+    P = R.shape[2] // 8
+    cdt = torch.complex128 if precision == "double" else torch.complex64
+    Rc = R.to(cdt)
+    X1, W1 = torch.empty_like(Rc), torch.empty_like(Rc)
+    bem._solve_direct(F, Rc, X1, W1, P)
+    bem.invert_factors(F)
+    with pytest.raises(RuntimeError, match="inverse"):
+        bem._solve_direct(F, Rc, X1.clone(), W1, P)
+    X2, W2 = torch.empty_like(Rc), torch.empty_like(Rc)
+    n, nn = F.n, F.n * F.n
+    nat.call("mnpbem_bem_solve_inv_batched", F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.GG[1]), nat.ptr(F.HH[0]),
+             nat.ptr(F.Et), nat.ptr(F.HH[1]), n, nn, nat.ptr(Rc), nat.ptr(W2), nat.ptr(X2), 8 * P, n * 8 * P,
+             nat.ptr(bem.g.nvec), nat.ptr(F.par), 0, F.B, nat.stream_handle(cuda))
+    a, b = X1.cpu().numpy(), X2.cpu().numpy()
+    assert np.abs(a - b).max() / np.abs(a).max() < tol
+
+
+def test_iter_block_preconditioner_inverse_vs_factors(cuda):
+    """The block preconditioner with explicit inverses (built in) and with the
+    triangular factors (precond_inverse=False): the same GMRES solution of a
+    dimer to the solver tolerance, within one iteration of each other."""
+    import mnpbem_amd as m
+    print("block preconditioner inverse vs factors", flush=True)
+    p1 = m.trisphere(144, 10.0)
+    p2 = p1.shift([11.0, 0.0, 0.0])
+    cp = m.ComParticle([m.EpsConst(1.0), m.EpsDrude(*DRUDE)], [p1, p2], [[2, 1], [2, 1]], 1, 2)
+    lams = np.array([520.0, 600.0])
+    exc = m.PlaneWaveRet(*POL2)
+    out = {}
+    for inv in (True, False):
+        bi = m.BEMRetIter(cp, precision="mixed", preconditioner="block", tol=1e-10, maxit=60, restart=60,
+                          precond_inverse=inv)
+        out[inv] = (bi.solve_sweep(lams, exc, chunk=2), bi.info())
+    assert np.max(np.abs(out[True][0] / out[False][0] - 1)) < 1e-8
+    its = [max(int(i[1]) for i in out[k][1][2]) for k in (True, False)]
+    assert abs(its[0] - its[1]) <= 1, its
