@@ -180,6 +180,22 @@ int mnpbem_trsm_left_rhs(int rhs);
  * A tuning knob of this library (no reference counterpart). */
 int mnpbem_gemm_thin_max(int n);
 
+/* 1 (built-in): complex64 matrix-vector products of at most 8 columns
+ * (mnpbem_gemv_t_batched and the solves built on it, n even, 16-byte aligned
+ * buffer) run on the FP32 matrix cores; 0: on the vector ALUs (two output
+ * rows per lane).  Same FP32 arithmetic in another summation order.  Returns
+ * the previous setting.  A tuning knob of this library (no reference
+ * counterpart). */
+int mnpbem_gemv_mfma(int on);
+
+/* Tile updates of the windowed transposed triangular solves (complex64,
+ * many systems, n even, 16-byte aligned factors) on the FP32 matrix cores:
+ * bit 0 for 5-8 right-hand-side columns, bit 1 for at most 4; mask < 0
+ * restores the built-in 3 (both).  Same FP32 arithmetic in another summation
+ * order.  Returns the previous mask.  A tuning knob of this library (no
+ * reference counterpart). */
+int mnpbem_trsv_mfma(int mask);
+
 /* 1 (built-in): in those block-row solves every block row after the first is
  * one launch, C <- D (C - A B), the inverse-block leaf folded into the update
  * product's epilogue (bit-identical to the two-launch form); 0: update and

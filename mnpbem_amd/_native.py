@@ -68,6 +68,8 @@ SIGNATURES = {
     "mnpbem_trsm_rows_max": [_i],
     "mnpbem_trsm_left_rhs": [_i],
     "mnpbem_gemm_thin_max": [_i],
+    "mnpbem_gemv_mfma": [_i],
+    "mnpbem_trsv_mfma": [_i],
     "mnpbem_trsm_leaf_fused": [_i],
     "mnpbem_getri_batched": [_i, _i, _p, _l, _l, _p, _l, _p, _l, _l, _p, _l, _i, _p],
     "mnpbem_getrs_blockinv_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _p, _l, _i, _p, _l, _l, _i, _p],

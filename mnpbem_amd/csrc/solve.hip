@@ -182,6 +182,105 @@ __global__ __launch_bounds__(256) void k_gemv_t2(int n, int C, const float* __re
       }
 }
 
+// complex64 form of k_gemv_t for C <= 8 columns on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32): the VALU form spends four FMAs per complex entry
+// and column, about as many cycles as HBM needs to deliver the entry, so the
+// two did not overlap fully.  Real form of the complex product with K over
+// the buffer rows j: for a pair of output rows (2p, 2p + 1) a lane supplies
+// A[p][q] = re / im of M(j + q, row) (one 16-byte load: both rows, both
+// parts), and B[q][c] = [Xr | Xi] (columns 0-7 | 8-15) against the real
+// parts and [-Xi | Xr] against the imaginary parts, so that D[p][c] holds
+// Re(M X) for c < 8 and Im(M X) for c >= 8.  One accumulator per row parity.
+// A workgroup of 4 waves owns 32 output rows; the waves split the buffer
+// rows in quarters (multiples of 32) and their partial sums are added in
+// wave order.  FP32 arithmetic like the VALU form, in another summation
+// order.  Needs n, ldm and sM even and M 16-byte aligned.
+typedef float gv_f32x4 __attribute__((ext_vector_type(4)));
+int g_gemv_mfma = 1;  // complex64 GEMVs of <= 8 columns on the matrix cores
+int g_trsv_mfma = 3;  // transposed-solve tile updates on the matrix cores: bit 0 for 5-8 columns, bit 1 for <= 4
+
+__global__ __launch_bounds__(256) void k_gemv_t_mfma(int n, int C, const float* __restrict__ M, long ldm, long sM,
+                                                     const float* __restrict__ X, long ldx, long sX,
+                                                     float* __restrict__ Y, long ldy, long sY, double alr, double ali,
+                                                     double beta) {
+  constexpr int UNR = 8;  // 4-row groups of the buffer in flight per wave
+  const long b = blockIdx.y;
+  M += 2 * b * sM;
+  X += 2 * b * sX;
+  Y += 2 * b * sY;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int p = lane & 15, q = lane >> 4;
+  const int r0 = blockIdx.x * 32;
+  const int ir = min(r0 + 2 * p, n - 2);  // rows past n read row n - 2 (discarded)
+  const int qs = ((n + 3) / 4 + 4 * UNR - 1) / (4 * UNR) * (4 * UNR);
+  const int jb = wv * qs, je = min(n, jb + qs);
+  const int c = p & 7;
+  const bool cim = p >= 8, cok = c < C;
+  gv_f32x4 de = {0.f, 0.f, 0.f, 0.f}, dodd = {0.f, 0.f, 0.f, 0.f};
+  int j = jb;
+  for (; j + 4 * UNR <= je; j += 4 * UNR) {
+    float4 m[UNR];
+    float2 x[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) m[u] = *reinterpret_cast<const float4*>(M + 2 * ((long)(j + 4 * u + q) * ldm + ir));
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      x[u] = cok ? *reinterpret_cast<const float2*>(X + 2 * ((long)(j + 4 * u + q) * ldx + c)) : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const float b1 = cim ? x[u].y : x[u].x, b2 = cim ? x[u].x : -x[u].y;
+      de = __builtin_amdgcn_mfma_f32_16x16x4f32(m[u].x, b1, de, 0, 0, 0);
+      dodd = __builtin_amdgcn_mfma_f32_16x16x4f32(m[u].z, b1, dodd, 0, 0, 0);
+      de = __builtin_amdgcn_mfma_f32_16x16x4f32(m[u].y, b2, de, 0, 0, 0);
+      dodd = __builtin_amdgcn_mfma_f32_16x16x4f32(m[u].w, b2, dodd, 0, 0, 0);
+    }
+  }
+  for (; j < je; j += 4) {
+    const int jj = j + q;
+    float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 x = make_float2(0.f, 0.f);
+    if (jj < je) {
+      m = *reinterpret_cast<const float4*>(M + 2 * ((long)jj * ldm + ir));
+      if (cok) x = *reinterpret_cast<const float2*>(X + 2 * ((long)jj * ldx + c));
+    }
+    const float b1 = cim ? x.y : x.x, b2 = cim ? x.x : -x.y;
+    de = __builtin_amdgcn_mfma_f32_16x16x4f32(m.x, b1, de, 0, 0, 0);
+    dodd = __builtin_amdgcn_mfma_f32_16x16x4f32(m.z, b1, dodd, 0, 0, 0);
+    de = __builtin_amdgcn_mfma_f32_16x16x4f32(m.y, b2, de, 0, 0, 0);
+    dodd = __builtin_amdgcn_mfma_f32_16x16x4f32(m.w, b2, dodd, 0, 0, 0);
+  }
+  // lane holds D[4 q + v][p] for v = 0..3; parts [wave][parity][lane][v]
+  __shared__ float part[4][2][64][4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    part[wv][0][lane][v] = de[v];
+    part[wv][1][lane][v] = dodd[v];
+  }
+  __syncthreads();
+  // one thread per (output row rho of the 32, column cc of the 8)
+  const int rho = tid >> 3, cc = tid & 7;
+  const int row = r0 + rho;
+  if (row >= n || cc >= C) return;
+  const int par = rho & 1, i = rho >> 1;          // D row i of the parity's accumulator
+  const int lr = (i >> 2) * 16 + cc, li = lr + 8;  // lanes holding columns cc (re) and cc + 8 (im)
+  const int v = i & 3;
+  float sr = 0.f, si = 0.f;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    sr += part[w][par][lr][v];
+    si += part[w][par][li][v];
+  }
+  float* py = Y + 2 * ((long)row * ldy + cc);
+  double yr = alr * (double)sr - ali * (double)si, yi = alr * (double)si + ali * (double)sr;
+  if (beta != 0.0) {
+    const cplx<float> y0 = cld(py);
+    yr += beta * (double)y0.re;
+    yi += beta * (double)y0.im;
+  }
+  cst(py, cmake<float>((float)yr, (float)yi));
+}
+
 // The four products of the complex128 residual, Y = [G1^T.. buffers applied:
 // G1 X1 | H1 X1 | G2 X2 | H2 X2], in one launch (blockIdx.z = product), with
 // accumulators for exactly the CC = 4P columns.  One product per launch with
@@ -403,7 +502,7 @@ __global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const
 // CM >= C: right-hand-side columns held per row.  Deterministic: the window
 // solve runs the same operations in the same order whichever workgroup runs it.
 constexpr int TRSV_WIN = 256;
-template <typename T, bool FWD, int CM, int TILE>
+template <typename T, bool FWD, int CM, int TILE, bool MF = false>
 __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1, int have, int ahead,
                                                     const T* __restrict__ LU, long ld, long sLU, T* V, long ldv,
                                                     long sV, T* Z, long ldz, long sZ, const T* __restrict__ Dinv,
@@ -491,6 +590,104 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
   // tile update with the whole window
   T* upd = FWD ? V : Z;
   const long ldu = FWD ? ldv : ldz;
+  if constexpr (MF) {
+    // complex64 tile update on the FP32 matrix cores (the real form of
+    // k_gemv_t_mfma): wave wv owns the tile rows [64 wv, 64 wv + 64) as two
+    // 32-row groups, lane (p, q) supplies rows 2p, 2p + 1 of each group (one
+    // 16-byte load per group) and window row l + q; D[g][parity][i][c] holds
+    // Re (c < 8) or Im (c >= 8) of column c & 7 for tile row 32 g + 2 i + parity
+    static_assert(TILE == 256 && sizeof(T) == 4, "matrix-core tile update: complex64, 256-row tiles");
+    const int p = lane & 15, q = lane >> 4, c = p & 7;
+    const bool cim = p >= 8, cok = c < C;
+    const int kt = FWD ? w1 + blockIdx.x * 256 : w0 - 256 * (blockIdx.x + 1);  // first row of the tile
+    const int kr0 = min(max(kt + 64 * wv + 2 * p, 0), n - 2), kr1 = min(max(kt + 64 * wv + 32 + 2 * p, 0), n - 2);
+    gv_f32x4 d00 = {0.f, 0.f, 0.f, 0.f}, d01 = d00, d10 = d00, d11 = d00;
+    auto step = [&](const float4& m0, const float4& m1, float xr, float xi) {
+      const float b1 = cim ? xi : xr, b2 = cim ? xr : -xi;
+      d00 = __builtin_amdgcn_mfma_f32_16x16x4f32(m0.x, b1, d00, 0, 0, 0);
+      d01 = __builtin_amdgcn_mfma_f32_16x16x4f32(m0.z, b1, d01, 0, 0, 0);
+      d10 = __builtin_amdgcn_mfma_f32_16x16x4f32(m1.x, b1, d10, 0, 0, 0);
+      d11 = __builtin_amdgcn_mfma_f32_16x16x4f32(m1.z, b1, d11, 0, 0, 0);
+      d00 = __builtin_amdgcn_mfma_f32_16x16x4f32(m0.y, b2, d00, 0, 0, 0);
+      d01 = __builtin_amdgcn_mfma_f32_16x16x4f32(m0.w, b2, d01, 0, 0, 0);
+      d10 = __builtin_amdgcn_mfma_f32_16x16x4f32(m1.y, b2, d10, 0, 0, 0);
+      d11 = __builtin_amdgcn_mfma_f32_16x16x4f32(m1.w, b2, d11, 0, 0, 0);
+    };
+    const float* LUf = reinterpret_cast<const float*>(LU);
+    int l = 0;
+    for (; l + 16 <= W; l += 16) {
+      float4 m0[4], m1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const long row = (long)(w0 + l + 4 * u + q) * ld;
+        m0[u] = *reinterpret_cast<const float4*>(LUf + 2 * (row + kr0));
+        m1[u] = *reinterpret_cast<const float4*>(LUf + 2 * (row + kr1));
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const cplx<T> x = cok ? rs[l + 4 * u + q][c] : cmake<T>(0, 0);
+        step(m0[u], m1[u], x.re, x.im);
+      }
+    }
+    for (; l < W; l += 4) {
+      const int ll = l + q;
+      float4 m0 = make_float4(0.f, 0.f, 0.f, 0.f), m1 = m0;
+      cplx<T> x = cmake<T>(0, 0);
+      if (ll < W) {
+        const long row = (long)(w0 + ll) * ld;
+        m0 = *reinterpret_cast<const float4*>(LUf + 2 * (row + kr0));
+        m1 = *reinterpret_cast<const float4*>(LUf + 2 * (row + kr1));
+        if (cok) x = rs[ll][c];
+      }
+      step(m0, m1, x.re, x.im);
+    }
+    // new right-hand side of this lane's (row, column, part) entries
+    float nvs[2][2][4];
+    float* updf = reinterpret_cast<float*>(upd);
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const gv_f32x4& dd = g ? (par ? d11 : d10) : (par ? d01 : d00);
+          const int k = kt + 64 * wv + 32 * g + 8 * q + 2 * v + par;
+          const bool lv = FWD ? k < n : k >= 0;
+          float nv = 0.f;
+          if (lv && cok) {
+            float* pu = updf + 2 * ((long)k * ldu + c) + (cim ? 1 : 0);
+            nv = *pu - dd[v];
+            *pu = nv;
+          }
+          nvs[g][par][v] = nv;
+        }
+    if (ahead && blockIdx.x == 0) {
+      const int s0 = FWD ? w1 : max(0, w0 - 256), s1 = FWD ? min(n, w1 + 256) : w0;
+      if (s1 > s0) {
+        __syncthreads();  // every wave done with the current window in rs
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+          for (int par = 0; par < 2; ++par)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int k = kt + 64 * wv + 32 * g + 8 * q + 2 * v + par;
+              const bool lv = FWD ? k < n : k >= 0;
+              const int r = k - s0;
+              if (lv && r >= 0 && r < TRSV_WIN && c < CM) {
+                T* e = reinterpret_cast<T*>(&rs[r][c]);
+                e[cim ? 1 : 0] = nvs[g][par][v];
+              }
+            }
+        solve_window(s0, s1);
+        for (int e = tid; e < (s1 - s0) * C; e += 256) {
+          const int rr = e / C, cc = e % C;
+          cst(dst + 2 * ((long)(s0 + rr) * ldd_ + cc), rs[rr][cc]);
+        }
+      }
+    }
+    return;
+  }
   cplx<T> acc[CM];
 #pragma unroll
   for (int c = 0; c < CM; ++c) acc[c] = cmake<T>(0, 0);
@@ -1140,6 +1337,13 @@ int gemv_t(int n, int C, const T* M, long ldm, long sM, const T* X, long ldx, lo
     const int cc = std::min(MAXC, C - c0);
     if constexpr (sizeof(T) == 4) {
       if (n % 2 == 0 && ldm % 2 == 0 && sM % 2 == 0 && ((uintptr_t)M & 15) == 0) {
+        if (cc > 4 && cc <= 8 && g_gemv_mfma) {  // at <= 4 columns the VALU form keeps up with HBM
+          hipLaunchKernelGGL(k_gemv_t_mfma, dim3(mnp_ceil_div(n, 32), batch), dim3(256), 0, st, n, cc,
+                             (const float*)M, ldm, sM, (const float*)(X + 2 * c0), ldx, sX, (float*)(Y + 2 * c0), ldy,
+                             sY, alr, ali, beta);
+          MNP_LAUNCH_CHECK();
+          continue;
+        }
         const dim3 grid2(mnp_ceil_div(n, 128), batch);
 #define MNP_GEMV2(CM)                                                                                             \
   hipLaunchKernelGGL((k_gemv_t2<CM>), grid2, dim3(256), 0, st, n, cc, (const float*)M, ldm, sM,                   \
@@ -1182,12 +1386,20 @@ int trsv_t_windows_cm(int n, int C, const T* LU, long ld, long sLU, T* V, long l
   // solution, written by workgroup 0 of the launch before
   const int la = TILE == 256 ? 1 : 0;
   bool first = true;
+  // complex64 tile updates on the matrix cores: row pairs by 16-byte loads
+  constexpr bool can_mf = TILE == 256 && sizeof(T) == 4;
+  const bool mf = can_mf && (g_trsv_mfma & (CM > 4 ? 1 : 2)) && n % 2 == 0 && ld % 2 == 0 && sLU % 2 == 0 &&
+                  ((uintptr_t)LU & 15) == 0;
   for (int b0 = 0; b0 < nblk; b0 += R) {
     const int w0 = 64 * b0, w1 = std::min(n, 64 * (b0 + R));
     if (!first && la && w1 >= n) break;  // solved by the previous launch, nothing left to update
     dim3 grid(std::max(1, mnp_ceil_div(n - w1, TILE)), batch);
-    hipLaunchKernelGGL((k_trsv_t_win<T, true, CM, TILE>), grid, dim3(256), 0, st, n, C, w0, w1, first ? 0 : la, la,
-                       LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
+    if (mf)
+      hipLaunchKernelGGL((k_trsv_t_win<T, true, CM, TILE, can_mf>), grid, dim3(256), 0, st, n, C, w0, w1,
+                         first ? 0 : la, la, LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
+    else
+      hipLaunchKernelGGL((k_trsv_t_win<T, true, CM, TILE>), grid, dim3(256), 0, st, n, C, w0, w1, first ? 0 : la,
+                         la, LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
     MNP_LAUNCH_CHECK();
     first = false;
   }
@@ -1197,8 +1409,12 @@ int trsv_t_windows_cm(int n, int C, const T* LU, long ld, long sLU, T* V, long l
     const int w0 = 64 * b0, w1 = std::min(n, 64 * b1);
     if (!first && la && w0 <= 0) break;
     dim3 grid(std::max(1, mnp_ceil_div(w0, TILE)), batch);
-    hipLaunchKernelGGL((k_trsv_t_win<T, false, CM, TILE>), grid, dim3(256), 0, st, n, C, w0, w1, first ? 0 : la, la,
-                       LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
+    if (mf)
+      hipLaunchKernelGGL((k_trsv_t_win<T, false, CM, TILE, can_mf>), grid, dim3(256), 0, st, n, C, w0, w1,
+                         first ? 0 : la, la, LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
+    else
+      hipLaunchKernelGGL((k_trsv_t_win<T, false, CM, TILE>), grid, dim3(256), 0, st, n, C, w0, w1, first ? 0 : la,
+                         la, LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
     MNP_LAUNCH_CHECK();
     first = false;
   }
@@ -1260,6 +1476,18 @@ int lu_solve_t(int n, int C, const T* LU, long ld, long sLU, const int* ipiv, lo
 }
 
 }  // namespace
+
+extern "C" int mnpbem_gemv_mfma(int on) {
+  const int prev = g_gemv_mfma;
+  g_gemv_mfma = on != 0;
+  return prev;
+}
+
+extern "C" int mnpbem_trsv_mfma(int mask) {
+  const int prev = g_trsv_mfma;
+  g_trsv_mfma = mask < 0 ? 3 : (mask & 3);
+  return prev;
+}
 
 extern "C" int mnpbem_gemv_t_batched(int dtype, int n, int C, const void* M, long ldm, long sM, const void* X,
                                      long ldx, long sX, void* Y, long ldy, long sY, double alr, double ali,

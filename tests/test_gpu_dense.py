@@ -292,13 +292,26 @@ def test_lu_solve_transposed(cuda, dtype, tol, n, C, blockinv):
     assert err < tol, err
 
 
-@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 1e-4)])
-@pytest.mark.parametrize("n,C,B", [(700, 8, 200), (1000, 2, 130), (513, 5, 260)])
-def test_lu_solve_transposed_many_systems(cuda, dtype, tol, n, C, B):
+@pytest.mark.parametrize("dtype,tol,mf", [(np.complex128, 1e-11, 3), (np.complex64, 1e-4, 3),
+                                          (np.complex64, 1e-4, 0)])
+@pytest.mark.parametrize("n,C,B", [(700, 8, 200), (1000, 2, 130), (513, 5, 260), (1030, 6, 128)])
+def test_lu_solve_transposed_many_systems(cuda, dtype, tol, mf, n, C, B):
     """Enough systems for the 256-row tiles of the windowed solve, where
     workgroup 0 of each launch solves the next window once and the next launch
-    reads that solution (look-ahead), against numpy's solve."""
+    reads that solution (look-ahead), against numpy's solve; complex64 tile
+    updates on the matrix cores (knob mnpbem_trsv_mfma 3, n even) and on the
+    vector ALUs (0)."""
     nat = _lib()
+    lib = nat.load()
+    prev = lib.mnpbem_trsv_mfma(mf)
+    print("lu_solve_t many: trsv_mfma", mf, flush=True)
+    try:
+        _solve_t_many(cuda, nat, dtype, tol, n, C, B)
+    finally:
+        lib.mnpbem_trsv_mfma(prev)
+
+
+def _solve_t_many(cuda, nat, dtype, tol, n, C, B):
     rng = np.random.default_rng(n + C + B)
     M = (_rand(rng, (B, n, n), dtype) + (2.0 * np.sqrt(n)) * np.eye(n)[None]).astype(dtype)
     V = _rand(rng, (B, n, C), dtype)
@@ -809,13 +822,16 @@ def test_factor_interchanges_fallback(cuda, monkeypatch):
     assert np.abs(out / ref - 1).max() < 1e-12
 
 
-@pytest.mark.parametrize("n,C", [(2884, 8), (1030, 6), (130, 3), (512, 16), (257, 8)])
-def test_gemv_t_paired_rows_bit_identical(cuda, n, C):
+@pytest.mark.parametrize("mfma", [0, 1])
+@pytest.mark.parametrize("n,C", [(2884, 8), (1030, 6), (130, 3), (512, 16), (257, 8), (66, 1), (98, 5)])
+def test_gemv_t_paired_rows_and_matrix_cores(cuda, n, C, mfma):
     """mnpbem_gemv_t_batched (Y = beta Y + alpha M^T X, M the row-major
-    buffer) in complex64 against numpy; with n even and a 16-byte aligned
-    buffer two output rows per lane (16-byte loads), otherwise one: the same
-    buffer at a misaligned address takes the one-row kernel and must give the
-    same bits."""
+    buffer) in complex64 against numpy.  With n even and a 16-byte aligned
+    buffer the product runs on the FP32 matrix cores for C <= 8 (knob
+    mnpbem_gemv_mfma 1) or with two output rows per lane (knob 0); otherwise
+    one row per lane.  The two-row kernel gives the same bits as the one-row
+    kernel (same buffer at a misaligned address); the matrix-core kernel sums
+    in another order and must agree with it to FP32 rounding."""
     nat = _lib()
     rng = np.random.default_rng(n + C)
     B = 3
@@ -826,19 +842,29 @@ def test_gemv_t_paired_rows_bit_identical(cuda, n, C):
     ref = beta * Y0.astype(np.complex128) + al * np.einsum("bji,bjc->bic", M.astype(np.complex128),
                                                             X.astype(np.complex128))
     st = nat.stream_handle(cuda)
-    print("gemv_t paired rows", n, C, flush=True)
+    print("gemv_t", n, C, "mfma", mfma, flush=True)
     outs = []
-    for off in (0, 1):  # complex elements before the buffer: 0 aligned, 1 misaligned (8 bytes)
-        buf = torch.zeros(B * n * n + 2, dtype=torch.complex64, device=cuda)
-        buf[off:off + B * n * n] = torch.from_numpy(M.reshape(-1)).to(cuda)
-        tX = torch.from_numpy(X.copy()).to(cuda)
-        tY = torch.from_numpy(Y0.copy()).to(cuda)
-        nat.call("mnpbem_gemv_t_batched", nat.DTYPE_C64, n, C, nat.ptr_off(buf, off), n, n * n, nat.ptr(tX), C,
-                 n * C, nat.ptr(tY), C, n * C, al.real, al.imag, beta, B, st)
-        outs.append(tY.cpu().numpy())
-    err = np.abs(outs[0] - ref).max() / np.abs(ref).max()
-    assert err < 1e-5, err
-    assert np.array_equal(outs[0], outs[1])
+    lib = nat.load()
+    prev = lib.mnpbem_gemv_mfma(mfma)
+    try:
+        for off in (0, 1):  # complex elements before the buffer: 0 aligned, 1 misaligned (8 bytes)
+            buf = torch.zeros(B * n * n + 2, dtype=torch.complex64, device=cuda)
+            buf[off:off + B * n * n] = torch.from_numpy(M.reshape(-1)).to(cuda)
+            tX = torch.from_numpy(X.copy()).to(cuda)
+            tY = torch.from_numpy(Y0.copy()).to(cuda)
+            nat.call("mnpbem_gemv_t_batched", nat.DTYPE_C64, n, C, nat.ptr_off(buf, off), n, n * n, nat.ptr(tX),
+                     C, n * C, nat.ptr(tY), C, n * C, al.real, al.imag, beta, B, st)
+            outs.append(tY.cpu().numpy())
+    finally:
+        lib.mnpbem_gemv_mfma(prev)
+    scale = np.abs(ref).max()
+    for o in outs:
+        err = np.abs(o - ref).max() / scale
+        assert err < 1e-5, err
+    if mfma and C <= 8 and n % 2 == 0:
+        assert np.abs(outs[0] - outs[1]).max() / scale < 1e-5
+    else:
+        assert np.array_equal(outs[0], outs[1])
 
 
 @pytest.mark.parametrize("dt_name,bs", [("c64x", 128), ("c64x", 64), ("c64", 0), ("c128", 0)])
