@@ -502,12 +502,32 @@ __global__ __launch_bounds__(256) void k_trsv_t_step(int n, int C, int i0, const
 // CM >= C: right-hand-side columns held per row.  Deterministic: the window
 // solve runs the same operations in the same order whichever workgroup runs it.
 constexpr int TRSV_WIN = 256;
+constexpr int WSU = 32;  // loads in flight per thread in the window solve (16: 2-6 % slower solves)
 template <typename T, bool FWD, int CM, int TILE, bool MF = false>
 __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1, int have, int ahead,
                                                     const T* __restrict__ LU, long ld, long sLU, T* V, long ldv,
                                                     long sV, T* Z, long ldz, long sZ, const T* __restrict__ Dinv,
                                                     long sD, int half, long hoff) {
-  const long b = blockIdx.y;
+  // 256-row tiles: one-dimensional grid, workgroup 0 of every system first
+  // (it solves the next window after its own tile, so starting it before the
+  // other tiles lets that serial solve overlap their updates), then tiles
+  // 1.. of system 0, of system 1, ...; 64-row tiles: (tile, system) grid
+  long b;
+  int bx;
+  if constexpr (TILE == 256) {
+    const int nt = max(1, FWD ? (n - w1 + 255) / 256 : (w0 + 255) / 256);
+    const int nb_sys = gridDim.x / nt, L = blockIdx.x;
+    if (L < nb_sys) {
+      b = L;
+      bx = 0;
+    } else {
+      b = (L - nb_sys) / (nt - 1);
+      bx = 1 + (L - nb_sys) % (nt - 1);
+    }
+  } else {
+    b = blockIdx.y;
+    bx = blockIdx.x;
+  }
   LU += 2 * b * sLU;
   Dinv += 2 * b * sD;
   V += 2 * ((b % half) * sV + (b / half) * hoff);
@@ -530,13 +550,13 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
       cplx<T> sl[CM / 4];
 #pragma unroll
       for (int v = 0; v < CM / 4; ++v) sl[v] = cmake<T>(0, 0);
-      for (int q0 = 0; q0 < nb; q0 += 16) {
-        cplx<T> d[16];
+      for (int q0 = 0; q0 < nb; q0 += WSU) {
+        cplx<T> d[WSU];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
+        for (int u = 0; u < WSU; ++u)
           d[u] = (q0 + u < nb && lane < nb) ? cld(D + 2 * ((long)(q0 + u) * 64 + lane)) : cmake<T>(0, 0);
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
+        for (int u = 0; u < WSU; ++u)
 #pragma unroll
           for (int v = 0; v < CM / 4; ++v) sl[v] = cfma(sl[v], d[u], rs[64 * r + q0 + u][wv + 4 * v]);
       }
@@ -552,13 +572,13 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
         cplx<T> acc[CM];
 #pragma unroll
         for (int c = 0; c < CM; ++c) acc[c] = cmake<T>(0, 0);
-        for (int l0 = 0; l0 < nb; l0 += 16) {
-          cplx<T> m[16];
+        for (int l0 = 0; l0 < nb; l0 += WSU) {
+          cplx<T> m[WSU];
 #pragma unroll
-          for (int u = 0; u < 16; ++u)
+          for (int u = 0; u < WSU; ++u)
             m[u] = l0 + u < nb ? cld(LU + 2 * ((long)(i0 + l0 + u) * ld + k)) : cmake<T>(0, 0);
 #pragma unroll
-          for (int u = 0; u < 16; ++u)
+          for (int u = 0; u < WSU; ++u)
 #pragma unroll
             for (int c = 0; c < CM; ++c) acc[c] = cfma(acc[c], m[u], rs[64 * r + l0 + u][c]);
         }
@@ -579,7 +599,7 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
   }
   if (!have) {
     solve_window(w0, w1);
-    if (blockIdx.x == 0)
+    if (bx == 0)
       for (int e = tid; e < W * C; e += 256) {
         const int rr = e / C, c = e % C;
         cst(dst + 2 * ((long)(w0 + rr) * ldd_ + c), rs[rr][c]);
@@ -599,7 +619,7 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
     static_assert(TILE == 256 && sizeof(T) == 4, "matrix-core tile update: complex64, 256-row tiles");
     const int p = lane & 15, q = lane >> 4, c = p & 7;
     const bool cim = p >= 8, cok = c < C;
-    const int kt = FWD ? w1 + blockIdx.x * 256 : w0 - 256 * (blockIdx.x + 1);  // first row of the tile
+    const int kt = FWD ? w1 + bx * 256 : w0 - 256 * (bx + 1);  // first row of the tile
     const int kr0 = min(max(kt + 64 * wv + 2 * p, 0), n - 2), kr1 = min(max(kt + 64 * wv + 32 + 2 * p, 0), n - 2);
     gv_f32x4 d00 = {0.f, 0.f, 0.f, 0.f}, d01 = d00, d10 = d00, d11 = d00;
     auto step = [&](const float4& m0, const float4& m1, float xr, float xi) {
@@ -661,7 +681,7 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
           }
           nvs[g][par][v] = nv;
         }
-    if (ahead && blockIdx.x == 0) {
+    if (ahead && bx == 0) {
       const int s0 = FWD ? w1 : max(0, w0 - 256), s1 = FWD ? min(n, w1 + 256) : w0;
       if (s1 > s0) {
         __syncthreads();  // every wave done with the current window in rs
@@ -694,10 +714,10 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
   int k;
   bool live;
   if constexpr (TILE == 256) {
-    k = FWD ? w1 + blockIdx.x * 256 + tid : w0 - 256 * (blockIdx.x + 1) + tid;
+    k = FWD ? w1 + bx * 256 + tid : w0 - 256 * (bx + 1) + tid;
     live = FWD ? k < n : k >= 0;
   } else {
-    k = (FWD ? w1 : 0) + blockIdx.x * 64 + lane;
+    k = (FWD ? w1 : 0) + bx * 64 + lane;
     live = k < (FWD ? n : w0);
   }
   const int la = TILE == 64 ? 64 * wv : 0, lb = TILE == 64 ? min(W, 64 * wv + 64) : W;
@@ -740,7 +760,7 @@ __global__ __launch_bounds__(256) void k_trsv_t_win(int n, int C, int w0, int w1
       }
   if constexpr (TILE == 256) {
     // workgroup 0 holds the next window's updated right-hand side: solve it
-    if (ahead && blockIdx.x == 0) {
+    if (ahead && bx == 0) {
       const int s0 = FWD ? w1 : max(0, w0 - 256), s1 = FWD ? min(n, w1 + 256) : w0;
       if (s1 > s0) {
         __syncthreads();  // every thread done with the current window in rs
@@ -1393,7 +1413,8 @@ int trsv_t_windows_cm(int n, int C, const T* LU, long ld, long sLU, T* V, long l
   for (int b0 = 0; b0 < nblk; b0 += R) {
     const int w0 = 64 * b0, w1 = std::min(n, 64 * (b0 + R));
     if (!first && la && w1 >= n) break;  // solved by the previous launch, nothing left to update
-    dim3 grid(std::max(1, mnp_ceil_div(n - w1, TILE)), batch);
+    const int ntf = std::max(1, mnp_ceil_div(n - w1, TILE));
+    const dim3 grid = TILE == 256 ? dim3(ntf * batch) : dim3(ntf, batch);
     if (mf)
       hipLaunchKernelGGL((k_trsv_t_win<T, true, CM, TILE, can_mf>), grid, dim3(256), 0, st, n, C, w0, w1,
                          first ? 0 : la, la, LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
@@ -1408,7 +1429,8 @@ int trsv_t_windows_cm(int n, int C, const T* LU, long ld, long sLU, T* V, long l
     const int b0 = std::max(0, b1 - R);
     const int w0 = 64 * b0, w1 = std::min(n, 64 * b1);
     if (!first && la && w0 <= 0) break;
-    dim3 grid(std::max(1, mnp_ceil_div(w0, TILE)), batch);
+    const int ntb = std::max(1, mnp_ceil_div(w0, TILE));
+    const dim3 grid = TILE == 256 ? dim3(ntb * batch) : dim3(ntb, batch);
     if (mf)
       hipLaunchKernelGGL((k_trsv_t_win<T, false, CM, TILE, can_mf>), grid, dim3(256), 0, st, n, C, w0, w1,
                          first ? 0 : la, la, LU, ld, sLU, V, ldv, sV, Z, ldz, sZ, Dinv, sD, half, hoff);
