@@ -338,6 +338,16 @@ int mnpbem_bem_solve_adjoint_batched(int dtype, int n, int P, const void* G1t, c
                                      const void* DS, long sD, const void* C, void* W, void* Y, long ldv, long sv,
                                      const double* nvec, const double* par, int batch, void* stream);
 
+/* The same adjoint solve with explicit inverses: G1i, G2i, Si are the
+ * buffers that mnpbem_getri_blockinv_batched leaves in place of the factors
+ * of G1t, G2t, St (S1t, Et as above); each pair of triangular solves becomes
+ * one matrix product.  C and Y must be different buffers.  No counterpart in
+ * the reference (see above). */
+int mnpbem_bem_solve_adjoint_inv_batched(int dtype, int n, int P, const void* G1i, const void* G2i,
+                                         const void* S1t, const void* Et, const void* Si, long ldm, long smat,
+                                         const void* C, void* W, void* Y, long ldv, long sv, const double* nvec,
+                                         const double* par, int batch, void* stream);
+
 /* Out = C - A^T Y (complex128; C NULL: Out = A^T Y) with unfactored
  * transposed G1, H1, G2, H2; Z (n x 8P) and T (n x 16P, ldt >= 16P) are work
  * space; nrm[b] = {||C - A^T Y||^2, ||C||^2} (may be NULL). */

@@ -54,6 +54,8 @@ SIGNATURES = {
                                     _p, _i, _i, _p],
     "mnpbem_bem_solve_adjoint_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _p, _p, _p, _l,
                                          _p, _p, _p, _l, _l, _p, _p, _i, _p],
+    "mnpbem_bem_solve_adjoint_inv_batched": [_i, _i, _i, _p, _p, _p, _p, _p, _l, _l, _p, _p, _p, _l, _l, _p, _p,
+                                             _i, _p],
     "mnpbem_bem_residual_adjoint_batched": [_i, _i, _p, _p, _p, _p, _l, _l, _p, _p, _p, _l, _l, _p, _p, _l, _l,
                                             _p, _p, _p, _i, _p],
     "mnpbem_gemv_n_batched": [_i, _i, _i, _p, _l, _l, _p, _l, _l, _p, _l, _l, _d, _i, _p],

@@ -38,10 +38,15 @@ def _inside_drude(p):
     return m_in, eps
 
 
-def extinction_with_drude_gradient(bem, enei_list, planewave, chunk=16):
+def extinction_with_drude_gradient(bem, enei_list, planewave, chunk=16, inverse=False):
     """Extinction (n_lambda, npol) and its derivatives (n_lambda, npol, 3)
     with respect to (eps0, wp, gammad) of the inside Drude material, for a
-    BEMRet solver ``bem`` (double or mixed precision)."""
+    BEMRet solver ``bem`` (double or mixed precision).  With ``inverse`` the
+    factors are replaced by explicit inverses (BEMRet.invert_factors) before
+    the two solves, which then run as matrix-vector products; off by default:
+    two forward solves (about six applications of the windowed triangular
+    solves) do not repay the three inversions (2880 faces, 21 wavelengths:
+    0.177 against 0.158 s)."""
     if not isinstance(bem, BEMRet):
         raise TypeError("extinction_with_drude_gradient: expects a BEMRet solver")
     p, g = bem.p, bem.g
@@ -55,6 +60,8 @@ def extinction_with_drude_gradient(bem, enei_list, planewave, chunk=16):
         lam = enei_list[c0:c0 + chunk]
         B = len(lam)
         F = bem.factor(lam)
+        if inverse:
+            bem.invert_factors(F)
         R = planewave.rhs(p, lam, g)
         X = bem.solve_batch(F, R)
         P = R.shape[2] // 8
@@ -201,7 +208,7 @@ def _chain_to_vertices(cp, Gr):
     return out
 
 
-def extinction_with_shape_gradient(bem, enei_list, planewave, chunk=16, per_face=False):
+def extinction_with_shape_gradient(bem, enei_list, planewave, chunk=16, per_face=False, inverse=True):
     """Extinction (n_lambda, npol) and its derivatives with respect to the
     mesh: one adjoint solve A^T y = c per wavelength with the factors of the
     forward solve (c the optical-theorem functional), then the HIP kernels
@@ -212,7 +219,12 @@ def extinction_with_shape_gradient(bem, enei_list, planewave, chunk=16, per_face
     list per particle of dicts: 'nodes' (n_lambda, npol, nverts2, 3) and, for
     particles built from corner triangles, 'verts' (n_lambda, npol, nverts, 3).
     With ``per_face`` the per-face gradients (n_lambda, npol, n, 31) are
-    returned as a third value."""
+    returned as a third value.  With ``inverse`` the factors are replaced by
+    explicit inverses (BEMRet.invert_factors) before the forward and adjoint
+    solves (each applies them about three times), which then run as
+    matrix-vector products instead of triangular solves: the adjoint's
+    row-major triangular steps (~10 ms per solve at 2880 faces and 21
+    wavelengths) cost more than the three inversions (~29 ms)."""
     if not isinstance(bem, BEMRet):
         raise TypeError("extinction_with_shape_gradient: expects a BEMRet solver")
     p, g = bem.p, bem.g
@@ -232,6 +244,8 @@ def extinction_with_shape_gradient(bem, enei_list, planewave, chunk=16, per_face
         lam = enei_list[c0:c0 + chunk]
         B = len(lam)
         F = bem.factor(lam)
+        if inverse:
+            bem.invert_factors(F)
         R = planewave.rhs(p, lam, g)
         X = bem.solve_batch(F, R)
         ext = planewave.extinction_batch(p, lam, X, g)

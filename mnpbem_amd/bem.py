@@ -256,10 +256,12 @@ class BEMRet:
         """Replace the LU factors of G1^T, G2^T and Sigma^T in ``F`` by the
         inverses (getri in place; the transposed buffers of G1^-1, G2^-1,
         Sigma^-1), for a factorisation that is applied many times to a few
-        right-hand sides (the block preconditioner inside GMRES): each solve
-        is then a matrix-vector product (``solve_inverse``) instead of chains
-        of latency-bound triangular steps.  Work space: one n x n matrix per
-        wavelength while it runs.  ``F`` no longer serves ``_solve_direct``."""
+        right-hand sides (the block preconditioner inside GMRES, the forward
+        and adjoint solves of the gradients): each solve is then a few
+        matrix-vector products instead of chains of latency-bound triangular
+        steps.  Work space: one n x n matrix per wavelength while it runs.
+        ``solve_batch`` and ``solve_adjoint_batch`` take the inverses from
+        here on."""
         if getattr(F, "inverted", False):
             return F
         n, B, dt = F.n, F.B, F.dtype
@@ -275,10 +277,14 @@ class BEMRet:
         return F
 
     def _solve_direct(self, F, R, X, W, P):
-        if getattr(F, "inverted", False):
-            raise RuntimeError("BEMRet: the factors were replaced by inverses (invert_factors); use the inverse solve")
         n = F.n
         nn = n * n
+        if getattr(F, "inverted", False):
+            nat.call("mnpbem_bem_solve_inv_batched", F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.GG[1]),
+                     nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]), n, nn, nat.ptr(R), nat.ptr(W), nat.ptr(X),
+                     8 * P, n * 8 * P, nat.ptr(self.g.nvec), nat.ptr(F.par), getattr(self.g, "gdiv", 0), F.B,
+                     nat.stream_handle(self.device))
+            return
         nat.call("mnpbem_bem_solve_batched", F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.piv[0]),
                  nat.ptr(F.GG[1]), nat.ptr(F.piv[1]), nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]),
                  nat.ptr(F.piv[3]), n, nn, n, nat.ptr(F.DG[0]), nat.ptr(F.DG[1]), nat.ptr(F.DS), F.sD,
@@ -369,6 +375,11 @@ class BEMRet:
         nn = n * n
 
         def direct(Cin, Yout, Wk):
+            if getattr(F, "inverted", False):
+                nat.call("mnpbem_bem_solve_adjoint_inv_batched", F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.GG[1]),
+                         nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]), n, nn, nat.ptr(Cin), nat.ptr(Wk),
+                         nat.ptr(Yout), 8 * P, n * 8 * P, nat.ptr(self.g.nvec), nat.ptr(F.par), B, st)
+                return
             nat.call("mnpbem_bem_solve_adjoint_batched", F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.piv[0]),
                      nat.ptr(F.GG[1]), nat.ptr(F.piv[1]), nat.ptr(F.HH[0]), nat.ptr(F.Et), nat.ptr(F.HH[1]),
                      nat.ptr(F.piv[3]), n, nn, n, nat.ptr(F.DG[0]), nat.ptr(F.DG[1]), nat.ptr(F.DS), F.sD,

@@ -2478,6 +2478,63 @@ extern "C" int mnpbem_bem_solve_adjoint_batched(int dtype, int n, int P, const v
   return 1;
 }
 
+// The adjoint solve with explicit inverses (the buffers that getri leaves in
+// place of the factors of mnpbem_bem_solve_adjoint_batched: G1i, G2i, Si;
+// S1t and Et as there): each triangular pair becomes one matrix product in
+// the same orientation.  C and Y must not overlap; W is work space of the
+// shape of C.
+extern "C" int mnpbem_bem_solve_adjoint_inv_batched(int dtype, int n, int P, const void* G1i, const void* G2i,
+                                                    const void* S1t, const void* Et, const void* Si, long ldm,
+                                                    long smat, const void* C, void* W, void* Y, long ldv, long sv,
+                                                    const double* nvec, const double* par, int batch,
+                                                    void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  MNP_REQUIRE(n > 0 && P > 0 && P * 4 <= MAXC && ldv >= 8 * P, "bem_solve_adjoint_inv: bad size (P <= 4)");
+  MNP_REQUIRE(C != Y, "bem_solve_adjoint_inv: C and Y must be different buffers");
+  if (batch <= 0) return 0;
+  dim3 eg(mnp_ceil_div(n, 256), batch);
+#define MNP_ADJ_INV(T)                                                                                             \
+  {                                                                                                                \
+    const T* c = (const T*)C;                                                                                      \
+    T* y = (T*)Y;                                                                                                  \
+    T* w = (T*)W;                                                                                                  \
+    int rc;                                                                                                        \
+    /* a1: Y = [G1^-T c1 | G2^-T c2] */                                                                            \
+    if ((rc = gemv_n<T>(n, 4 * P, (const T*)G1i, ldm, smat, c, ldv, sv, y, ldv, sv, 0.0, batch, st))) return rc;  \
+    if ((rc = gemv_n<T>(n, 4 * P, (const T*)G2i, ldm, smat, c + 2 * 4 * P, ldv, sv, y + 2 * 4 * P, ldv, sv, 0.0,   \
+                        batch, st)))                                                                               \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_adj_sum<T>, eg, dim3(256), 0, st, n, P, y, w, ldv, sv);                                   \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    if ((rc = gemv_n<T>(n, 3 * P, (const T*)Et, ldm, smat, w + 2 * P, ldv, sv, w + 2 * 5 * P, ldv, sv, 0.0,      \
+                        batch, st)))                                                                               \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_adj_rhs<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                           \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    /* a5: Sigma^-T rhs through W[:, 0:P] (free here), back into W[:, 4P:5P] */                                   \
+    if ((rc = gemv_n<T>(n, P, (const T*)Si, ldm, smat, w + 2 * 4 * P, ldv, sv, w, ldv, sv, 0.0, batch, st)))      \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_copy_cols<T>, dim3(mnp_ceil_div((long)n * P, 256), batch), dim3(256), 0, st, n, P,      \
+                       (const T*)w, w + 2 * 4 * P, ldv, sv);                                                       \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    hipLaunchKernelGGL(k_adj_t<T>, eg, dim3(256), 0, st, n, P, w, ldv, sv, nvec, par);                             \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    if ((rc = gemv_n<T>(n, 3 * P, (const T*)Et, ldm, smat, w + 2 * P, ldv, sv, w + 2 * 5 * P, ldv, sv, 0.0,      \
+                        batch, st)))                                                                               \
+      return rc;                                                                                                   \
+    if ((rc = gemv_n<T>(n, 4 * P, (const T*)S1t, ldm, smat, w + 2 * 4 * P, ldv, sv, w, ldv, sv, 0.0, batch, st))) \
+      return rc;                                                                                                   \
+    hipLaunchKernelGGL(k_adj_final<T>, eg, dim3(256), 0, st, n, P, y, w, ldv, sv, nvec, par);                      \
+    MNP_LAUNCH_CHECK();                                                                                            \
+    return 0;                                                                                                      \
+  }
+  if (dtype == MNP_DTYPE_C128) MNP_ADJ_INV(double)
+  if (mnp_is_c64(dtype)) MNP_ADJ_INV(float)
+#undef MNP_ADJ_INV
+  mnp_set_error("bem_solve_adjoint_inv: unknown dtype");
+  return 1;
+}
+
 // Out = C - A^T Y in complex128 (nrm: ||C - A^T Y||^2 and ||C||^2 per
 // wavelength, may be NULL; with C NULL, Out = A^T Y).  G1t.. are the
 // complex128 transposed Green matrices; Z (batch, n, 8P) and T (batch, n,

@@ -33,15 +33,22 @@ def _particles(m):
     yield "rod", m.trirod(10.0, 24.0, [10, 5, 6], triangles=True)
 
 
+@pytest.mark.parametrize("inverse", [False, True])
 @pytest.mark.parametrize("precision", ["double", "mixed"])
-def test_adjoint_solve(cuda, precision):
+def test_adjoint_solve(cuda, precision, inverse):
+    """The forward and adjoint solves on the LU factors and, with
+    ``inverse``, on the explicit inverses that BEMRet.invert_factors leaves in
+    their place (mnpbem_bem_solve_inv_batched,
+    mnpbem_bem_solve_adjoint_inv_batched): same checks, same tolerances."""
     import mnpbem_amd as m
     lams = np.array([480.0, 560.0, 700.0])
     for name, part in _particles(m):
-        print("adjoint solve", name, precision, part.n, flush=True)
+        print("adjoint solve", name, precision, "inverse" if inverse else "factors", part.n, flush=True)
         cp = m.ComParticle([m.EpsConst(1.33 ** 2), m.EpsDrude(*DRUDE)], [part], [[2, 1]], 1)
         bem = m.BEMRet(cp, precision=precision)
         F = bem.factor(lams)
+        if inverse:
+            bem.invert_factors(F)
         n, B = cp.n, len(lams)
         for P in (1, 2):
             X = _rand((B, n, 8 * P), 1, cuda)

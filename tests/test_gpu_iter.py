@@ -359,8 +359,8 @@ def test_bem_solve_with_inverses_matches_factors(cuda, precision, tol):
     X1, W1 = torch.empty_like(Rc), torch.empty_like(Rc)
     bem._solve_direct(F, Rc, X1, W1, P)
     bem.invert_factors(F)
-    with pytest.raises(RuntimeError, match="inverse"):
-        bem._solve_direct(F, Rc, X1.clone(), W1, P)
+    X3 = torch.empty_like(Rc)
+    bem._solve_direct(F, Rc, X3, W1, P)  # takes the inverses from here on
     X2, W2 = torch.empty_like(Rc), torch.empty_like(Rc)
     n, nn = F.n, F.n * F.n
     nat.call("mnpbem_bem_solve_inv_batched", F.dtype, n, P, nat.ptr(F.GG[0]), nat.ptr(F.GG[1]), nat.ptr(F.HH[0]),
@@ -368,6 +368,7 @@ def test_bem_solve_with_inverses_matches_factors(cuda, precision, tol):
              nat.ptr(bem.g.nvec), nat.ptr(F.par), 0, F.B, nat.stream_handle(cuda))
     a, b = X1.cpu().numpy(), X2.cpu().numpy()
     assert np.abs(a - b).max() / np.abs(a).max() < tol
+    assert torch.equal(X2, X3)
 
 
 def test_iter_block_preconditioner_inverse_vs_factors(cuda):

@@ -178,6 +178,30 @@ def test_shape_gradient_mixed_and_autograd(cuda):
     assert np.abs(verts.grad.numpy() - want).max() < 1e-12 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_shape_gradient_inverse_vs_factors(cuda, precision):
+    """Shape gradients with the solves on explicit inverses (built in) and on
+    the LU factors (inverse=False), and the Drude-parameter gradient on the
+    factors (built in) and on the inverses: the same values to the
+    refinement's accuracy."""
+    import mnpbem_amd as m
+    from mnpbem_amd.adjoint import extinction_with_drude_gradient, extinction_with_shape_gradient
+    lams = np.array([640.0, 700.0])
+    pw = m.PlaneWaveRet([[0, 0, 1], [0, 1, 0]], [[1, 0, 0], [1, 0, 0]])
+    cp = _rod_cp(m)
+    print("shape gradient inverse vs factors", precision, flush=True)
+    bem = m.BEMRet(cp, precision=precision)
+    e1, g1 = extinction_with_shape_gradient(bem, lams, pw, inverse=False)
+    e2, g2 = extinction_with_shape_gradient(bem, lams, pw)
+    assert np.abs(e2 / e1 - 1).max() < 1e-10
+    for key in g1[0]:
+        assert np.abs(g2[0][key] - g1[0][key]).max() < 1e-8 * np.abs(g1[0][key]).max(), key
+    d1 = extinction_with_drude_gradient(bem, lams, pw)
+    d2 = extinction_with_drude_gradient(bem, lams, pw, inverse=True)
+    for a, b in zip(d1, d2):
+        assert np.abs(b - a).max() < 1e-8 * np.abs(a).max()
+
+
 def test_shape_gradient_two_particles(cuda):
     """Dimer of two curved 60-face spheres 1 nm apart (refined pairs across
     the gap, one closed-surface correction per sphere): node gradients of
